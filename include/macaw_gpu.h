@@ -1,0 +1,126 @@
+/*
+ * macaw_gpu.h — C-ABI of the MI355X G-MCTDH propagation engine.
+ *
+ * The library replaces the CMF2 hot path of Macaw (DavidPicconi/Macaw,
+ * Fortran 90).  Every entry point takes plain pointers and sizes, returns 0
+ * on success and a non-zero status on error (message via mg_last_error), so
+ * a Fortran driver can bind it through ISO_C_BINDING.  Integers are int64
+ * (Macaw is built with -fdefault-integer-8, Makefile:18); complex numbers are
+ * interleaved (re, im) doubles, i.e. Fortran `double complex`.  Arrays are
+ * column-major with the Fortran index meaning given per field.
+ *
+ * One context = one HIP stream on the current device; a context is not
+ * thread-safe.  State (A-vector, SPFs) stays resident on the device between
+ * calls; host copies are made only by mg_set_state / mg_get_state and by the
+ * host-pointer forms of the kernels.
+ */
+#ifndef MACAW_GPU_H
+#define MACAW_GPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Problem descriptor: Macaw's module data after read_input and the DVR part
+ * of init_calc (psidef.f90:11-55, dvr.f90:14-27, hamilton.f90:24-36,
+ * globals.f90:20-49).  Only DVR (grid) modes are described here. */
+typedef struct mg_problem {
+    int64_t nstates, nmodes, ndof;
+    int64_t maxdof;               /* leading dimension of iDOF            */
+    const int64_t *nmodeDOF;      /* [nmodes]                              */
+    const int64_t *iDOF;          /* [maxdof, nmodes], 1-based DOF ids     */
+    const int64_t *MultiSet;      /* [nmodes], 1 = multi-set               */
+    const int64_t *nSPF;          /* [nmodes, nstates]                     */
+    const int64_t *ngp;           /* [ndof] grid points per DOF            */
+    int64_t ngpmax;               /* leading dimension of the grid arrays  */
+    const double *grid;           /* [ngpmax, ndof]                        */
+    const double *d1;             /* [ngpmax, ngpmax, ndof] d/dx           */
+    const double *d2;             /* [ngpmax, ngpmax, ndof] d^2/dx^2       */
+    const double *xd1;            /* [ngpmax, ngpmax, ndof] x d/dx + d/dx x*/
+    int64_t nham;
+    const double *ae;             /* [2*nham] complex coefficients         */
+    const int64_t *lhs_el;        /* [nham] 1-based                        */
+    const int64_t *rhs_el;        /* [nham] 1-based                        */
+    const int64_t *iOpDof;        /* [ndof, nham] primitive operator codes */
+    double eps_int, eps_psi, eps_rho;
+    double writetime;             /* output interval, a.u.                 */
+    double min_stepsize;          /* a.u.                                  */
+    int64_t int_type_psi;         /* 1 = Cash-Karp, 2 = Dormand-Prince     */
+} mg_problem;
+
+/* Observables written at every output time (output.f90:35-87,143-150). */
+typedef struct mg_output {
+    double time;                  /* a.u. */
+    double auto_re, auto_im;      /* C(t) = <Psi(0)|Psi(t)>                */
+    double norm;                  /* sum of state populations              */
+    double energy;                /* a.u., energyMF / norm                 */
+} mg_output;
+
+/* Per-attempt CMF2 record (the reference keeps rejected attempts internal,
+ * cmf2_base.f90:180-184; they are logged here). */
+typedef struct mg_attempt {
+    double time, h, err_A, err_psi;
+    int64_t accepted, nkrylov1, nkrylov2;
+} mg_attempt;
+
+typedef struct mg_ctx mg_ctx;
+
+/* ~ init_calc.f90:1-226 (tables) + alloc_h1mode/alloc_IsIdentity
+ * (hamilton.f90:91-373) + SetupCMF2 (cmf2_base.f90:15-43).  precision:
+ * 0 = complex FP64 everywhere. */
+int mg_create(const mg_problem *prob, int64_t precision, mg_ctx **ctx);
+int mg_destroy(mg_ctx *ctx);
+const char *mg_last_error(const mg_ctx *ctx);
+
+/* Sizes: nAConftot (psidef.f90:14) and dimpsi (init_calc.f90:160-181). */
+int64_t mg_nconf(const mg_ctx *ctx);
+int64_t mg_dimpsi(const mg_ctx *ctx);
+
+/* Upload / download AVector and psi (psidef.f90:44-45); host pointers. */
+int mg_set_state(mg_ctx *ctx, const double *A, const double *psi);
+int mg_get_state(mg_ctx *ctx, double *A, double *psi);
+/* AVector_Start / psi_Start for C(t) (init_calc.f90:222-223). */
+int mg_set_start(mg_ctx *ctx, const double *A, const double *psi);
+
+/* Calc_h1mode for all modes from the resident psi (hamilton.f90:416-531).
+ * h1mode_out (host, optional): [nspfmax, nspfmax, nopmax, nmodes]. */
+int mg_calc_h1mode(mg_ctx *ctx, double *h1mode_out);
+/* Calc_rho1mode for all modes from the resident A (gmctdh_objects.f90:408-477).
+ * rho_out (host, optional): [nspfmax, nspfmax, nmodes, nstates], full
+ * Hermitian matrices. */
+int mg_calc_rho(mg_ctx *ctx, double *rho_out);
+/* Calc_MeanField, Hamiltonian part, + Calc_rhom1MF
+ * (gmctdh_objects.f90:481-605, 1147-1194).  Requires mg_calc_h1mode and
+ * mg_calc_rho first.  Outputs optional: [nspfmax, nspfmax, nopmax, nmodes]. */
+int mg_calc_meanfield(mg_ctx *ctx, double *mf_out, double *rhom1mf_out);
+/* Calc_DHD_B (cmf2_derivatives.f90:1-80): HB = H B with the current h1mode;
+ * host pointers, complex [nconf]. */
+int mg_apply_h(mg_ctx *ctx, const double *B, double *HB);
+/* Same on device pointers, ordered on the context stream. */
+int mg_apply_h_device(mg_ctx *ctx, const void *dB, void *dHB);
+/* Calc_Der_SPF, DVR branch (cmf2_derivatives.f90:84-173), mode im (1-based)
+ * with the current rho^-1 * MeanField; host pointers, complex [dimpsi]. */
+int mg_der_spf(mg_ctx *ctx, int64_t im, const double *psi_in, double *dpsi_out);
+/* One CMF2 step (cmf2_base.f90:47-206) on the resident state. */
+int mg_cmf2_step(mg_ctx *ctx, double t, double htry, double *hdid, double *hnext);
+/* WriteOutput observables (output.f90:17-150) for the resident state;
+ * pops: [nstates]. */
+int mg_observables(mg_ctx *ctx, double t, mg_output *out, double *pops);
+/* propagate_wf, CMF2 branch (propagator.f90:87-106, 257-306): propagate the
+ * resident state from t0 to tfinal, filling up to max_out output records,
+ * pops [nstates, max_out] and up to max_att attempt records.  Counts are
+ * returned in n_out / n_att. */
+int mg_propagate(mg_ctx *ctx, double t0, double tfinal, mg_output *outs, double *pops,
+                 int64_t max_out, int64_t *n_out, mg_attempt *atts, int64_t max_att,
+                 int64_t *n_att);
+/* Timing of the last mg_apply_h_device launch sequence (ms, from HIP events
+ * on the context stream) and its algorithmic traffic / flops. */
+int mg_apply_h_stats(mg_ctx *ctx, double *bytes_alg, double *flops_alg);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MACAW_GPU_H */

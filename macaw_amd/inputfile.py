@@ -1,0 +1,290 @@
+"""Reader for the Macaw input-file and operator-file formats (the subset the
+CMF2 engine runs).
+
+Keyword grammar follows Macaw's ``read_input`` (source/input.f90:30-426):
+case-insensitive ``name = value`` lines, ``#`` comments, the ``SPF`` and
+``GRID`` blocks, ``INIT-WF ... END-INIT-WF`` and ``integration = CMF2``
+followed by the two integrator lines.  Defaults are those of
+source/input.f90:44-82.  Operator lines follow ``read_declare_ham``
+(source/input.f90:812-1144): ``coeff | elec | dof op | dof op ...`` up to a
+line starting with ``----``.
+
+Times are converted to atomic units with Macaw's constant
+``au2fs = 0.02418884326505`` (source/globals.f90:17).
+"""
+from __future__ import annotations
+
+import os
+import re
+from dataclasses import dataclass, field
+
+AU2EV = 27.2113961
+AU2FS = 0.02418884326505
+
+# primitive operator codes (source/input.f90:1098-1137, hamilton.f90:24-31)
+OP_CODES = {"p": -1, "dq^2": -2, "qp": -3, "q": 1}
+for _n in range(2, 11):
+    OP_CODES[f"q^{_n}"] = _n
+
+
+class InputError(ValueError):
+    """Raised for input the reference rejects with ``stop``."""
+
+
+@dataclass
+class SPFDecl:
+    multiset: bool
+    dofs: list          # 1-based DOF ids of the logical mode
+    nspf: list          # per electronic state
+
+
+@dataclass
+class GridDecl:
+    kind: str           # 'HO' (other grids are not part of this engine)
+    npts: int
+    x0: float
+    mw: float
+
+
+@dataclass
+class InitSPF:
+    kind: str           # 'gauss'
+    x0: float
+    p0: float
+    dx: float
+
+
+@dataclass
+class InputDeck:
+    nstates: int = 1
+    ndof: int = -1
+    nmodes: int = 0
+    spf: list = field(default_factory=list)
+    grid: dict = field(default_factory=dict)
+    hamiltonian: str = ""
+    tinit: float = 0.0
+    tfinal: float = 0.0
+    tout: float = 1.0 / AU2FS
+    eps_rho: float = 1e-8
+    eps_c: float = 1e-8
+    eps_ovl: float = 1e-8
+    eps_int: float = 1e-8
+    eps_a: float = 1e-8
+    eps_psi: float = 1e-8
+    min_stepsize: float = 1e-9 / AU2FS
+    int_type: int = 1
+    int_type_a: int = 0
+    int_type_psi: int = 0
+    steps: bool = False
+    normalize: int = 0
+    init_state: int = 1
+    init_spf: dict = field(default_factory=dict)
+    basedir: str = "."
+
+
+def _strip(line: str) -> str:
+    i = line.find("#")
+    return line[:i] if i >= 0 else line
+
+
+def _fortran_float(s: str) -> float:
+    return float(s.strip().lower().replace("d", "e"))
+
+
+def parse_input(path: str) -> InputDeck:
+    """Parse a Macaw input file (source/input.f90:113-354)."""
+    deck = InputDeck(basedir=os.path.dirname(os.path.abspath(path)))
+    with open(path) as fh:
+        lines = fh.read().splitlines()
+    i = 0
+    n = len(lines)
+    while i < n:
+        line = _strip(lines[i])
+        i += 1
+        if line.strip() == "INIT-WF":
+            block = []
+            while i < n and _strip(lines[i]).strip() != "END-INIT-WF":
+                block.append(_strip(lines[i]))
+                i += 1
+            if i >= n:
+                raise InputError("Closing instruction END-INIT-WF missing")
+            i += 1
+            _parse_init_wf(deck, block)
+            continue
+        if "=" in line:
+            name, value = line.split("=", 1)
+            name = name.strip().lower()
+            value = value.strip()
+        else:
+            name, value = line.strip().lower(), "blank"
+        if name == "":
+            continue
+        if name == "nstates":
+            deck.nstates = int(value)
+        elif name == "ndof":
+            deck.ndof = int(value)
+            if deck.ndof > 99:
+                raise InputError("the maximum number of degrees of freedom is 99")
+        elif name == "nmodes":
+            deck.nmodes = int(value)
+        elif name == "spf":
+            if deck.nmodes == 0 or deck.ndof <= 0:
+                raise InputError("with the keyword SPF, nmodes and ndof must be > 0")
+            for _ in range(deck.nmodes):
+                deck.spf.append(_parse_spf(_strip(lines[i]), deck.nstates))
+                i += 1
+        elif name == "grid":
+            for _ in range(deck.ndof):
+                k, g = _parse_grid(_strip(lines[i]))
+                deck.grid[k] = g
+                i += 1
+        elif name == "hamiltonian":
+            deck.hamiltonian = value
+        elif name == "tinit":
+            deck.tinit = _fortran_float(value) / AU2FS
+        elif name == "tfinal":
+            deck.tfinal = _fortran_float(value) / AU2FS
+        elif name == "tout":
+            deck.tout = _fortran_float(value) / AU2FS
+        elif name == "eps_rho":
+            deck.eps_rho = _fortran_float(value)
+        elif name == "eps_c":
+            deck.eps_c = _fortran_float(value)
+        elif name == "eps_int":
+            deck.eps_int = _fortran_float(value)
+        elif name == "eps_s":
+            deck.eps_ovl = _fortran_float(value)
+        elif name == "minstep":
+            deck.min_stepsize = _fortran_float(value) / AU2FS
+        elif name == "steps":
+            deck.steps = True
+        elif name == "normalize":
+            deck.normalize = int(value)
+        elif name == "integration":
+            v = value.lower()
+            if v.startswith("cmf2"):
+                deck.int_type = 10
+                a_name, a_eps = lines[i].split()[:2]
+                p_name, p_eps = lines[i + 1].split()[:2]
+                i += 2
+                deck.int_type_a = {"lanczos": 5, "rk45": 1, "dopri": 2, "bs": 3}.get(a_name.lower(), -1)
+                if deck.int_type_a < 0:
+                    raise InputError(f"Integrator {a_name} not possible for the A coefficients")
+                deck.eps_a = _fortran_float(a_eps)
+                deck.int_type_psi = {"rk45": 1, "dopri": 2, "bs": 3}.get(p_name.lower(), -1)
+                if deck.int_type_psi < 0:
+                    raise InputError(f"Integrator {p_name} not possible for the SPFs")
+                deck.eps_psi = _fortran_float(p_eps)
+            elif v == "rk45":
+                deck.int_type = 1
+            elif v == "dopri":
+                deck.int_type = 2
+            else:
+                raise InputError(f"unknown integrator: {value}")
+        else:
+            raise InputError(f"Input instruction: {line.strip()} cannot be understood")
+    if not deck.hamiltonian:
+        raise InputError("Hamiltonian file not given")
+    return deck
+
+
+def _parse_spf(line: str, nstates: int) -> SPFDecl:
+    """One line of the SPF block (source/input.f90:451-512)."""
+    if "=" not in line:
+        raise InputError(f"Wrong SPF declaration: {line.strip()}")
+    sdof, sspf = line.split("=", 1)
+    parts = [p.strip() for p in sdof.split(",")]
+    multiset = parts[0][:1] != "S"
+    dofs = [int(p) for p in parts[1:]]
+    nspf = [int(p) for p in sspf.split(",")]
+    if not multiset:
+        nspf = [nspf[0]] * nstates
+    elif len(nspf) < nstates:
+        raise InputError("Wrong SPF declaration (SPF)")
+    return SPFDecl(multiset, dofs, nspf[:nstates])
+
+
+def _parse_grid(line: str):
+    """One line of the GRID block (source/dvr.f90:32-103, HO branch)."""
+    j = line.find("HO")
+    if j < 0:
+        raise InputError(f"only HO grids are supported by this engine: {line.strip()}")
+    k = int(line[:j])
+    toks = line[j + 2:].split()
+    return k, GridDecl("HO", int(toks[0]), _fortran_float(toks[1]), _fortran_float(toks[2]))
+
+
+def _parse_init_wf(deck: InputDeck, block: list) -> None:
+    """single-conf INIT-WF block (source/init_calc.f90:482-752, el/gauss branches)."""
+    rows = [b for b in block if b.strip()]
+    if not rows or rows[0].strip() != "single-conf":
+        raise InputError("The INIT-WF block must start with the keyword single-conf")
+    m = re.search(r"el\s+(\d+)", rows[1])
+    if not m:
+        raise InputError("single-conf: expected 'el <state>'")
+    deck.init_state = int(m.group(1))
+    for row in rows[2:]:
+        j = row.find("gauss")
+        if j < 0:
+            raise InputError(f"The chosen initial SPF is unknown: {row.strip()}")
+        k = int(row[:j])
+        x0, p0, dx = (_fortran_float(t) for t in row[j + 5:].split()[:3])
+        deck.init_spf[k] = InitSPF("gauss", x0, p0, dx)
+
+
+@dataclass
+class HamTerm:
+    coeff: complex
+    lhs: int
+    rhs: int
+    ops: dict           # dof -> primitive operator code (identity omitted)
+
+
+def parse_operator_file(path: str, nstates: int) -> list:
+    """Hamiltonian terms of an operator file (source/input.f90:752-1144).
+
+    ``1`` and a bare coefficient give one term per electronic state, ``Sa&b``
+    with a != b gives both directions with the same coefficient, ``Za&b`` one
+    direction.
+    """
+    terms = []
+    with open(path) as fh:
+        for raw in fh:
+            if raw[:4] == "----":
+                break
+            if not raw.strip():
+                continue
+            line = _strip(raw)
+            if not line.strip():
+                continue
+            fields = line.split("|")
+            cstr = fields[0]
+            coeff = _fortran_float(cstr.replace("i", "")) * (1j if "i" in cstr else 1.0)
+            pairs = []
+            if len(fields) == 1 or fields[1].strip() == "1":
+                pairs = [(s, s) for s in range(1, nstates + 1)]
+            else:
+                el = fields[1].strip()
+                m = re.match(r"([SZ])\s*(\d+)\s*&\s*(\d+)", el)
+                if not m:
+                    raise InputError('Only operator labels "1", "Sa&b" or "Za&b" are allowed')
+                a, b = int(m.group(2)), int(m.group(3))
+                if m.group(1) == "S":
+                    pairs = [(a, b)] if a == b else [(a, b), (b, a)]
+                else:
+                    if a == b:
+                        raise InputError('Electronic operators "Za&b" are allowed only between different states')
+                    pairs = [(a, b)]
+            ops = {}
+            for f in fields[2:]:
+                f = f.rstrip()
+                kdof = int(f[:2]) if f[:2].strip() else int(f.split()[0])
+                name = f[2:].strip()
+                if name == "1":
+                    continue
+                if name not in OP_CODES:
+                    raise InputError(f"Operator {name} not implemented")
+                ops[kdof] = OP_CODES[name]
+            for a, b in pairs:
+                terms.append(HamTerm(complex(coeff), a, b, dict(ops)))
+    return terms

@@ -1,0 +1,145 @@
+"""Seeded linear-vibronic-coupling (LVC) model generator.
+
+Writes Macaw-format input and operator files for the benchmark workloads:
+
+* ``c1``: 2 states, 5 modes, HO-DVR 24 points, 6 SPFs, 100 fs.
+* ``c2``: 4 states, 8 modes (+ diagonal quadratic terms), HO-DVR 32 points,
+  8 SPFs (8^8 configurations per state), 200 fs.
+* ``tiny``: small variants used by the tests.
+
+Energies are drawn in eV with ``numpy.random.default_rng(seed)`` and written
+in Hartree (divided by Macaw's ``au2eV = 27.2113961``).  Hamiltonian lines:
+``-w/2 | 1 |i dq^2``, ``w/2 | 1 |i q^2``, ``E_s | Ss&s``, ``kappa | Ss&s |i q``,
+``gamma | Ss&s |i q^2`` and ``lambda | Sa&b |j q``.
+"""
+from __future__ import annotations
+
+import itertools
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+from .inputfile import AU2EV
+
+
+@dataclass
+class LVCSpec:
+    name: str
+    seed: int
+    nstates: int
+    nmodes: int
+    npts: int
+    nspf: object                 # int, or one count per electronic state
+    tfinal: float
+    tout: float = 0.5
+    omega: tuple = (0.02, 0.40)
+    kappa_sd: float = 0.10
+    gamma_sd: float = 0.0
+    n_lambda: int = 1
+    lam: tuple = (0.10, 0.10)
+    gaps: tuple = ()
+    eps_int: float = 1e-7
+    eps_psi: float = 1e-7
+    spf_integrator: str = "rk45"
+    init_state: int = 0          # 0 -> top state
+    multiset: bool = True
+
+
+SPECS = {
+    "c1": LVCSpec("c1", 101, 2, 5, 24, 6, 100.0, omega=(0.02, 0.40), kappa_sd=0.10,
+                  n_lambda=1, lam=(0.10, 0.10), gaps=(-0.25, 0.25)),
+    "c2": LVCSpec("c2", 202, 4, 8, 32, 8, 200.0, omega=(0.05, 0.40), kappa_sd=0.12,
+                  gamma_sd=0.02, n_lambda=6, lam=(0.05, 0.20),
+                  gaps=(0.0, 0.3, 0.6, 0.9)),
+    # small cases for the tests
+    "t1": LVCSpec("t1", 11, 2, 3, 8, 3, 5.0, omega=(0.05, 0.30), kappa_sd=0.10,
+                  n_lambda=1, lam=(0.10, 0.10), gaps=(-0.2, 0.2)),
+    "t2": LVCSpec("t2", 12, 3, 3, 7, (3, 4, 2), 5.0, omega=(0.05, 0.30), kappa_sd=0.12,
+                  gamma_sd=0.02, n_lambda=3, lam=(0.05, 0.20), gaps=(0.0, 0.25, 0.5)),
+    "t3": LVCSpec("t3", 13, 2, 4, 10, 4, 10.0, omega=(0.05, 0.30), kappa_sd=0.10,
+                  gamma_sd=0.02, n_lambda=2, lam=(0.05, 0.15), gaps=(-0.1, 0.1),
+                  spf_integrator="dopri"),
+    "t4": LVCSpec("t4", 14, 2, 3, 8, 3, 5.0, omega=(0.05, 0.30), kappa_sd=0.10,
+                  n_lambda=1, lam=(0.10, 0.10), gaps=(-0.2, 0.2), multiset=False),
+}
+
+
+def draw(spec: LVCSpec):
+    """Seeded parameters in eV: omega[f], kappa[S,f], gamma[S,f], couplings."""
+    rng = np.random.default_rng(spec.seed)
+    omega = rng.uniform(spec.omega[0], spec.omega[1], size=spec.nmodes)
+    kappa = rng.normal(0.0, spec.kappa_sd, size=(spec.nstates, spec.nmodes))
+    gamma = (rng.normal(0.0, spec.gamma_sd, size=(spec.nstates, spec.nmodes))
+             if spec.gamma_sd > 0 else np.zeros((spec.nstates, spec.nmodes)))
+    pairs = list(itertools.combinations(range(1, spec.nstates + 1), 2))
+    combos = [(m, p) for m in range(1, spec.nmodes + 1) for p in pairs]
+    couplings = []
+    if spec.n_lambda == 1 and spec.lam[0] == spec.lam[1]:
+        j = int(rng.integers(1, spec.nmodes + 1))
+        couplings.append((j, pairs[0], spec.lam[0]))
+    else:
+        pick = rng.choice(len(combos), size=min(spec.n_lambda, len(combos)), replace=False)
+        for c in pick:
+            j, p = combos[int(c)]
+            couplings.append((j, p, float(rng.uniform(spec.lam[0], spec.lam[1]))))
+    return omega, kappa, gamma, couplings
+
+
+def _ev(x: float) -> str:
+    return f"{x / AU2EV: .12e}".replace("e", "d")
+
+
+def write_lvc(spec: LVCSpec, outdir: str) -> str:
+    """Write ``<name>.inp`` and ``<name>.op``; return the input path."""
+    os.makedirs(outdir, exist_ok=True)
+    omega, kappa, gamma, couplings = draw(spec)
+    S, f = spec.nstates, spec.nmodes
+    op = [f"# LVC model {spec.name}, seed {spec.seed}; energies in Hartree"]
+    for i in range(f):
+        op.append(f"{_ev(-omega[i] / 2)}  |  1  |{i + 1:<2d} dq^2")
+        op.append(f"{_ev(omega[i] / 2)}  |  1  |{i + 1:<2d} q^2")
+    for s in range(S):
+        gap = spec.gaps[s] if spec.gaps else 0.0
+        op.append(f"{_ev(gap)}  |  S{s + 1}&{s + 1}")
+    for s in range(S):
+        for i in range(f):
+            op.append(f"{_ev(kappa[s, i])}  |  S{s + 1}&{s + 1}  |{i + 1:<2d} q")
+    if spec.gamma_sd > 0:
+        for s in range(S):
+            for i in range(f):
+                op.append(f"{_ev(gamma[s, i])}  |  S{s + 1}&{s + 1}  |{i + 1:<2d} q^2")
+    for j, (a, b), lam in couplings:
+        op.append(f"{_ev(lam)}  |  S{a}&{b}  |{j:<2d} q")
+    op.append("----")
+    opname = f"{spec.name}.op"
+    with open(os.path.join(outdir, opname), "w") as fh:
+        fh.write("\n".join(op) + "\n")
+    top = spec.init_state or S
+    set_tag = "M" if spec.multiset else "S"
+    lines = [f"tfinal = {spec.tfinal}", f"tout = {spec.tout}", "",
+             f"nstates = {S}", f"ndof = {f}", f"nmodes = {f}", "", "steps", "", "SPF"]
+    for i in range(f):
+        counts = list(spec.nspf) if isinstance(spec.nspf, (list, tuple)) else [spec.nspf] * S
+        lines.append(f" {set_tag}, {i + 1} = " + ",".join(str(c) for c in counts))
+    lines += ["", "GRID"]
+    for i in range(f):
+        lines.append(f"  {i + 1}  HO  {spec.npts}  0.0  1.0")
+    lines += ["", f"Hamiltonian = {opname}", "", "INIT-WF", "   single-conf", f"   el  {top}"]
+    for i in range(f):
+        lines.append(f"   {i + 1}   gauss  0.0  0.0  0.707107")
+    fd = lambda x: f"{x:.1e}".replace("e", "d")
+    lines += ["END-INIT-WF", "", f"eps_int = {fd(spec.eps_int)}",
+              "", "integration = CMF2", "Lanczos  1.d-7",
+              f"{spec.spf_integrator}     {fd(spec.eps_psi)}", ""]
+    path = os.path.join(outdir, f"{spec.name}.inp")
+    with open(path, "w") as fh:
+        fh.write("\n".join(lines) + "\n")
+    return path
+
+
+def make_problem(spec: LVCSpec | str, outdir: str):
+    from .problem import load_problem
+    if isinstance(spec, str):
+        spec = SPECS[spec]
+    return load_problem(write_lvc(spec, outdir))

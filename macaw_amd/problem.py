@@ -1,0 +1,222 @@
+"""Problem descriptor: the module data Macaw holds after ``read_input`` and
+the DVR part of ``init_calc`` (source/init_calc.f90:1-226), plus the initial
+wavefunction of the ``single-conf`` INIT-WF branch
+(source/init_calc.f90:482-752 and ``SetSPF`` :763-927).
+
+All integer tables are int64 and column-major with 1-based entries, exactly
+as the Fortran arrays they mirror (``nSPF(mode, state)``, ``iDOF(k, mode)``,
+``iOpDof(dof, term)``), so the same buffers can be handed over by the
+Fortran driver or by this module.
+"""
+from __future__ import annotations
+
+import os
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from .dvr import harm_dvr
+from .inputfile import InputDeck, parse_input, parse_operator_file
+
+
+@dataclass
+class Problem:
+    nstates: int
+    nmodes: int
+    ndof: int
+    nmodeDOF: np.ndarray          # int64[nmodes]
+    iDOF: np.ndarray              # int64[maxDOF, nmodes] (Fortran order)
+    MultiSet: np.ndarray          # int64[nmodes]
+    nSPF: np.ndarray              # int64[nmodes, nstates] (Fortran order)
+    ngp: np.ndarray               # int64[ndof]
+    grid: np.ndarray              # float64[ngpMax, ndof]
+    d1: np.ndarray                # float64[ngpMax, ngpMax, ndof]
+    d2: np.ndarray
+    xd1: np.ndarray
+    ae: np.ndarray                # complex128[nHam]
+    lhs_el: np.ndarray            # int64[nHam]
+    rhs_el: np.ndarray            # int64[nHam]
+    iOpDof: np.ndarray            # int64[ndof, nHam] (Fortran order)
+    eps_int: float = 1e-8
+    eps_psi: float = 1e-8
+    eps_rho: float = 1e-8
+    writetime: float = 1.0 / 0.02418884326505
+    min_stepsize: float = 1e-9 / 0.02418884326505
+    tinit: float = 0.0
+    tfinal: float = 0.0
+    int_type_psi: int = 1
+    A0: np.ndarray = None         # complex128[nAConftot]
+    psi0: np.ndarray = None       # complex128[dimpsi]
+    wgrid: np.ndarray = None
+    extra: dict = field(default_factory=dict)
+
+    # ---- derived layout (source/init_calc.f90:25-34, 160-181) ----
+    @property
+    def nHam(self) -> int:
+        return int(self.ae.shape[0])
+
+    @property
+    def maxDOF(self) -> int:
+        return int(self.iDOF.shape[0])
+
+    @property
+    def ngpMax(self) -> int:
+        return int(self.grid.shape[0])
+
+    def nAConf(self) -> np.ndarray:
+        return np.prod(self.nSPF, axis=0).astype(np.int64)
+
+    def AStateOffset(self) -> np.ndarray:
+        n = self.nAConf()
+        return np.concatenate([[0], np.cumsum(n)[:-1]]).astype(np.int64)
+
+    @property
+    def nAConftot(self) -> int:
+        return int(self.nAConf().sum())
+
+    def ngpm(self) -> np.ndarray:
+        out = np.ones(self.nmodes, dtype=np.int64)
+        for m in range(self.nmodes):
+            for k in range(int(self.nmodeDOF[m])):
+                out[m] *= self.ngp[self.iDOF[k, m] - 1]
+        return out
+
+    def ipsiSPF(self):
+        """1-based start of each (mode, state) SPF block and dimpsi."""
+        ip = np.zeros((self.nmodes, self.nstates), dtype=np.int64)
+        dim = 0
+        ngpm = self.ngpm()
+        for s in range(self.nstates):
+            for m in range(self.nmodes):
+                if not self.MultiSet[m] and s > 0:
+                    ip[m, s] = ip[m, 0]
+                    continue
+                ip[m, s] = dim + 1
+                dim += int(ngpm[m] * self.nSPF[m, s])
+        return ip, dim
+
+    @property
+    def dimpsi(self) -> int:
+        return self.ipsiSPF()[1]
+
+
+def build_problem(deck: InputDeck, operator_path: str | None = None) -> Problem:
+    ns, nm, nd = deck.nstates, deck.nmodes, deck.ndof
+    maxdof = max(len(s.dofs) for s in deck.spf)
+    iDOF = np.zeros((maxdof, nm), dtype=np.int64, order="F")
+    nmodeDOF = np.zeros(nm, dtype=np.int64)
+    ms = np.zeros(nm, dtype=np.int64)
+    nSPF = np.zeros((nm, ns), dtype=np.int64, order="F")
+    for m, s in enumerate(deck.spf):
+        nmodeDOF[m] = len(s.dofs)
+        iDOF[: len(s.dofs), m] = s.dofs
+        ms[m] = 1 if s.multiset else 0
+        nSPF[m, :] = s.nspf
+    ngp = np.array([deck.grid[k].npts for k in range(1, nd + 1)], dtype=np.int64)
+    gmax = int(ngp.max())
+    grid = np.zeros((gmax, nd), order="F")
+    wgrid = np.zeros((gmax, nd), order="F")
+    d1 = np.zeros((gmax, gmax, nd), order="F")
+    d2 = np.zeros_like(d1)
+    xd1 = np.zeros_like(d1)
+    for k in range(nd):
+        g = deck.grid[k + 1]
+        x, w, a1, a2, ax, _ = harm_dvr(g.npts, g.x0, g.mw)
+        n = g.npts
+        grid[:n, k] = x
+        wgrid[:n, k] = w
+        d1[:n, :n, k] = a1
+        d2[:n, :n, k] = a2
+        xd1[:n, :n, k] = ax
+    path = operator_path or os.path.join(deck.basedir, deck.hamiltonian)
+    terms = parse_operator_file(path, ns)
+    nham = len(terms)
+    ae = np.array([t.coeff for t in terms], dtype=np.complex128)
+    lhs = np.array([t.lhs for t in terms], dtype=np.int64)
+    rhs = np.array([t.rhs for t in terms], dtype=np.int64)
+    iop = np.zeros((nd, nham), dtype=np.int64, order="F")
+    for j, t in enumerate(terms):
+        for kd, code in t.ops.items():
+            iop[kd - 1, j] = code
+    prob = Problem(ns, nm, nd, nmodeDOF, iDOF, ms, nSPF, ngp, grid, d1, d2, xd1,
+                   ae, lhs, rhs, iop, eps_int=deck.eps_int, eps_psi=deck.eps_psi,
+                   eps_rho=deck.eps_rho, writetime=deck.tout,
+                   min_stepsize=deck.min_stepsize, tinit=deck.tinit,
+                   tfinal=deck.tfinal, int_type_psi=deck.int_type_psi or 1,
+                   wgrid=wgrid)
+    prob.A0, prob.psi0 = initial_wavefunction(prob, deck)
+    return prob
+
+
+def load_problem(input_path: str) -> Problem:
+    return build_problem(parse_input(input_path))
+
+
+def initial_wavefunction(prob: Problem, deck: InputDeck):
+    """single-conf initial state: A = e_1 on the chosen state, SPFs from
+    ``SetSPF`` (q-multiplication, Gram-Schmidt against SPF 1, then a
+    Householder QR of the virtual SPFs), then normalisation
+    (source/init_calc.f90:489-497, 525-546, 763-927, 215-221)."""
+    ns, nm = prob.nstates, prob.nmodes
+    gmax = prob.ngpMax
+    phi = np.zeros((gmax, prob.ndof), dtype=np.complex128)
+    eps = np.finfo(np.float64).eps
+    for k in range(1, prob.ndof + 1):
+        spec = deck.init_spf.get(k)
+        if spec is None:
+            raise ValueError(f"no initial SPF given for dof {k}")
+        n = int(prob.ngp[k - 1])
+        x = prob.grid[:n, k - 1]
+        w = prob.wgrid[:n, k - 1]
+        if abs(spec.x0) < eps and abs(spec.p0) < eps:
+            for j in range(1, n // 2 + 2):
+                phi[j - 1, k - 1] = np.exp(-(0.5 * x[j - 1] / spec.dx) ** 2) * np.sqrt(w[j - 1])
+                phi[n - j, k - 1] = phi[j - 1, k - 1]
+        else:
+            xx = x - spec.x0
+            phi[:n, k - 1] = np.exp(-(0.5 * xx / spec.dx) ** 2 + 1j * spec.p0 * xx) * np.sqrt(w)
+    ip, dimpsi = prob.ipsiSPF()
+    psi = np.zeros(dimpsi, dtype=np.complex128)
+    ngpm = prob.ngpm()
+    for m in range(nm):
+        nmd = int(prob.nmodeDOF[m])
+        dofs = [int(d) for d in prob.iDOF[:nmd, m]]
+        sizes = [int(prob.ngp[d - 1]) for d in dofs]
+        ngpt = int(ngpm[m])
+        nS = int(prob.nSPF[m, :].max())
+        # product grid, first DOF fastest
+        idx = np.indices(sizes[::-1]).reshape(nmd, -1)[::-1]
+        psi_t = np.zeros((ngpt, nS), dtype=np.complex128)
+        col = np.ones(ngpt, dtype=np.complex128)
+        for k in range(nmd):
+            col = col * phi[idx[k], dofs[k] - 1]
+        psi_t[:, 0] = col / np.linalg.norm(col)
+        kg = 0
+        jj = 1
+        for j in range(2, nS + 1):
+            jj += 1
+            kg = kg % nmd + 1
+            lref = (jj - 2) // nmd + 1
+            op = prob.grid[idx[kg - 1], dofs[kg - 1] - 1]
+            psi_t[:, j - 1] = psi_t[:, lref - 1] * op
+        for j in range(1, nS):
+            c = np.vdot(psi_t[:, 0], psi_t[:, j])
+            psi_t[:, j] = psi_t[:, j] - c * psi_t[:, 0]
+        if nS > 1:
+            q, _ = np.linalg.qr(psi_t[:, 1:], mode="reduced")
+            psi_t[:, 1:] = q
+        for s in range(ns):
+            n_s = int(prob.nSPF[m, s])
+            start = ip[m, s] - 1
+            psi[start:start + ngpt * n_s] = psi_t[:, :n_s].reshape(-1, order="F")
+    A = np.zeros(prob.nAConftot, dtype=np.complex128)
+    A[prob.AStateOffset()[deck.init_state - 1]] = 1.0
+    # norm_wf of a single populated configuration: product of the first-SPF norms
+    cnorm = 1.0 + 0.0j
+    s = deck.init_state - 1
+    for m in range(nm):
+        n = int(ngpm[m])
+        v = psi[ip[m, s] - 1: ip[m, s] - 1 + n]
+        cnorm = cnorm * np.vdot(v, v)
+    A = A / np.sqrt(cnorm.real)
+    return A, psi

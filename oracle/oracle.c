@@ -1,0 +1,1226 @@
+/*
+ * oracle.c — CPU FP64 restatement of Macaw's CMF2 propagation path
+ * (DavidPicconi/Macaw, source/ *.f90), DVR (grid) modes, multi-set and
+ * single-set.
+ *
+ * TEST INFRASTRUCTURE ONLY: the checker for the HIP engine.  Nothing in the
+ * engine links or calls it.
+ *
+ * Complex arithmetic is C99 `double complex`.  The reference works on
+ * conj(A) and right-multiplies (e.g. cmf2_derivatives.f90:20-21,77); that is
+ * exactly the complex conjugate of left-multiplying A, which is what this
+ * file does (conjugation is exact in IEEE arithmetic).  Every mode product
+ * below is  out[l, i, r] = sum_j M[i, j] in[l, j, r]  over the mode index.
+ *
+ * Parity pins: see tests/test_oracle_pins.py (brute-force Kronecker H,
+ * einsum mean fields, eigh-based regularised inverses, expm Lanczos,
+ * full-basis MCTDH == exact propagation, Hartree/Rabi closed forms).
+ */
+#include <complex.h>
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "oracle.h"
+
+typedef double complex cplx;
+
+#define NKRYLOV_MAX 20            /* cmf2_integration.f90:12 */
+static const double EPS_B = 1e-8; /* cmf2_integration.f90:13 */
+
+typedef struct {
+    int64_t lhs, rhs;             /* electronic labels (1-based)      */
+    int64_t codes[8];             /* primitive codes per DOF of mode  */
+} oper_t;
+
+struct orc {
+    mg_problem p;
+    /* owned copies of the descriptor arrays */
+    int64_t *nmodeDOF, *iDOF, *MultiSet, *nSPF, *ngp, *lhs, *rhs, *iOpDof;
+    double *grid, *d1, *d2, *xd1;
+    cplx *ae;
+    int64_t ns, nm, nham, nspfmax, nopmax, ngpmax;
+    int64_t *nAConf, *AStateOffset, nconf, dimpsi;
+    int64_t *ipsiSPF;             /* [nm, ns] 1-based */
+    int64_t *ngpm;                /* [nm] */
+    int64_t *nOper;               /* [nm] */
+    oper_t *Oper;                 /* [nm * nopmax] */
+    int64_t *iOper;               /* [nm * nham] 1-based op index */
+    int64_t *IsIdentity;          /* [nm * nopmax] */
+    cplx *A, *psi, *Astart, *psistart;
+    cplx *h1;                     /* [nspfmax^2 * nopmax * nm] */
+    cplx *rho;                    /* [nspfmax^2 * nm * ns] full Hermitian */
+    cplx *rho_ave;                /* [nspfmax^2 * nm] */
+    cplx *mf, *rmf;               /* [nspfmax^2 * nopmax * nm] */
+    double energyMF;
+    /* Lanczos storage */
+    cplx *BK;                     /* [nconf * NKRYLOV_MAX] */
+    double lan_alpha[NKRYLOV_MAX], lan_U[NKRYLOV_MAX * NKRYLOV_MAX];
+    int nkrylov;
+    cplx der_factor;
+    char err[256];
+};
+
+#define H1(o, m, op) ((o)->h1 + (size_t)(o)->nspfmax * (o)->nspfmax * ((op) + (o)->nopmax * (m)))
+#define MF(o, m, op) ((o)->mf + (size_t)(o)->nspfmax * (o)->nspfmax * ((op) + (o)->nopmax * (m)))
+#define RMF(o, m, op) ((o)->rmf + (size_t)(o)->nspfmax * (o)->nspfmax * ((op) + (o)->nopmax * (m)))
+#define RHO(o, m, s) ((o)->rho + (size_t)(o)->nspfmax * (o)->nspfmax * ((m) + (o)->nm * (s)))
+#define RHOAVE(o, m) ((o)->rho_ave + (size_t)(o)->nspfmax * (o)->nspfmax * (m))
+#define NSPF(o, m, s) ((o)->nSPF[(m) + (o)->nm * (s)])
+#define IPSI(o, m, s) ((o)->ipsiSPF[(m) + (o)->nm * (s)])
+
+static void *xcalloc(size_t n, size_t sz) {
+    void *v = calloc(n ? n : 1, sz);
+    if (!v) { fprintf(stderr, "oracle: out of memory\n"); abort(); }
+    return v;
+}
+
+static int64_t *dup_i64(const int64_t *src, size_t n) {
+    int64_t *d = xcalloc(n, sizeof(int64_t));
+    memcpy(d, src, n * sizeof(int64_t));
+    return d;
+}
+
+static double *dup_f64(const double *src, size_t n) {
+    double *d = xcalloc(n, sizeof(double));
+    memcpy(d, src, n * sizeof(double));
+    return d;
+}
+
+/* ---------------------------------------------------------------------- */
+/* Mode product on a tensor with per-mode sizes `sz` (mode 0 fastest):
+ * out[l, i, r] = sum_j M[i + ldm*j] in[l, j, r]; i < nout, j < sz[k].
+ * Restates Mult_HeM_MatV / Mult_GM_MatV (gmctdh_objects.f90:1407-1468) in
+ * left-multiplication form. */
+static void mode_product(int64_t nm, const int64_t *sz, int64_t k, const cplx *M, int64_t ldm,
+                         int64_t nout, const cplx *in, cplx *out) {
+    int64_t ll = 1, nl = 1;
+    for (int64_t q = 0; q < k; q++) ll *= sz[q];
+    for (int64_t q = k + 1; q < nm; q++) nl *= sz[q];
+    const int64_t nin = sz[k];
+#pragma omp parallel for schedule(static)
+    for (int64_t r = 0; r < nl; r++) {
+        const cplx *ib = in + r * ll * nin;
+        cplx *ob = out + r * ll * nout;
+        for (int64_t i = 0; i < nout; i++) {
+            cplx *o = ob + i * ll;
+            for (int64_t l = 0; l < ll; l++) o[l] = 0;
+            for (int64_t j = 0; j < nin; j++) {
+                const cplx mij = M[i + ldm * j];
+                const cplx *x = ib + j * ll;
+                for (int64_t l = 0; l < ll; l++) o[l] += mij * x[l];
+            }
+        }
+    }
+}
+
+/* Hole contraction over mode k: G[j1 + ldg*j2] += alpha * sum_{l,r}
+ * conj(X[l, j1, r]) Y[l, j2, r].  X has n1 values on mode k, Y has n2, both
+ * share ll (modes < k) and nl (modes > k).  Restates the zgemm('T','N') /
+ * zher2k blocks of gmctdh_objects.f90:452-457, 592-600. */
+static void hole_gram(int64_t ll, int64_t nl, int64_t n1, int64_t n2, const cplx *X,
+                      const cplx *Y, cplx alpha, cplx *G, int64_t ldg) {
+    for (int64_t j2 = 0; j2 < n2; j2++) {
+        for (int64_t j1 = 0; j1 < n1; j1++) {
+            cplx acc = 0;
+#pragma omp parallel for reduction(+:acc) schedule(static)
+            for (int64_t r = 0; r < nl; r++) {
+                const cplx *x = X + (r * n1 + j1) * ll;
+                const cplx *y = Y + (r * n2 + j2) * ll;
+                cplx a = 0;
+                for (int64_t l = 0; l < ll; l++) a += conj(x[l]) * y[l];
+                acc += a;
+            }
+            G[j1 + ldg * j2] += alpha * acc;
+        }
+    }
+}
+
+/* ---------------------------------------------------------------------- */
+/* Small dense linear algebra (replaces zheev / dstev / zposv). */
+
+/* Cyclic complex Jacobi eigen-decomposition of the Hermitian n x n matrix A
+ * (column-major, full).  w ascending, U columns = eigenvectors. */
+static void herm_eig(int64_t n, const cplx *Ain, double *w, cplx *U) {
+    cplx *A = xcalloc((size_t)n * n, sizeof(cplx));
+    memcpy(A, Ain, sizeof(cplx) * n * n);
+    for (int64_t i = 0; i < n; i++)
+        for (int64_t j = 0; j < n; j++) U[i + n * j] = (i == j) ? 1.0 : 0.0;
+    for (int sweep = 0; sweep < 100; sweep++) {
+        double off = 0.0, diag = 0.0;
+        for (int64_t j = 0; j < n; j++)
+            for (int64_t i = 0; i < n; i++) {
+                double a2 = creal(A[i + n * j] * conj(A[i + n * j]));
+                if (i != j) off += a2; else diag += a2;
+            }
+        if (off <= 1e-32 * (diag + 1e-300) || off == 0.0) break;
+        for (int64_t p = 0; p < n - 1; p++)
+            for (int64_t q = p + 1; q < n; q++) {
+                cplx apq = A[p + n * q];
+                double aabs = cabs(apq);
+                if (aabs == 0.0) continue;
+                cplx ph = apq / aabs;                /* e^{i phi} */
+                double app = creal(A[p + n * p]), aqq = creal(A[q + n * q]);
+                double theta = (aqq - app) / (2.0 * aabs);
+                double t = (theta >= 0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
+                double c = 1.0 / sqrt(t * t + 1.0), s = t * c;
+                /* J: J[p][p]=c, J[q][q]=c*conj(ph), J[p][q]=s, J[q][p]=-s*conj(ph) */
+                cplx jpp = c, jqq = c * conj(ph), jpq = s, jqp = -s * conj(ph);
+                /* A <- A J (columns p, q) */
+                for (int64_t k = 0; k < n; k++) {
+                    cplx akp = A[k + n * p], akq = A[k + n * q];
+                    A[k + n * p] = akp * jpp + akq * jqp;
+                    A[k + n * q] = akp * jpq + akq * jqq;
+                }
+                /* A <- J^H A (rows p, q) */
+                for (int64_t k = 0; k < n; k++) {
+                    cplx apk = A[p + n * k], aqk = A[q + n * k];
+                    A[p + n * k] = conj(jpp) * apk + conj(jqp) * aqk;
+                    A[q + n * k] = conj(jpq) * apk + conj(jqq) * aqk;
+                }
+                A[p + n * q] = 0;
+                A[q + n * p] = 0;
+                A[p + n * p] = creal(A[p + n * p]);
+                A[q + n * q] = creal(A[q + n * q]);
+                for (int64_t k = 0; k < n; k++) {
+                    cplx ukp = U[k + n * p], ukq = U[k + n * q];
+                    U[k + n * p] = ukp * jpp + ukq * jqp;
+                    U[k + n * q] = ukp * jpq + ukq * jqq;
+                }
+            }
+    }
+    for (int64_t i = 0; i < n; i++) w[i] = creal(A[i + n * i]);
+    /* sort ascending */
+    for (int64_t i = 0; i < n; i++) {
+        int64_t k = i;
+        for (int64_t j = i + 1; j < n; j++) if (w[j] < w[k]) k = j;
+        if (k != i) {
+            double tw = w[i]; w[i] = w[k]; w[k] = tw;
+            for (int64_t r = 0; r < n; r++) {
+                cplx tu = U[r + n * i]; U[r + n * i] = U[r + n * k]; U[r + n * k] = tu;
+            }
+        }
+    }
+    free(A);
+}
+
+/* StdRegInv (maths.f90:8-83): B = U f(W) U^H with f from the regularisation
+ * type: 1 'tikh' w/(w^2+eps^2); 2 'poly' (w<eps -> (eps+w^2/eps)/2, then 1/w). */
+static void std_reg_inv(int64_t n, const cplx *A, double eps, int type, cplx *B) {
+    double *w = xcalloc(n, sizeof(double));
+    cplx *U = xcalloc((size_t)n * n, sizeof(cplx));
+    herm_eig(n, A, w, U);
+    for (int64_t j = 0; j < n; j++) {
+        if (type == 1) {
+            w[j] = w[j] / (w[j] * w[j] + eps * eps);
+        } else {
+            if (w[j] < eps) w[j] = 0.5 * (eps + w[j] * w[j] / eps);
+            w[j] = 1.0 / w[j];
+        }
+    }
+    for (int64_t j2 = 0; j2 < n; j2++)
+        for (int64_t j1 = 0; j1 < n; j1++) {
+            cplx acc = 0;
+            for (int64_t k = 0; k < n; k++) acc += w[k] * U[j1 + n * k] * conj(U[j2 + n * k]);
+            B[j1 + n * j2] = acc;
+        }
+    for (int64_t j = 0; j < n; j++) B[j + n * j] = creal(B[j + n * j]);
+    for (int64_t j2 = 0; j2 < n; j2++)
+        for (int64_t j1 = j2 + 1; j1 < n; j1++) B[j1 + n * j2] = conj(B[j2 + n * j1]);
+    free(w);
+    free(U);
+}
+
+/* Cholesky solve S X = Y for Hermitian positive definite S (zposv 'U',
+ * cmf2_derivatives.f90:170).  S: n x n (full), Y: n x m overwritten by X. */
+static int chol_solve(int64_t n, const cplx *S, int64_t lds, cplx *Y, int64_t ldy, int64_t m) {
+    cplx *U = xcalloc((size_t)n * n, sizeof(cplx));
+    for (int64_t j = 0; j < n; j++) {
+        double d = creal(S[j + lds * j]);
+        for (int64_t k = 0; k < j; k++) d -= creal(conj(U[k + n * j]) * U[k + n * j]);
+        if (!(d > 0.0)) { free(U); return -1; }
+        double ujj = sqrt(d);
+        U[j + n * j] = ujj;
+        for (int64_t i = j + 1; i < n; i++) {
+            cplx v = S[j + lds * i];
+            for (int64_t k = 0; k < j; k++) v -= conj(U[k + n * j]) * U[k + n * i];
+            U[j + n * i] = v / ujj;
+        }
+    }
+    for (int64_t c = 0; c < m; c++) {
+        cplx *y = Y + ldy * c;
+        /* U^H z = y */
+        for (int64_t i = 0; i < n; i++) {
+            cplx v = y[i];
+            for (int64_t k = 0; k < i; k++) v -= conj(U[k + n * i]) * y[k];
+            y[i] = v / U[i + n * i];
+        }
+        /* U x = z */
+        for (int64_t i = n - 1; i >= 0; i--) {
+            cplx v = y[i];
+            for (int64_t k = i + 1; k < n; k++) v -= U[i + n * k] * y[k];
+            y[i] = v / U[i + n * i];
+        }
+    }
+    free(U);
+    return 0;
+}
+
+/* Real symmetric tridiagonal eigenproblem (replaces dstev, cmf2_integration.f90:85)
+ * by cyclic Jacobi on the dense matrix; w ascending, U column eigenvectors. */
+static void tridiag_eig(int64_t n, const double *alpha, const double *beta, double *w, double *U) {
+    double A[NKRYLOV_MAX * NKRYLOV_MAX];
+    for (int64_t i = 0; i < n * n; i++) A[i] = 0.0;
+    for (int64_t i = 0; i < n; i++) {
+        A[i + n * i] = alpha[i];
+        if (i + 1 < n) { A[i + n * (i + 1)] = beta[i]; A[(i + 1) + n * i] = beta[i]; }
+    }
+    for (int64_t i = 0; i < n; i++)
+        for (int64_t j = 0; j < n; j++) U[i + n * j] = (i == j);
+    for (int sweep = 0; sweep < 100; sweep++) {
+        double off = 0.0, dg = 0.0;
+        for (int64_t j = 0; j < n; j++)
+            for (int64_t i = 0; i < n; i++) {
+                if (i != j) off += A[i + n * j] * A[i + n * j]; else dg += A[i + n * j] * A[i + n * j];
+            }
+        if (off <= 1e-32 * (dg + 1e-300) || off == 0.0) break;
+        for (int64_t p = 0; p < n - 1; p++)
+            for (int64_t q = p + 1; q < n; q++) {
+                double apq = A[p + n * q];
+                if (apq == 0.0) continue;
+                double theta = (A[q + n * q] - A[p + n * p]) / (2.0 * apq);
+                double t = (theta >= 0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
+                double c = 1.0 / sqrt(t * t + 1.0), s = t * c;
+                for (int64_t k = 0; k < n; k++) {
+                    double akp = A[k + n * p], akq = A[k + n * q];
+                    A[k + n * p] = c * akp - s * akq;
+                    A[k + n * q] = s * akp + c * akq;
+                }
+                for (int64_t k = 0; k < n; k++) {
+                    double apk = A[p + n * k], aqk = A[q + n * k];
+                    A[p + n * k] = c * apk - s * aqk;
+                    A[q + n * k] = s * apk + c * aqk;
+                }
+                A[p + n * q] = A[q + n * p] = 0.0;
+                for (int64_t k = 0; k < n; k++) {
+                    double ukp = U[k + n * p], ukq = U[k + n * q];
+                    U[k + n * p] = c * ukp - s * ukq;
+                    U[k + n * q] = s * ukp + c * ukq;
+                }
+            }
+    }
+    for (int64_t i = 0; i < n; i++) w[i] = A[i + n * i];
+    for (int64_t i = 0; i < n; i++) {
+        int64_t k = i;
+        for (int64_t j = i + 1; j < n; j++) if (w[j] < w[k]) k = j;
+        if (k != i) {
+            double tw = w[i]; w[i] = w[k]; w[k] = tw;
+            for (int64_t r = 0; r < n; r++) { double tu = U[r + n * i]; U[r + n * i] = U[r + n * k]; U[r + n * k] = tu; }
+        }
+    }
+}
+
+/* ---------------------------------------------------------------------- */
+/* Setup: layout (init_calc.f90:25-34, 160-181) and operator basis
+ * (alloc_h1mode hamilton.f90:91-147, alloc_IsIdentity :353-373). */
+
+orc *orc_create(const mg_problem *p) {
+    orc *o = xcalloc(1, sizeof(orc));
+    o->p = *p;
+    o->ns = p->nstates;
+    o->nm = p->nmodes;
+    o->nham = p->nham;
+    o->ngpmax = p->ngpmax;
+    const int64_t nm = o->nm, ns = o->ns, nd = p->ndof;
+    o->nmodeDOF = dup_i64(p->nmodeDOF, nm);
+    o->iDOF = dup_i64(p->iDOF, (size_t)p->maxdof * nm);
+    o->MultiSet = dup_i64(p->MultiSet, nm);
+    o->nSPF = dup_i64(p->nSPF, (size_t)nm * ns);
+    o->ngp = dup_i64(p->ngp, nd);
+    o->lhs = dup_i64(p->lhs_el, p->nham);
+    o->rhs = dup_i64(p->rhs_el, p->nham);
+    o->iOpDof = dup_i64(p->iOpDof, (size_t)nd * p->nham);
+    o->grid = dup_f64(p->grid, (size_t)p->ngpmax * nd);
+    o->d1 = dup_f64(p->d1, (size_t)p->ngpmax * p->ngpmax * nd);
+    o->d2 = dup_f64(p->d2, (size_t)p->ngpmax * p->ngpmax * nd);
+    o->xd1 = dup_f64(p->xd1, (size_t)p->ngpmax * p->ngpmax * nd);
+    o->ae = xcalloc(p->nham, sizeof(cplx));
+    for (int64_t t = 0; t < p->nham; t++) o->ae[t] = p->ae[2 * t] + I * p->ae[2 * t + 1];
+    o->der_factor = -I;
+    o->nspfmax = 0;
+    for (int64_t i = 0; i < nm * ns; i++) if (o->nSPF[i] > o->nspfmax) o->nspfmax = o->nSPF[i];
+    o->nAConf = xcalloc(ns, sizeof(int64_t));
+    o->AStateOffset = xcalloc(ns, sizeof(int64_t));
+    int64_t ic = 0;
+    for (int64_t s = 0; s < ns; s++) {
+        int64_t n = 1;
+        for (int64_t m = 0; m < nm; m++) n *= NSPF(o, m, s);
+        o->nAConf[s] = n;
+        o->AStateOffset[s] = ic;
+        ic += n;
+    }
+    o->nconf = ic;
+    o->ngpm = xcalloc(nm, sizeof(int64_t));
+    for (int64_t m = 0; m < nm; m++) {
+        int64_t g = 1;
+        for (int64_t k = 0; k < o->nmodeDOF[m]; k++) g *= o->ngp[o->iDOF[k + p->maxdof * m] - 1];
+        o->ngpm[m] = g;
+    }
+    o->ipsiSPF = xcalloc((size_t)nm * ns, sizeof(int64_t));
+    int64_t dim = 0;
+    for (int64_t s = 0; s < ns; s++)
+        for (int64_t m = 0; m < nm; m++) {
+            if (!o->MultiSet[m] && s > 0) { IPSI(o, m, s) = IPSI(o, m, 0); continue; }
+            IPSI(o, m, s) = dim + 1;
+            dim += o->ngpm[m] * NSPF(o, m, s);
+        }
+    o->dimpsi = dim;
+    /* operator basis */
+    oper_t *tmp = xcalloc((size_t)nm * o->nham, sizeof(oper_t));
+    o->nOper = xcalloc(nm, sizeof(int64_t));
+    o->iOper = xcalloc((size_t)nm * o->nham, sizeof(int64_t));
+    for (int64_t m = 0; m < nm; m++) {
+        int64_t nop = 0, nmd = o->nmodeDOF[m];
+        for (int64_t t = 0; t < o->nham; t++) {
+            oper_t v;
+            memset(&v, 0, sizeof v);
+            if (o->MultiSet[m]) { v.lhs = o->lhs[t]; v.rhs = o->rhs[t]; }
+            else { v.lhs = 1; v.rhs = 1; }
+            for (int64_t k = 0; k < nmd; k++)
+                v.codes[k] = o->iOpDof[(o->iDOF[k + p->maxdof * m] - 1) + nd * t];
+            int64_t found = -1;
+            for (int64_t i = 0; i < nop; i++) {
+                oper_t *u = &tmp[m * o->nham + i];
+                int same = (u->lhs == v.lhs) && (u->rhs == v.rhs);
+                for (int64_t k = 0; k < nmd && same; k++) same = (u->codes[k] == v.codes[k]);
+                if (same) { found = i; break; }
+            }
+            if (found < 0) { tmp[m * o->nham + nop] = v; found = nop++; }
+            o->iOper[m + nm * t] = found + 1;
+        }
+        o->nOper[m] = nop;
+        if (nop > o->nopmax) o->nopmax = nop;
+    }
+    o->Oper = xcalloc((size_t)nm * o->nopmax, sizeof(oper_t));
+    o->IsIdentity = xcalloc((size_t)nm * o->nopmax, sizeof(int64_t));
+    for (int64_t m = 0; m < nm; m++)
+        for (int64_t i = 0; i < o->nOper[m]; i++) {
+            oper_t v = tmp[m * o->nham + i];
+            o->Oper[m * o->nopmax + i] = v;
+            int ident = (v.lhs == v.rhs);
+            for (int64_t k = 0; k < o->nmodeDOF[m] && ident; k++) ident = (v.codes[k] == 0);
+            o->IsIdentity[m + nm * i] = ident;
+        }
+    free(tmp);
+    const size_t nn = (size_t)o->nspfmax * o->nspfmax;
+    o->h1 = xcalloc(nn * o->nopmax * nm, sizeof(cplx));
+    o->mf = xcalloc(nn * o->nopmax * nm, sizeof(cplx));
+    o->rmf = xcalloc(nn * o->nopmax * nm, sizeof(cplx));
+    o->rho = xcalloc(nn * nm * ns, sizeof(cplx));
+    o->rho_ave = xcalloc(nn * nm, sizeof(cplx));
+    /* identity h1mode for same-state identity operators (hamilton.f90:296-306) */
+    for (int64_t m = 0; m < nm; m++)
+        for (int64_t i = 0; i < o->nOper[m]; i++)
+            if (o->IsIdentity[m + nm * i]) {
+                int64_t s = o->Oper[m * o->nopmax + i].lhs - 1;
+                for (int64_t j = 0; j < NSPF(o, m, s); j++) H1(o, m, i)[j + o->nspfmax * j] = 1.0;
+            }
+    o->A = xcalloc(o->nconf, sizeof(cplx));
+    o->Astart = xcalloc(o->nconf, sizeof(cplx));
+    o->psi = xcalloc(o->dimpsi, sizeof(cplx));
+    o->psistart = xcalloc(o->dimpsi, sizeof(cplx));
+    o->BK = xcalloc((size_t)o->nconf * NKRYLOV_MAX, sizeof(cplx));
+    for (int64_t m = 0; m < nm; m++)
+        if (o->nmodeDOF[m] > 8) snprintf(o->err, sizeof o->err, "more than 8 DOFs in a mode");
+    return o;
+}
+
+void orc_destroy(orc *o) {
+    if (!o) return;
+    void *ptrs[] = {o->nmodeDOF, o->iDOF, o->MultiSet, o->nSPF, o->ngp, o->lhs, o->rhs,
+                    o->iOpDof, o->grid, o->d1, o->d2, o->xd1, o->ae, o->nAConf,
+                    o->AStateOffset, o->ipsiSPF, o->ngpm, o->nOper, o->Oper, o->iOper,
+                    o->IsIdentity, o->A, o->psi, o->Astart, o->psistart, o->h1, o->rho,
+                    o->rho_ave, o->mf, o->rmf, o->BK};
+    for (size_t i = 0; i < sizeof ptrs / sizeof ptrs[0]; i++) free(ptrs[i]);
+    free(o);
+}
+
+const char *orc_error(const orc *o) { return o->err; }
+int64_t orc_nconf(const orc *o) { return o->nconf; }
+int64_t orc_dimpsi(const orc *o) { return o->dimpsi; }
+int64_t orc_nspfmax(const orc *o) { return o->nspfmax; }
+int64_t orc_nopmax(const orc *o) { return o->nopmax; }
+
+void orc_basis(const orc *o, int64_t *nOper, int64_t *iOper, int64_t *isIdentity) {
+    memcpy(nOper, o->nOper, sizeof(int64_t) * o->nm);
+    memcpy(iOper, o->iOper, sizeof(int64_t) * o->nm * o->nham);
+    memcpy(isIdentity, o->IsIdentity, sizeof(int64_t) * o->nm * o->nopmax);
+}
+
+void orc_set_state(orc *o, const double *A, const double *psi) {
+    memcpy(o->A, A, sizeof(cplx) * o->nconf);
+    memcpy(o->psi, psi, sizeof(cplx) * o->dimpsi);
+}
+void orc_get_state(const orc *o, double *A, double *psi) {
+    memcpy(A, o->A, sizeof(cplx) * o->nconf);
+    memcpy(psi, o->psi, sizeof(cplx) * o->dimpsi);
+}
+void orc_set_start(orc *o, const double *A, const double *psi) {
+    memcpy(o->Astart, A, sizeof(cplx) * o->nconf);
+    memcpy(o->psistart, psi, sizeof(cplx) * o->dimpsi);
+}
+
+/* ---------------------------------------------------------------------- */
+/* DVR primitive operators on an SPF block (MultTens / MultTensDiag,
+ * hamilton.f90:577-629, and the case list of Calc_h1mode :452-470).
+ * blk: ngpm x nS (column-major), the DOF k of mode m is acted on. */
+static void apply_prim(orc *o, int64_t m, int64_t k, int64_t code, int64_t nS, cplx *blk) {
+    const int64_t maxdof = o->p.maxdof, gmax = o->ngpmax;
+    const int64_t dof = o->iDOF[k + maxdof * m] - 1;
+    int64_t ll = 1, nl = 1;
+    for (int64_t q = 0; q < k; q++) ll *= o->ngp[o->iDOF[q + maxdof * m] - 1];
+    for (int64_t q = k + 1; q < o->nmodeDOF[m]; q++) nl *= o->ngp[o->iDOF[q + maxdof * m] - 1];
+    const int64_t ng = o->ngp[dof];
+    const int64_t nblk = nS * nl;
+    if (code >= 1 && code <= 20) {
+        const double *x = o->grid + gmax * dof;
+        for (int64_t b = 0; b < nblk; b++)
+            for (int64_t g = 0; g < ng; g++) {
+                double v = x[g];
+                for (int64_t e = 1; e < code; e++) v *= x[g];
+                cplx *c = blk + (b * ng + g) * ll;
+                for (int64_t l = 0; l < ll; l++) c[l] *= v;
+            }
+        return;
+    }
+    const double *Mat;
+    cplx fac = 1.0;
+    switch (code) {
+    case -1: Mat = o->d1 + gmax * gmax * dof; fac = I; break;
+    case -2: Mat = o->d2 + gmax * gmax * dof; break;
+    case -3: Mat = o->xd1 + gmax * gmax * dof; fac = I; break;
+    default:
+        snprintf(o->err, sizeof o->err, "primitive operator %lld not supported", (long long)code);
+        return;
+    }
+    cplx *tmp = xcalloc(ng * ll, sizeof(cplx));
+    for (int64_t b = 0; b < nblk; b++) {
+        cplx *c = blk + b * ng * ll;
+        /* zlacrm: C(l, g') = sum_g A(l, g) Mat(g, g') */
+        for (int64_t g2 = 0; g2 < ng; g2++)
+            for (int64_t l = 0; l < ll; l++) {
+                cplx acc = 0;
+                for (int64_t g = 0; g < ng; g++) acc += c[l + ll * g] * Mat[g + gmax * g2];
+                tmp[l + ll * g2] = acc;
+            }
+        for (int64_t i = 0; i < ng * ll; i++) c[i] = fac * tmp[i];
+    }
+    free(tmp);
+}
+
+/* Calc_h1mode, DVR branch (hamilton.f90:433-485). */
+static void calc_h1mode_mode(orc *o, const cplx *psi0, int64_t m) {
+    const int64_t nmax = o->nspfmax, nG = o->ngpm[m];
+    for (int64_t i = 0; i < o->nOper[m]; i++) {
+        if (o->IsIdentity[m + o->nm * i]) continue;
+        oper_t *op = &o->Oper[m * o->nopmax + i];
+        int64_t s1 = op->lhs - 1, s2 = op->rhs - 1;
+        int64_t nS1 = NSPF(o, m, s1), nS2 = NSPF(o, m, s2);
+        const cplx *phi2 = psi0 + IPSI(o, m, s2) - 1;
+        const cplx *phi1 = psi0 + IPSI(o, m, s1) - 1;
+        cplx *sect = xcalloc(nG * nS2, sizeof(cplx));
+        memcpy(sect, phi2, sizeof(cplx) * nG * nS2);
+        for (int64_t k = 0; k < o->nmodeDOF[m]; k++)
+            if (op->codes[k] != 0) apply_prim(o, m, k, op->codes[k], nS2, sect);
+        cplx *h = H1(o, m, i);
+        if (s1 == s2) {
+            /* zher2k('U','C', 1/2): h = (phi^H sect + sect^H phi)/2, Hermitian */
+            for (int64_t b = 0; b < nS2; b++)
+                for (int64_t a = 0; a <= b; a++) {
+                    cplx x = 0, y = 0;
+                    for (int64_t g = 0; g < nG; g++) {
+                        x += conj(phi2[g + nG * a]) * sect[g + nG * b];
+                        y += conj(sect[g + nG * a]) * phi2[g + nG * b];
+                    }
+                    cplx v = 0.5 * x + 0.5 * y;
+                    if (a == b) v = creal(v);
+                    h[a + nmax * b] = v;
+                    h[b + nmax * a] = conj(v);
+                }
+        } else {
+            for (int64_t b = 0; b < nS2; b++)
+                for (int64_t a = 0; a < nS1; a++) {
+                    cplx x = 0;
+                    for (int64_t g = 0; g < nG; g++) x += conj(phi1[g + nG * a]) * sect[g + nG * b];
+                    h[a + nmax * b] = x;
+                }
+        }
+        free(sect);
+    }
+}
+
+void orc_calc_h1mode(orc *o, double *h1_out) {
+    for (int64_t m = 0; m < o->nm; m++) calc_h1mode_mode(o, o->psi, m);
+    if (h1_out) memcpy(h1_out, o->h1, sizeof(cplx) * o->nspfmax * o->nspfmax * o->nopmax * o->nm);
+}
+
+/* Calc_rho1mode (gmctdh_objects.f90:408-477) for DVR modes. */
+static void calc_rho(orc *o, const cplx *A0) {
+    const int64_t nm = o->nm, nmax = o->nspfmax;
+    for (int64_t m = 0; m < nm; m++) {
+        for (int64_t s = 0; s < o->ns; s++) {
+            cplx *r = RHO(o, m, s);
+            for (int64_t i = 0; i < nmax * nmax; i++) r[i] = 0;
+            int64_t ll = 1, nl = 1, nS = NSPF(o, m, s);
+            for (int64_t q = 0; q < m; q++) ll *= NSPF(o, q, s);
+            for (int64_t q = m + 1; q < nm; q++) nl *= NSPF(o, q, s);
+            const cplx *As = A0 + o->AStateOffset[s];
+            hole_gram(ll, nl, nS, nS, As, As, 1.0, r, nmax);
+            for (int64_t j = 0; j < nS; j++) r[j + nmax * j] = creal(r[j + nmax * j]);
+            for (int64_t b = 0; b < nS; b++)
+                for (int64_t a = b + 1; a < nS; a++) r[a + nmax * b] = conj(r[b + nmax * a]);
+        }
+        if (!o->MultiSet[m]) {
+            cplx *ra = RHOAVE(o, m);
+            for (int64_t i = 0; i < nmax * nmax; i++) {
+                cplx acc = 0;
+                for (int64_t s = 0; s < o->ns; s++) acc += RHO(o, m, s)[i];
+                ra[i] = acc;
+            }
+        }
+    }
+}
+
+void orc_calc_rho(orc *o, double *rho_out) {
+    calc_rho(o, o->A);
+    if (rho_out) memcpy(rho_out, o->rho, sizeof(cplx) * o->nspfmax * o->nspfmax * o->nm * o->ns);
+}
+
+/* Largest tensor a chain over mixed states can produce: prod_m max_s n(m,s). */
+static int64_t mixed_size(const orc *o) {
+    int64_t big = 1;
+    for (int64_t m = 0; m < o->nm; m++) {
+        int64_t mx = 0;
+        for (int64_t s = 0; s < o->ns; s++) if (NSPF(o, m, s) > mx) mx = NSPF(o, m, s);
+        big *= mx;
+    }
+    return big;
+}
+
+/* Calc_MeanField, Hamiltonian part (gmctdh_objects.f90:518-605), then
+ * Calc_rhom1MF (:1147-1194). */
+static int calc_meanfield(orc *o, const cplx *A0) {
+    const int64_t nm = o->nm, nmax = o->nspfmax, nn = nmax * nmax;
+    memset(o->mf, 0, sizeof(cplx) * nn * o->nopmax * nm);
+    o->energyMF = 0.0;
+    /* mode products can change the size of a state block (n_s1 vs n_s2) */
+    const int64_t big = mixed_size(o);
+    cplx *AL = xcalloc((size_t)big * nm, sizeof(cplx));
+    cplx *AR = xcalloc((size_t)big * (nm + 1), sizeof(cplx));
+    int64_t sz[64];
+    for (int64_t t = 0; t < o->nham; t++) {
+        const int64_t s1 = o->lhs[t] - 1, s2 = o->rhs[t] - 1;
+        const cplx ae = o->ae[t];
+        /* right chain: R_0 = A_s2, modes f..1 */
+        for (int64_t m = 0; m < nm; m++) sz[m] = NSPF(o, m, s2);
+        int64_t cur = o->nAConf[s2];
+        memcpy(AR, A0 + o->AStateOffset[s2], sizeof(cplx) * cur);
+        for (int64_t im = 1; im <= nm; im++) {
+            int64_t jm = nm - im;
+            int64_t op = o->iOper[jm + nm * t] - 1;
+            cplx *src = AR + (size_t)big * (im - 1), *dst = AR + (size_t)big * im;
+            if (o->IsIdentity[jm + nm * op]) { memcpy(dst, src, sizeof(cplx) * cur); continue; }
+            int64_t nS1 = NSPF(o, jm, s1);
+            mode_product(nm, sz, jm, H1(o, jm, op), nmax, nS1, src, dst);
+            cur = cur / sz[jm] * nS1;
+            sz[jm] = nS1;
+        }
+        /* left chain: L_0 = A_s1, modes 1..f-1, with h^H */
+        for (int64_t m = 0; m < nm; m++) sz[m] = NSPF(o, m, s1);
+        cur = o->nAConf[s1];
+        memcpy(AL, A0 + o->AStateOffset[s1], sizeof(cplx) * cur);
+        cplx *hh = xcalloc(nn, sizeof(cplx));
+        for (int64_t im = 1; im <= nm - 1; im++) {
+            int64_t m = im - 1;
+            int64_t op = o->iOper[m + nm * t] - 1;
+            cplx *src = AL + (size_t)big * (im - 1), *dst = AL + (size_t)big * im;
+            if (o->IsIdentity[m + nm * op]) { memcpy(dst, src, sizeof(cplx) * cur); continue; }
+            int64_t nS1 = NSPF(o, m, s1), nS2 = NSPF(o, m, s2);
+            for (int64_t a = 0; a < nS2; a++)
+                for (int64_t b = 0; b < nS1; b++) hh[a + nmax * b] = conj(H1(o, m, op)[b + nmax * a]);
+            mode_product(nm, sz, m, hh, nmax, nS2, src, dst);
+            cur = cur / sz[m] * nS2;
+            sz[m] = nS2;
+        }
+        free(hh);
+        /* energy: Re(ae * <A_s1 | R_f>) */
+        {
+            const cplx *a1 = A0 + o->AStateOffset[s1];
+            const cplx *rf = AR + (size_t)big * nm;
+            cplx acc = 0;
+            for (int64_t i = 0; i < o->nAConf[s1]; i++) acc += conj(a1[i]) * rf[i];
+            o->energyMF += creal(ae * acc);
+        }
+        /* mean fields */
+        for (int64_t im = 1; im <= nm; im++) {
+            int64_t m = im - 1;
+            int64_t op = o->iOper[m + nm * t] - 1;
+            if (o->IsIdentity[m + nm * op]) continue;
+            int64_t jm = nm - im;
+            int64_t nS1 = NSPF(o, m, s1), nS2 = NSPF(o, m, s2);
+            int64_t ll = 1, nl = 1;
+            for (int64_t q = 0; q < m; q++) ll *= NSPF(o, q, s2);
+            for (int64_t q = m + 1; q < nm; q++) nl *= NSPF(o, q, s1);
+            hole_gram(ll, nl, nS1, nS2, AL + (size_t)big * (im - 1), AR + (size_t)big * jm, ae,
+                      MF(o, m, op), nmax);
+        }
+    }
+    free(AL);
+    free(AR);
+    /* Calc_rhom1MF */
+    cplx *rinv = xcalloc(nn, sizeof(cplx)), *rsq = xcalloc(nn, sizeof(cplx));
+    for (int64_t m = 0; m < nm; m++) {
+        int64_t nsets = o->MultiSet[m] ? o->ns : 1;
+        for (int64_t s = 0; s < nsets; s++) {
+            int64_t nS = NSPF(o, m, s);
+            const cplx *r = o->MultiSet[m] ? RHO(o, m, s) : RHOAVE(o, m);
+            for (int64_t b = 0; b < nS; b++)
+                for (int64_t a = 0; a < nS; a++) rsq[a + nS * b] = r[a + nmax * b];
+            std_reg_inv(nS, rsq, o->p.eps_rho, 2, rinv);
+            for (int64_t i = 0; i < o->nOper[m]; i++) {
+                oper_t *op = &o->Oper[m * o->nopmax + i];
+                if (o->IsIdentity[m + nm * i]) continue;
+                if (o->MultiSet[m] && op->lhs - 1 != s) continue;
+                int64_t nS2 = NSPF(o, m, op->rhs - 1);
+                cplx *dst = RMF(o, m, i);
+                const cplx *src = MF(o, m, i);
+                for (int64_t b = 0; b < nS2; b++)
+                    for (int64_t a = 0; a < nS; a++) {
+                        cplx acc = 0;
+                        for (int64_t k = 0; k < nS; k++) acc += rinv[a + nS * k] * src[k + nmax * b];
+                        dst[a + nmax * b] = acc;
+                    }
+            }
+        }
+    }
+    free(rinv);
+    free(rsq);
+    return 0;
+}
+
+int orc_calc_meanfield(orc *o, double *mf_out, double *rmf_out, double *energy_out) {
+    calc_meanfield(o, o->A);
+    size_t n = (size_t)o->nspfmax * o->nspfmax * o->nopmax * o->nm;
+    if (mf_out) memcpy(mf_out, o->mf, sizeof(cplx) * n);
+    if (rmf_out) memcpy(rmf_out, o->rmf, sizeof(cplx) * n);
+    if (energy_out) *energy_out = o->energyMF;
+    return 0;
+}
+
+/* Calc_DHD_B, DVR modes (cmf2_derivatives.f90:1-80). */
+static void apply_h(orc *o, const cplx *B, cplx *HB) {
+    const int64_t nm = o->nm, nmax = o->nspfmax;
+    memset(HB, 0, sizeof(cplx) * o->nconf);
+    const int64_t big = mixed_size(o);
+    cplx *b1 = xcalloc(big, sizeof(cplx)), *b2 = xcalloc(big, sizeof(cplx));
+    int64_t sz[64];
+    for (int64_t t = 0; t < o->nham; t++) {
+        const int64_t s1 = o->lhs[t] - 1, s2 = o->rhs[t] - 1;
+        for (int64_t m = 0; m < nm; m++) sz[m] = NSPF(o, m, s2);
+        memcpy(b2, B + o->AStateOffset[s2], sizeof(cplx) * o->nAConf[s2]);
+        for (int64_t m = 0; m < nm; m++) {
+            int64_t op = o->iOper[m + nm * t] - 1;
+            if (o->IsIdentity[m + nm * op]) continue;
+            int64_t nS1 = NSPF(o, m, s1);
+            mode_product(nm, sz, m, H1(o, m, op), nmax, nS1, b2, b1);
+            cplx *sw = b1; b1 = b2; b2 = sw;
+            sz[m] = nS1;
+        }
+        cplx ae = o->ae[t];
+        cplx *out = HB + o->AStateOffset[s1];
+        for (int64_t i = 0; i < o->nAConf[s1]; i++) out[i] += ae * b2[i];
+    }
+    free(b1);
+    free(b2);
+}
+
+void orc_apply_h(orc *o, const double *B, double *HB) { apply_h(o, (const cplx *)B, (cplx *)HB); }
+
+/* Calc_Der_SPF, DVR branch (cmf2_derivatives.f90:118-173); mode m is 0-based. */
+static int der_spf(orc *o, int64_t m, const cplx *psi0, cplx *dpsi) {
+    const int64_t nmax = o->nspfmax, nG = o->ngpm[m], nm = o->nm;
+    int64_t nsets = o->MultiSet[m] ? o->ns : 1;
+    for (int64_t s = 0; s < nsets; s++)
+        memset(dpsi + IPSI(o, m, s) - 1, 0, sizeof(cplx) * nG * NSPF(o, m, s));
+    cplx *sect = xcalloc(nG * nmax, sizeof(cplx));
+    for (int64_t i = 0; i < o->nOper[m]; i++) {
+        if (o->IsIdentity[m + nm * i]) continue;
+        oper_t *op = &o->Oper[m * o->nopmax + i];
+        int64_t s1 = op->lhs - 1, s2 = op->rhs - 1;
+        int64_t nS1 = NSPF(o, m, s1), nS2 = NSPF(o, m, s2);
+        memcpy(sect, psi0 + IPSI(o, m, s2) - 1, sizeof(cplx) * nG * nS2);
+        for (int64_t k = 0; k < o->nmodeDOF[m]; k++)
+            if (op->codes[k] != 0) apply_prim(o, m, k, op->codes[k], nS2, sect);
+        cplx *d = dpsi + IPSI(o, m, s1) - 1;
+        const cplx *r = RMF(o, m, i);
+        /* zgemm('N','T'): d += der_factor * sect * rmf^T */
+        for (int64_t j1 = 0; j1 < nS1; j1++)
+            for (int64_t g = 0; g < nG; g++) {
+                cplx acc = 0;
+                for (int64_t j2 = 0; j2 < nS2; j2++) acc += sect[g + nG * j2] * r[j1 + nmax * j2];
+                d[g + nG * j1] += o->der_factor * acc;
+            }
+    }
+    free(sect);
+    /* projector (1 - phi (phi^H phi)^-1 phi^H) */
+    cplx *ovl = xcalloc(nmax * nmax, sizeof(cplx)), *od = xcalloc(nmax * nmax, sizeof(cplx));
+    int rc = 0;
+    for (int64_t s = 0; s < nsets; s++) {
+        int64_t nS = NSPF(o, m, s);
+        const cplx *phi = psi0 + IPSI(o, m, s) - 1;
+        cplx *d = dpsi + IPSI(o, m, s) - 1;
+        for (int64_t b = 0; b < nS; b++)
+            for (int64_t a = 0; a < nS; a++) {
+                cplx x = 0, y = 0;
+                for (int64_t g = 0; g < nG; g++) {
+                    x += conj(phi[g + nG * a]) * phi[g + nG * b];
+                    y += conj(phi[g + nG * a]) * d[g + nG * b];
+                }
+                ovl[a + nS * b] = x;
+                od[a + nS * b] = y;
+            }
+        if (chol_solve(nS, ovl, nS, od, nS, nS) != 0) {
+            snprintf(o->err, sizeof o->err, "SPF overlap not positive definite (mode %lld)", (long long)m + 1);
+            rc = -1;
+        }
+        for (int64_t b = 0; b < nS; b++)
+            for (int64_t g = 0; g < nG; g++) {
+                cplx acc = 0;
+                for (int64_t k = 0; k < nS; k++) acc += phi[g + nG * k] * od[k + nS * b];
+                d[g + nG * b] -= acc;
+            }
+    }
+    free(ovl);
+    free(od);
+    return rc;
+}
+
+void orc_der_spf(orc *o, int64_t im, const double *psi_in, double *dpsi_out) {
+    der_spf(o, im - 1, (const cplx *)psi_in, (cplx *)dpsi_out);
+}
+
+/* Calc_Err_SPF, DVR branch (cmf2_derivatives.f90:239-291). */
+static double err_spf(orc *o, int64_t m, const cplx *p1, const cplx *p2) {
+    const int64_t nmax = o->nspfmax, nG = o->ngpm[m];
+    int64_t nsets = o->MultiSet[m] ? o->ns : 1;
+    double error = 0.0;
+    cplx *dl = xcalloc(nG * nmax, sizeof(cplx));
+    for (int64_t s = 0; s < nsets; s++) {
+        int64_t nS = NSPF(o, m, s), off = IPSI(o, m, s) - 1;
+        for (int64_t i = 0; i < nG * nS; i++) dl[i] = p1[off + i] - p2[off + i];
+        const cplx *r = o->MultiSet[m] ? RHO(o, m, s) : RHOAVE(o, m);
+        for (int64_t j2 = 0; j2 < nS; j2++) {
+            for (int64_t j1 = 0; j1 < j2; j1++) {
+                cplx ov = 0;
+                for (int64_t g = 0; g < nG; g++) ov += conj(dl[g + nG * j1]) * dl[g + nG * j2];
+                error += 2.0 * creal(r[j1 + nmax * j2] * conj(ov));
+            }
+            double dd = 0.0;
+            for (int64_t g = 0; g < nG; g++) dd += creal(conj(dl[g + nG * j2]) * dl[g + nG * j2]);
+            error += creal(r[j2 + nmax * j2] * dd);
+        }
+    }
+    free(dl);
+    return sqrt(fabs(error));
+}
+
+/* rk_SPF_ck / rk_SPF_dp (cmf2_integration.f90:107-412) on the blocks of mode m. */
+static int rk_spf(orc *o, int64_t m, cplx *psi0, double htry, double *hdid, double *hnext) {
+    const double safety = 0.9, pgrow = -0.2, pshrnk = -0.25, minscal = 0.1, maxscal = 5.0;
+    const double errcon = pow(5.0 / safety, 1.0 / pgrow);
+    const int dp = (o->p.int_type_psi == 2);
+    const int64_t D = o->dimpsi, nG = o->ngpm[m];
+    int64_t nsets = o->MultiSet[m] ? o->ns : 1;
+    cplx *k[7], *tmp = xcalloc(D, sizeof(cplx)), *perr = xcalloc(D, sizeof(cplx));
+    for (int i = 0; i < 7; i++) k[i] = xcalloc(D, sizeof(cplx));
+    /* Cash-Karp (1990) and Dormand-Prince (1980) tableaux */
+    static const double ckB[6][5] = {
+        {0, 0, 0, 0, 0}, {0.2, 0, 0, 0, 0}, {0.075, 0.225, 0, 0, 0}, {0.3, -0.9, 1.2, 0, 0},
+        {-11.0 / 54.0, 2.5, -70.0 / 27.0, 35.0 / 27.0, 0},
+        {1631.0 / 55296.0, 175.0 / 512.0, 575.0 / 13824.0, 44275.0 / 110592.0, 253.0 / 4096.0}};
+    static const double ckC[6] = {37.0 / 378.0, 0, 250.0 / 621.0, 125.0 / 594.0, 0, 512.0 / 1771.0};
+    static const double ckD[6] = {2825.0 / 27648.0, 0, 18575.0 / 48384.0, 13525.0 / 55296.0,
+                                  277.0 / 14336.0, 0.25};
+    static const double dpB[6][5] = {
+        {0, 0, 0, 0, 0}, {0.2, 0, 0, 0, 0}, {0.075, 0.225, 0, 0, 0},
+        {44.0 / 45.0, -56.0 / 15.0, 32.0 / 9.0, 0, 0},
+        {19372.0 / 6561.0, -25360.0 / 2187.0, 64448.0 / 6561.0, -212.0 / 729.0, 0},
+        {9017.0 / 3168.0, -355.0 / 33.0, 46732.0 / 5247.0, 49.0 / 176.0, -5103.0 / 18656.0}};
+    static const double dpC[6] = {35.0 / 384.0, 0, 500.0 / 1113.0, 125.0 / 192.0,
+                                  -2187.0 / 6784.0, 11.0 / 84.0};
+    static const double dpD[7] = {5179.0 / 57600.0, 0, 7571.0 / 16695.0, 393.0 / 640.0,
+                                  -92097.0 / 339200.0, 187.0 / 2100.0, 0.025};
+    double dt = htry, error = 0.0;
+    int rc = 0;
+    for (;;) {
+        rc |= der_spf(o, m, psi0, k[0]);
+        for (int st = 1; st < 6; st++) {
+            const double *b = dp ? dpB[st] : ckB[st];
+            for (int64_t s = 0; s < nsets; s++) {
+                int64_t i1 = IPSI(o, m, s) - 1, n = nG * NSPF(o, m, s);
+                for (int64_t i = i1; i < i1 + n; i++) {
+                    if (st == 1) { tmp[i] = psi0[i] + b[0] * dt * k[0][i]; continue; }
+                    cplx acc = 0;
+                    for (int q = 0; q < st; q++) acc += b[q] * k[q][i];
+                    tmp[i] = psi0[i] + dt * acc;
+                }
+            }
+            rc |= der_spf(o, m, tmp, k[st]);
+        }
+        for (int64_t s = 0; s < nsets; s++) {
+            int64_t i1 = IPSI(o, m, s) - 1, n = nG * NSPF(o, m, s);
+            for (int64_t i = i1; i < i1 + n; i++) {
+                const double *c = dp ? dpC : ckC;
+                cplx acc = 0;
+                for (int q = 0; q < 6; q++) if (c[q] != 0.0) acc += c[q] * k[q][i];
+                tmp[i] = psi0[i] + dt * acc;
+                if (!dp) {
+                    cplx ae = 0;
+                    for (int q = 0; q < 6; q++) if (ckD[q] != 0.0) ae += ckD[q] * k[q][i];
+                    perr[i] = psi0[i] + dt * ae;
+                }
+            }
+        }
+        if (dp) {
+            rc |= der_spf(o, m, tmp, k[6]);
+            for (int64_t s = 0; s < nsets; s++) {
+                int64_t i1 = IPSI(o, m, s) - 1, n = nG * NSPF(o, m, s);
+                for (int64_t i = i1; i < i1 + n; i++) {
+                    cplx ae = 0;
+                    for (int q = 0; q < 7; q++) if (dpD[q] != 0.0) ae += dpD[q] * k[q][i];
+                    perr[i] = psi0[i] + dt * ae;
+                }
+            }
+        }
+        error = err_spf(o, m, tmp, perr) / o->p.eps_psi;
+        if (error < 1.0) break;
+        double a = minscal * dt, b2 = safety * dt * pow(error, pshrnk);
+        dt = a > b2 ? a : b2;
+        if (rc) break;
+    }
+    *hnext = (error > errcon) ? safety * dt * pow(error, pgrow) : dt * maxscal;
+    *hdid = dt;
+    for (int64_t s = 0; s < nsets; s++) {
+        int64_t i1 = IPSI(o, m, s) - 1, n = nG * NSPF(o, m, s);
+        memcpy(psi0 + i1, tmp + i1, sizeof(cplx) * n);
+    }
+    for (int i = 0; i < 7; i++) free(k[i]);
+    free(tmp);
+    free(perr);
+    return rc;
+}
+
+/* propagate_SPF (cmf2_base.f90:210-260). */
+static int propagate_spf(orc *o, double dt, int64_t m, cplx *psi) {
+    double dt_done = 0.0, stepsize = dt, step_done, step_next;
+    const double minstep = o->p.min_stepsize;
+    int rc = 0;
+    for (;;) {
+        rc |= rk_spf(o, m, psi, stepsize, &step_done, &step_next);
+        dt_done = dt_done + step_done;
+        if (dt_done >= dt || rc) break;
+        if (step_next < minstep * 2) step_next = minstep * 2;
+        stepsize = (dt - dt_done) / (double)((int64_t)((dt - dt_done) / step_next) + 1);
+        if (stepsize > dt - dt_done) stepsize = dt - dt_done;
+    }
+    return rc;
+}
+
+int orc_propagate_spf(orc *o, double dt, int64_t i, double *psi) {
+    return propagate_spf(o, dt, i - 1, (cplx *)psi);
+}
+
+/* norm_wf and WF_Ovl, DVR modes (gmctdh_objects.f90:21-97, 101-180). */
+static double norm_wf(orc *o, const cplx *A0, const cplx *psi0) {
+    const int64_t nm = o->nm, nmax = o->nspfmax;
+    int64_t big = 0;
+    for (int64_t s = 0; s < o->ns; s++) if (o->nAConf[s] > big) big = o->nAConf[s];
+    cplx *p1 = xcalloc(big, sizeof(cplx)), *p2 = xcalloc(big, sizeof(cplx));
+    cplx *S = xcalloc(nmax * nmax, sizeof(cplx));
+    double wfnorm = 0.0;
+    int64_t sz[64];
+    for (int64_t s = 0; s < o->ns; s++) {
+        for (int64_t m = 0; m < nm; m++) sz[m] = NSPF(o, m, s);
+        memcpy(p1, A0 + o->AStateOffset[s], sizeof(cplx) * o->nAConf[s]);
+        for (int64_t m = 0; m < nm; m++) {
+            int64_t nS = sz[m], nG = o->ngpm[m];
+            const cplx *phi = psi0 + IPSI(o, m, s) - 1;
+            for (int64_t b = 0; b < nS; b++)
+                for (int64_t a = 0; a <= b; a++) {
+                    cplx x = 0;
+                    for (int64_t g = 0; g < nG; g++) x += conj(phi[g + nG * a]) * phi[g + nG * b];
+                    if (a == b) x = creal(x);
+                    S[a + nmax * b] = x;
+                    S[b + nmax * a] = conj(x);
+                }
+            mode_product(nm, sz, m, S, nmax, nS, p1, p2);
+            cplx *sw = p1; p1 = p2; p2 = sw;
+        }
+        const cplx *a = A0 + o->AStateOffset[s];
+        cplx acc = 0;
+        for (int64_t i = 0; i < o->nAConf[s]; i++) acc += conj(p1[i]) * a[i];
+        wfnorm += creal(acc);
+    }
+    free(p1); free(p2); free(S);
+    return wfnorm;
+}
+
+static cplx wf_ovl(orc *o, const cplx *A1, const cplx *psi1, const cplx *A2, const cplx *psi2) {
+    const int64_t nm = o->nm, nmax = o->nspfmax;
+    int64_t big = 0;
+    for (int64_t s = 0; s < o->ns; s++) if (o->nAConf[s] > big) big = o->nAConf[s];
+    cplx *p1 = xcalloc(big, sizeof(cplx)), *p2 = xcalloc(big, sizeof(cplx));
+    cplx *S = xcalloc(nmax * nmax, sizeof(cplx));
+    cplx overlap = 0;
+    int64_t sz[64];
+    for (int64_t s = 0; s < o->ns; s++) {
+        for (int64_t m = 0; m < nm; m++) sz[m] = NSPF(o, m, s);
+        memcpy(p1, A2 + o->AStateOffset[s], sizeof(cplx) * o->nAConf[s]);
+        for (int64_t m = 0; m < nm; m++) {
+            int64_t nS = sz[m], nG = o->ngpm[m];
+            const cplx *f1 = psi1 + IPSI(o, m, s) - 1, *f2 = psi2 + IPSI(o, m, s) - 1;
+            for (int64_t b = 0; b < nS; b++)
+                for (int64_t a = 0; a < nS; a++) {
+                    cplx x = 0;
+                    for (int64_t g = 0; g < nG; g++) x += conj(f1[g + nG * a]) * f2[g + nG * b];
+                    S[a + nmax * b] = x;
+                }
+            mode_product(nm, sz, m, S, nmax, nS, p1, p2);
+            cplx *sw = p1; p1 = p2; p2 = sw;
+        }
+        const cplx *a = A1 + o->AStateOffset[s];
+        for (int64_t i = 0; i < o->nAConf[s]; i++) overlap += conj(a[i]) * p1[i];
+    }
+    free(p1); free(p2); free(S);
+    return overlap;
+}
+
+double orc_norm_wf(orc *o, const double *A, const double *psi) {
+    return norm_wf(o, (const cplx *)A, (const cplx *)psi);
+}
+void orc_wf_ovl(orc *o, const double *A1, const double *psi1, const double *A2,
+                const double *psi2, double *ovl) {
+    cplx v = wf_ovl(o, (const cplx *)A1, (const cplx *)psi1, (const cplx *)A2, (const cplx *)psi2);
+    ovl[0] = creal(v);
+    ovl[1] = cimag(v);
+}
+
+/* DeltaPsi, npackets = 1 (gmctdh_objects.f90:206-227). */
+static double delta_psi(orc *o, const cplx *A1, const cplx *psi1, const cplx *A2, const cplx *psi2) {
+    double o11 = norm_wf(o, A1, psi1), o22 = norm_wf(o, A2, psi2);
+    cplx o12 = wf_ovl(o, A1, psi1, A2, psi2);
+    return sqrt(fabs(2.0 * (1.0 - creal(o12) / sqrt(o11 * o22))));
+}
+
+static double vnorm(const cplx *x, int64_t n) {
+    double acc = 0.0;
+    for (int64_t i = 0; i < n; i++) acc += creal(x[i]) * creal(x[i]) + cimag(x[i]) * cimag(x[i]);
+    return sqrt(acc);
+}
+
+/* Lanczos (cmf2_integration.f90:30-103). Returns nKrylov, or -1 when the
+ * maximum Krylov size is reached (the reference stops there). */
+static int lanczos(orc *o, const cplx *B0, cplx *BT, double dt, int build) {
+    const int64_t N = o->nconf;
+    cplx *BK = o->BK;
+    if (build) {
+        double beta[NKRYLOV_MAX], alpha[NKRYLOV_MAX];
+        memcpy(BK, B0, sizeof(cplx) * N);
+        double d0 = vnorm(BK, N);
+        for (int64_t i = 0; i < N; i++) BK[i] /= d0;
+        apply_h(o, BK, BK + N);
+        cplx acc = 0;
+        for (int64_t i = 0; i < N; i++) acc += conj(BK[i]) * BK[N + i];
+        alpha[0] = creal(acc);
+        for (int64_t i = 0; i < N; i++) BK[N + i] -= alpha[0] * BK[i];
+        beta[0] = vnorm(BK + N, N);
+        for (int64_t i = 0; i < N; i++) BK[N + i] /= beta[0];
+        double error = beta[0] * fabs(dt);
+        int j = 2;
+        for (;;) {
+            if (j == NKRYLOV_MAX) {
+                snprintf(o->err, sizeof o->err, "Maximum number of Krylov vectors reached");
+                return -1;
+            }
+            cplx *vj = BK + (size_t)N * (j - 1), *vn = BK + (size_t)N * j, *vp = BK + (size_t)N * (j - 2);
+            apply_h(o, vj, vn);
+            acc = 0;
+            for (int64_t i = 0; i < N; i++) acc += conj(vj[i]) * vn[i];
+            alpha[j - 1] = creal(acc);
+            for (int64_t i = 0; i < N; i++) vn[i] -= beta[j - 2] * vp[i];
+            for (int64_t i = 0; i < N; i++) vn[i] -= alpha[j - 1] * vj[i];
+            beta[j - 1] = vnorm(vn, N);
+            for (int64_t i = 0; i < N; i++) vn[i] /= beta[j - 1];
+            error = error * beta[j - 1] / (double)j * fabs(dt);
+            if (error < EPS_B) break;
+            j++;
+        }
+        o->nkrylov = j;
+        double w[NKRYLOV_MAX];
+        tridiag_eig(j, alpha, beta, w, o->lan_U);
+        for (int r = 0; r < j; r++) o->lan_alpha[r] = w[r];
+    }
+    const int n = o->nkrylov;
+    cplx coef[NKRYLOV_MAX];
+    for (int i = 0; i < n; i++) {
+        cplx acc = 0;
+        for (int r = 0; r < n; r++)
+            acc += o->lan_U[i + n * r] * o->lan_U[0 + n * r] * cexp(o->der_factor * o->lan_alpha[r] * dt);
+        coef[i] = acc;
+    }
+    for (int64_t q = 0; q < N; q++) {
+        cplx acc = 0;
+        for (int i = 0; i < n; i++) acc += BK[(size_t)N * i + q] * coef[i];
+        BT[q] = acc;
+    }
+    double dn = vnorm(BT, N);
+    for (int64_t q = 0; q < N; q++) BT[q] /= dn;
+    return n;
+}
+
+int orc_lanczos(orc *o, const double *B0, double *BT, double dt, int build) {
+    return lanczos(o, (const cplx *)B0, (cplx *)BT, dt, build);
+}
+
+/* cmf2 for DVR-only wavefunctions, where A_to_B / B_to_A are copies
+ * (cmf2_base.f90:47-206). */
+static int cmf2(orc *o, double tt, double htry, double *hdid_out, double *hnext_out,
+                mg_attempt *atts, int64_t max_att, int64_t *n_att) {
+    const int64_t N = o->nconf, D = o->dimpsi;
+    const double safe = 0.95, maxscal = 1.5;
+    cplx *B0 = xcalloc(N, sizeof(cplx)), *BH = xcalloc(N, sizeof(cplx)), *B02 = xcalloc(N, sizeof(cplx));
+    cplx *pH = xcalloc(D, sizeof(cplx)), *pH2 = xcalloc(D, sizeof(cplx));
+    double hdid = htry;
+    int rc = 0;
+    for (;;) {
+        calc_rho(o, o->A);
+        for (int64_t m = 0; m < o->nm; m++) calc_h1mode_mode(o, o->psi, m);
+        calc_meanfield(o, o->A);
+        memcpy(B0, o->A, sizeof(cplx) * N);
+        int nk1 = lanczos(o, B0, BH, 0.5 * hdid, 1);
+        if (nk1 < 0) { rc = -1; break; }
+        memcpy(pH, o->psi, sizeof(cplx) * D);
+        for (int64_t m = 0; m < o->nm; m++) rc |= propagate_spf(o, 0.5 * hdid, m, pH);
+        /* AHalf = BHalf */
+        calc_rho(o, BH);
+        for (int64_t m = 0; m < o->nm; m++) calc_h1mode_mode(o, pH, m);
+        calc_meanfield(o, BH);
+        memcpy(pH2, o->psi, sizeof(cplx) * D);
+        for (int64_t m = 0; m < o->nm; m++) rc |= propagate_spf(o, 0.5 * hdid, m, pH2);
+        double err_psi = delta_psi(o, BH, pH, BH, pH2);
+        for (int64_t m = 0; m < o->nm; m++) rc |= propagate_spf(o, 0.5 * hdid, m, pH2);
+        for (int64_t m = 0; m < o->nm; m++) calc_h1mode_mode(o, pH2, m);
+        int nk2 = lanczos(o, BH, B02, -0.5 * hdid, 1);
+        if (nk2 < 0) { rc = -1; break; }
+        for (int64_t i = 0; i < N; i++) B02[i] -= B0[i];
+        double err_A = vnorm(B02, N) * 0.5;
+        double error = (err_A > err_psi ? err_A : err_psi) / o->p.eps_int;
+        if (atts && n_att && *n_att < max_att) {
+            mg_attempt *a = &atts[(*n_att)++];
+            a->time = tt; a->h = hdid; a->err_A = err_A; a->err_psi = err_psi;
+            a->accepted = !(error > 1.0); a->nkrylov1 = nk1; a->nkrylov2 = nk2;
+        }
+        if (rc) break;
+        if (error > 1.0) {
+            hdid = hdid / sqrt(error) * safe;
+            /* The reference has no lower bound here (cmf2_base.f90:181-184) and
+             * loops forever once err_psi sits on its rounding floor
+             * (~sqrt(4 eps) = 2.1e-8 > eps_int); stop with an error instead. */
+            if (hdid < o->p.min_stepsize) {
+                snprintf(o->err, sizeof o->err, "CMF2 step size fell below minstep");
+                rc = -2;
+                break;
+            }
+            continue;
+        }
+        lanczos(o, BH, B0, 0.5 * hdid, 0);
+        memcpy(o->A, B0, sizeof(cplx) * N);
+        memcpy(o->psi, pH2, sizeof(cplx) * D);
+        double hn = hdid / sqrt(error), h2 = hdid * maxscal;
+        hn = hn < h2 ? hn : h2;
+        hn = hn < o->p.writetime ? hn : o->p.writetime;
+        *hnext_out = hn;
+        break;
+    }
+    *hdid_out = hdid;
+    free(B0); free(BH); free(B02); free(pH); free(pH2);
+    return rc;
+}
+
+int orc_cmf2_step(orc *o, double t, double htry, double *hdid, double *hnext,
+                  mg_attempt *atts, int64_t max_att, int64_t *n_att) {
+    return cmf2(o, t, htry, hdid, hnext, atts, max_att, n_att);
+}
+
+/* WriteOutput observables (output.f90:33-87, 143-150). */
+int orc_observables(orc *o, double t, mg_output *out, double *pops) {
+    calc_rho(o, o->A);
+    for (int64_t m = 0; m < o->nm; m++) calc_h1mode_mode(o, o->psi, m);
+    calc_meanfield(o, o->A);
+    double tot = 0.0;
+    for (int64_t s = 0; s < o->ns; s++) {
+        int64_t jel = o->MultiSet[0] ? s : 0;
+        int64_t nS = NSPF(o, 0, jel);
+        double pop = 0.0;
+        for (int64_t i = 0; i < nS; i++) pop += creal(RHO(o, 0, s)[i + o->nspfmax * i]);
+        if (pops) pops[s] = pop;
+        tot += pop;
+    }
+    cplx c = wf_ovl(o, o->Astart, o->psistart, o->A, o->psi);
+    out->time = t;
+    out->auto_re = creal(c);
+    out->auto_im = cimag(c);
+    out->norm = tot;
+    out->energy = o->energyMF / tot;
+    return 0;
+}
+
+/* propagate_wf, CMF2 branch (propagator.f90:87-106, 257-306). */
+int orc_propagate(orc *o, double t0, double tfinal, mg_output *outs, double *pops,
+                  int64_t max_out, int64_t *n_out, mg_attempt *atts, int64_t max_att,
+                  int64_t *n_att) {
+    const double wt = o->p.writetime, minstep = o->p.min_stepsize;
+    double maxtime = tfinal + wt * 0.1;
+    double time = t0, stepsize = wt * o->p.eps_int;
+    *n_out = 0;
+    if (n_att) *n_att = 0;
+    while (time < maxtime) {
+        if (*n_out < max_out) {
+            orc_observables(o, time, &outs[*n_out], pops ? pops + o->ns * *n_out : NULL);
+            (*n_out)++;
+        }
+        double tnext = time + wt;
+        for (;;) {
+            double done, next;
+            if (cmf2(o, time, stepsize, &done, &next, atts, max_att, n_att) != 0) return -1;
+            time = time + done;
+            if (time >= tnext - minstep) break;
+            int64_t k = (int64_t)((tnext - time) / next);
+            stepsize = (k == 0) ? tnext - time : (tnext - time) / (double)(k + 1);
+        }
+    }
+    return 0;
+}
+
+/* test hooks */
+void orc_std_reg_inv(int64_t n, const double *A, double eps, int64_t type, double *B) {
+    std_reg_inv(n, (const cplx *)A, eps, (int)type, (cplx *)B);
+}
+void orc_herm_eig(int64_t n, const double *A, double *w, double *U) {
+    herm_eig(n, (const cplx *)A, w, (cplx *)U);
+}
+void orc_tridiag_eig(int64_t n, const double *alpha, const double *beta, double *w, double *U) {
+    tridiag_eig(n, alpha, beta, w, U);
+}

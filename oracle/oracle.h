@@ -1,0 +1,65 @@
+/*
+ * oracle.h — CPU FP64 restatement of Macaw's CMF2 propagation path.
+ *
+ * TEST INFRASTRUCTURE ONLY: this library is the checker for the HIP engine.
+ * Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may
+ * load it; the engine never links or calls it.
+ *
+ * It consumes the same mg_problem descriptor as the engine
+ * (include/macaw_gpu.h) and follows the reference routine by routine; each
+ * function in oracle.c cites the file:line it restates.
+ */
+#ifndef MACAW_ORACLE_H
+#define MACAW_ORACLE_H
+
+#include <stdint.h>
+#include "macaw_gpu.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct orc orc;
+
+orc *orc_create(const mg_problem *p);
+void orc_destroy(orc *o);
+const char *orc_error(const orc *o);
+int64_t orc_nconf(const orc *o);
+int64_t orc_dimpsi(const orc *o);
+int64_t orc_nspfmax(const orc *o);
+int64_t orc_nopmax(const orc *o);
+/* operator basis: nOper[nmodes], iOper[nmodes, nham] (1-based), IsIdentity[nmodes, nopmax] */
+void orc_basis(const orc *o, int64_t *nOper, int64_t *iOper, int64_t *isIdentity);
+
+void orc_set_state(orc *o, const double *A, const double *psi);
+void orc_get_state(const orc *o, double *A, double *psi);
+void orc_set_start(orc *o, const double *A, const double *psi);
+
+void orc_calc_h1mode(orc *o, double *h1_out);
+void orc_calc_rho(orc *o, double *rho_out);
+int orc_calc_meanfield(orc *o, double *mf_out, double *rmf_out, double *energy_out);
+void orc_apply_h(orc *o, const double *B, double *HB);
+void orc_der_spf(orc *o, int64_t im, const double *psi_in, double *dpsi_out);
+double orc_norm_wf(orc *o, const double *A, const double *psi);
+void orc_wf_ovl(orc *o, const double *A1, const double *psi1, const double *A2,
+                const double *psi2, double *ovl);
+/* Lanczos (cmf2_integration.f90:30-103); returns nKrylov or -1 on overflow. */
+int orc_lanczos(orc *o, const double *B0, double *BT, double dt, int build);
+/* propagate_SPF (cmf2_base.f90:210-260) for DVR mode index i (1-based) on psi. */
+int orc_propagate_spf(orc *o, double dt, int64_t i, double *psi);
+int orc_cmf2_step(orc *o, double t, double htry, double *hdid, double *hnext,
+                  mg_attempt *atts, int64_t max_att, int64_t *n_att);
+int orc_observables(orc *o, double t, mg_output *out, double *pops);
+int orc_propagate(orc *o, double t0, double tfinal, mg_output *outs, double *pops,
+                  int64_t max_out, int64_t *n_out, mg_attempt *atts, int64_t max_att,
+                  int64_t *n_att);
+
+/* Small dense kernels exposed for pinning tests. */
+void orc_std_reg_inv(int64_t n, const double *A, double eps, int64_t type, double *B);
+void orc_herm_eig(int64_t n, const double *A, double *w, double *U);
+void orc_tridiag_eig(int64_t n, const double *alpha, const double *beta, double *w, double *U);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

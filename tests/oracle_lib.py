@@ -1,0 +1,211 @@
+"""ctypes wrapper of the CPU oracle (oracle/liboracle.so) for the tests."""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+from macaw_amd.abi import MgAttempt, MgOutput, MgProblem, cplx_ptr, f64p, i64p, make_mg_problem
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB = os.path.join(ROOT, "oracle", "liboracle.so")
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            subprocess.check_call(["make", "-s", "-C", os.path.join(ROOT, "oracle")])
+        L = C.CDLL(LIB)
+        L.orc_create.restype = C.c_void_p
+        L.orc_create.argtypes = [C.POINTER(MgProblem)]
+        L.orc_destroy.argtypes = [C.c_void_p]
+        L.orc_error.restype = C.c_char_p
+        L.orc_error.argtypes = [C.c_void_p]
+        for f in ("orc_nconf", "orc_dimpsi", "orc_nspfmax", "orc_nopmax"):
+            getattr(L, f).restype = C.c_int64
+            getattr(L, f).argtypes = [C.c_void_p]
+        L.orc_basis.argtypes = [C.c_void_p, i64p, i64p, i64p]
+        L.orc_set_state.argtypes = [C.c_void_p, f64p, f64p]
+        L.orc_get_state.argtypes = [C.c_void_p, f64p, f64p]
+        L.orc_set_start.argtypes = [C.c_void_p, f64p, f64p]
+        L.orc_calc_h1mode.argtypes = [C.c_void_p, f64p]
+        L.orc_calc_rho.argtypes = [C.c_void_p, f64p]
+        L.orc_calc_meanfield.argtypes = [C.c_void_p, f64p, f64p, f64p]
+        L.orc_apply_h.argtypes = [C.c_void_p, f64p, f64p]
+        L.orc_der_spf.argtypes = [C.c_void_p, C.c_int64, f64p, f64p]
+        L.orc_norm_wf.restype = C.c_double
+        L.orc_norm_wf.argtypes = [C.c_void_p, f64p, f64p]
+        L.orc_wf_ovl.argtypes = [C.c_void_p, f64p, f64p, f64p, f64p, f64p]
+        L.orc_lanczos.restype = C.c_int
+        L.orc_lanczos.argtypes = [C.c_void_p, f64p, f64p, C.c_double, C.c_int]
+        L.orc_propagate_spf.restype = C.c_int
+        L.orc_propagate_spf.argtypes = [C.c_void_p, C.c_double, C.c_int64, f64p]
+        L.orc_cmf2_step.restype = C.c_int
+        L.orc_cmf2_step.argtypes = [C.c_void_p, C.c_double, C.c_double, f64p, f64p,
+                                    C.POINTER(MgAttempt), C.c_int64, i64p]
+        L.orc_observables.argtypes = [C.c_void_p, C.c_double, C.POINTER(MgOutput), f64p]
+        L.orc_propagate.restype = C.c_int
+        L.orc_propagate.argtypes = [C.c_void_p, C.c_double, C.c_double, C.POINTER(MgOutput),
+                                    f64p, C.c_int64, i64p, C.POINTER(MgAttempt), C.c_int64, i64p]
+        L.orc_std_reg_inv.argtypes = [C.c_int64, f64p, C.c_double, C.c_int64, f64p]
+        L.orc_herm_eig.argtypes = [C.c_int64, f64p, f64p, f64p]
+        L.orc_tridiag_eig.argtypes = [C.c_int64, f64p, f64p, f64p, f64p]
+        _lib = L
+    return _lib
+
+
+class Oracle:
+    def __init__(self, prob):
+        self.prob = prob
+        self.mgp, self._keep = make_mg_problem(prob)
+        self.h = lib().orc_create(C.byref(self.mgp))
+        L = lib()
+        self.nconf = L.orc_nconf(self.h)
+        self.dimpsi = L.orc_dimpsi(self.h)
+        self.nspfmax = L.orc_nspfmax(self.h)
+        self.nopmax = L.orc_nopmax(self.h)
+        self.set_state(prob.A0, prob.psi0)
+        self.set_start(prob.A0, prob.psi0)
+
+    def __del__(self):
+        try:
+            lib().orc_destroy(self.h)
+        except Exception:
+            pass
+
+    def basis(self):
+        nm, nh = self.prob.nmodes, self.prob.nHam
+        nop = np.zeros(nm, np.int64)
+        iop = np.zeros((nm, nh), np.int64, order="F")
+        ident = np.zeros((nm, self.nopmax), np.int64, order="F")
+        lib().orc_basis(self.h, cptr_i(nop), cptr_i(iop), cptr_i(ident))
+        return nop, iop, ident
+
+    def set_state(self, A, psi):
+        A = np.ascontiguousarray(A, np.complex128)
+        psi = np.ascontiguousarray(psi, np.complex128)
+        lib().orc_set_state(self.h, cplx_ptr(A), cplx_ptr(psi))
+
+    def set_start(self, A, psi):
+        A = np.ascontiguousarray(A, np.complex128)
+        psi = np.ascontiguousarray(psi, np.complex128)
+        lib().orc_set_start(self.h, cplx_ptr(A), cplx_ptr(psi))
+
+    def get_state(self):
+        A = np.zeros(self.nconf, np.complex128)
+        psi = np.zeros(self.dimpsi, np.complex128)
+        lib().orc_get_state(self.h, cplx_ptr(A), cplx_ptr(psi))
+        return A, psi
+
+    def _mat(self, *shape):
+        return np.zeros(shape[::-1], np.complex128)
+
+    def h1mode(self):
+        out = np.zeros((self.prob.nmodes, self.nopmax, self.nspfmax, self.nspfmax), np.complex128)
+        lib().orc_calc_h1mode(self.h, cplx_ptr(out))
+        return out      # [mode, op, j2, j1] (C view of the Fortran array)
+
+    def rho(self):
+        out = np.zeros((self.prob.nstates, self.prob.nmodes, self.nspfmax, self.nspfmax), np.complex128)
+        lib().orc_calc_rho(self.h, cplx_ptr(out))
+        return out      # [state, mode, j2, j1]
+
+    def meanfield(self):
+        shape = (self.prob.nmodes, self.nopmax, self.nspfmax, self.nspfmax)
+        mf = np.zeros(shape, np.complex128)
+        rmf = np.zeros(shape, np.complex128)
+        e = C.c_double(0.0)
+        lib().orc_calc_meanfield(self.h, cplx_ptr(mf), cplx_ptr(rmf), C.byref(e))
+        return mf, rmf, e.value
+
+    def apply_h(self, B):
+        B = np.ascontiguousarray(B, np.complex128)
+        HB = np.zeros_like(B)
+        lib().orc_apply_h(self.h, cplx_ptr(B), cplx_ptr(HB))
+        return HB
+
+    def der_spf(self, im, psi):
+        psi = np.ascontiguousarray(psi, np.complex128)
+        d = np.zeros_like(psi)
+        lib().orc_der_spf(self.h, im, cplx_ptr(psi), cplx_ptr(d))
+        return d
+
+    def norm_wf(self, A, psi):
+        return lib().orc_norm_wf(self.h, cplx_ptr(np.ascontiguousarray(A)), cplx_ptr(np.ascontiguousarray(psi)))
+
+    def wf_ovl(self, A1, psi1, A2, psi2):
+        out = np.zeros(2)
+        lib().orc_wf_ovl(self.h, *(cplx_ptr(np.ascontiguousarray(x)) for x in (A1, psi1, A2, psi2)),
+                         out.ctypes.data_as(f64p))
+        return complex(out[0], out[1])
+
+    def lanczos(self, B0, dt, build=True):
+        B0 = np.ascontiguousarray(B0, np.complex128)
+        BT = np.zeros_like(B0)
+        nk = lib().orc_lanczos(self.h, cplx_ptr(B0), cplx_ptr(BT), dt, 1 if build else 0)
+        return BT, nk
+
+    def propagate_spf(self, dt, i, psi):
+        psi = np.array(psi, np.complex128)
+        rc = lib().orc_propagate_spf(self.h, dt, i, cplx_ptr(psi))
+        return psi, rc
+
+    def cmf2_step(self, t, htry, max_att=1000):
+        hd, hn = C.c_double(0), C.c_double(0)
+        atts = (MgAttempt * max_att)()
+        na = C.c_int64(0)
+        rc = lib().orc_cmf2_step(self.h, t, htry, C.byref(hd), C.byref(hn), atts, max_att, C.byref(na))
+        return rc, hd.value, hn.value, list(atts[: na.value])
+
+    def observables(self, t):
+        out = MgOutput()
+        pops = np.zeros(self.prob.nstates)
+        lib().orc_observables(self.h, t, C.byref(out), pops.ctypes.data_as(f64p))
+        return out, pops
+
+    def propagate(self, t0, tfinal, max_out=100000, max_att=200000):
+        outs = (MgOutput * max_out)()
+        pops = np.zeros((max_out, self.prob.nstates))
+        atts = (MgAttempt * max_att)()
+        no, na = C.c_int64(0), C.c_int64(0)
+        rc = lib().orc_propagate(self.h, t0, tfinal, outs, pops.ctypes.data_as(f64p), max_out,
+                                 C.byref(no), atts, max_att, C.byref(na))
+        if rc != 0:
+            raise RuntimeError(lib().orc_error(self.h).decode())
+        return list(outs[: no.value]), pops[: no.value], list(atts[: na.value])
+
+
+def cptr_i(a):
+    return a.ctypes.data_as(i64p)
+
+
+def std_reg_inv(A, eps, kind):
+    A = np.ascontiguousarray(np.asarray(A, np.complex128).T)  # column-major
+    n = A.shape[0]
+    B = np.zeros_like(A)
+    lib().orc_std_reg_inv(n, cplx_ptr(A), eps, kind, cplx_ptr(B))
+    return B.T
+
+
+def herm_eig(A):
+    A = np.ascontiguousarray(np.asarray(A, np.complex128).T)
+    n = A.shape[0]
+    w = np.zeros(n)
+    U = np.zeros_like(A)
+    lib().orc_herm_eig(n, cplx_ptr(A), w.ctypes.data_as(f64p), cplx_ptr(U))
+    return w, U.T
+
+
+def tridiag_eig(alpha, beta):
+    n = len(alpha)
+    a = np.ascontiguousarray(alpha, np.float64)
+    b = np.ascontiguousarray(np.concatenate([beta, [0.0]]), np.float64)
+    w = np.zeros(n)
+    U = np.zeros((n, n))
+    lib().orc_tridiag_eig(n, a.ctypes.data_as(f64p), b.ctypes.data_as(f64p),
+                          w.ctypes.data_as(f64p), U.ctypes.data_as(f64p))
+    return w, U.T

@@ -1,0 +1,271 @@
+"""Pins of the CPU oracle against the mathematics (no reference run exists:
+Macaw is Fortran + LAPACK and is not built here).
+
+Each check is chosen so that a plausible slip in the oracle (a transposed
+operand, a missing conjugate, a wrong state label, a dropped term, a wrong
+regularisation branch) fails it.
+"""
+import numpy as np
+import pytest
+import scipy.linalg
+
+import brute
+from oracle_lib import Oracle, herm_eig, std_reg_inv, tridiag_eig
+from macaw_amd.lvc import SPECS, LVCSpec, make_problem
+
+CASES = ["t1", "t2", "t4"]
+
+
+def rand_c(rng, *shape):
+    return rng.normal(size=shape) + 1j * rng.normal(size=shape)
+
+
+def perturbed_state(prob, seed):
+    """A generic state: random A and non-orthonormal random SPFs."""
+    rng = np.random.default_rng(seed)
+    A = rand_c(rng, prob.nAConftot)
+    A /= np.linalg.norm(A)
+    psi = prob.psi0 + 0.05 * rand_c(rng, prob.dimpsi)
+    return A, psi
+
+
+@pytest.fixture(scope="module", params=CASES)
+def case(request, tmp_path_factory):
+    d = tmp_path_factory.mktemp(request.param)
+    return make_problem(request.param, str(d))
+
+
+def test_operator_basis(case):
+    o = Oracle(case)
+    nop, iop, ident = o.basis()
+    ref = brute.basis(case)
+    for m in range(case.nmodes):
+        assert nop[m] == len(ref[m])
+        for t in range(case.nHam):
+            assert ref[m][iop[m, t] - 1] == brute.op_key(case, m, t)
+        for i, key in enumerate(ref[m]):
+            expect = key[0] == key[1] and all(c == 0 for c in key[2:])
+            assert bool(ident[m, i]) == expect
+
+
+def test_h1mode_matches_dense_grid_operators(case):
+    A, psi = perturbed_state(case, 1)
+    o = Oracle(case)
+    o.set_state(A, psi)
+    h1 = o.h1mode()
+    nop, iop, ident = o.basis()
+    for m in range(case.nmodes):
+        for t in range(case.nHam):
+            i = iop[m, t] - 1
+            M = brute.term_mode_matrix(case, psi, m, t)
+            a, b = M.shape
+            got = h1[m, i].T[:a, :b]
+            if ident[m, i]:
+                np.testing.assert_allclose(got, np.eye(a), atol=0)
+            else:
+                np.testing.assert_allclose(got, M, rtol=0, atol=1e-12 * max(1, np.abs(M).max()))
+
+
+def test_apply_h_equals_full_kronecker_hamiltonian(case):
+    A, psi = perturbed_state(case, 2)
+    o = Oracle(case)
+    o.set_state(A, psi)
+    o.h1mode()
+    H = brute.full_hamiltonian(case, psi)
+    rng = np.random.default_rng(3)
+    B = rand_c(rng, case.nAConftot)
+    HB = o.apply_h(B)
+    ref = H @ B
+    assert np.abs(HB - ref).max() <= 1e-12 * np.abs(ref).max()
+
+
+def test_apply_h_hermitian_for_orthonormal_spfs(case):
+    # with orthonormal SPFs the projected Hamiltonian is Hermitian: <x|Hy> = <Hx|y>
+    o = Oracle(case)
+    o.h1mode()
+    rng = np.random.default_rng(4)
+    x, y = rand_c(rng, case.nAConftot), rand_c(rng, case.nAConftot)
+    lhs = np.vdot(x, o.apply_h(y))
+    rhs = np.vdot(o.apply_h(x), y)
+    assert abs(lhs - rhs) <= 1e-12 * abs(lhs)
+
+
+def test_density_matrices(case):
+    A, psi = perturbed_state(case, 5)
+    o = Oracle(case)
+    o.set_state(A, psi)
+    rho = o.rho()
+    for s in range(case.nstates):
+        for m in range(case.nmodes):
+            n = case.nSPF[m, s]
+            ref = brute.density(case, A, m, s)
+            np.testing.assert_allclose(rho[s, m].T[:n, :n], ref, atol=1e-13)
+
+
+def test_mean_fields_and_regularised_inverse(case):
+    A, psi = perturbed_state(case, 6)
+    o = Oracle(case)
+    o.set_state(A, psi)
+    o.h1mode()
+    rho = o.rho()
+    mf, rmf, energy = o.meanfield()
+    ref = brute.mean_fields(case, A, psi)
+    keys = brute.basis(case)
+    for (m, key), G in ref.items():
+        i = keys[m].index(key)
+        a, b = G.shape
+        np.testing.assert_allclose(mf[m, i].T[:a, :b], G, atol=1e-12 * max(1, np.abs(G).max()))
+        s = key[0] - 1
+        if case.MultiSet[m]:
+            r = rho[s, m].T[:a, :a]
+        else:
+            r = sum(rho[q, m].T[:a, :a] for q in range(case.nstates))
+        R = brute.reg_inv(r, case.eps_rho, "poly") @ G
+        np.testing.assert_allclose(rmf[m, i].T[:a, :b], R, atol=1e-9 * max(1, np.abs(R).max()))
+    # energy = Re <A|H|A> with the same SPFs
+    H = brute.full_hamiltonian(case, psi)
+    assert abs(energy - np.vdot(A, H @ A).real) <= 1e-12 * abs(energy)
+
+
+@pytest.mark.parametrize("n", [1, 2, 5, 8])
+def test_hermitian_eigensolver_and_reg_inverse(n):
+    rng = np.random.default_rng(n)
+    X = rand_c(rng, n, n)
+    A = X @ X.conj().T
+    A[:, -1] *= 1e-9          # nearly singular direction
+    A[-1, :] *= 1e-9
+    w, U = herm_eig(A)
+    wr = np.linalg.eigvalsh(A)
+    np.testing.assert_allclose(w, wr, atol=1e-12 * np.abs(wr).max())
+    np.testing.assert_allclose(U @ np.diag(w) @ U.conj().T, A, atol=1e-12 * np.abs(A).max())
+    for kind, code in (("tikh", 1), ("poly", 2)):
+        for eps in (1e-8, 1e-3):
+            B = std_reg_inv(A, eps, code)
+            R = brute.reg_inv(A, eps, kind)
+            np.testing.assert_allclose(B, R, atol=1e-7 * np.abs(R).max())
+            # poly regularisation: eigenvalues above eps are inverted exactly
+            if kind == "poly":
+                big = wr > 10 * eps
+                if big.all():
+                    np.testing.assert_allclose(B @ A, np.eye(n), atol=1e-8)
+
+
+def test_tridiagonal_eigensolver():
+    rng = np.random.default_rng(9)
+    a, b = rng.normal(size=12), rng.uniform(0.1, 1.0, size=11)
+    w, U = tridiag_eig(a, b)
+    T = np.diag(a) + np.diag(b, 1) + np.diag(b, -1)
+    np.testing.assert_allclose(w, np.linalg.eigvalsh(T), atol=1e-13)
+    np.testing.assert_allclose(U @ np.diag(w) @ U.T, T, atol=1e-13)
+
+
+@pytest.mark.parametrize("dt", [5.0, -20.0, 60.0])
+def test_lanczos_matches_expm_with_renormalisation(case, dt):
+    o = Oracle(case)
+    o.h1mode()
+    H = brute.full_hamiltonian(case, case.psi0)
+    rng = np.random.default_rng(10)
+    B0 = rand_c(rng, case.nAConftot) * 0.3       # not normalised on purpose
+    BT, nk = o.lanczos(B0, dt, True)
+    assert 2 <= nk < 20
+    ref = brute.lanczos_expm(H, B0, dt)
+    assert np.linalg.norm(BT) == pytest.approx(1.0, abs=1e-14)
+    assert np.abs(BT - ref).max() < 1e-7
+    # reusing the Krylov space propagates its start vector (B0) again,
+    # whatever vector is passed (cmf2_integration.f90:92-99)
+    other, _ = o.lanczos(BT, -dt, False)
+    np.testing.assert_allclose(other, brute.lanczos_expm(H, B0, -dt), atol=2e-7)
+
+
+def test_der_spf_matches_dense_projection(case):
+    A, psi = perturbed_state(case, 11)
+    o = Oracle(case)
+    o.set_state(A, psi)
+    o.h1mode()
+    o.rho()
+    mf, rmf, _ = o.meanfield()
+    keys = brute.basis(case)
+    ip, _ = case.ipsiSPF()
+    for m in range(case.nmodes):
+        d = o.der_spf(m + 1, psi)
+        nsets = case.nstates if case.MultiSet[m] else 1
+        for s in range(nsets):
+            phi = brute.spf_block(case, psi, m, s)
+            acc = np.zeros_like(phi)
+            for i, key in enumerate(keys[m]):
+                a, b = key[0] - 1, key[1] - 1
+                if a == b and all(c == 0 for c in key[2:]):
+                    continue
+                if a != s:
+                    continue
+                pb = brute.spf_block(case, psi, m, b)
+                Op = brute.mode_grid_op(case, m, key[2:])
+                nb = pb.shape[1]
+                acc += -1j * (Op @ pb) @ rmf[m, i].T[: phi.shape[1], :nb].T
+            P = phi @ np.linalg.solve(phi.conj().T @ phi, phi.conj().T)
+            ref = acc - P @ acc
+            got = brute.spf_block(case, d, m, s)
+            np.testing.assert_allclose(got, ref, atol=1e-9 * max(1, np.abs(ref).max()))
+
+
+def test_norm_and_overlap(case):
+    A, psi = perturbed_state(case, 12)
+    o = Oracle(case)
+    ov = o.wf_ovl(case.A0, case.psi0, A, psi)
+    # <Psi0|Psi> on the full grid product space
+    def full(Av, pv):
+        vec = 0
+        for s in range(case.nstates):
+            T = brute.state_tensor(case, Av, s)
+            for m in range(case.nmodes):
+                T = brute.apply_mode(T, brute.spf_block(case, pv, m, s), m)
+            vec = vec if s else []
+            vec.append(T.reshape(-1, order="F"))
+        return np.concatenate(vec)
+    v0, v1 = full(case.A0, case.psi0), full(A, psi)
+    assert abs(ov - np.vdot(v0, v1)) < 1e-12
+    assert o.norm_wf(A, psi) == pytest.approx(np.vdot(v1, v1).real, rel=1e-12)
+
+
+def test_full_basis_cmf2_equals_exact_propagation(tmp_path):
+    """n_k = G_k: SPFs span the grid, the projector vanishes, and CMF2 must
+    reproduce exp(-iHt) on the product grid to its error tolerance."""
+    spec = LVCSpec("fb", 21, 2, 2, 6, 6, 8.0, tout=1.0, omega=(0.05, 0.30), kappa_sd=0.10,
+                   n_lambda=1, lam=(0.10, 0.10), gaps=(-0.2, 0.2), eps_int=1e-7, eps_psi=1e-9)
+    prob = make_problem(spec, str(tmp_path))
+    o = Oracle(prob)
+    outs, pops, atts = o.propagate(0.0, prob.tfinal)
+    H = brute.full_hamiltonian(prob, prob.psi0)
+    w, V = np.linalg.eigh(H)
+    c0 = V.conj().T @ prob.A0
+    for out in outs:
+        c_ex = np.vdot(prob.A0, V @ (np.exp(-1j * w * out.time) * c0))
+        assert abs(complex(out.auto_re, out.auto_im) - c_ex) < 5e-7
+        assert out.norm == pytest.approx(1.0, abs=1e-10)
+    assert len(outs) == 9
+    assert any(a.accepted for a in atts)
+
+
+def test_short_cmf2_conserves_norm_and_energy(tmp_path):
+    prob = make_problem("t3", str(tmp_path))
+    o = Oracle(prob)
+    outs, pops, atts = o.propagate(0.0, 3.0 / 0.02418884326505)
+    e0 = outs[0].energy
+    for out, p in zip(outs, pops):
+        assert out.norm == pytest.approx(1.0, abs=1e-9)
+        assert p.sum() == pytest.approx(out.norm, abs=1e-12)
+        assert abs(out.energy - e0) < 1e-6 * abs(e0) + 1e-8
+    assert abs(complex(outs[0].auto_re, outs[0].auto_im) - 1.0) < 1e-12
+    rej = sum(1 for a in atts if not a.accepted)
+    assert rej < len(atts)
+
+
+def test_cmf2_min_step_guard(tmp_path):
+    # eps_int below the rounding floor of DeltaPsi: every attempt is rejected;
+    # the step must stop with an error instead of shrinking forever
+    spec = LVCSpec("mg", 21, 2, 2, 6, 6, 8.0, tout=1.0, omega=(0.05, 0.30), kappa_sd=0.10,
+                   n_lambda=1, lam=(0.10, 0.10), gaps=(-0.2, 0.2), eps_int=1e-12, eps_psi=1e-9)
+    prob = make_problem(spec, str(tmp_path))
+    o = Oracle(prob)
+    with pytest.raises(RuntimeError, match="minstep"):
+        o.propagate(0.0, prob.tfinal)
