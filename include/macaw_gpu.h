@@ -74,9 +74,12 @@ int mg_create(const mg_problem *prob, int64_t precision, mg_ctx **ctx);
 int mg_destroy(mg_ctx *ctx);
 const char *mg_last_error(const mg_ctx *ctx);
 
-/* Sizes: nAConftot (psidef.f90:14) and dimpsi (init_calc.f90:160-181). */
+/* Sizes: nAConftot (psidef.f90:14) and dimpsi (init_calc.f90:160-181);
+ * nSPFmax (psidef.f90:15) and nOpMax (hamilton.f90:25), the leading
+ * dimensions of the matrix outputs below. */
 int64_t mg_nconf(const mg_ctx *ctx);
 int64_t mg_dimpsi(const mg_ctx *ctx);
+int mg_dims(const mg_ctx *ctx, int64_t *nspfmax, int64_t *nopmax);
 
 /* Upload / download AVector and psi (psidef.f90:44-45); host pointers. */
 int mg_set_state(mg_ctx *ctx, const double *A, const double *psi);
@@ -115,9 +118,15 @@ int mg_observables(mg_ctx *ctx, double t, mg_output *out, double *pops);
 int mg_propagate(mg_ctx *ctx, double t0, double tfinal, mg_output *outs, double *pops,
                  int64_t max_out, int64_t *n_out, mg_attempt *atts, int64_t max_att,
                  int64_t *n_att);
-/* Timing of the last mg_apply_h_device launch sequence (ms, from HIP events
- * on the context stream) and its algorithmic traffic / flops. */
+/* Algorithmic traffic (read B + write HB) and flops (8 per complex
+ * multiply-add over all mode products) of one mg_apply_h. */
 int mg_apply_h_stats(mg_ctx *ctx, double *bytes_alg, double *flops_alg);
+/* One mg_apply_h_device with a HIP event pair around every kernel launch:
+ * returns the summed kernel time (ms) and the number of launches. */
+int mg_apply_h_profile(mg_ctx *ctx, const void *dB, void *dHB, double *kernel_ms,
+                       int64_t *nlaunch);
+/* Current CMF2 step size carried between mg_propagate calls (a.u.). */
+int mg_stepsize(const mg_ctx *ctx, double *h);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,1470 @@
+// Host side of the MI355X G-MCTDH engine: problem layout, operator basis,
+// term classification, device-resident CMF2 orchestration and the C-ABI of
+// include/macaw_gpu.h.
+//
+// The state (A-vector, SPFs) and every work array live in HBM for the whole
+// propagation; the host only issues launches and reads back the few scalars
+// the control flow needs (Lanczos alpha/beta, step errors).
+#include "macaw_gpu.h"
+#include "mg_internal.h"
+
+#include <algorithm>
+#include <cmath>
+#include <array>
+#include <complex>
+#include <map>
+#include <memory>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+using namespace mg;
+using cplx = std::complex<double>;
+
+namespace {
+
+struct HipError {
+    std::string msg;
+};
+
+#define HIP_CHECK(x)                                                                          \
+    do {                                                                                      \
+        hipError_t e_ = (x);                                                                  \
+        if (e_ != hipSuccess)                                                                 \
+            throw HipError{std::string(#x) + ": " + hipGetErrorString(e_)};                   \
+    } while (0)
+
+struct Oper {
+    int lhs, rhs;                 // 1-based labels (1,1 for single-set modes)
+    int codes[kMaxDofPerMode];
+};
+
+struct ChainOp {
+    int mode, op;
+};
+
+struct Chain {
+    int lhs, rhs;                 // 0-based states
+    cplx ae;
+    std::vector<ChainOp> ops;     // ascending modes, non-identity only
+};
+
+struct DevBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+    ~DevBuf() { if (p) (void)hipFree(p); }
+    template <class T> T *as() const { return static_cast<T *>(p); }
+    void alloc(size_t b) {
+        if (p) { (void)hipFree(p); p = nullptr; }
+        bytes = b;
+        if (b) HIP_CHECK(hipMalloc(&p, b));
+    }
+};
+
+// A cached launch schedule: task tables uploaded once, replayed per call.
+struct Launch {
+    size_t off;
+    int n;
+    int64_t maxblocks;
+};
+struct Plan {
+    DevBuf buf;
+    std::vector<Launch> launches;
+};
+using PlanKey = std::array<const void *, 4>;
+struct SpfPlan {
+    DevBuf buf;
+    int n = 0;
+};
+
+}  // namespace
+
+struct mg_ctx {
+    std::string err;
+    hipStream_t stream = nullptr;
+    int precision = 0;
+    // ---- problem ----
+    int ns = 0, nm = 0, ndof = 0, nham = 0, maxdof = 0, ngpmax = 0;
+    std::vector<int64_t> nmodeDOF, iDOF, MultiSet, nSPF, ngp, lhs, rhs, iOpDof;
+    std::vector<double> grid, d1, d2, xd1;
+    std::vector<cplx> ae;
+    double eps_int = 1e-8, eps_psi = 1e-8, eps_rho = 1e-8, writetime = 1.0, min_stepsize = 0.0;
+    int int_type_psi = 1;
+    // ---- layout ----
+    int nspfmax = 0, nopmax = 0;
+    std::vector<int64_t> nAConf, off, ngpm;
+    std::vector<int64_t> ipsi;    // 0-based (mode + nm*state)
+    int64_t nconf = 0, dimpsi = 0, mixed = 0;
+    // ---- operator basis ----
+    std::vector<int> nOper;
+    std::vector<Oper> opers;      // [m * nopmax + i]
+    std::vector<int> iOper;       // [m + nm * t], 0-based
+    std::vector<char> isIdent;    // [m + nm * i]
+    // ---- terms ----
+    std::vector<int> hs_rowptr, hs_src;     // diagonal sums per (s, m)
+    std::vector<cplx> hs_coef;
+    std::vector<Chain> chains;              // H.A chain terms
+    std::vector<int> mf1_rowptr, mf1_src;   // one-mode same-state MF = ae * rho
+    std::vector<cplx> mf1_coef;
+    std::vector<int> mf_chain_terms;        // terms needing AL/AR chains
+    std::vector<int> rmf_jobs;
+    int n_rmf_jobs = 0;
+    // ---- grouping ----
+    int brick = 2048;
+    std::vector<std::pair<int, int>> groups;
+    // ---- device buffers ----
+    DevBuf dA, dpsi, dAstart, dpsistart, dBK, dB0, dBH, dB02, dpH, dpH2, dHtmp, dIn, dOut;
+    DevBuf dh1, dhsum, drho, dmf, drmf, dS, dS2;
+    DevBuf dchain;                 // 2 * nchains * mixed
+    DevBuf dmfbuf;                 // 2 * nm * mixed (AL / AR)
+    DevBuf dkron;                  // 2 * nconf-sized temps for overlap chains
+    DevBuf dspfscr, dpartials, dscal;
+    DevBuf dhs_rowptr, dhs_src, dhs_coef, dmf1_rowptr, dmf1_src, dmf1_coef, drmf_jobs;
+    DevBuf dgrid, dd1, dd2, dxd1, dngp, dmode_dofs, dmode_ndof, dspfmodes, dstatus;
+    int nblocks_red = 1024;
+    // timing of apply_h_device
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    double last_ms = 0.0, bytes_alg = 0.0, flops_alg = 0.0;
+    double stepsize = 0.0;         // carried between mg_propagate calls
+    bool prof = false;             // event pair around every group launch
+    double prof_ms = 0.0;
+    int64_t prof_launches = 0;
+    std::vector<hipEvent_t> prof_ev;
+    // cached launch plans keyed by the buffers they read and write
+    std::map<PlanKey, std::unique_ptr<Plan>> plans;
+    std::map<PlanKey, std::unique_ptr<SpfPlan>> spf_plans;
+    std::vector<double> lan_alpha, lan_U;
+    int nkrylov = 0;
+
+    int nsets(int m) const { return MultiSet[m] ? ns : 1; }
+    int64_t NS(int m, int s) const { return nSPF[m + (int64_t)nm * s]; }
+    int64_t IP(int m, int s) const { return ipsi[m + (int64_t)nm * s]; }
+    double2 *h1(int m, int op) const {
+        return dh1.as<double2>() + (size_t)nspfmax * nspfmax * (op + (size_t)nopmax * m);
+    }
+    double2 *hsum(int s, int m) const { return dhsum.as<double2>() + (size_t)nspfmax * nspfmax * (m + (size_t)nm * s); }
+    double2 *rho(int m, int s) const { return drho.as<double2>() + (size_t)nspfmax * nspfmax * (m + (size_t)nm * s); }
+    double2 *rho_ave(int m) const { return drho.as<double2>() + (size_t)nspfmax * nspfmax * (nm * ns + m); }
+    double2 *mf(int m, int op) const { return dmf.as<double2>() + (size_t)nspfmax * nspfmax * (op + (size_t)nopmax * m); }
+    double2 *S(int m, int s) const { return dS.as<double2>() + (size_t)nspfmax * nspfmax * (m + (size_t)nm * s); }
+};
+
+namespace {
+
+double2 d2(cplx z) { return make_double2(z.real(), z.imag()); }
+
+// Kernel launches report configuration errors asynchronously; surface them.
+void check_launch() { HIP_CHECK(hipGetLastError()); }
+
+int64_t largest_divisor_le(int64_t L, int64_t cap) {
+    if (cap >= L) return L;
+    for (int64_t d = cap; d >= 1; d--)
+        if (L % d == 0) return d;
+    return 1;
+}
+
+// ----------------------------------------------------------------------------
+// Setup: layout (init_calc.f90:25-34,160-181), operator basis
+// (hamilton.f90:91-147, 353-373) and the classification of Hamiltonian terms.
+void setup(mg_ctx *c, const mg_problem *p) {
+    c->ns = (int)p->nstates;
+    c->nm = (int)p->nmodes;
+    c->ndof = (int)p->ndof;
+    c->nham = (int)p->nham;
+    c->maxdof = (int)p->maxdof;
+    c->ngpmax = (int)p->ngpmax;
+    const int nm = c->nm, ns = c->ns, nd = c->ndof;
+    c->nmodeDOF.assign(p->nmodeDOF, p->nmodeDOF + nm);
+    c->iDOF.assign(p->iDOF, p->iDOF + (size_t)c->maxdof * nm);
+    c->MultiSet.assign(p->MultiSet, p->MultiSet + nm);
+    c->nSPF.assign(p->nSPF, p->nSPF + (size_t)nm * ns);
+    c->ngp.assign(p->ngp, p->ngp + nd);
+    c->lhs.assign(p->lhs_el, p->lhs_el + c->nham);
+    c->rhs.assign(p->rhs_el, p->rhs_el + c->nham);
+    c->iOpDof.assign(p->iOpDof, p->iOpDof + (size_t)nd * c->nham);
+    const size_t gm = (size_t)c->ngpmax;
+    c->grid.assign(p->grid, p->grid + gm * nd);
+    c->d1.assign(p->d1, p->d1 + gm * gm * nd);
+    c->d2.assign(p->d2, p->d2 + gm * gm * nd);
+    c->xd1.assign(p->xd1, p->xd1 + gm * gm * nd);
+    c->ae.resize(c->nham);
+    for (int t = 0; t < c->nham; t++) c->ae[t] = cplx(p->ae[2 * t], p->ae[2 * t + 1]);
+    c->eps_int = p->eps_int;
+    c->eps_psi = p->eps_psi;
+    c->eps_rho = p->eps_rho;
+    c->writetime = p->writetime;
+    c->min_stepsize = p->min_stepsize;
+    c->int_type_psi = (int)p->int_type_psi;
+    for (int m = 0; m < nm; m++) {
+        if (c->nmodeDOF[m] > kMaxDofPerMode) throw HipError{"too many DOFs in a logical mode"};
+    }
+    if (ns > 8) throw HipError{"at most 8 electronic states are supported"};
+    for (int64_t v : c->iOpDof)
+        if (v < -3 || v > 20) throw HipError{"primitive operator code not supported by the GPU engine"};
+    // layout
+    c->nspfmax = 0;
+    for (int64_t v : c->nSPF) c->nspfmax = std::max<int>(c->nspfmax, (int)v);
+    if (c->nspfmax > 16) throw HipError{"more than 16 SPFs per mode are not supported"};
+    c->nAConf.assign(ns, 1);
+    c->off.assign(ns, 0);
+    int64_t ic = 0;
+    for (int s = 0; s < ns; s++) {
+        for (int m = 0; m < nm; m++) c->nAConf[s] *= c->NS(m, s);
+        c->off[s] = ic;
+        ic += c->nAConf[s];
+    }
+    c->nconf = ic;
+    c->mixed = 1;
+    for (int m = 0; m < nm; m++) {
+        int64_t mx = 0;
+        for (int s = 0; s < ns; s++) mx = std::max(mx, c->NS(m, s));
+        c->mixed *= mx;
+    }
+    c->ngpm.assign(nm, 1);
+    for (int m = 0; m < nm; m++)
+        for (int k = 0; k < c->nmodeDOF[m]; k++) c->ngpm[m] *= c->ngp[c->iDOF[k + (size_t)c->maxdof * m] - 1];
+    c->ipsi.assign((size_t)nm * ns, 0);
+    int64_t dim = 0;
+    for (int s = 0; s < ns; s++)
+        for (int m = 0; m < nm; m++) {
+            if (!c->MultiSet[m] && s > 0) { c->ipsi[m + (size_t)nm * s] = c->ipsi[m]; continue; }
+            c->ipsi[m + (size_t)nm * s] = dim;
+            dim += c->ngpm[m] * c->NS(m, s);
+        }
+    c->dimpsi = dim;
+    // operator basis
+    c->nOper.assign(nm, 0);
+    c->iOper.assign((size_t)nm * c->nham, 0);
+    std::vector<std::vector<Oper>> per(nm);
+    for (int m = 0; m < nm; m++) {
+        for (int t = 0; t < c->nham; t++) {
+            Oper v{};
+            if (c->MultiSet[m]) { v.lhs = (int)c->lhs[t]; v.rhs = (int)c->rhs[t]; }
+            else { v.lhs = 1; v.rhs = 1; }
+            for (int k = 0; k < c->nmodeDOF[m]; k++)
+                v.codes[k] = (int)c->iOpDof[(c->iDOF[k + (size_t)c->maxdof * m] - 1) + (size_t)nd * t];
+            int found = -1;
+            for (size_t i = 0; i < per[m].size(); i++) {
+                const Oper &u = per[m][i];
+                bool same = u.lhs == v.lhs && u.rhs == v.rhs;
+                for (int k = 0; k < c->nmodeDOF[m] && same; k++) same = u.codes[k] == v.codes[k];
+                if (same) { found = (int)i; break; }
+            }
+            if (found < 0) { per[m].push_back(v); found = (int)per[m].size() - 1; }
+            c->iOper[m + (size_t)nm * t] = found;
+        }
+        c->nOper[m] = (int)per[m].size();
+        c->nopmax = std::max(c->nopmax, c->nOper[m]);
+    }
+    c->opers.assign((size_t)nm * c->nopmax, Oper{});
+    c->isIdent.assign((size_t)nm * c->nopmax, 0);
+    for (int m = 0; m < nm; m++)
+        for (int i = 0; i < c->nOper[m]; i++) {
+            const Oper &v = per[m][i];
+            c->opers[(size_t)m * c->nopmax + i] = v;
+            bool id = v.lhs == v.rhs;
+            for (int k = 0; k < c->nmodeDOF[m] && id; k++) id = v.codes[k] == 0;
+            c->isIdent[m + (size_t)nm * i] = id;
+        }
+    // term classification
+    std::vector<std::vector<std::pair<int, cplx>>> hs((size_t)ns * nm), mf1((size_t)nm * c->nopmax);
+    c->chains.clear();
+    c->mf_chain_terms.clear();
+    for (int t = 0; t < c->nham; t++) {
+        const int a = (int)c->lhs[t] - 1, b = (int)c->rhs[t] - 1;
+        std::vector<ChainOp> ops;
+        for (int m = 0; m < nm; m++) {
+            const int op = c->iOper[m + (size_t)nm * t];
+            if (!c->isIdent[m + (size_t)nm * op]) ops.push_back({m, op});
+        }
+        if (a == b && ops.size() <= 1) {
+            if (ops.empty()) hs[(size_t)a * nm + 0].push_back({-1, c->ae[t]});
+            else hs[(size_t)a * nm + ops[0].mode].push_back({ops[0].op + c->nopmax * ops[0].mode, c->ae[t]});
+        } else {
+            c->chains.push_back({a, b, c->ae[t], ops});
+        }
+        // mean fields
+        if (ops.empty()) continue;
+        if (a == b && ops.size() == 1) {
+            const int m = ops[0].mode;
+            mf1[(size_t)m * c->nopmax + ops[0].op].push_back({m + nm * a, c->ae[t]});
+        } else {
+            c->mf_chain_terms.push_back(t);
+        }
+    }
+    c->hs_rowptr.assign(1, 0);
+    c->hs_src.clear();
+    c->hs_coef.clear();
+    for (auto &row : hs) {
+        for (auto &e : row) { c->hs_src.push_back(e.first); c->hs_coef.push_back(e.second); }
+        c->hs_rowptr.push_back((int)c->hs_src.size());
+    }
+    c->mf1_rowptr.assign(1, 0);
+    c->mf1_src.clear();
+    c->mf1_coef.clear();
+    for (auto &row : mf1) {
+        for (auto &e : row) { c->mf1_src.push_back(e.first); c->mf1_coef.push_back(e.second); }
+        c->mf1_rowptr.push_back((int)c->mf1_src.size());
+    }
+    // rho^-1 MF jobs (Calc_rhom1MF): per (mode, set), the ops with lhs == set
+    c->rmf_jobs.clear();
+    c->n_rmf_jobs = 0;
+    for (int m = 0; m < nm; m++) {
+        for (int s = 0; s < c->nsets(m); s++) {
+            std::vector<int> job(36, 0);
+            job[0] = m;
+            job[1] = c->MultiSet[m] ? (m + nm * s) : (nm * ns + m);
+            job[2] = (int)c->NS(m, s);
+            int nops = 0;
+            for (int i = 0; i < c->nOper[m]; i++) {
+                const Oper &v = c->opers[(size_t)m * c->nopmax + i];
+                if (c->isIdent[m + (size_t)nm * i]) continue;
+                if (c->MultiSet[m] && v.lhs - 1 != s) continue;
+                if (nops >= 16) throw HipError{"too many operators per mode and state"};
+                job[4 + 2 * nops] = i;
+                job[5 + 2 * nops] = (int)c->NS(m, v.rhs - 1);
+                nops++;
+            }
+            job[3] = nops;
+            c->rmf_jobs.insert(c->rmf_jobs.end(), job.begin(), job.end());
+            c->n_rmf_jobs++;
+        }
+    }
+    // mode groups (sizes: max over states)
+    c->groups.clear();
+    {
+        std::vector<int64_t> mx(nm, 0);
+        for (int m = 0; m < nm; m++)
+            for (int s = 0; s < ns; s++) mx[m] = std::max(mx[m], c->NS(m, s));
+        int m = 0;
+        while (m < nm) {
+            const int64_t limit = (m == 0) ? c->brick : std::max<int64_t>(c->brick / 8, mx[m]);
+            int64_t prod = mx[m];
+            int e = m + 1;
+            while (e < nm && e - m < kMaxGroupModes && prod * mx[e] <= limit) prod *= mx[e++];
+            c->groups.push_back({m, e});
+            m = e;
+        }
+    }
+}
+
+// ----------------------------------------------------------------------------
+// Device allocation and constant tables.
+void allocate(mg_ctx *c) {
+    const size_t z = sizeof(double2);
+    const size_t N = (size_t)c->nconf, D = (size_t)c->dimpsi;
+    const size_t nn = (size_t)c->nspfmax * c->nspfmax;
+    c->dA.alloc(z * N);
+    c->dpsi.alloc(z * D);
+    c->dAstart.alloc(z * N);
+    c->dpsistart.alloc(z * D);
+    c->dBK.alloc(z * N * kNKrylovMax);
+    c->dB0.alloc(z * N);
+    c->dBH.alloc(z * N);
+    c->dB02.alloc(z * N);
+    c->dHtmp.alloc(z * N);
+    c->dIn.alloc(z * N);
+    c->dOut.alloc(z * N);
+    c->dpH.alloc(z * D);
+    c->dpH2.alloc(z * D);
+    c->dh1.alloc(z * nn * c->nopmax * c->nm);
+    c->dhsum.alloc(z * nn * c->nm * c->ns);
+    c->drho.alloc(z * nn * c->nm * (c->ns + 1));
+    c->dmf.alloc(z * nn * c->nopmax * c->nm);
+    c->drmf.alloc(z * nn * c->nopmax * c->nm);
+    c->dS.alloc(z * nn * c->nm * c->ns);
+    c->dS2.alloc(z * nn * c->nm * c->ns);
+    c->dchain.alloc(z * 2 * std::max<size_t>(1, c->chains.size()) * (size_t)c->mixed);
+    c->dmfbuf.alloc(z * 2 * (size_t)c->nm * (size_t)c->mixed);
+    c->dkron.alloc(z * 2 * N);
+    c->dpartials.alloc(sizeof(double2) * 64 * (size_t)c->nblocks_red);
+    c->dscal.alloc(sizeof(double) * 64);
+    HIP_CHECK(hipMemsetAsync(c->dh1.p, 0, c->dh1.bytes, c->stream));
+    HIP_CHECK(hipMemsetAsync(c->dmf.p, 0, c->dmf.bytes, c->stream));
+    HIP_CHECK(hipMemsetAsync(c->drmf.p, 0, c->drmf.bytes, c->stream));
+    HIP_CHECK(hipMemsetAsync(c->drho.p, 0, c->drho.bytes, c->stream));
+    HIP_CHECK(hipMemsetAsync(c->dS.p, 0, c->dS.bytes, c->stream));
+    HIP_CHECK(hipMemsetAsync(c->dS2.p, 0, c->dS2.bytes, c->stream));
+    // identity h1mode for same-state identity operators (hamilton.f90:296-306)
+    {
+        std::vector<double2> h((size_t)nn * c->nopmax * c->nm, make_double2(0, 0));
+        for (int m = 0; m < c->nm; m++)
+            for (int i = 0; i < c->nOper[m]; i++)
+                if (c->isIdent[m + (size_t)c->nm * i]) {
+                    const int s = c->opers[(size_t)m * c->nopmax + i].lhs - 1;
+                    for (int j = 0; j < c->NS(m, s); j++)
+                        h[nn * (i + (size_t)c->nopmax * m) + j + (size_t)c->nspfmax * j] = make_double2(1, 0);
+                }
+        HIP_CHECK(hipMemcpy(c->dh1.p, h.data(), z * h.size(), hipMemcpyHostToDevice));
+    }
+    auto upload_i = [&](DevBuf &b, const std::vector<int> &v) {
+        b.alloc(sizeof(int) * std::max<size_t>(1, v.size()));
+        if (!v.empty()) HIP_CHECK(hipMemcpy(b.p, v.data(), sizeof(int) * v.size(), hipMemcpyHostToDevice));
+    };
+    auto upload_c = [&](DevBuf &b, const std::vector<cplx> &v) {
+        b.alloc(sizeof(double2) * std::max<size_t>(1, v.size()));
+        std::vector<double2> t(v.size());
+        for (size_t i = 0; i < v.size(); i++) t[i] = d2(v[i]);
+        if (!t.empty()) HIP_CHECK(hipMemcpy(b.p, t.data(), sizeof(double2) * t.size(), hipMemcpyHostToDevice));
+    };
+    upload_i(c->dhs_rowptr, c->hs_rowptr);
+    upload_i(c->dhs_src, c->hs_src);
+    upload_c(c->dhs_coef, c->hs_coef);
+    upload_i(c->dmf1_rowptr, c->mf1_rowptr);
+    upload_i(c->dmf1_src, c->mf1_src);
+    upload_c(c->dmf1_coef, c->mf1_coef);
+    upload_i(c->drmf_jobs, c->rmf_jobs);
+    // DOF tables
+    const size_t gm = (size_t)c->ngpmax;
+    c->dgrid.alloc(sizeof(double) * gm * c->ndof);
+    c->dd1.alloc(sizeof(double) * gm * gm * c->ndof);
+    c->dd2.alloc(sizeof(double) * gm * gm * c->ndof);
+    c->dxd1.alloc(sizeof(double) * gm * gm * c->ndof);
+    HIP_CHECK(hipMemcpy(c->dgrid.p, c->grid.data(), c->dgrid.bytes, hipMemcpyHostToDevice));
+    HIP_CHECK(hipMemcpy(c->dd1.p, c->d1.data(), c->dd1.bytes, hipMemcpyHostToDevice));
+    HIP_CHECK(hipMemcpy(c->dd2.p, c->d2.data(), c->dd2.bytes, hipMemcpyHostToDevice));
+    HIP_CHECK(hipMemcpy(c->dxd1.p, c->xd1.data(), c->dxd1.bytes, hipMemcpyHostToDevice));
+    std::vector<int> ngp(c->ndof), mdofs((size_t)c->nm * kMaxDofPerMode, 0), mnd(c->nm);
+    for (int k = 0; k < c->ndof; k++) ngp[k] = (int)c->ngp[k];
+    for (int m = 0; m < c->nm; m++) {
+        mnd[m] = (int)c->nmodeDOF[m];
+        for (int k = 0; k < mnd[m]; k++) mdofs[(size_t)m * kMaxDofPerMode + k] = (int)c->iDOF[k + (size_t)c->maxdof * m] - 1;
+    }
+    upload_i(c->dngp, ngp);
+    upload_i(c->dmode_dofs, mdofs);
+    upload_i(c->dmode_ndof, mnd);
+    // SPF propagation descriptors and scratch
+    std::vector<SpfModeDesc> md(c->nm);
+    size_t scr = 0;
+    for (int m = 0; m < c->nm; m++) {
+        SpfModeDesc &d = md[m];
+        std::memset(&d, 0, sizeof d);
+        d.mode = m;
+        d.nG = (int)c->ngpm[m];
+        d.nsets = c->nsets(m);
+        int blk = 0;
+        for (int s = 0; s < d.nsets; s++) {
+            d.off[s] = (int)c->IP(m, s);
+            d.nspf[s] = (int)c->NS(m, s);
+            d.rho_index[s] = c->MultiSet[m] ? (m + c->nm * s) : (c->nm * c->ns + m);
+            blk += d.nG * d.nspf[s];
+        }
+        d.blk = blk;
+        int nop = 0;
+        for (int i = 0; i < c->nOper[m]; i++) {
+            if (c->isIdent[m + (size_t)c->nm * i]) continue;
+            if (nop >= 32) throw HipError{"too many operators for one mode"};
+            const Oper &v = c->opers[(size_t)m * c->nopmax + i];
+            d.op_lhs_set[nop] = c->MultiSet[m] ? v.lhs - 1 : 0;
+            d.op_rhs_set[nop] = c->MultiSet[m] ? v.rhs - 1 : 0;
+            for (int k = 0; k < kMaxDofPerMode; k++) d.op_codes[nop][k] = v.codes[k];
+            d.op_index[nop] = i + c->nopmax * m;
+            nop++;
+        }
+        d.nop = nop;
+        d.scr_off = (int)scr;
+        scr += (size_t)10 * blk + (size_t)2 * d.nG * c->nspfmax + 2 * nn;
+    }
+    c->dspfmodes.alloc(sizeof(SpfModeDesc) * md.size());
+    HIP_CHECK(hipMemcpy(c->dspfmodes.p, md.data(), c->dspfmodes.bytes, hipMemcpyHostToDevice));
+    c->dspfscr.alloc(z * scr);
+    c->dstatus.alloc(sizeof(double2) * 2);
+    // SPF product task scratch: 2 * nG * nspfmax per task
+    int64_t maxng = 0;
+    for (int m = 0; m < c->nm; m++) maxng = std::max(maxng, c->ngpm[m]);
+    HIP_CHECK(hipEventCreate(&c->ev0));
+    HIP_CHECK(hipEventCreate(&c->ev1));
+    // one scratch region per possible SPF task
+    const size_t ntask_max = (size_t)c->nm * std::max(c->nopmax, c->ns) * 2 + 1;
+    c->dspfscr.alloc(z * std::max(scr, (size_t)0) + z * ntask_max * 2 * (size_t)maxng * c->nspfmax);
+    HIP_CHECK(hipStreamSynchronize(c->stream));
+}
+
+DofTables dof_tables(const mg_ctx *c) {
+    DofTables t;
+    t.grid = c->dgrid.as<double>();
+    t.d1 = c->dd1.as<double>();
+    t.d2 = c->dd2.as<double>();
+    t.xd1 = c->dxd1.as<double>();
+    t.ngpmax = c->ngpmax;
+    t.ngp = c->dngp.as<int>();
+    t.mode_dofs = c->dmode_dofs.as<int>();
+    t.mode_ndof = c->dmode_ndof.as<int>();
+    return t;
+}
+
+size_t spf_scratch_base(const mg_ctx *c) {
+    // SPF propagation scratch comes first; SPF-product task scratch after it
+    size_t scr = 0;
+    const size_t nn = (size_t)c->nspfmax * c->nspfmax;
+    for (int m = 0; m < c->nm; m++) {
+        size_t blk = 0;
+        for (int s = 0; s < c->nsets(m); s++) blk += (size_t)c->ngpm[m] * c->NS(m, s);
+        scr += 10 * blk + 2 * (size_t)c->ngpm[m] * c->nspfmax + 2 * nn;
+    }
+    return scr;
+}
+
+// ----------------------------------------------------------------------------
+// Group passes.
+int nmax_template(const mg_ctx *c) { return c->nspfmax <= 4 ? 4 : (c->nspfmax <= 8 ? 8 : 16); }
+
+// Collects the launches of one operation; phases run in order on the stream.
+struct PlanBuilder {
+    std::vector<GroupTask> tasks;
+    std::vector<Launch> launches;
+    void phase(const std::vector<GroupTask> &ts) {
+        if (ts.empty()) return;
+        int64_t mx = 0;
+        for (auto &t : ts) mx = std::max(mx, (t.L / t.Lt) * t.R);
+        launches.push_back({tasks.size(), (int)ts.size(), mx});
+        tasks.insert(tasks.end(), ts.begin(), ts.end());
+    }
+    std::unique_ptr<Plan> finish() {
+        auto p = std::make_unique<Plan>();
+        p->buf.alloc(sizeof(GroupTask) * std::max<size_t>(1, tasks.size()));
+        if (!tasks.empty())
+            HIP_CHECK(hipMemcpy(p->buf.p, tasks.data(), sizeof(GroupTask) * tasks.size(), hipMemcpyHostToDevice));
+        p->launches = launches;
+        return p;
+    }
+};
+
+void run_plan(mg_ctx *c, const Plan &p) {
+    const GroupTask *base = p.buf.as<GroupTask>();
+    if (!c->prof) {
+        for (auto &l : p.launches)
+            launch_group_tasks(base + l.off, l.n, l.maxblocks, nmax_template(c), c->brick, c->stream);
+        return;
+    }
+    while (c->prof_ev.size() < 2 * p.launches.size()) {
+        hipEvent_t e;
+        HIP_CHECK(hipEventCreate(&e));
+        c->prof_ev.push_back(e);
+    }
+    for (size_t i = 0; i < p.launches.size(); i++) {
+        const Launch &l = p.launches[i];
+        HIP_CHECK(hipEventRecord(c->prof_ev[2 * i], c->stream));
+        launch_group_tasks(base + l.off, l.n, l.maxblocks, nmax_template(c), c->brick, c->stream);
+        HIP_CHECK(hipEventRecord(c->prof_ev[2 * i + 1], c->stream));
+    }
+    HIP_CHECK(hipStreamSynchronize(c->stream));
+    for (size_t i = 0; i < p.launches.size(); i++) {
+        float ms = 0.f;
+        HIP_CHECK(hipEventElapsedTime(&ms, c->prof_ev[2 * i], c->prof_ev[2 * i + 1]));
+        c->prof_ms += ms;
+        c->prof_launches++;
+    }
+}
+
+// Build a group-pass task over group g for a tensor with current sizes `sz`
+// (modes < g.first already in output sizes).  ops: (mode, matrix, adj, nin, nout).
+struct OpSpec {
+    int mode;
+    const double2 *mat;
+    int adj, nin, nout;
+};
+
+GroupTask make_task(const mg_ctx *c, int g, const std::vector<int64_t> &sz, const std::vector<OpSpec> &ops, int kind,
+                    const double2 *X, double2 *Y, cplx alpha, int accumulate) {
+    GroupTask t;
+    std::memset(&t, 0, sizeof t);
+    const int m0 = c->groups[g].first, m1 = c->groups[g].second;
+    t.X = X;
+    t.Y = Y;
+    t.ng = m1 - m0;
+    int64_t L = 1, R = 1;
+    for (int m = 0; m < m0; m++) L *= sz[m];
+    for (int m = m1; m < c->nm; m++) R *= sz[m];
+    std::vector<int64_t> cur(sz.begin() + m0, sz.begin() + m1);
+    int64_t Min = 1;
+    for (auto v : cur) Min *= v;
+    int64_t Mmax = Min;
+    std::vector<int> nout(cur.begin(), cur.end());
+    t.nops = (int)ops.size();
+    if (t.nops > kMaxOpsPerTask) throw HipError{"too many ops in a group task"};
+    for (int q = 0; q < t.nops; q++) {
+        const OpSpec &o = ops[q];
+        const int p = o.mode - m0;
+        t.opmode[q] = p;
+        t.opmat[q] = o.mat;
+        t.opadj[q] = o.adj;
+        t.opnin[q] = o.nin;
+        t.opnout[q] = o.nout;
+        if (cur[p] != o.nin) throw HipError{"op size mismatch in group task"};
+        if (kind == 0) {
+            cur[p] = o.nout;
+            int64_t M = 1;
+            for (auto v : cur) M *= v;
+            Mmax = std::max(Mmax, M);
+            nout[p] = o.nout;
+        }
+    }
+    for (int q = 0; q < t.ng; q++) {
+        t.nin[q] = (int)sz[m0 + q];
+        t.nout[q] = nout[q];
+    }
+    t.L = L;
+    t.R = R;
+    t.Lt = largest_divisor_le(L, std::max<int64_t>(1, c->brick / Mmax));
+    if (t.Lt * Mmax > c->brick) throw HipError{"brick too small for a mode group"};
+    t.kind = kind;
+    t.ld = c->nspfmax;
+    t.alpha = d2(alpha);
+    t.accumulate = accumulate;
+    return t;
+}
+
+// Calc_DHD_B (cmf2_derivatives.f90:1-80): HB = sum_s sum_k B_s x_k Hsum(s,k)
+// + sum_chains ae (x_k O_k) B_rhs.
+std::unique_ptr<Plan> build_apply_h(mg_ctx *c, const double2 *B, double2 *HB) {
+    PlanBuilder pb;
+    const int G = (int)c->groups.size();
+    const size_t mixed = (size_t)c->mixed;
+    double2 *ctmp = c->dchain.as<double2>();
+    std::vector<const double2 *> cur(c->chains.size());
+    std::vector<std::vector<int64_t>> csz(c->chains.size());
+    std::vector<int> last(c->chains.size(), -1), parity(c->chains.size(), 0);
+    for (size_t k = 0; k < c->chains.size(); k++) {
+        const Chain &ch = c->chains[k];
+        cur[k] = B + c->off[ch.rhs];
+        csz[k].resize(c->nm);
+        for (int m = 0; m < c->nm; m++) csz[k][m] = c->NS(m, ch.rhs);
+        for (auto &o : ch.ops)
+            for (int g = 0; g < G; g++)
+                if (o.mode >= c->groups[g].first && o.mode < c->groups[g].second) last[k] = std::max(last[k], g);
+        if (last[k] < 0) last[k] = 0;
+    }
+    for (int g = 0; g < G; g++) {
+        std::vector<GroupTask> tasks;
+        // diagonal sums for every state
+        for (int s = 0; s < c->ns; s++) {
+            std::vector<int64_t> sz(c->nm);
+            for (int m = 0; m < c->nm; m++) sz[m] = c->NS(m, s);
+            std::vector<OpSpec> ops;
+            for (int m = c->groups[g].first; m < c->groups[g].second; m++) {
+                const int row = s * c->nm + m;
+                if (c->hs_rowptr[row + 1] == c->hs_rowptr[row]) continue;
+                ops.push_back({m, c->hsum(s, m), 0, (int)sz[m], (int)sz[m]});
+            }
+            if (ops.empty() && g > 0) continue;
+            tasks.push_back(make_task(c, g, sz, ops, 1, B + c->off[s], HB + c->off[s], 1.0, g > 0 ? 1 : 0));
+        }
+        // non-final chain passes
+        for (size_t k = 0; k < c->chains.size(); k++) {
+            if (g >= last[k]) continue;
+            const Chain &ch = c->chains[k];
+            std::vector<OpSpec> ops;
+            for (auto &o : ch.ops)
+                if (o.mode >= c->groups[g].first && o.mode < c->groups[g].second)
+                    ops.push_back({o.mode, c->h1(o.mode, o.op), 0, (int)csz[k][o.mode], (int)c->NS(o.mode, ch.lhs)});
+            if (ops.empty()) continue;
+            double2 *Y = ctmp + mixed * (2 * k + parity[k]);
+            tasks.push_back(make_task(c, g, csz[k], ops, 0, cur[k], Y, 1.0, 0));
+            for (auto &o : ops) csz[k][o.mode] = o.nout;
+            cur[k] = Y;
+            parity[k] ^= 1;
+        }
+        pb.phase(tasks);
+    }
+    // final chain passes, serialised per output state
+    std::vector<std::vector<size_t>> per_state(c->ns);
+    for (size_t k = 0; k < c->chains.size(); k++) per_state[c->chains[k].lhs].push_back(k);
+    size_t rounds = 0;
+    for (auto &v : per_state) rounds = std::max(rounds, v.size());
+    for (size_t r = 0; r < rounds; r++) {
+        std::vector<GroupTask> tasks;
+        for (int s = 0; s < c->ns; s++) {
+            if (r >= per_state[s].size()) continue;
+            const size_t k = per_state[s][r];
+            const Chain &ch = c->chains[k];
+            const int g = last[k];
+            std::vector<OpSpec> ops;
+            for (auto &o : ch.ops)
+                if (o.mode >= c->groups[g].first && o.mode < c->groups[g].second)
+                    ops.push_back({o.mode, c->h1(o.mode, o.op), 0, (int)csz[k][o.mode], (int)c->NS(o.mode, ch.lhs)});
+            tasks.push_back(make_task(c, g, csz[k], ops, 0, cur[k], HB + c->off[ch.lhs], ch.ae, 1));
+        }
+        pb.phase(tasks);
+    }
+    return pb.finish();
+}
+
+const Plan &get_plan(mg_ctx *c, const PlanKey &key, std::unique_ptr<Plan> (*build)(mg_ctx *, const PlanKey &)) {
+    auto it = c->plans.find(key);
+    if (it != c->plans.end()) return *it->second;
+    auto p = build(c, key);
+    const Plan &ref = *p;
+    c->plans[key] = std::move(p);
+    return ref;
+}
+
+void apply_h(mg_ctx *c, const double2 *B, double2 *HB) {
+    PlanKey key{(const void *)1, B, HB, nullptr};
+    const Plan &p = get_plan(c, key, [](mg_ctx *cc, const PlanKey &k) {
+        return build_apply_h(cc, (const double2 *)k[1], (double2 *)k[2]);
+    });
+    run_plan(c, p);
+    check_launch();
+}
+
+// Y_s = (x_m M_(m,s)) X_s for every state (norm_wf / WF_Ovl chains).
+std::unique_ptr<Plan> build_apply_kron(mg_ctx *c, const double2 *X, double2 *Y, const double2 *mats) {
+    PlanBuilder pb;
+    const int G = (int)c->groups.size();
+    const size_t nn = (size_t)c->nspfmax * c->nspfmax;
+    double2 *tmp = c->dkron.as<double2>();
+    std::vector<const double2 *> cur(c->ns);
+    std::vector<int> parity(c->ns, 0);
+    for (int s = 0; s < c->ns; s++) cur[s] = X + c->off[s];
+    for (int g = 0; g < G; g++) {
+        std::vector<GroupTask> tasks;
+        for (int s = 0; s < c->ns; s++) {
+            std::vector<int64_t> sz(c->nm);
+            for (int m = 0; m < c->nm; m++) sz[m] = c->NS(m, s);
+            std::vector<OpSpec> ops;
+            for (int m = c->groups[g].first; m < c->groups[g].second; m++)
+                ops.push_back({m, mats + nn * (m + (size_t)c->nm * s), 0, (int)sz[m], (int)sz[m]});
+            double2 *out = (g == G - 1) ? Y + c->off[s] : tmp + (size_t)c->nconf * parity[s] + c->off[s];
+            tasks.push_back(make_task(c, g, sz, ops, 0, cur[s], out, 1.0, 0));
+            cur[s] = out;
+            parity[s] ^= 1;
+        }
+        pb.phase(tasks);
+    }
+    return pb.finish();
+}
+
+void apply_kron(mg_ctx *c, const double2 *X, double2 *Y, const double2 *mats) {
+    PlanKey key{(const void *)2, X, Y, mats};
+    const Plan &p = get_plan(c, key, [](mg_ctx *cc, const PlanKey &k) {
+        return build_apply_kron(cc, (const double2 *)k[1], (double2 *)k[2], (const double2 *)k[3]);
+    });
+    run_plan(c, p);
+    check_launch();
+}
+
+double2 dot(mg_ctx *c, const double2 *x, const double2 *y, int64_t n) {
+    double *part = c->dpartials.as<double>();
+    double2 *out = reinterpret_cast<double2 *>(c->dscal.as<double>() + 8);
+    launch_dot(x, y, n, part, c->nblocks_red, out, c->stream);
+    double2 h;
+    HIP_CHECK(hipMemcpyAsync(&h, out, sizeof h, hipMemcpyDeviceToHost, c->stream));
+    HIP_CHECK(hipStreamSynchronize(c->stream));
+    return h;
+}
+
+double norm2(mg_ctx *c, const double2 *x, int64_t n) {
+    double *part = c->dpartials.as<double>();
+    double *out = c->dscal.as<double>() + 12;
+    launch_norm2(x, n, part, c->nblocks_red, out, c->stream);
+    double h;
+    HIP_CHECK(hipMemcpyAsync(&h, out, sizeof h, hipMemcpyDeviceToHost, c->stream));
+    HIP_CHECK(hipStreamSynchronize(c->stream));
+    return h;
+}
+
+// ----------------------------------------------------------------------------
+// SPF matrices: h1mode (hamilton.f90:433-485) and SPF overlaps.
+void run_spf_tasks(mg_ctx *c, const PlanKey &key, std::vector<SpfTask> (*build)(mg_ctx *, const PlanKey &)) {
+    auto &cache = c->spf_plans;
+    auto it = cache.find(key);
+    if (it == cache.end()) {
+        std::vector<SpfTask> tasks = build(c, key);
+        const size_t base = spf_scratch_base(c);
+        int64_t maxng = 0;
+        for (int m = 0; m < c->nm; m++) maxng = std::max(maxng, c->ngpm[m]);
+        for (size_t i = 0; i < tasks.size(); i++)
+            tasks[i].scratch = c->dspfscr.as<double2>() + base + i * 2 * (size_t)maxng * c->nspfmax;
+        auto p = std::make_unique<SpfPlan>();
+        p->n = (int)tasks.size();
+        p->buf.alloc(sizeof(SpfTask) * std::max<size_t>(1, tasks.size()));
+        if (!tasks.empty())
+            HIP_CHECK(hipMemcpy(p->buf.p, tasks.data(), sizeof(SpfTask) * tasks.size(), hipMemcpyHostToDevice));
+        it = cache.emplace(key, std::move(p)).first;
+    }
+    launch_spf_tasks(it->second->buf.as<SpfTask>(), it->second->n, dof_tables(c), 0, c->stream);
+    check_launch();
+}
+
+std::vector<SpfTask> build_h1mode(mg_ctx *c, const PlanKey &key) {
+    const double2 *psi = (const double2 *)key[1];
+    std::vector<SpfTask> tasks;
+    for (int m = 0; m < c->nm; m++)
+        for (int i = 0; i < c->nOper[m]; i++) {
+            if (c->isIdent[m + (size_t)c->nm * i]) continue;
+            const Oper &v = c->opers[(size_t)m * c->nopmax + i];
+            const int s1 = v.lhs - 1, s2 = v.rhs - 1;
+            SpfTask t;
+            std::memset(&t, 0, sizeof t);
+            t.phiA = psi + c->IP(m, s1);
+            t.phiB = psi + c->IP(m, s2);
+            t.nG = (int)c->ngpm[m];
+            t.na = (int)c->NS(m, s1);
+            t.nb = (int)c->NS(m, s2);
+            t.mode = m;
+            t.ncode = (int)c->nmodeDOF[m];
+            for (int k = 0; k < t.ncode; k++) t.codes[k] = v.codes[k];
+            t.sym = (s1 == s2);
+            t.out = c->h1(m, i);
+            t.ld = c->nspfmax;
+            tasks.push_back(t);
+        }
+    return tasks;
+}
+
+void calc_h1mode(mg_ctx *c, const double2 *psi) {
+    run_spf_tasks(c, PlanKey{(const void *)3, psi, nullptr, nullptr}, build_h1mode);
+    launch_hsum(c->dhsum.as<double2>(), c->dh1.as<double2>(), c->dhs_rowptr.as<int>(), c->dhs_src.as<int>(),
+                c->dhs_coef.as<double2>(), c->ns * c->nm, c->nspfmax, c->stream);
+}
+
+// out(m, s) = <phi1_(m,s) | phi2_(m,s)>; sym for phi1 == phi2 (zherk).
+std::vector<SpfTask> build_overlaps(mg_ctx *c, const PlanKey &key) {
+    const double2 *psi1 = (const double2 *)key[1], *psi2 = (const double2 *)key[2];
+    double2 *out = (double2 *)key[3];
+    const int sym = (psi1 == psi2) ? 1 : 0;
+    std::vector<SpfTask> tasks;
+    const size_t nn = (size_t)c->nspfmax * c->nspfmax;
+    for (int s = 0; s < c->ns; s++)
+        for (int m = 0; m < c->nm; m++) {
+            SpfTask t;
+            std::memset(&t, 0, sizeof t);
+            t.phiA = psi1 + c->IP(m, s);
+            t.phiB = psi2 + c->IP(m, s);
+            t.nG = (int)c->ngpm[m];
+            t.na = t.nb = (int)c->NS(m, s);
+            t.mode = m;
+            t.ncode = 0;
+            t.sym = sym;
+            t.out = out + nn * (m + (size_t)c->nm * s);
+            t.ld = c->nspfmax;
+            tasks.push_back(t);
+        }
+    return tasks;
+}
+
+void calc_overlaps(mg_ctx *c, const double2 *psi1, const double2 *psi2, double2 *out) {
+    run_spf_tasks(c, PlanKey{(const void *)4, psi1, psi2, out}, build_overlaps);
+}
+
+// norm_wf (gmctdh_objects.f90:21-97)
+double norm_wf(mg_ctx *c, const double2 *A, const double2 *psi) {
+    calc_overlaps(c, psi, psi, c->dS.as<double2>());
+    apply_kron(c, A, c->dHtmp.as<double2>(), c->dS.as<double2>());
+    return dot(c, c->dHtmp.as<double2>(), A, c->nconf).x;
+}
+
+// WF_Ovl (gmctdh_objects.f90:101-180): <Psi1|Psi2>
+cplx wf_ovl(mg_ctx *c, const double2 *A1, const double2 *psi1, const double2 *A2, const double2 *psi2) {
+    calc_overlaps(c, psi1, psi2, c->dS2.as<double2>());
+    apply_kron(c, A2, c->dHtmp.as<double2>(), c->dS2.as<double2>());
+    double2 v = dot(c, A1, c->dHtmp.as<double2>(), c->nconf);
+    return cplx(v.x, v.y);
+}
+
+double delta_psi(mg_ctx *c, const double2 *A1, const double2 *p1, const double2 *A2, const double2 *p2) {
+    const double o11 = norm_wf(c, A1, p1), o22 = norm_wf(c, A2, p2);
+    const cplx o12 = wf_ovl(c, A1, p1, A2, p2);
+    return std::sqrt(std::fabs(2.0 * (1.0 - o12.real() / std::sqrt(o11 * o22))));
+}
+
+// ----------------------------------------------------------------------------
+// Density matrices and mean fields.
+void calc_rho(mg_ctx *c, const double2 *A) {
+    for (int s = 0; s < c->ns; s++)
+        for (int m = 0; m < c->nm; m++) {
+            GramTask g;
+            std::memset(&g, 0, sizeof g);
+            int64_t ll = 1, nl = 1;
+            for (int q = 0; q < m; q++) ll *= c->NS(q, s);
+            for (int q = m + 1; q < c->nm; q++) nl *= c->NS(q, s);
+            g.X = g.Y = A + c->off[s];
+            g.ll = ll;
+            g.nl = nl;
+            g.n1 = g.n2 = (int)c->NS(m, s);
+            g.G = c->rho(m, s);
+            g.ld = c->nspfmax;
+            g.alpha = make_double2(1, 0);
+            g.accumulate = 0;
+            g.hermitian = 1;
+            launch_gram(g, c->dpartials.as<double2>(), c->nblocks_red, c->nspfmax, c->stream);
+        }
+    // rho_ave for single-set modes (gmctdh_objects.f90:464-474)
+    std::vector<int> rowptr(1, 0), src;
+    std::vector<double2> coef;
+    std::vector<int> targets;
+    for (int m = 0; m < c->nm; m++) {
+        if (c->MultiSet[m]) continue;
+        targets.push_back(m);
+        for (int s = 0; s < c->ns; s++) { src.push_back(m + c->nm * s); coef.push_back(make_double2(1, 0)); }
+        rowptr.push_back((int)src.size());
+    }
+    if (!targets.empty()) {
+        // single-set modes are rare; do it with tiny axpys
+        const size_t nn = (size_t)c->nspfmax * c->nspfmax;
+        for (int m : targets) {
+            launch_fill(c->rho_ave(m), (int64_t)nn, make_double2(0, 0), c->stream);
+            for (int s = 0; s < c->ns; s++)
+                launch_mat_axpy(c->rho_ave(m), c->rho(m, s), make_double2(1, 0), c->nspfmax, c->nspfmax, c->stream);
+        }
+    }
+    check_launch();
+}
+
+// Calc_MeanField, Hamiltonian part (gmctdh_objects.f90:518-605) + Calc_rhom1MF
+// (:1147-1194).  Requires calc_rho(A) and calc_h1mode(psi).
+void calc_meanfield(mg_ctx *c, const double2 *A) {
+    const size_t nn = (size_t)c->nspfmax * c->nspfmax;
+    // one-mode same-state terms: MF += ae * rho  (all other modes are identity)
+    launch_hsum(c->dmf.as<double2>(), c->drho.as<double2>(), c->dmf1_rowptr.as<int>(), c->dmf1_src.as<int>(),
+                c->dmf1_coef.as<double2>(), c->nm * c->nopmax, c->nspfmax, c->stream);
+    const size_t mixed = (size_t)c->mixed;
+    double2 *AL = c->dmfbuf.as<double2>();
+    double2 *AR = AL + mixed * c->nm;
+    for (int t : c->mf_chain_terms) {
+        const int a = (int)c->lhs[t] - 1, b = (int)c->rhs[t] - 1;
+        const cplx ae = c->ae[t];
+        // right chain R_i (modes f-1 .. 0), pointers alias on identity modes
+        std::vector<const double2 *> R(c->nm);
+        std::vector<int64_t> sz(c->nm);
+        for (int m = 0; m < c->nm; m++) sz[m] = c->NS(m, b);
+        R[0] = A + c->off[b];
+        for (int i = 1; i < c->nm; i++) {
+            const int jm = c->nm - i;
+            const int op = c->iOper[jm + (size_t)c->nm * t];
+            if (c->isIdent[jm + (size_t)c->nm * op]) { R[i] = R[i - 1]; continue; }
+            ModeTask mt;
+            int64_t ll = 1, nl = 1;
+            for (int q = 0; q < jm; q++) ll *= sz[q];
+            for (int q = jm + 1; q < c->nm; q++) nl *= sz[q];
+            mt.X = R[i - 1];
+            mt.Y = AR + mixed * i;
+            mt.ll = ll;
+            mt.nl = nl;
+            mt.nin = (int)sz[jm];
+            mt.nout = (int)c->NS(jm, a);
+            mt.M = c->h1(jm, op);
+            mt.ld = c->nspfmax;
+            mt.adj = 0;
+            launch_mode_product(mt, c->nspfmax, c->stream);
+            sz[jm] = mt.nout;
+            R[i] = mt.Y;
+        }
+        // left chain L_i (modes 0 .. f-2) with h^H
+        std::vector<const double2 *> Lc(c->nm);
+        for (int m = 0; m < c->nm; m++) sz[m] = c->NS(m, a);
+        Lc[0] = A + c->off[a];
+        for (int i = 1; i < c->nm; i++) {
+            const int m = i - 1;
+            const int op = c->iOper[m + (size_t)c->nm * t];
+            if (c->isIdent[m + (size_t)c->nm * op]) { Lc[i] = Lc[i - 1]; continue; }
+            ModeTask mt;
+            int64_t ll = 1, nl = 1;
+            for (int q = 0; q < m; q++) ll *= sz[q];
+            for (int q = m + 1; q < c->nm; q++) nl *= sz[q];
+            mt.X = Lc[i - 1];
+            mt.Y = AL + mixed * i;
+            mt.ll = ll;
+            mt.nl = nl;
+            mt.nin = (int)sz[m];
+            mt.nout = (int)c->NS(m, b);
+            mt.M = c->h1(m, op);
+            mt.ld = c->nspfmax;
+            mt.adj = 1;
+            launch_mode_product(mt, c->nspfmax, c->stream);
+            sz[m] = mt.nout;
+            Lc[i] = mt.Y;
+        }
+        for (int m = 0; m < c->nm; m++) {
+            const int op = c->iOper[m + (size_t)c->nm * t];
+            if (c->isIdent[m + (size_t)c->nm * op]) continue;
+            int64_t ll = 1, nl = 1;
+            for (int q = 0; q < m; q++) ll *= c->NS(q, b);
+            for (int q = m + 1; q < c->nm; q++) nl *= c->NS(q, a);
+            GramTask g;
+            std::memset(&g, 0, sizeof g);
+            g.X = Lc[m];
+            g.Y = R[c->nm - 1 - m];
+            g.ll = ll;
+            g.nl = nl;
+            g.n1 = (int)c->NS(m, a);
+            g.n2 = (int)c->NS(m, b);
+            g.G = c->mf(m, op);
+            g.ld = c->nspfmax;
+            g.alpha = d2(ae);
+            g.accumulate = 1;
+            g.hermitian = 0;
+            launch_gram(g, c->dpartials.as<double2>(), c->nblocks_red, c->nspfmax, c->stream);
+        }
+    }
+    (void)nn;
+    launch_rmf(c->drho.as<double2>(), c->dmf.as<double2>(), c->drmf.as<double2>(), c->drmf_jobs.as<int>(),
+               c->n_rmf_jobs, c->nspfmax, c->nopmax, c->eps_rho, c->stream);
+    check_launch();
+}
+
+// ----------------------------------------------------------------------------
+// propagate_SPF for all DVR modes (cmf2_base.f90:210-260), one workgroup each.
+SpfParams spf_params(mg_ctx *c, double dt) {
+    SpfParams P;
+    P.rmf = c->drmf.as<double2>();
+    P.rho = c->drho.as<double2>();
+    P.nspfmax = c->nspfmax;
+    P.nopmax = c->nopmax;
+    P.eps_psi = c->eps_psi;
+    P.min_stepsize = c->min_stepsize;
+    P.dt = dt;
+    P.dormand_prince = (c->int_type_psi == 2);
+    P.der_factor = make_double2(0.0, -1.0);
+    P.max_attempts = 200000;
+    return P;
+}
+
+void propagate_spf(mg_ctx *c, double2 *psi, double dt) {
+    SpfParams P = spf_params(c, dt);
+    launch_fill(reinterpret_cast<double2 *>(c->dstatus.p), 1, make_double2(0, 0), c->stream);
+    launch_spf_propagate(c->dspfmodes.as<SpfModeDesc>(), c->nm, psi, c->dspfscr.as<double2>(), P, dof_tables(c),
+                         c->dstatus.as<int>(), c->stream);
+    check_launch();
+    int st = 0;
+    HIP_CHECK(hipMemcpyAsync(&st, c->dstatus.p, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIP_CHECK(hipStreamSynchronize(c->stream));
+    if (st == 1) throw HipError{"SPF overlap matrix not positive definite"};
+    if (st == 2) throw HipError{"SPF Runge-Kutta did not converge"};
+}
+
+// ----------------------------------------------------------------------------
+// Lanczos (cmf2_integration.f90:30-103).  Returns nKrylov.
+void tridiag_eig(int n, const double *alpha, const double *beta, double *w, double *U) {
+    std::vector<double> A((size_t)n * n, 0.0);
+    for (int i = 0; i < n; i++) {
+        A[i + n * i] = alpha[i];
+        if (i + 1 < n) { A[i + n * (i + 1)] = beta[i]; A[(i + 1) + n * i] = beta[i]; }
+    }
+    for (int i = 0; i < n; i++)
+        for (int j = 0; j < n; j++) U[i + n * j] = (i == j);
+    for (int sweep = 0; sweep < 100; sweep++) {
+        double off = 0.0, dg = 0.0;
+        for (int j = 0; j < n; j++)
+            for (int i = 0; i < n; i++) (i != j ? off : dg) += A[i + n * j] * A[i + n * j];
+        if (off <= 1e-32 * (dg + 1e-300) || off == 0.0) break;
+        for (int p = 0; p < n - 1; p++)
+            for (int q = p + 1; q < n; q++) {
+                const double apq = A[p + n * q];
+                if (apq == 0.0) continue;
+                const double theta = (A[q + n * q] - A[p + n * p]) / (2.0 * apq);
+                const double t = (theta >= 0 ? 1.0 : -1.0) / (std::fabs(theta) + std::sqrt(theta * theta + 1.0));
+                const double cc = 1.0 / std::sqrt(t * t + 1.0), ss = t * cc;
+                for (int k = 0; k < n; k++) {
+                    const double akp = A[k + n * p], akq = A[k + n * q];
+                    A[k + n * p] = cc * akp - ss * akq;
+                    A[k + n * q] = ss * akp + cc * akq;
+                }
+                for (int k = 0; k < n; k++) {
+                    const double apk = A[p + n * k], aqk = A[q + n * k];
+                    A[p + n * k] = cc * apk - ss * aqk;
+                    A[q + n * k] = ss * apk + cc * aqk;
+                }
+                A[p + n * q] = A[q + n * p] = 0.0;
+                for (int k = 0; k < n; k++) {
+                    const double ukp = U[k + n * p], ukq = U[k + n * q];
+                    U[k + n * p] = cc * ukp - ss * ukq;
+                    U[k + n * q] = ss * ukp + cc * ukq;
+                }
+            }
+    }
+    for (int i = 0; i < n; i++) w[i] = A[i + n * i];
+}
+
+int lanczos(mg_ctx *c, const double2 *B0, double2 *BT, double dt, bool build) {
+    const int64_t N = c->nconf;
+    double2 *BK = c->dBK.as<double2>();
+    double *dsc = c->dscal.as<double>();
+    double *part = c->dpartials.as<double>();
+    const double eps_B = 1e-8;
+    if (build) {
+        double alpha[kNKrylovMax], beta[kNKrylovMax];
+        HIP_CHECK(hipMemcpyAsync(BK, B0, sizeof(double2) * N, hipMemcpyDeviceToDevice, c->stream));
+        launch_norm2(BK, N, part, c->nblocks_red, dsc + 0, c->stream);
+        launch_scale(BK, dsc + 0, N, c->stream);
+        int j = 1;      // number of basis vectors whose H-image is being formed
+        double error = 0.0;
+        for (;;) {
+            double2 *vj = BK + (size_t)N * (j - 1), *w = BK + (size_t)N * j;
+            const double2 *vp = (j >= 2) ? BK + (size_t)N * (j - 2) : nullptr;
+            if (j >= 2 && j == kNKrylovMax) throw HipError{"Maximum number of Krylov vectors reached"};
+            apply_h(c, vj, w);
+            double2 *alpha_d = reinterpret_cast<double2 *>(dsc + 2);
+            launch_dot(vj, w, N, part, c->nblocks_red, alpha_d, c->stream);
+            launch_lanczos_update(w, vj, vp, dsc + 2, j >= 2 ? beta[j - 2] : 0.0, N, part, c->nblocks_red, dsc + 4,
+                                  c->stream);
+            launch_scale(w, dsc + 4, N, c->stream);
+            double h[3];
+            HIP_CHECK(hipMemcpyAsync(h, dsc + 2, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+            HIP_CHECK(hipStreamSynchronize(c->stream));
+            alpha[j - 1] = h[0];
+            beta[j - 1] = std::sqrt(h[2]);
+            if (j == 1) {
+                error = beta[0] * std::fabs(dt);
+                j = 2;
+                continue;
+            }
+            error = error * beta[j - 1] / (double)j * std::fabs(dt);
+            if (error < eps_B) break;
+            j++;
+        }
+        c->nkrylov = j;
+        c->lan_alpha.assign(j, 0.0);
+        c->lan_U.assign((size_t)j * j, 0.0);
+        std::vector<double> w(j);
+        tridiag_eig(j, alpha, beta, w.data(), c->lan_U.data());
+        for (int r = 0; r < j; r++) c->lan_alpha[r] = w[r];
+    }
+    const int n = c->nkrylov;
+    std::vector<double2> coef(n);
+    for (int i = 0; i < n; i++) {
+        cplx acc = 0;
+        for (int r = 0; r < n; r++)
+            acc += c->lan_U[i + n * r] * c->lan_U[0 + n * r] * std::exp(cplx(0.0, -1.0) * c->lan_alpha[r] * dt);
+        coef[i] = d2(acc);
+    }
+    double2 *dcoef = reinterpret_cast<double2 *>(dsc + 16);
+    HIP_CHECK(hipMemcpyAsync(dcoef, coef.data(), sizeof(double2) * n, hipMemcpyHostToDevice, c->stream));
+    launch_combine(BT, BK, N, dcoef, n, N, c->stream);
+    launch_norm2(BT, N, part, c->nblocks_red, dsc + 6, c->stream);
+    launch_scale(BT, dsc + 6, N, c->stream);
+    HIP_CHECK(hipStreamSynchronize(c->stream));
+    return n;
+}
+
+// ----------------------------------------------------------------------------
+// cmf2 for DVR-only wavefunctions (cmf2_base.f90:47-206).
+void copy_d2d(mg_ctx *c, double2 *dst, const double2 *src, int64_t n) {
+    HIP_CHECK(hipMemcpyAsync(dst, src, sizeof(double2) * n, hipMemcpyDeviceToDevice, c->stream));
+}
+
+void cmf2_step(mg_ctx *c, double tt, double htry, double *hdid_out, double *hnext_out, std::vector<mg_attempt> *log) {
+    const int64_t N = c->nconf, D = c->dimpsi;
+    double2 *A = c->dA.as<double2>(), *psi = c->dpsi.as<double2>();
+    double2 *B0 = c->dB0.as<double2>(), *BH = c->dBH.as<double2>(), *B02 = c->dB02.as<double2>();
+    double2 *pH = c->dpH.as<double2>(), *pH2 = c->dpH2.as<double2>();
+    const double safe = 0.95, maxscal = 1.5;
+    double hdid = htry;
+    for (;;) {
+        calc_rho(c, A);
+        calc_h1mode(c, psi);
+        calc_meanfield(c, A);
+        copy_d2d(c, B0, A, N);
+        const int nk1 = lanczos(c, B0, BH, 0.5 * hdid, true);
+        copy_d2d(c, pH, psi, D);
+        propagate_spf(c, pH, 0.5 * hdid);
+        calc_rho(c, BH);
+        calc_h1mode(c, pH);
+        calc_meanfield(c, BH);
+        copy_d2d(c, pH2, psi, D);
+        propagate_spf(c, pH2, 0.5 * hdid);
+        const double err_psi = delta_psi(c, BH, pH, BH, pH2);
+        propagate_spf(c, pH2, 0.5 * hdid);
+        calc_h1mode(c, pH2);
+        const int nk2 = lanczos(c, BH, B02, -0.5 * hdid, true);
+        launch_axpby(B02, B0, -1.0, 1.0, N, c->stream);
+        const double err_A = std::sqrt(norm2(c, B02, N)) * 0.5;
+        const double error = std::max(err_A, err_psi) / c->eps_int;
+        if (log) log->push_back({tt, hdid, err_A, err_psi, (int64_t)(!(error > 1.0)), nk1, nk2});
+        if (error > 1.0) {
+            hdid = hdid / std::sqrt(error) * safe;
+            if (hdid < c->min_stepsize) throw HipError{"CMF2 step size fell below minstep"};
+            continue;
+        }
+        lanczos(c, BH, B0, 0.5 * hdid, false);
+        copy_d2d(c, A, B0, N);
+        copy_d2d(c, psi, pH2, D);
+        double hn = std::min(hdid / std::sqrt(error), hdid * maxscal);
+        hn = std::min(hn, c->writetime);
+        *hnext_out = hn;
+        break;
+    }
+    *hdid_out = hdid;
+    HIP_CHECK(hipStreamSynchronize(c->stream));
+}
+
+// WriteOutput observables (output.f90:33-87, 143-150).
+void observables(mg_ctx *c, double t, mg_output *out, double *pops) {
+    double2 *A = c->dA.as<double2>(), *psi = c->dpsi.as<double2>();
+    calc_rho(c, A);
+    calc_h1mode(c, psi);
+    apply_h(c, A, c->dOut.as<double2>());
+    const double energy = dot(c, A, c->dOut.as<double2>(), c->nconf).x;
+    const size_t nn = (size_t)c->nspfmax * c->nspfmax;
+    std::vector<double2> r(nn * c->nm * c->ns);
+    HIP_CHECK(hipMemcpyAsync(r.data(), c->drho.p, sizeof(double2) * r.size(), hipMemcpyDeviceToHost, c->stream));
+    HIP_CHECK(hipStreamSynchronize(c->stream));
+    double tot = 0.0;
+    for (int s = 0; s < c->ns; s++) {
+        const int jel = c->MultiSet[0] ? s : 0;
+        const int64_t n = c->NS(0, jel);
+        double pop = 0.0;
+        for (int64_t i = 0; i < n; i++) pop += r[nn * (0 + (size_t)c->nm * s) + i + (size_t)c->nspfmax * i].x;
+        if (pops) pops[s] = pop;
+        tot += pop;
+    }
+    const cplx ct = wf_ovl(c, c->dAstart.as<double2>(), c->dpsistart.as<double2>(), A, psi);
+    out->time = t;
+    out->auto_re = ct.real();
+    out->auto_im = ct.imag();
+    out->norm = tot;
+    out->energy = energy / tot;
+}
+
+template <class F>
+int guarded(mg_ctx *c, F &&f) {
+    if (!c) return -1;
+    try {
+        f();
+        return 0;
+    } catch (const HipError &e) {
+        c->err = e.msg;
+        return 1;
+    } catch (const std::exception &e) {
+        c->err = e.what();
+        return 2;
+    }
+}
+
+}  // namespace
+
+// ============================================================================
+// C-ABI
+extern "C" {
+
+int mg_create(const mg_problem *prob, int64_t precision, mg_ctx **out) {
+    if (!out || !prob) return -1;
+    *out = nullptr;
+    mg_ctx *c = new mg_ctx();
+    c->precision = (int)precision;
+    int rc = guarded(c, [&] {
+        if (precision != 0) throw HipError{"only precision 0 (complex FP64) is available"};
+        int ndev = 0;
+        HIP_CHECK(hipGetDeviceCount(&ndev));
+        if (ndev < 1) throw HipError{"no HIP device"};
+        HIP_CHECK(hipStreamCreate(&c->stream));
+        setup(c, prob);
+        allocate(c);
+        c->stepsize = c->writetime * c->eps_int;  // propagator.f90:103
+    });
+    if (rc != 0) {
+        fprintf(stderr, "mg_create: %s\n", c->err.c_str());
+        *out = c;  // caller can read the error, then destroy
+        return rc;
+    }
+    *out = c;
+    return 0;
+}
+
+int mg_destroy(mg_ctx *c) {
+    if (!c) return 0;
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    if (c->ev0) (void)hipEventDestroy(c->ev0);
+    if (c->ev1) (void)hipEventDestroy(c->ev1);
+    for (auto e : c->prof_ev) (void)hipEventDestroy(e);
+    hipStream_t s = c->stream;
+    delete c;
+    if (s) (void)hipStreamDestroy(s);
+    return 0;
+}
+
+const char *mg_last_error(const mg_ctx *c) { return c ? c->err.c_str() : "null context"; }
+
+int mg_dims(const mg_ctx *c, int64_t *nspfmax, int64_t *nopmax) {
+    if (!c) return -1;
+    if (nspfmax) *nspfmax = c->nspfmax;
+    if (nopmax) *nopmax = c->nopmax;
+    return 0;
+}
+
+int mg_stepsize(const mg_ctx *c, double *h) {
+    if (!c || !h) return -1;
+    *h = c->stepsize;
+    return 0;
+}
+
+int mg_apply_h_profile(mg_ctx *c, const void *dB, void *dHB, double *kernel_ms, int64_t *nlaunch) {
+    return guarded(c, [&] {
+        c->prof = true;
+        c->prof_ms = 0.0;
+        c->prof_launches = 0;
+        try {
+            apply_h(c, static_cast<const double2 *>(dB), static_cast<double2 *>(dHB));
+        } catch (...) {
+            c->prof = false;
+            throw;
+        }
+        c->prof = false;
+        if (kernel_ms) *kernel_ms = c->prof_ms;
+        if (nlaunch) *nlaunch = c->prof_launches;
+    });
+}
+int64_t mg_nconf(const mg_ctx *c) { return c ? c->nconf : -1; }
+int64_t mg_dimpsi(const mg_ctx *c) { return c ? c->dimpsi : -1; }
+
+int mg_set_state(mg_ctx *c, const double *A, const double *psi) {
+    return guarded(c, [&] {
+        HIP_CHECK(hipMemcpy(c->dA.p, A, sizeof(double2) * c->nconf, hipMemcpyHostToDevice));
+        HIP_CHECK(hipMemcpy(c->dpsi.p, psi, sizeof(double2) * c->dimpsi, hipMemcpyHostToDevice));
+    });
+}
+
+int mg_get_state(mg_ctx *c, double *A, double *psi) {
+    return guarded(c, [&] {
+        HIP_CHECK(hipStreamSynchronize(c->stream));
+        if (A) HIP_CHECK(hipMemcpy(A, c->dA.p, sizeof(double2) * c->nconf, hipMemcpyDeviceToHost));
+        if (psi) HIP_CHECK(hipMemcpy(psi, c->dpsi.p, sizeof(double2) * c->dimpsi, hipMemcpyDeviceToHost));
+    });
+}
+
+int mg_set_start(mg_ctx *c, const double *A, const double *psi) {
+    return guarded(c, [&] {
+        HIP_CHECK(hipMemcpy(c->dAstart.p, A, sizeof(double2) * c->nconf, hipMemcpyHostToDevice));
+        HIP_CHECK(hipMemcpy(c->dpsistart.p, psi, sizeof(double2) * c->dimpsi, hipMemcpyHostToDevice));
+    });
+}
+
+int mg_calc_h1mode(mg_ctx *c, double *h1_out) {
+    return guarded(c, [&] {
+        calc_h1mode(c, c->dpsi.as<double2>());
+        if (h1_out) {
+            HIP_CHECK(hipMemcpyAsync(h1_out, c->dh1.p, c->dh1.bytes, hipMemcpyDeviceToHost, c->stream));
+        }
+        HIP_CHECK(hipStreamSynchronize(c->stream));
+    });
+}
+
+int mg_calc_rho(mg_ctx *c, double *rho_out) {
+    return guarded(c, [&] {
+        calc_rho(c, c->dA.as<double2>());
+        if (rho_out) {
+            const size_t nn = (size_t)c->nspfmax * c->nspfmax;
+            HIP_CHECK(hipMemcpyAsync(rho_out, c->drho.p, sizeof(double2) * nn * c->nm * c->ns, hipMemcpyDeviceToHost,
+                                     c->stream));
+        }
+        HIP_CHECK(hipStreamSynchronize(c->stream));
+    });
+}
+
+int mg_calc_meanfield(mg_ctx *c, double *mf_out, double *rmf_out) {
+    return guarded(c, [&] {
+        calc_meanfield(c, c->dA.as<double2>());
+        if (mf_out) HIP_CHECK(hipMemcpyAsync(mf_out, c->dmf.p, c->dmf.bytes, hipMemcpyDeviceToHost, c->stream));
+        if (rmf_out) HIP_CHECK(hipMemcpyAsync(rmf_out, c->drmf.p, c->drmf.bytes, hipMemcpyDeviceToHost, c->stream));
+        HIP_CHECK(hipStreamSynchronize(c->stream));
+    });
+}
+
+int mg_apply_h(mg_ctx *c, const double *B, double *HB) {
+    return guarded(c, [&] {
+        HIP_CHECK(hipMemcpyAsync(c->dIn.p, B, sizeof(double2) * c->nconf, hipMemcpyHostToDevice, c->stream));
+        apply_h(c, c->dIn.as<double2>(), c->dOut.as<double2>());
+        HIP_CHECK(hipMemcpyAsync(HB, c->dOut.p, sizeof(double2) * c->nconf, hipMemcpyDeviceToHost, c->stream));
+        HIP_CHECK(hipStreamSynchronize(c->stream));
+    });
+}
+
+int mg_apply_h_device(mg_ctx *c, const void *dB, void *dHB) {
+    return guarded(c, [&] {
+        HIP_CHECK(hipEventRecord(c->ev0, c->stream));
+        apply_h(c, static_cast<const double2 *>(dB), static_cast<double2 *>(dHB));
+        HIP_CHECK(hipEventRecord(c->ev1, c->stream));
+        HIP_CHECK(hipEventSynchronize(c->ev1));
+        float ms = 0.f;
+        HIP_CHECK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+        c->last_ms = ms;
+    });
+}
+
+int mg_apply_h_stats(mg_ctx *c, double *bytes_alg, double *flops_alg) {
+    return guarded(c, [&] {
+        // bytes: read B once, write HB once; flops: 8 per complex MAC over all
+        // mode products of the diagonal sums and the chains.
+        double fl = 0.0;
+        for (int s = 0; s < c->ns; s++)
+            for (int m = 0; m < c->nm; m++) {
+                const int row = s * c->nm + m;
+                if (c->hs_rowptr[row + 1] == c->hs_rowptr[row]) continue;
+                fl += 8.0 * c->NS(m, s) * (double)c->nAConf[s];
+            }
+        for (auto &ch : c->chains) {
+            std::vector<int64_t> sz(c->nm);
+            for (int m = 0; m < c->nm; m++) sz[m] = c->NS(m, ch.rhs);
+            for (auto &o : ch.ops) {
+                double tot = 1.0;
+                for (int m = 0; m < c->nm; m++) tot *= (double)sz[m];
+                const double nout = (double)c->NS(o.mode, ch.lhs);
+                fl += 8.0 * nout * tot;  // tot/nin fibers * nin*nout MACs
+                sz[o.mode] = (int64_t)nout;
+            }
+        }
+        if (bytes_alg) *bytes_alg = 2.0 * 16.0 * (double)c->nconf;
+        if (flops_alg) *flops_alg = fl;
+        c->bytes_alg = 2.0 * 16.0 * (double)c->nconf;
+        c->flops_alg = fl;
+    });
+}
+
+int mg_der_spf(mg_ctx *c, int64_t im, const double *psi_in, double *dpsi_out) {
+    return guarded(c, [&] {
+        if (im < 1 || im > c->nm) throw HipError{"mode index out of range"};
+        HIP_CHECK(hipMemcpyAsync(c->dpH.p, psi_in, sizeof(double2) * c->dimpsi, hipMemcpyHostToDevice, c->stream));
+        launch_fill(c->dpH2.as<double2>(), c->dimpsi, make_double2(0, 0), c->stream);
+        launch_fill(reinterpret_cast<double2 *>(c->dstatus.p), 1, make_double2(0, 0), c->stream);
+        SpfParams P = spf_params(c, 0.0);
+        launch_der_spf(c->dspfmodes.as<SpfModeDesc>(), (int)im - 1, c->dpH.as<double2>(), c->dpH2.as<double2>(),
+                       c->dspfscr.as<double2>(), P, dof_tables(c), c->dstatus.as<int>(), c->stream);
+        HIP_CHECK(hipMemcpyAsync(dpsi_out, c->dpH2.p, sizeof(double2) * c->dimpsi, hipMemcpyDeviceToHost, c->stream));
+        HIP_CHECK(hipStreamSynchronize(c->stream));
+    });
+}
+
+int mg_cmf2_step(mg_ctx *c, double t, double htry, double *hdid, double *hnext) {
+    return guarded(c, [&] { cmf2_step(c, t, htry, hdid, hnext, nullptr); });
+}
+
+int mg_observables(mg_ctx *c, double t, mg_output *out, double *pops) {
+    return guarded(c, [&] { observables(c, t, out, pops); });
+}
+
+int mg_propagate(mg_ctx *c, double t0, double tfinal, mg_output *outs, double *pops, int64_t max_out, int64_t *n_out,
+                 mg_attempt *atts, int64_t max_att, int64_t *n_att) {
+    return guarded(c, [&] {
+        const double wt = c->writetime, minstep = c->min_stepsize;
+        const double maxtime = tfinal + wt * 0.1;
+        double time = t0, stepsize = c->stepsize;
+        *n_out = 0;
+        if (n_att) *n_att = 0;
+        std::vector<mg_attempt> log;
+        while (time < maxtime) {
+            if (*n_out < max_out) {
+                observables(c, time, &outs[*n_out], pops ? pops + c->ns * *n_out : nullptr);
+                (*n_out)++;
+            }
+            const double tnext = time + wt;
+            for (;;) {
+                double done = 0.0, next = 0.0;
+                log.clear();
+                cmf2_step(c, time, stepsize, &done, &next, atts ? &log : nullptr);
+                if (atts && n_att)
+                    for (auto &a : log)
+                        if (*n_att < max_att) atts[(*n_att)++] = a;
+                time = time + done;
+                if (time >= tnext - minstep) break;
+                const int64_t k = (int64_t)((tnext - time) / next);
+                stepsize = (k == 0) ? tnext - time : (tnext - time) / (double)(k + 1);
+                c->stepsize = stepsize;
+            }
+        }
+        c->stepsize = stepsize;
+    });
+}
+
+}  // extern "C"

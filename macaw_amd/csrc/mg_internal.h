@@ -1,0 +1,156 @@
+// Internal declarations shared by the device kernels (mg_kernels.hip) and the
+// host engine (mg_engine.hip).  Not part of the C-ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace mg {
+
+constexpr int kMaxGroupModes = 8;   // modes fused into one LDS brick pass
+constexpr int kMaxOpsPerTask = 8;
+constexpr int kMaxDofPerMode = 4;
+constexpr int kNKrylovMax = 20;     // cmf2_integration.f90:12
+
+// Complex helpers on double2 / float2 (interleaved re, im).
+__host__ __device__ inline double2 cmul(double2 a, double2 b) {
+    return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__host__ __device__ inline double2 cconjmul(double2 a, double2 b) {  // conj(a) * b
+    return make_double2(a.x * b.x + a.y * b.y, a.x * b.y - a.y * b.x);
+}
+__host__ __device__ inline double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
+__host__ __device__ inline double2 csub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
+__host__ __device__ inline double2 cscale(double s, double2 a) { return make_double2(s * a.x, s * a.y); }
+__host__ __device__ inline double2 cconj(double2 a) { return make_double2(a.x, -a.y); }
+
+// One pass over a group of consecutive modes [m0, m0+ng) of a tensor laid out
+// as X[l, g, r]: l over the modes before the group (size L, already in output
+// sizes), g the group multi-index (first mode fastest), r over the modes after
+// the group (size R).  A workgroup holds an Lt x M brick in LDS.
+//   kind 0 (chain): brick <- brick x_{opmode[0]} M_0 x_{opmode[1]} M_1 ...
+//   kind 1 (sum):   brick <- sum_q brick x_{opmode[q]} M_q   (square ops)
+// Y[...] = alpha * brick  (accumulate = 0)  or  Y += alpha * brick (1).
+struct GroupTask {
+    const double2 *X;
+    double2 *Y;
+    int64_t L, Lt, R;
+    int ng;
+    int nin[kMaxGroupModes];     // group mode sizes of X
+    int nout[kMaxGroupModes];    // group mode sizes of Y
+    int kind;
+    int nops;
+    int opmode[kMaxOpsPerTask];  // position inside the group
+    int opadj[kMaxOpsPerTask];   // 1: use conj-transpose of the matrix
+    int opnin[kMaxOpsPerTask];   // columns of the effective op matrix
+    int opnout[kMaxOpsPerTask];  // rows of the effective op matrix
+    const double2 *opmat[kMaxOpsPerTask];
+    int ld;                      // leading dimension of the op matrices
+    double2 alpha;
+    int accumulate;
+};
+
+// Single-mode product: Y[l, i, r] = sum_j M[i, j] X[l, j, r]  (adj: conj(M[j, i])).
+struct ModeTask {
+    const double2 *X;
+    double2 *Y;
+    int64_t ll, nl;
+    int nin, nout;
+    const double2 *M;
+    int ld;
+    int adj;
+};
+
+// Hole contraction: G[j1 + ld*j2] (+)= alpha * sum_{l,r} conj(X[l,j1,r]) Y[l,j2,r].
+struct GramTask {
+    const double2 *X, *Y;
+    int64_t ll, nl;
+    int n1, n2;
+    double2 *G;
+    int ld;
+    double2 alpha;
+    int accumulate;
+    int hermitian;               // 1: X == Y, result made exactly Hermitian
+};
+
+// Matrix elements between SPF blocks of one mode:
+//   out[a + ld*b] = <phiA_a | O | phiB_b>,  O = product of primitive DOF ops.
+// sym = 1: same-state Hermitian symmetrisation (zher2k with 1/2).
+struct SpfTask {
+    const double2 *phiA, *phiB;
+    int nG, na, nb;
+    int mode;                    // 0-based logical mode (for DOF tables)
+    int ncode;
+    int codes[kMaxDofPerMode];   // primitive operator per DOF of the mode
+    int sym;
+    double2 *out;
+    int ld;
+    double2 *scratch;            // 2 * nG * nb
+};
+
+// Per-DOF primitive data on the device (grid values and DVR matrices).
+struct DofTables {
+    const double *grid;          // [ngpmax, ndof]
+    const double *d1, *d2, *xd1; // [ngpmax, ngpmax, ndof]
+    int ngpmax;
+    const int *ngp;              // [ndof]
+    const int *mode_dofs;        // [nmodes, kMaxDofPerMode] 0-based DOF ids
+    const int *mode_ndof;        // [nmodes]
+};
+
+// SPF propagation of one logical mode over dt (propagate_SPF + rk_SPF_ck/dp).
+struct SpfModeDesc {
+    int mode;
+    int nG;
+    int nsets;
+    int off[8];                  // offsets (complex elements) of the set blocks in psi
+    int nspf[8];                 // SPFs per set
+    int nop;                     // operators of the mode (non-identity)
+    int op_lhs_set[32], op_rhs_set[32];
+    int op_codes[32][kMaxDofPerMode];
+    int op_index[32];            // index in rmf
+    int rho_index[8];            // index of the density matrix per set (into rho array)
+    int blk;                     // complex elements of all set blocks of the mode
+    int scr_off;                 // offset into the scratch pool (complex elements)
+};
+
+struct SpfParams {
+    const double2 *rmf;          // [nspfmax^2 * nopmax * nmodes]
+    const double2 *rho;          // density matrices, index via rho_index
+    int nspfmax, nopmax;
+    double eps_psi, min_stepsize, dt;
+    int dormand_prince;
+    double2 der_factor;
+    int max_attempts;
+};
+
+// Host-side launchers (mg_kernels.hip).
+void launch_group_tasks(const GroupTask *d_tasks, int ntasks, int64_t max_blocks_per_task,
+                        int nmax, int brick, hipStream_t s);
+void launch_mode_product(const ModeTask &t, int nmax, hipStream_t s);
+void launch_gram(const GramTask &t, double2 *partials, int nblocks, int nmax, hipStream_t s);
+void launch_spf_tasks(const SpfTask *d_tasks, int ntasks, DofTables dt, int max_ng, hipStream_t s);
+void launch_fill(double2 *p, int64_t n, double2 v, hipStream_t s);
+void launch_axpby(double2 *y, const double2 *x, double a, double b, int64_t n, hipStream_t s);
+void launch_dot(const double2 *x, const double2 *y, int64_t n, double *partials, int nblocks,
+                double2 *out, hipStream_t s);
+void launch_lanczos_update(double2 *w, const double2 *vj, const double2 *vprev, const double *alpha,
+                           double beta_prev, int64_t n, double *partials, int nblocks, double *norm2,
+                           hipStream_t s);
+void launch_scale(double2 *x, const double *norm2, int64_t n, hipStream_t s);
+void launch_combine(double2 *out, const double2 *basis, int64_t stride, const double2 *coef, int k,
+                    int64_t n, hipStream_t s);
+void launch_norm2(const double2 *x, int64_t n, double *partials, int nblocks, double *out,
+                  hipStream_t s);
+void launch_hsum(double2 *hsum, const double2 *h1, const int *rowptr, const int *src,
+                 const double2 *coef, int nmat, int nmax, hipStream_t s);
+int group_capacity(int brick);
+void launch_rmf(const double2 *rho, const double2 *mf, double2 *rmf, const int *jobs, int njobs,
+                int nspfmax, int nopmax, double eps, hipStream_t s);
+void launch_spf_propagate(const SpfModeDesc *d_modes, int nmodes, double2 *psi, double2 *scratch,
+                          SpfParams p, DofTables dt, int *d_status, hipStream_t s);
+void launch_der_spf(const SpfModeDesc *d_modes, int mode, const double2 *psi, double2 *dpsi,
+                    double2 *scratch, SpfParams p, DofTables dt, int *d_status, hipStream_t s);
+void launch_mat_axpy(double2 *dst, const double2 *src, double2 alpha, int n, int ld, hipStream_t s);
+
+}  // namespace mg

@@ -1,0 +1,1008 @@
+// Device kernels of the MI355X G-MCTDH engine (gfx950, wave64).
+//
+// Hot path: the sum-of-products Hamiltonian applied to the coefficient tensor
+// (Calc_DHD_B, cmf2_derivatives.f90:1-80) is expressed as passes over groups
+// of consecutive modes; each workgroup stages an Lt x M brick of the tensor in
+// LDS and applies all mode products of the group there (k_group), so a chain
+// of f mode products costs one HBM read + write per group instead of per mode.
+// Hole contractions (density matrices, mean fields; gmctdh_objects.f90:408-605)
+// are split-K reductions with a deterministic second stage (k_gram*).
+// The SPF equations of motion and their adaptive Runge-Kutta integration
+// (cmf2_integration.f90:107-412, cmf2_base.f90:210-260) run entirely on the
+// device, one workgroup per logical mode (k_spf_propagate).
+#include "mg_internal.h"
+
+#include <math.h>
+
+namespace mg {
+
+__device__ __forceinline__ double2 zero2() { return make_double2(0.0, 0.0); }
+
+__device__ __forceinline__ void cfma(double2 &acc, double2 m, double2 x) {
+    acc.x = fma(m.x, x.x, acc.x);
+    acc.x = fma(-m.y, x.y, acc.x);
+    acc.y = fma(m.x, x.y, acc.y);
+    acc.y = fma(m.y, x.x, acc.y);
+}
+
+// LDS index padding: one slot every 8 complex values breaks the bank
+// conflicts of contiguous fibers (stride-1 mode inside a brick).
+__device__ __forceinline__ int pad(int i) { return i + (i >> 3); }
+
+// ---------------------------------------------------------------------------
+// Group pass (see GroupTask).  LDS: [op matrices][buf0][buf1], each buffer
+// holding one padded brick; chain kind ping-pongs buf0/buf1, sum kind keeps
+// the input brick in buf0 and accumulates into buf1.
+template <int NMAX>
+__global__ void __launch_bounds__(256) k_group_run(const GroupTask *__restrict__ tasks, int capacity) {
+    const GroupTask &t = tasks[blockIdx.y];
+    const int64_t nlb = t.L / t.Lt;
+    const int64_t b = blockIdx.x;
+    if (b >= nlb * t.R) return;
+    const int64_t lb = b % nlb, r = b / nlb;
+    const int64_t l0 = lb * t.Lt;
+    const int Lt = (int)t.Lt;
+    const int64_t L = t.L;
+    extern __shared__ double2 lds[];
+    double2 *mats = lds;
+    double2 *buf0 = lds + kMaxOpsPerTask * NMAX * NMAX;
+    double2 *buf1 = buf0 + capacity;
+    double2 *buf2 = buf1;
+    const int tid = threadIdx.x, nth = blockDim.x;
+    int cur[kMaxGroupModes];
+    int Min = 1, Mout = 1;
+    for (int q = 0; q < t.ng; q++) {
+        cur[q] = t.nin[q];
+        Min *= t.nin[q];
+        Mout *= t.nout[q];
+    }
+    for (int q = 0; q < t.nops; q++) {
+        for (int e = tid; e < NMAX * NMAX; e += nth) {
+            int i = e % NMAX, j = e / NMAX;
+            const double2 *M = t.opmat[q];
+            double2 v = zero2();
+            if (i < t.opnout[q] && j < t.opnin[q]) v = t.opadj[q] ? cconj(M[j + t.ld * i]) : M[i + t.ld * j];
+            mats[q * NMAX * NMAX + i + NMAX * j] = v;
+        }
+    }
+    // load the X brick
+    const double2 *X = t.X;
+    const int nin_tot = Lt * Min;
+    for (int idx = tid; idx < nin_tot; idx += nth) {
+        int lt = idx % Lt, g = idx / Lt;
+        buf0[pad(idx)] = X[l0 + lt + L * ((int64_t)g + (int64_t)Min * r)];
+    }
+    __syncthreads();
+    double2 *src = buf0, *dst = buf1;
+    if (t.kind == 0) {
+        int tot = nin_tot;
+        for (int q = 0; q < t.nops; q++) {
+            const int p = t.opmode[q];
+            int stride = Lt;
+            for (int pp = 0; pp < p; pp++) stride *= cur[pp];
+            const int ni = cur[p], no = t.nout[p];
+            const int nfib = tot / ni;
+            const double2 *Mq = mats + q * NMAX * NMAX;
+            for (int f = tid; f < nfib; f += nth) {
+                const int inner = f % stride, outer = f / stride;
+                const int bi = inner + outer * stride * ni;
+                const int bo = inner + outer * stride * no;
+                double2 x[NMAX];
+#pragma unroll
+                for (int j = 0; j < NMAX; j++) x[j] = (j < ni) ? src[pad(bi + j * stride)] : zero2();
+#pragma unroll
+                for (int i = 0; i < NMAX; i++) {
+                    if (i < no) {
+                        double2 acc = zero2();
+#pragma unroll
+                        for (int j = 0; j < NMAX; j++)
+                            if (j < ni) cfma(acc, Mq[i + NMAX * j], x[j]);
+                        dst[pad(bo + i * stride)] = acc;
+                    }
+                }
+            }
+            tot = tot / ni * no;
+            cur[p] = no;
+            __syncthreads();
+            double2 *sw = src; src = dst; dst = sw;
+        }
+    } else {
+        // sum of square mode products into buf2 (accumulator)
+        for (int idx = tid; idx < nin_tot; idx += nth) buf2[pad(idx)] = zero2();
+        __syncthreads();
+        for (int q = 0; q < t.nops; q++) {
+            const int p = t.opmode[q];
+            int stride = Lt;
+            for (int pp = 0; pp < p; pp++) stride *= cur[pp];
+            const int n = cur[p];
+            const int nfib = nin_tot / n;
+            const double2 *Mq = mats + q * NMAX * NMAX;
+            for (int f = tid; f < nfib; f += nth) {
+                const int inner = f % stride, outer = f / stride;
+                const int bi = inner + outer * stride * n;
+                double2 x[NMAX];
+#pragma unroll
+                for (int j = 0; j < NMAX; j++) x[j] = (j < n) ? src[pad(bi + j * stride)] : zero2();
+#pragma unroll
+                for (int i = 0; i < NMAX; i++) {
+                    if (i < n) {
+                        double2 acc = buf2[pad(bi + i * stride)];
+#pragma unroll
+                        for (int j = 0; j < NMAX; j++)
+                            if (j < n) cfma(acc, Mq[i + NMAX * j], x[j]);
+                        buf2[pad(bi + i * stride)] = acc;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+        src = buf2;
+    }
+    // write Y
+    double2 *Y = t.Y;
+    const int nout_tot = Lt * Mout;
+    const double2 alpha = t.alpha;
+    for (int idx = tid; idx < nout_tot; idx += nth) {
+        int lt = idx % Lt, g = idx / Lt;
+        int64_t a = l0 + lt + L * ((int64_t)g + (int64_t)Mout * r);
+        double2 v = cmul(alpha, src[pad(idx)]);
+        if (t.accumulate) v = cadd(Y[a], v);
+        Y[a] = v;
+    }
+}
+
+template <int NMAX>
+static void launch_group_nmax(const GroupTask *d_tasks, int ntasks, int64_t max_blocks, int capacity,
+                              hipStream_t s) {
+    static bool attr_set = false;
+    const size_t shm = sizeof(double2) * (kMaxOpsPerTask * NMAX * NMAX + 2 * (size_t)capacity);
+    if (!attr_set) {
+        (void)hipFuncSetAttribute((const void *)k_group_run<NMAX>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  160 * 1024);
+        attr_set = true;
+    }
+    hipLaunchKernelGGL(k_group_run<NMAX>, dim3((unsigned)max_blocks, (unsigned)ntasks), dim3(256), shm, s,
+                       d_tasks, capacity);
+}
+
+int group_capacity(int brick) { return brick + (brick >> 3) + 1; }
+
+void launch_group_tasks(const GroupTask *d_tasks, int ntasks, int64_t max_blocks_per_task, int nmax,
+                        int brick, hipStream_t s) {
+    if (ntasks <= 0 || max_blocks_per_task <= 0) return;
+    const int capacity = group_capacity(brick);
+    if (nmax <= 4) launch_group_nmax<4>(d_tasks, ntasks, max_blocks_per_task, capacity, s);
+    else if (nmax <= 8) launch_group_nmax<8>(d_tasks, ntasks, max_blocks_per_task, capacity, s);
+    else launch_group_nmax<16>(d_tasks, ntasks, max_blocks_per_task, capacity, s);
+}
+
+// ---------------------------------------------------------------------------
+// Single-mode product straight from global memory (one thread per fiber).
+template <int NMAX>
+__global__ void __launch_bounds__(256) k_mode_product(ModeTask t) {
+    __shared__ double2 M[NMAX * NMAX];
+    for (int e = threadIdx.x; e < NMAX * NMAX; e += blockDim.x) {
+        int i = e % NMAX, j = e / NMAX;
+        double2 v = zero2();
+        if (i < t.nout && j < t.nin)
+            v = t.adj ? cconj(t.M[j + t.ld * i]) : t.M[i + t.ld * j];
+        M[e] = v;
+    }
+    __syncthreads();
+    const int64_t nfib = t.ll * t.nl;
+    for (int64_t f = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; f < nfib;
+         f += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t l = f % t.ll, r = f / t.ll;
+        const double2 *x0 = t.X + l + t.ll * (int64_t)t.nin * r;
+        double2 *y0 = t.Y + l + t.ll * (int64_t)t.nout * r;
+        double2 x[NMAX];
+#pragma unroll
+        for (int j = 0; j < NMAX; j++) x[j] = (j < t.nin) ? x0[t.ll * j] : zero2();
+#pragma unroll
+        for (int i = 0; i < NMAX; i++) {
+            if (i < t.nout) {
+                double2 acc = zero2();
+#pragma unroll
+                for (int j = 0; j < NMAX; j++)
+                    if (j < t.nin) cfma(acc, M[i + NMAX * j], x[j]);
+                y0[t.ll * i] = acc;
+            }
+        }
+    }
+}
+
+void launch_mode_product(const ModeTask &t, int nmax, hipStream_t s) {
+    const int64_t nfib = t.ll * t.nl;
+    int64_t blocks = (nfib + 255) / 256;
+    if (blocks > 65536) blocks = 65536;
+    if (blocks < 1) blocks = 1;
+    if (nmax <= 4) hipLaunchKernelGGL(k_mode_product<4>, dim3((unsigned)blocks), dim3(256), 0, s, t);
+    else if (nmax <= 8) hipLaunchKernelGGL(k_mode_product<8>, dim3((unsigned)blocks), dim3(256), 0, s, t);
+    else hipLaunchKernelGGL(k_mode_product<16>, dim3((unsigned)blocks), dim3(256), 0, s, t);
+}
+
+// ---------------------------------------------------------------------------
+// Hole contraction, stage 1: per-block partial n1 x n2 sums (chunked to 8x8).
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+__global__ void __launch_bounds__(256) k_gram_partial(GramTask t, int j1o, int n1c, int j2o, int n2c,
+                                                      double2 *partials) {
+    double2 acc[8][8];
+#pragma unroll
+    for (int a = 0; a < 8; a++)
+#pragma unroll
+        for (int b = 0; b < 8; b++) acc[a][b] = zero2();
+    const int64_t npos = t.ll * t.nl;
+    for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < npos;
+         p += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t l = p % t.ll, r = p / t.ll;
+        double2 x[8], y[8];
+        const double2 *xb = t.X + l + t.ll * ((int64_t)t.n1 * r + j1o);
+        const double2 *yb = t.Y + l + t.ll * ((int64_t)t.n2 * r + j2o);
+#pragma unroll
+        for (int a = 0; a < 8; a++) x[a] = (a < n1c) ? xb[t.ll * a] : zero2();
+#pragma unroll
+        for (int b = 0; b < 8; b++) y[b] = (b < n2c) ? yb[t.ll * b] : zero2();
+#pragma unroll
+        for (int a = 0; a < 8; a++)
+#pragma unroll
+            for (int b = 0; b < 8; b++) {
+                acc[a][b].x = fma(x[a].x, y[b].x, acc[a][b].x);
+                acc[a][b].x = fma(x[a].y, y[b].y, acc[a][b].x);
+                acc[a][b].y = fma(x[a].x, y[b].y, acc[a][b].y);
+                acc[a][b].y = fma(-x[a].y, y[b].x, acc[a][b].y);
+            }
+    }
+    __shared__ double2 red[4][64];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int a = 0; a < 8; a++)
+#pragma unroll
+        for (int b = 0; b < 8; b++) {
+            double sx = wave_sum(acc[a][b].x), sy = wave_sum(acc[a][b].y);
+            if (lane == 0) red[wave][a + 8 * b] = make_double2(sx, sy);
+        }
+    __syncthreads();
+    if (threadIdx.x < 64) {
+        double2 v = zero2();
+        for (int w = 0; w < (int)(blockDim.x >> 6); w++) v = cadd(v, red[w][threadIdx.x]);
+        partials[(int64_t)blockIdx.x * 64 + threadIdx.x] = v;
+    }
+}
+
+__global__ void k_gram_final(GramTask t, int j1o, int n1c, int j2o, int n2c, const double2 *partials,
+                             int nblocks) {
+    const int e = threadIdx.x;
+    if (e >= 64) return;
+    const int a = e % 8, b = e / 8;
+    if (a >= n1c || b >= n2c) return;
+    double2 v = zero2();
+    for (int k = 0; k < nblocks; k++) v = cadd(v, partials[(int64_t)k * 64 + e]);
+    v = cmul(t.alpha, v);
+    const int j1 = j1o + a, j2 = j2o + b;
+    double2 *g = t.G + j1 + (int64_t)t.ld * j2;
+    if (t.accumulate) v = cadd(*g, v);
+    *g = v;
+}
+
+__global__ void k_gram_hermitize(double2 *G, int n, int ld) {
+    const int e = threadIdx.x;
+    const bool in = e < n * n;
+    const int a = in ? e % n : 0, b = in ? e / n : 0;
+    if (in && a == b) G[a + ld * a].y = 0.0;
+    __syncthreads();
+    if (in && a > b) G[a + ld * b] = cconj(G[b + ld * a]);
+}
+
+void launch_gram(const GramTask &t, double2 *partials, int nblocks, int nmax, hipStream_t s) {
+    (void)nmax;
+    for (int j1o = 0; j1o < t.n1; j1o += 8) {
+        const int n1c = (t.n1 - j1o) < 8 ? (t.n1 - j1o) : 8;
+        for (int j2o = 0; j2o < t.n2; j2o += 8) {
+            const int n2c = (t.n2 - j2o) < 8 ? (t.n2 - j2o) : 8;
+            hipLaunchKernelGGL(k_gram_partial, dim3(nblocks), dim3(256), 0, s, t, j1o, n1c, j2o, n2c,
+                               partials);
+            hipLaunchKernelGGL(k_gram_final, dim3(1), dim3(64), 0, s, t, j1o, n1c, j2o, n2c, partials,
+                               nblocks);
+        }
+    }
+    if (t.hermitian) hipLaunchKernelGGL(k_gram_hermitize, dim3(1), dim3(256), 0, s, t.G, t.n1, t.ld);
+}
+
+// ---------------------------------------------------------------------------
+// SPF block products (Calc_h1mode DVR branch, hamilton.f90:433-485, and the
+// SPF overlap matrices of norm_wf / WF_Ovl).
+__device__ void apply_prim_block(const DofTables &dt, int mode, int k, int code, int nS, int nG,
+                                 double2 *blk, double2 *tmp) {
+    const int dof = dt.mode_dofs[mode * kMaxDofPerMode + k];
+    int ll = 1, nl = 1;
+    for (int q = 0; q < k; q++) ll *= dt.ngp[dt.mode_dofs[mode * kMaxDofPerMode + q]];
+    for (int q = k + 1; q < dt.mode_ndof[mode]; q++) nl *= dt.ngp[dt.mode_dofs[mode * kMaxDofPerMode + q]];
+    const int ng = dt.ngp[dof];
+    const int total = nG * nS;
+    const int tid = threadIdx.x, nth = blockDim.x;
+    if (code >= 1) {
+        const double *x = dt.grid + (int64_t)dt.ngpmax * dof;
+        for (int e = tid; e < total; e += nth) {
+            const int g = (e / ll) % ng;
+            double v = x[g];
+            for (int q = 1; q < code; q++) v *= x[g];
+            blk[e] = cscale(v, blk[e]);
+        }
+        __syncthreads();
+        return;
+    }
+    const double *Mat = (code == -1 ? dt.d1 : (code == -2 ? dt.d2 : dt.xd1)) +
+                        (int64_t)dt.ngpmax * dt.ngpmax * dof;
+    const bool imag_fac = (code == -1 || code == -3);
+    (void)nl;
+    // tmp(l, g2, rest) = sum_g blk(l, g, rest) * Mat(g, g2)
+    for (int e = tid; e < total; e += nth) {
+        const int l = e % ll;
+        const int g2 = (e / ll) % ng;
+        const int rest = e / (ll * ng);
+        const double2 *c = blk + (int64_t)rest * ng * ll + l;
+        double2 acc = zero2();
+        for (int g = 0; g < ng; g++) {
+            const double m = Mat[g + (int64_t)dt.ngpmax * g2];
+            acc.x = fma(m, c[g * ll].x, acc.x);
+            acc.y = fma(m, c[g * ll].y, acc.y);
+        }
+        tmp[e] = imag_fac ? make_double2(-acc.y, acc.x) : acc;
+    }
+    __syncthreads();
+    for (int e = tid; e < total; e += nth) blk[e] = tmp[e];
+    __syncthreads();
+}
+
+__global__ void __launch_bounds__(256) k_spf_products(const SpfTask *__restrict__ tasks, DofTables dt) {
+    const SpfTask t = tasks[blockIdx.x];
+    const int tid = threadIdx.x, nth = blockDim.x;
+    double2 *sect = t.scratch, *tmp = t.scratch + (int64_t)t.nG * t.nb;
+    const int total = t.nG * t.nb;
+    for (int e = tid; e < total; e += nth) sect[e] = t.phiB[e];
+    __syncthreads();
+    for (int k = 0; k < t.ncode; k++)
+        if (t.codes[k] != 0) apply_prim_block(dt, t.mode, k, t.codes[k], t.nb, t.nG, sect, tmp);
+    for (int e = tid; e < t.na * t.nb; e += nth) {
+        const int a = e % t.na, b = e / t.na;
+        if (t.sym && a > b) continue;
+        double2 x = zero2(), y = zero2();
+        for (int g = 0; g < t.nG; g++) {
+            x = cadd(x, cconjmul(t.phiA[g + t.nG * a], sect[g + t.nG * b]));
+            if (t.sym) y = cadd(y, cconjmul(sect[g + t.nG * a], t.phiB[g + t.nG * b]));
+        }
+        if (t.sym) {
+            double2 v = make_double2(0.5 * x.x + 0.5 * y.x, 0.5 * x.y + 0.5 * y.y);
+            if (a == b) v.y = 0.0;
+            t.out[a + t.ld * b] = v;
+            t.out[b + t.ld * a] = cconj(v);
+        } else {
+            t.out[a + t.ld * b] = x;
+        }
+    }
+}
+
+void launch_spf_tasks(const SpfTask *d_tasks, int ntasks, DofTables dt, int max_ng, hipStream_t s) {
+    (void)max_ng;
+    if (ntasks <= 0) return;
+    hipLaunchKernelGGL(k_spf_products, dim3(ntasks), dim3(256), 0, s, d_tasks, dt);
+}
+
+// ---------------------------------------------------------------------------
+// Vector kernels (Lanczos, cmf2_integration.f90:30-103).
+__global__ void k_fill(double2 *p, int64_t n, double2 v) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+void launch_fill(double2 *p, int64_t n, double2 v, hipStream_t s) {
+    int64_t blocks = (n + 255) / 256;
+    if (blocks > 8192) blocks = 8192;
+    if (blocks < 1) blocks = 1;
+    hipLaunchKernelGGL(k_fill, dim3((unsigned)blocks), dim3(256), 0, s, p, n, v);
+}
+
+__global__ void k_axpby(double2 *y, const double2 *x, double a, double b, int64_t n) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        double2 xv = x[i], yv = y[i];
+        y[i] = make_double2(a * xv.x + b * yv.x, a * xv.y + b * yv.y);
+    }
+}
+
+void launch_axpby(double2 *y, const double2 *x, double a, double b, int64_t n, hipStream_t s) {
+    int64_t blocks = (n + 255) / 256;
+    if (blocks > 8192) blocks = 8192;
+    if (blocks < 1) blocks = 1;
+    hipLaunchKernelGGL(k_axpby, dim3((unsigned)blocks), dim3(256), 0, s, y, x, a, b, n);
+}
+
+__global__ void __launch_bounds__(256) k_dot_partial(const double2 *x, const double2 *y, int64_t n, double *partials) {
+    double sx = 0.0, sy = 0.0;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        double2 a = x[i], b = y[i];
+        sx = fma(a.x, b.x, sx);
+        sx = fma(a.y, b.y, sx);
+        sy = fma(a.x, b.y, sy);
+        sy = fma(-a.y, b.x, sy);
+    }
+    __shared__ double red[2][4];
+    sx = wave_sum(sx);
+    sy = wave_sum(sy);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0) { red[0][wave] = sx; red[1][wave] = sy; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double ax = 0.0, ay = 0.0;
+        for (int w = 0; w < (int)(blockDim.x >> 6); w++) { ax += red[0][w]; ay += red[1][w]; }
+        partials[2 * blockIdx.x] = ax;
+        partials[2 * blockIdx.x + 1] = ay;
+    }
+}
+
+__global__ void k_dot_final(const double *partials, int nblocks, double2 *out) {
+    if (threadIdx.x != 0) return;
+    double ax = 0.0, ay = 0.0;
+    for (int k = 0; k < nblocks; k++) { ax += partials[2 * k]; ay += partials[2 * k + 1]; }
+    *out = make_double2(ax, ay);
+}
+
+void launch_dot(const double2 *x, const double2 *y, int64_t n, double *partials, int nblocks, double2 *out,
+                hipStream_t s) {
+    hipLaunchKernelGGL(k_dot_partial, dim3(nblocks), dim3(256), 0, s, x, y, n, partials);
+    hipLaunchKernelGGL(k_dot_final, dim3(1), dim3(64), 0, s, partials, nblocks, out);
+}
+
+__global__ void __launch_bounds__(256) k_norm2_partial(const double2 *x, int64_t n, double *partials) {
+    double acc = 0.0;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        double2 a = x[i];
+        acc = fma(a.x, a.x, acc);
+        acc = fma(a.y, a.y, acc);
+    }
+    __shared__ double red[4];
+    acc = wave_sum(acc);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double a = 0.0;
+        for (int w = 0; w < (int)(blockDim.x >> 6); w++) a += red[w];
+        partials[blockIdx.x] = a;
+    }
+}
+
+__global__ void k_norm2_final(const double *partials, int nblocks, double *out) {
+    if (threadIdx.x != 0) return;
+    double a = 0.0;
+    for (int k = 0; k < nblocks; k++) a += partials[k];
+    *out = a;
+}
+
+void launch_norm2(const double2 *x, int64_t n, double *partials, int nblocks, double *out, hipStream_t s) {
+    hipLaunchKernelGGL(k_norm2_partial, dim3(nblocks), dim3(256), 0, s, x, n, partials);
+    hipLaunchKernelGGL(k_norm2_final, dim3(1), dim3(64), 0, s, partials, nblocks, out);
+}
+
+// w <- (w - beta_prev * vprev) - alpha * vj, with alpha = Re(<vj|w>) read from
+// the device; accumulates |w|^2 partials.
+__global__ void __launch_bounds__(256) k_lanczos_update(double2 *w, const double2 *vj, const double2 *vprev,
+                                                        const double *alpha_re, double beta_prev, int64_t n,
+                                                        double *partials) {
+    const double alpha = alpha_re[0];
+    double acc = 0.0;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        double2 a = w[i];
+        if (vprev) {
+            double2 p = vprev[i];
+            a.x = a.x - beta_prev * p.x;
+            a.y = a.y - beta_prev * p.y;
+        }
+        double2 v = vj[i];
+        a.x = a.x - alpha * v.x;
+        a.y = a.y - alpha * v.y;
+        w[i] = a;
+        acc = fma(a.x, a.x, acc);
+        acc = fma(a.y, a.y, acc);
+    }
+    __shared__ double red[4];
+    acc = wave_sum(acc);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double a = 0.0;
+        for (int q = 0; q < (int)(blockDim.x >> 6); q++) a += red[q];
+        partials[blockIdx.x] = a;
+    }
+}
+
+void launch_lanczos_update(double2 *w, const double2 *vj, const double2 *vprev, const double *alpha,
+                           double beta_prev, int64_t n, double *partials, int nblocks, double *norm2,
+                           hipStream_t s) {
+    hipLaunchKernelGGL(k_lanczos_update, dim3(nblocks), dim3(256), 0, s, w, vj, vprev, alpha, beta_prev, n,
+                       partials);
+    hipLaunchKernelGGL(k_norm2_final, dim3(1), dim3(64), 0, s, partials, nblocks, norm2);
+}
+
+__global__ void k_scale(double2 *x, const double *norm2, int64_t n) {
+    const double d = sqrt(norm2[0]);
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        double2 a = x[i];
+        x[i] = make_double2(a.x / d, a.y / d);
+    }
+}
+
+void launch_scale(double2 *x, const double *norm2, int64_t n, hipStream_t s) {
+    int64_t blocks = (n + 255) / 256;
+    if (blocks > 8192) blocks = 8192;
+    if (blocks < 1) blocks = 1;
+    hipLaunchKernelGGL(k_scale, dim3((unsigned)blocks), dim3(256), 0, s, x, norm2, n);
+}
+
+__global__ void k_combine(double2 *out, const double2 *basis, int64_t stride, const double2 *coef, int k, int64_t n) {
+    __shared__ double2 c[kNKrylovMax];
+    if (threadIdx.x < k) c[threadIdx.x] = coef[threadIdx.x];
+    __syncthreads();
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        double2 acc = zero2();
+        for (int j = 0; j < k; j++) cfma(acc, c[j], basis[j * stride + i]);
+        out[i] = acc;
+    }
+}
+
+void launch_combine(double2 *out, const double2 *basis, int64_t stride, const double2 *coef, int k, int64_t n,
+                    hipStream_t s) {
+    int64_t blocks = (n + 255) / 256;
+    if (blocks > 8192) blocks = 8192;
+    if (blocks < 1) blocks = 1;
+    hipLaunchKernelGGL(k_combine, dim3((unsigned)blocks), dim3(256), 0, s, out, basis, stride, coef, k, n);
+}
+
+// hsum[t][e] = sum_{c in row t} coef[c] * (src[c] >= 0 ? h1[src[c]][e] : I[e])
+__global__ void k_hsum(double2 *hsum, const double2 *h1, const int *rowptr, const int *src, const double2 *coef,
+                       int nmat, int nmax) {
+    const int nn = nmax * nmax;
+    const int64_t id = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (id >= (int64_t)nmat * nn) return;
+    const int t = (int)(id / nn), e = (int)(id % nn);
+    double2 acc = zero2();
+    for (int c = rowptr[t]; c < rowptr[t + 1]; c++) {
+        double2 v;
+        if (src[c] >= 0) v = h1[(int64_t)src[c] * nn + e];
+        else v = make_double2((e % nmax) == (e / nmax) ? 1.0 : 0.0, 0.0);
+        acc = cadd(acc, cmul(coef[c], v));
+    }
+    hsum[id] = acc;
+}
+
+void launch_hsum(double2 *hsum, const double2 *h1, const int *rowptr, const int *src, const double2 *coef,
+                 int nmat, int nmax, hipStream_t s) {
+    if (nmat <= 0) return;
+    const int64_t tot = (int64_t)nmat * nmax * nmax;
+    hipLaunchKernelGGL(k_hsum, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, hsum, h1, rowptr, src, coef,
+                       nmat, nmax);
+}
+
+__global__ void k_mat_axpy(double2 *dst, const double2 *src, double2 alpha, int n, int ld) {
+    const int e = threadIdx.x;
+    if (e >= n * n) return;
+    const int a = e % n, b = e / n;
+    dst[a + ld * b] = cadd(dst[a + ld * b], cmul(alpha, src[a + ld * b]));
+}
+
+void launch_mat_axpy(double2 *dst, const double2 *src, double2 alpha, int n, int ld, hipStream_t s) {
+    hipLaunchKernelGGL(k_mat_axpy, dim3(1), dim3(256), 0, s, dst, src, alpha, n, ld);
+}
+
+// ---------------------------------------------------------------------------
+// Small dense FP64 linear algebra in one thread (n <= 16).
+constexpr int kSmall = 16;
+
+// Cyclic complex Jacobi: A (n x n, ld kSmall, Hermitian) -> eigenvalues w,
+// eigenvectors U (columns).  Same rotation as the oracle's herm_eig.
+__device__ void dev_herm_eig(int n, double2 *A, double *w, double2 *U) {
+    for (int i = 0; i < n; i++)
+        for (int j = 0; j < n; j++) U[i + kSmall * j] = make_double2(i == j ? 1.0 : 0.0, 0.0);
+    for (int sweep = 0; sweep < 100; sweep++) {
+        double off = 0.0, diag = 0.0;
+        for (int j = 0; j < n; j++)
+            for (int i = 0; i < n; i++) {
+                double2 a = A[i + kSmall * j];
+                double a2 = a.x * a.x + a.y * a.y;
+                if (i != j) off += a2; else diag += a2;
+            }
+        if (off <= 1e-32 * (diag + 1e-300) || off == 0.0) break;
+        for (int p = 0; p < n - 1; p++)
+            for (int q = p + 1; q < n; q++) {
+                double2 apq = A[p + kSmall * q];
+                double aabs = hypot(apq.x, apq.y);
+                if (aabs == 0.0) continue;
+                double2 ph = make_double2(apq.x / aabs, apq.y / aabs);
+                double app = A[p + kSmall * p].x, aqq = A[q + kSmall * q].x;
+                double theta = (aqq - app) / (2.0 * aabs);
+                double tt = (theta >= 0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
+                double c = 1.0 / sqrt(tt * tt + 1.0), s = tt * c;
+                double2 jpp = make_double2(c, 0.0), jqq = cscale(c, cconj(ph)), jpq = make_double2(s, 0.0),
+                        jqp = cscale(-s, cconj(ph));
+                for (int k = 0; k < n; k++) {
+                    double2 akp = A[k + kSmall * p], akq = A[k + kSmall * q];
+                    A[k + kSmall * p] = cadd(cmul(akp, jpp), cmul(akq, jqp));
+                    A[k + kSmall * q] = cadd(cmul(akp, jpq), cmul(akq, jqq));
+                }
+                for (int k = 0; k < n; k++) {
+                    double2 apk = A[p + kSmall * k], aqk = A[q + kSmall * k];
+                    A[p + kSmall * k] = cadd(cmul(cconj(jpp), apk), cmul(cconj(jqp), aqk));
+                    A[q + kSmall * k] = cadd(cmul(cconj(jpq), apk), cmul(cconj(jqq), aqk));
+                }
+                A[p + kSmall * q] = zero2();
+                A[q + kSmall * p] = zero2();
+                A[p + kSmall * p].y = 0.0;
+                A[q + kSmall * q].y = 0.0;
+                for (int k = 0; k < n; k++) {
+                    double2 ukp = U[k + kSmall * p], ukq = U[k + kSmall * q];
+                    U[k + kSmall * p] = cadd(cmul(ukp, jpp), cmul(ukq, jqp));
+                    U[k + kSmall * q] = cadd(cmul(ukp, jpq), cmul(ukq, jqq));
+                }
+            }
+    }
+    for (int i = 0; i < n; i++) w[i] = A[i + kSmall * i].x;
+}
+
+// Calc_rhom1MF (gmctdh_objects.f90:1147-1194): per job (mode, set) the
+// 'poly'-regularised inverse of rho (maths.f90:8-83) times each mean field.
+// job layout: [mode, rho_index, n, nops, (op, n2) x nops], stride 36.
+__global__ void k_rmf(const double2 *rho, const double2 *mf, double2 *rmf, const int *jobs, int njobs,
+                      int nspfmax, int nopmax, double eps) {
+    const int jb = blockIdx.x * blockDim.x + threadIdx.x;
+    if (jb >= njobs) return;
+    const int *J = jobs + 36 * jb;
+    const int m = J[0], ri = J[1], n = J[2], nops = J[3];
+    double2 A[kSmall * kSmall], U[kSmall * kSmall];
+    double w[kSmall];
+    const int nn = nspfmax * nspfmax;
+    const double2 *r = rho + (int64_t)ri * nn;
+    for (int j = 0; j < n; j++)
+        for (int i = 0; i < n; i++) A[i + kSmall * j] = r[i + nspfmax * j];
+    dev_herm_eig(n, A, w, U);
+    for (int j = 0; j < n; j++) {
+        double v = w[j];
+        if (v < eps) v = 0.5 * (eps + v * v / eps);
+        w[j] = 1.0 / v;
+    }
+    // rinv (Hermitian, upper then mirrored) into A
+    for (int j2 = 0; j2 < n; j2++)
+        for (int j1 = 0; j1 <= j2; j1++) {
+            double2 acc = zero2();
+            for (int k = 0; k < n; k++) acc = cadd(acc, cscale(w[k], cmul(U[j1 + kSmall * k], cconj(U[j2 + kSmall * k]))));
+            if (j1 == j2) acc.y = 0.0;
+            A[j1 + kSmall * j2] = acc;
+            A[j2 + kSmall * j1] = cconj(acc);
+        }
+    for (int q = 0; q < nops; q++) {
+        const int op = J[4 + 2 * q], n2 = J[5 + 2 * q];
+        const double2 *src = mf + (int64_t)nn * (op + (int64_t)nopmax * m);
+        double2 *dst = rmf + (int64_t)nn * (op + (int64_t)nopmax * m);
+        for (int b = 0; b < n2; b++)
+            for (int a = 0; a < n; a++) {
+                double2 acc = zero2();
+                for (int k = 0; k < n; k++) acc = cadd(acc, cmul(A[a + kSmall * k], src[k + nspfmax * b]));
+                dst[a + nspfmax * b] = acc;
+            }
+    }
+}
+
+void launch_rmf(const double2 *rho, const double2 *mf, double2 *rmf, const int *jobs, int njobs, int nspfmax,
+                int nopmax, double eps, hipStream_t s) {
+    if (njobs <= 0) return;
+    hipLaunchKernelGGL(k_rmf, dim3((njobs + 63) / 64), dim3(64), 0, s, rho, mf, rmf, jobs, njobs, nspfmax, nopmax,
+                       eps);
+}
+
+// ---------------------------------------------------------------------------
+// SPF propagation: one workgroup per logical mode.
+struct SpfScratch {
+    double2 *y0, *k[7], *tmp, *perr, *sect, *sect2, *ovl, *od;
+};
+
+__device__ SpfScratch carve(double2 *base, const SpfModeDesc &d, int nspfmax) {
+    SpfScratch s;
+    double2 *p = base + d.scr_off;
+    s.y0 = p; p += d.blk;
+    for (int i = 0; i < 7; i++) { s.k[i] = p; p += d.blk; }
+    s.tmp = p; p += d.blk;
+    s.perr = p; p += d.blk;
+    s.sect = p; p += d.nG * nspfmax;
+    s.sect2 = p; p += d.nG * nspfmax;
+    s.ovl = p; p += nspfmax * nspfmax;
+    s.od = p;
+    return s;
+}
+
+// Calc_Der_SPF, DVR branch, on mode-local blocks (cmf2_derivatives.f90:118-173).
+__device__ void dev_der_spf(const SpfModeDesc &d, const SpfParams &P, const DofTables &dt, const double2 *in,
+                            double2 *out, SpfScratch &S, int *fail) {
+    const int tid = threadIdx.x, nth = blockDim.x;
+    const int nG = d.nG, nmax = P.nspfmax;
+    int loff[8];
+    {
+        int o = 0;
+        for (int s = 0; s < d.nsets; s++) { loff[s] = o; o += nG * d.nspf[s]; }
+    }
+    for (int e = tid; e < d.blk; e += nth) out[e] = zero2();
+    __syncthreads();
+    for (int q = 0; q < d.nop; q++) {
+        const int s1 = d.op_lhs_set[q], s2 = d.op_rhs_set[q];
+        const int n1 = d.nspf[s1], n2 = d.nspf[s2];
+        for (int e = tid; e < nG * n2; e += nth) S.sect[e] = in[loff[s2] + e];
+        __syncthreads();
+        for (int k = 0; k < dt.mode_ndof[d.mode]; k++)
+            if (d.op_codes[q][k] != 0) apply_prim_block(dt, d.mode, k, d.op_codes[q][k], n2, nG, S.sect, S.sect2);
+        const double2 *r = P.rmf + (int64_t)nmax * nmax * d.op_index[q];
+        double2 *o = out + loff[s1];
+        for (int e = tid; e < nG * n1; e += nth) {
+            const int g = e % nG, j1 = e / nG;
+            double2 acc = zero2();
+            for (int j2 = 0; j2 < n2; j2++) cfma(acc, r[j1 + nmax * j2], S.sect[g + nG * j2]);
+            o[e] = cadd(o[e], cmul(P.der_factor, acc));
+        }
+        __syncthreads();
+    }
+    // projector: d <- d - phi (phi^H phi)^-1 phi^H d
+    for (int s = 0; s < d.nsets; s++) {
+        const int n = d.nspf[s];
+        const double2 *phi = in + loff[s];
+        double2 *dd = out + loff[s];
+        for (int e = tid; e < n * n; e += nth) {
+            const int a = e % n, b = e / n;
+            double2 x = zero2(), y = zero2();
+            for (int g = 0; g < nG; g++) {
+                x = cadd(x, cconjmul(phi[g + nG * a], phi[g + nG * b]));
+                y = cadd(y, cconjmul(phi[g + nG * a], dd[g + nG * b]));
+            }
+            S.ovl[a + n * b] = x;
+            S.od[a + n * b] = y;
+        }
+        __syncthreads();
+        if (tid == 0) {
+            // Cholesky S = U^H U (upper), then solve for each column of od
+            double2 *U = S.ovl;
+            for (int j = 0; j < n; j++) {
+                double dg = U[j + n * j].x;
+                for (int k = 0; k < j; k++) dg -= U[k + n * j].x * U[k + n * j].x + U[k + n * j].y * U[k + n * j].y;
+                if (!(dg > 0.0)) { *fail = 1; dg = 1.0; }
+                const double ujj = sqrt(dg);
+                U[j + n * j] = make_double2(ujj, 0.0);
+                for (int i = j + 1; i < n; i++) {
+                    double2 v = U[j + n * i];
+                    for (int k = 0; k < j; k++) v = csub(v, cconjmul(U[k + n * j], U[k + n * i]));
+                    U[j + n * i] = make_double2(v.x / ujj, v.y / ujj);
+                }
+            }
+            for (int c = 0; c < n; c++) {
+                double2 *y = S.od + n * c;
+                for (int i = 0; i < n; i++) {
+                    double2 v = y[i];
+                    for (int k = 0; k < i; k++) v = csub(v, cconjmul(U[k + n * i], y[k]));
+                    const double u = U[i + n * i].x;
+                    y[i] = make_double2(v.x / u, v.y / u);
+                }
+                for (int i = n - 1; i >= 0; i--) {
+                    double2 v = y[i];
+                    for (int k = i + 1; k < n; k++) v = csub(v, cmul(U[i + n * k], y[k]));
+                    const double u = U[i + n * i].x;
+                    y[i] = make_double2(v.x / u, v.y / u);
+                }
+            }
+        }
+        __syncthreads();
+        for (int e = tid; e < nG * n; e += nth) {
+            const int g = e % nG, b = e / nG;
+            double2 acc = zero2();
+            for (int k = 0; k < n; k++) cfma(acc, S.od[k + n * b], phi[g + nG * k]);
+            dd[e] = csub(dd[e], acc);
+        }
+        __syncthreads();
+    }
+}
+
+// Calc_Err_SPF, DVR branch (cmf2_derivatives.f90:253-291).
+__device__ double dev_err_spf(const SpfModeDesc &d, const SpfParams &P, const double2 *p1, const double2 *p2,
+                              SpfScratch &S) {
+    const int tid = threadIdx.x, nth = blockDim.x;
+    const int nG = d.nG, nmax = P.nspfmax;
+    __shared__ double err_sh;
+    if (tid == 0) err_sh = 0.0;
+    int o = 0;
+    for (int s = 0; s < d.nsets; s++) {
+        const int n = d.nspf[s];
+        for (int e = tid; e < n * n; e += nth) {
+            const int a = e % n, b = e / n;
+            double2 x = zero2();
+            if (a <= b)
+                for (int g = 0; g < nG; g++) {
+                    double2 da = csub(p1[o + g + nG * a], p2[o + g + nG * a]);
+                    double2 db = csub(p1[o + g + nG * b], p2[o + g + nG * b]);
+                    x = cadd(x, cconjmul(da, db));
+                }
+            S.ovl[a + n * b] = x;
+        }
+        __syncthreads();
+        if (tid == 0) {
+            const double2 *r = P.rho + (int64_t)nmax * nmax * d.rho_index[s];
+            double err = err_sh;
+            for (int j2 = 0; j2 < n; j2++) {
+                for (int j1 = 0; j1 < j2; j1++) {
+                    double2 prod = cmul(r[j1 + nmax * j2], cconj(S.ovl[j1 + n * j2]));
+                    err += 2.0 * prod.x;
+                }
+                double2 prod = cmul(r[j2 + nmax * j2], make_double2(S.ovl[j2 + n * j2].x, 0.0));
+                err += prod.x;
+            }
+            err_sh = err;
+        }
+        __syncthreads();
+        o += nG * n;
+    }
+    const double e = sqrt(fabs(err_sh));
+    __syncthreads();
+    return e;
+}
+
+__constant__ double c_ckB[6][5] = {
+    {0, 0, 0, 0, 0}, {0.2, 0, 0, 0, 0}, {0.075, 0.225, 0, 0, 0}, {0.3, -0.9, 1.2, 0, 0},
+    {-11.0 / 54.0, 2.5, -70.0 / 27.0, 35.0 / 27.0, 0},
+    {1631.0 / 55296.0, 175.0 / 512.0, 575.0 / 13824.0, 44275.0 / 110592.0, 253.0 / 4096.0}};
+__constant__ double c_ckC[6] = {37.0 / 378.0, 0, 250.0 / 621.0, 125.0 / 594.0, 0, 512.0 / 1771.0};
+__constant__ double c_ckD[6] = {2825.0 / 27648.0, 0, 18575.0 / 48384.0, 13525.0 / 55296.0, 277.0 / 14336.0, 0.25};
+__constant__ double c_dpB[6][5] = {
+    {0, 0, 0, 0, 0}, {0.2, 0, 0, 0, 0}, {0.075, 0.225, 0, 0, 0}, {44.0 / 45.0, -56.0 / 15.0, 32.0 / 9.0, 0, 0},
+    {19372.0 / 6561.0, -25360.0 / 2187.0, 64448.0 / 6561.0, -212.0 / 729.0, 0},
+    {9017.0 / 3168.0, -355.0 / 33.0, 46732.0 / 5247.0, 49.0 / 176.0, -5103.0 / 18656.0}};
+__constant__ double c_dpC[6] = {35.0 / 384.0, 0, 500.0 / 1113.0, 125.0 / 192.0, -2187.0 / 6784.0, 11.0 / 84.0};
+__constant__ double c_dpD[7] = {5179.0 / 57600.0, 0, 7571.0 / 16695.0, 393.0 / 640.0, -92097.0 / 339200.0,
+                                187.0 / 2100.0, 0.025};
+
+// propagate_SPF over P.dt (cmf2_base.f90:210-260) with rk_SPF_ck / rk_SPF_dp
+// (cmf2_integration.f90:107-412).  status: 1 = Cholesky failure, 2 = too many
+// attempts.
+__global__ void __launch_bounds__(256) k_spf_propagate(const SpfModeDesc *__restrict__ modes, double2 *psi,
+                                                       double2 *scratch, SpfParams P, DofTables dt, int *status) {
+    const SpfModeDesc d = modes[blockIdx.x];
+    const int tid = threadIdx.x, nth = blockDim.x;
+    SpfScratch S = carve(scratch, d, P.nspfmax);
+    __shared__ int fail;
+    if (tid == 0) fail = 0;
+    // gather the mode's set blocks
+    {
+        int o = 0;
+        for (int s = 0; s < d.nsets; s++) {
+            const int n = d.nG * d.nspf[s];
+            for (int e = tid; e < n; e += nth) S.y0[o + e] = psi[d.off[s] + e];
+            o += n;
+        }
+    }
+    __syncthreads();
+    const double safety = 0.9, pgrow = -0.2, pshrnk = -0.25, minscal = 0.1, maxscal = 5.0;
+    const double errcon = pow(5.0 / safety, 1.0 / pgrow);
+    const bool dp = P.dormand_prince != 0;
+    double dt_done = 0.0, stepsize = P.dt;
+    int attempts = 0;
+    for (;;) {
+        // ---- one adaptive RK step (rk_SPF_ck / rk_SPF_dp) ----
+        double h = stepsize, error = 0.0;
+        for (;;) {
+            dev_der_spf(d, P, dt, S.y0, S.k[0], S, &fail);
+            for (int st = 1; st < 6; st++) {
+                for (int e = tid; e < d.blk; e += nth) {
+                    double2 v;
+                    if (st == 1) {
+                        const double c = (dp ? c_dpB[1][0] : c_ckB[1][0]) * h;
+                        v = cadd(S.y0[e], cscale(c, S.k[0][e]));
+                    } else {
+                        double2 acc = zero2();
+                        for (int q = 0; q < st; q++) {
+                            const double b = dp ? c_dpB[st][q] : c_ckB[st][q];
+                            acc = cadd(acc, cscale(b, S.k[q][e]));
+                        }
+                        v = cadd(S.y0[e], cscale(h, acc));
+                    }
+                    S.tmp[e] = v;
+                }
+                __syncthreads();
+                dev_der_spf(d, P, dt, S.tmp, S.k[st], S, &fail);
+            }
+            for (int e = tid; e < d.blk; e += nth) {
+                double2 acc = zero2();
+                for (int q = 0; q < 6; q++) {
+                    const double c = dp ? c_dpC[q] : c_ckC[q];
+                    if (c != 0.0) acc = cadd(acc, cscale(c, S.k[q][e]));
+                }
+                S.tmp[e] = cadd(S.y0[e], cscale(h, acc));
+                if (!dp) {
+                    double2 ae = zero2();
+                    for (int q = 0; q < 6; q++)
+                        if (c_ckD[q] != 0.0) ae = cadd(ae, cscale(c_ckD[q], S.k[q][e]));
+                    S.perr[e] = cadd(S.y0[e], cscale(h, ae));
+                }
+            }
+            __syncthreads();
+            if (dp) {
+                dev_der_spf(d, P, dt, S.tmp, S.k[6], S, &fail);
+                for (int e = tid; e < d.blk; e += nth) {
+                    double2 ae = zero2();
+                    for (int q = 0; q < 7; q++)
+                        if (c_dpD[q] != 0.0) ae = cadd(ae, cscale(c_dpD[q], S.k[q][e]));
+                    S.perr[e] = cadd(S.y0[e], cscale(h, ae));
+                }
+                __syncthreads();
+            }
+            error = dev_err_spf(d, P, S.tmp, S.perr, S) / P.eps_psi;
+            attempts++;
+            if (error < 1.0) break;
+            if (fail || attempts > P.max_attempts || !(error == error)) break;
+            const double a = minscal * h, b = safety * h * pow(error, pshrnk);
+            h = a > b ? a : b;
+        }
+        if (fail || attempts > P.max_attempts || !(error == error)) break;
+        double hnext = (error > errcon) ? safety * h * pow(error, pgrow) : h * maxscal;
+        for (int e = tid; e < d.blk; e += nth) S.y0[e] = S.tmp[e];
+        __syncthreads();
+        // ---- step landing (propagate_SPF) ----
+        dt_done = dt_done + h;
+        if (dt_done >= P.dt) break;
+        if (hnext < P.min_stepsize * 2) hnext = P.min_stepsize * 2;
+        stepsize = (P.dt - dt_done) / (double)((long long)((P.dt - dt_done) / hnext) + 1);
+        if (stepsize > P.dt - dt_done) stepsize = P.dt - dt_done;
+    }
+    __syncthreads();
+    if (tid == 0 && (fail || attempts > P.max_attempts)) atomicMax(status, fail ? 1 : 2);
+    {
+        int o = 0;
+        for (int s = 0; s < d.nsets; s++) {
+            const int n = d.nG * d.nspf[s];
+            for (int e = tid; e < n; e += nth) psi[d.off[s] + e] = S.y0[o + e];
+            o += n;
+        }
+    }
+}
+
+// Calc_Der_SPF for one mode on a full psi vector (dpsi receives that mode's blocks).
+__global__ void __launch_bounds__(256) k_der_spf(const SpfModeDesc *__restrict__ modes, int mode, const double2 *psi,
+                                                 double2 *dpsi, double2 *scratch, SpfParams P, DofTables dt,
+                                                 int *status) {
+    const SpfModeDesc d = modes[mode];
+    const int tid = threadIdx.x, nth = blockDim.x;
+    SpfScratch S = carve(scratch, d, P.nspfmax);
+    __shared__ int fail;
+    if (tid == 0) fail = 0;
+    int o = 0;
+    for (int s = 0; s < d.nsets; s++) {
+        const int n = d.nG * d.nspf[s];
+        for (int e = tid; e < n; e += nth) S.y0[o + e] = psi[d.off[s] + e];
+        o += n;
+    }
+    __syncthreads();
+    dev_der_spf(d, P, dt, S.y0, S.k[0], S, &fail);
+    o = 0;
+    for (int s = 0; s < d.nsets; s++) {
+        const int n = d.nG * d.nspf[s];
+        for (int e = tid; e < n; e += nth) dpsi[d.off[s] + e] = S.k[0][o + e];
+        o += n;
+    }
+    if (tid == 0 && fail) atomicMax(status, 1);
+}
+
+void launch_der_spf(const SpfModeDesc *d_modes, int mode, const double2 *psi, double2 *dpsi, double2 *scratch,
+                    SpfParams p, DofTables dt, int *d_status, hipStream_t s) {
+    hipLaunchKernelGGL(k_der_spf, dim3(1), dim3(256), 0, s, d_modes, mode, psi, dpsi, scratch, p, dt, d_status);
+}
+
+void launch_spf_propagate(const SpfModeDesc *d_modes, int nmodes, double2 *psi, double2 *scratch, SpfParams p,
+                          DofTables dt, int *d_status, hipStream_t s) {
+    hipLaunchKernelGGL(k_spf_propagate, dim3(nmodes), dim3(256), 0, s, d_modes, psi, scratch, p, dt, d_status);
+}
+
+}  // namespace mg

@@ -1,0 +1,180 @@
+"""Python binding of the HIP engine's C-ABI (include/macaw_gpu.h).
+
+The product path: every call goes to libmacaw_gpu.so (HIP, gfx950).  There is
+no CPU fallback; loading fails loudly when the library is missing.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+from .abi import MgAttempt, MgOutput, MgProblem, cplx_ptr, f64p, i64p, make_mg_problem
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libmacaw_gpu.so")
+_lib = None
+
+
+class EngineError(RuntimeError):
+    pass
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise EngineError(f"HIP engine not built: {LIB_PATH} missing (run __graft_entry__.build())")
+        L = C.CDLL(LIB_PATH)
+        vp = C.c_void_p
+        L.mg_create.restype = C.c_int
+        L.mg_create.argtypes = [C.POINTER(MgProblem), C.c_int64, C.POINTER(vp)]
+        L.mg_destroy.argtypes = [vp]
+        L.mg_last_error.restype = C.c_char_p
+        L.mg_last_error.argtypes = [vp]
+        L.mg_nconf.restype = C.c_int64
+        L.mg_nconf.argtypes = [vp]
+        L.mg_dimpsi.restype = C.c_int64
+        L.mg_dimpsi.argtypes = [vp]
+        L.mg_dims.argtypes = [vp, i64p, i64p]
+        L.mg_stepsize.argtypes = [vp, f64p]
+        L.mg_apply_h_profile.argtypes = [vp, vp, vp, f64p, i64p]
+        for f in ("mg_set_state", "mg_get_state", "mg_set_start"):
+            getattr(L, f).argtypes = [vp, f64p, f64p]
+        L.mg_calc_h1mode.argtypes = [vp, f64p]
+        L.mg_calc_rho.argtypes = [vp, f64p]
+        L.mg_calc_meanfield.argtypes = [vp, f64p, f64p]
+        L.mg_apply_h.argtypes = [vp, f64p, f64p]
+        L.mg_apply_h_device.argtypes = [vp, vp, vp]
+        L.mg_apply_h_stats.argtypes = [vp, f64p, f64p]
+        L.mg_der_spf.argtypes = [vp, C.c_int64, f64p, f64p]
+        L.mg_cmf2_step.argtypes = [vp, C.c_double, C.c_double, f64p, f64p]
+        L.mg_observables.argtypes = [vp, C.c_double, C.POINTER(MgOutput), f64p]
+        L.mg_propagate.argtypes = [vp, C.c_double, C.c_double, C.POINTER(MgOutput), f64p, C.c_int64,
+                                   i64p, C.POINTER(MgAttempt), C.c_int64, i64p]
+        _lib = L
+    return _lib
+
+
+class Engine:
+    """One engine context (one HIP stream) for a problem descriptor."""
+
+    def __init__(self, prob, precision: int = 0):
+        self.prob = prob
+        self.mgp, self._keep = make_mg_problem(prob)
+        h = C.c_void_p()
+        rc = lib().mg_create(C.byref(self.mgp), precision, C.byref(h))
+        self.h = h
+        if rc != 0:
+            msg = lib().mg_last_error(h).decode() if h else "mg_create failed"
+            lib().mg_destroy(h)
+            self.h = None
+            raise EngineError(msg)
+        self.nconf = lib().mg_nconf(self.h)
+        self.dimpsi = lib().mg_dimpsi(self.h)
+        a, b = C.c_int64(0), C.c_int64(0)
+        self._check(lib().mg_dims(self.h, C.byref(a), C.byref(b)))
+        self.nspfmax, self.nopmax = a.value, b.value
+        self.set_state(prob.A0, prob.psi0)
+        self.set_start(prob.A0, prob.psi0)
+
+    def close(self):
+        if self.h:
+            lib().mg_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc):
+        if rc != 0:
+            raise EngineError(lib().mg_last_error(self.h).decode())
+
+    def set_state(self, A, psi):
+        A = np.ascontiguousarray(A, np.complex128)
+        psi = np.ascontiguousarray(psi, np.complex128)
+        self._check(lib().mg_set_state(self.h, cplx_ptr(A), cplx_ptr(psi)))
+
+    def set_start(self, A, psi):
+        A = np.ascontiguousarray(A, np.complex128)
+        psi = np.ascontiguousarray(psi, np.complex128)
+        self._check(lib().mg_set_start(self.h, cplx_ptr(A), cplx_ptr(psi)))
+
+    def get_state(self):
+        A = np.zeros(self.nconf, np.complex128)
+        psi = np.zeros(self.dimpsi, np.complex128)
+        self._check(lib().mg_get_state(self.h, cplx_ptr(A), cplx_ptr(psi)))
+        return A, psi
+
+    def h1mode(self, nopmax=None):
+        nopmax = self.nopmax
+        out = np.zeros((self.prob.nmodes, nopmax, self.nspfmax, self.nspfmax), np.complex128)
+        self._check(lib().mg_calc_h1mode(self.h, cplx_ptr(out)))
+        return out
+
+    def rho(self):
+        out = np.zeros((self.prob.nstates, self.prob.nmodes, self.nspfmax, self.nspfmax), np.complex128)
+        self._check(lib().mg_calc_rho(self.h, cplx_ptr(out)))
+        return out
+
+    def meanfield(self, nopmax=None):
+        shape = (self.prob.nmodes, self.nopmax, self.nspfmax, self.nspfmax)
+        mf = np.zeros(shape, np.complex128)
+        rmf = np.zeros(shape, np.complex128)
+        self._check(lib().mg_calc_meanfield(self.h, cplx_ptr(mf), cplx_ptr(rmf)))
+        return mf, rmf
+
+    def apply_h(self, B):
+        B = np.ascontiguousarray(B, np.complex128)
+        HB = np.zeros_like(B)
+        self._check(lib().mg_apply_h(self.h, cplx_ptr(B), cplx_ptr(HB)))
+        return HB
+
+    def apply_h_device(self, dB_ptr: int, dHB_ptr: int):
+        self._check(lib().mg_apply_h_device(self.h, C.c_void_p(dB_ptr), C.c_void_p(dHB_ptr)))
+
+    def apply_h_profile(self, dB_ptr: int, dHB_ptr: int):
+        ms, n = C.c_double(0), C.c_int64(0)
+        self._check(lib().mg_apply_h_profile(self.h, C.c_void_p(dB_ptr), C.c_void_p(dHB_ptr),
+                                             C.byref(ms), C.byref(n)))
+        return ms.value, n.value
+
+    def stepsize(self):
+        h = C.c_double(0)
+        self._check(lib().mg_stepsize(self.h, C.byref(h)))
+        return h.value
+
+    def apply_h_stats(self):
+        b, f = C.c_double(0), C.c_double(0)
+        self._check(lib().mg_apply_h_stats(self.h, C.byref(b), C.byref(f)))
+        return b.value, f.value
+
+    def der_spf(self, im, psi):
+        psi = np.ascontiguousarray(psi, np.complex128)
+        d = np.zeros_like(psi)
+        self._check(lib().mg_der_spf(self.h, im, cplx_ptr(psi), cplx_ptr(d)))
+        return d
+
+    def cmf2_step(self, t, htry):
+        hd, hn = C.c_double(0), C.c_double(0)
+        self._check(lib().mg_cmf2_step(self.h, t, htry, C.byref(hd), C.byref(hn)))
+        return hd.value, hn.value
+
+    def observables(self, t):
+        out = MgOutput()
+        pops = np.zeros(self.prob.nstates)
+        self._check(lib().mg_observables(self.h, t, C.byref(out), pops.ctypes.data_as(f64p)))
+        return out, pops
+
+    def propagate(self, t0, tfinal, max_out=100000, max_att=200000):
+        outs = (MgOutput * max_out)()
+        pops = np.zeros((max_out, self.prob.nstates))
+        atts = (MgAttempt * max_att)()
+        no, na = C.c_int64(0), C.c_int64(0)
+        self._check(lib().mg_propagate(self.h, t0, tfinal, outs, pops.ctypes.data_as(f64p), max_out,
+                                       C.byref(no), atts, max_att, C.byref(na)))
+        return list(outs[: no.value]), pops[: no.value], list(atts[: na.value])

@@ -99,20 +99,24 @@ static void mode_product(int64_t nm, const int64_t *sz, int64_t k, const cplx *M
     for (int64_t q = 0; q < k; q++) ll *= sz[q];
     for (int64_t q = k + 1; q < nm; q++) nl *= sz[q];
     const int64_t nin = sz[k];
-#pragma omp parallel for schedule(static)
-    for (int64_t r = 0; r < nl; r++) {
-        const cplx *ib = in + r * ll * nin;
-        cplx *ob = out + r * ll * nout;
-        for (int64_t i = 0; i < nout; i++) {
-            cplx *o = ob + i * ll;
-            for (int64_t l = 0; l < ll; l++) o[l] = 0;
-            for (int64_t j = 0; j < nin; j++) {
-                const cplx mij = M[i + ldm * j];
-                const cplx *x = ib + j * ll;
-                for (int64_t l = 0; l < ll; l++) o[l] += mij * x[l];
+    const int64_t chunk = 256;
+    const int64_t nch = (ll + chunk - 1) / chunk;
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int64_t r = 0; r < nl; r++)
+        for (int64_t c = 0; c < nch; c++) {
+            const int64_t l0 = c * chunk, l1 = (l0 + chunk < ll) ? l0 + chunk : ll;
+            const cplx *ib = in + r * ll * nin;
+            cplx *ob = out + r * ll * nout;
+            for (int64_t i = 0; i < nout; i++) {
+                cplx *o = ob + i * ll;
+                for (int64_t l = l0; l < l1; l++) o[l] = 0;
+                for (int64_t j = 0; j < nin; j++) {
+                    const cplx mij = M[i + ldm * j];
+                    const cplx *x = ib + j * ll;
+                    for (int64_t l = l0; l < l1; l++) o[l] += mij * x[l];
+                }
             }
         }
-    }
 }
 
 /* Hole contraction over mode k: G[j1 + ldg*j2] += alpha * sum_{l,r}
@@ -740,6 +744,7 @@ static void apply_h(orc *o, const cplx *B, cplx *HB) {
         }
         cplx ae = o->ae[t];
         cplx *out = HB + o->AStateOffset[s1];
+#pragma omp parallel for schedule(static)
         for (int64_t i = 0; i < o->nAConf[s1]; i++) out[i] += ae * b2[i];
     }
     free(b1);

@@ -1,0 +1,173 @@
+"""HIP engine vs the CPU oracle, through the C-ABI, on the same inputs.
+
+Tolerances: kernels 1e-12 relative (FP64, different summation order);
+trajectories 1e-8 absolute in C(t), populations and norm (the FP64 parity
+bar for this path).
+"""
+import numpy as np
+import pytest
+
+from oracle_lib import Oracle
+from macaw_amd.lvc import LVCSpec, make_problem
+
+gpu = pytest.mark.gpu
+CASES = ["t1", "t2", "t3", "t4"]
+
+
+def engine_for(prob):
+    from macaw_amd.engine import Engine
+    return Engine(prob)
+
+
+def rand_c(rng, *shape):
+    return rng.normal(size=shape) + 1j * rng.normal(size=shape)
+
+
+def perturbed_state(prob, seed):
+    rng = np.random.default_rng(seed)
+    A = rand_c(rng, prob.nAConftot)
+    A /= np.linalg.norm(A)
+    psi = prob.psi0 + 0.05 * rand_c(rng, prob.dimpsi)
+    return A, psi
+
+
+@pytest.fixture(scope="module", params=CASES)
+def case(request, tmp_path_factory):
+    return make_problem(request.param, str(tmp_path_factory.mktemp(request.param)))
+
+
+def close(a, b, rel):
+    scale = max(1.0, float(np.abs(b).max()))
+    return float(np.abs(a - b).max()) <= rel * scale
+
+
+@gpu
+def test_h1mode_rho_meanfield_match_oracle(case):
+    A, psi = perturbed_state(case, 1)
+    o = Oracle(case)
+    o.set_state(A, psi)
+    e = engine_for(case)
+    e.set_state(A, psi)
+    h_o = o.h1mode()
+    h_e = e.h1mode(o.nopmax)
+    assert close(h_e, h_o, 1e-12)
+    r_o = o.rho()
+    r_e = e.rho()
+    assert close(r_e, r_o, 1e-12)
+    mf_o, rmf_o, _ = o.meanfield()
+    mf_e, rmf_e = e.meanfield(o.nopmax)
+    assert close(mf_e, mf_o, 1e-12)
+    assert close(rmf_e, rmf_o, 1e-8)
+
+
+@gpu
+def test_apply_h_matches_oracle(case):
+    A, psi = perturbed_state(case, 2)
+    o = Oracle(case)
+    o.set_state(A, psi)
+    o.h1mode()
+    e = engine_for(case)
+    e.set_state(A, psi)
+    e.h1mode(o.nopmax)
+    rng = np.random.default_rng(3)
+    for _ in range(3):
+        B = rand_c(rng, case.nAConftot)
+        assert close(e.apply_h(B), o.apply_h(B), 1e-12)
+
+
+@gpu
+def test_der_spf_matches_oracle(case):
+    A, psi = perturbed_state(case, 4)
+    o = Oracle(case)
+    o.set_state(A, psi)
+    o.h1mode()
+    o.rho()
+    o.meanfield()
+    e = engine_for(case)
+    e.set_state(A, psi)
+    e.h1mode(o.nopmax)
+    e.rho()
+    e.meanfield(o.nopmax)
+    for m in range(1, case.nmodes + 1):
+        d_o = o.der_spf(m, psi)
+        d_e = e.der_spf(m, psi)
+        ip, _ = case.ipsiSPF()
+        assert close(d_e, d_o, 1e-9), m
+
+
+@gpu
+def test_cmf2_step_matches_oracle(case):
+    o = Oracle(case)
+    e = engine_for(case)
+    t, h = 0.0, 2.0
+    for _ in range(3):
+        rc, hd_o, hn_o, _ = o.cmf2_step(t, h)
+        assert rc == 0
+        hd_e, hn_e = e.cmf2_step(t, h)
+        assert hd_e == pytest.approx(hd_o, rel=1e-9)
+        assert hn_e == pytest.approx(hn_o, rel=1e-6)
+        A_o, psi_o = o.get_state()
+        A_e, psi_e = e.get_state()
+        assert np.abs(A_e - A_o).max() < 1e-9
+        assert np.abs(psi_e - psi_o).max() < 1e-9
+        t += hd_o
+        h = hn_o
+
+
+@gpu
+@pytest.mark.parametrize("name,tf_fs", [("t1", 4.0), ("t3", 2.0), ("t4", 3.0)])
+def test_propagation_matches_oracle(name, tf_fs, tmp_path):
+    prob = make_problem(name, str(tmp_path))
+    tf = tf_fs / 0.02418884326505
+    o = Oracle(prob)
+    outs_o, pops_o, atts_o = o.propagate(0.0, tf)
+    e = engine_for(prob)
+    outs_e, pops_e, atts_e = e.propagate(0.0, tf)
+    assert len(outs_e) == len(outs_o)
+    for a, b in zip(outs_e, outs_o):
+        assert a.time == pytest.approx(b.time, abs=1e-9)
+        assert abs(complex(a.auto_re, a.auto_im) - complex(b.auto_re, b.auto_im)) < 1e-8
+        assert abs(a.norm - b.norm) < 1e-8
+        assert abs(a.energy - b.energy) < 1e-8 * max(1.0, abs(b.energy))
+    assert np.abs(pops_e - pops_o).max() < 1e-8
+    assert [a.accepted for a in atts_e] == [a.accepted for a in atts_o]
+
+
+@gpu
+def test_c1_first_femtoseconds_match_oracle(tmp_path):
+    prob = make_problem("c1", str(tmp_path))
+    tf = 2.0 / 0.02418884326505
+    o = Oracle(prob)
+    outs_o, pops_o, _ = o.propagate(0.0, tf)
+    e = engine_for(prob)
+    outs_e, pops_e, _ = e.propagate(0.0, tf)
+    assert len(outs_e) == len(outs_o) == 5
+    for a, b in zip(outs_e, outs_o):
+        assert abs(complex(a.auto_re, a.auto_im) - complex(b.auto_re, b.auto_im)) < 1e-8
+    assert np.abs(pops_e - pops_o).max() < 1e-8
+
+
+@gpu
+def test_full_size_apply_h_invariants():
+    """At the 4-state 8-mode 8^8 shape: Hermiticity <x|Hy> = <Hx|y> and
+    linearity H(ax + y) = aHx + Hy (size-independent properties)."""
+    import os
+    import tempfile
+    from macaw_amd.lvc import SPECS
+    import dataclasses
+    spec = dataclasses.replace(SPECS["c2"], name="c2h")
+    with tempfile.TemporaryDirectory() as d:
+        prob = make_problem(spec, d)
+    e = engine_for(prob)
+    rng = np.random.default_rng(7)
+    n = prob.nAConftot
+    x = (rng.normal(size=n) + 1j * rng.normal(size=n)) / np.sqrt(n)
+    y = (rng.normal(size=n) + 1j * rng.normal(size=n)) / np.sqrt(n)
+    e.h1mode()
+    hx = e.apply_h(x)
+    hy = e.apply_h(y)
+    lhs, rhs = np.vdot(x, hy), np.vdot(hx, y)
+    assert abs(lhs - rhs) < 1e-10 * max(abs(lhs), 1e-3)
+    a = 0.3 - 0.7j
+    hz = e.apply_h(a * x + y)
+    assert np.abs(hz - (a * hx + hy)).max() < 1e-12 * np.abs(hz).max()
