@@ -122,9 +122,16 @@ int mg_propagate(mg_ctx *ctx, double t0, double tfinal, mg_output *outs, double 
  * multiply-add over all mode products) of one mg_apply_h. */
 int mg_apply_h_stats(mg_ctx *ctx, double *bytes_alg, double *flops_alg);
 /* One mg_apply_h_device with a HIP event pair around every kernel launch:
- * returns the summed kernel time (ms) and the number of launches. */
+ * returns the summed kernel time (ms) and the number of launches.  Null
+ * pointers select the context's own buffers (the last mg_apply_h input). */
 int mg_apply_h_profile(mg_ctx *ctx, const void *dB, void *dHB, double *kernel_ms,
                        int64_t *nlaunch);
+/* Replay: the next n CMF2 attempts use the step sizes and accept/reject
+ * decisions of `log` (e.g. the attempt log of another run) instead of the
+ * error controller, so two runs can be compared arithmetic-only.  The errors
+ * are still computed and logged.  n = 0 clears.  No reference counterpart
+ * (a test/measurement aid). */
+int mg_set_replay(mg_ctx *ctx, const mg_attempt *log, int64_t n);
 /* Current CMF2 step size carried between mg_propagate calls (a.u.). */
 int mg_stepsize(const mg_ctx *ctx, double *h);
 

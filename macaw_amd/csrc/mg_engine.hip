@@ -15,6 +15,7 @@
 #include <map>
 #include <memory>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -127,6 +128,8 @@ struct mg_ctx {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     double last_ms = 0.0, bytes_alg = 0.0, flops_alg = 0.0;
     double stepsize = 0.0;         // carried between mg_propagate calls
+    std::vector<mg_attempt> replay;
+    size_t replay_pos = 0;
     bool prof = false;             // event pair around every group launch
     double prof_ms = 0.0;
     int64_t prof_launches = 0;
@@ -331,7 +334,12 @@ void setup(mg_ctx *c, const mg_problem *p) {
             c->n_rmf_jobs++;
         }
     }
-    // mode groups (sizes: max over states)
+    // mode groups (sizes: max over states); MG_BRICK overrides the LDS brick
+    // capacity in complex elements (tuning)
+    if (const char *e = std::getenv("MG_BRICK")) {
+        const int b = std::atoi(e);
+        if (b >= 64 && b <= 4608) c->brick = b;
+    }
     c->groups.clear();
     {
         std::vector<int64_t> mx(nm, 0);
@@ -1153,6 +1161,12 @@ void cmf2_step(mg_ctx *c, double tt, double htry, double *hdid_out, double *hnex
     const double safe = 0.95, maxscal = 1.5;
     double hdid = htry;
     for (;;) {
+        int forced = -1;
+        if (c->replay_pos < c->replay.size()) {
+            const mg_attempt &r = c->replay[c->replay_pos++];
+            hdid = r.h;
+            forced = (int)r.accepted;
+        }
         calc_rho(c, A);
         calc_h1mode(c, psi);
         calc_meanfield(c, A);
@@ -1172,8 +1186,9 @@ void cmf2_step(mg_ctx *c, double tt, double htry, double *hdid_out, double *hnex
         launch_axpby(B02, B0, -1.0, 1.0, N, c->stream);
         const double err_A = std::sqrt(norm2(c, B02, N)) * 0.5;
         const double error = std::max(err_A, err_psi) / c->eps_int;
-        if (log) log->push_back({tt, hdid, err_A, err_psi, (int64_t)(!(error > 1.0)), nk1, nk2});
-        if (error > 1.0) {
+        const bool reject = (forced >= 0) ? (forced == 0) : (error > 1.0);
+        if (log) log->push_back({tt, hdid, err_A, err_psi, (int64_t)(!reject), nk1, nk2});
+        if (reject) {
             hdid = hdid / std::sqrt(error) * safe;
             if (hdid < c->min_stepsize) throw HipError{"CMF2 step size fell below minstep"};
             continue;
@@ -1284,6 +1299,13 @@ int mg_dims(const mg_ctx *c, int64_t *nspfmax, int64_t *nopmax) {
     return 0;
 }
 
+int mg_set_replay(mg_ctx *c, const mg_attempt *log, int64_t n) {
+    if (!c || (n > 0 && !log)) return -1;
+    c->replay.assign(log, log + (n > 0 ? n : 0));
+    c->replay_pos = 0;
+    return 0;
+}
+
 int mg_stepsize(const mg_ctx *c, double *h) {
     if (!c || !h) return -1;
     *h = c->stepsize;
@@ -1295,8 +1317,10 @@ int mg_apply_h_profile(mg_ctx *c, const void *dB, void *dHB, double *kernel_ms, 
         c->prof = true;
         c->prof_ms = 0.0;
         c->prof_launches = 0;
+        const double2 *in = dB ? static_cast<const double2 *>(dB) : c->dIn.as<double2>();
+        double2 *out = dHB ? static_cast<double2 *>(dHB) : c->dOut.as<double2>();
         try {
-            apply_h(c, static_cast<const double2 *>(dB), static_cast<double2 *>(dHB));
+            apply_h(c, in, out);
         } catch (...) {
             c->prof = false;
             throw;

@@ -40,6 +40,7 @@ def lib():
         L.mg_dims.argtypes = [vp, i64p, i64p]
         L.mg_stepsize.argtypes = [vp, f64p]
         L.mg_apply_h_profile.argtypes = [vp, vp, vp, f64p, i64p]
+        L.mg_set_replay.argtypes = [vp, C.POINTER(MgAttempt), C.c_int64]
         for f in ("mg_set_state", "mg_get_state", "mg_set_start"):
             getattr(L, f).argtypes = [vp, f64p, f64p]
         L.mg_calc_h1mode.argtypes = [vp, f64p]
@@ -137,11 +138,17 @@ class Engine:
     def apply_h_device(self, dB_ptr: int, dHB_ptr: int):
         self._check(lib().mg_apply_h_device(self.h, C.c_void_p(dB_ptr), C.c_void_p(dHB_ptr)))
 
-    def apply_h_profile(self, dB_ptr: int, dHB_ptr: int):
+    def apply_h_profile(self, dB_ptr: int = 0, dHB_ptr: int = 0):
+        """Event-timed H.A; null pointers use the buffers of the last apply_h."""
         ms, n = C.c_double(0), C.c_int64(0)
-        self._check(lib().mg_apply_h_profile(self.h, C.c_void_p(dB_ptr), C.c_void_p(dHB_ptr),
+        self._check(lib().mg_apply_h_profile(self.h, C.c_void_p(dB_ptr or None), C.c_void_p(dHB_ptr or None),
                                              C.byref(ms), C.byref(n)))
         return ms.value, n.value
+
+    def set_replay(self, attempts):
+        n = len(attempts)
+        arr = (MgAttempt * max(1, n))(*attempts)
+        self._check(lib().mg_set_replay(self.h, arr, n))
 
     def stepsize(self):
         h = C.c_double(0)
