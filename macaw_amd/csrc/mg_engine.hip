@@ -68,6 +68,8 @@ struct Launch {
     size_t off;
     int n;
     int64_t maxblocks;
+    int brick;          // largest Lt * M * Rt over the launch's tasks (LDS sizing)
+    int nbuf;           // LDS buffers: 1 if every task is an in-place chain
 };
 struct Plan {
     DevBuf buf;
@@ -109,6 +111,7 @@ struct mg_ctx {
     std::vector<int> mf1_rowptr, mf1_src;   // one-mode same-state MF = ae * rho
     std::vector<cplx> mf1_coef;
     std::vector<int> mf_chain_terms;        // terms needing AL/AR chains
+    std::vector<int> mf_partner;            // per chain term: mirror term index, -1 none, -2 derived
     std::vector<int> rmf_jobs;
     int n_rmf_jobs = 0;
     // ---- grouping ----
@@ -116,14 +119,14 @@ struct mg_ctx {
     std::vector<std::pair<int, int>> groups;
     // ---- device buffers ----
     DevBuf dA, dpsi, dAstart, dpsistart, dBK, dB0, dBH, dB02, dpH, dpH2, dHtmp, dIn, dOut;
-    DevBuf dh1, dhsum, drho, dmf, drmf, dS, dS2;
+    DevBuf dh1, dhsum, drho, dmf, drmf, dS, dS2, dGtmp;
     DevBuf dchain;                 // 2 * nchains * mixed
     DevBuf dmfbuf;                 // 2 * nm * mixed (AL / AR)
     DevBuf dkron;                  // 2 * nconf-sized temps for overlap chains
     DevBuf dspfscr, dpartials, dscal;
     DevBuf dhs_rowptr, dhs_src, dhs_coef, dmf1_rowptr, dmf1_src, dmf1_coef, drmf_jobs;
     DevBuf dgrid, dd1, dd2, dxd1, dngp, dmode_dofs, dmode_ndof, dspfmodes, dstatus;
-    int nblocks_red = 1024;
+    int nblocks_red = 512;
     // timing of apply_h_device
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     double last_ms = 0.0, bytes_alg = 0.0, flops_alg = 0.0;
@@ -296,6 +299,28 @@ void setup(mg_ctx *c, const mg_problem *p) {
             c->mf_chain_terms.push_back(t);
         }
     }
+    // Mirror terms: t' = (b <- a) with conj(ae) and the same primitive codes
+    // as t = (a <- b).  All supported primitive operators are Hermitian, so
+    // h^(ba) = (h^(ab))^H on every mode and the mean-field contribution of t'
+    // is the conjugate transpose of that of t (gmctdh_objects.f90:582-601
+    // evaluates both; here t' is derived from t).
+    c->mf_partner.assign(c->mf_chain_terms.size(), -1);
+    for (size_t i = 0; i < c->mf_chain_terms.size(); i++) {
+        if (c->mf_partner[i] != -1) continue;
+        const int t = c->mf_chain_terms[i];
+        for (size_t j = i + 1; j < c->mf_chain_terms.size(); j++) {
+            if (c->mf_partner[j] != -1) continue;
+            const int u = c->mf_chain_terms[j];
+            if (c->lhs[u] != c->rhs[t] || c->rhs[u] != c->lhs[t] || c->lhs[t] == c->rhs[t]) continue;
+            if (std::abs(c->ae[u] - std::conj(c->ae[t])) > 0.0) continue;
+            bool same = true;
+            for (int d = 0; d < nd && same; d++) same = c->iOpDof[d + (size_t)nd * t] == c->iOpDof[d + (size_t)nd * u];
+            if (!same) continue;
+            c->mf_partner[i] = u;
+            c->mf_partner[j] = -2;
+            break;
+        }
+    }
     c->hs_rowptr.assign(1, 0);
     c->hs_src.clear();
     c->hs_coef.clear();
@@ -383,6 +408,7 @@ void allocate(mg_ctx *c) {
     c->drmf.alloc(z * nn * c->nopmax * c->nm);
     c->dS.alloc(z * nn * c->nm * c->ns);
     c->dS2.alloc(z * nn * c->nm * c->ns);
+    c->dGtmp.alloc(z * nn);
     c->dchain.alloc(z * 2 * std::max<size_t>(1, c->chains.size()) * (size_t)c->mixed);
     c->dmfbuf.alloc(z * 2 * (size_t)c->nm * (size_t)c->mixed);
     c->dkron.alloc(z * 2 * N);
@@ -518,6 +544,23 @@ size_t spf_scratch_base(const mg_ctx *c) {
 // Group passes.
 int nmax_template(const mg_ctx *c) { return c->nspfmax <= 4 ? 4 : (c->nspfmax <= 8 ? 8 : 16); }
 
+// Largest LDS brick (complex elements) a task touches: Lt times the largest
+// intermediate group size along its op sequence.
+int task_brick(const GroupTask &t) {
+    int cur[kMaxGroupModes];
+    int64_t M = 1;
+    for (int q = 0; q < t.ng; q++) { cur[q] = t.nin[q]; M *= cur[q]; }
+    int64_t mx = M;
+    if (t.kind == 0)
+        for (int q = 0; q < t.nops; q++) {
+            const int p = t.opmode[q];
+            M = M / cur[p] * t.opnout[q];
+            cur[p] = t.opnout[q];
+            mx = std::max(mx, M);
+        }
+    return (int)(t.Lt * mx * t.Rt);
+}
+
 // Collects the launches of one operation; phases run in order on the stream.
 struct PlanBuilder {
     std::vector<GroupTask> tasks;
@@ -525,8 +568,13 @@ struct PlanBuilder {
     void phase(const std::vector<GroupTask> &ts) {
         if (ts.empty()) return;
         int64_t mx = 0;
-        for (auto &t : ts) mx = std::max(mx, (t.L / t.Lt) * t.R);
-        launches.push_back({tasks.size(), (int)ts.size(), mx});
+        int br = 1, nbuf = 1;
+        for (auto &t : ts) {
+            mx = std::max(mx, (t.L / t.Lt) * (t.R / t.Rt));
+            br = std::max(br, task_brick(t));
+            if (!t.inplace) nbuf = 2;
+        }
+        launches.push_back({tasks.size(), (int)ts.size(), mx, br, nbuf});
         tasks.insert(tasks.end(), ts.begin(), ts.end());
     }
     std::unique_ptr<Plan> finish() {
@@ -543,7 +591,7 @@ void run_plan(mg_ctx *c, const Plan &p) {
     const GroupTask *base = p.buf.as<GroupTask>();
     if (!c->prof) {
         for (auto &l : p.launches)
-            launch_group_tasks(base + l.off, l.n, l.maxblocks, nmax_template(c), c->brick, c->stream);
+            launch_group_tasks(base + l.off, l.n, l.maxblocks, nmax_template(c), l.brick, l.nbuf, c->stream);
         return;
     }
     while (c->prof_ev.size() < 2 * p.launches.size()) {
@@ -554,7 +602,7 @@ void run_plan(mg_ctx *c, const Plan &p) {
     for (size_t i = 0; i < p.launches.size(); i++) {
         const Launch &l = p.launches[i];
         HIP_CHECK(hipEventRecord(c->prof_ev[2 * i], c->stream));
-        launch_group_tasks(base + l.off, l.n, l.maxblocks, nmax_template(c), c->brick, c->stream);
+        launch_group_tasks(base + l.off, l.n, l.maxblocks, nmax_template(c), l.brick, l.nbuf, c->stream);
         HIP_CHECK(hipEventRecord(c->prof_ev[2 * i + 1], c->stream));
     }
     HIP_CHECK(hipStreamSynchronize(c->stream));
@@ -617,6 +665,12 @@ GroupTask make_task(const mg_ctx *c, int g, const std::vector<int64_t> &sz, cons
     t.R = R;
     t.Lt = largest_divisor_le(L, std::max<int64_t>(1, c->brick / Mmax));
     if (t.Lt * Mmax > c->brick) throw HipError{"brick too small for a mode group"};
+    // when the whole leading block fits, consecutive outer indices are
+    // contiguous too: give each workgroup Rt of them
+    t.Rt = (t.Lt == L) ? largest_divisor_le(R, std::max<int64_t>(1, c->brick / (L * Mmax))) : 1;
+    bool square = true;
+    for (int q = 0; q < t.nops; q++) square = square && (t.opnin[q] == t.opnout[q]);
+    t.inplace = (kind == 0 && square) ? 1 : 0;
     t.kind = kind;
     t.ld = c->nspfmax;
     t.alpha = d2(alpha);
@@ -931,7 +985,10 @@ void calc_meanfield(mg_ctx *c, const double2 *A) {
     const size_t mixed = (size_t)c->mixed;
     double2 *AL = c->dmfbuf.as<double2>();
     double2 *AR = AL + mixed * c->nm;
-    for (int t : c->mf_chain_terms) {
+    for (size_t ti = 0; ti < c->mf_chain_terms.size(); ti++) {
+        const int t = c->mf_chain_terms[ti];
+        const int partner = c->mf_partner[ti];
+        if (partner == -2) continue;   // derived from its mirror term
         const int a = (int)c->lhs[t] - 1, b = (int)c->rhs[t] - 1;
         const cplx ae = c->ae[t];
         // right chain R_i (modes f-1 .. 0), pointers alias on identity modes
@@ -999,12 +1056,21 @@ void calc_meanfield(mg_ctx *c, const double2 *A) {
             g.nl = nl;
             g.n1 = (int)c->NS(m, a);
             g.n2 = (int)c->NS(m, b);
-            g.G = c->mf(m, op);
             g.ld = c->nspfmax;
             g.alpha = d2(ae);
-            g.accumulate = 1;
             g.hermitian = 0;
-            launch_gram(g, c->dpartials.as<double2>(), c->nblocks_red, c->nspfmax, c->stream);
+            if (partner < 0) {
+                g.G = c->mf(m, op);
+                g.accumulate = 1;
+                launch_gram(g, c->dpartials.as<double2>(), c->nblocks_red, c->nspfmax, c->stream);
+            } else {
+                double2 *G = c->dGtmp.as<double2>();
+                g.G = G;
+                g.accumulate = 0;
+                launch_gram(g, c->dpartials.as<double2>(), c->nblocks_red, c->nspfmax, c->stream);
+                const int op2 = c->iOper[m + (size_t)c->nm * partner];
+                launch_mf_accum(c->mf(m, op), c->mf(m, op2), G, g.n1, g.n2, c->nspfmax, c->stream);
+            }
         }
     }
     (void)nn;

@@ -34,7 +34,8 @@ __host__ __device__ inline double2 cconj(double2 a) { return make_double2(a.x, -
 struct GroupTask {
     const double2 *X;
     double2 *Y;
-    int64_t L, Lt, R;
+    int64_t L, Lt, R, Rt;        // Rt consecutive outer indices per brick (when Lt == L)
+    int inplace;                 // chain of square ops: one LDS buffer, fibers updated in place
     int ng;
     int nin[kMaxGroupModes];     // group mode sizes of X
     int nout[kMaxGroupModes];    // group mode sizes of Y
@@ -126,7 +127,7 @@ struct SpfParams {
 
 // Host-side launchers (mg_kernels.hip).
 void launch_group_tasks(const GroupTask *d_tasks, int ntasks, int64_t max_blocks_per_task,
-                        int nmax, int brick, hipStream_t s);
+                        int nmax, int brick, int nbuf, hipStream_t s);
 void launch_mode_product(const ModeTask &t, int nmax, hipStream_t s);
 void launch_gram(const GramTask &t, double2 *partials, int nblocks, int nmax, hipStream_t s);
 void launch_spf_tasks(const SpfTask *d_tasks, int ntasks, DofTables dt, int max_ng, hipStream_t s);
@@ -151,6 +152,8 @@ void launch_spf_propagate(const SpfModeDesc *d_modes, int nmodes, double2 *psi, 
                           SpfParams p, DofTables dt, int *d_status, hipStream_t s);
 void launch_der_spf(const SpfModeDesc *d_modes, int mode, const double2 *psi, double2 *dpsi,
                     double2 *scratch, SpfParams p, DofTables dt, int *d_status, hipStream_t s);
+void launch_mf_accum(double2 *dst1, double2 *dst2, const double2 *G, int n1, int n2, int ld,
+                     hipStream_t s);
 void launch_mat_axpy(double2 *dst, const double2 *src, double2 alpha, int n, int ld, hipStream_t s);
 
 }  // namespace mg

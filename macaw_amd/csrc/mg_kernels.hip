@@ -36,18 +36,17 @@ __device__ __forceinline__ int pad(int i) { return i + (i >> 3); }
 template <int NMAX>
 __global__ void __launch_bounds__(256) k_group_run(const GroupTask *__restrict__ tasks, int capacity) {
     const GroupTask &t = tasks[blockIdx.y];
-    const int64_t nlb = t.L / t.Lt;
+    const int64_t nlb = t.L / t.Lt, nrb = t.R / t.Rt;
     const int64_t b = blockIdx.x;
-    if (b >= nlb * t.R) return;
-    const int64_t lb = b % nlb, r = b / nlb;
-    const int64_t l0 = lb * t.Lt;
-    const int Lt = (int)t.Lt;
+    if (b >= nlb * nrb) return;
+    const int64_t lb = b % nlb, rb = b / nlb;
+    const int64_t l0 = lb * t.Lt, r0 = rb * t.Rt;
+    const int Lt = (int)t.Lt, Rt = (int)t.Rt;
     const int64_t L = t.L;
     extern __shared__ double2 lds[];
     double2 *mats = lds;
     double2 *buf0 = lds + kMaxOpsPerTask * NMAX * NMAX;
-    double2 *buf1 = buf0 + capacity;
-    double2 *buf2 = buf1;
+    double2 *buf1 = t.inplace ? buf0 : buf0 + capacity;
     const int tid = threadIdx.x, nth = blockDim.x;
     int cur[kMaxGroupModes];
     int Min = 1, Mout = 1;
@@ -65,12 +64,14 @@ __global__ void __launch_bounds__(256) k_group_run(const GroupTask *__restrict__
             mats[q * NMAX * NMAX + i + NMAX * j] = v;
         }
     }
-    // load the X brick
+    // load the X brick: LDS index lt + Lt*(g + Min*rt)
     const double2 *X = t.X;
-    const int nin_tot = Lt * Min;
+    const int LM_in = Lt * Min;
+    const int nin_tot = LM_in * Rt;
     for (int idx = tid; idx < nin_tot; idx += nth) {
-        int lt = idx % Lt, g = idx / Lt;
-        buf0[pad(idx)] = X[l0 + lt + L * ((int64_t)g + (int64_t)Min * r)];
+        const int lt = idx % Lt, rest = idx / Lt;
+        const int g = rest % Min, rt = rest / Min;
+        buf0[pad(idx)] = X[l0 + lt + L * ((int64_t)g + (int64_t)Min * (r0 + rt))];
     }
     __syncthreads();
     double2 *src = buf0, *dst = buf1;
@@ -107,8 +108,9 @@ __global__ void __launch_bounds__(256) k_group_run(const GroupTask *__restrict__
             double2 *sw = src; src = dst; dst = sw;
         }
     } else {
-        // sum of square mode products into buf2 (accumulator)
-        for (int idx = tid; idx < nin_tot; idx += nth) buf2[pad(idx)] = zero2();
+        // sum of square mode products; buf1 accumulates
+        double2 *acc_buf = buf1;
+        for (int idx = tid; idx < nin_tot; idx += nth) acc_buf[pad(idx)] = zero2();
         __syncthreads();
         for (int q = 0; q < t.nops; q++) {
             const int p = t.opmode[q];
@@ -126,25 +128,27 @@ __global__ void __launch_bounds__(256) k_group_run(const GroupTask *__restrict__
 #pragma unroll
                 for (int i = 0; i < NMAX; i++) {
                     if (i < n) {
-                        double2 acc = buf2[pad(bi + i * stride)];
+                        double2 acc = acc_buf[pad(bi + i * stride)];
 #pragma unroll
                         for (int j = 0; j < NMAX; j++)
                             if (j < n) cfma(acc, Mq[i + NMAX * j], x[j]);
-                        buf2[pad(bi + i * stride)] = acc;
+                        acc_buf[pad(bi + i * stride)] = acc;
                     }
                 }
             }
             __syncthreads();
         }
-        src = buf2;
+        src = acc_buf;
     }
     // write Y
     double2 *Y = t.Y;
-    const int nout_tot = Lt * Mout;
+    const int LM_out = Lt * Mout;
+    const int nout_tot = LM_out * Rt;
     const double2 alpha = t.alpha;
     for (int idx = tid; idx < nout_tot; idx += nth) {
-        int lt = idx % Lt, g = idx / Lt;
-        int64_t a = l0 + lt + L * ((int64_t)g + (int64_t)Mout * r);
+        const int lt = idx % Lt, rest = idx / Lt;
+        const int g = rest % Mout, rt = rest / Mout;
+        const int64_t a = l0 + lt + L * ((int64_t)g + (int64_t)Mout * (r0 + rt));
         double2 v = cmul(alpha, src[pad(idx)]);
         if (t.accumulate) v = cadd(Y[a], v);
         Y[a] = v;
@@ -152,10 +156,10 @@ __global__ void __launch_bounds__(256) k_group_run(const GroupTask *__restrict__
 }
 
 template <int NMAX>
-static void launch_group_nmax(const GroupTask *d_tasks, int ntasks, int64_t max_blocks, int capacity,
+static void launch_group_nmax(const GroupTask *d_tasks, int ntasks, int64_t max_blocks, int capacity, int nbuf,
                               hipStream_t s) {
     static bool attr_set = false;
-    const size_t shm = sizeof(double2) * (kMaxOpsPerTask * NMAX * NMAX + 2 * (size_t)capacity);
+    const size_t shm = sizeof(double2) * (kMaxOpsPerTask * NMAX * NMAX + (size_t)nbuf * capacity);
     if (!attr_set) {
         (void)hipFuncSetAttribute((const void *)k_group_run<NMAX>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                   160 * 1024);
@@ -167,13 +171,15 @@ static void launch_group_nmax(const GroupTask *d_tasks, int ntasks, int64_t max_
 
 int group_capacity(int brick) { return brick + (brick >> 3) + 1; }
 
+// `brick`: the largest Lt * M * Rt (complex elements) of any task in this
+// launch; `nbuf`: 1 when every task is an in-place chain, else 2.
 void launch_group_tasks(const GroupTask *d_tasks, int ntasks, int64_t max_blocks_per_task, int nmax,
-                        int brick, hipStream_t s) {
+                        int brick, int nbuf, hipStream_t s) {
     if (ntasks <= 0 || max_blocks_per_task <= 0) return;
     const int capacity = group_capacity(brick);
-    if (nmax <= 4) launch_group_nmax<4>(d_tasks, ntasks, max_blocks_per_task, capacity, s);
-    else if (nmax <= 8) launch_group_nmax<8>(d_tasks, ntasks, max_blocks_per_task, capacity, s);
-    else launch_group_nmax<16>(d_tasks, ntasks, max_blocks_per_task, capacity, s);
+    if (nmax <= 4) launch_group_nmax<4>(d_tasks, ntasks, max_blocks_per_task, capacity, nbuf, s);
+    else if (nmax <= 8) launch_group_nmax<8>(d_tasks, ntasks, max_blocks_per_task, capacity, nbuf, s);
+    else launch_group_nmax<16>(d_tasks, ntasks, max_blocks_per_task, capacity, nbuf, s);
 }
 
 // ---------------------------------------------------------------------------
@@ -270,19 +276,29 @@ __global__ void __launch_bounds__(256) k_gram_partial(GramTask t, int j1o, int n
     if (threadIdx.x < 64) {
         double2 v = zero2();
         for (int w = 0; w < (int)(blockDim.x >> 6); w++) v = cadd(v, red[w][threadIdx.x]);
-        partials[(int64_t)blockIdx.x * 64 + threadIdx.x] = v;
+        partials[(int64_t)threadIdx.x * gridDim.x + blockIdx.x] = v;   // entry-major
     }
 }
 
-__global__ void k_gram_final(GramTask t, int j1o, int n1c, int j2o, int n2c, const double2 *partials,
-                             int nblocks) {
-    const int e = threadIdx.x;
+// Stage 2: one wave per matrix entry sums that entry's block partials in a
+// fixed order (deterministic), then applies alpha / accumulate.
+__global__ void __launch_bounds__(256) k_gram_final(GramTask t, int j1o, int n1c, int j2o, int n2c,
+                                                    const double2 *partials, int nblocks) {
+    const int lane = threadIdx.x & 63;
+    const int e = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
     if (e >= 64) return;
     const int a = e % 8, b = e / 8;
     if (a >= n1c || b >= n2c) return;
-    double2 v = zero2();
-    for (int k = 0; k < nblocks; k++) v = cadd(v, partials[(int64_t)k * 64 + e]);
-    v = cmul(t.alpha, v);
+    const double2 *p = partials + (int64_t)e * nblocks;
+    double sx = 0.0, sy = 0.0;
+    for (int k = lane; k < nblocks; k += 64) {
+        sx += p[k].x;
+        sy += p[k].y;
+    }
+    sx = wave_sum(sx);
+    sy = wave_sum(sy);
+    if (lane != 0) return;
+    double2 v = cmul(t.alpha, make_double2(sx, sy));
     const int j1 = j1o + a, j2 = j2o + b;
     double2 *g = t.G + j1 + (int64_t)t.ld * j2;
     if (t.accumulate) v = cadd(*g, v);
@@ -306,7 +322,7 @@ void launch_gram(const GramTask &t, double2 *partials, int nblocks, int nmax, hi
             const int n2c = (t.n2 - j2o) < 8 ? (t.n2 - j2o) : 8;
             hipLaunchKernelGGL(k_gram_partial, dim3(nblocks), dim3(256), 0, s, t, j1o, n1c, j2o, n2c,
                                partials);
-            hipLaunchKernelGGL(k_gram_final, dim3(1), dim3(64), 0, s, t, j1o, n1c, j2o, n2c, partials,
+            hipLaunchKernelGGL(k_gram_final, dim3(16), dim3(256), 0, s, t, j1o, n1c, j2o, n2c, partials,
                                nblocks);
         }
     }
@@ -584,6 +600,25 @@ void launch_hsum(double2 *hsum, const double2 *h1, const int *rowptr, const int 
     const int64_t tot = (int64_t)nmat * nmax * nmax;
     hipLaunchKernelGGL(k_hsum, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, hsum, h1, rowptr, src, coef,
                        nmat, nmax);
+}
+
+// Mean-field accumulation of a term and of its mirror term:
+// dst1 += G (n1 x n2), dst2 += G^H (n2 x n1).
+__global__ void k_mf_accum(double2 *dst1, double2 *dst2, const double2 *G, int n1, int n2, int ld) {
+    const int e = threadIdx.x;
+    if (e >= n1 * n2) return;
+    const int a = e % n1, b = e / n1;
+    const double2 g = G[a + ld * b];
+    if (dst2 == dst1) {   // same operator (single-set mode): dst += G + G^H, n1 == n2
+        dst1[a + ld * b] = cadd(dst1[a + ld * b], cadd(g, cconj(G[b + ld * a])));
+        return;
+    }
+    dst1[a + ld * b] = cadd(dst1[a + ld * b], g);
+    if (dst2) dst2[b + ld * a] = cadd(dst2[b + ld * a], cconj(g));
+}
+
+void launch_mf_accum(double2 *dst1, double2 *dst2, const double2 *G, int n1, int n2, int ld, hipStream_t s) {
+    hipLaunchKernelGGL(k_mf_accum, dim3(1), dim3(256), 0, s, dst1, dst2, G, n1, n2, ld);
 }
 
 __global__ void k_mat_axpy(double2 *dst, const double2 *src, double2 alpha, int n, int ld) {
