@@ -115,7 +115,7 @@ struct mg_ctx {
     std::vector<int> rmf_jobs;
     int n_rmf_jobs = 0;
     // ---- grouping ----
-    int brick = 2048;
+    int brick = 1024;
     std::vector<std::pair<int, int>> groups;
     // ---- device buffers ----
     DevBuf dA, dpsi, dAstart, dpsistart, dBK, dB0, dBH, dB02, dpH, dpH2, dHtmp, dIn, dOut;
@@ -221,6 +221,8 @@ void setup(mg_ctx *c, const mg_problem *p) {
         ic += c->nAConf[s];
     }
     c->nconf = ic;
+    for (int s = 0; s < ns; s++)
+        if (c->nAConf[s] >= (int64_t(1) << 31)) throw HipError{"more than 2^31 configurations in one state"};
     c->mixed = 1;
     for (int m = 0; m < nm; m++) {
         int64_t mx = 0;
@@ -606,11 +608,15 @@ void run_plan(mg_ctx *c, const Plan &p) {
         HIP_CHECK(hipEventRecord(c->prof_ev[2 * i + 1], c->stream));
     }
     HIP_CHECK(hipStreamSynchronize(c->stream));
+    static const bool verbose = std::getenv("MG_VERBOSE") != nullptr;
     for (size_t i = 0; i < p.launches.size(); i++) {
         float ms = 0.f;
         HIP_CHECK(hipEventElapsedTime(&ms, c->prof_ev[2 * i], c->prof_ev[2 * i + 1]));
         c->prof_ms += ms;
         c->prof_launches++;
+        if (verbose)
+            fprintf(stderr, "  launch %zu: tasks=%d blocks=%lld brick=%d nbuf=%d  %.3f ms\n", i, p.launches[i].n,
+                    (long long)p.launches[i].maxblocks, p.launches[i].brick, p.launches[i].nbuf, ms);
     }
 }
 
@@ -671,6 +677,23 @@ GroupTask make_task(const mg_ctx *c, int g, const std::vector<int64_t> &sz, cons
     bool square = true;
     for (int q = 0; q < t.nops; q++) square = square && (t.opnin[q] == t.opnout[q]);
     t.inplace = (kind == 0 && square) ? 1 : 0;
+    // index helpers: divisors of the brick addressing and per-op fiber strides
+    int64_t Mo = 1;
+    for (int q = 0; q < t.ng; q++) Mo *= t.nout[q];
+    t.fd_Lt = make_fastdiv((uint32_t)t.Lt);
+    t.fd_Min = make_fastdiv((uint32_t)Min);
+    t.fd_Mout = make_fastdiv((uint32_t)Mo);
+    {
+        std::vector<int64_t> cs(sz.begin() + m0, sz.begin() + m1);
+        for (int q = 0; q < t.nops; q++) {
+            const int p = t.opmode[q];
+            int64_t st = t.Lt;
+            for (int pp = 0; pp < p; pp++) st *= cs[pp];
+            t.opstride[q] = (int)st;
+            t.fd_opstride[q] = make_fastdiv((uint32_t)st);
+            if (kind == 0) cs[p] = t.opnout[q];
+        }
+    }
     t.kind = kind;
     t.ld = c->nspfmax;
     t.alpha = d2(alpha);
@@ -713,7 +736,9 @@ std::unique_ptr<Plan> build_apply_h(mg_ctx *c, const double2 *B, double2 *HB) {
             if (ops.empty() && g > 0) continue;
             tasks.push_back(make_task(c, g, sz, ops, 1, B + c->off[s], HB + c->off[s], 1.0, g > 0 ? 1 : 0));
         }
-        // non-final chain passes
+        pb.phase(tasks);
+        tasks.clear();
+        // non-final chain passes (separate launch: in-place tasks need one LDS buffer)
         for (size_t k = 0; k < c->chains.size(); k++) {
             if (g >= last[k]) continue;
             const Chain &ch = c->chains[k];

@@ -24,6 +24,30 @@ __host__ __device__ inline double2 csub(double2 a, double2 b) { return make_doub
 __host__ __device__ inline double2 cscale(double s, double2 a) { return make_double2(s * a.x, s * a.y); }
 __host__ __device__ inline double2 cconj(double2 a) { return make_double2(a.x, -a.y); }
 
+// Division by a loop-invariant divisor d >= 1 with a multiply-high and a
+// shift (round-up method, valid for n < 2^31): q = umulhi(n, m) >> s, d > 1.
+struct FastDiv {
+    uint32_t d, m, s;
+};
+inline FastDiv make_fastdiv(uint32_t d) {
+    FastDiv f{d, 0u, 0u};
+    if (d > 1) {
+        uint32_t l = 0;
+        while ((1ull << l) < d) l++;           // ceil(log2 d)
+        const uint32_t p = 31 + l;
+        f.m = (uint32_t)(((1ull << p) + d - 1) / d);
+        f.s = p - 32;
+    }
+    return f;
+}
+__host__ __device__ inline uint32_t fdiv(uint32_t n, const FastDiv &f) {
+#ifdef __HIP_DEVICE_COMPILE__
+    return f.d == 1 ? n : (__umulhi(n, f.m) >> f.s);
+#else
+    return f.d == 1 ? n : (uint32_t)((((uint64_t)n * f.m) >> 32) >> f.s);
+#endif
+}
+
 // One pass over a group of consecutive modes [m0, m0+ng) of a tensor laid out
 // as X[l, g, r]: l over the modes before the group (size L, already in output
 // sizes), g the group multi-index (first mode fastest), r over the modes after
@@ -49,6 +73,10 @@ struct GroupTask {
     int ld;                      // leading dimension of the op matrices
     double2 alpha;
     int accumulate;
+    // precomputed index helpers (host): divisors of the brick addressing
+    FastDiv fd_Lt, fd_Min, fd_Mout;
+    int opstride[kMaxOpsPerTask];        // fiber stride of op q inside the brick
+    FastDiv fd_opstride[kMaxOpsPerTask];
 };
 
 // Single-mode product: Y[l, i, r] = sum_j M[i, j] X[l, j, r]  (adj: conj(M[j, i])).
@@ -60,6 +88,7 @@ struct ModeTask {
     const double2 *M;
     int ld;
     int adj;
+    FastDiv fd_ll;               // set by launch_mode_product
 };
 
 // Hole contraction: G[j1 + ld*j2] (+)= alpha * sum_{l,r} conj(X[l,j1,r]) Y[l,j2,r].
@@ -72,6 +101,7 @@ struct GramTask {
     double2 alpha;
     int accumulate;
     int hermitian;               // 1: X == Y, result made exactly Hermitian
+    FastDiv fd_ll;               // set by launch_gram
 };
 
 // Matrix elements between SPF blocks of one mode:

@@ -69,8 +69,8 @@ __global__ void __launch_bounds__(256) k_group_run(const GroupTask *__restrict__
     const int LM_in = Lt * Min;
     const int nin_tot = LM_in * Rt;
     for (int idx = tid; idx < nin_tot; idx += nth) {
-        const int lt = idx % Lt, rest = idx / Lt;
-        const int g = rest % Min, rt = rest / Min;
+        const int rest = (int)fdiv((uint32_t)idx, t.fd_Lt), lt = idx - rest * Lt;
+        const int rt = (int)fdiv((uint32_t)rest, t.fd_Min), g = rest - rt * Min;
         buf0[pad(idx)] = X[l0 + lt + L * ((int64_t)g + (int64_t)Min * (r0 + rt))];
     }
     __syncthreads();
@@ -79,13 +79,13 @@ __global__ void __launch_bounds__(256) k_group_run(const GroupTask *__restrict__
         int tot = nin_tot;
         for (int q = 0; q < t.nops; q++) {
             const int p = t.opmode[q];
-            int stride = Lt;
-            for (int pp = 0; pp < p; pp++) stride *= cur[pp];
+            const int stride = t.opstride[q];
+            const FastDiv fds = t.fd_opstride[q];
             const int ni = cur[p], no = t.nout[p];
             const int nfib = tot / ni;
             const double2 *Mq = mats + q * NMAX * NMAX;
             for (int f = tid; f < nfib; f += nth) {
-                const int inner = f % stride, outer = f / stride;
+                const int outer = (int)fdiv((uint32_t)f, fds), inner = f - outer * stride;
                 const int bi = inner + outer * stride * ni;
                 const int bo = inner + outer * stride * no;
                 double2 x[NMAX];
@@ -114,13 +114,13 @@ __global__ void __launch_bounds__(256) k_group_run(const GroupTask *__restrict__
         __syncthreads();
         for (int q = 0; q < t.nops; q++) {
             const int p = t.opmode[q];
-            int stride = Lt;
-            for (int pp = 0; pp < p; pp++) stride *= cur[pp];
+            const int stride = t.opstride[q];
+            const FastDiv fds = t.fd_opstride[q];
             const int n = cur[p];
             const int nfib = nin_tot / n;
             const double2 *Mq = mats + q * NMAX * NMAX;
             for (int f = tid; f < nfib; f += nth) {
-                const int inner = f % stride, outer = f / stride;
+                const int outer = (int)fdiv((uint32_t)f, fds), inner = f - outer * stride;
                 const int bi = inner + outer * stride * n;
                 double2 x[NMAX];
 #pragma unroll
@@ -146,8 +146,8 @@ __global__ void __launch_bounds__(256) k_group_run(const GroupTask *__restrict__
     const int nout_tot = LM_out * Rt;
     const double2 alpha = t.alpha;
     for (int idx = tid; idx < nout_tot; idx += nth) {
-        const int lt = idx % Lt, rest = idx / Lt;
-        const int g = rest % Mout, rt = rest / Mout;
+        const int rest = (int)fdiv((uint32_t)idx, t.fd_Lt), lt = idx - rest * Lt;
+        const int rt = (int)fdiv((uint32_t)rest, t.fd_Mout), g = rest - rt * Mout;
         const int64_t a = l0 + lt + L * ((int64_t)g + (int64_t)Mout * (r0 + rt));
         double2 v = cmul(alpha, src[pad(idx)]);
         if (t.accumulate) v = cadd(Y[a], v);
@@ -195,10 +195,9 @@ __global__ void __launch_bounds__(256) k_mode_product(ModeTask t) {
         M[e] = v;
     }
     __syncthreads();
-    const int64_t nfib = t.ll * t.nl;
-    for (int64_t f = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; f < nfib;
-         f += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t l = f % t.ll, r = f / t.ll;
+    const uint32_t nfib = (uint32_t)(t.ll * t.nl);
+    for (uint32_t f = blockIdx.x * blockDim.x + threadIdx.x; f < nfib; f += gridDim.x * blockDim.x) {
+        const uint32_t r = fdiv(f, t.fd_ll), l = f - r * (uint32_t)t.ll;
         const double2 *x0 = t.X + l + t.ll * (int64_t)t.nin * r;
         double2 *y0 = t.Y + l + t.ll * (int64_t)t.nout * r;
         double2 x[NMAX];
@@ -217,7 +216,9 @@ __global__ void __launch_bounds__(256) k_mode_product(ModeTask t) {
     }
 }
 
-void launch_mode_product(const ModeTask &t, int nmax, hipStream_t s) {
+void launch_mode_product(const ModeTask &t0, int nmax, hipStream_t s) {
+    ModeTask t = t0;
+    t.fd_ll = make_fastdiv((uint32_t)t.ll);
     const int64_t nfib = t.ll * t.nl;
     int64_t blocks = (nfib + 255) / 256;
     if (blocks > 65536) blocks = 65536;
@@ -242,10 +243,9 @@ __global__ void __launch_bounds__(256) k_gram_partial(GramTask t, int j1o, int n
     for (int a = 0; a < 8; a++)
 #pragma unroll
         for (int b = 0; b < 8; b++) acc[a][b] = zero2();
-    const int64_t npos = t.ll * t.nl;
-    for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < npos;
-         p += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t l = p % t.ll, r = p / t.ll;
+    const uint32_t npos = (uint32_t)(t.ll * t.nl);
+    for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < npos; p += gridDim.x * blockDim.x) {
+        const uint32_t r = fdiv(p, t.fd_ll), l = p - r * (uint32_t)t.ll;
         double2 x[8], y[8];
         const double2 *xb = t.X + l + t.ll * ((int64_t)t.n1 * r + j1o);
         const double2 *yb = t.Y + l + t.ll * ((int64_t)t.n2 * r + j2o);
@@ -314,8 +314,10 @@ __global__ void k_gram_hermitize(double2 *G, int n, int ld) {
     if (in && a > b) G[a + ld * b] = cconj(G[b + ld * a]);
 }
 
-void launch_gram(const GramTask &t, double2 *partials, int nblocks, int nmax, hipStream_t s) {
+void launch_gram(const GramTask &t0, double2 *partials, int nblocks, int nmax, hipStream_t s) {
     (void)nmax;
+    GramTask t = t0;
+    t.fd_ll = make_fastdiv((uint32_t)t.ll);
     for (int j1o = 0; j1o < t.n1; j1o += 8) {
         const int n1c = (t.n1 - j1o) < 8 ? (t.n1 - j1o) : 8;
         for (int j2o = 0; j2o < t.n2; j2o += 8) {
