@@ -1,15 +1,19 @@
-"""Throughput of the CMF2 propagation on one MI355X.
+"""Throughput of the CMF2 propagation on one MI355X, mixed precision vs FP64.
 
 A "step" is one output interval (tout = 0.5 fs) of CMF2 propagation of the
 4-state 8-mode LVC model (HO-DVR 32 points, 8 SPFs per mode and state, 8^8
 configurations per state), including the observables written at the start of
 the interval, exactly as Macaw's propagate_wf does it.  W intervals (which
 include the reference's step-size ramp from tout*eps_int) run untimed, then K
-intervals are timed.  value = femtoseconds propagated per wall second.
+intervals are timed.  value = femtoseconds propagated per wall second by the
+mixed-precision engine (complex FP32 H.A / Krylov / mean-field chains, FP64
+state, SPFs and reductions).  The same W+K intervals are then run by the FP64
+engine; max|dC(t)| over every output time is the accuracy gate (<= 1e-6) and
+the FP64 rate is reported beside it.
 
 Also reported: the roofline of the dominant kernel (the grouped mode-product
-pass of H.A) timed with a HIP event pair around every launch, and a CPU
-baseline from the FP64 oracle run on the host.
+pass of H.A) timed with a HIP event pair around every launch on the engine's
+stream, and a CPU baseline from the FP64 oracle run on the host.
 """
 from __future__ import annotations
 
@@ -27,6 +31,7 @@ import numpy as np  # noqa: E402
 
 AU2FS = 0.02418884326505
 PEAK_FP64_TFLOPS = 78.6
+PEAK_FP32_TFLOPS = 157.3
 PEAK_HBM_GBS = 8000.0
 METRIC = "fs propagated/s at ≤1e-6 C(t) error vs FP64; mixed-prec vs FP64, 1 GPU"
 
@@ -41,7 +46,7 @@ def build_problem(name):
         return make_problem(name, d)
 
 
-def roofline(eng, prob, reps):
+def roofline(eng, prob, reps, peak):
     # input resident in HBM: one host upload through apply_h, then event-timed
     # H.A on the engine's own device buffers
     n = prob.nAConftot
@@ -61,8 +66,8 @@ def roofline(eng, prob, reps):
     flops_per_launch = flops_alg / launches_per_ha
     achieved = flops_per_launch / (avg_launch_ms * 1e-3) / 1e12
     return {
-        "bound": "mfma", "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
-        "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None,
+        "bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
+        "frac": achieved / peak, "traffic": None,
         "kernel": "k_group_run (grouped mode-product pass of H.A)",
         "avg_launch_ms": avg_launch_ms, "launches_per_HA": launches_per_ha,
         "HA_ms": avg_launch_ms * launches_per_ha, "HA_flops_alg": flops_alg, "HA_bytes_alg": bytes_alg,
@@ -92,6 +97,27 @@ def cpu_baseline(prob, ha_per_fs, threads):
     }
 
 
+def run_engine(prob, precision, warmup, steps):
+    from macaw_amd.engine import Engine
+    tout = prob.writetime
+    eng = Engine(prob, precision=precision)
+    tag = "mixed" if precision else "fp64"
+    log(f"[{tag}] warmup: {warmup} output intervals")
+    t_w = time.perf_counter()
+    outs_w, _, atts_w = eng.propagate(0.0, (warmup - 1) * tout)
+    log(f"[{tag}] warmup done in {time.perf_counter() - t_w:.1f} s, {len(atts_w)} attempts")
+    t_a = warmup * tout
+    log(f"[{tag}] timed: {steps} output intervals from {t_a * AU2FS:.2f} fs")
+    t0 = time.perf_counter()
+    outs, pops, atts = eng.propagate(t_a, t_a + (steps - 1) * tout)
+    wall = time.perf_counter() - t0
+    fs = steps * tout * AU2FS
+    nha = sum(a.nkrylov1 + a.nkrylov2 for a in atts) + len(outs)
+    log(f"[{tag}] timed done: {wall:.2f} s for {fs:.2f} fs ({fs / wall:.4f} fs/s), {len(atts)} attempts "
+        f"({sum(1 for a in atts if not a.accepted)} rejected), {nha} H.A")
+    return eng, {"outs": outs_w + outs, "wall": wall, "fs": fs, "atts": atts, "nha": nha}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -100,52 +126,54 @@ def main():
     ap.add_argument("--workload", default="c2")
     ap.add_argument("--roofline-reps", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 reference run (profiling)")
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this engine runs on one GPU; multi-GPU figures are not produced "
               "(see the sharding design in DESIGN.md)", file=sys.stderr)
         sys.exit(2)
-    from macaw_amd.engine import Engine
 
     log(f"building workload {args.workload}")
     prob = build_problem(args.workload)
-    tout = prob.writetime
     log(f"nconf={prob.nAConftot} dimpsi={prob.dimpsi} nham={prob.nHam}")
-    eng = Engine(prob)
-    log(f"warmup: {args.warmup} output intervals")
-    t_w = time.perf_counter()
-    outs_w, _, atts_w = eng.propagate(0.0, (args.warmup - 1) * tout)
-    log(f"warmup done in {time.perf_counter() - t_w:.1f} s, {len(atts_w)} attempts")
-    t_a = args.warmup * tout
-    log(f"timed: {args.steps} output intervals from {t_a * AU2FS:.2f} fs")
-    t0 = time.perf_counter()
-    outs, pops, atts = eng.propagate(t_a, t_a + (args.steps - 1) * tout)
-    wall = time.perf_counter() - t0
-    fs = args.steps * tout * AU2FS
-    value = fs / wall
-    nha = sum(a.nkrylov1 + a.nkrylov2 for a in atts) + len(outs)
-    log(f"timed done: {wall:.2f} s for {fs:.2f} fs, {len(atts)} attempts "
-        f"({sum(1 for a in atts if not a.accepted)} rejected), {nha} H.A")
-    rl = roofline(eng, prob, args.roofline_reps)
-    log(f"roofline: {rl['achieved']:.2f} TFLOP/s, H.A {rl['HA_ms']:.2f} ms")
-    last = outs[-1]
+    eng, mx = run_engine(prob, 1, args.warmup, args.steps)
+    rl = roofline(eng, prob, args.roofline_reps, PEAK_FP32_TFLOPS)
+    log(f"roofline (mixed): {rl['achieved']:.2f} TFLOP/s, H.A {rl['HA_ms']:.2f} ms")
+    eng.close()
+    value = mx["fs"] / mx["wall"]
+    last = mx["outs"][-1]
     result = {
         "metric": METRIC, "value": value, "unit": "fs/s", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": wall / args.steps * 1e3, "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": "c128", "data": "synthetic",
+        "warmup": args.warmup, "ms_per_step": mx["wall"] / args.steps * 1e3, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "c64/c128", "data": "synthetic",
         "config": {"workload": "LVC 4 states x 8 modes, HO-DVR 32, 8 SPFs (8^8 conf/state), CMF2 eps 1e-7",
-                   "precision": "complex FP64 (mixed-precision form not built yet)",
-                   "step": "one 0.5 fs output interval", "attempts": len(atts),
-                   "HA_per_fs": nha / fs, "C_t_last": [last.auto_re, last.auto_im],
+                   "precision": "mixed: complex FP32 H.A, Krylov v1.., mean-field chains; FP64 A, SPFs, "
+                                "rho, mean fields, reductions, DeltaPsi",
+                   "step": "one 0.5 fs output interval", "attempts": len(mx["atts"]),
+                   "HA_per_fs": mx["nha"] / mx["fs"], "C_t_last": [last.auto_re, last.auto_im],
                    "norm_last": last.norm},
         "roofline": {k: rl[k] for k in ("bound", "achieved", "peak", "unit", "frac", "traffic")},
         "roofline_detail": {k: v for k, v in rl.items() if k not in ("bound", "achieved", "peak", "unit", "frac", "traffic")},
     }
-    eng.close()
+    result["roofline_detail"]["peak_note"] = "MI355X dense FP32 (vector and matrix) 157.3 TFLOP/s"
+    if not args.no_fp64:
+        eng64, fx = run_engine(prob, 0, args.warmup, args.steps)
+        rl64 = roofline(eng64, prob, args.roofline_reps, PEAK_FP64_TFLOPS)
+        eng64.close()
+        n = min(len(fx["outs"]), len(mx["outs"]))
+        dc = max(abs(complex(a.auto_re, a.auto_im) - complex(b.auto_re, b.auto_im))
+                 for a, b in zip(mx["outs"][:n], fx["outs"][:n]))
+        v64 = fx["fs"] / fx["wall"]
+        log(f"max|dC(t)| mixed vs FP64 over {n} output times: {dc:.3e}; fp64 {v64:.4f} fs/s")
+        result["accuracy"] = {"max_abs_dC_vs_fp64": dc, "output_times": n,
+                              "span_fs": (n - 1) * prob.writetime * AU2FS, "gate": 1e-6, "pass": dc <= 1e-6}
+        result["fp64"] = {"value": v64, "unit": "fs/s", "speedup_mixed": value / v64,
+                          "HA_ms": rl64["HA_ms"], "roofline_frac": rl64["frac"],
+                          "achieved_TFLOPs": rl64["achieved"]}
     if not args.no_cpu:
         log(f"cpu baseline on {args.cpu_threads} threads")
-        result["cpu_baseline"] = cpu_baseline(prob, nha / fs, args.cpu_threads)
+        result["cpu_baseline"] = cpu_baseline(prob, mx["nha"] / mx["fs"], args.cpu_threads)
     print(json.dumps(result), flush=True)
 
 

@@ -74,6 +74,7 @@ struct Launch {
 struct Plan {
     DevBuf buf;
     std::vector<Launch> launches;
+    int prec = 0;       // storage type of the tensors the tasks touch
 };
 using PlanKey = std::array<const void *, 4>;
 struct SpfPlan {
@@ -86,7 +87,12 @@ struct SpfPlan {
 struct mg_ctx {
     std::string err;
     hipStream_t stream = nullptr;
-    int precision = 0;
+    // 0: complex FP64 throughout.  1: mixed form -- the A-vector, SPFs, rho,
+    // mean-field matrices and every reduction stay FP64; the H.A chains, the
+    // Krylov vectors beyond the first and the mean-field mode-product chains
+    // are stored and multiplied in complex FP32.
+    int prec = 0;
+    size_t es = sizeof(double2);   // bytes per complex element of the FP32/FP64 work tensors
     // ---- problem ----
     int ns = 0, nm = 0, ndof = 0, nham = 0, maxdof = 0, ngpmax = 0;
     std::vector<int64_t> nmodeDOF, iDOF, MultiSet, nSPF, ngp, lhs, rhs, iOpDof;
@@ -126,7 +132,10 @@ struct mg_ctx {
     DevBuf dspfscr, dpartials, dscal;
     DevBuf dhs_rowptr, dhs_src, dhs_coef, dmf1_rowptr, dmf1_src, dmf1_coef, drmf_jobs;
     DevBuf dgrid, dd1, dd2, dxd1, dngp, dmode_dofs, dmode_ndof, dspfmodes, dstatus;
-    int nblocks_red = 512;
+    // mixed form only: FP64 first Krylov vector, FP32 copy of the A-vector,
+    // FP32 in/out of the C-ABI H.A, FP32 copies of h1mode and the diagonal sums
+    DevBuf dV0, dAf, dXf, dYf, dh1f, dhsumf;
+    int nblocks_red = 2048;      // blocks of the streaming reductions and Gram partial sums
     // timing of apply_h_device
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     double last_ms = 0.0, bytes_alg = 0.0, flops_alg = 0.0;
@@ -154,6 +163,18 @@ struct mg_ctx {
     double2 *rho_ave(int m) const { return drho.as<double2>() + (size_t)nspfmax * nspfmax * (nm * ns + m); }
     double2 *mf(int m, int op) const { return dmf.as<double2>() + (size_t)nspfmax * nspfmax * (op + (size_t)nopmax * m); }
     double2 *S(int m, int s) const { return dS.as<double2>() + (size_t)nspfmax * nspfmax * (m + (size_t)nm * s); }
+    // operator matrices in the work-tensor storage type
+    const void *h1op(int m, int op) const {
+        if (!prec) return h1(m, op);
+        return dh1f.as<float2>() + (size_t)nspfmax * nspfmax * (op + (size_t)nopmax * m);
+    }
+    const void *hsumop(int s, int m) const {
+        if (!prec) return hsum(s, m);
+        return dhsumf.as<float2>() + (size_t)nspfmax * nspfmax * (m + (size_t)nm * s);
+    }
+    // p + n elements of the work-tensor storage type
+    void *at(void *p, int64_t n) const { return static_cast<char *>(p) + (size_t)n * es; }
+    const void *at(const void *p, int64_t n) const { return static_cast<const char *>(p) + (size_t)n * es; }
 };
 
 namespace {
@@ -362,10 +383,14 @@ void setup(mg_ctx *c, const mg_problem *p) {
         }
     }
     // mode groups (sizes: max over states); MG_BRICK overrides the LDS brick
-    // capacity in complex elements (tuning)
+    // capacity in complex elements (tuning).  Two padded bricks plus the op
+    // matrices must fit the 160 KB of LDS a workgroup may address.
+    c->brick = c->prec ? 2048 : 1024;
     if (const char *e = std::getenv("MG_BRICK")) {
         const int b = std::atoi(e);
-        if (b >= 64 && b <= 4608) c->brick = b;
+        const int nmx = c->nspfmax <= 4 ? 4 : (c->nspfmax <= 8 ? 8 : 16);
+        const size_t lds = c->es * ((size_t)kMaxOpsPerTask * nmx * nmx + 2 * (size_t)group_capacity(b));
+        if (b >= 64 && lds <= 160 * 1024) c->brick = b;
     }
     c->groups.clear();
     {
@@ -394,7 +419,16 @@ void allocate(mg_ctx *c) {
     c->dpsi.alloc(z * D);
     c->dAstart.alloc(z * N);
     c->dpsistart.alloc(z * D);
-    c->dBK.alloc(z * N * kNKrylovMax);
+    c->dBK.alloc(c->es * N * kNKrylovMax);
+    if (c->prec) {
+        const size_t zf = sizeof(float2);
+        c->dV0.alloc(z * N);
+        c->dAf.alloc(zf * N);
+        c->dXf.alloc(zf * N);
+        c->dYf.alloc(zf * N);
+        c->dh1f.alloc(zf * nn * c->nopmax * c->nm);
+        c->dhsumf.alloc(zf * nn * c->nm * c->ns);
+    }
     c->dB0.alloc(z * N);
     c->dBH.alloc(z * N);
     c->dB02.alloc(z * N);
@@ -411,8 +445,8 @@ void allocate(mg_ctx *c) {
     c->dS.alloc(z * nn * c->nm * c->ns);
     c->dS2.alloc(z * nn * c->nm * c->ns);
     c->dGtmp.alloc(z * nn);
-    c->dchain.alloc(z * 2 * std::max<size_t>(1, c->chains.size()) * (size_t)c->mixed);
-    c->dmfbuf.alloc(z * 2 * (size_t)c->nm * (size_t)c->mixed);
+    c->dchain.alloc(c->es * 2 * std::max<size_t>(1, c->chains.size()) * (size_t)c->mixed);
+    c->dmfbuf.alloc(c->es * 2 * (size_t)c->nm * (size_t)c->mixed);
     c->dkron.alloc(z * 2 * N);
     c->dpartials.alloc(sizeof(double2) * 64 * (size_t)c->nblocks_red);
     c->dscal.alloc(sizeof(double) * 64);
@@ -433,6 +467,7 @@ void allocate(mg_ctx *c) {
                         h[nn * (i + (size_t)c->nopmax * m) + j + (size_t)c->nspfmax * j] = make_double2(1, 0);
                 }
         HIP_CHECK(hipMemcpy(c->dh1.p, h.data(), z * h.size(), hipMemcpyHostToDevice));
+        if (c->prec) launch_convert(c->dh1f.p, 1, c->dh1.p, 0, (int64_t)h.size(), c->stream);
     }
     auto upload_i = [&](DevBuf &b, const std::vector<int> &v) {
         b.alloc(sizeof(int) * std::max<size_t>(1, v.size()));
@@ -579,8 +614,9 @@ struct PlanBuilder {
         launches.push_back({tasks.size(), (int)ts.size(), mx, br, nbuf});
         tasks.insert(tasks.end(), ts.begin(), ts.end());
     }
-    std::unique_ptr<Plan> finish() {
+    std::unique_ptr<Plan> finish(int prec) {
         auto p = std::make_unique<Plan>();
+        p->prec = prec;
         p->buf.alloc(sizeof(GroupTask) * std::max<size_t>(1, tasks.size()));
         if (!tasks.empty())
             HIP_CHECK(hipMemcpy(p->buf.p, tasks.data(), sizeof(GroupTask) * tasks.size(), hipMemcpyHostToDevice));
@@ -593,7 +629,7 @@ void run_plan(mg_ctx *c, const Plan &p) {
     const GroupTask *base = p.buf.as<GroupTask>();
     if (!c->prof) {
         for (auto &l : p.launches)
-            launch_group_tasks(base + l.off, l.n, l.maxblocks, nmax_template(c), l.brick, l.nbuf, c->stream);
+            launch_group_tasks(base + l.off, l.n, l.maxblocks, nmax_template(c), l.brick, l.nbuf, p.prec, c->stream);
         return;
     }
     while (c->prof_ev.size() < 2 * p.launches.size()) {
@@ -604,7 +640,7 @@ void run_plan(mg_ctx *c, const Plan &p) {
     for (size_t i = 0; i < p.launches.size(); i++) {
         const Launch &l = p.launches[i];
         HIP_CHECK(hipEventRecord(c->prof_ev[2 * i], c->stream));
-        launch_group_tasks(base + l.off, l.n, l.maxblocks, nmax_template(c), l.brick, l.nbuf, c->stream);
+        launch_group_tasks(base + l.off, l.n, l.maxblocks, nmax_template(c), l.brick, l.nbuf, p.prec, c->stream);
         HIP_CHECK(hipEventRecord(c->prof_ev[2 * i + 1], c->stream));
     }
     HIP_CHECK(hipStreamSynchronize(c->stream));
@@ -624,12 +660,12 @@ void run_plan(mg_ctx *c, const Plan &p) {
 // (modes < g.first already in output sizes).  ops: (mode, matrix, adj, nin, nout).
 struct OpSpec {
     int mode;
-    const double2 *mat;
+    const void *mat;          // storage type of the task
     int adj, nin, nout;
 };
 
 GroupTask make_task(const mg_ctx *c, int g, const std::vector<int64_t> &sz, const std::vector<OpSpec> &ops, int kind,
-                    const double2 *X, double2 *Y, cplx alpha, int accumulate) {
+                    const void *X, void *Y, cplx alpha, int accumulate) {
     GroupTask t;
     std::memset(&t, 0, sizeof t);
     const int m0 = c->groups[g].first, m1 = c->groups[g].second;
@@ -703,17 +739,18 @@ GroupTask make_task(const mg_ctx *c, int g, const std::vector<int64_t> &sz, cons
 
 // Calc_DHD_B (cmf2_derivatives.f90:1-80): HB = sum_s sum_k B_s x_k Hsum(s,k)
 // + sum_chains ae (x_k O_k) B_rhs.
-std::unique_ptr<Plan> build_apply_h(mg_ctx *c, const double2 *B, double2 *HB) {
+// B, HB and the chain temporaries hold the work-tensor storage type.
+std::unique_ptr<Plan> build_apply_h(mg_ctx *c, const void *B, void *HB) {
     PlanBuilder pb;
     const int G = (int)c->groups.size();
     const size_t mixed = (size_t)c->mixed;
-    double2 *ctmp = c->dchain.as<double2>();
-    std::vector<const double2 *> cur(c->chains.size());
+    void *ctmp = c->dchain.p;
+    std::vector<const void *> cur(c->chains.size());
     std::vector<std::vector<int64_t>> csz(c->chains.size());
     std::vector<int> last(c->chains.size(), -1), parity(c->chains.size(), 0);
     for (size_t k = 0; k < c->chains.size(); k++) {
         const Chain &ch = c->chains[k];
-        cur[k] = B + c->off[ch.rhs];
+        cur[k] = c->at(B, c->off[ch.rhs]);
         csz[k].resize(c->nm);
         for (int m = 0; m < c->nm; m++) csz[k][m] = c->NS(m, ch.rhs);
         for (auto &o : ch.ops)
@@ -731,10 +768,10 @@ std::unique_ptr<Plan> build_apply_h(mg_ctx *c, const double2 *B, double2 *HB) {
             for (int m = c->groups[g].first; m < c->groups[g].second; m++) {
                 const int row = s * c->nm + m;
                 if (c->hs_rowptr[row + 1] == c->hs_rowptr[row]) continue;
-                ops.push_back({m, c->hsum(s, m), 0, (int)sz[m], (int)sz[m]});
+                ops.push_back({m, c->hsumop(s, m), 0, (int)sz[m], (int)sz[m]});
             }
             if (ops.empty() && g > 0) continue;
-            tasks.push_back(make_task(c, g, sz, ops, 1, B + c->off[s], HB + c->off[s], 1.0, g > 0 ? 1 : 0));
+            tasks.push_back(make_task(c, g, sz, ops, 1, c->at(B, c->off[s]), c->at(HB, c->off[s]), 1.0, g > 0 ? 1 : 0));
         }
         pb.phase(tasks);
         tasks.clear();
@@ -745,9 +782,9 @@ std::unique_ptr<Plan> build_apply_h(mg_ctx *c, const double2 *B, double2 *HB) {
             std::vector<OpSpec> ops;
             for (auto &o : ch.ops)
                 if (o.mode >= c->groups[g].first && o.mode < c->groups[g].second)
-                    ops.push_back({o.mode, c->h1(o.mode, o.op), 0, (int)csz[k][o.mode], (int)c->NS(o.mode, ch.lhs)});
+                    ops.push_back({o.mode, c->h1op(o.mode, o.op), 0, (int)csz[k][o.mode], (int)c->NS(o.mode, ch.lhs)});
             if (ops.empty()) continue;
-            double2 *Y = ctmp + mixed * (2 * k + parity[k]);
+            void *Y = c->at(ctmp, (int64_t)(mixed * (2 * k + parity[k])));
             tasks.push_back(make_task(c, g, csz[k], ops, 0, cur[k], Y, 1.0, 0));
             for (auto &o : ops) csz[k][o.mode] = o.nout;
             cur[k] = Y;
@@ -770,12 +807,12 @@ std::unique_ptr<Plan> build_apply_h(mg_ctx *c, const double2 *B, double2 *HB) {
             std::vector<OpSpec> ops;
             for (auto &o : ch.ops)
                 if (o.mode >= c->groups[g].first && o.mode < c->groups[g].second)
-                    ops.push_back({o.mode, c->h1(o.mode, o.op), 0, (int)csz[k][o.mode], (int)c->NS(o.mode, ch.lhs)});
-            tasks.push_back(make_task(c, g, csz[k], ops, 0, cur[k], HB + c->off[ch.lhs], ch.ae, 1));
+                    ops.push_back({o.mode, c->h1op(o.mode, o.op), 0, (int)csz[k][o.mode], (int)c->NS(o.mode, ch.lhs)});
+            tasks.push_back(make_task(c, g, csz[k], ops, 0, cur[k], c->at(HB, c->off[ch.lhs]), ch.ae, 1));
         }
         pb.phase(tasks);
     }
-    return pb.finish();
+    return pb.finish(c->prec);
 }
 
 const Plan &get_plan(mg_ctx *c, const PlanKey &key, std::unique_ptr<Plan> (*build)(mg_ctx *, const PlanKey &)) {
@@ -787,10 +824,10 @@ const Plan &get_plan(mg_ctx *c, const PlanKey &key, std::unique_ptr<Plan> (*buil
     return ref;
 }
 
-void apply_h(mg_ctx *c, const double2 *B, double2 *HB) {
+void apply_h(mg_ctx *c, const void *B, void *HB) {
     PlanKey key{(const void *)1, B, HB, nullptr};
     const Plan &p = get_plan(c, key, [](mg_ctx *cc, const PlanKey &k) {
-        return build_apply_h(cc, (const double2 *)k[1], (double2 *)k[2]);
+        return build_apply_h(cc, k[1], (void *)k[2]);
     });
     run_plan(c, p);
     check_launch();
@@ -820,7 +857,7 @@ std::unique_ptr<Plan> build_apply_kron(mg_ctx *c, const double2 *X, double2 *Y, 
         }
         pb.phase(tasks);
     }
-    return pb.finish();
+    return pb.finish(0);
 }
 
 void apply_kron(mg_ctx *c, const double2 *X, double2 *Y, const double2 *mats) {
@@ -832,20 +869,20 @@ void apply_kron(mg_ctx *c, const double2 *X, double2 *Y, const double2 *mats) {
     check_launch();
 }
 
-double2 dot(mg_ctx *c, const double2 *x, const double2 *y, int64_t n) {
+double2 dot(mg_ctx *c, const void *x, const void *y, int64_t n, int prec = 0) {
     double *part = c->dpartials.as<double>();
     double2 *out = reinterpret_cast<double2 *>(c->dscal.as<double>() + 8);
-    launch_dot(x, y, n, part, c->nblocks_red, out, c->stream);
+    launch_dot(x, y, n, part, c->nblocks_red, out, prec, c->stream);
     double2 h;
     HIP_CHECK(hipMemcpyAsync(&h, out, sizeof h, hipMemcpyDeviceToHost, c->stream));
     HIP_CHECK(hipStreamSynchronize(c->stream));
     return h;
 }
 
-double norm2(mg_ctx *c, const double2 *x, int64_t n) {
+double norm2(mg_ctx *c, const void *x, int64_t n, int prec = 0) {
     double *part = c->dpartials.as<double>();
     double *out = c->dscal.as<double>() + 12;
-    launch_norm2(x, n, part, c->nblocks_red, out, c->stream);
+    launch_norm2(x, n, part, c->nblocks_red, out, prec, c->stream);
     double h;
     HIP_CHECK(hipMemcpyAsync(&h, out, sizeof h, hipMemcpyDeviceToHost, c->stream));
     HIP_CHECK(hipStreamSynchronize(c->stream));
@@ -905,6 +942,10 @@ void calc_h1mode(mg_ctx *c, const double2 *psi) {
     run_spf_tasks(c, PlanKey{(const void *)3, psi, nullptr, nullptr}, build_h1mode);
     launch_hsum(c->dhsum.as<double2>(), c->dh1.as<double2>(), c->dhs_rowptr.as<int>(), c->dhs_src.as<int>(),
                 c->dhs_coef.as<double2>(), c->ns * c->nm, c->nspfmax, c->stream);
+    if (c->prec) {
+        launch_convert(c->dh1f.p, 1, c->dh1.p, 0, (int64_t)(c->dh1.bytes / sizeof(double2)), c->stream);
+        launch_convert(c->dhsumf.p, 1, c->dhsum.p, 0, (int64_t)(c->dhsum.bytes / sizeof(double2)), c->stream);
+    }
 }
 
 // out(m, s) = <phi1_(m,s) | phi2_(m,s)>; sym for phi1 == phi2 (zherk).
@@ -976,7 +1017,7 @@ void calc_rho(mg_ctx *c, const double2 *A) {
             g.alpha = make_double2(1, 0);
             g.accumulate = 0;
             g.hermitian = 1;
-            launch_gram(g, c->dpartials.as<double2>(), c->nblocks_red, c->nspfmax, c->stream);
+            launch_gram(g, c->dpartials.as<double2>(), c->nblocks_red, c->nspfmax, 0, c->stream);
         }
     // rho_ave for single-set modes (gmctdh_objects.f90:464-474)
     std::vector<int> rowptr(1, 0), src;
@@ -1002,14 +1043,20 @@ void calc_rho(mg_ctx *c, const double2 *A) {
 
 // Calc_MeanField, Hamiltonian part (gmctdh_objects.f90:518-605) + Calc_rhom1MF
 // (:1147-1194).  Requires calc_rho(A) and calc_h1mode(psi).
-void calc_meanfield(mg_ctx *c, const double2 *A) {
-    const size_t nn = (size_t)c->nspfmax * c->nspfmax;
+void calc_meanfield(mg_ctx *c, const double2 *A64) {
     // one-mode same-state terms: MF += ae * rho  (all other modes are identity)
     launch_hsum(c->dmf.as<double2>(), c->drho.as<double2>(), c->dmf1_rowptr.as<int>(), c->dmf1_src.as<int>(),
                 c->dmf1_coef.as<double2>(), c->nm * c->nopmax, c->nspfmax, c->stream);
     const size_t mixed = (size_t)c->mixed;
-    double2 *AL = c->dmfbuf.as<double2>();
-    double2 *AR = AL + mixed * c->nm;
+    // the chains run in the work-tensor type: the mixed form multiplies an
+    // FP32 copy of A; the Gram contractions accumulate in FP64 either way
+    const void *A = A64;
+    if (c->prec && !c->mf_chain_terms.empty()) {
+        launch_convert(c->dAf.p, 1, A64, 0, c->nconf, c->stream);
+        A = c->dAf.p;
+    }
+    void *AL = c->dmfbuf.p;
+    void *AR = c->at(AL, (int64_t)(mixed * c->nm));
     for (size_t ti = 0; ti < c->mf_chain_terms.size(); ti++) {
         const int t = c->mf_chain_terms[ti];
         const int partner = c->mf_partner[ti];
@@ -1017,10 +1064,10 @@ void calc_meanfield(mg_ctx *c, const double2 *A) {
         const int a = (int)c->lhs[t] - 1, b = (int)c->rhs[t] - 1;
         const cplx ae = c->ae[t];
         // right chain R_i (modes f-1 .. 0), pointers alias on identity modes
-        std::vector<const double2 *> R(c->nm);
+        std::vector<const void *> R(c->nm);
         std::vector<int64_t> sz(c->nm);
         for (int m = 0; m < c->nm; m++) sz[m] = c->NS(m, b);
-        R[0] = A + c->off[b];
+        R[0] = c->at(A, c->off[b]);
         for (int i = 1; i < c->nm; i++) {
             const int jm = c->nm - i;
             const int op = c->iOper[jm + (size_t)c->nm * t];
@@ -1030,22 +1077,22 @@ void calc_meanfield(mg_ctx *c, const double2 *A) {
             for (int q = 0; q < jm; q++) ll *= sz[q];
             for (int q = jm + 1; q < c->nm; q++) nl *= sz[q];
             mt.X = R[i - 1];
-            mt.Y = AR + mixed * i;
+            mt.Y = c->at(AR, (int64_t)(mixed * i));
             mt.ll = ll;
             mt.nl = nl;
             mt.nin = (int)sz[jm];
             mt.nout = (int)c->NS(jm, a);
-            mt.M = c->h1(jm, op);
+            mt.M = c->h1op(jm, op);
             mt.ld = c->nspfmax;
             mt.adj = 0;
-            launch_mode_product(mt, c->nspfmax, c->stream);
+            launch_mode_product(mt, c->nspfmax, c->prec, c->stream);
             sz[jm] = mt.nout;
             R[i] = mt.Y;
         }
         // left chain L_i (modes 0 .. f-2) with h^H
-        std::vector<const double2 *> Lc(c->nm);
+        std::vector<const void *> Lc(c->nm);
         for (int m = 0; m < c->nm; m++) sz[m] = c->NS(m, a);
-        Lc[0] = A + c->off[a];
+        Lc[0] = c->at(A, c->off[a]);
         for (int i = 1; i < c->nm; i++) {
             const int m = i - 1;
             const int op = c->iOper[m + (size_t)c->nm * t];
@@ -1055,15 +1102,15 @@ void calc_meanfield(mg_ctx *c, const double2 *A) {
             for (int q = 0; q < m; q++) ll *= sz[q];
             for (int q = m + 1; q < c->nm; q++) nl *= sz[q];
             mt.X = Lc[i - 1];
-            mt.Y = AL + mixed * i;
+            mt.Y = c->at(AL, (int64_t)(mixed * i));
             mt.ll = ll;
             mt.nl = nl;
             mt.nin = (int)sz[m];
             mt.nout = (int)c->NS(m, b);
-            mt.M = c->h1(m, op);
+            mt.M = c->h1op(m, op);
             mt.ld = c->nspfmax;
             mt.adj = 1;
-            launch_mode_product(mt, c->nspfmax, c->stream);
+            launch_mode_product(mt, c->nspfmax, c->prec, c->stream);
             sz[m] = mt.nout;
             Lc[i] = mt.Y;
         }
@@ -1087,18 +1134,17 @@ void calc_meanfield(mg_ctx *c, const double2 *A) {
             if (partner < 0) {
                 g.G = c->mf(m, op);
                 g.accumulate = 1;
-                launch_gram(g, c->dpartials.as<double2>(), c->nblocks_red, c->nspfmax, c->stream);
+                launch_gram(g, c->dpartials.as<double2>(), c->nblocks_red, c->nspfmax, c->prec, c->stream);
             } else {
                 double2 *G = c->dGtmp.as<double2>();
                 g.G = G;
                 g.accumulate = 0;
-                launch_gram(g, c->dpartials.as<double2>(), c->nblocks_red, c->nspfmax, c->stream);
+                launch_gram(g, c->dpartials.as<double2>(), c->nblocks_red, c->nspfmax, c->prec, c->stream);
                 const int op2 = c->iOper[m + (size_t)c->nm * partner];
                 launch_mf_accum(c->mf(m, op), c->mf(m, op2), G, g.n1, g.n2, c->nspfmax, c->stream);
             }
         }
     }
-    (void)nn;
     launch_rmf(c->drho.as<double2>(), c->dmf.as<double2>(), c->drmf.as<double2>(), c->drmf_jobs.as<int>(),
                c->n_rmf_jobs, c->nspfmax, c->nopmax, c->eps_rho, c->stream);
     check_launch();
@@ -1177,29 +1223,36 @@ void tridiag_eig(int n, const double *alpha, const double *beta, double *w, doub
     for (int i = 0; i < n; i++) w[i] = A[i + n * i];
 }
 
+// B0 and BT are FP64.  In the mixed form the first Krylov vector is kept in
+// FP64 (dV0) beside its FP32 copy; H.A and the vectors v_1.. are FP32, so the
+// FP32 rounding enters the result only through the coefficients c_1..
+// (|c_i| ~ (beta dt)^i), not through the O(1) coefficient of v_0.
 int lanczos(mg_ctx *c, const double2 *B0, double2 *BT, double dt, bool build) {
     const int64_t N = c->nconf;
-    double2 *BK = c->dBK.as<double2>();
+    const int pr = c->prec;
+    void *BK = c->dBK.p;
     double *dsc = c->dscal.as<double>();
     double *part = c->dpartials.as<double>();
     const double eps_B = 1e-8;
     if (build) {
         double alpha[kNKrylovMax], beta[kNKrylovMax];
-        HIP_CHECK(hipMemcpyAsync(BK, B0, sizeof(double2) * N, hipMemcpyDeviceToDevice, c->stream));
-        launch_norm2(BK, N, part, c->nblocks_red, dsc + 0, c->stream);
-        launch_scale(BK, dsc + 0, N, c->stream);
+        double2 *v0 = pr ? c->dV0.as<double2>() : c->dBK.as<double2>();
+        HIP_CHECK(hipMemcpyAsync(v0, B0, sizeof(double2) * N, hipMemcpyDeviceToDevice, c->stream));
+        launch_norm2(v0, N, part, c->nblocks_red, dsc + 0, 0, c->stream);
+        launch_scale(v0, dsc + 0, N, 0, c->stream);
+        if (pr) launch_convert(BK, 1, v0, 0, N, c->stream);
         int j = 1;      // number of basis vectors whose H-image is being formed
         double error = 0.0;
         for (;;) {
-            double2 *vj = BK + (size_t)N * (j - 1), *w = BK + (size_t)N * j;
-            const double2 *vp = (j >= 2) ? BK + (size_t)N * (j - 2) : nullptr;
+            void *vj = c->at(BK, N * (j - 1)), *w = c->at(BK, N * j);
+            const void *vp = (j >= 2) ? c->at(BK, N * (j - 2)) : nullptr;
             if (j >= 2 && j == kNKrylovMax) throw HipError{"Maximum number of Krylov vectors reached"};
             apply_h(c, vj, w);
             double2 *alpha_d = reinterpret_cast<double2 *>(dsc + 2);
-            launch_dot(vj, w, N, part, c->nblocks_red, alpha_d, c->stream);
+            launch_dot(vj, w, N, part, c->nblocks_red, alpha_d, pr, c->stream);
             launch_lanczos_update(w, vj, vp, dsc + 2, j >= 2 ? beta[j - 2] : 0.0, N, part, c->nblocks_red, dsc + 4,
-                                  c->stream);
-            launch_scale(w, dsc + 4, N, c->stream);
+                                  pr, c->stream);
+            launch_scale(w, dsc + 4, N, pr, c->stream);
             double h[3];
             HIP_CHECK(hipMemcpyAsync(h, dsc + 2, sizeof(h), hipMemcpyDeviceToHost, c->stream));
             HIP_CHECK(hipStreamSynchronize(c->stream));
@@ -1231,9 +1284,10 @@ int lanczos(mg_ctx *c, const double2 *B0, double2 *BT, double dt, bool build) {
     }
     double2 *dcoef = reinterpret_cast<double2 *>(dsc + 16);
     HIP_CHECK(hipMemcpyAsync(dcoef, coef.data(), sizeof(double2) * n, hipMemcpyHostToDevice, c->stream));
-    launch_combine(BT, BK, N, dcoef, n, N, c->stream);
-    launch_norm2(BT, N, part, c->nblocks_red, dsc + 6, c->stream);
-    launch_scale(BT, dsc + 6, N, c->stream);
+    if (pr) launch_combine_mixed(BT, c->dV0.as<double2>(), c->dBK.as<float2>(), N, dcoef, n, N, c->stream);
+    else launch_combine(BT, BK, N, dcoef, n, N, 0, c->stream);
+    launch_norm2(BT, N, part, c->nblocks_red, dsc + 6, 0, c->stream);
+    launch_scale(BT, dsc + 6, N, 0, c->stream);
     HIP_CHECK(hipStreamSynchronize(c->stream));
     return n;
 }
@@ -1274,7 +1328,7 @@ void cmf2_step(mg_ctx *c, double tt, double htry, double *hdid_out, double *hnex
         propagate_spf(c, pH2, 0.5 * hdid);
         calc_h1mode(c, pH2);
         const int nk2 = lanczos(c, BH, B02, -0.5 * hdid, true);
-        launch_axpby(B02, B0, -1.0, 1.0, N, c->stream);
+        launch_axpby(B02, B0, -1.0, 1.0, N, 0, c->stream);
         const double err_A = std::sqrt(norm2(c, B02, N)) * 0.5;
         const double error = std::max(err_A, err_psi) / c->eps_int;
         const bool reject = (forced >= 0) ? (forced == 0) : (error > 1.0);
@@ -1301,8 +1355,15 @@ void observables(mg_ctx *c, double t, mg_output *out, double *pops) {
     double2 *A = c->dA.as<double2>(), *psi = c->dpsi.as<double2>();
     calc_rho(c, A);
     calc_h1mode(c, psi);
-    apply_h(c, A, c->dOut.as<double2>());
-    const double energy = dot(c, A, c->dOut.as<double2>(), c->nconf).x;
+    double energy;
+    if (c->prec) {
+        launch_convert(c->dAf.p, 1, A, 0, c->nconf, c->stream);
+        apply_h(c, c->dAf.p, c->dYf.p);
+        energy = dot(c, c->dAf.p, c->dYf.p, c->nconf, 1).x;
+    } else {
+        apply_h(c, A, c->dOut.p);
+        energy = dot(c, A, c->dOut.p, c->nconf).x;
+    }
     const size_t nn = (size_t)c->nspfmax * c->nspfmax;
     std::vector<double2> r(nn * c->nm * c->ns);
     HIP_CHECK(hipMemcpyAsync(r.data(), c->drho.p, sizeof(double2) * r.size(), hipMemcpyDeviceToHost, c->stream));
@@ -1349,9 +1410,11 @@ int mg_create(const mg_problem *prob, int64_t precision, mg_ctx **out) {
     if (!out || !prob) return -1;
     *out = nullptr;
     mg_ctx *c = new mg_ctx();
-    c->precision = (int)precision;
+    c->prec = (int)precision;
+    c->es = precision ? sizeof(float2) : sizeof(double2);
     int rc = guarded(c, [&] {
-        if (precision != 0) throw HipError{"only precision 0 (complex FP64) is available"};
+        if (precision != 0 && precision != 1)
+            throw HipError{"precision must be 0 (complex FP64) or 1 (mixed complex FP32/FP64)"};
         int ndev = 0;
         HIP_CHECK(hipGetDeviceCount(&ndev));
         if (ndev < 1) throw HipError{"no HIP device"};
@@ -1408,8 +1471,8 @@ int mg_apply_h_profile(mg_ctx *c, const void *dB, void *dHB, double *kernel_ms, 
         c->prof = true;
         c->prof_ms = 0.0;
         c->prof_launches = 0;
-        const double2 *in = dB ? static_cast<const double2 *>(dB) : c->dIn.as<double2>();
-        double2 *out = dHB ? static_cast<double2 *>(dHB) : c->dOut.as<double2>();
+        const void *in = dB ? dB : (c->prec ? c->dXf.p : c->dIn.p);
+        void *out = dHB ? dHB : (c->prec ? c->dYf.p : c->dOut.p);
         try {
             apply_h(c, in, out);
         } catch (...) {
@@ -1480,7 +1543,13 @@ int mg_calc_meanfield(mg_ctx *c, double *mf_out, double *rmf_out) {
 int mg_apply_h(mg_ctx *c, const double *B, double *HB) {
     return guarded(c, [&] {
         HIP_CHECK(hipMemcpyAsync(c->dIn.p, B, sizeof(double2) * c->nconf, hipMemcpyHostToDevice, c->stream));
-        apply_h(c, c->dIn.as<double2>(), c->dOut.as<double2>());
+        if (c->prec) {
+            launch_convert(c->dXf.p, 1, c->dIn.p, 0, c->nconf, c->stream);
+            apply_h(c, c->dXf.p, c->dYf.p);
+            launch_convert(c->dOut.p, 0, c->dYf.p, 1, c->nconf, c->stream);
+        } else {
+            apply_h(c, c->dIn.p, c->dOut.p);
+        }
         HIP_CHECK(hipMemcpyAsync(HB, c->dOut.p, sizeof(double2) * c->nconf, hipMemcpyDeviceToHost, c->stream));
         HIP_CHECK(hipStreamSynchronize(c->stream));
     });
@@ -1489,7 +1558,7 @@ int mg_apply_h(mg_ctx *c, const double *B, double *HB) {
 int mg_apply_h_device(mg_ctx *c, const void *dB, void *dHB) {
     return guarded(c, [&] {
         HIP_CHECK(hipEventRecord(c->ev0, c->stream));
-        apply_h(c, static_cast<const double2 *>(dB), static_cast<double2 *>(dHB));
+        apply_h(c, dB, dHB);
         HIP_CHECK(hipEventRecord(c->ev1, c->stream));
         HIP_CHECK(hipEventSynchronize(c->ev1));
         float ms = 0.f;
@@ -1520,9 +1589,10 @@ int mg_apply_h_stats(mg_ctx *c, double *bytes_alg, double *flops_alg) {
                 sz[o.mode] = (int64_t)nout;
             }
         }
-        if (bytes_alg) *bytes_alg = 2.0 * 16.0 * (double)c->nconf;
+        const double by = 2.0 * (double)c->es * (double)c->nconf;
+        if (bytes_alg) *bytes_alg = by;
         if (flops_alg) *flops_alg = fl;
-        c->bytes_alg = 2.0 * 16.0 * (double)c->nconf;
+        c->bytes_alg = by;
         c->flops_alg = fl;
     });
 }

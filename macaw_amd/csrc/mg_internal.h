@@ -56,8 +56,8 @@ __host__ __device__ inline uint32_t fdiv(uint32_t n, const FastDiv &f) {
 //   kind 1 (sum):   brick <- sum_q brick x_{opmode[q]} M_q   (square ops)
 // Y[...] = alpha * brick  (accumulate = 0)  or  Y += alpha * brick (1).
 struct GroupTask {
-    const double2 *X;
-    double2 *Y;
+    const void *X;               // storage type: double2 (FP64) or float2 (mixed)
+    void *Y;
     int64_t L, Lt, R, Rt;        // Rt consecutive outer indices per brick (when Lt == L)
     int inplace;                 // chain of square ops: one LDS buffer, fibers updated in place
     int ng;
@@ -69,7 +69,7 @@ struct GroupTask {
     int opadj[kMaxOpsPerTask];   // 1: use conj-transpose of the matrix
     int opnin[kMaxOpsPerTask];   // columns of the effective op matrix
     int opnout[kMaxOpsPerTask];  // rows of the effective op matrix
-    const double2 *opmat[kMaxOpsPerTask];
+    const void *opmat[kMaxOpsPerTask];   // same type as X
     int ld;                      // leading dimension of the op matrices
     double2 alpha;
     int accumulate;
@@ -81,11 +81,11 @@ struct GroupTask {
 
 // Single-mode product: Y[l, i, r] = sum_j M[i, j] X[l, j, r]  (adj: conj(M[j, i])).
 struct ModeTask {
-    const double2 *X;
-    double2 *Y;
+    const void *X;               // storage type
+    void *Y;
     int64_t ll, nl;
     int nin, nout;
-    const double2 *M;
+    const void *M;               // storage type
     int ld;
     int adj;
     FastDiv fd_ll;               // set by launch_mode_product
@@ -93,7 +93,7 @@ struct ModeTask {
 
 // Hole contraction: G[j1 + ld*j2] (+)= alpha * sum_{l,r} conj(X[l,j1,r]) Y[l,j2,r].
 struct GramTask {
-    const double2 *X, *Y;
+    const void *X, *Y;           // storage type; G accumulates in FP64
     int64_t ll, nl;
     int n1, n2;
     double2 *G;
@@ -155,24 +155,28 @@ struct SpfParams {
     int max_attempts;
 };
 
-// Host-side launchers (mg_kernels.hip).
+// Host-side launchers (mg_kernels.hip).  prec: 0 = complex FP64 storage,
+// 1 = complex FP32 storage (reductions always FP64).
 void launch_group_tasks(const GroupTask *d_tasks, int ntasks, int64_t max_blocks_per_task,
-                        int nmax, int brick, int nbuf, hipStream_t s);
-void launch_mode_product(const ModeTask &t, int nmax, hipStream_t s);
-void launch_gram(const GramTask &t, double2 *partials, int nblocks, int nmax, hipStream_t s);
+                        int nmax, int brick, int nbuf, int prec, hipStream_t s);
+void launch_mode_product(const ModeTask &t, int nmax, int prec, hipStream_t s);
+void launch_gram(const GramTask &t, double2 *partials, int nblocks, int nmax, int prec, hipStream_t s);
 void launch_spf_tasks(const SpfTask *d_tasks, int ntasks, DofTables dt, int max_ng, hipStream_t s);
 void launch_fill(double2 *p, int64_t n, double2 v, hipStream_t s);
-void launch_axpby(double2 *y, const double2 *x, double a, double b, int64_t n, hipStream_t s);
-void launch_dot(const double2 *x, const double2 *y, int64_t n, double *partials, int nblocks,
-                double2 *out, hipStream_t s);
-void launch_lanczos_update(double2 *w, const double2 *vj, const double2 *vprev, const double *alpha,
+void launch_axpby(void *y, const void *x, double a, double b, int64_t n, int prec, hipStream_t s);
+void launch_dot(const void *x, const void *y, int64_t n, double *partials, int nblocks,
+                double2 *out, int prec, hipStream_t s);
+void launch_lanczos_update(void *w, const void *vj, const void *vprev, const double *alpha,
                            double beta_prev, int64_t n, double *partials, int nblocks, double *norm2,
-                           hipStream_t s);
-void launch_scale(double2 *x, const double *norm2, int64_t n, hipStream_t s);
-void launch_combine(double2 *out, const double2 *basis, int64_t stride, const double2 *coef, int k,
-                    int64_t n, hipStream_t s);
-void launch_norm2(const double2 *x, int64_t n, double *partials, int nblocks, double *out,
+                           int prec, hipStream_t s);
+void launch_scale(void *x, const double *norm2, int64_t n, int prec, hipStream_t s);
+void launch_combine(void *out, const void *basis, int64_t stride, const double2 *coef, int k,
+                    int64_t n, int prec, hipStream_t s);
+void launch_combine_mixed(double2 *out, const double2 *v0, const float2 *basis, int64_t stride,
+                          const double2 *coef, int k, int64_t n, hipStream_t s);
+void launch_norm2(const void *x, int64_t n, double *partials, int nblocks, double *out, int prec,
                   hipStream_t s);
+void launch_convert(void *dst, int pdst, const void *src, int psrc, int64_t n, hipStream_t s);
 void launch_hsum(double2 *hsum, const double2 *h1, const int *rowptr, const int *src,
                  const double2 *coef, int nmat, int nmax, hipStream_t s);
 int group_capacity(int brick);

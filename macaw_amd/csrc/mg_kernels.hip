@@ -14,6 +14,8 @@
 
 #include <math.h>
 
+#include <algorithm>
+
 namespace mg {
 
 __device__ __forceinline__ double2 zero2() { return make_double2(0.0, 0.0); }
@@ -25,15 +27,43 @@ __device__ __forceinline__ void cfma(double2 &acc, double2 m, double2 x) {
     acc.y = fma(m.y, x.x, acc.y);
 }
 
+// Complex FP32: four FMAs per multiply-add (on the VALU the 3-multiply form
+// needs more instructions than this).
+__device__ __forceinline__ void cfma(float2 &acc, float2 m, float2 x) {
+    acc.x = fmaf(m.x, x.x, acc.x);
+    acc.x = fmaf(-m.y, x.y, acc.x);
+    acc.y = fmaf(m.x, x.y, acc.y);
+    acc.y = fmaf(m.y, x.x, acc.y);
+}
+
+// Storage-type helpers: complex FP64 (double2) or complex FP32 (float2).
+template <class T> __device__ __forceinline__ T zeroT();
+template <> __device__ __forceinline__ double2 zeroT<double2>() { return make_double2(0.0, 0.0); }
+template <> __device__ __forceinline__ float2 zeroT<float2>() { return make_float2(0.f, 0.f); }
+__device__ __forceinline__ double2 to_d2(double2 a) { return a; }
+__device__ __forceinline__ double2 to_d2(float2 a) { return make_double2((double)a.x, (double)a.y); }
+template <class T> __device__ __forceinline__ T from_d2(double2 a);
+template <> __device__ __forceinline__ double2 from_d2<double2>(double2 a) { return a; }
+template <> __device__ __forceinline__ float2 from_d2<float2>(double2 a) { return make_float2((float)a.x, (float)a.y); }
+__device__ __forceinline__ float2 cmulT(float2 a, float2 b) {
+    return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ double2 cmulT(double2 a, double2 b) { return cmul(a, b); }
+__device__ __forceinline__ float2 caddT(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ double2 caddT(double2 a, double2 b) { return cadd(a, b); }
+__device__ __forceinline__ float2 cconjT(float2 a) { return make_float2(a.x, -a.y); }
+__device__ __forceinline__ double2 cconjT(double2 a) { return cconj(a); }
+
 // LDS index padding: one slot every 8 complex values breaks the bank
 // conflicts of contiguous fibers (stride-1 mode inside a brick).
 __device__ __forceinline__ int pad(int i) { return i + (i >> 3); }
 
 // ---------------------------------------------------------------------------
 // Group pass (see GroupTask).  LDS: [op matrices][buf0][buf1], each buffer
-// holding one padded brick; chain kind ping-pongs buf0/buf1, sum kind keeps
-// the input brick in buf0 and accumulates into buf1.
-template <int NMAX>
+// holding one padded brick; chain kind ping-pongs buf0/buf1 (or updates buf0
+// in place when every op is square), sum kind keeps the input brick in buf0
+// and accumulates into buf1.  T is the storage and arithmetic type.
+template <int NMAX, class T>
 __global__ void __launch_bounds__(256) k_group_run(const GroupTask *__restrict__ tasks, int capacity) {
     const GroupTask &t = tasks[blockIdx.y];
     const int64_t nlb = t.L / t.Lt, nrb = t.R / t.Rt;
@@ -43,10 +73,10 @@ __global__ void __launch_bounds__(256) k_group_run(const GroupTask *__restrict__
     const int64_t l0 = lb * t.Lt, r0 = rb * t.Rt;
     const int Lt = (int)t.Lt, Rt = (int)t.Rt;
     const int64_t L = t.L;
-    extern __shared__ double2 lds[];
-    double2 *mats = lds;
-    double2 *buf0 = lds + kMaxOpsPerTask * NMAX * NMAX;
-    double2 *buf1 = t.inplace ? buf0 : buf0 + capacity;
+    extern __shared__ char lds_raw[];
+    T *mats = reinterpret_cast<T *>(lds_raw);
+    T *buf0 = mats + kMaxOpsPerTask * NMAX * NMAX;
+    T *buf1 = t.inplace ? buf0 : buf0 + capacity;
     const int tid = threadIdx.x, nth = blockDim.x;
     int cur[kMaxGroupModes];
     int Min = 1, Mout = 1;
@@ -58,14 +88,14 @@ __global__ void __launch_bounds__(256) k_group_run(const GroupTask *__restrict__
     for (int q = 0; q < t.nops; q++) {
         for (int e = tid; e < NMAX * NMAX; e += nth) {
             int i = e % NMAX, j = e / NMAX;
-            const double2 *M = t.opmat[q];
-            double2 v = zero2();
-            if (i < t.opnout[q] && j < t.opnin[q]) v = t.opadj[q] ? cconj(M[j + t.ld * i]) : M[i + t.ld * j];
+            const T *M = static_cast<const T *>(t.opmat[q]);
+            T v = zeroT<T>();
+            if (i < t.opnout[q] && j < t.opnin[q]) v = t.opadj[q] ? cconjT(M[j + t.ld * i]) : M[i + t.ld * j];
             mats[q * NMAX * NMAX + i + NMAX * j] = v;
         }
     }
     // load the X brick: LDS index lt + Lt*(g + Min*rt)
-    const double2 *X = t.X;
+    const T *X = static_cast<const T *>(t.X);
     const int LM_in = Lt * Min;
     const int nin_tot = LM_in * Rt;
     for (int idx = tid; idx < nin_tot; idx += nth) {
@@ -74,7 +104,7 @@ __global__ void __launch_bounds__(256) k_group_run(const GroupTask *__restrict__
         buf0[pad(idx)] = X[l0 + lt + L * ((int64_t)g + (int64_t)Min * (r0 + rt))];
     }
     __syncthreads();
-    double2 *src = buf0, *dst = buf1;
+    T *src = buf0, *dst = buf1;
     if (t.kind == 0) {
         int tot = nin_tot;
         for (int q = 0; q < t.nops; q++) {
@@ -83,18 +113,18 @@ __global__ void __launch_bounds__(256) k_group_run(const GroupTask *__restrict__
             const FastDiv fds = t.fd_opstride[q];
             const int ni = cur[p], no = t.nout[p];
             const int nfib = tot / ni;
-            const double2 *Mq = mats + q * NMAX * NMAX;
+            const T *Mq = mats + q * NMAX * NMAX;
             for (int f = tid; f < nfib; f += nth) {
                 const int outer = (int)fdiv((uint32_t)f, fds), inner = f - outer * stride;
                 const int bi = inner + outer * stride * ni;
                 const int bo = inner + outer * stride * no;
-                double2 x[NMAX];
+                T x[NMAX];
 #pragma unroll
-                for (int j = 0; j < NMAX; j++) x[j] = (j < ni) ? src[pad(bi + j * stride)] : zero2();
+                for (int j = 0; j < NMAX; j++) x[j] = (j < ni) ? src[pad(bi + j * stride)] : zeroT<T>();
 #pragma unroll
                 for (int i = 0; i < NMAX; i++) {
                     if (i < no) {
-                        double2 acc = zero2();
+                        T acc = zeroT<T>();
 #pragma unroll
                         for (int j = 0; j < NMAX; j++)
                             if (j < ni) cfma(acc, Mq[i + NMAX * j], x[j]);
@@ -105,12 +135,11 @@ __global__ void __launch_bounds__(256) k_group_run(const GroupTask *__restrict__
             tot = tot / ni * no;
             cur[p] = no;
             __syncthreads();
-            double2 *sw = src; src = dst; dst = sw;
+            T *sw = src; src = dst; dst = sw;
         }
     } else {
-        // sum of square mode products; buf1 accumulates
-        double2 *acc_buf = buf1;
-        for (int idx = tid; idx < nin_tot; idx += nth) acc_buf[pad(idx)] = zero2();
+        T *acc_buf = buf1;
+        for (int idx = tid; idx < nin_tot; idx += nth) acc_buf[pad(idx)] = zeroT<T>();
         __syncthreads();
         for (int q = 0; q < t.nops; q++) {
             const int p = t.opmode[q];
@@ -118,17 +147,17 @@ __global__ void __launch_bounds__(256) k_group_run(const GroupTask *__restrict__
             const FastDiv fds = t.fd_opstride[q];
             const int n = cur[p];
             const int nfib = nin_tot / n;
-            const double2 *Mq = mats + q * NMAX * NMAX;
+            const T *Mq = mats + q * NMAX * NMAX;
             for (int f = tid; f < nfib; f += nth) {
                 const int outer = (int)fdiv((uint32_t)f, fds), inner = f - outer * stride;
                 const int bi = inner + outer * stride * n;
-                double2 x[NMAX];
+                T x[NMAX];
 #pragma unroll
-                for (int j = 0; j < NMAX; j++) x[j] = (j < n) ? src[pad(bi + j * stride)] : zero2();
+                for (int j = 0; j < NMAX; j++) x[j] = (j < n) ? src[pad(bi + j * stride)] : zeroT<T>();
 #pragma unroll
                 for (int i = 0; i < NMAX; i++) {
                     if (i < n) {
-                        double2 acc = acc_buf[pad(bi + i * stride)];
+                        T acc = acc_buf[pad(bi + i * stride)];
 #pragma unroll
                         for (int j = 0; j < NMAX; j++)
                             if (j < n) cfma(acc, Mq[i + NMAX * j], x[j]);
@@ -141,72 +170,81 @@ __global__ void __launch_bounds__(256) k_group_run(const GroupTask *__restrict__
         src = acc_buf;
     }
     // write Y
-    double2 *Y = t.Y;
+    T *Y = static_cast<T *>(t.Y);
     const int LM_out = Lt * Mout;
     const int nout_tot = LM_out * Rt;
-    const double2 alpha = t.alpha;
+    const T alpha = from_d2<T>(t.alpha);
     for (int idx = tid; idx < nout_tot; idx += nth) {
         const int rest = (int)fdiv((uint32_t)idx, t.fd_Lt), lt = idx - rest * Lt;
         const int rt = (int)fdiv((uint32_t)rest, t.fd_Mout), g = rest - rt * Mout;
         const int64_t a = l0 + lt + L * ((int64_t)g + (int64_t)Mout * (r0 + rt));
-        double2 v = cmul(alpha, src[pad(idx)]);
-        if (t.accumulate) v = cadd(Y[a], v);
+        T v = cmulT(alpha, src[pad(idx)]);
+        if (t.accumulate) v = caddT(Y[a], v);
         Y[a] = v;
     }
 }
 
-template <int NMAX>
+template <int NMAX, class T>
 static void launch_group_nmax(const GroupTask *d_tasks, int ntasks, int64_t max_blocks, int capacity, int nbuf,
                               hipStream_t s) {
     static bool attr_set = false;
-    const size_t shm = sizeof(double2) * (kMaxOpsPerTask * NMAX * NMAX + (size_t)nbuf * capacity);
+    const size_t shm = sizeof(T) * (kMaxOpsPerTask * NMAX * NMAX + (size_t)nbuf * capacity);
     if (!attr_set) {
-        (void)hipFuncSetAttribute((const void *)k_group_run<NMAX>, hipFuncAttributeMaxDynamicSharedMemorySize,
+        (void)hipFuncSetAttribute((const void *)k_group_run<NMAX, T>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                   160 * 1024);
         attr_set = true;
     }
-    hipLaunchKernelGGL(k_group_run<NMAX>, dim3((unsigned)max_blocks, (unsigned)ntasks), dim3(256), shm, s,
+    hipLaunchKernelGGL((k_group_run<NMAX, T>), dim3((unsigned)max_blocks, (unsigned)ntasks), dim3(256), shm, s,
                        d_tasks, capacity);
 }
 
 int group_capacity(int brick) { return brick + (brick >> 3) + 1; }
 
+template <class T>
+static void launch_group_T(const GroupTask *d_tasks, int ntasks, int64_t mb, int nmax, int capacity, int nbuf,
+                           hipStream_t s) {
+    if (nmax <= 4) launch_group_nmax<4, T>(d_tasks, ntasks, mb, capacity, nbuf, s);
+    else if (nmax <= 8) launch_group_nmax<8, T>(d_tasks, ntasks, mb, capacity, nbuf, s);
+    else launch_group_nmax<16, T>(d_tasks, ntasks, mb, capacity, nbuf, s);
+}
+
 // `brick`: the largest Lt * M * Rt (complex elements) of any task in this
 // launch; `nbuf`: 1 when every task is an in-place chain, else 2.
 void launch_group_tasks(const GroupTask *d_tasks, int ntasks, int64_t max_blocks_per_task, int nmax,
-                        int brick, int nbuf, hipStream_t s) {
+                        int brick, int nbuf, int prec, hipStream_t s) {
     if (ntasks <= 0 || max_blocks_per_task <= 0) return;
     const int capacity = group_capacity(brick);
-    if (nmax <= 4) launch_group_nmax<4>(d_tasks, ntasks, max_blocks_per_task, capacity, nbuf, s);
-    else if (nmax <= 8) launch_group_nmax<8>(d_tasks, ntasks, max_blocks_per_task, capacity, nbuf, s);
-    else launch_group_nmax<16>(d_tasks, ntasks, max_blocks_per_task, capacity, nbuf, s);
+    if (prec == 0) launch_group_T<double2>(d_tasks, ntasks, max_blocks_per_task, nmax, capacity, nbuf, s);
+    else launch_group_T<float2>(d_tasks, ntasks, max_blocks_per_task, nmax, capacity, nbuf, s);
 }
 
 // ---------------------------------------------------------------------------
 // Single-mode product straight from global memory (one thread per fiber).
-template <int NMAX>
+template <int NMAX, class T>
 __global__ void __launch_bounds__(256) k_mode_product(ModeTask t) {
-    __shared__ double2 M[NMAX * NMAX];
+    __shared__ T M[NMAX * NMAX];
+    const T *Mg = static_cast<const T *>(t.M);
     for (int e = threadIdx.x; e < NMAX * NMAX; e += blockDim.x) {
         int i = e % NMAX, j = e / NMAX;
-        double2 v = zero2();
-        if (i < t.nout && j < t.nin)
-            v = t.adj ? cconj(t.M[j + t.ld * i]) : t.M[i + t.ld * j];
+        T v = zeroT<T>();
+        if (i < t.nout && j < t.nin) v = t.adj ? cconjT(Mg[j + t.ld * i]) : Mg[i + t.ld * j];
         M[e] = v;
     }
     __syncthreads();
+    const T *X = static_cast<const T *>(t.X);
+    T *Y = static_cast<T *>(t.Y);
     const uint32_t nfib = (uint32_t)(t.ll * t.nl);
     for (uint32_t f = blockIdx.x * blockDim.x + threadIdx.x; f < nfib; f += gridDim.x * blockDim.x) {
         const uint32_t r = fdiv(f, t.fd_ll), l = f - r * (uint32_t)t.ll;
-        const double2 *x0 = t.X + l + t.ll * (int64_t)t.nin * r;
-        double2 *y0 = t.Y + l + t.ll * (int64_t)t.nout * r;
-        double2 x[NMAX];
+        const T *x0 = X + l + t.ll * (int64_t)t.nin * r;
+        T *y0 = Y + l + t.ll * (int64_t)t.nout * r;
+        T x[NMAX];
 #pragma unroll
-        for (int j = 0; j < NMAX; j++) x[j] = (j < t.nin) ? x0[t.ll * j] : zero2();
+        for (int j = 0; j < NMAX; j++) x[j] = (j < t.nin) ? x0[t.ll * j] : zeroT<T>();
 #pragma unroll
         for (int i = 0; i < NMAX; i++) {
             if (i < t.nout) {
-                double2 acc = zero2();
+                T acc = zeroT<T>();
 #pragma unroll
                 for (int j = 0; j < NMAX; j++)
                     if (j < t.nin) cfma(acc, M[i + NMAX * j], x[j]);
@@ -216,68 +254,111 @@ __global__ void __launch_bounds__(256) k_mode_product(ModeTask t) {
     }
 }
 
-void launch_mode_product(const ModeTask &t0, int nmax, hipStream_t s) {
+void launch_mode_product(const ModeTask &t0, int nmax, int prec, hipStream_t s) {
     ModeTask t = t0;
     t.fd_ll = make_fastdiv((uint32_t)t.ll);
     const int64_t nfib = t.ll * t.nl;
     int64_t blocks = (nfib + 255) / 256;
     if (blocks > 65536) blocks = 65536;
     if (blocks < 1) blocks = 1;
-    if (nmax <= 4) hipLaunchKernelGGL(k_mode_product<4>, dim3((unsigned)blocks), dim3(256), 0, s, t);
-    else if (nmax <= 8) hipLaunchKernelGGL(k_mode_product<8>, dim3((unsigned)blocks), dim3(256), 0, s, t);
-    else hipLaunchKernelGGL(k_mode_product<16>, dim3((unsigned)blocks), dim3(256), 0, s, t);
+    const dim3 g((unsigned)blocks), b(256);
+    if (prec == 0) {
+        if (nmax <= 4) hipLaunchKernelGGL((k_mode_product<4, double2>), g, b, 0, s, t);
+        else if (nmax <= 8) hipLaunchKernelGGL((k_mode_product<8, double2>), g, b, 0, s, t);
+        else hipLaunchKernelGGL((k_mode_product<16, double2>), g, b, 0, s, t);
+    } else {
+        if (nmax <= 4) hipLaunchKernelGGL((k_mode_product<4, float2>), g, b, 0, s, t);
+        else if (nmax <= 8) hipLaunchKernelGGL((k_mode_product<8, float2>), g, b, 0, s, t);
+        else hipLaunchKernelGGL((k_mode_product<16, float2>), g, b, 0, s, t);
+    }
 }
 
 // ---------------------------------------------------------------------------
-// Hole contraction, stage 1: per-block partial n1 x n2 sums (chunked to 8x8).
+// Hole contraction, stage 1: per-block partial n1 x n2 sums (chunked to 8x8),
+// accumulated in FP64 whatever the storage type.
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
     return v;
 }
 
-__global__ void __launch_bounds__(256) k_gram_partial(GramTask t, int j1o, int n1c, int j2o, int n2c,
+// The positions (l, r) of the contraction are cut into tiles of Lt x Rt
+// (Lt | ll, and Rt | nl when Lt == ll) so that each tile is a set of
+// contiguous runs in memory; a tile of X (n1c planes) and of Y (n2c planes) is
+// staged in LDS as [plane][position], converted to FP64 on use.  Wave w owns
+// the 4x4 sub-block (4*(w&1), 4*(w>>1)) of the 8x8 chunk and keeps 16 FP64
+// accumulators per lane; lanes stride over the tile's positions.
+struct GramTile {
+    int Lt, Rt;
+    int64_t ntl, ntiles;         // tiles along l, total
+    FastDiv fd_Lt, fd_n1c, fd_n2c;
+};
+
+template <class T>
+__device__ __forceinline__ void gram_load_tile(T *dst, const T *src, int64_t l0, int64_t r0, int64_t ll, int n,
+                                               int jo, int nc, const GramTile &g, FastDiv fd_nc) {
+    const int TP = g.Lt * g.Rt;
+    const int tot = TP * nc;
+    for (int idx = threadIdx.x; idx < tot; idx += blockDim.x) {
+        const int rest = (int)fdiv((uint32_t)idx, g.fd_Lt), lt = idx - rest * g.Lt;
+        const int rt = (int)fdiv((uint32_t)rest, fd_nc), a = rest - rt * nc;
+        dst[a * TP + lt + g.Lt * rt] = src[l0 + lt + ll * ((int64_t)(jo + a) + (int64_t)n * (r0 + rt))];
+    }
+}
+
+template <class T>
+__global__ void __launch_bounds__(256) k_gram_partial(GramTask t, int j1o, int n1c, int j2o, int n2c, GramTile g,
                                                       double2 *partials) {
-    double2 acc[8][8];
-#pragma unroll
-    for (int a = 0; a < 8; a++)
-#pragma unroll
-        for (int b = 0; b < 8; b++) acc[a][b] = zero2();
-    const uint32_t npos = (uint32_t)(t.ll * t.nl);
-    for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < npos; p += gridDim.x * blockDim.x) {
-        const uint32_t r = fdiv(p, t.fd_ll), l = p - r * (uint32_t)t.ll;
-        double2 x[8], y[8];
-        const double2 *xb = t.X + l + t.ll * ((int64_t)t.n1 * r + j1o);
-        const double2 *yb = t.Y + l + t.ll * ((int64_t)t.n2 * r + j2o);
-#pragma unroll
-        for (int a = 0; a < 8; a++) x[a] = (a < n1c) ? xb[t.ll * a] : zero2();
-#pragma unroll
-        for (int b = 0; b < 8; b++) y[b] = (b < n2c) ? yb[t.ll * b] : zero2();
-#pragma unroll
-        for (int a = 0; a < 8; a++)
-#pragma unroll
-            for (int b = 0; b < 8; b++) {
-                acc[a][b].x = fma(x[a].x, y[b].x, acc[a][b].x);
-                acc[a][b].x = fma(x[a].y, y[b].y, acc[a][b].x);
-                acc[a][b].y = fma(x[a].x, y[b].y, acc[a][b].y);
-                acc[a][b].y = fma(-x[a].y, y[b].x, acc[a][b].y);
-            }
-    }
-    __shared__ double2 red[4][64];
+    extern __shared__ char lds_raw[];
+    const int TP = g.Lt * g.Rt;
+    T *xs = reinterpret_cast<T *>(lds_raw);
+    const bool same = (t.X == t.Y) && (j1o == j2o);
+    T *ys = same ? xs : xs + n1c * TP;
+    const T *X = static_cast<const T *>(t.X);
+    const T *Yv = static_cast<const T *>(t.Y);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int A = (wave & 1) * 4, B = (wave >> 1) * 4;
+    const bool active = (A < n1c) && (B < n2c);
+    double2 acc[4][4];
 #pragma unroll
-    for (int a = 0; a < 8; a++)
+    for (int i = 0; i < 4; i++)
 #pragma unroll
-        for (int b = 0; b < 8; b++) {
-            double sx = wave_sum(acc[a][b].x), sy = wave_sum(acc[a][b].y);
-            if (lane == 0) red[wave][a + 8 * b] = make_double2(sx, sy);
+        for (int j = 0; j < 4; j++) acc[i][j] = zero2();
+    for (int64_t tile = blockIdx.x; tile < g.ntiles; tile += gridDim.x) {
+        const int64_t lb = tile % g.ntl, rb = tile / g.ntl;
+        const int64_t l0 = lb * g.Lt, r0 = rb * g.Rt;
+        gram_load_tile(xs, X, l0, r0, t.ll, t.n1, j1o, n1c, g, g.fd_n1c);
+        if (!same) gram_load_tile(ys, Yv, l0, r0, t.ll, t.n2, j2o, n2c, g, g.fd_n2c);
+        __syncthreads();
+        if (active) {
+            for (int q = lane; q < TP; q += 64) {
+                double2 x[4], y[4];
+#pragma unroll
+                for (int i = 0; i < 4; i++) x[i] = (A + i < n1c) ? to_d2(xs[(A + i) * TP + q]) : zero2();
+#pragma unroll
+                for (int j = 0; j < 4; j++) y[j] = (B + j < n2c) ? to_d2(ys[(B + j) * TP + q]) : zero2();
+#pragma unroll
+                for (int i = 0; i < 4; i++)
+#pragma unroll
+                    for (int j = 0; j < 4; j++) {
+                        acc[i][j].x = fma(x[i].x, y[j].x, acc[i][j].x);
+                        acc[i][j].x = fma(x[i].y, y[j].y, acc[i][j].x);
+                        acc[i][j].y = fma(x[i].x, y[j].y, acc[i][j].y);
+                        acc[i][j].y = fma(-x[i].y, y[j].x, acc[i][j].y);
+                    }
+            }
         }
-    __syncthreads();
-    if (threadIdx.x < 64) {
-        double2 v = zero2();
-        for (int w = 0; w < (int)(blockDim.x >> 6); w++) v = cadd(v, red[w][threadIdx.x]);
-        partials[(int64_t)threadIdx.x * gridDim.x + blockIdx.x] = v;   // entry-major
+        __syncthreads();
     }
+    if (!active) return;
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const double sx = wave_sum(acc[i][j].x), sy = wave_sum(acc[i][j].y);
+            if (lane == 0 && A + i < n1c && B + j < n2c)
+                partials[(int64_t)((A + i) + 8 * (B + j)) * gridDim.x + blockIdx.x] = make_double2(sx, sy);
+        }
 }
 
 // Stage 2: one wave per matrix entry sums that entry's block partials in a
@@ -314,18 +395,44 @@ __global__ void k_gram_hermitize(double2 *G, int n, int ld) {
     if (in && a > b) G[a + ld * b] = cconj(G[b + ld * a]);
 }
 
-void launch_gram(const GramTask &t0, double2 *partials, int nblocks, int nmax, hipStream_t s) {
+static int64_t largest_divisor_le(int64_t L, int64_t cap) {
+    if (cap >= L) return L;
+    for (int64_t d = cap; d >= 1; d--)
+        if (L % d == 0) return d;
+    return 1;
+}
+
+void launch_gram(const GramTask &t0, double2 *partials, int nblocks, int nmax, int prec, hipStream_t s) {
     (void)nmax;
     GramTask t = t0;
     t.fd_ll = make_fastdiv((uint32_t)t.ll);
+    // tile: ~256 positions (FP32) or 128 (FP64): 2 x 8 planes stay <= 32 KB
+    const int64_t target = prec ? 256 : 128;
+    GramTile g;
+    g.Lt = (int)largest_divisor_le(t.ll, target);
+    g.Rt = (g.Lt == t.ll) ? (int)largest_divisor_le(t.nl, std::max<int64_t>(1, target / t.ll)) : 1;
+    g.ntl = t.ll / g.Lt;
+    g.ntiles = g.ntl * (t.nl / g.Rt);
+    g.fd_Lt = make_fastdiv((uint32_t)g.Lt);
+    const int TP = g.Lt * g.Rt;
+    int nb = nblocks;
+    if (g.ntiles < nb) nb = (int)g.ntiles;
+    if (nb < 1) nb = 1;
     for (int j1o = 0; j1o < t.n1; j1o += 8) {
         const int n1c = (t.n1 - j1o) < 8 ? (t.n1 - j1o) : 8;
         for (int j2o = 0; j2o < t.n2; j2o += 8) {
             const int n2c = (t.n2 - j2o) < 8 ? (t.n2 - j2o) : 8;
-            hipLaunchKernelGGL(k_gram_partial, dim3(nblocks), dim3(256), 0, s, t, j1o, n1c, j2o, n2c,
-                               partials);
-            hipLaunchKernelGGL(k_gram_final, dim3(16), dim3(256), 0, s, t, j1o, n1c, j2o, n2c, partials,
-                               nblocks);
+            g.fd_n1c = make_fastdiv((uint32_t)n1c);
+            g.fd_n2c = make_fastdiv((uint32_t)n2c);
+            const bool same = (t.X == t.Y) && (j1o == j2o);
+            const size_t planes = (size_t)n1c + (same ? 0 : (size_t)n2c);
+            if (prec == 0)
+                hipLaunchKernelGGL(k_gram_partial<double2>, dim3(nb), dim3(256), sizeof(double2) * planes * TP, s, t,
+                                   j1o, n1c, j2o, n2c, g, partials);
+            else
+                hipLaunchKernelGGL(k_gram_partial<float2>, dim3(nb), dim3(256), sizeof(float2) * planes * TP, s, t,
+                                   j1o, n1c, j2o, n2c, g, partials);
+            hipLaunchKernelGGL(k_gram_final, dim3(16), dim3(256), 0, s, t, j1o, n1c, j2o, n2c, partials, nb);
         }
     }
     if (t.hermitian) hipLaunchKernelGGL(k_gram_hermitize, dim3(1), dim3(256), 0, s, t.G, t.n1, t.ld);
@@ -333,7 +440,7 @@ void launch_gram(const GramTask &t0, double2 *partials, int nblocks, int nmax, h
 
 // ---------------------------------------------------------------------------
 // SPF block products (Calc_h1mode DVR branch, hamilton.f90:433-485, and the
-// SPF overlap matrices of norm_wf / WF_Ovl).
+// SPF overlap matrices of norm_wf / WF_Ovl).  Always FP64.
 __device__ void apply_prim_block(const DofTables &dt, int mode, int k, int code, int nS, int nG,
                                  double2 *blk, double2 *tmp) {
     const int dof = dt.mode_dofs[mode * kMaxDofPerMode + k];
@@ -412,37 +519,45 @@ void launch_spf_tasks(const SpfTask *d_tasks, int ntasks, DofTables dt, int max_
 }
 
 // ---------------------------------------------------------------------------
-// Vector kernels (Lanczos, cmf2_integration.f90:30-103).
-__global__ void k_fill(double2 *p, int64_t n, double2 v) {
+// Vector kernels (Lanczos, cmf2_integration.f90:30-103): storage T, FP64
+// arithmetic and FP64 reductions.
+static int64_t grid_for(int64_t n) {
+    int64_t blocks = (n + 255) / 256;
+    if (blocks > 8192) blocks = 8192;
+    if (blocks < 1) blocks = 1;
+    return blocks;
+}
+
+template <class T>
+__global__ void k_fill(T *p, int64_t n, double2 v) {
+    const T vv = from_d2<T>(v);
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-        p[i] = v;
+        p[i] = vv;
 }
 
 void launch_fill(double2 *p, int64_t n, double2 v, hipStream_t s) {
-    int64_t blocks = (n + 255) / 256;
-    if (blocks > 8192) blocks = 8192;
-    if (blocks < 1) blocks = 1;
-    hipLaunchKernelGGL(k_fill, dim3((unsigned)blocks), dim3(256), 0, s, p, n, v);
+    hipLaunchKernelGGL(k_fill<double2>, dim3((unsigned)grid_for(n)), dim3(256), 0, s, p, n, v);
 }
 
-__global__ void k_axpby(double2 *y, const double2 *x, double a, double b, int64_t n) {
+template <class T>
+__global__ void k_axpby(T *y, const T *x, double a, double b, int64_t n) {
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-        double2 xv = x[i], yv = y[i];
-        y[i] = make_double2(a * xv.x + b * yv.x, a * xv.y + b * yv.y);
+        const double2 xv = to_d2(x[i]), yv = to_d2(y[i]);
+        y[i] = from_d2<T>(make_double2(a * xv.x + b * yv.x, a * xv.y + b * yv.y));
     }
 }
 
-void launch_axpby(double2 *y, const double2 *x, double a, double b, int64_t n, hipStream_t s) {
-    int64_t blocks = (n + 255) / 256;
-    if (blocks > 8192) blocks = 8192;
-    if (blocks < 1) blocks = 1;
-    hipLaunchKernelGGL(k_axpby, dim3((unsigned)blocks), dim3(256), 0, s, y, x, a, b, n);
+void launch_axpby(void *y, const void *x, double a, double b, int64_t n, int prec, hipStream_t s) {
+    const dim3 g((unsigned)grid_for(n)), bl(256);
+    if (prec == 0) hipLaunchKernelGGL(k_axpby<double2>, g, bl, 0, s, (double2 *)y, (const double2 *)x, a, b, n);
+    else hipLaunchKernelGGL(k_axpby<float2>, g, bl, 0, s, (float2 *)y, (const float2 *)x, a, b, n);
 }
 
-__global__ void __launch_bounds__(256) k_dot_partial(const double2 *x, const double2 *y, int64_t n, double *partials) {
+template <class T>
+__global__ void __launch_bounds__(256) k_dot_partial(const T *x, const T *y, int64_t n, double *partials) {
     double sx = 0.0, sy = 0.0;
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-        double2 a = x[i], b = y[i];
+        const double2 a = to_d2(x[i]), b = to_d2(y[i]);
         sx = fma(a.x, b.x, sx);
         sx = fma(a.y, b.y, sx);
         sy = fma(a.x, b.y, sy);
@@ -462,23 +577,31 @@ __global__ void __launch_bounds__(256) k_dot_partial(const double2 *x, const dou
     }
 }
 
-__global__ void k_dot_final(const double *partials, int nblocks, double2 *out) {
-    if (threadIdx.x != 0) return;
+// One wave sums the block partials in a fixed order (deterministic).
+__global__ void __launch_bounds__(64) k_dot_final(const double *partials, int nblocks, double2 *out) {
     double ax = 0.0, ay = 0.0;
-    for (int k = 0; k < nblocks; k++) { ax += partials[2 * k]; ay += partials[2 * k + 1]; }
-    *out = make_double2(ax, ay);
+    for (int k = threadIdx.x; k < nblocks; k += 64) { ax += partials[2 * k]; ay += partials[2 * k + 1]; }
+    ax = wave_sum(ax);
+    ay = wave_sum(ay);
+    if (threadIdx.x == 0) *out = make_double2(ax, ay);
 }
 
-void launch_dot(const double2 *x, const double2 *y, int64_t n, double *partials, int nblocks, double2 *out,
+void launch_dot(const void *x, const void *y, int64_t n, double *partials, int nblocks, double2 *out, int prec,
                 hipStream_t s) {
-    hipLaunchKernelGGL(k_dot_partial, dim3(nblocks), dim3(256), 0, s, x, y, n, partials);
+    if (prec == 0)
+        hipLaunchKernelGGL(k_dot_partial<double2>, dim3(nblocks), dim3(256), 0, s, (const double2 *)x,
+                           (const double2 *)y, n, partials);
+    else
+        hipLaunchKernelGGL(k_dot_partial<float2>, dim3(nblocks), dim3(256), 0, s, (const float2 *)x,
+                           (const float2 *)y, n, partials);
     hipLaunchKernelGGL(k_dot_final, dim3(1), dim3(64), 0, s, partials, nblocks, out);
 }
 
-__global__ void __launch_bounds__(256) k_norm2_partial(const double2 *x, int64_t n, double *partials) {
+template <class T>
+__global__ void __launch_bounds__(256) k_norm2_partial(const T *x, int64_t n, double *partials) {
     double acc = 0.0;
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-        double2 a = x[i];
+        const double2 a = to_d2(x[i]);
         acc = fma(a.x, a.x, acc);
         acc = fma(a.y, a.y, acc);
     }
@@ -493,38 +616,45 @@ __global__ void __launch_bounds__(256) k_norm2_partial(const double2 *x, int64_t
     }
 }
 
-__global__ void k_norm2_final(const double *partials, int nblocks, double *out) {
-    if (threadIdx.x != 0) return;
+__global__ void __launch_bounds__(64) k_norm2_final(const double *partials, int nblocks, double *out) {
     double a = 0.0;
-    for (int k = 0; k < nblocks; k++) a += partials[k];
-    *out = a;
+    for (int k = threadIdx.x; k < nblocks; k += 64) a += partials[k];
+    a = wave_sum(a);
+    if (threadIdx.x == 0) *out = a;
 }
 
-void launch_norm2(const double2 *x, int64_t n, double *partials, int nblocks, double *out, hipStream_t s) {
-    hipLaunchKernelGGL(k_norm2_partial, dim3(nblocks), dim3(256), 0, s, x, n, partials);
+void launch_norm2(const void *x, int64_t n, double *partials, int nblocks, double *out, int prec, hipStream_t s) {
+    if (prec == 0)
+        hipLaunchKernelGGL(k_norm2_partial<double2>, dim3(nblocks), dim3(256), 0, s, (const double2 *)x, n,
+                           partials);
+    else
+        hipLaunchKernelGGL(k_norm2_partial<float2>, dim3(nblocks), dim3(256), 0, s, (const float2 *)x, n,
+                           partials);
     hipLaunchKernelGGL(k_norm2_final, dim3(1), dim3(64), 0, s, partials, nblocks, out);
 }
 
 // w <- (w - beta_prev * vprev) - alpha * vj, with alpha = Re(<vj|w>) read from
 // the device; accumulates |w|^2 partials.
-__global__ void __launch_bounds__(256) k_lanczos_update(double2 *w, const double2 *vj, const double2 *vprev,
-                                                        const double *alpha_re, double beta_prev, int64_t n,
-                                                        double *partials) {
+template <class T>
+__global__ void __launch_bounds__(256) k_lanczos_update(T *w, const T *vj, const T *vprev, const double *alpha_re,
+                                                        double beta_prev, int64_t n, double *partials) {
     const double alpha = alpha_re[0];
     double acc = 0.0;
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-        double2 a = w[i];
+        double2 a = to_d2(w[i]);
         if (vprev) {
-            double2 p = vprev[i];
+            const double2 p = to_d2(vprev[i]);
             a.x = a.x - beta_prev * p.x;
             a.y = a.y - beta_prev * p.y;
         }
-        double2 v = vj[i];
+        const double2 v = to_d2(vj[i]);
         a.x = a.x - alpha * v.x;
         a.y = a.y - alpha * v.y;
-        w[i] = a;
-        acc = fma(a.x, a.x, acc);
-        acc = fma(a.y, a.y, acc);
+        const T st = from_d2<T>(a);
+        w[i] = st;
+        const double2 b = to_d2(st);
+        acc = fma(b.x, b.x, acc);
+        acc = fma(b.y, b.y, acc);
     }
     __shared__ double red[4];
     acc = wave_sum(acc);
@@ -537,46 +667,89 @@ __global__ void __launch_bounds__(256) k_lanczos_update(double2 *w, const double
     }
 }
 
-void launch_lanczos_update(double2 *w, const double2 *vj, const double2 *vprev, const double *alpha,
-                           double beta_prev, int64_t n, double *partials, int nblocks, double *norm2,
-                           hipStream_t s) {
-    hipLaunchKernelGGL(k_lanczos_update, dim3(nblocks), dim3(256), 0, s, w, vj, vprev, alpha, beta_prev, n,
-                       partials);
+void launch_lanczos_update(void *w, const void *vj, const void *vprev, const double *alpha, double beta_prev,
+                           int64_t n, double *partials, int nblocks, double *norm2, int prec, hipStream_t s) {
+    if (prec == 0)
+        hipLaunchKernelGGL(k_lanczos_update<double2>, dim3(nblocks), dim3(256), 0, s, (double2 *)w,
+                           (const double2 *)vj, (const double2 *)vprev, alpha, beta_prev, n, partials);
+    else
+        hipLaunchKernelGGL(k_lanczos_update<float2>, dim3(nblocks), dim3(256), 0, s, (float2 *)w,
+                           (const float2 *)vj, (const float2 *)vprev, alpha, beta_prev, n, partials);
     hipLaunchKernelGGL(k_norm2_final, dim3(1), dim3(64), 0, s, partials, nblocks, norm2);
 }
 
-__global__ void k_scale(double2 *x, const double *norm2, int64_t n) {
+template <class T>
+__global__ void k_scale(T *x, const double *norm2, int64_t n) {
     const double d = sqrt(norm2[0]);
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-        double2 a = x[i];
-        x[i] = make_double2(a.x / d, a.y / d);
+        const double2 a = to_d2(x[i]);
+        x[i] = from_d2<T>(make_double2(a.x / d, a.y / d));
     }
 }
 
-void launch_scale(double2 *x, const double *norm2, int64_t n, hipStream_t s) {
-    int64_t blocks = (n + 255) / 256;
-    if (blocks > 8192) blocks = 8192;
-    if (blocks < 1) blocks = 1;
-    hipLaunchKernelGGL(k_scale, dim3((unsigned)blocks), dim3(256), 0, s, x, norm2, n);
+void launch_scale(void *x, const double *norm2, int64_t n, int prec, hipStream_t s) {
+    const dim3 g((unsigned)grid_for(n)), b(256);
+    if (prec == 0) hipLaunchKernelGGL(k_scale<double2>, g, b, 0, s, (double2 *)x, norm2, n);
+    else hipLaunchKernelGGL(k_scale<float2>, g, b, 0, s, (float2 *)x, norm2, n);
 }
 
-__global__ void k_combine(double2 *out, const double2 *basis, int64_t stride, const double2 *coef, int k, int64_t n) {
+template <class T>
+__global__ void k_combine(T *out, const T *basis, int64_t stride, const double2 *coef, int k, int64_t n) {
     __shared__ double2 c[kNKrylovMax];
     if (threadIdx.x < k) c[threadIdx.x] = coef[threadIdx.x];
     __syncthreads();
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
         double2 acc = zero2();
-        for (int j = 0; j < k; j++) cfma(acc, c[j], basis[j * stride + i]);
+        for (int j = 0; j < k; j++) cfma(acc, c[j], to_d2(basis[j * stride + i]));
+        out[i] = from_d2<T>(acc);
+    }
+}
+
+void launch_combine(void *out, const void *basis, int64_t stride, const double2 *coef, int k, int64_t n, int prec,
+                    hipStream_t s) {
+    const dim3 g((unsigned)grid_for(n)), b(256);
+    if (prec == 0)
+        hipLaunchKernelGGL(k_combine<double2>, g, b, 0, s, (double2 *)out, (const double2 *)basis, stride, coef, k, n);
+    else
+        hipLaunchKernelGGL(k_combine<float2>, g, b, 0, s, (float2 *)out, (const float2 *)basis, stride, coef, k, n);
+}
+
+// Mixed form: out = coef[0] v0 (FP64) + sum_{j>=1} coef[j] basis_j (FP32).
+__global__ void k_combine_mixed(double2 *out, const double2 *v0, const float2 *basis, int64_t stride,
+                                const double2 *coef, int k, int64_t n) {
+    __shared__ double2 c[kNKrylovMax];
+    if (threadIdx.x < k) c[threadIdx.x] = coef[threadIdx.x];
+    __syncthreads();
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        double2 acc = cmul(c[0], v0[i]);
+        for (int j = 1; j < k; j++) cfma(acc, c[j], to_d2(basis[j * stride + i]));
         out[i] = acc;
     }
 }
 
-void launch_combine(double2 *out, const double2 *basis, int64_t stride, const double2 *coef, int k, int64_t n,
-                    hipStream_t s) {
-    int64_t blocks = (n + 255) / 256;
-    if (blocks > 8192) blocks = 8192;
-    if (blocks < 1) blocks = 1;
-    hipLaunchKernelGGL(k_combine, dim3((unsigned)blocks), dim3(256), 0, s, out, basis, stride, coef, k, n);
+void launch_combine_mixed(double2 *out, const double2 *v0, const float2 *basis, int64_t stride,
+                          const double2 *coef, int k, int64_t n, hipStream_t s) {
+    hipLaunchKernelGGL(k_combine_mixed, dim3((unsigned)grid_for(n)), dim3(256), 0, s, out, v0, basis, stride, coef,
+                       k, n);
+}
+
+// Precision conversion of complex vectors / matrices.
+template <class TI, class TO>
+__global__ void k_convert(TO *dst, const TI *src, int64_t n) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        dst[i] = from_d2<TO>(to_d2(src[i]));
+}
+
+void launch_convert(void *dst, int pdst, const void *src, int psrc, int64_t n, hipStream_t s) {
+    const dim3 g((unsigned)grid_for(n)), b(256);
+    if (psrc == 0 && pdst == 1)
+        hipLaunchKernelGGL((k_convert<double2, float2>), g, b, 0, s, (float2 *)dst, (const double2 *)src, n);
+    else if (psrc == 1 && pdst == 0)
+        hipLaunchKernelGGL((k_convert<float2, double2>), g, b, 0, s, (double2 *)dst, (const float2 *)src, n);
+    else if (psrc == 0)
+        hipLaunchKernelGGL((k_convert<double2, double2>), g, b, 0, s, (double2 *)dst, (const double2 *)src, n);
+    else
+        hipLaunchKernelGGL((k_convert<float2, float2>), g, b, 0, s, (float2 *)dst, (const float2 *)src, n);
 }
 
 // hsum[t][e] = sum_{c in row t} coef[c] * (src[c] >= 0 ? h1[src[c]][e] : I[e])
