@@ -59,6 +59,104 @@ __device__ __forceinline__ double2 cconjT(double2 a) { return cconj(a); }
 __device__ __forceinline__ int pad(int i) { return i + (i >> 3); }
 
 // ---------------------------------------------------------------------------
+// 8x8 complex mode product on the matrix cores.  The complex product y = M x
+// is the real 16x16 product Yr = Mr Xr, done for 16 fibers at once with four
+// 16x16x4 MFMAs (K = 16 in chunks of 4).  Lane l = j + 16 q (j: fiber of the
+// group, q = l / 16) holds:
+//   B (chunk c): Xr[4c + q][j];  we order Xr's rows so that lane q's four
+//     values are x_q.re, x_q.im, x_{q+4}.re, x_{q+4}.im (two complex loads):
+//     row k -> x = 4 (k / 8) + k % 4, part = (k / 4) % 2;
+//   A (chunk c): Mr[i = j][k = 4c + q], built once per op;
+//   D: FP32 16x16x4 gives rows 4q + v (v = 0..3) of column j, FP64 gives rows
+//     4v + q.  Output rows are ordered so that both land as two complex
+//     values per lane: FP32 row i -> y = i / 2, part = i % 2 (y_{2q}, y_{2q+1});
+//     FP64 row i -> y = 4 (i / 8) + i % 4, part = (i / 4) % 2 (y_q, y_{q+4}).
+// Lane layouts of both builtins were checked on gfx950 with a one-wave probe.
+typedef float mg_floatx4 __attribute__((ext_vector_type(4)));
+typedef double mg_doublex4 __attribute__((ext_vector_type(4)));
+
+template <class T> struct MfmaTraits;
+template <> struct MfmaTraits<float2> {
+    typedef float S;
+    typedef mg_floatx4 V;
+    static __device__ __forceinline__ V mma(S a, S b, V c) {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+    static __device__ __forceinline__ int out_y(int i) { return i >> 1; }
+    static __device__ __forceinline__ int out_part(int i) { return i & 1; }
+    // complex outputs held by lane group q: (regs 0,1) and (regs 2,3)
+    static __device__ __forceinline__ int y0(int q) { return 2 * q; }
+    static __device__ __forceinline__ int y1(int q) { return 2 * q + 1; }
+};
+template <> struct MfmaTraits<double2> {
+    typedef double S;
+    typedef mg_doublex4 V;
+    static __device__ __forceinline__ V mma(S a, S b, V c) {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+    static __device__ __forceinline__ int out_y(int i) { return 4 * (i >> 3) + (i & 3); }
+    static __device__ __forceinline__ int out_part(int i) { return (i >> 2) & 1; }
+    static __device__ __forceinline__ int y0(int q) { return q; }
+    static __device__ __forceinline__ int y1(int q) { return q + 4; }
+};
+
+// One op over all fibers of a brick: dst(fiber) = M src(fiber)  (acc == 0) or
+// dst(fiber) += M src(fiber)  (acc == 1, dst holds the running sum).  M is the
+// 8x8 op matrix in LDS, element (y, x) at M[y + 8 x].  Every lane of a wave
+// takes part (MFMA needs full EXEC); lanes past nfib load zeros and skip the
+// store.  In place (src == dst) is safe: a wave reads its 16 fibers before
+// it writes them, and no other wave touches them.
+template <class T>
+__device__ __forceinline__ void mfma_op8(const T *src, T *dst, const T *M, int nfib, int stride, FastDiv fds,
+                                         int acc) {
+    typedef MfmaTraits<T> Tr;
+    typedef typename Tr::S S;
+    typedef typename Tr::V V;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    const int j = lane & 15, q = lane >> 4;
+    S a[4];
+    {
+        const int i = j;
+        const int y = Tr::out_y(i), po = Tr::out_part(i);
+#pragma unroll
+        for (int c = 0; c < 4; c++) {
+            const int k = 4 * c + q;
+            const int x = 4 * (k >> 3) + (k & 3), pi = (k >> 2) & 1;
+            const T m = M[y + 8 * x];
+            a[c] = (po == 0) ? (pi == 0 ? m.x : -m.y) : (pi == 0 ? m.y : m.x);
+        }
+    }
+    const int ya = Tr::y0(q), yb = Tr::y1(q);
+    for (int g = wave; g * 16 < nfib; g += nw) {
+        const int f = g * 16 + j;
+        const bool valid = f < nfib;
+        const int ff = valid ? f : 0;
+        const int outer = (int)fdiv((uint32_t)ff, fds), inner = ff - outer * stride;
+        const int base = inner + outer * stride * 8;
+        T x0 = src[pad(base + q * stride)];
+        T x1 = src[pad(base + (q + 4) * stride)];
+        if (!valid) { x0 = zeroT<T>(); x1 = zeroT<T>(); }
+        V d;
+        if (acc) {
+            const T c0 = dst[pad(base + ya * stride)], c1 = dst[pad(base + yb * stride)];
+            d[0] = c0.x; d[1] = c0.y; d[2] = c1.x; d[3] = c1.y;
+        } else {
+            d[0] = 0; d[1] = 0; d[2] = 0; d[3] = 0;
+        }
+        d = Tr::mma(a[0], x0.x, d);
+        d = Tr::mma(a[1], x0.y, d);
+        d = Tr::mma(a[2], x1.x, d);
+        d = Tr::mma(a[3], x1.y, d);
+        if (valid) {
+            T o0, o1;
+            o0.x = d[0]; o0.y = d[1]; o1.x = d[2]; o1.y = d[3];
+            dst[pad(base + ya * stride)] = o0;
+            dst[pad(base + yb * stride)] = o1;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Group pass (see GroupTask).  LDS: [op matrices][buf0][buf1], each buffer
 // holding one padded brick; chain kind ping-pongs buf0/buf1 (or updates buf0
 // in place when every op is square), sum kind keeps the input brick in buf0
@@ -114,6 +212,9 @@ __global__ void __launch_bounds__(256) k_group_run(const GroupTask *__restrict__
             const int ni = cur[p], no = t.nout[p];
             const int nfib = tot / ni;
             const T *Mq = mats + q * NMAX * NMAX;
+            if (NMAX == 8 && ni == 8 && no == 8) {
+                mfma_op8<T>(src, dst, Mq, nfib, stride, fds, 0);
+            } else
             for (int f = tid; f < nfib; f += nth) {
                 const int outer = (int)fdiv((uint32_t)f, fds), inner = f - outer * stride;
                 const int bi = inner + outer * stride * ni;
@@ -148,6 +249,9 @@ __global__ void __launch_bounds__(256) k_group_run(const GroupTask *__restrict__
             const int n = cur[p];
             const int nfib = nin_tot / n;
             const T *Mq = mats + q * NMAX * NMAX;
+            if (NMAX == 8 && n == 8) {
+                mfma_op8<T>(src, acc_buf, Mq, nfib, stride, fds, 1);
+            } else
             for (int f = tid; f < nfib; f += nth) {
                 const int outer = (int)fdiv((uint32_t)f, fds), inner = f - outer * stride;
                 const int bi = inner + outer * stride * n;

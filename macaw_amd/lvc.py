@@ -62,6 +62,9 @@ SPECS = {
                   spf_integrator="dopri"),
     "t4": LVCSpec("t4", 14, 2, 3, 8, 3, 5.0, omega=(0.05, 0.30), kappa_sd=0.10,
                   n_lambda=1, lam=(0.10, 0.10), gaps=(-0.2, 0.2), multiset=False),
+    # 8 SPFs everywhere (the 8x8 matrix-core path), two mode groups
+    "t5": LVCSpec("t5", 15, 3, 4, 12, 8, 5.0, omega=(0.05, 0.30), kappa_sd=0.10,
+                  gamma_sd=0.02, n_lambda=3, lam=(0.05, 0.15), gaps=(-0.2, 0.0, 0.2)),
 }
 
 

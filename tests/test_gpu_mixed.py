@@ -16,7 +16,7 @@ from oracle_lib import Oracle
 from macaw_amd.lvc import make_problem
 
 gpu = pytest.mark.gpu
-CASES = ["t1", "t2", "t3", "t4"]
+CASES = ["t1", "t2", "t3", "t4", "t5"]
 AU2FS = 0.02418884326505
 
 
@@ -91,7 +91,7 @@ def max_errors(outs_e, pops_e, outs_o, pops_o):
 
 
 @gpu
-@pytest.mark.parametrize("name,tf_fs", [("t1", 4.0), ("t2", 2.0), ("t3", 2.0), ("t4", 3.0), ("c1", 2.0)])
+@pytest.mark.parametrize("name,tf_fs", [("t1", 4.0), ("t2", 2.0), ("t3", 2.0), ("t4", 3.0), ("t5", 1.0), ("c1", 2.0)])
 def test_mixed_propagation_within_1e6_of_fp64(name, tf_fs, tmp_path):
     prob = make_problem(name, str(tmp_path))
     tf = tf_fs / AU2FS

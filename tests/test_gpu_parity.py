@@ -11,7 +11,7 @@ from oracle_lib import Oracle
 from macaw_amd.lvc import LVCSpec, make_problem
 
 gpu = pytest.mark.gpu
-CASES = ["t1", "t2", "t3", "t4"]
+CASES = ["t1", "t2", "t3", "t4", "t5"]
 
 
 def engine_for(prob):
@@ -144,7 +144,7 @@ def test_propagation_matches_oracle(name, tf_fs, tmp_path):
 
 
 @gpu
-@pytest.mark.parametrize("name,tf_fs", [("t1", 4.0), ("t2", 2.0), ("t3", 2.0), ("t4", 3.0)])
+@pytest.mark.parametrize("name,tf_fs", [("t1", 4.0), ("t2", 2.0), ("t3", 2.0), ("t4", 3.0), ("t5", 1.0)])
 def test_propagation_replay_matches_oracle(name, tf_fs, tmp_path):
     """Same attempt sequence as the oracle (replay): arithmetic-only parity."""
     prob = make_problem(name, str(tmp_path))
