@@ -128,6 +128,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 reference run (profiling)")
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
+    ap.add_argument("--ha-only", action="store_true",
+                    help="only the event-timed H.A applications and a plain copy of nconf elements "
+                         "(the workload of the counter passes)")
+    ap.add_argument("--precision", type=int, default=1, help="--ha-only: 1 mixed (default), 0 FP64")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this engine runs on one GPU; multi-GPU figures are not produced "
@@ -137,6 +141,18 @@ def main():
     log(f"building workload {args.workload}")
     prob = build_problem(args.workload)
     log(f"nconf={prob.nAConftot} dimpsi={prob.dimpsi} nham={prob.nHam}")
+    if args.ha_only:
+        from macaw_amd.engine import Engine
+        eng = Engine(prob, precision=args.precision)
+        rl = roofline(eng, prob, args.roofline_reps, PEAK_FP32_TFLOPS if args.precision else PEAK_FP64_TFLOPS)
+        es = 8 if args.precision else 16
+        cms = min(eng.profile_copy(prob.nAConftot) for _ in range(3))
+        rl["copy_ms"] = cms
+        rl["copy_bytes"] = 2.0 * es * prob.nAConftot
+        rl["copy_GBs"] = rl["copy_bytes"] / (cms * 1e-3) / 1e9
+        eng.close()
+        print(json.dumps({"ha_only": True, "precision": args.precision, **rl}), flush=True)
+        return
     eng, mx = run_engine(prob, 1, args.warmup, args.steps)
     rl = roofline(eng, prob, args.roofline_reps, PEAK_FP32_TFLOPS)
     log(f"roofline (mixed): {rl['achieved']:.2f} TFLOP/s, H.A {rl['HA_ms']:.2f} ms")

@@ -134,6 +134,11 @@ int mg_apply_h_profile(mg_ctx *ctx, const void *dB, void *dHB, double *kernel_ms
 int mg_set_replay(mg_ctx *ctx, const mg_attempt *log, int64_t n);
 /* Current CMF2 step size carried between mg_propagate calls (a.u.). */
 int mg_stepsize(const mg_ctx *ctx, double *h);
+/* Plain device copy of n complex elements of the context's work type
+ * (complex FP32 for precision 1, FP64 otherwise) between two internal
+ * buffers, event-timed; n <= nconf.  A traffic/bandwidth reference for
+ * profiling (bytes moved = 2 n element size).  No reference counterpart. */
+int mg_profile_copy(mg_ctx *ctx, int64_t n, double *kernel_ms);
 
 #ifdef __cplusplus
 }

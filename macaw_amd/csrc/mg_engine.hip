@@ -739,6 +739,15 @@ GroupTask make_task(const mg_ctx *c, int g, const std::vector<int64_t> &sz, cons
 
 // Calc_DHD_B (cmf2_derivatives.f90:1-80): HB = sum_s sum_k B_s x_k Hsum(s,k)
 // + sum_chains ae (x_k O_k) B_rhs.
+// Ops of chain k that fall in group g.
+std::vector<OpSpec> chain_ops_in_group(const mg_ctx *c, const Chain &ch, int g, const std::vector<int64_t> &sz) {
+    std::vector<OpSpec> ops;
+    for (auto &o : ch.ops)
+        if (o.mode >= c->groups[g].first && o.mode < c->groups[g].second)
+            ops.push_back({o.mode, c->h1op(o.mode, o.op), 0, (int)sz[o.mode], (int)c->NS(o.mode, ch.lhs)});
+    return ops;
+}
+
 // B, HB and the chain temporaries hold the work-tensor storage type.
 std::unique_ptr<Plan> build_apply_h(mg_ctx *c, const void *B, void *HB) {
     PlanBuilder pb;
@@ -1482,6 +1491,20 @@ int mg_apply_h_profile(mg_ctx *c, const void *dB, void *dHB, double *kernel_ms, 
         c->prof = false;
         if (kernel_ms) *kernel_ms = c->prof_ms;
         if (nlaunch) *nlaunch = c->prof_launches;
+    });
+}
+int mg_profile_copy(mg_ctx *c, int64_t n, double *kernel_ms) {
+    return guarded(c, [&] {
+        if (n < 0 || n > c->nconf) throw HipError{"copy size out of range"};
+        void *src = c->prec ? c->dXf.p : c->dIn.p;
+        void *dst = c->prec ? c->dYf.p : c->dOut.p;
+        HIP_CHECK(hipEventRecord(c->ev0, c->stream));
+        launch_copy(dst, src, n, c->prec, c->stream);
+        HIP_CHECK(hipEventRecord(c->ev1, c->stream));
+        HIP_CHECK(hipEventSynchronize(c->ev1));
+        float ms = 0.f;
+        HIP_CHECK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+        if (kernel_ms) *kernel_ms = ms;
     });
 }
 int64_t mg_nconf(const mg_ctx *c) { return c ? c->nconf : -1; }

@@ -26,25 +26,24 @@ __host__ __device__ inline double2 cconj(double2 a) { return make_double2(a.x, -
 
 // Division by a loop-invariant divisor d >= 1 with a multiply-high and a
 // shift (round-up method, valid for n < 2^31): q = umulhi(n, m) >> s, d > 1.
+// Division by a run-time constant d >= 1 (Granlund & Montgomery 1994, the
+// round-up multiplier with an added dividend): with l = ceil(log2 d) and
+// m = floor(2^32 (2^l - d) / d) + 1, n / d = (umulhi(n, m) + n) >> l for every
+// n < 2^31, d = 1 included (m = 1), without a branch.
 struct FastDiv {
     uint32_t d, m, s;
 };
 inline FastDiv make_fastdiv(uint32_t d) {
-    FastDiv f{d, 0u, 0u};
-    if (d > 1) {
-        uint32_t l = 0;
-        while ((1ull << l) < d) l++;           // ceil(log2 d)
-        const uint32_t p = 31 + l;
-        f.m = (uint32_t)(((1ull << p) + d - 1) / d);
-        f.s = p - 32;
-    }
+    uint32_t l = 0;
+    while ((1ull << l) < d) l++;
+    FastDiv f{d, (uint32_t)(((((1ull << l) - d) << 32) / d) + 1), l};
     return f;
 }
 __host__ __device__ inline uint32_t fdiv(uint32_t n, const FastDiv &f) {
 #ifdef __HIP_DEVICE_COMPILE__
-    return f.d == 1 ? n : (__umulhi(n, f.m) >> f.s);
+    return (__umulhi(n, f.m) + n) >> f.s;
 #else
-    return f.d == 1 ? n : (uint32_t)((((uint64_t)n * f.m) >> 32) >> f.s);
+    return (uint32_t)(((((uint64_t)n * f.m) >> 32) + n) >> f.s);
 #endif
 }
 
@@ -177,6 +176,7 @@ void launch_combine_mixed(double2 *out, const double2 *v0, const float2 *basis, 
 void launch_norm2(const void *x, int64_t n, double *partials, int nblocks, double *out, int prec,
                   hipStream_t s);
 void launch_convert(void *dst, int pdst, const void *src, int psrc, int64_t n, hipStream_t s);
+void launch_copy(void *dst, const void *src, int64_t n, int prec, hipStream_t s);
 void launch_hsum(double2 *hsum, const double2 *h1, const int *rowptr, const int *src,
                  const double2 *coef, int nmat, int nmax, hipStream_t s);
 int group_capacity(int brick);

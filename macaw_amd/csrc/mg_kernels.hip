@@ -58,6 +58,26 @@ __device__ __forceinline__ double2 cconjT(double2 a) { return cconj(a); }
 // conflicts of contiguous fibers (stride-1 mode inside a brick).
 __device__ __forceinline__ int pad(int i) { return i + (i >> 3); }
 
+// Global-memory loads/stores through address space 1: the task tables hand
+// the kernels generic pointers, which would otherwise become flat accesses
+// that also count against the LDS wait counter.
+typedef unsigned long long mg_u64x2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ float2 gload(const float2 *p) {
+    const unsigned long long v = *(const __attribute__((address_space(1))) unsigned long long *)p;
+    return __builtin_bit_cast(float2, v);
+}
+__device__ __forceinline__ double2 gload(const double2 *p) {
+    const mg_u64x2 v = *(const __attribute__((address_space(1))) mg_u64x2 *)p;
+    return __builtin_bit_cast(double2, v);
+}
+__device__ __forceinline__ void gstore(float2 *p, float2 v) {
+    *(__attribute__((address_space(1))) unsigned long long *)p = __builtin_bit_cast(unsigned long long, v);
+}
+__device__ __forceinline__ void gstore(double2 *p, double2 v) {
+    *(__attribute__((address_space(1))) mg_u64x2 *)p = __builtin_bit_cast(mg_u64x2, v);
+}
+constexpr int kLoadUnroll = 8;
+
 // ---------------------------------------------------------------------------
 // 8x8 complex mode product on the matrix cores.  The complex product y = M x
 // is the real 16x16 product Yr = Mr Xr, done for 16 fibers at once with four
@@ -192,14 +212,29 @@ __global__ void __launch_bounds__(256) k_group_run(const GroupTask *__restrict__
             mats[q * NMAX * NMAX + i + NMAX * j] = v;
         }
     }
-    // load the X brick: LDS index lt + Lt*(g + Min*rt)
+    // load the X brick: LDS index lt + Lt*(g + Min*rt).  kLoadUnroll loads
+    // per thread are issued before the first LDS store so their latencies
+    // overlap (a plain loop waits for every load in turn).
     const T *X = static_cast<const T *>(t.X);
+    const FastDiv fd_Lt = t.fd_Lt, fd_Min = t.fd_Min;
     const int LM_in = Lt * Min;
     const int nin_tot = LM_in * Rt;
-    for (int idx = tid; idx < nin_tot; idx += nth) {
-        const int rest = (int)fdiv((uint32_t)idx, t.fd_Lt), lt = idx - rest * Lt;
-        const int rt = (int)fdiv((uint32_t)rest, t.fd_Min), g = rest - rt * Min;
-        buf0[pad(idx)] = X[l0 + lt + L * ((int64_t)g + (int64_t)Min * (r0 + rt))];
+    for (int base = 0; base < nin_tot; base += nth * kLoadUnroll) {
+        T v[kLoadUnroll];
+        const bool full = base + nth * kLoadUnroll <= nin_tot;
+#pragma unroll
+        for (int u = 0; u < kLoadUnroll; u++) {
+            const int idx = base + u * nth + tid;
+            const int ci = (full || idx < nin_tot) ? idx : 0;
+            const int rest = (int)fdiv((uint32_t)ci, fd_Lt), lt = ci - rest * Lt;
+            const int rt = (int)fdiv((uint32_t)rest, fd_Min), g = rest - rt * Min;
+            v[u] = gload(X + (l0 + lt + L * ((int64_t)g + (int64_t)Min * (r0 + rt))));
+        }
+#pragma unroll
+        for (int u = 0; u < kLoadUnroll; u++) {
+            const int idx = base + u * nth + tid;
+            if (full || idx < nin_tot) buf0[pad(idx)] = v[u];
+        }
     }
     __syncthreads();
     T *src = buf0, *dst = buf1;
@@ -273,18 +308,35 @@ __global__ void __launch_bounds__(256) k_group_run(const GroupTask *__restrict__
         }
         src = acc_buf;
     }
-    // write Y
+    // write Y (accumulate: the Y reads of a chunk are issued together too)
     T *Y = static_cast<T *>(t.Y);
+    const FastDiv fd_Mout = t.fd_Mout;
+    const int accumulate = t.accumulate;
     const int LM_out = Lt * Mout;
     const int nout_tot = LM_out * Rt;
     const T alpha = from_d2<T>(t.alpha);
-    for (int idx = tid; idx < nout_tot; idx += nth) {
-        const int rest = (int)fdiv((uint32_t)idx, t.fd_Lt), lt = idx - rest * Lt;
-        const int rt = (int)fdiv((uint32_t)rest, t.fd_Mout), g = rest - rt * Mout;
-        const int64_t a = l0 + lt + L * ((int64_t)g + (int64_t)Mout * (r0 + rt));
-        T v = cmulT(alpha, src[pad(idx)]);
-        if (t.accumulate) v = caddT(Y[a], v);
-        Y[a] = v;
+    for (int base = 0; base < nout_tot; base += nth * kLoadUnroll) {
+        int64_t a[kLoadUnroll];
+        T yv[kLoadUnroll];
+        const bool full = base + nth * kLoadUnroll <= nout_tot;
+#pragma unroll
+        for (int u = 0; u < kLoadUnroll; u++) {
+            const int idx = base + u * nth + tid;
+            const int ci = (full || idx < nout_tot) ? idx : 0;
+            const int rest = (int)fdiv((uint32_t)ci, fd_Lt), lt = ci - rest * Lt;
+            const int rt = (int)fdiv((uint32_t)rest, fd_Mout), g = rest - rt * Mout;
+            a[u] = l0 + lt + L * ((int64_t)g + (int64_t)Mout * (r0 + rt));
+            if (accumulate) yv[u] = gload(Y + a[u]);
+        }
+#pragma unroll
+        for (int u = 0; u < kLoadUnroll; u++) {
+            const int idx = base + u * nth + tid;
+            if (full || idx < nout_tot) {
+                T v = cmulT(alpha, src[pad(idx)]);
+                if (accumulate) v = caddT(yv[u], v);
+                gstore(Y + a[u], v);
+            }
+        }
     }
 }
 
@@ -342,19 +394,21 @@ __global__ void __launch_bounds__(256) k_mode_product(ModeTask t) {
         const uint32_t r = fdiv(f, t.fd_ll), l = f - r * (uint32_t)t.ll;
         const T *x0 = X + l + t.ll * (int64_t)t.nin * r;
         T *y0 = Y + l + t.ll * (int64_t)t.nout * r;
-        T x[NMAX];
+        T x[NMAX], y[NMAX];
 #pragma unroll
-        for (int j = 0; j < NMAX; j++) x[j] = (j < t.nin) ? x0[t.ll * j] : zeroT<T>();
+        for (int j = 0; j < NMAX; j++) x[j] = (j < t.nin) ? gload(x0 + t.ll * j) : zeroT<T>();
 #pragma unroll
         for (int i = 0; i < NMAX; i++) {
-            if (i < t.nout) {
-                T acc = zeroT<T>();
+            T acc = zeroT<T>();
 #pragma unroll
-                for (int j = 0; j < NMAX; j++)
-                    if (j < t.nin) cfma(acc, M[i + NMAX * j], x[j]);
-                y0[t.ll * i] = acc;
-            }
+            for (int j = 0; j < NMAX; j++)
+                if (j < t.nin) cfma(acc, M[i + NMAX * j], x[j]);
+            y[i] = acc;
         }
+        // all outputs first, then the stores (no store-data register reuse)
+#pragma unroll
+        for (int i = 0; i < NMAX; i++)
+            if (i < t.nout) gstore(y0 + t.ll * i, y[i]);
     }
 }
 
@@ -403,10 +457,22 @@ __device__ __forceinline__ void gram_load_tile(T *dst, const T *src, int64_t l0,
                                                int jo, int nc, const GramTile &g, FastDiv fd_nc) {
     const int TP = g.Lt * g.Rt;
     const int tot = TP * nc;
-    for (int idx = threadIdx.x; idx < tot; idx += blockDim.x) {
-        const int rest = (int)fdiv((uint32_t)idx, g.fd_Lt), lt = idx - rest * g.Lt;
-        const int rt = (int)fdiv((uint32_t)rest, fd_nc), a = rest - rt * nc;
-        dst[a * TP + lt + g.Lt * rt] = src[l0 + lt + ll * ((int64_t)(jo + a) + (int64_t)n * (r0 + rt))];
+    const int nth = blockDim.x;
+    for (int base = 0; base < tot; base += nth * kLoadUnroll) {
+        T v[kLoadUnroll];
+        int d[kLoadUnroll];
+#pragma unroll
+        for (int u = 0; u < kLoadUnroll; u++) {
+            const int idx = base + u * nth + threadIdx.x;
+            const int ci = idx < tot ? idx : 0;
+            const int rest = (int)fdiv((uint32_t)ci, g.fd_Lt), lt = ci - rest * g.Lt;
+            const int rt = (int)fdiv((uint32_t)rest, fd_nc), a = rest - rt * nc;
+            d[u] = idx < tot ? a * TP + lt + g.Lt * rt : -1;
+            v[u] = gload(src + (l0 + lt + ll * ((int64_t)(jo + a) + (int64_t)n * (r0 + rt))));
+        }
+#pragma unroll
+        for (int u = 0; u < kLoadUnroll; u++)
+            if (d[u] >= 0) dst[d[u]] = v[u];
     }
 }
 
@@ -854,6 +920,19 @@ void launch_convert(void *dst, int pdst, const void *src, int psrc, int64_t n, h
         hipLaunchKernelGGL((k_convert<double2, double2>), g, b, 0, s, (double2 *)dst, (const double2 *)src, n);
     else
         hipLaunchKernelGGL((k_convert<float2, float2>), g, b, 0, s, (float2 *)dst, (const float2 *)src, n);
+}
+
+// Plain copy (profiling reference for traffic counters).
+template <class T>
+__global__ void __launch_bounds__(256) k_copy(T *__restrict__ dst, const T *__restrict__ src, int64_t n) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
+void launch_copy(void *dst, const void *src, int64_t n, int prec, hipStream_t s) {
+    const dim3 g((unsigned)grid_for(n)), b(256);
+    if (prec == 0) hipLaunchKernelGGL(k_copy<double2>, g, b, 0, s, (double2 *)dst, (const double2 *)src, n);
+    else hipLaunchKernelGGL(k_copy<float2>, g, b, 0, s, (float2 *)dst, (const float2 *)src, n);
 }
 
 // hsum[t][e] = sum_{c in row t} coef[c] * (src[c] >= 0 ? h1[src[c]][e] : I[e])

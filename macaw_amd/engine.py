@@ -41,6 +41,7 @@ def lib():
         L.mg_stepsize.argtypes = [vp, f64p]
         L.mg_apply_h_profile.argtypes = [vp, vp, vp, f64p, i64p]
         L.mg_set_replay.argtypes = [vp, C.POINTER(MgAttempt), C.c_int64]
+        L.mg_profile_copy.argtypes = [vp, C.c_int64, f64p]
         for f in ("mg_set_state", "mg_get_state", "mg_set_start"):
             getattr(L, f).argtypes = [vp, f64p, f64p]
         L.mg_calc_h1mode.argtypes = [vp, f64p]
@@ -144,6 +145,12 @@ class Engine:
         self._check(lib().mg_apply_h_profile(self.h, C.c_void_p(dB_ptr or None), C.c_void_p(dHB_ptr or None),
                                              C.byref(ms), C.byref(n)))
         return ms.value, n.value
+
+    def profile_copy(self, n):
+        """Event-timed plain copy of n work-type elements (bandwidth reference)."""
+        ms = C.c_double(0)
+        self._check(lib().mg_profile_copy(self.h, n, C.byref(ms)))
+        return ms.value
 
     def set_replay(self, attempts):
         n = len(attempts)
