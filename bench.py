@@ -97,6 +97,20 @@ def cpu_baseline(prob, ha_per_fs, threads):
     }
 
 
+def measured_traffic(workload, precision, launches_per_ha):
+    """HBM bytes per launch of the dominant kernel from the committed counter
+    runs (FETCH_SIZE / WRITE_SIZE), used only while the launch schedule they
+    were measured on is unchanged."""
+    path = os.path.join(ROOT, "profiles", "k_group_run_traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        t = json.load(f)
+    if (t.get("workload"), t.get("precision"), t.get("launches_per_HA")) != (workload, precision, launches_per_ha):
+        return None
+    return t
+
+
 def run_engine(prob, precision, warmup, steps):
     from macaw_amd.engine import Engine
     tout = prob.writetime
@@ -173,6 +187,10 @@ def main():
         "roofline_detail": {k: v for k, v in rl.items() if k not in ("bound", "achieved", "peak", "unit", "frac", "traffic")},
     }
     result["roofline_detail"]["peak_note"] = "MI355X dense FP32 (vector and matrix) 157.3 TFLOP/s"
+    tr = measured_traffic(args.workload, 1, rl["launches_per_HA"])
+    if tr is not None:
+        result["roofline"]["traffic"] = tr["bytes_per_launch"]
+        result["roofline_detail"]["traffic_source"] = tr["source"]
     if not args.no_fp64:
         eng64, fx = run_engine(prob, 0, args.warmup, args.steps)
         rl64 = roofline(eng64, prob, args.roofline_reps, PEAK_FP64_TFLOPS)
