@@ -87,6 +87,11 @@ struct SpfPlan {
 struct mg_ctx {
     std::string err;
     hipStream_t stream = nullptr;
+    // second stream: the SPF propagation runs beside the Lanczos / DeltaPsi
+    // work it does not depend on (default priority, like the first)
+    hipStream_t stream2 = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    int *spf_status_host = nullptr;   // pinned: status of the last SPF propagation
     // 0: complex FP64 throughout.  1: mixed form -- the A-vector, SPFs, rho,
     // mean-field matrices and every reduction stay FP64; the H.A chains, the
     // Krylov vectors beyond the first and the mean-field mode-product chains
@@ -125,7 +130,7 @@ struct mg_ctx {
     std::vector<std::pair<int, int>> groups;
     // ---- device buffers ----
     DevBuf dA, dpsi, dAstart, dpsistart, dBK, dB0, dBH, dB02, dpH, dpH2, dHtmp, dIn, dOut;
-    DevBuf dh1, dhsum, drho, dmf, drmf, dS, dS2, dGtmp;
+    DevBuf dh1, dhsum, drho, dmf, drmf, dS, dS2, dS3, dGtmp;
     DevBuf dchain;                 // 2 * nchains * mixed
     DevBuf dmfbuf;                 // 2 * nm * mixed (AL / AR)
     DevBuf dkron;                  // 2 * nconf-sized temps for overlap chains
@@ -444,6 +449,7 @@ void allocate(mg_ctx *c) {
     c->drmf.alloc(z * nn * c->nopmax * c->nm);
     c->dS.alloc(z * nn * c->nm * c->ns);
     c->dS2.alloc(z * nn * c->nm * c->ns);
+    c->dS3.alloc(z * nn * c->nm * c->ns);
     c->dGtmp.alloc(z * nn);
     c->dchain.alloc(c->es * 2 * std::max<size_t>(1, c->chains.size()) * (size_t)c->mixed);
     c->dmfbuf.alloc(c->es * 2 * (size_t)c->nm * (size_t)c->mixed);
@@ -456,6 +462,7 @@ void allocate(mg_ctx *c) {
     HIP_CHECK(hipMemsetAsync(c->drho.p, 0, c->drho.bytes, c->stream));
     HIP_CHECK(hipMemsetAsync(c->dS.p, 0, c->dS.bytes, c->stream));
     HIP_CHECK(hipMemsetAsync(c->dS2.p, 0, c->dS2.bytes, c->stream));
+    HIP_CHECK(hipMemsetAsync(c->dS3.p, 0, c->dS3.bytes, c->stream));
     // identity h1mode for same-state identity operators (hamilton.f90:296-306)
     {
         std::vector<double2> h((size_t)nn * c->nopmax * c->nm, make_double2(0, 0));
@@ -1001,15 +1008,38 @@ cplx wf_ovl(mg_ctx *c, const double2 *A1, const double2 *psi1, const double2 *A2
     return cplx(v.x, v.y);
 }
 
-double delta_psi(mg_ctx *c, const double2 *A1, const double2 *p1, const double2 *A2, const double2 *p2) {
-    const double o11 = norm_wf(c, A1, p1), o22 = norm_wf(c, A2, p2);
-    const cplx o12 = wf_ovl(c, A1, p1, A2, p2);
+// DeltaPsi (cmf2_base.f90:146-153 via norm_wf / WF_Ovl) in two halves: the
+// SPF overlap matrices (which read p1, p2), then the A-vector contractions
+// (which do not), so p2 may be updated on another stream in between.
+void delta_psi_overlaps(mg_ctx *c, const double2 *p1, const double2 *p2) {
+    calc_overlaps(c, p1, p1, c->dS.as<double2>());
+    calc_overlaps(c, p2, p2, c->dS3.as<double2>());
+    calc_overlaps(c, p1, p2, c->dS2.as<double2>());
+}
+
+double delta_psi_contract(mg_ctx *c, const double2 *A1, const double2 *A2) {
+    apply_kron(c, A1, c->dHtmp.as<double2>(), c->dS.as<double2>());
+    const double o11 = dot(c, c->dHtmp.as<double2>(), A1, c->nconf).x;
+    apply_kron(c, A2, c->dHtmp.as<double2>(), c->dS3.as<double2>());
+    const double o22 = dot(c, c->dHtmp.as<double2>(), A2, c->nconf).x;
+    apply_kron(c, A2, c->dHtmp.as<double2>(), c->dS2.as<double2>());
+    const double2 v = dot(c, A1, c->dHtmp.as<double2>(), c->nconf);
+    const cplx o12(v.x, v.y);
     return std::sqrt(std::fabs(2.0 * (1.0 - o12.real() / std::sqrt(o11 * o22))));
 }
 
 // ----------------------------------------------------------------------------
 // Density matrices and mean fields.
-void calc_rho(mg_ctx *c, const double2 *A) {
+// In the mixed form the Gram contractions read an FP32 copy of A (left in
+// dAf for the mean-field chains that follow) and accumulate in FP64;
+// exact = true reads the FP64 A (the reported populations and norm).
+void calc_rho(mg_ctx *c, const double2 *A64, bool exact = false) {
+    const void *A = A64;
+    const int gp = exact ? 0 : c->prec;
+    if (gp) {
+        launch_convert(c->dAf.p, 1, A64, 0, c->nconf, c->stream);
+        A = c->dAf.p;
+    }
     for (int s = 0; s < c->ns; s++)
         for (int m = 0; m < c->nm; m++) {
             GramTask g;
@@ -1017,7 +1047,7 @@ void calc_rho(mg_ctx *c, const double2 *A) {
             int64_t ll = 1, nl = 1;
             for (int q = 0; q < m; q++) ll *= c->NS(q, s);
             for (int q = m + 1; q < c->nm; q++) nl *= c->NS(q, s);
-            g.X = g.Y = A + c->off[s];
+            g.X = g.Y = gp ? c->at(A, c->off[s]) : (const void *)(A64 + c->off[s]);
             g.ll = ll;
             g.nl = nl;
             g.n1 = g.n2 = (int)c->NS(m, s);
@@ -1026,7 +1056,7 @@ void calc_rho(mg_ctx *c, const double2 *A) {
             g.alpha = make_double2(1, 0);
             g.accumulate = 0;
             g.hermitian = 1;
-            launch_gram(g, c->dpartials.as<double2>(), c->nblocks_red, c->nspfmax, 0, c->stream);
+            launch_gram(g, c->dpartials.as<double2>(), c->nblocks_red, c->nspfmax, gp, c->stream);
         }
     // rho_ave for single-set modes (gmctdh_objects.f90:464-474)
     std::vector<int> rowptr(1, 0), src;
@@ -1052,7 +1082,8 @@ void calc_rho(mg_ctx *c, const double2 *A) {
 
 // Calc_MeanField, Hamiltonian part (gmctdh_objects.f90:518-605) + Calc_rhom1MF
 // (:1147-1194).  Requires calc_rho(A) and calc_h1mode(psi).
-void calc_meanfield(mg_ctx *c, const double2 *A64) {
+// af_ready: the mixed form's dAf already holds A (calc_rho(A) just ran).
+void calc_meanfield(mg_ctx *c, const double2 *A64, bool af_ready = false) {
     // one-mode same-state terms: MF += ae * rho  (all other modes are identity)
     launch_hsum(c->dmf.as<double2>(), c->drho.as<double2>(), c->dmf1_rowptr.as<int>(), c->dmf1_src.as<int>(),
                 c->dmf1_coef.as<double2>(), c->nm * c->nopmax, c->nspfmax, c->stream);
@@ -1061,7 +1092,7 @@ void calc_meanfield(mg_ctx *c, const double2 *A64) {
     // FP32 copy of A; the Gram contractions accumulate in FP64 either way
     const void *A = A64;
     if (c->prec && !c->mf_chain_terms.empty()) {
-        launch_convert(c->dAf.p, 1, A64, 0, c->nconf, c->stream);
+        if (!af_ready) launch_convert(c->dAf.p, 1, A64, 0, c->nconf, c->stream);
         A = c->dAf.p;
     }
     void *AL = c->dmfbuf.p;
@@ -1176,17 +1207,42 @@ SpfParams spf_params(mg_ctx *c, double dt) {
     return P;
 }
 
-void propagate_spf(mg_ctx *c, double2 *psi, double dt) {
+// Launch on stream `st`; the status word lands in pinned host memory and is
+// checked by spf_check once `st` has been synchronised.
+void propagate_spf_launch(mg_ctx *c, double2 *psi, double dt, hipStream_t st) {
     SpfParams P = spf_params(c, dt);
-    launch_fill(reinterpret_cast<double2 *>(c->dstatus.p), 1, make_double2(0, 0), c->stream);
+    launch_fill(reinterpret_cast<double2 *>(c->dstatus.p), 1, make_double2(0, 0), st);
     launch_spf_propagate(c->dspfmodes.as<SpfModeDesc>(), c->nm, psi, c->dspfscr.as<double2>(), P, dof_tables(c),
-                         c->dstatus.as<int>(), c->stream);
+                         c->dstatus.as<int>(), st);
     check_launch();
-    int st = 0;
-    HIP_CHECK(hipMemcpyAsync(&st, c->dstatus.p, sizeof(int), hipMemcpyDeviceToHost, c->stream));
-    HIP_CHECK(hipStreamSynchronize(c->stream));
+    HIP_CHECK(hipMemcpyAsync(c->spf_status_host, c->dstatus.p, sizeof(int), hipMemcpyDeviceToHost, st));
+}
+
+void spf_check(mg_ctx *c) {
+    const int st = *c->spf_status_host;
     if (st == 1) throw HipError{"SPF overlap matrix not positive definite"};
     if (st == 2) throw HipError{"SPF Runge-Kutta did not converge"};
+}
+
+void propagate_spf(mg_ctx *c, double2 *psi, double dt) {
+    propagate_spf_launch(c, psi, dt, c->stream);
+    HIP_CHECK(hipStreamSynchronize(c->stream));
+    spf_check(c);
+}
+
+// Start an SPF propagation on stream2 after everything queued on stream so far.
+void propagate_spf_fork(mg_ctx *c, double2 *psi, double dt) {
+    HIP_CHECK(hipEventRecord(c->ev_fork, c->stream));
+    HIP_CHECK(hipStreamWaitEvent(c->stream2, c->ev_fork, 0));
+    propagate_spf_launch(c, psi, dt, c->stream2);
+    HIP_CHECK(hipEventRecord(c->ev_join, c->stream2));
+}
+
+// Make stream wait for the forked propagation and check its status.
+void propagate_spf_join(mg_ctx *c) {
+    HIP_CHECK(hipStreamWaitEvent(c->stream, c->ev_join, 0));
+    HIP_CHECK(hipEventSynchronize(c->ev_join));
+    spf_check(c);
 }
 
 // ----------------------------------------------------------------------------
@@ -1323,18 +1379,25 @@ void cmf2_step(mg_ctx *c, double tt, double htry, double *hdid_out, double *hnex
         }
         calc_rho(c, A);
         calc_h1mode(c, psi);
-        calc_meanfield(c, A);
+        calc_meanfield(c, A, true);
         copy_d2d(c, B0, A, N);
-        const int nk1 = lanczos(c, B0, BH, 0.5 * hdid, true);
         copy_d2d(c, pH, psi, D);
-        propagate_spf(c, pH, 0.5 * hdid);
+        // the SPF half step needs only rho(A) and the mean fields: it runs on
+        // stream2 while the Lanczos propagation of A proceeds
+        propagate_spf_fork(c, pH, 0.5 * hdid);
+        const int nk1 = lanczos(c, B0, BH, 0.5 * hdid, true);
+        propagate_spf_join(c);
         calc_rho(c, BH);
         calc_h1mode(c, pH);
-        calc_meanfield(c, BH);
+        calc_meanfield(c, BH, true);
         copy_d2d(c, pH2, psi, D);
         propagate_spf(c, pH2, 0.5 * hdid);
-        const double err_psi = delta_psi(c, BH, pH, BH, pH2);
-        propagate_spf(c, pH2, 0.5 * hdid);
+        // DeltaPsi reads pH2 only through its SPF overlaps: once those are
+        // queued, the second SPF half step overlaps the A-vector contractions
+        delta_psi_overlaps(c, pH, pH2);
+        propagate_spf_fork(c, pH2, 0.5 * hdid);
+        const double err_psi = delta_psi_contract(c, BH, BH);
+        propagate_spf_join(c);
         calc_h1mode(c, pH2);
         const int nk2 = lanczos(c, BH, B02, -0.5 * hdid, true);
         launch_axpby(B02, B0, -1.0, 1.0, N, 0, c->stream);
@@ -1362,7 +1425,7 @@ void cmf2_step(mg_ctx *c, double tt, double htry, double *hdid_out, double *hnex
 // WriteOutput observables (output.f90:33-87, 143-150).
 void observables(mg_ctx *c, double t, mg_output *out, double *pops) {
     double2 *A = c->dA.as<double2>(), *psi = c->dpsi.as<double2>();
-    calc_rho(c, A);
+    calc_rho(c, A, true);
     calc_h1mode(c, psi);
     double energy;
     if (c->prec) {
@@ -1428,6 +1491,11 @@ int mg_create(const mg_problem *prob, int64_t precision, mg_ctx **out) {
         HIP_CHECK(hipGetDeviceCount(&ndev));
         if (ndev < 1) throw HipError{"no HIP device"};
         HIP_CHECK(hipStreamCreate(&c->stream));
+        HIP_CHECK(hipStreamCreate(&c->stream2));
+        HIP_CHECK(hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming));
+        HIP_CHECK(hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming));
+        HIP_CHECK(hipHostMalloc((void **)&c->spf_status_host, sizeof(int), hipHostMallocDefault));
+        *c->spf_status_host = 0;
         setup(c, prob);
         allocate(c);
         c->stepsize = c->writetime * c->eps_int;  // propagator.f90:103
@@ -1447,9 +1515,14 @@ int mg_destroy(mg_ctx *c) {
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
     for (auto e : c->prof_ev) (void)hipEventDestroy(e);
-    hipStream_t s = c->stream;
+    if (c->stream2) (void)hipStreamSynchronize(c->stream2);
+    if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
+    if (c->ev_join) (void)hipEventDestroy(c->ev_join);
+    if (c->spf_status_host) (void)hipHostFree(c->spf_status_host);
+    hipStream_t s = c->stream, s2 = c->stream2;
     delete c;
     if (s) (void)hipStreamDestroy(s);
+    if (s2) (void)hipStreamDestroy(s2);
     return 0;
 }
 

@@ -122,23 +122,38 @@ static void mode_product(int64_t nm, const int64_t *sz, int64_t k, const cplx *M
 /* Hole contraction over mode k: G[j1 + ldg*j2] += alpha * sum_{l,r}
  * conj(X[l, j1, r]) Y[l, j2, r].  X has n1 values on mode k, Y has n2, both
  * share ll (modes < k) and nl (modes > k).  Restates the zgemm('T','N') /
- * zher2k blocks of gmctdh_objects.f90:452-457, 592-600. */
+ * zher2k blocks of gmctdh_objects.f90:452-457, 592-600.
+ * The r range is cut into a fixed number of chunks (reduction_chunks) whose
+ * partial sums are added in chunk order, so the result does not depend on the
+ * thread count or scheduling; changing the chunk count changes only the
+ * summation order (a test of how rounding alone moves a trajectory). */
+static int64_t reduction_chunks = 64;
+
 static void hole_gram(int64_t ll, int64_t nl, int64_t n1, int64_t n2, const cplx *X,
                       const cplx *Y, cplx alpha, cplx *G, int64_t ldg) {
+    const int64_t nc = nl < reduction_chunks ? nl : reduction_chunks;
+    cplx *part = (cplx *)malloc(sizeof(cplx) * (size_t)(nc > 0 ? nc : 1));
     for (int64_t j2 = 0; j2 < n2; j2++) {
         for (int64_t j1 = 0; j1 < n1; j1++) {
-            cplx acc = 0;
-#pragma omp parallel for reduction(+:acc) schedule(static)
-            for (int64_t r = 0; r < nl; r++) {
-                const cplx *x = X + (r * n1 + j1) * ll;
-                const cplx *y = Y + (r * n2 + j2) * ll;
-                cplx a = 0;
-                for (int64_t l = 0; l < ll; l++) a += conj(x[l]) * y[l];
-                acc += a;
+#pragma omp parallel for schedule(static)
+            for (int64_t c = 0; c < nc; c++) {
+                const int64_t r0 = nl * c / nc, r1 = nl * (c + 1) / nc;
+                cplx acc = 0;
+                for (int64_t r = r0; r < r1; r++) {
+                    const cplx *x = X + (r * n1 + j1) * ll;
+                    const cplx *y = Y + (r * n2 + j2) * ll;
+                    cplx a = 0;
+                    for (int64_t l = 0; l < ll; l++) a += conj(x[l]) * y[l];
+                    acc += a;
+                }
+                part[c] = acc;
             }
+            cplx acc = 0;
+            for (int64_t c = 0; c < nc; c++) acc += part[c];
             G[j1 + ldg * j2] += alpha * acc;
         }
     }
+    free(part);
 }
 
 /* ---------------------------------------------------------------------- */
@@ -1220,6 +1235,7 @@ int orc_propagate(orc *o, double t0, double tfinal, mg_output *outs, double *pop
 }
 
 /* test hooks */
+void orc_set_reduction_chunks(int64_t n) { reduction_chunks = n < 1 ? 1 : n; }
 void orc_std_reg_inv(int64_t n, const double *A, double eps, int64_t type, double *B) {
     std_reg_inv(n, (const cplx *)A, eps, (int)type, (cplx *)B);
 }

@@ -39,6 +39,8 @@ void orc_calc_h1mode(orc *o, double *h1_out);
 void orc_calc_rho(orc *o, double *rho_out);
 int orc_calc_meanfield(orc *o, double *mf_out, double *rmf_out, double *energy_out);
 void orc_apply_h(orc *o, const double *B, double *HB);
+/* Test hook: chunk count of the Gram reductions (summation order only). */
+void orc_set_reduction_chunks(int64_t n);
 void orc_der_spf(orc *o, int64_t im, const double *psi_in, double *dpsi_out);
 double orc_norm_wf(orc *o, const double *A, const double *psi);
 void orc_wf_ovl(orc *o, const double *A1, const double *psi1, const double *A2,

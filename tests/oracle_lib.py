@@ -54,6 +54,7 @@ def lib():
         L.orc_std_reg_inv.argtypes = [C.c_int64, f64p, C.c_double, C.c_int64, f64p]
         L.orc_herm_eig.argtypes = [C.c_int64, f64p, f64p, f64p]
         L.orc_tridiag_eig.argtypes = [C.c_int64, f64p, f64p, f64p, f64p]
+        L.orc_set_reduction_chunks.argtypes = [C.c_int64]
         _lib = L
     return _lib
 

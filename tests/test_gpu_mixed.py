@@ -2,10 +2,12 @@
 
 The mixed form keeps the A-vector, SPFs, density and mean-field matrices and
 every reduction in FP64 and runs the H.A chains, the Krylov vectors beyond the
-first and the mean-field mode products in complex FP32.  Bars:
+first, the mean-field mode products and the density-matrix Grams on complex
+FP32 data.  Bars:
   * kernels: norm-wise relative error <= 1e-5 (FP32 rounding over a chain of
     up to 8 mode products; a wrong term or index is an O(1) error);
-  * FP64 parts (h1mode, rho): 1e-12, as in the FP64 engine;
+  * h1mode (FP64 SPFs): 1e-12, as in the FP64 engine; rho (Gram of the
+    FP32-rounded A, accumulated in FP64): 1e-6 norm-wise;
   * trajectories: max |dC(t)|, |dpop|, |dnorm| <= 1e-6 against the FP64
     oracle, free-running and with the oracle's attempt sequence replayed.
 """
@@ -58,7 +60,7 @@ def test_mixed_matrices_match_oracle(case):
     assert np.abs(h_e - h_o).max() <= 1e-12 * max(1.0, np.abs(h_o).max())
     r_o = o.rho()
     r_e = e.rho()
-    assert np.abs(r_e - r_o).max() <= 1e-12
+    assert rel(r_e, r_o) <= 1e-6          # FP32-rounded A, FP64 accumulation
     mf_o, rmf_o, _ = o.meanfield()
     mf_e, rmf_e = e.meanfield()
     assert rel(mf_e, mf_o) <= 1e-5
