@@ -68,7 +68,7 @@ def roofline(eng, prob, reps, peak):
     return {
         "bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
         "frac": achieved / peak, "traffic": None,
-        "kernel": "k_group_run (grouped mode-product pass of H.A)",
+        "kernel": "k_brick (mode-group brick programs of the H.A schedule)",
         "avg_launch_ms": avg_launch_ms, "launches_per_HA": launches_per_ha,
         "HA_ms": avg_launch_ms * launches_per_ha, "HA_flops_alg": flops_alg, "HA_bytes_alg": bytes_alg,
         "HA_alg_GBs": bytes_alg / (avg_launch_ms * launches_per_ha * 1e-3) / 1e9,
@@ -101,7 +101,7 @@ def measured_traffic(workload, precision, launches_per_ha):
     """HBM bytes per launch of the dominant kernel from the committed counter
     runs (FETCH_SIZE / WRITE_SIZE), used only while the launch schedule they
     were measured on is unchanged."""
-    path = os.path.join(ROOT, "profiles", "k_group_run_traffic.json")
+    path = os.path.join(ROOT, "profiles", "k_brick_traffic.json")
     if not os.path.exists(path):
         return None
     with open(path) as f:

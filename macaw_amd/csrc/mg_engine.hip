@@ -69,7 +69,8 @@ struct Launch {
     int n;
     int64_t maxblocks;
     int brick;          // largest Lt * M * Rt over the launch's tasks (LDS sizing)
-    int nbuf;           // LDS buffers: 1 if every task is an in-place chain
+    int nbuf;           // LDS buffers the launch's programs use
+    int nmat;           // op matrices held in LDS
 };
 struct Plan {
     DevBuf buf;
@@ -394,7 +395,7 @@ void setup(mg_ctx *c, const mg_problem *p) {
     if (const char *e = std::getenv("MG_BRICK")) {
         const int b = std::atoi(e);
         const int nmx = c->nspfmax <= 4 ? 4 : (c->nspfmax <= 8 ? 8 : 16);
-        const size_t lds = c->es * ((size_t)kMaxOpsPerTask * nmx * nmx + 2 * (size_t)group_capacity(b));
+        const size_t lds = c->es * ((size_t)kMaxBrickMats * nmx * nmx + kMaxBrickBufs * (size_t)group_capacity(b));
         if (b >= 64 && lds <= 160 * 1024) c->brick = b;
     }
     c->groups.clear();
@@ -585,58 +586,177 @@ size_t spf_scratch_base(const mg_ctx *c) {
 }
 
 // ----------------------------------------------------------------------------
-// Group passes.
+// Brick programs (host side).
 int nmax_template(const mg_ctx *c) { return c->nspfmax <= 4 ? 4 : (c->nspfmax <= 8 ? 8 : 16); }
 
-// Largest LDS brick (complex elements) a task touches: Lt times the largest
-// intermediate group size along its op sequence.
-int task_brick(const GroupTask &t) {
-    int cur[kMaxGroupModes];
-    int64_t M = 1;
-    for (int q = 0; q < t.ng; q++) { cur[q] = t.nin[q]; M *= cur[q]; }
-    int64_t mx = M;
-    if (t.kind == 0)
-        for (int q = 0; q < t.nops; q++) {
-            const int p = t.opmode[q];
-            M = M / cur[p] * t.opnout[q];
-            cur[p] = t.opnout[q];
-            mx = std::max(mx, M);
+// Builds one BrickTask over mode group g for tensors sharing L (modes before
+// the group) and R (modes after).  Each buffer carries the group-mode sizes
+// of the data it holds; ops check and update them.
+struct TaskBuilder {
+    const mg_ctx *c;
+    int m0, m1;
+    BrickTask t;
+    std::vector<int64_t> shape[kMaxBrickBufs];
+    int nbuf = 0;
+    int64_t maxM = 1;
+
+    TaskBuilder(const mg_ctx *cc, int g, int64_t L, int64_t R) : c(cc) {
+        std::memset(&t, 0, sizeof t);
+        m0 = c->groups[g].first;
+        m1 = c->groups[g].second;
+        // Mmax: the largest group multi-index any state's tensor can have
+        int64_t Mmax = 1;
+        for (int m = m0; m < m1; m++) {
+            int64_t mx = 0;
+            for (int s = 0; s < c->ns; s++) mx = std::max(mx, c->NS(m, s));
+            Mmax *= mx;
         }
-    return (int)(t.Lt * mx * t.Rt);
-}
+        t.L = L;
+        t.R = R;
+        t.Lt = largest_divisor_le(L, std::max<int64_t>(1, c->brick / Mmax));
+        if (t.Lt * Mmax > c->brick) throw HipError{"brick too small for a mode group"};
+        // when the whole leading block fits, give each workgroup Rt consecutive
+        // outer indices (contiguous too)
+        t.Rt = (t.Lt == L) ? largest_divisor_le(R, std::max<int64_t>(1, c->brick / (L * Mmax))) : 1;
+        t.fd_Lt = make_fastdiv((uint32_t)t.Lt);
+        t.ld = c->nspfmax;
+    }
+    static int64_t prod(const std::vector<int64_t> &v) {
+        int64_t p = 1;
+        for (auto x : v) p *= x;
+        return p;
+    }
+    BrickInstr &emit(int code) {
+        if (t.ninstr >= kMaxBrickInstr) throw HipError{"brick program too long"};
+        BrickInstr &I = t.ins[t.ninstr++];
+        std::memset(&I, 0, sizeof I);
+        I.code = code;
+        I.alpha = make_double2(1, 0);
+        return I;
+    }
+    void use(int buf) {
+        if (buf >= kMaxBrickBufs) throw HipError{"too many brick buffers"};
+        nbuf = std::max(nbuf, buf + 1);
+    }
+    void set_shape(int buf, const std::vector<int64_t> &sh) {
+        use(buf);
+        shape[buf] = sh;
+        maxM = std::max(maxM, prod(sh));
+    }
+    // group part of full mode sizes
+    std::vector<int64_t> group_sizes(const std::vector<int64_t> &sz) const {
+        return std::vector<int64_t>(sz.begin() + m0, sz.begin() + m1);
+    }
+    void load(int buf, const void *X, const std::vector<int64_t> &sz, int add = 0, cplx alpha = 1.0) {
+        const std::vector<int64_t> sh = group_sizes(sz);
+        if (add && shape[buf] != sh) throw HipError{"brick load-add shape mismatch"};
+        set_shape(buf, sh);
+        BrickInstr &I = emit(add ? kBLoadAdd : kBLoad);
+        I.a = buf;
+        I.ptr = X;
+        I.M = (int)prod(sh);
+        I.fd_M = make_fastdiv((uint32_t)I.M);
+        I.alpha = d2(alpha);
+    }
+    void zero(int buf, const std::vector<int64_t> &sz) {
+        set_shape(buf, group_sizes(sz));
+        BrickInstr &I = emit(kBZero);
+        I.a = buf;
+        I.M = (int)prod(shape[buf]);
+    }
+    // dst (<-|+=) Mat x_mode src; Mat: nout x nin, stored with leading
+    // dimension nspfmax (adj: use its conjugate transpose), times scale
+    void op(int src, int dst, int mode, const void *mat, int adj, int nout, int acc, cplx scale = 1.0) {
+        if (mode < m0 || mode >= m1) throw HipError{"op outside the brick's mode group"};
+        if (t.nmat >= kMaxBrickMats) throw HipError{"too many op matrices in a brick program"};
+        const int p = mode - m0;
+        std::vector<int64_t> sh = shape[src];
+        const int ni = (int)sh[p];
+        int64_t st = t.Lt;
+        for (int q = 0; q < p; q++) st *= sh[q];
+        const int64_t nfib = t.Lt * prod(sh) * t.Rt / ni;
+        if (src == dst && ni != nout) throw HipError{"in-place op must be square"};
+        sh[p] = nout;
+        if (acc) {
+            if (shape[dst] != sh) throw HipError{"accumulating op shape mismatch"};
+        } else {
+            set_shape(dst, sh);
+        }
+        const int q = t.nmat++;
+        t.mat[q] = mat;
+        t.matadj[q] = adj;
+        t.matnin[q] = ni;
+        t.matnout[q] = nout;
+        t.matscale[q] = d2(scale);
+        BrickInstr &I = emit(kBOp);
+        I.a = src;
+        I.b = dst;
+        I.acc = acc;
+        I.mat = q;
+        I.ni = ni;
+        I.no = nout;
+        I.stride = (int)st;
+        I.fd_stride = make_fastdiv((uint32_t)st);
+        I.nfib = (int)nfib;
+    }
+    void axpy(int src, int dst, cplx alpha) {
+        if (shape[src] != shape[dst]) throw HipError{"brick axpy shape mismatch"};
+        BrickInstr &I = emit(kBAxpy);
+        I.a = src;
+        I.b = dst;
+        I.M = (int)prod(shape[src]);
+        I.alpha = d2(alpha);
+    }
+    void store(int buf, void *Y, int acc, cplx alpha = 1.0) {
+        BrickInstr &I = emit(kBStore);
+        I.a = buf;
+        I.ptr = Y;
+        I.acc = acc;
+        I.M = (int)prod(shape[buf]);
+        I.fd_M = make_fastdiv((uint32_t)I.M);
+        I.alpha = d2(alpha);
+    }
+    int brick() const { return (int)(t.Lt * maxM * t.Rt); }
+};
 
 // Collects the launches of one operation; phases run in order on the stream.
 struct PlanBuilder {
-    std::vector<GroupTask> tasks;
+    std::vector<BrickTask> tasks;
     std::vector<Launch> launches;
-    void phase(const std::vector<GroupTask> &ts) {
+    void phase(const std::vector<TaskBuilder> &ts) {
         if (ts.empty()) return;
         int64_t mx = 0;
-        int br = 1, nbuf = 1;
-        for (auto &t : ts) {
-            mx = std::max(mx, (t.L / t.Lt) * (t.R / t.Rt));
-            br = std::max(br, task_brick(t));
-            if (!t.inplace) nbuf = 2;
+        int br = 1, nbuf = 1, nmat = 0;
+        for (auto &tb : ts) {
+            mx = std::max(mx, (tb.t.L / tb.t.Lt) * (tb.t.R / tb.t.Rt));
+            br = std::max(br, tb.brick());
+            nbuf = std::max(nbuf, tb.nbuf);
+            nmat = std::max(nmat, tb.t.nmat);
         }
-        launches.push_back({tasks.size(), (int)ts.size(), mx, br, nbuf});
-        tasks.insert(tasks.end(), ts.begin(), ts.end());
+        launches.push_back({tasks.size(), (int)ts.size(), mx, br, nbuf, nmat});
+        for (auto &tb : ts) tasks.push_back(tb.t);
     }
     std::unique_ptr<Plan> finish(int prec) {
         auto p = std::make_unique<Plan>();
         p->prec = prec;
-        p->buf.alloc(sizeof(GroupTask) * std::max<size_t>(1, tasks.size()));
+        p->buf.alloc(sizeof(BrickTask) * std::max<size_t>(1, tasks.size()));
         if (!tasks.empty())
-            HIP_CHECK(hipMemcpy(p->buf.p, tasks.data(), sizeof(GroupTask) * tasks.size(), hipMemcpyHostToDevice));
+            HIP_CHECK(hipMemcpy(p->buf.p, tasks.data(), sizeof(BrickTask) * tasks.size(), hipMemcpyHostToDevice));
         p->launches = launches;
         return p;
     }
 };
 
 void run_plan(mg_ctx *c, const Plan &p) {
-    const GroupTask *base = p.buf.as<GroupTask>();
+    const BrickTask *base = p.buf.as<BrickTask>();
+    const int nmax = nmax_template(c);
+    const size_t es = p.prec ? sizeof(float2) : sizeof(double2);
+    for (auto &l : p.launches)
+        if (es * ((size_t)l.nmat * nmax * nmax + (size_t)l.nbuf * group_capacity(l.brick)) > 160 * 1024)
+            throw HipError{"brick program exceeds the 160 KB of LDS"};
     if (!c->prof) {
         for (auto &l : p.launches)
-            launch_group_tasks(base + l.off, l.n, l.maxblocks, nmax_template(c), l.brick, l.nbuf, p.prec, c->stream);
+            launch_brick_tasks(base + l.off, l.n, l.maxblocks, nmax, l.brick, l.nbuf, l.nmat, p.prec, c->stream);
         return;
     }
     while (c->prof_ev.size() < 2 * p.launches.size()) {
@@ -647,7 +767,7 @@ void run_plan(mg_ctx *c, const Plan &p) {
     for (size_t i = 0; i < p.launches.size(); i++) {
         const Launch &l = p.launches[i];
         HIP_CHECK(hipEventRecord(c->prof_ev[2 * i], c->stream));
-        launch_group_tasks(base + l.off, l.n, l.maxblocks, nmax_template(c), l.brick, l.nbuf, p.prec, c->stream);
+        launch_brick_tasks(base + l.off, l.n, l.maxblocks, nmax, l.brick, l.nbuf, l.nmat, p.prec, c->stream);
         HIP_CHECK(hipEventRecord(c->prof_ev[2 * i + 1], c->stream));
     }
     HIP_CHECK(hipStreamSynchronize(c->stream));
@@ -658,175 +778,191 @@ void run_plan(mg_ctx *c, const Plan &p) {
         c->prof_ms += ms;
         c->prof_launches++;
         if (verbose)
-            fprintf(stderr, "  launch %zu: tasks=%d blocks=%lld brick=%d nbuf=%d  %.3f ms\n", i, p.launches[i].n,
-                    (long long)p.launches[i].maxblocks, p.launches[i].brick, p.launches[i].nbuf, ms);
+            fprintf(stderr, "  launch %zu: tasks=%d blocks=%lld brick=%d nbuf=%d nmat=%d  %.3f ms\n", i,
+                    p.launches[i].n, (long long)p.launches[i].maxblocks, p.launches[i].brick, p.launches[i].nbuf,
+                    p.launches[i].nmat, ms);
     }
 }
 
-// Build a group-pass task over group g for a tensor with current sizes `sz`
-// (modes < g.first already in output sizes).  ops: (mode, matrix, adj, nin, nout).
-struct OpSpec {
-    int mode;
-    const void *mat;          // storage type of the task
-    int adj, nin, nout;
-};
-
-GroupTask make_task(const mg_ctx *c, int g, const std::vector<int64_t> &sz, const std::vector<OpSpec> &ops, int kind,
-                    const void *X, void *Y, cplx alpha, int accumulate) {
-    GroupTask t;
-    std::memset(&t, 0, sizeof t);
-    const int m0 = c->groups[g].first, m1 = c->groups[g].second;
-    t.X = X;
-    t.Y = Y;
-    t.ng = m1 - m0;
-    int64_t L = 1, R = 1;
-    for (int m = 0; m < m0; m++) L *= sz[m];
-    for (int m = m1; m < c->nm; m++) R *= sz[m];
-    std::vector<int64_t> cur(sz.begin() + m0, sz.begin() + m1);
-    int64_t Min = 1;
-    for (auto v : cur) Min *= v;
-    int64_t Mmax = Min;
-    std::vector<int> nout(cur.begin(), cur.end());
-    t.nops = (int)ops.size();
-    if (t.nops > kMaxOpsPerTask) throw HipError{"too many ops in a group task"};
-    for (int q = 0; q < t.nops; q++) {
-        const OpSpec &o = ops[q];
-        const int p = o.mode - m0;
-        t.opmode[q] = p;
-        t.opmat[q] = o.mat;
-        t.opadj[q] = o.adj;
-        t.opnin[q] = o.nin;
-        t.opnout[q] = o.nout;
-        if (cur[p] != o.nin) throw HipError{"op size mismatch in group task"};
-        if (kind == 0) {
-            cur[p] = o.nout;
-            int64_t M = 1;
-            for (auto v : cur) M *= v;
-            Mmax = std::max(Mmax, M);
-            nout[p] = o.nout;
-        }
-    }
-    for (int q = 0; q < t.ng; q++) {
-        t.nin[q] = (int)sz[m0 + q];
-        t.nout[q] = nout[q];
-    }
-    t.L = L;
-    t.R = R;
-    t.Lt = largest_divisor_le(L, std::max<int64_t>(1, c->brick / Mmax));
-    if (t.Lt * Mmax > c->brick) throw HipError{"brick too small for a mode group"};
-    // when the whole leading block fits, consecutive outer indices are
-    // contiguous too: give each workgroup Rt of them
-    t.Rt = (t.Lt == L) ? largest_divisor_le(R, std::max<int64_t>(1, c->brick / (L * Mmax))) : 1;
-    bool square = true;
-    for (int q = 0; q < t.nops; q++) square = square && (t.opnin[q] == t.opnout[q]);
-    t.inplace = (kind == 0 && square) ? 1 : 0;
-    // index helpers: divisors of the brick addressing and per-op fiber strides
-    int64_t Mo = 1;
-    for (int q = 0; q < t.ng; q++) Mo *= t.nout[q];
-    t.fd_Lt = make_fastdiv((uint32_t)t.Lt);
-    t.fd_Min = make_fastdiv((uint32_t)Min);
-    t.fd_Mout = make_fastdiv((uint32_t)Mo);
-    {
-        std::vector<int64_t> cs(sz.begin() + m0, sz.begin() + m1);
-        for (int q = 0; q < t.nops; q++) {
-            const int p = t.opmode[q];
-            int64_t st = t.Lt;
-            for (int pp = 0; pp < p; pp++) st *= cs[pp];
-            t.opstride[q] = (int)st;
-            t.fd_opstride[q] = make_fastdiv((uint32_t)st);
-            if (kind == 0) cs[p] = t.opnout[q];
-        }
-    }
-    t.kind = kind;
-    t.ld = c->nspfmax;
-    t.alpha = d2(alpha);
-    t.accumulate = accumulate;
-    return t;
+int64_t prod_range(const std::vector<int64_t> &sz, int a, int b) {
+    int64_t p = 1;
+    for (int m = a; m < b; m++) p *= sz[m];
+    return p;
 }
 
-// Calc_DHD_B (cmf2_derivatives.f90:1-80): HB = sum_s sum_k B_s x_k Hsum(s,k)
-// + sum_chains ae (x_k O_k) B_rhs.
-// Ops of chain k that fall in group g.
-std::vector<OpSpec> chain_ops_in_group(const mg_ctx *c, const Chain &ch, int g, const std::vector<int64_t> &sz) {
-    std::vector<OpSpec> ops;
-    for (auto &o : ch.ops)
-        if (o.mode >= c->groups[g].first && o.mode < c->groups[g].second)
-            ops.push_back({o.mode, c->h1op(o.mode, o.op), 0, (int)sz[o.mode], (int)c->NS(o.mode, ch.lhs)});
-    return ops;
-}
-
-// B, HB and the chain temporaries hold the work-tensor storage type.
+// Calc_DHD_B (cmf2_derivatives.f90:1-80): HB_s = sum_k (h_k on mode k) B_s
+// (the diagonal sums Hsum) + sum_chains ae (x_k O_k) B_rhs.  Schedule over
+// the mode groups g = 0..G-1:
+//   first  (g = 0, G > 1), one task per input state b: load B_b once, write
+//          the group-0 diagonal sum to HB_b and every chain out of b through
+//          its group-0 ops to its temporary;
+//   middle (0 < g < G-1): diagonal tasks (HB_s += group-g sum of B_s) and
+//          chain tasks (temporaries through their group-g ops, in place when
+//          square);
+//   final  (g = G-1), one task per output state s: HB_s partial + last-group
+//          sum of B_s + every chain into s through its last-group ops (ae
+//          folded into its last matrix), written once.
+// With one group only the final kind runs, reading the chains' inputs.
 std::unique_ptr<Plan> build_apply_h(mg_ctx *c, const void *B, void *HB) {
     PlanBuilder pb;
     const int G = (int)c->groups.size();
+    const int nm = c->nm;
     const size_t mixed = (size_t)c->mixed;
-    void *ctmp = c->dchain.p;
-    std::vector<const void *> cur(c->chains.size());
-    std::vector<std::vector<int64_t>> csz(c->chains.size());
-    std::vector<int> last(c->chains.size(), -1), parity(c->chains.size(), 0);
-    for (size_t k = 0; k < c->chains.size(); k++) {
-        const Chain &ch = c->chains[k];
-        cur[k] = c->at(B, c->off[ch.rhs]);
-        csz[k].resize(c->nm);
-        for (int m = 0; m < c->nm; m++) csz[k][m] = c->NS(m, ch.rhs);
+    const size_t K = c->chains.size();
+    auto Tbuf = [&](size_t k, int i) { return c->at(c->dchain.p, (int64_t)(mixed * (2 * k + i))); };
+    auto state_sizes = [&](int s) {
+        std::vector<int64_t> sz(nm);
+        for (int m = 0; m < nm; m++) sz[m] = c->NS(m, s);
+        return sz;
+    };
+    auto diag_ops = [&](int s, int g) {
+        std::vector<int> ms;
+        for (int m = c->groups[g].first; m < c->groups[g].second; m++) {
+            const int row = s * nm + m;
+            if (c->hs_rowptr[row + 1] != c->hs_rowptr[row]) ms.push_back(m);
+        }
+        return ms;
+    };
+    auto chain_ops = [&](const Chain &ch, int g) {
+        std::vector<ChainOp> v;
         for (auto &o : ch.ops)
-            for (int g = 0; g < G; g++)
-                if (o.mode >= c->groups[g].first && o.mode < c->groups[g].second) last[k] = std::max(last[k], g);
-        if (last[k] < 0) last[k] = 0;
+            if (o.mode >= c->groups[g].first && o.mode < c->groups[g].second) v.push_back(o);
+        return v;
+    };
+    std::vector<const void *> cur(K);
+    std::vector<int> curT(K, -1);
+    std::vector<std::vector<int64_t>> csz(K);
+    for (size_t k = 0; k < K; k++) {
+        cur[k] = c->at(B, c->off[c->chains[k].rhs]);
+        csz[k] = state_sizes(c->chains[k].rhs);
     }
-    for (int g = 0; g < G; g++) {
-        std::vector<GroupTask> tasks;
-        // diagonal sums for every state
-        for (int s = 0; s < c->ns; s++) {
-            std::vector<int64_t> sz(c->nm);
-            for (int m = 0; m < c->nm; m++) sz[m] = c->NS(m, s);
-            std::vector<OpSpec> ops;
-            for (int m = c->groups[g].first; m < c->groups[g].second; m++) {
-                const int row = s * c->nm + m;
-                if (c->hs_rowptr[row + 1] == c->hs_rowptr[row]) continue;
-                ops.push_back({m, c->hsumop(s, m), 0, (int)sz[m], (int)sz[m]});
+    // Apply chain k's ops of group g to the data in buffer `in`; the result is
+    // left in the returned buffer (in place when square, else ping-pong with
+    // `spare`).  last_to: if >= 0, the last op accumulates into that buffer
+    // with `scale` folded into its matrix.
+    auto chain_group_ops = [&](TaskBuilder &tb, size_t k, int g, int in, int spare, int last_to, cplx scale) {
+        const Chain &ch = c->chains[k];
+        std::vector<ChainOp> ops = chain_ops(ch, g);
+        int curb = in;
+        for (size_t i = 0; i < ops.size(); i++) {
+            const ChainOp &o = ops[i];
+            const int nout = (int)c->NS(o.mode, ch.lhs);
+            const bool last = (i + 1 == ops.size());
+            if (last && last_to >= 0) {
+                tb.op(curb, last_to, o.mode, c->h1op(o.mode, o.op), 0, nout, 1, scale);
+                curb = last_to;
+            } else {
+                const int ni = (int)tb.shape[curb][o.mode - tb.m0];
+                const int dst = (ni == nout) ? curb : (curb == in ? spare : in);
+                tb.op(curb, dst, o.mode, c->h1op(o.mode, o.op), 0, nout, 0);
+                curb = dst;
             }
-            if (ops.empty() && g > 0) continue;
-            tasks.push_back(make_task(c, g, sz, ops, 1, c->at(B, c->off[s]), c->at(HB, c->off[s]), 1.0, g > 0 ? 1 : 0));
+            csz[k][o.mode] = nout;
         }
-        pb.phase(tasks);
-        tasks.clear();
-        // non-final chain passes (separate launch: in-place tasks need one LDS buffer)
-        for (size_t k = 0; k < c->chains.size(); k++) {
-            if (g >= last[k]) continue;
-            const Chain &ch = c->chains[k];
-            std::vector<OpSpec> ops;
-            for (auto &o : ch.ops)
-                if (o.mode >= c->groups[g].first && o.mode < c->groups[g].second)
-                    ops.push_back({o.mode, c->h1op(o.mode, o.op), 0, (int)csz[k][o.mode], (int)c->NS(o.mode, ch.lhs)});
-            if (ops.empty()) continue;
-            void *Y = c->at(ctmp, (int64_t)(mixed * (2 * k + parity[k])));
-            tasks.push_back(make_task(c, g, csz[k], ops, 0, cur[k], Y, 1.0, 0));
-            for (auto &o : ops) csz[k][o.mode] = o.nout;
-            cur[k] = Y;
-            parity[k] ^= 1;
+        return curb;
+    };
+
+    if (G >= 2) {
+        // first pass, per input state
+        std::vector<TaskBuilder> first;
+        for (int b = 0; b < c->ns; b++) {
+            const std::vector<int64_t> sz = state_sizes(b);
+            TaskBuilder tb(c, 0, 1, prod_range(sz, c->groups[0].second, nm));
+            tb.load(0, c->at(B, c->off[b]), sz);
+            const std::vector<int> dm = diag_ops(b, 0);
+            if (dm.empty()) {
+                tb.zero(1, sz);
+            } else {
+                for (size_t i = 0; i < dm.size(); i++)
+                    tb.op(0, 1, dm[i], c->hsumop(b, dm[i]), 0, (int)sz[dm[i]], i > 0 ? 1 : 0);
+            }
+            tb.store(1, c->at(HB, c->off[b]), 0);
+            for (size_t k = 0; k < K; k++) {
+                const Chain &ch = c->chains[k];
+                if (ch.rhs != b || chain_ops(ch, 0).empty()) continue;
+                // first op reads the untouched input brick (buffer 0)
+                const std::vector<ChainOp> ops = chain_ops(ch, 0);
+                const ChainOp &o = ops[0];
+                tb.op(0, 1, o.mode, c->h1op(o.mode, o.op), 0, (int)c->NS(o.mode, ch.lhs), 0);
+                csz[k][o.mode] = c->NS(o.mode, ch.lhs);
+                int curb = 1;
+                for (size_t i = 1; i < ops.size(); i++) {
+                    const int nout = (int)c->NS(ops[i].mode, ch.lhs);
+                    const int ni = (int)tb.shape[curb][ops[i].mode - tb.m0];
+                    const int dst = (ni == nout) ? curb : (curb == 1 ? 2 : 1);
+                    tb.op(curb, dst, ops[i].mode, c->h1op(ops[i].mode, ops[i].op), 0, nout, 0);
+                    csz[k][ops[i].mode] = nout;
+                    curb = dst;
+                }
+                tb.store(curb, Tbuf(k, 0), 0);
+                cur[k] = Tbuf(k, 0);
+                curT[k] = 0;
+            }
+            first.push_back(tb);
         }
-        pb.phase(tasks);
+        pb.phase(first);
+        // middle passes
+        for (int g = 1; g + 1 < G; g++) {
+            std::vector<TaskBuilder> diag, chn;
+            for (int s = 0; s < c->ns; s++) {
+                const std::vector<int> dm = diag_ops(s, g);
+                if (dm.empty()) continue;
+                const std::vector<int64_t> sz = state_sizes(s);
+                TaskBuilder tb(c, g, prod_range(sz, 0, c->groups[g].first), prod_range(sz, c->groups[g].second, nm));
+                tb.load(0, c->at(B, c->off[s]), sz);
+                for (size_t i = 0; i < dm.size(); i++)
+                    tb.op(0, 1, dm[i], c->hsumop(s, dm[i]), 0, (int)sz[dm[i]], i > 0 ? 1 : 0);
+                tb.store(1, c->at(HB, c->off[s]), 1);
+                diag.push_back(tb);
+            }
+            for (size_t k = 0; k < K; k++) {
+                const Chain &ch = c->chains[k];
+                if (chain_ops(ch, g).empty()) continue;
+                TaskBuilder tb(c, g, prod_range(csz[k], 0, c->groups[g].first),
+                               prod_range(csz[k], c->groups[g].second, nm));
+                tb.load(0, cur[k], csz[k]);
+                bool square = true;
+                for (auto &o : chain_ops(ch, g)) square = square && (csz[k][o.mode] == c->NS(o.mode, ch.lhs));
+                const int resb = chain_group_ops(tb, k, g, 0, 1, -1, 1.0);
+                // in place (same tensor) only when the tensor's layout is unchanged
+                const int next = (curT[k] >= 0 && square) ? curT[k] : (curT[k] == 0 ? 1 : 0);
+                tb.store(resb, Tbuf(k, next), 0);
+                cur[k] = Tbuf(k, next);
+                curT[k] = next;
+                chn.push_back(tb);
+            }
+            pb.phase(diag);
+            pb.phase(chn);
+        }
     }
-    // final chain passes, serialised per output state
-    std::vector<std::vector<size_t>> per_state(c->ns);
-    for (size_t k = 0; k < c->chains.size(); k++) per_state[c->chains[k].lhs].push_back(k);
-    size_t rounds = 0;
-    for (auto &v : per_state) rounds = std::max(rounds, v.size());
-    for (size_t r = 0; r < rounds; r++) {
-        std::vector<GroupTask> tasks;
+    // final pass, per output state
+    {
+        const int g = G - 1;
+        std::vector<TaskBuilder> fin;
         for (int s = 0; s < c->ns; s++) {
-            if (r >= per_state[s].size()) continue;
-            const size_t k = per_state[s][r];
-            const Chain &ch = c->chains[k];
-            const int g = last[k];
-            std::vector<OpSpec> ops;
-            for (auto &o : ch.ops)
-                if (o.mode >= c->groups[g].first && o.mode < c->groups[g].second)
-                    ops.push_back({o.mode, c->h1op(o.mode, o.op), 0, (int)csz[k][o.mode], (int)c->NS(o.mode, ch.lhs)});
-            tasks.push_back(make_task(c, g, csz[k], ops, 0, cur[k], c->at(HB, c->off[ch.lhs]), ch.ae, 1));
+            const std::vector<int64_t> sz = state_sizes(s);
+            const int64_t L = prod_range(sz, 0, c->groups[g].first);
+            TaskBuilder tb(c, g, L, 1);
+            if (G >= 2) tb.load(1, c->at(HB, c->off[s]), sz);
+            else tb.zero(1, sz);
+            const std::vector<int> dm = diag_ops(s, g);
+            if (!dm.empty()) {
+                tb.load(0, c->at(B, c->off[s]), sz);
+                for (int m : dm) tb.op(0, 1, m, c->hsumop(s, m), 0, (int)sz[m], 1);
+            }
+            for (size_t k = 0; k < K; k++) {
+                const Chain &ch = c->chains[k];
+                if (ch.lhs != s) continue;
+                if (prod_range(csz[k], 0, c->groups[g].first) != L)
+                    throw HipError{"chain temporary does not match its output state's layout"};
+                tb.load(0, cur[k], csz[k]);
+                if (chain_ops(ch, g).empty()) tb.axpy(0, 1, ch.ae);
+                else chain_group_ops(tb, k, g, 0, 2, 1, ch.ae);
+            }
+            tb.store(1, c->at(HB, c->off[s]), 0);
+            fin.push_back(tb);
         }
-        pb.phase(tasks);
+        pb.phase(fin);
     }
     return pb.finish(c->prec);
 }
@@ -849,27 +985,28 @@ void apply_h(mg_ctx *c, const void *B, void *HB) {
     check_launch();
 }
 
-// Y_s = (x_m M_(m,s)) X_s for every state (norm_wf / WF_Ovl chains).
+// Y_s = (x_m M_(m,s)) X_s for every state (norm_wf / WF_Ovl chains), FP64.
 std::unique_ptr<Plan> build_apply_kron(mg_ctx *c, const double2 *X, double2 *Y, const double2 *mats) {
     PlanBuilder pb;
     const int G = (int)c->groups.size();
     const size_t nn = (size_t)c->nspfmax * c->nspfmax;
     double2 *tmp = c->dkron.as<double2>();
     std::vector<const double2 *> cur(c->ns);
-    std::vector<int> parity(c->ns, 0);
     for (int s = 0; s < c->ns; s++) cur[s] = X + c->off[s];
     for (int g = 0; g < G; g++) {
-        std::vector<GroupTask> tasks;
+        std::vector<TaskBuilder> tasks;
         for (int s = 0; s < c->ns; s++) {
             std::vector<int64_t> sz(c->nm);
             for (int m = 0; m < c->nm; m++) sz[m] = c->NS(m, s);
-            std::vector<OpSpec> ops;
+            TaskBuilder tb(c, g, prod_range(sz, 0, c->groups[g].first), prod_range(sz, c->groups[g].second, c->nm));
+            tb.load(0, cur[s], sz);
             for (int m = c->groups[g].first; m < c->groups[g].second; m++)
-                ops.push_back({m, mats + nn * (m + (size_t)c->nm * s), 0, (int)sz[m], (int)sz[m]});
-            double2 *out = (g == G - 1) ? Y + c->off[s] : tmp + (size_t)c->nconf * parity[s] + c->off[s];
-            tasks.push_back(make_task(c, g, sz, ops, 0, cur[s], out, 1.0, 0));
+                tb.op(0, 0, m, mats + nn * (m + (size_t)c->nm * s), 0, (int)sz[m], 0);
+            // the temporary is updated in place from the second group on
+            double2 *out = (g == G - 1) ? Y + c->off[s] : tmp + c->off[s];
+            tb.store(0, out, 0);
             cur[s] = out;
-            parity[s] ^= 1;
+            tasks.push_back(tb);
         }
         pb.phase(tasks);
     }

@@ -7,8 +7,7 @@
 
 namespace mg {
 
-constexpr int kMaxGroupModes = 8;   // modes fused into one LDS brick pass
-constexpr int kMaxOpsPerTask = 8;
+constexpr int kMaxGroupModes = 8;   // modes fused into one LDS brick
 constexpr int kMaxDofPerMode = 4;
 constexpr int kNKrylovMax = 20;     // cmf2_integration.f90:12
 
@@ -47,38 +46,12 @@ __host__ __device__ inline uint32_t fdiv(uint32_t n, const FastDiv &f) {
 #endif
 }
 
-// One pass over a group of consecutive modes [m0, m0+ng) of a tensor laid out
-// as X[l, g, r]: l over the modes before the group (size L, already in output
-// sizes), g the group multi-index (first mode fastest), r over the modes after
-// the group (size R).  A workgroup holds an Lt x M brick in LDS.
-//   kind 0 (chain): brick <- brick x_{opmode[0]} M_0 x_{opmode[1]} M_1 ...
-//   kind 1 (sum):   brick <- sum_q brick x_{opmode[q]} M_q   (square ops)
-// Y[...] = alpha * brick  (accumulate = 0)  or  Y += alpha * brick (1).
-struct GroupTask {
-    const void *X;               // storage type: double2 (FP64) or float2 (mixed)
-    void *Y;
-    int64_t L, Lt, R, Rt;        // Rt consecutive outer indices per brick (when Lt == L)
-    int inplace;                 // chain of square ops: one LDS buffer, fibers updated in place
-    int ng;
-    int nin[kMaxGroupModes];     // group mode sizes of X
-    int nout[kMaxGroupModes];    // group mode sizes of Y
-    int kind;
-    int nops;
-    int opmode[kMaxOpsPerTask];  // position inside the group
-    int opadj[kMaxOpsPerTask];   // 1: use conj-transpose of the matrix
-    int opnin[kMaxOpsPerTask];   // columns of the effective op matrix
-    int opnout[kMaxOpsPerTask];  // rows of the effective op matrix
-    const void *opmat[kMaxOpsPerTask];   // same type as X
-    int ld;                      // leading dimension of the op matrices
-    double2 alpha;
-    int accumulate;
-    // precomputed index helpers (host): divisors of the brick addressing
-    FastDiv fd_Lt, fd_Min, fd_Mout;
-    int opstride[kMaxOpsPerTask];        // fiber stride of op q inside the brick
-    FastDiv fd_opstride[kMaxOpsPerTask];
-};
-
-// Single-mode product: Y[l, i, r] = sum_j M[i, j] X[l, j, r]  (adj: conj(M[j, i])).
+// Mode-group bricks.  A tensor over a group of consecutive modes [m0, m1) is
+// viewed as X[l, g, r]: l over the modes before the group (size L), g the
+// group multi-index (first mode fastest, size M), r over the modes after the
+// group (size R).  A workgroup holds an Lt x M x Rt brick in LDS (Rt > 1 only
+// when Lt == L, so that the brick is contiguous), stored as
+// lt + Lt * (g + M * rt) with one padding slot every 8 elements.
 struct ModeTask {
     const void *X;               // storage type
     void *Y;
@@ -102,6 +75,46 @@ struct GramTask {
     int hermitian;               // 1: X == Y, result made exactly Hermitian
     FastDiv fd_ll;               // set by launch_gram
 };
+
+// Brick programs.  A task owns the bricks (Lt x M x Rt, as above) of one
+// mode group of tensors that share L and R; each workgroup runs the task's
+// instruction list on one brick position, with up to kMaxBrickBufs LDS
+// buffers and the task's op matrices in LDS.  Instructions (buffers a, b):
+//   kBLoad     a <- X            (M: group multi-index size of X)
+//   kBLoadAdd  a += alpha X
+//   kBOp       b <- Mat x_p a    (acc = 0) or b += Mat x_p a (acc = 1), mode p
+//              of the group; ni/no: op sizes, stride/nfib: fiber geometry of a
+//   kBAxpy     b += alpha a
+//   kBStore    Y <- alpha a      (acc = 0) or Y += alpha a (acc = 1)
+//   kBZero     a <- 0
+constexpr int kMaxBrickInstr = 40;
+constexpr int kMaxBrickMats = 24;
+constexpr int kMaxBrickBufs = 3;
+enum BrickCode { kBLoad = 0, kBLoadAdd = 1, kBOp = 2, kBAxpy = 3, kBStore = 4, kBZero = 5 };
+struct BrickInstr {
+    int code;
+    int a, b;
+    int acc;
+    int mat;
+    int ni, no;
+    int stride, nfib;
+    int M;
+    FastDiv fd_stride, fd_M;
+    const void *ptr;             // kBLoad / kBLoadAdd / kBStore: tensor (work type of the plan)
+    double2 alpha;
+};
+struct BrickTask {
+    int64_t L, Lt, R, Rt;
+    FastDiv fd_Lt;
+    int ninstr, nmat, ld;
+    const void *mat[kMaxBrickMats];
+    int matadj[kMaxBrickMats];
+    int matnin[kMaxBrickMats];   // columns of the effective matrix
+    int matnout[kMaxBrickMats];  // rows
+    double2 matscale[kMaxBrickMats];
+    BrickInstr ins[kMaxBrickInstr];
+};
+
 
 // Matrix elements between SPF blocks of one mode:
 //   out[a + ld*b] = <phiA_a | O | phiB_b>,  O = product of primitive DOF ops.
@@ -156,8 +169,8 @@ struct SpfParams {
 
 // Host-side launchers (mg_kernels.hip).  prec: 0 = complex FP64 storage,
 // 1 = complex FP32 storage (reductions always FP64).
-void launch_group_tasks(const GroupTask *d_tasks, int ntasks, int64_t max_blocks_per_task,
-                        int nmax, int brick, int nbuf, int prec, hipStream_t s);
+void launch_brick_tasks(const BrickTask *d_tasks, int ntasks, int64_t max_blocks_per_task, int nmax,
+                        int brick, int nbuf, int nmat, int prec, hipStream_t s);
 void launch_mode_product(const ModeTask &t, int nmax, int prec, hipStream_t s);
 void launch_gram(const GramTask &t, double2 *partials, int nblocks, int nmax, int prec, hipStream_t s);
 void launch_spf_tasks(const SpfTask *d_tasks, int ntasks, DofTables dt, int max_ng, hipStream_t s);

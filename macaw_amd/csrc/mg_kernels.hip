@@ -15,6 +15,7 @@
 #include <math.h>
 
 #include <algorithm>
+#include <cstdlib>
 
 namespace mg {
 
@@ -176,202 +177,182 @@ __device__ __forceinline__ void mfma_op8(const T *src, T *dst, const T *M, int n
     }
 }
 
+int group_capacity(int brick) { return brick + (brick >> 3) + 1; }
+
 // ---------------------------------------------------------------------------
-// Group pass (see GroupTask).  LDS: [op matrices][buf0][buf1], each buffer
-// holding one padded brick; chain kind ping-pongs buf0/buf1 (or updates buf0
-// in place when every op is square), sum kind keeps the input brick in buf0
-// and accumulates into buf1.  T is the storage and arithmetic type.
-template <int NMAX, class T>
-__global__ void __launch_bounds__(256) k_group_run(const GroupTask *__restrict__ tasks, int capacity) {
-    const GroupTask &t = tasks[blockIdx.y];
-    const int64_t nlb = t.L / t.Lt, nrb = t.R / t.Rt;
-    const int64_t b = blockIdx.x;
-    if (b >= nlb * nrb) return;
-    const int64_t lb = b % nlb, rb = b / nlb;
-    const int64_t l0 = lb * t.Lt, r0 = rb * t.Rt;
-    const int Lt = (int)t.Lt, Rt = (int)t.Rt;
-    const int64_t L = t.L;
-    extern __shared__ char lds_raw[];
-    T *mats = reinterpret_cast<T *>(lds_raw);
-    T *buf0 = mats + kMaxOpsPerTask * NMAX * NMAX;
-    T *buf1 = t.inplace ? buf0 : buf0 + capacity;
-    const int tid = threadIdx.x, nth = blockDim.x;
-    int cur[kMaxGroupModes];
-    int Min = 1, Mout = 1;
-    for (int q = 0; q < t.ng; q++) {
-        cur[q] = t.nin[q];
-        Min *= t.nin[q];
-        Mout *= t.nout[q];
-    }
-    for (int q = 0; q < t.nops; q++) {
-        for (int e = tid; e < NMAX * NMAX; e += nth) {
-            int i = e % NMAX, j = e / NMAX;
-            const T *M = static_cast<const T *>(t.opmat[q]);
-            T v = zeroT<T>();
-            if (i < t.opnout[q] && j < t.opnin[q]) v = t.opadj[q] ? cconjT(M[j + t.ld * i]) : M[i + t.ld * j];
-            mats[q * NMAX * NMAX + i + NMAX * j] = v;
-        }
-    }
-    // load the X brick: LDS index lt + Lt*(g + Min*rt).  kLoadUnroll loads
-    // per thread are issued before the first LDS store so their latencies
-    // overlap (a plain loop waits for every load in turn).
-    const T *X = static_cast<const T *>(t.X);
-    const FastDiv fd_Lt = t.fd_Lt, fd_Min = t.fd_Min;
-    const int LM_in = Lt * Min;
-    const int nin_tot = LM_in * Rt;
-    for (int base = 0; base < nin_tot; base += nth * kLoadUnroll) {
+// Brick programs (see BrickTask).  LDS: [nmat op matrices][nbuf buffers].
+template <class T>
+__device__ __forceinline__ void brick_load(T *buf, const T *X, int64_t l0, int64_t r0, int64_t L, int Lt, int Rt,
+                                           FastDiv fd_Lt, int M, FastDiv fd_M, int add, T alpha) {
+    const int nth = blockDim.x, tid = threadIdx.x;
+    const int tot = Lt * M * Rt;
+    for (int base = 0; base < tot; base += nth * kLoadUnroll) {
         T v[kLoadUnroll];
-        const bool full = base + nth * kLoadUnroll <= nin_tot;
+        const bool full = base + nth * kLoadUnroll <= tot;
 #pragma unroll
         for (int u = 0; u < kLoadUnroll; u++) {
             const int idx = base + u * nth + tid;
-            const int ci = (full || idx < nin_tot) ? idx : 0;
+            const int ci = (full || idx < tot) ? idx : 0;
             const int rest = (int)fdiv((uint32_t)ci, fd_Lt), lt = ci - rest * Lt;
-            const int rt = (int)fdiv((uint32_t)rest, fd_Min), g = rest - rt * Min;
-            v[u] = gload(X + (l0 + lt + L * ((int64_t)g + (int64_t)Min * (r0 + rt))));
+            const int rt = (int)fdiv((uint32_t)rest, fd_M), g = rest - rt * M;
+            v[u] = gload(X + (l0 + lt + L * ((int64_t)g + (int64_t)M * (r0 + rt))));
         }
 #pragma unroll
         for (int u = 0; u < kLoadUnroll; u++) {
             const int idx = base + u * nth + tid;
-            if (full || idx < nin_tot) buf0[pad(idx)] = v[u];
+            if (full || idx < tot) {
+                if (add) buf[pad(idx)] = caddT(buf[pad(idx)], cmulT(alpha, v[u]));
+                else buf[pad(idx)] = v[u];
+            }
         }
     }
-    __syncthreads();
-    T *src = buf0, *dst = buf1;
-    if (t.kind == 0) {
-        int tot = nin_tot;
-        for (int q = 0; q < t.nops; q++) {
-            const int p = t.opmode[q];
-            const int stride = t.opstride[q];
-            const FastDiv fds = t.fd_opstride[q];
-            const int ni = cur[p], no = t.nout[p];
-            const int nfib = tot / ni;
-            const T *Mq = mats + q * NMAX * NMAX;
-            if (NMAX == 8 && ni == 8 && no == 8) {
-                mfma_op8<T>(src, dst, Mq, nfib, stride, fds, 0);
-            } else
-            for (int f = tid; f < nfib; f += nth) {
-                const int outer = (int)fdiv((uint32_t)f, fds), inner = f - outer * stride;
-                const int bi = inner + outer * stride * ni;
-                const int bo = inner + outer * stride * no;
-                T x[NMAX];
-#pragma unroll
-                for (int j = 0; j < NMAX; j++) x[j] = (j < ni) ? src[pad(bi + j * stride)] : zeroT<T>();
-#pragma unroll
-                for (int i = 0; i < NMAX; i++) {
-                    if (i < no) {
-                        T acc = zeroT<T>();
-#pragma unroll
-                        for (int j = 0; j < NMAX; j++)
-                            if (j < ni) cfma(acc, Mq[i + NMAX * j], x[j]);
-                        dst[pad(bo + i * stride)] = acc;
-                    }
-                }
-            }
-            tot = tot / ni * no;
-            cur[p] = no;
-            __syncthreads();
-            T *sw = src; src = dst; dst = sw;
-        }
-    } else {
-        T *acc_buf = buf1;
-        for (int idx = tid; idx < nin_tot; idx += nth) acc_buf[pad(idx)] = zeroT<T>();
-        __syncthreads();
-        for (int q = 0; q < t.nops; q++) {
-            const int p = t.opmode[q];
-            const int stride = t.opstride[q];
-            const FastDiv fds = t.fd_opstride[q];
-            const int n = cur[p];
-            const int nfib = nin_tot / n;
-            const T *Mq = mats + q * NMAX * NMAX;
-            if (NMAX == 8 && n == 8) {
-                mfma_op8<T>(src, acc_buf, Mq, nfib, stride, fds, 1);
-            } else
-            for (int f = tid; f < nfib; f += nth) {
-                const int outer = (int)fdiv((uint32_t)f, fds), inner = f - outer * stride;
-                const int bi = inner + outer * stride * n;
-                T x[NMAX];
-#pragma unroll
-                for (int j = 0; j < NMAX; j++) x[j] = (j < n) ? src[pad(bi + j * stride)] : zeroT<T>();
-#pragma unroll
-                for (int i = 0; i < NMAX; i++) {
-                    if (i < n) {
-                        T acc = acc_buf[pad(bi + i * stride)];
-#pragma unroll
-                        for (int j = 0; j < NMAX; j++)
-                            if (j < n) cfma(acc, Mq[i + NMAX * j], x[j]);
-                        acc_buf[pad(bi + i * stride)] = acc;
-                    }
-                }
-            }
-            __syncthreads();
-        }
-        src = acc_buf;
-    }
-    // write Y (accumulate: the Y reads of a chunk are issued together too)
-    T *Y = static_cast<T *>(t.Y);
-    const FastDiv fd_Mout = t.fd_Mout;
-    const int accumulate = t.accumulate;
-    const int LM_out = Lt * Mout;
-    const int nout_tot = LM_out * Rt;
-    const T alpha = from_d2<T>(t.alpha);
-    for (int base = 0; base < nout_tot; base += nth * kLoadUnroll) {
+}
+
+template <class T>
+__device__ __forceinline__ void brick_store(const T *buf, T *Y, int64_t l0, int64_t r0, int64_t L, int Lt, int Rt,
+                                            FastDiv fd_Lt, int M, FastDiv fd_M, int acc, T alpha) {
+    const int nth = blockDim.x, tid = threadIdx.x;
+    const int tot = Lt * M * Rt;
+    for (int base = 0; base < tot; base += nth * kLoadUnroll) {
         int64_t a[kLoadUnroll];
         T yv[kLoadUnroll];
-        const bool full = base + nth * kLoadUnroll <= nout_tot;
+        const bool full = base + nth * kLoadUnroll <= tot;
 #pragma unroll
         for (int u = 0; u < kLoadUnroll; u++) {
             const int idx = base + u * nth + tid;
-            const int ci = (full || idx < nout_tot) ? idx : 0;
+            const int ci = (full || idx < tot) ? idx : 0;
             const int rest = (int)fdiv((uint32_t)ci, fd_Lt), lt = ci - rest * Lt;
-            const int rt = (int)fdiv((uint32_t)rest, fd_Mout), g = rest - rt * Mout;
-            a[u] = l0 + lt + L * ((int64_t)g + (int64_t)Mout * (r0 + rt));
-            if (accumulate) yv[u] = gload(Y + a[u]);
+            const int rt = (int)fdiv((uint32_t)rest, fd_M), g = rest - rt * M;
+            a[u] = l0 + lt + L * ((int64_t)g + (int64_t)M * (r0 + rt));
+            if (acc) yv[u] = gload(Y + a[u]);
         }
 #pragma unroll
         for (int u = 0; u < kLoadUnroll; u++) {
             const int idx = base + u * nth + tid;
-            if (full || idx < nout_tot) {
-                T v = cmulT(alpha, src[pad(idx)]);
-                if (accumulate) v = caddT(yv[u], v);
+            if (full || idx < tot) {
+                T v = cmulT(alpha, buf[pad(idx)]);
+                if (acc) v = caddT(yv[u], v);
                 gstore(Y + a[u], v);
             }
         }
     }
 }
 
+// Generic mode product over the fibers of one mode (any sizes <= NMAX).
 template <int NMAX, class T>
-static void launch_group_nmax(const GroupTask *d_tasks, int ntasks, int64_t max_blocks, int capacity, int nbuf,
-                              hipStream_t s) {
+__device__ __forceinline__ void brick_op_generic(const T *src, T *dst, const T *Mq, int ni, int no, int nfib,
+                                                 int stride, FastDiv fds, int acc) {
+    const int nth = blockDim.x, tid = threadIdx.x;
+    for (int f = tid; f < nfib; f += nth) {
+        const int outer = (int)fdiv((uint32_t)f, fds), inner = f - outer * stride;
+        const int bi = inner + outer * stride * ni;
+        const int bo = inner + outer * stride * no;
+        T x[NMAX];
+#pragma unroll
+        for (int j = 0; j < NMAX; j++) x[j] = (j < ni) ? src[pad(bi + j * stride)] : zeroT<T>();
+#pragma unroll
+        for (int i = 0; i < NMAX; i++) {
+            if (i < no) {
+                T r = acc ? dst[pad(bo + i * stride)] : zeroT<T>();
+#pragma unroll
+                for (int j = 0; j < NMAX; j++)
+                    if (j < ni) cfma(r, Mq[i + NMAX * j], x[j]);
+                dst[pad(bo + i * stride)] = r;
+            }
+        }
+    }
+}
+
+template <int NMAX, class T>
+__global__ void __launch_bounds__(1024) k_brick(const BrickTask *__restrict__ tasks, int capacity, int nmat_lds) {
+    const BrickTask &t = tasks[blockIdx.y];
+    const int64_t L = t.L, R = t.R;
+    const int Lt = (int)t.Lt, Rt = (int)t.Rt;
+    const int64_t nlb = L / Lt, nrb = R / Rt;
+    const int64_t b = blockIdx.x;
+    if (b >= nlb * nrb) return;
+    const int64_t l0 = (b % nlb) * Lt, r0 = (b / nlb) * Rt;
+    const FastDiv fd_Lt = t.fd_Lt;
+    extern __shared__ char lds_raw[];
+    T *mats = reinterpret_cast<T *>(lds_raw);
+    T *bufs = mats + (size_t)nmat_lds * NMAX * NMAX;
+    const int tid = threadIdx.x, nth = blockDim.x;
+    const int nmat = t.nmat, ld = t.ld;
+    for (int e = tid; e < nmat * NMAX * NMAX; e += nth) {
+        const int q = e / (NMAX * NMAX), r = e - q * NMAX * NMAX;
+        const int i = r % NMAX, j = r / NMAX;
+        T v = zeroT<T>();
+        if (i < t.matnout[q] && j < t.matnin[q]) {
+            const T *Mg = static_cast<const T *>(t.mat[q]);
+            v = t.matadj[q] ? cconjT(Mg[j + ld * i]) : Mg[i + ld * j];
+            v = cmulT(from_d2<T>(t.matscale[q]), v);
+        }
+        mats[e] = v;
+    }
+    __syncthreads();
+    const int nins = t.ninstr;
+    for (int k = 0; k < nins; k++) {
+        const BrickInstr &I = t.ins[k];
+        const int code = I.code;
+        T *A = bufs + (size_t)I.a * capacity;
+        T *B = bufs + (size_t)I.b * capacity;
+        if (code == kBLoad || code == kBLoadAdd) {
+            brick_load(A, static_cast<const T *>(I.ptr), l0, r0, L, Lt, Rt, fd_Lt, I.M, I.fd_M, code == kBLoadAdd,
+                       from_d2<T>(I.alpha));
+        } else if (code == kBOp) {
+            const T *Mq = mats + I.mat * NMAX * NMAX;
+            if (NMAX == 8 && I.ni == 8 && I.no == 8)
+                mfma_op8<T>(A, B, Mq, I.nfib, I.stride, I.fd_stride, I.acc);
+            else
+                brick_op_generic<NMAX, T>(A, B, Mq, I.ni, I.no, I.nfib, I.stride, I.fd_stride, I.acc);
+        } else if (code == kBAxpy) {
+            const int tot = Lt * I.M * Rt;
+            const T al = from_d2<T>(I.alpha);
+            for (int idx = tid; idx < tot; idx += nth) B[pad(idx)] = caddT(B[pad(idx)], cmulT(al, A[pad(idx)]));
+        } else if (code == kBStore) {
+            brick_store(A, static_cast<T *>(const_cast<void *>(I.ptr)), l0, r0, L, Lt, Rt, fd_Lt, I.M, I.fd_M,
+                        I.acc, from_d2<T>(I.alpha));
+        } else if (code == kBZero) {
+            const int tot = Lt * I.M * Rt;
+            for (int idx = tid; idx < tot; idx += nth) A[pad(idx)] = zeroT<T>();
+        }
+        __syncthreads();
+    }
+}
+
+template <int NMAX, class T>
+static void launch_brick_nmax(const BrickTask *d_tasks, int ntasks, int64_t max_blocks, int capacity, int nbuf,
+                              int nmat, hipStream_t s) {
     static bool attr_set = false;
-    const size_t shm = sizeof(T) * (kMaxOpsPerTask * NMAX * NMAX + (size_t)nbuf * capacity);
+    const size_t shm = sizeof(T) * ((size_t)nmat * NMAX * NMAX + (size_t)nbuf * capacity);
     if (!attr_set) {
-        (void)hipFuncSetAttribute((const void *)k_group_run<NMAX, T>, hipFuncAttributeMaxDynamicSharedMemorySize,
+        (void)hipFuncSetAttribute((const void *)k_brick<NMAX, T>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                   160 * 1024);
         attr_set = true;
     }
-    hipLaunchKernelGGL((k_group_run<NMAX, T>), dim3((unsigned)max_blocks, (unsigned)ntasks), dim3(256), shm, s,
-                       d_tasks, capacity);
+    static const int nthreads = [] {
+        const char *e = std::getenv("MG_THREADS");
+        const int v = e ? std::atoi(e) : 256;
+        return (v == 128 || v == 256 || v == 512 || v == 1024) ? v : 256;
+    }();
+    hipLaunchKernelGGL((k_brick<NMAX, T>), dim3((unsigned)max_blocks, (unsigned)ntasks), dim3(nthreads), shm, s,
+                       d_tasks, capacity, nmat);
 }
-
-int group_capacity(int brick) { return brick + (brick >> 3) + 1; }
 
 template <class T>
-static void launch_group_T(const GroupTask *d_tasks, int ntasks, int64_t mb, int nmax, int capacity, int nbuf,
-                           hipStream_t s) {
-    if (nmax <= 4) launch_group_nmax<4, T>(d_tasks, ntasks, mb, capacity, nbuf, s);
-    else if (nmax <= 8) launch_group_nmax<8, T>(d_tasks, ntasks, mb, capacity, nbuf, s);
-    else launch_group_nmax<16, T>(d_tasks, ntasks, mb, capacity, nbuf, s);
+static void launch_brick_T(const BrickTask *d_tasks, int ntasks, int64_t mb, int nmax, int capacity, int nbuf,
+                           int nmat, hipStream_t s) {
+    if (nmax <= 4) launch_brick_nmax<4, T>(d_tasks, ntasks, mb, capacity, nbuf, nmat, s);
+    else if (nmax <= 8) launch_brick_nmax<8, T>(d_tasks, ntasks, mb, capacity, nbuf, nmat, s);
+    else launch_brick_nmax<16, T>(d_tasks, ntasks, mb, capacity, nbuf, nmat, s);
 }
 
-// `brick`: the largest Lt * M * Rt (complex elements) of any task in this
-// launch; `nbuf`: 1 when every task is an in-place chain, else 2.
-void launch_group_tasks(const GroupTask *d_tasks, int ntasks, int64_t max_blocks_per_task, int nmax,
-                        int brick, int nbuf, int prec, hipStream_t s) {
+void launch_brick_tasks(const BrickTask *d_tasks, int ntasks, int64_t max_blocks_per_task, int nmax, int brick,
+                        int nbuf, int nmat, int prec, hipStream_t s) {
     if (ntasks <= 0 || max_blocks_per_task <= 0) return;
     const int capacity = group_capacity(brick);
-    if (prec == 0) launch_group_T<double2>(d_tasks, ntasks, max_blocks_per_task, nmax, capacity, nbuf, s);
-    else launch_group_T<float2>(d_tasks, ntasks, max_blocks_per_task, nmax, capacity, nbuf, s);
+    if (prec == 0) launch_brick_T<double2>(d_tasks, ntasks, max_blocks_per_task, nmax, capacity, nbuf, nmat, s);
+    else launch_brick_T<float2>(d_tasks, ntasks, max_blocks_per_task, nmax, capacity, nbuf, nmat, s);
 }
 
 // ---------------------------------------------------------------------------

@@ -207,3 +207,31 @@ def test_full_size_apply_h_invariants():
     a = 0.3 - 0.7j
     hz = e.apply_h(a * x + y)
     assert np.abs(hz - (a * hx + hy)).max() < 1e-12 * np.abs(hz).max()
+
+
+@gpu
+@pytest.mark.parametrize("name", ["t2", "t5", "c1"])
+@pytest.mark.parametrize("brick", ["64", "256"])
+def test_apply_h_small_bricks_match_oracle(name, brick, tmp_path, monkeypatch):
+    """Small LDS bricks force three or more mode groups (first, middle and
+    final passes of the H.A schedule, chain temporaries updated in place or,
+    for t2's state-dependent SPF counts, through a second buffer)."""
+    monkeypatch.setenv("MG_BRICK", brick)
+    prob = make_problem(name, str(tmp_path))
+    A, psi = perturbed_state(prob, 5)
+    o = Oracle(prob)
+    o.set_state(A, psi)
+    o.h1mode()
+    e = engine_for(prob)
+    e.set_state(A, psi)
+    e.h1mode()
+    rng = np.random.default_rng(6)
+    for _ in range(2):
+        B = rand_c(rng, prob.nAConftot)
+        assert close(e.apply_h(B), o.apply_h(B), 1e-12)
+    # norm_wf / WF_Ovl chains run through the same brick programs
+    outs_o, pops_o, atts_o = Oracle(prob).propagate(0.0, 0.5 / 0.02418884326505)
+    e2 = engine_for(prob)
+    e2.set_replay(atts_o)
+    outs_e, pops_e, _ = e2.propagate(0.0, 0.5 / 0.02418884326505)
+    compare_outputs(outs_e, pops_e, outs_o, pops_o, 1e-10)
