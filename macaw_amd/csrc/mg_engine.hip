@@ -18,6 +18,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <string>
+#include <tuple>
 #include <vector>
 
 using namespace mg;
@@ -71,11 +72,17 @@ struct Launch {
     int brick;          // largest Lt * M * Rt over the launch's tasks (LDS sizing)
     int nbuf;           // LDS buffers the launch's programs use
     int nmat;           // op matrices held in LDS
+    int ngram = 0;      // Gram slots (persistent grid, partials, final reduction)
+    int slot_off = 0;   // first slot's entry in Plan::targets
+    int gram_blocks = 0;
 };
 struct Plan {
     DevBuf buf;
     std::vector<Launch> launches;
     int prec = 0;       // storage type of the tensors the tasks touch
+    DevBuf targets;     // double2 * per Gram slot
+    DevBuf herm_ptrs, herm_n;   // matrices made exactly Hermitian after the plan
+    int nherm = 0;
 };
 using PlanKey = std::array<const void *, 4>;
 struct SpfPlan {
@@ -141,6 +148,11 @@ struct mg_ctx {
     // mixed form only: FP64 first Krylov vector, FP32 copy of the A-vector,
     // FP32 in/out of the C-ABI H.A, FP32 copies of h1mode and the diagonal sums
     DevBuf dV0, dAf, dXf, dYf, dh1f, dhsumf;
+    // Gram-slot partial sums of brick programs; per-slot mean-field Grams;
+    // combination tables turning them (and ae * rho) into the mean fields
+    DevBuf dgpart, dgslots, dmf_dst, dmf_rowptr, dmf_src, dmf_coef, dmf_trans, dident;
+    bool mf_bricks = false;        // mean fields through brick programs (square chain ops, <= 8 SPFs)
+    std::vector<const void *> mf_L, mf_R;   // per term: L tensor, R_g tensors (work type)
     int nblocks_red = 2048;      // blocks of the streaming reductions and Gram partial sums
     // timing of apply_h_device
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -413,6 +425,12 @@ void setup(mg_ctx *c, const mg_problem *p) {
             m = e;
         }
     }
+    // mean fields through brick programs when every chain op is square and
+    // the SPF counts fit one 8-wide Gram slot
+    c->mf_bricks = c->nspfmax <= 8 && std::getenv("MG_MF_LEGACY") == nullptr;
+    for (int t : c->mf_chain_terms)
+        for (int m = 0; m < nm; m++)
+            if (c->NS(m, (int)c->lhs[t] - 1) != c->NS(m, (int)c->rhs[t] - 1)) c->mf_bricks = false;
 }
 
 // ----------------------------------------------------------------------------
@@ -453,7 +471,18 @@ void allocate(mg_ctx *c) {
     c->dS3.alloc(z * nn * c->nm * c->ns);
     c->dGtmp.alloc(z * nn);
     c->dchain.alloc(c->es * 2 * std::max<size_t>(1, c->chains.size()) * (size_t)c->mixed);
-    c->dmfbuf.alloc(c->es * 2 * (size_t)c->nm * (size_t)c->mixed);
+    {
+        size_t nterms = 0;
+        for (size_t ti = 0; ti < c->mf_chain_terms.size(); ti++)
+            if (c->mf_partner[ti] != -2) nterms++;
+        size_t mfb = 2 * (size_t)c->nm;
+        if (c->mf_bricks) mfb = std::max(mfb, c->groups.size() * std::max<size_t>(1, nterms));
+        c->dmfbuf.alloc(c->es * mfb * (size_t)c->mixed);
+        if (c->mf_bricks) {
+            c->dgslots.alloc(z * nn * c->nm * std::max<size_t>(1, nterms));
+            HIP_CHECK(hipMemset(c->dgslots.p, 0, c->dgslots.bytes));
+        }
+    }
     c->dkron.alloc(z * 2 * N);
     c->dpartials.alloc(sizeof(double2) * 64 * (size_t)c->nblocks_red);
     c->dscal.alloc(sizeof(double) * 64);
@@ -599,6 +628,7 @@ struct TaskBuilder {
     std::vector<int64_t> shape[kMaxBrickBufs];
     int nbuf = 0;
     int64_t maxM = 1;
+    std::vector<double2 *> targets;   // per Gram slot
 
     TaskBuilder(const mg_ctx *cc, int g, int64_t L, int64_t R) : c(cc) {
         std::memset(&t, 0, sizeof t);
@@ -716,6 +746,28 @@ struct TaskBuilder {
         I.fd_M = make_fastdiv((uint32_t)I.M);
         I.alpha = d2(alpha);
     }
+    // Gram slot += sum over fibers of `mode` of conj(buffer la) x (buffer rb);
+    // the buffers agree on every group mode but `mode` (sizes <= 8 there)
+    void gram(int la, int rb, int mode, double2 *target) {
+        if (t.ngram >= kMaxGrams) throw HipError{"too many Gram slots in a brick program"};
+        const int p = mode - m0;
+        const std::vector<int64_t> &sa = shape[la], &sb = shape[rb];
+        for (size_t q = 0; q < sa.size(); q++)
+            if ((int)q != p && sa[q] != sb[q]) throw HipError{"Gram operand shapes differ"};
+        if (sa[p] > 8 || sb[p] > 8) throw HipError{"Gram slot wider than 8"};
+        int64_t st = t.Lt;
+        for (int q = 0; q < p; q++) st *= sa[q];
+        BrickInstr &I = emit(kBGram);
+        I.a = la;
+        I.b = rb;
+        I.mat = t.ngram++;
+        I.ni = (int)sa[p];
+        I.no = (int)sb[p];
+        I.stride = (int)st;
+        I.fd_stride = make_fastdiv((uint32_t)st);
+        I.nfib = (int)(t.Lt * prod(sa) * t.Rt / sa[p]);
+        targets.push_back(target);
+    }
     int brick() const { return (int)(t.Lt * maxM * t.Rt); }
 };
 
@@ -723,18 +775,33 @@ struct TaskBuilder {
 struct PlanBuilder {
     std::vector<BrickTask> tasks;
     std::vector<Launch> launches;
-    void phase(const std::vector<TaskBuilder> &ts) {
+    std::vector<double2 *> targets;
+    std::vector<double2 *> herm;
+    std::vector<int> herm_n;
+    void phase(std::vector<TaskBuilder> ts) {
         if (ts.empty()) return;
         int64_t mx = 0;
-        int br = 1, nbuf = 1, nmat = 0;
+        int br = 1, nbuf = 1, nmat = 0, ngram = 0;
+        const int slot_off = (int)targets.size();
         for (auto &tb : ts) {
             mx = std::max(mx, (tb.t.L / tb.t.Lt) * (tb.t.R / tb.t.Rt));
             br = std::max(br, tb.brick());
             nbuf = std::max(nbuf, tb.nbuf);
             nmat = std::max(nmat, tb.t.nmat);
+            tb.t.gram_base = ngram;
+            ngram += tb.t.ngram;
+            targets.insert(targets.end(), tb.targets.begin(), tb.targets.end());
         }
-        launches.push_back({tasks.size(), (int)ts.size(), mx, br, nbuf, nmat});
+        Launch l{tasks.size(), (int)ts.size(), mx, br, nbuf, nmat};
+        l.ngram = ngram;
+        l.slot_off = slot_off;
+        l.gram_blocks = (int)std::min<int64_t>(512, mx);
+        launches.push_back(l);
         for (auto &tb : ts) tasks.push_back(tb.t);
+    }
+    void hermitian(double2 *G, int n) {
+        herm.push_back(G);
+        herm_n.push_back(n);
     }
     std::unique_ptr<Plan> finish(int prec) {
         auto p = std::make_unique<Plan>();
@@ -743,9 +810,37 @@ struct PlanBuilder {
         if (!tasks.empty())
             HIP_CHECK(hipMemcpy(p->buf.p, tasks.data(), sizeof(BrickTask) * tasks.size(), hipMemcpyHostToDevice));
         p->launches = launches;
+        p->targets.alloc(sizeof(double2 *) * std::max<size_t>(1, targets.size()));
+        if (!targets.empty())
+            HIP_CHECK(hipMemcpy(p->targets.p, targets.data(), sizeof(double2 *) * targets.size(),
+                                hipMemcpyHostToDevice));
+        p->nherm = (int)herm.size();
+        p->herm_ptrs.alloc(sizeof(double2 *) * std::max<size_t>(1, herm.size()));
+        p->herm_n.alloc(sizeof(int) * std::max<size_t>(1, herm.size()));
+        if (!herm.empty()) {
+            HIP_CHECK(hipMemcpy(p->herm_ptrs.p, herm.data(), sizeof(double2 *) * herm.size(), hipMemcpyHostToDevice));
+            HIP_CHECK(hipMemcpy(p->herm_n.p, herm_n.data(), sizeof(int) * herm.size(), hipMemcpyHostToDevice));
+        }
         return p;
     }
 };
+
+void launch_plan_entry(mg_ctx *c, const Plan &p, const Launch &l, int nmax) {
+    const BrickTask *base = p.buf.as<BrickTask>();
+    if (l.ngram > 0) {
+        const size_t need = sizeof(double2) * 64 * (size_t)l.ngram * (size_t)l.gram_blocks;
+        if (c->dgpart.bytes < need) {
+            HIP_CHECK(hipStreamSynchronize(c->stream));
+            c->dgpart.alloc(need);
+        }
+        launch_brick_tasks(base + l.off, l.n, l.maxblocks, nmax, l.brick, l.nbuf, l.nmat, p.prec, c->stream, l.ngram,
+                           c->dgpart.as<double2>(), l.gram_blocks);
+        launch_gram_slots_final(c->dgpart.as<double2>(), l.ngram, l.gram_blocks,
+                                p.targets.as<double2 *>() + l.slot_off, c->nspfmax, c->stream);
+    } else {
+        launch_brick_tasks(base + l.off, l.n, l.maxblocks, nmax, l.brick, l.nbuf, l.nmat, p.prec, c->stream);
+    }
+}
 
 void run_plan(mg_ctx *c, const Plan &p) {
     const BrickTask *base = p.buf.as<BrickTask>();
@@ -754,9 +849,10 @@ void run_plan(mg_ctx *c, const Plan &p) {
     for (auto &l : p.launches)
         if (es * ((size_t)l.nmat * nmax * nmax + (size_t)l.nbuf * group_capacity(l.brick)) > 160 * 1024)
             throw HipError{"brick program exceeds the 160 KB of LDS"};
+    (void)base;
     if (!c->prof) {
-        for (auto &l : p.launches)
-            launch_brick_tasks(base + l.off, l.n, l.maxblocks, nmax, l.brick, l.nbuf, l.nmat, p.prec, c->stream);
+        for (auto &l : p.launches) launch_plan_entry(c, p, l, nmax);
+        if (p.nherm) launch_herm_batch(p.herm_ptrs.as<double2 *>(), p.herm_n.as<int>(), p.nherm, c->nspfmax, c->stream);
         return;
     }
     while (c->prof_ev.size() < 2 * p.launches.size()) {
@@ -767,9 +863,10 @@ void run_plan(mg_ctx *c, const Plan &p) {
     for (size_t i = 0; i < p.launches.size(); i++) {
         const Launch &l = p.launches[i];
         HIP_CHECK(hipEventRecord(c->prof_ev[2 * i], c->stream));
-        launch_brick_tasks(base + l.off, l.n, l.maxblocks, nmax, l.brick, l.nbuf, l.nmat, p.prec, c->stream);
+        launch_plan_entry(c, p, l, nmax);
         HIP_CHECK(hipEventRecord(c->prof_ev[2 * i + 1], c->stream));
     }
+    if (p.nherm) launch_herm_batch(p.herm_ptrs.as<double2 *>(), p.herm_n.as<int>(), p.nherm, c->nspfmax, c->stream);
     HIP_CHECK(hipStreamSynchronize(c->stream));
     static const bool verbose = std::getenv("MG_VERBOSE") != nullptr;
     for (size_t i = 0; i < p.launches.size(); i++) {
@@ -1177,6 +1274,34 @@ void calc_rho(mg_ctx *c, const double2 *A64, bool exact = false) {
         launch_convert(c->dAf.p, 1, A64, 0, c->nconf, c->stream);
         A = c->dAf.p;
     }
+    if (c->nspfmax <= 8) {
+        // one brick pass per mode group and state: all the group's Grams at once
+        const void *Ain = gp ? A : (const void *)A64;
+        PlanKey key{(const void *)(gp ? 5 : 6), Ain, nullptr, nullptr};
+        auto it = c->plans.find(key);
+        if (it == c->plans.end()) {
+            const size_t esz = gp ? sizeof(float2) : sizeof(double2);
+            PlanBuilder pb;
+            const int G = (int)c->groups.size();
+            for (int g = 0; g < G; g++) {
+                std::vector<TaskBuilder> tasks;
+                for (int s = 0; s < c->ns; s++) {
+                    std::vector<int64_t> sz(c->nm);
+                    for (int m = 0; m < c->nm; m++) sz[m] = c->NS(m, s);
+                    TaskBuilder tb(c, g, prod_range(sz, 0, c->groups[g].first),
+                                   prod_range(sz, c->groups[g].second, c->nm));
+                    tb.load(0, static_cast<const char *>(Ain) + esz * (size_t)c->off[s], sz);
+                    for (int m = c->groups[g].first; m < c->groups[g].second; m++) tb.gram(0, 0, m, c->rho(m, s));
+                    tasks.push_back(tb);
+                }
+                pb.phase(tasks);
+            }
+            for (int s = 0; s < c->ns; s++)
+                for (int m = 0; m < c->nm; m++) pb.hermitian(c->rho(m, s), (int)c->NS(m, s));
+            it = c->plans.emplace(key, pb.finish(gp)).first;
+        }
+        run_plan(c, *it->second);
+    } else
     for (int s = 0; s < c->ns; s++)
         for (int m = 0; m < c->nm; m++) {
             GramTask g;
@@ -1217,10 +1342,201 @@ void calc_rho(mg_ctx *c, const double2 *A64, bool exact = false) {
     check_launch();
 }
 
+// Mean fields through brick programs.  For chain term t (lhs a, rhs b, op O_m
+// on every non-identity mode m) the hole contraction over mode m is
+// Gram(L_m, R_m) with L_m = (x_{q<m} O_q^H) A_a and R_m = (x_{q>m} O_q) A_b.
+// Over the mode groups: R passes (right to left) store R_g = ops of groups > g
+// applied to A_b; Gram pass g loads L_g and R_g, and for each mode m of the
+// group (ascending) forms R_m from R_g in a work buffer, accumulates the
+// Gram, then moves L on by O_m^H; L_{g+1} is stored in place for pass g+1.
+// The per-(term, mode) Grams and ae * rho for one-mode terms are then summed
+// into the mean-field matrices (mirror terms as ae* G^H) by k_mat_combine.
+std::unique_ptr<Plan> build_meanfield(mg_ctx *c, const void *A) {
+    PlanBuilder pb;
+    const int G = (int)c->groups.size(), nm = c->nm;
+    const size_t mixed = (size_t)c->mixed, nn = (size_t)c->nspfmax * c->nspfmax;
+    std::vector<int> terms;
+    for (size_t ti = 0; ti < c->mf_chain_terms.size(); ti++)
+        if (c->mf_partner[ti] != -2) terms.push_back((int)ti);
+    if (c->dmfbuf.bytes < c->es * mixed * (size_t)G * terms.size() ||
+        c->dgslots.bytes < sizeof(double2) * nn * nm * terms.size())
+        throw HipError{"mean-field work buffers too small"};
+    auto tensor = [&](size_t k, int g) { return c->at(c->dmfbuf.p, (int64_t)(mixed * (k * G + g))); };
+    auto is_op = [&](int t, int m) {
+        const int op = c->iOper[m + (size_t)nm * t];
+        return !c->isIdent[m + (size_t)nm * op];
+    };
+    // Gram slot per (term, mode): dgslots[k * nm + m]
+    auto slot = [&](size_t k, int m) { return c->dgslots.as<double2>() + nn * (k * nm + m); };
+    std::vector<std::vector<const void *>> R(terms.size(), std::vector<const void *>(G));
+    // R passes
+    for (size_t k = 0; k < terms.size(); k++)
+        R[k][G - 1] = c->at(A, c->off[(int)c->rhs[c->mf_chain_terms[terms[k]]] - 1]);
+    for (int g = G - 2; g >= 0; g--) {
+        std::vector<TaskBuilder> tasks;
+        for (size_t k = 0; k < terms.size(); k++) {
+            const int t = c->mf_chain_terms[terms[k]];
+            const int a = (int)c->lhs[t] - 1, b = (int)c->rhs[t] - 1;
+            const int g1 = g + 1;
+            std::vector<int> ms;
+            for (int m = c->groups[g1].first; m < c->groups[g1].second; m++)
+                if (is_op(t, m)) ms.push_back(m);
+            if (ms.empty()) {
+                R[k][g] = R[k][g1];
+                continue;
+            }
+            std::vector<int64_t> sz(nm);
+            for (int m = 0; m < nm; m++) sz[m] = c->NS(m, b);
+            TaskBuilder tb(c, g1, prod_range(sz, 0, c->groups[g1].first), prod_range(sz, c->groups[g1].second, nm));
+            tb.load(0, R[k][g1], sz);
+            for (int m : ms) {
+                const int op = c->iOper[m + (size_t)nm * t];
+                tb.op(0, 0, m, c->h1op(m, op), 0, (int)c->NS(m, a), 0);
+            }
+            tb.store(0, tensor(k, g), 0);
+            R[k][g] = tensor(k, g);
+            tasks.push_back(tb);
+        }
+        pb.phase(tasks);
+    }
+    // Gram passes
+    std::vector<const void *> Lcur(terms.size());
+    for (size_t k = 0; k < terms.size(); k++)
+        Lcur[k] = c->at(A, c->off[(int)c->lhs[c->mf_chain_terms[terms[k]]] - 1]);
+    int last_mode_with_op = -1;
+    for (int g = 0; g < G; g++) {
+        std::vector<TaskBuilder> tasks;
+        for (size_t k = 0; k < terms.size(); k++) {
+            const int t = c->mf_chain_terms[terms[k]];
+            const int a = (int)c->lhs[t] - 1, b = (int)c->rhs[t] - 1;
+            (void)a;
+            std::vector<int> ms;
+            for (int m = c->groups[g].first; m < c->groups[g].second; m++)
+                if (is_op(t, m)) ms.push_back(m);
+            if (ms.empty()) continue;
+            last_mode_with_op = -1;
+            for (int m = nm - 1; m >= 0; m--)
+                if (is_op(t, m)) { last_mode_with_op = m; break; }
+            std::vector<int64_t> sz(nm);
+            for (int m = 0; m < nm; m++) sz[m] = c->NS(m, b);
+            TaskBuilder tb(c, g, prod_range(sz, 0, c->groups[g].first), prod_range(sz, c->groups[g].second, nm));
+            tb.load(0, Lcur[k], sz);
+            tb.load(1, R[k][g], sz);
+            for (size_t i = 0; i < ms.size(); i++) {
+                const int m = ms[i];
+                int rsrc = 1;
+                for (size_t j = i + 1; j < ms.size(); j++) {
+                    const int op = c->iOper[ms[j] + (size_t)nm * t];
+                    tb.op(rsrc, 2, ms[j], c->h1op(ms[j], op), 0, (int)sz[ms[j]], 0);
+                    rsrc = 2;
+                }
+                tb.gram(0, rsrc, m, slot(k, m));
+                if (m != last_mode_with_op) {
+                    const int op = c->iOper[m + (size_t)nm * t];
+                    tb.op(0, 0, m, c->h1op(m, op), 1, (int)sz[m], 0);
+                }
+            }
+            if (g + 1 < G) {
+                // L moves on only if a later group still has Grams for this term
+                void *Lout = tensor(k, G - 1);
+                tb.store(0, Lout, 0);
+                Lcur[k] = Lout;
+            }
+            tasks.push_back(tb);
+        }
+        pb.phase(tasks);
+    }
+    return pb.finish(c->prec);
+}
+
+// Combination table: MF(m, op) = sum ae * rho (one-mode terms) + sum ae G_t(m)
+// + sum conj(ae) G_t(m)^H (mirror terms), for every (m, op).
+void build_mf_combine(mg_ctx *c) {
+    const int nm = c->nm;
+    const size_t nn = (size_t)c->nspfmax * c->nspfmax;
+    std::vector<int> terms;
+    for (size_t ti = 0; ti < c->mf_chain_terms.size(); ti++)
+        if (c->mf_partner[ti] != -2) terms.push_back((int)ti);
+    const int nrow = nm * c->nopmax;
+    std::vector<std::vector<std::tuple<const double2 *, double2, int>>> rows(nrow);
+    {   // identity matrix for ae * 1 contributions
+        std::vector<double2> I(nn, make_double2(0, 0));
+        for (int j = 0; j < c->nspfmax; j++) I[j + (size_t)c->nspfmax * j] = make_double2(1, 0);
+        c->dident.alloc(sizeof(double2) * nn);
+        HIP_CHECK(hipMemcpy(c->dident.p, I.data(), sizeof(double2) * nn, hipMemcpyHostToDevice));
+    }
+    for (int r = 0; r < nrow; r++)
+        for (int q = c->mf1_rowptr[r]; q < c->mf1_rowptr[r + 1]; q++) {
+            const double2 *src = c->mf1_src[q] >= 0 ? c->drho.as<double2>() + nn * c->mf1_src[q] : c->dident.as<double2>();
+            rows[r].push_back({src, d2(c->mf1_coef[q]), 0});
+        }
+    for (size_t k = 0; k < terms.size(); k++) {
+        const int ti = terms[k];
+        const int t = c->mf_chain_terms[ti];
+        const int partner = c->mf_partner[ti];
+        for (int m = 0; m < nm; m++) {
+            const int op = c->iOper[m + (size_t)nm * t];
+            if (c->isIdent[m + (size_t)nm * op]) continue;
+            const double2 *G = c->dgslots.as<double2>() + nn * (k * nm + m);
+            rows[m * c->nopmax + op].push_back({G, d2(c->ae[t]), 0});
+            if (partner >= 0) {
+                const int op2 = c->iOper[m + (size_t)nm * partner];
+                rows[m * c->nopmax + op2].push_back({G, d2(std::conj(c->ae[t])), 1});
+            }
+        }
+    }
+    std::vector<double2 *> dst(nrow);
+    std::vector<int> rowptr(1, 0), trans;
+    std::vector<const double2 *> src;
+    std::vector<double2> coef;
+    for (int r = 0; r < nrow; r++) {
+        dst[r] = c->dmf.as<double2>() + nn * r;
+        for (auto &e : rows[r]) {
+            src.push_back(std::get<0>(e));
+            coef.push_back(std::get<1>(e));
+            trans.push_back(std::get<2>(e));
+        }
+        rowptr.push_back((int)src.size());
+    }
+    auto up = [&](DevBuf &b, const void *h, size_t bytes) {
+        b.alloc(std::max<size_t>(bytes, 8));
+        if (bytes) HIP_CHECK(hipMemcpy(b.p, h, bytes, hipMemcpyHostToDevice));
+    };
+    up(c->dmf_dst, dst.data(), sizeof(double2 *) * dst.size());
+    up(c->dmf_rowptr, rowptr.data(), sizeof(int) * rowptr.size());
+    up(c->dmf_src, src.data(), sizeof(double2 *) * src.size());
+    up(c->dmf_coef, coef.data(), sizeof(double2) * coef.size());
+    up(c->dmf_trans, trans.data(), sizeof(int) * trans.size());
+}
+
+void calc_meanfield_bricks(mg_ctx *c, const double2 *A64, bool af_ready) {
+    const void *A = A64;
+    if (c->prec) {
+        if (!af_ready) launch_convert(c->dAf.p, 1, A64, 0, c->nconf, c->stream);
+        A = c->dAf.p;
+    }
+    PlanKey key{(const void *)7, A, nullptr, nullptr};
+    auto it = c->plans.find(key);
+    if (it == c->plans.end()) {
+        it = c->plans.emplace(key, build_meanfield(c, A)).first;
+        if (!c->dmf_rowptr.p) build_mf_combine(c);
+    }
+    if (!c->mf_chain_terms.empty()) run_plan(c, *it->second);
+    launch_mat_combine(c->dmf_dst.as<double2 *>(), c->dmf_rowptr.as<int>(), c->dmf_src.as<const double2 *>(),
+                       c->dmf_coef.as<double2>(), c->dmf_trans.as<int>(), c->nm * c->nopmax, c->nspfmax, c->stream);
+    launch_rmf(c->drho.as<double2>(), c->dmf.as<double2>(), c->drmf.as<double2>(), c->drmf_jobs.as<int>(),
+               c->n_rmf_jobs, c->nspfmax, c->nopmax, c->eps_rho, c->stream);
+    check_launch();
+}
+
 // Calc_MeanField, Hamiltonian part (gmctdh_objects.f90:518-605) + Calc_rhom1MF
 // (:1147-1194).  Requires calc_rho(A) and calc_h1mode(psi).
 // af_ready: the mixed form's dAf already holds A (calc_rho(A) just ran).
 void calc_meanfield(mg_ctx *c, const double2 *A64, bool af_ready = false) {
+    if (c->mf_bricks) {
+        calc_meanfield_bricks(c, A64, af_ready);
+        return;
+    }
     // one-mode same-state terms: MF += ae * rho  (all other modes are identity)
     launch_hsum(c->dmf.as<double2>(), c->drho.as<double2>(), c->dmf1_rowptr.as<int>(), c->dmf1_src.as<int>(),
                 c->dmf1_coef.as<double2>(), c->nm * c->nopmax, c->nspfmax, c->stream);

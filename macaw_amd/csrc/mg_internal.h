@@ -90,7 +90,12 @@ struct GramTask {
 constexpr int kMaxBrickInstr = 40;
 constexpr int kMaxBrickMats = 24;
 constexpr int kMaxBrickBufs = 3;
-enum BrickCode { kBLoad = 0, kBLoadAdd = 1, kBOp = 2, kBAxpy = 3, kBStore = 4, kBZero = 5 };
+//   kBGram     slot (mat) += sum over the brick's fibers of mode p of
+//              conj(a[f, i]) b[f, j], i < ni, j < no (ni, no <= 8): partial
+//              Gram/hole contractions, FP64 across bricks, one partial per
+//              workgroup in the launch's partial array (see launch_brick_tasks)
+constexpr int kMaxGrams = 8;
+enum BrickCode { kBLoad = 0, kBLoadAdd = 1, kBOp = 2, kBAxpy = 3, kBStore = 4, kBZero = 5, kBGram = 6 };
 struct BrickInstr {
     int code;
     int a, b;
@@ -107,6 +112,7 @@ struct BrickTask {
     int64_t L, Lt, R, Rt;
     FastDiv fd_Lt;
     int ninstr, nmat, ld;
+    int ngram, gram_base;        // Gram slots used, index of the first in the launch's partials
     const void *mat[kMaxBrickMats];
     int matadj[kMaxBrickMats];
     int matnin[kMaxBrickMats];   // columns of the effective matrix
@@ -169,8 +175,17 @@ struct SpfParams {
 
 // Host-side launchers (mg_kernels.hip).  prec: 0 = complex FP64 storage,
 // 1 = complex FP32 storage (reductions always FP64).
+// Gram launches (ngram_total > 0) run a persistent grid of `gram_blocks`
+// workgroups over the bricks and write partials[(slot * 64 + e) * gram_blocks
+// + block] (e = i + 8 j); launch_gram_slots_final sums them into G[slot].
 void launch_brick_tasks(const BrickTask *d_tasks, int ntasks, int64_t max_blocks_per_task, int nmax,
-                        int brick, int nbuf, int nmat, int prec, hipStream_t s);
+                        int brick, int nbuf, int nmat, int prec, hipStream_t s, int ngram_total = 0,
+                        double2 *partials = nullptr, int gram_blocks = 0);
+void launch_gram_slots_final(const double2 *partials, int nslots, int nblocks, double2 *const *targets,
+                             int ld, hipStream_t s);
+void launch_herm_batch(double2 *const *targets, const int *n, int count, int ld, hipStream_t s);
+void launch_mat_combine(double2 *const *dst, const int *rowptr, const double2 *const *src, const double2 *coef,
+                        const int *trans, int ndst, int ld, hipStream_t s);
 void launch_mode_product(const ModeTask &t, int nmax, int prec, hipStream_t s);
 void launch_gram(const GramTask &t, double2 *partials, int nblocks, int nmax, int prec, hipStream_t s);
 void launch_spf_tasks(const SpfTask *d_tasks, int ntasks, DofTables dt, int max_ng, hipStream_t s);

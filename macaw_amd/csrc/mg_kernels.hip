@@ -55,6 +55,12 @@ __device__ __forceinline__ double2 caddT(double2 a, double2 b) { return cadd(a, 
 __device__ __forceinline__ float2 cconjT(float2 a) { return make_float2(a.x, -a.y); }
 __device__ __forceinline__ double2 cconjT(double2 a) { return cconj(a); }
 
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
 // LDS index padding: one slot every 8 complex values breaks the bank
 // conflicts of contiguous fibers (stride-1 mode inside a brick).
 __device__ __forceinline__ int pad(int i) { return i + (i >> 3); }
@@ -263,15 +269,70 @@ __device__ __forceinline__ void brick_op_generic(const T *src, T *dst, const T *
     }
 }
 
-template <int NMAX, class T>
-__global__ void __launch_bounds__(1024) k_brick(const BrickTask *__restrict__ tasks, int capacity, int nmat_lds) {
+// Gram accumulation on the matrix cores: G = X^H Y over the fibers of one
+// mode is the real 16x16 product [Xr Xi]^T [Yr Yi] (K = fibers, 4 per MFMA):
+// Re G = D[a][b] + D[a+8][b+8], Im G = D[a][b+8] - D[a+8][b].  Lane l holds
+// the real part (l % 16 < 8) or imaginary part of element l % 8 of fiber
+// 4 s + l / 16 for both operands.  FP32 MFMA within a brick (<= 256 fibers),
+// FP64 across bricks.
+template <class T> struct GramMfma;
+template <> struct GramMfma<float2> {
+    static __device__ __forceinline__ void run(const float2 *X, const float2 *Y, int n1, int n2, int nfib,
+                                               int stride, FastDiv fds, double acc[4]) {
+        const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+        const int i = lane & 15, q = lane >> 4, e = i & 7, im = i >> 3;
+        mg_floatx4 d = {0.f, 0.f, 0.f, 0.f};
+        const int nsteps = (nfib + 3) >> 2;
+        for (int st = wave; st < nsteps; st += nw) {
+            const int f = 4 * st + q;
+            const int ff = f < nfib ? f : 0;
+            const int outer = (int)fdiv((uint32_t)ff, fds), inner = ff - outer * stride;
+            const float2 xv = X[pad(inner + outer * stride * n1 + (e < n1 ? e : 0) * stride)];
+            const float2 yv = Y[pad(inner + outer * stride * n2 + (e < n2 ? e : 0) * stride)];
+            const float av = (f < nfib && e < n1) ? (im ? xv.y : xv.x) : 0.f;
+            const float bv = (f < nfib && e < n2) ? (im ? yv.y : yv.x) : 0.f;
+            d = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bv, d, 0, 0, 0);
+        }
+#pragma unroll
+        for (int v = 0; v < 4; v++) acc[v] += (double)d[v];
+    }
+    // D row held in register v by lane group q
+    static __device__ __forceinline__ int row(int q, int v) { return 4 * q + v; }
+};
+template <> struct GramMfma<double2> {
+    static __device__ __forceinline__ void run(const double2 *X, const double2 *Y, int n1, int n2, int nfib,
+                                               int stride, FastDiv fds, double acc[4]) {
+        const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+        const int i = lane & 15, q = lane >> 4, e = i & 7, im = i >> 3;
+        mg_doublex4 d = {0.0, 0.0, 0.0, 0.0};
+        const int nsteps = (nfib + 3) >> 2;
+        for (int st = wave; st < nsteps; st += nw) {
+            const int f = 4 * st + q;
+            const int ff = f < nfib ? f : 0;
+            const int outer = (int)fdiv((uint32_t)ff, fds), inner = ff - outer * stride;
+            const double2 xv = X[pad(inner + outer * stride * n1 + (e < n1 ? e : 0) * stride)];
+            const double2 yv = Y[pad(inner + outer * stride * n2 + (e < n2 ? e : 0) * stride)];
+            const double av = (f < nfib && e < n1) ? (im ? xv.y : xv.x) : 0.0;
+            const double bv = (f < nfib && e < n2) ? (im ? yv.y : yv.x) : 0.0;
+            d = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, d, 0, 0, 0);
+        }
+#pragma unroll
+        for (int v = 0; v < 4; v++) acc[v] += d[v];
+    }
+    static __device__ __forceinline__ int row(int q, int v) { return 4 * v + q; }
+};
+
+// GRAM: the launch carries kBGram instructions; the grid is persistent over
+// the bricks and every workgroup ends by writing its Gram partials.
+template <int NMAX, class T, bool GRAM>
+__global__ void __launch_bounds__(GRAM ? 256 : 1024) k_brick(const BrickTask *__restrict__ tasks, int capacity, int nmat_lds,
+                                                double2 *__restrict__ partials) {
     const BrickTask &t = tasks[blockIdx.y];
     const int64_t L = t.L, R = t.R;
     const int Lt = (int)t.Lt, Rt = (int)t.Rt;
     const int64_t nlb = L / Lt, nrb = R / Rt;
-    const int64_t b = blockIdx.x;
-    if (b >= nlb * nrb) return;
-    const int64_t l0 = (b % nlb) * Lt, r0 = (b / nlb) * Rt;
+    const int64_t nbricks = nlb * nrb;
+    if (!GRAM && (int64_t)blockIdx.x >= nbricks) return;
     const FastDiv fd_Lt = t.fd_Lt;
     extern __shared__ char lds_raw[];
     T *mats = reinterpret_cast<T *>(lds_raw);
@@ -290,69 +351,196 @@ __global__ void __launch_bounds__(1024) k_brick(const BrickTask *__restrict__ ta
         mats[e] = v;
     }
     __syncthreads();
+    double gacc[GRAM ? kMaxGrams : 1][4];
+#pragma unroll
+    for (int g = 0; g < (GRAM ? kMaxGrams : 1); g++)
+#pragma unroll
+        for (int v = 0; v < 4; v++) gacc[g][v] = 0.0;
     const int nins = t.ninstr;
-    for (int k = 0; k < nins; k++) {
-        const BrickInstr &I = t.ins[k];
-        const int code = I.code;
-        T *A = bufs + (size_t)I.a * capacity;
-        T *B = bufs + (size_t)I.b * capacity;
-        if (code == kBLoad || code == kBLoadAdd) {
-            brick_load(A, static_cast<const T *>(I.ptr), l0, r0, L, Lt, Rt, fd_Lt, I.M, I.fd_M, code == kBLoadAdd,
-                       from_d2<T>(I.alpha));
-        } else if (code == kBOp) {
-            const T *Mq = mats + I.mat * NMAX * NMAX;
-            if (NMAX == 8 && I.ni == 8 && I.no == 8)
-                mfma_op8<T>(A, B, Mq, I.nfib, I.stride, I.fd_stride, I.acc);
-            else
-                brick_op_generic<NMAX, T>(A, B, Mq, I.ni, I.no, I.nfib, I.stride, I.fd_stride, I.acc);
-        } else if (code == kBAxpy) {
-            const int tot = Lt * I.M * Rt;
-            const T al = from_d2<T>(I.alpha);
-            for (int idx = tid; idx < tot; idx += nth) B[pad(idx)] = caddT(B[pad(idx)], cmulT(al, A[pad(idx)]));
-        } else if (code == kBStore) {
-            brick_store(A, static_cast<T *>(const_cast<void *>(I.ptr)), l0, r0, L, Lt, Rt, fd_Lt, I.M, I.fd_M,
-                        I.acc, from_d2<T>(I.alpha));
-        } else if (code == kBZero) {
-            const int tot = Lt * I.M * Rt;
-            for (int idx = tid; idx < tot; idx += nth) A[pad(idx)] = zeroT<T>();
+    for (int64_t bi = blockIdx.x; bi < nbricks; bi += (GRAM ? gridDim.x : nbricks)) {
+        const int64_t l0 = (bi % nlb) * Lt, r0 = (bi / nlb) * Rt;
+        for (int k = 0; k < nins; k++) {
+            const BrickInstr &I = t.ins[k];
+            const int code = I.code;
+            T *A = bufs + (size_t)I.a * capacity;
+            T *B = bufs + (size_t)I.b * capacity;
+            if (code == kBLoad || code == kBLoadAdd) {
+                brick_load(A, static_cast<const T *>(I.ptr), l0, r0, L, Lt, Rt, fd_Lt, I.M, I.fd_M,
+                           code == kBLoadAdd, from_d2<T>(I.alpha));
+            } else if (code == kBOp) {
+                const T *Mq = mats + I.mat * NMAX * NMAX;
+                if (NMAX == 8 && I.ni == 8 && I.no == 8)
+                    mfma_op8<T>(A, B, Mq, I.nfib, I.stride, I.fd_stride, I.acc);
+                else
+                    brick_op_generic<NMAX, T>(A, B, Mq, I.ni, I.no, I.nfib, I.stride, I.fd_stride, I.acc);
+            } else if (code == kBAxpy) {
+                const int tot = Lt * I.M * Rt;
+                const T al = from_d2<T>(I.alpha);
+                for (int idx = tid; idx < tot; idx += nth)
+                    B[pad(idx)] = caddT(B[pad(idx)], cmulT(al, A[pad(idx)]));
+            } else if (code == kBStore) {
+                brick_store(A, static_cast<T *>(const_cast<void *>(I.ptr)), l0, r0, L, Lt, Rt, fd_Lt, I.M, I.fd_M,
+                            I.acc, from_d2<T>(I.alpha));
+            } else if (code == kBZero) {
+                const int tot = Lt * I.M * Rt;
+                for (int idx = tid; idx < tot; idx += nth) A[pad(idx)] = zeroT<T>();
+            } else if (GRAM && code == kBGram) {
+                double part[4] = {0.0, 0.0, 0.0, 0.0};
+                GramMfma<T>::run(A, B, I.ni, I.no, I.nfib, I.stride, I.fd_stride, part);
+#pragma unroll
+                for (int g = 0; g < (GRAM ? kMaxGrams : 1); g++)
+                    if (g == I.mat)
+#pragma unroll
+                        for (int v = 0; v < 4; v++) gacc[g][v] += part[v];
+            }
+            __syncthreads();
         }
-        __syncthreads();
+    }
+    if (GRAM) {
+        // per slot: waves' 16x16 partial D -> LDS -> 8x8 complex partial
+        double *red = reinterpret_cast<double *>(lds_raw);
+        const int lane = tid & 63, wave = tid >> 6, nw = nth >> 6;
+        const int q = lane >> 4, col = lane & 15;
+#pragma unroll
+        for (int g = 0; g < (GRAM ? kMaxGrams : 1); g++) {
+            if (g >= t.ngram) break;
+#pragma unroll
+            for (int v = 0; v < 4; v++) red[wave * 256 + GramMfma<T>::row(q, v) * 16 + col] = gacc[g][v];
+            __syncthreads();
+            if (tid < 64) {
+                const int a = tid & 7, b = tid >> 3;
+                double d00 = 0.0, d11 = 0.0, d01 = 0.0, d10 = 0.0;
+                for (int w = 0; w < nw; w++) {
+                    const double *D = red + w * 256;
+                    d00 += D[a * 16 + b];
+                    d11 += D[(a + 8) * 16 + b + 8];
+                    d01 += D[a * 16 + b + 8];
+                    d10 += D[(a + 8) * 16 + b];
+                }
+                partials[((int64_t)(t.gram_base + g) * 64 + tid) * gridDim.x + blockIdx.x] =
+                    make_double2(d00 + d11, d01 - d10);
+            }
+            __syncthreads();
+        }
     }
 }
 
-template <int NMAX, class T>
+template <int NMAX, class T, bool GRAM>
 static void launch_brick_nmax(const BrickTask *d_tasks, int ntasks, int64_t max_blocks, int capacity, int nbuf,
-                              int nmat, hipStream_t s) {
+                              int nmat, hipStream_t s, double2 *partials) {
     static bool attr_set = false;
-    const size_t shm = sizeof(T) * ((size_t)nmat * NMAX * NMAX + (size_t)nbuf * capacity);
-    if (!attr_set) {
-        (void)hipFuncSetAttribute((const void *)k_brick<NMAX, T>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                  160 * 1024);
-        attr_set = true;
-    }
     static const int nthreads = [] {
         const char *e = std::getenv("MG_THREADS");
         const int v = e ? std::atoi(e) : 256;
         return (v == 128 || v == 256 || v == 512 || v == 1024) ? v : 256;
     }();
-    hipLaunchKernelGGL((k_brick<NMAX, T>), dim3((unsigned)max_blocks, (unsigned)ntasks), dim3(nthreads), shm, s,
-                       d_tasks, capacity, nmat);
+    size_t shm = sizeof(T) * ((size_t)nmat * NMAX * NMAX + (size_t)nbuf * capacity);
+    if (GRAM) shm = std::max(shm, sizeof(double) * 256 * 4);
+    if (!attr_set) {
+        (void)hipFuncSetAttribute((const void *)k_brick<NMAX, T, GRAM>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  160 * 1024);
+        attr_set = true;
+    }
+    hipLaunchKernelGGL((k_brick<NMAX, T, GRAM>), dim3((unsigned)max_blocks, (unsigned)ntasks), dim3(GRAM ? 256 : nthreads), shm,
+                       s, d_tasks, capacity, nmat, partials);
 }
 
-template <class T>
+template <class T, bool GRAM>
 static void launch_brick_T(const BrickTask *d_tasks, int ntasks, int64_t mb, int nmax, int capacity, int nbuf,
-                           int nmat, hipStream_t s) {
-    if (nmax <= 4) launch_brick_nmax<4, T>(d_tasks, ntasks, mb, capacity, nbuf, nmat, s);
-    else if (nmax <= 8) launch_brick_nmax<8, T>(d_tasks, ntasks, mb, capacity, nbuf, nmat, s);
-    else launch_brick_nmax<16, T>(d_tasks, ntasks, mb, capacity, nbuf, nmat, s);
+                           int nmat, hipStream_t s, double2 *partials) {
+    if (nmax <= 4) launch_brick_nmax<4, T, GRAM>(d_tasks, ntasks, mb, capacity, nbuf, nmat, s, partials);
+    else if (nmax <= 8) launch_brick_nmax<8, T, GRAM>(d_tasks, ntasks, mb, capacity, nbuf, nmat, s, partials);
+    else launch_brick_nmax<16, T, GRAM>(d_tasks, ntasks, mb, capacity, nbuf, nmat, s, partials);
 }
 
 void launch_brick_tasks(const BrickTask *d_tasks, int ntasks, int64_t max_blocks_per_task, int nmax, int brick,
-                        int nbuf, int nmat, int prec, hipStream_t s) {
+                        int nbuf, int nmat, int prec, hipStream_t s, int ngram_total, double2 *partials,
+                        int gram_blocks) {
     if (ntasks <= 0 || max_blocks_per_task <= 0) return;
     const int capacity = group_capacity(brick);
-    if (prec == 0) launch_brick_T<double2>(d_tasks, ntasks, max_blocks_per_task, nmax, capacity, nbuf, nmat, s);
-    else launch_brick_T<float2>(d_tasks, ntasks, max_blocks_per_task, nmax, capacity, nbuf, nmat, s);
+    if (ngram_total > 0) {
+        if (prec == 0)
+            launch_brick_T<double2, true>(d_tasks, ntasks, gram_blocks, nmax, capacity, nbuf, nmat, s, partials);
+        else
+            launch_brick_T<float2, true>(d_tasks, ntasks, gram_blocks, nmax, capacity, nbuf, nmat, s, partials);
+        return;
+    }
+    if (prec == 0)
+        launch_brick_T<double2, false>(d_tasks, ntasks, max_blocks_per_task, nmax, capacity, nbuf, nmat, s, nullptr);
+    else
+        launch_brick_T<float2, false>(d_tasks, ntasks, max_blocks_per_task, nmax, capacity, nbuf, nmat, s, nullptr);
+}
+
+// One wave per (slot, entry): the slot's block partials summed in a fixed
+// order into targets[slot][a + ld b] (a, b < min(8, ld)).
+__global__ void __launch_bounds__(256) k_gram_slots_final(const double2 *partials, int nslots, int nblocks,
+                                                          double2 *const *targets, int ld) {
+    const int lane = threadIdx.x & 63;
+    const int w = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    if (w >= nslots * 64) return;
+    const int slot = w >> 6, e = w & 63, a = e & 7, b = e >> 3;
+    const double2 *p = partials + (int64_t)w * nblocks;
+    double sx = 0.0, sy = 0.0;
+    for (int k = lane; k < nblocks; k += 64) {
+        sx += p[k].x;
+        sy += p[k].y;
+    }
+    sx = wave_sum(sx);
+    sy = wave_sum(sy);
+    if (lane == 0 && a < ld && b < ld) targets[slot][a + ld * b] = make_double2(sx, sy);
+}
+
+void launch_gram_slots_final(const double2 *partials, int nslots, int nblocks, double2 *const *targets, int ld,
+                             hipStream_t s) {
+    if (nslots <= 0) return;
+    const int waves = nslots * 64;
+    hipLaunchKernelGGL(k_gram_slots_final, dim3((unsigned)((waves + 3) / 4)), dim3(256), 0, s, partials, nslots,
+                       nblocks, targets, ld);
+}
+
+// Exact Hermitian symmetry for a batch of n x n matrices (upper triangle wins).
+__global__ void k_herm_batch(double2 *const *targets, const int *nv, int ld) {
+    double2 *G = targets[blockIdx.x];
+    const int n = nv[blockIdx.x];
+    for (int e = threadIdx.x; e < n * n; e += blockDim.x) {
+        const int a = e % n, b = e / n;
+        if (a == b) G[a + ld * a].y = 0.0;
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < n * n; e += blockDim.x) {
+        const int a = e % n, b = e / n;
+        if (a > b) G[a + ld * b] = cconj(G[b + ld * a]);
+    }
+}
+
+void launch_herm_batch(double2 *const *targets, const int *n, int count, int ld, hipStream_t s) {
+    if (count <= 0) return;
+    hipLaunchKernelGGL(k_herm_batch, dim3(count), dim3(256), 0, s, targets, n, ld);
+}
+
+// dst[t] = sum_{c in row t} coef[c] * (trans[c] ? src[c]^H : src[c]), ld x ld
+// matrices, overwriting dst[t]; one thread per (t, entry), fixed order.
+__global__ void k_mat_combine(double2 *const *dst, const int *rowptr, const double2 *const *src, const double2 *coef,
+                              const int *trans, int ndst, int ld) {
+    const int nn = ld * ld;
+    const int64_t id = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (id >= (int64_t)ndst * nn) return;
+    const int t = (int)(id / nn), e = (int)(id % nn);
+    const int a = e % ld, b = e / ld;
+    double2 acc = zero2();
+    for (int c = rowptr[t]; c < rowptr[t + 1]; c++) {
+        const double2 v = trans[c] ? cconj(src[c][b + ld * a]) : src[c][e];
+        acc = cadd(acc, cmul(coef[c], v));
+    }
+    dst[t][e] = acc;
+}
+
+void launch_mat_combine(double2 *const *dst, const int *rowptr, const double2 *const *src, const double2 *coef,
+                        const int *trans, int ndst, int ld, hipStream_t s) {
+    if (ndst <= 0) return;
+    const int64_t tot = (int64_t)ndst * ld * ld;
+    hipLaunchKernelGGL(k_mat_combine, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, dst, rowptr, src, coef,
+                       trans, ndst, ld);
 }
 
 // ---------------------------------------------------------------------------
@@ -415,11 +603,6 @@ void launch_mode_product(const ModeTask &t0, int nmax, int prec, hipStream_t s) 
 // ---------------------------------------------------------------------------
 // Hole contraction, stage 1: per-block partial n1 x n2 sums (chunked to 8x8),
 // accumulated in FP64 whatever the storage type.
-__device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-    return v;
-}
 
 // The positions (l, r) of the contraction are cut into tiles of Lt x Rt
 // (Lt | ll, and Rt | nl when Lt == ll) so that each tile is a set of

@@ -215,7 +215,8 @@ def test_full_size_apply_h_invariants():
 def test_apply_h_small_bricks_match_oracle(name, brick, tmp_path, monkeypatch):
     """Small LDS bricks force three or more mode groups (first, middle and
     final passes of the H.A schedule, chain temporaries updated in place or,
-    for t2's state-dependent SPF counts, through a second buffer)."""
+    for t2's state-dependent SPF counts, through a second buffer; the R and
+    Gram passes of the mean fields)."""
     monkeypatch.setenv("MG_BRICK", brick)
     prob = make_problem(name, str(tmp_path))
     A, psi = perturbed_state(prob, 5)
@@ -229,6 +230,12 @@ def test_apply_h_small_bricks_match_oracle(name, brick, tmp_path, monkeypatch):
     for _ in range(2):
         B = rand_c(rng, prob.nAConftot)
         assert close(e.apply_h(B), o.apply_h(B), 1e-12)
+    # density matrices and mean fields through the same mode groups
+    assert close(e.rho(), o.rho(), 1e-12)
+    mf_o, rmf_o, _ = o.meanfield()
+    mf_e, rmf_e = e.meanfield()
+    assert close(mf_e, mf_o, 1e-12)
+    assert close(rmf_e, rmf_o, 1e-8)
     # norm_wf / WF_Ovl chains run through the same brick programs
     outs_o, pops_o, atts_o = Oracle(prob).propagate(0.0, 0.5 / 0.02418884326505)
     e2 = engine_for(prob)
