@@ -856,6 +856,26 @@ static double err_spf(orc *o, int64_t m, const cplx *p1, const cplx *p2) {
     return sqrt(fabs(error));
 }
 
+/* Cash-Karp (1990) and Dormand-Prince (1980) tableaux, shared by the SPF
+ * integrators (cmf2_integration.f90:107-412) and the VMF ones
+ * (integrator.f90:95-221). */
+static const double ckB[6][5] = {
+    {0, 0, 0, 0, 0}, {0.2, 0, 0, 0, 0}, {0.075, 0.225, 0, 0, 0}, {0.3, -0.9, 1.2, 0, 0},
+    {-11.0 / 54.0, 2.5, -70.0 / 27.0, 35.0 / 27.0, 0},
+    {1631.0 / 55296.0, 175.0 / 512.0, 575.0 / 13824.0, 44275.0 / 110592.0, 253.0 / 4096.0}};
+static const double ckC[6] = {37.0 / 378.0, 0, 250.0 / 621.0, 125.0 / 594.0, 0, 512.0 / 1771.0};
+static const double ckD[6] = {2825.0 / 27648.0, 0, 18575.0 / 48384.0, 13525.0 / 55296.0,
+                              277.0 / 14336.0, 0.25};
+static const double dpB[6][5] = {
+    {0, 0, 0, 0, 0}, {0.2, 0, 0, 0, 0}, {0.075, 0.225, 0, 0, 0},
+    {44.0 / 45.0, -56.0 / 15.0, 32.0 / 9.0, 0, 0},
+    {19372.0 / 6561.0, -25360.0 / 2187.0, 64448.0 / 6561.0, -212.0 / 729.0, 0},
+    {9017.0 / 3168.0, -355.0 / 33.0, 46732.0 / 5247.0, 49.0 / 176.0, -5103.0 / 18656.0}};
+static const double dpC[6] = {35.0 / 384.0, 0, 500.0 / 1113.0, 125.0 / 192.0,
+                              -2187.0 / 6784.0, 11.0 / 84.0};
+static const double dpD[7] = {5179.0 / 57600.0, 0, 7571.0 / 16695.0, 393.0 / 640.0,
+                              -92097.0 / 339200.0, 187.0 / 2100.0, 0.025};
+
 /* rk_SPF_ck / rk_SPF_dp (cmf2_integration.f90:107-412) on the blocks of mode m. */
 static int rk_spf(orc *o, int64_t m, cplx *psi0, double htry, double *hdid, double *hnext) {
     const double safety = 0.9, pgrow = -0.2, pshrnk = -0.25, minscal = 0.1, maxscal = 5.0;
@@ -865,23 +885,6 @@ static int rk_spf(orc *o, int64_t m, cplx *psi0, double htry, double *hdid, doub
     int64_t nsets = o->MultiSet[m] ? o->ns : 1;
     cplx *k[7], *tmp = xcalloc(D, sizeof(cplx)), *perr = xcalloc(D, sizeof(cplx));
     for (int i = 0; i < 7; i++) k[i] = xcalloc(D, sizeof(cplx));
-    /* Cash-Karp (1990) and Dormand-Prince (1980) tableaux */
-    static const double ckB[6][5] = {
-        {0, 0, 0, 0, 0}, {0.2, 0, 0, 0, 0}, {0.075, 0.225, 0, 0, 0}, {0.3, -0.9, 1.2, 0, 0},
-        {-11.0 / 54.0, 2.5, -70.0 / 27.0, 35.0 / 27.0, 0},
-        {1631.0 / 55296.0, 175.0 / 512.0, 575.0 / 13824.0, 44275.0 / 110592.0, 253.0 / 4096.0}};
-    static const double ckC[6] = {37.0 / 378.0, 0, 250.0 / 621.0, 125.0 / 594.0, 0, 512.0 / 1771.0};
-    static const double ckD[6] = {2825.0 / 27648.0, 0, 18575.0 / 48384.0, 13525.0 / 55296.0,
-                                  277.0 / 14336.0, 0.25};
-    static const double dpB[6][5] = {
-        {0, 0, 0, 0, 0}, {0.2, 0, 0, 0, 0}, {0.075, 0.225, 0, 0, 0},
-        {44.0 / 45.0, -56.0 / 15.0, 32.0 / 9.0, 0, 0},
-        {19372.0 / 6561.0, -25360.0 / 2187.0, 64448.0 / 6561.0, -212.0 / 729.0, 0},
-        {9017.0 / 3168.0, -355.0 / 33.0, 46732.0 / 5247.0, 49.0 / 176.0, -5103.0 / 18656.0}};
-    static const double dpC[6] = {35.0 / 384.0, 0, 500.0 / 1113.0, 125.0 / 192.0,
-                                  -2187.0 / 6784.0, 11.0 / 84.0};
-    static const double dpD[7] = {5179.0 / 57600.0, 0, 7571.0 / 16695.0, 393.0 / 640.0,
-                                  -92097.0 / 339200.0, 187.0 / 2100.0, 0.025};
     double dt = htry, error = 0.0;
     int rc = 0;
     for (;;) {
@@ -1232,6 +1235,166 @@ int orc_propagate(orc *o, double t0, double tfinal, mg_output *outs, double *pop
         }
     }
     return 0;
+}
+
+/* ---------------------------------------------------------------------- */
+/* Variable mean field (VMF) integration, DVR modes, npackets = 1.          */
+
+/* Calc_Derivs (derivatives.f90:39-243): mean fields from (A, psi);
+ * dA = conj(Adot_tmp) = der_factor * H A (gmctdh_objects.f90:578,
+ * derivatives.f90:156, 237); dpsi from the DVR formula of every mode
+ * (derivatives.f90:81-143, the same as Calc_Der_SPF's). */
+static int calc_derivs(orc *o, const cplx *A0, const cplx *psi0, cplx *dA, cplx *dpsi) {
+    calc_rho(o, A0);
+    for (int64_t m = 0; m < o->nm; m++) calc_h1mode_mode(o, psi0, m);
+    if (calc_meanfield(o, A0) != 0) return -1;
+    apply_h(o, A0, dA);
+    for (int64_t i = 0; i < o->nconf; i++) dA[i] *= o->der_factor;
+    memset(dpsi, 0, sizeof(cplx) * o->dimpsi);
+    int rc = 0;
+    for (int64_t m = 0; m < o->nm; m++) rc |= der_spf(o, m, psi0, dpsi);
+    return rc;
+}
+
+int orc_calc_derivs(orc *o, const double *A, const double *psi, double *dA, double *dpsi) {
+    return calc_derivs(o, (const cplx *)A, (const cplx *)psi, (cplx *)dA, (cplx *)dpsi);
+}
+
+/* y_out = y0 + dt * sum_q w[q] k[q] over n entries (zero weights skipped). */
+static void rk_combine(int64_t n, const cplx *y0, double dt, const double *w, int nq, cplx *const *k, cplx *out) {
+    for (int64_t i = 0; i < n; i++) {
+        cplx acc = 0;
+        for (int q = 0; q < nq; q++) if (w[q] != 0.0) acc += w[q] * k[q][i];
+        out[i] = y0[i] + dt * acc;
+    }
+}
+
+/* rkck / rkdp (integrator.f90:95-221) on (A, psi).  kA[0], kp[0] hold the
+ * derivatives at (A0, psi0); for Dormand-Prince kA[6], kp[6] receive the
+ * derivatives at the 5th-order result (reused by the next step). */
+static int rk_vmf(orc *o, int dp, const cplx *A0, const cplx *p0, cplx **kA, cplx **kp, double dt, cplx *At,
+                  cplx *pt, cplx *Ae, cplx *pe) {
+    const int64_t N = o->nconf, D = o->dimpsi;
+    int rc = 0;
+    for (int st = 1; st < 6; st++) {
+        const double *b = dp ? dpB[st] : ckB[st];
+        rk_combine(N, A0, dt, b, st, kA, At);
+        rk_combine(D, p0, dt, b, st, kp, pt);
+        rc |= calc_derivs(o, At, pt, kA[st], kp[st]);
+    }
+    rk_combine(N, A0, dt, dp ? dpC : ckC, 6, kA, At);
+    rk_combine(D, p0, dt, dp ? dpC : ckC, 6, kp, pt);
+    if (dp) {
+        rc |= calc_derivs(o, At, pt, kA[6], kp[6]);
+        rk_combine(N, A0, dt, dpD, 7, kA, Ae);
+        rk_combine(D, p0, dt, dpD, 7, kp, pe);
+    } else {
+        rk_combine(N, A0, dt, ckD, 6, kA, Ae);
+        rk_combine(D, p0, dt, ckD, 6, kp, pe);
+    }
+    return rc;
+}
+
+/* rkqs (integrator.f90:36-91): error = DeltaPsi(5th, 4th order) / eps_int. */
+static int rkqs_vmf(orc *o, int dp, cplx **kA, cplx **kp, double htry, double tt, double *hdid, double *hnext,
+                    mg_attempt *atts, int64_t max_att, int64_t *n_att) {
+    const double safety = 0.9, pgrow = -0.2, pshrnk = -0.25, minscal = 0.1, maxscal = 5.0;
+    const double errcon = pow(5.0 / safety, 1.0 / pgrow);
+    const int64_t N = o->nconf, D = o->dimpsi;
+    const double minstep = o->p.min_stepsize;
+    cplx *At = xcalloc(N, sizeof(cplx)), *Ae = xcalloc(N, sizeof(cplx));
+    cplx *pt = xcalloc(D, sizeof(cplx)), *pe = xcalloc(D, sizeof(cplx));
+    double dt = htry, errmax = 0.0;
+    int rc = 0;
+    for (;;) {
+        rc |= rk_vmf(o, dp, o->A, o->psi, kA, kp, dt, At, pt, Ae, pe);
+        errmax = delta_psi(o, At, pt, Ae, pe) / o->p.eps_int;
+        const int accept = (errmax < 1.0 || dt <= minstep);
+        if (atts && n_att && *n_att < max_att)
+            atts[(*n_att)++] = (mg_attempt){tt, dt, errmax * o->p.eps_int, 0.0, accept, 0, 0};
+        if (accept || rc) break;
+        double a = minscal * dt, b = safety * dt * pow(errmax, pshrnk);
+        dt = a > b ? a : b;
+        if (dt < minstep) dt = minstep;
+    }
+    double hn = (errmax > errcon) ? safety * dt * pow(errmax, pgrow) : dt * maxscal;
+    if (hn < minstep) hn = minstep;
+    if (hn > o->p.writetime) hn = o->p.writetime;
+    *hnext = hn;
+    *hdid = dt;
+    memcpy(o->A, At, sizeof(cplx) * N);
+    memcpy(o->psi, pt, sizeof(cplx) * D);
+    if (dp) {   /* first-same-as-last: derivatives of the new point */
+        memcpy(kA[0], kA[6], sizeof(cplx) * N);
+        memcpy(kp[0], kp[6], sizeof(cplx) * D);
+    }
+    free(At); free(Ae); free(pt); free(pe);
+    return rc;
+}
+
+/* propagate_wf, Runge-Kutta branch (propagator.f90:108-140); method 1 =
+ * Cash-Karp, 2 = Dormand-Prince.  The reference enters its first
+ * Dormand-Prince step with derivative arrays it never filled
+ * (propagator.f90:97-100, 113); here they are computed at t0. */
+int orc_propagate_vmf(orc *o, int64_t method, double t0, double tfinal, mg_output *outs, double *pops,
+                      int64_t max_out, int64_t *n_out, mg_attempt *atts, int64_t max_att, int64_t *n_att) {
+    const int dp = (method == 2);
+    const double wt = o->p.writetime, minstep = o->p.min_stepsize;
+    const int64_t N = o->nconf, D = o->dimpsi;
+    double maxtime = tfinal + wt * 0.1;
+    double time = t0, stepsize = wt * o->p.eps_int;
+    cplx *kA[7], *kp[7];
+    for (int q = 0; q < 7; q++) { kA[q] = xcalloc(N, sizeof(cplx)); kp[q] = xcalloc(D, sizeof(cplx)); }
+    int rc = 0;
+    *n_out = 0;
+    if (n_att) *n_att = 0;
+    if (dp) rc |= calc_derivs(o, o->A, o->psi, kA[0], kp[0]);
+    while (time < maxtime && !rc) {
+        if (*n_out < max_out) {
+            orc_observables(o, time, &outs[*n_out], pops ? pops + o->ns * *n_out : NULL);
+            (*n_out)++;
+        }
+        double tnext = time + wt;
+        for (;;) {
+            double done, next;
+            if (!dp) rc |= calc_derivs(o, o->A, o->psi, kA[0], kp[0]);
+            rc |= rkqs_vmf(o, dp, kA, kp, stepsize, time, &done, &next, atts, max_att, n_att);
+            if (rc) break;
+            time = time + done;
+            if (time >= tnext - minstep) break;
+            int64_t k = (int64_t)((tnext - time) / next);
+            stepsize = (k == 0) ? tnext - time : (tnext - time) / (double)(k + 1);
+        }
+    }
+    for (int q = 0; q < 7; q++) { free(kA[q]); free(kp[q]); }
+    return rc ? -1 : 0;
+}
+
+/* Test hook: one explicit step of the Cash-Karp (1) or Dormand-Prince (2)
+ * pair on y' = lam y, y(0) = 1: y5 (propagated) and y4 (embedded). */
+void orc_rk_scalar_step(int64_t method, double lam_re, double lam_im, double dt, double *y5, double *y4) {
+    const int dp = (method == 2);
+    const cplx lam = lam_re + I * lam_im;
+    cplx k[7], y = 1.0;
+    k[0] = lam * y;
+    for (int st = 1; st < 6; st++) {
+        const double *b = dp ? dpB[st] : ckB[st];
+        cplx acc = 0;
+        for (int q = 0; q < st; q++) acc += b[q] * k[q];
+        k[st] = lam * (y + dt * acc);
+    }
+    cplx a5 = 0, a4 = 0;
+    for (int q = 0; q < 6; q++) a5 += (dp ? dpC : ckC)[q] * k[q];
+    const cplx r5 = y + dt * a5;
+    if (dp) {
+        k[6] = lam * r5;
+        for (int q = 0; q < 7; q++) a4 += dpD[q] * k[q];
+    } else {
+        for (int q = 0; q < 6; q++) a4 += ckD[q] * k[q];
+    }
+    const cplx r4 = y + dt * a4;
+    y5[0] = creal(r5); y5[1] = cimag(r5);
+    y4[0] = creal(r4); y4[1] = cimag(r4);
 }
 
 /* test hooks */

@@ -39,6 +39,14 @@ void orc_calc_h1mode(orc *o, double *h1_out);
 void orc_calc_rho(orc *o, double *rho_out);
 int orc_calc_meanfield(orc *o, double *mf_out, double *rmf_out, double *energy_out);
 void orc_apply_h(orc *o, const double *B, double *HB);
+/* VMF (derivatives.f90:39-243, integrator.f90:36-221, propagator.f90:108-140):
+ * derivatives of the current point; whole propagation (method 1 = Cash-Karp,
+ * 2 = Dormand-Prince). */
+int orc_calc_derivs(orc *o, const double *A, const double *psi, double *dA, double *dpsi);
+int orc_propagate_vmf(orc *o, int64_t method, double t0, double tfinal, mg_output *outs, double *pops,
+                      int64_t max_out, int64_t *n_out, mg_attempt *atts, int64_t max_att, int64_t *n_att);
+/* Test hook: one Cash-Karp / Dormand-Prince step of y' = lam y from y = 1. */
+void orc_rk_scalar_step(int64_t method, double lam_re, double lam_im, double dt, double *y5, double *y4);
 /* Test hook: chunk count of the Gram reductions (summation order only). */
 void orc_set_reduction_chunks(int64_t n);
 void orc_der_spf(orc *o, int64_t im, const double *psi_in, double *dpsi_out);

@@ -55,6 +55,12 @@ def lib():
         L.orc_herm_eig.argtypes = [C.c_int64, f64p, f64p, f64p]
         L.orc_tridiag_eig.argtypes = [C.c_int64, f64p, f64p, f64p, f64p]
         L.orc_set_reduction_chunks.argtypes = [C.c_int64]
+        L.orc_calc_derivs.restype = C.c_int
+        L.orc_calc_derivs.argtypes = [C.c_void_p, f64p, f64p, f64p, f64p]
+        L.orc_propagate_vmf.restype = C.c_int
+        L.orc_propagate_vmf.argtypes = [C.c_void_p, C.c_int64, C.c_double, C.c_double, C.POINTER(MgOutput),
+                                        f64p, C.c_int64, i64p, C.POINTER(MgAttempt), C.c_int64, i64p]
+        L.orc_rk_scalar_step.argtypes = [C.c_int64, C.c_double, C.c_double, C.c_double, f64p, f64p]
         _lib = L
     return _lib
 
@@ -168,6 +174,27 @@ class Oracle:
         lib().orc_observables(self.h, t, C.byref(out), pops.ctypes.data_as(f64p))
         return out, pops
 
+    def calc_derivs(self, A, psi):
+        A = np.ascontiguousarray(A, np.complex128)
+        psi = np.ascontiguousarray(psi, np.complex128)
+        dA = np.zeros_like(A)
+        dpsi = np.zeros_like(psi)
+        rc = lib().orc_calc_derivs(self.h, cplx_ptr(A), cplx_ptr(psi), cplx_ptr(dA), cplx_ptr(dpsi))
+        if rc != 0:
+            raise RuntimeError(lib().orc_error(self.h).decode())
+        return dA, dpsi
+
+    def propagate_vmf(self, method, t0, tfinal, max_out=100000, max_att=200000):
+        outs = (MgOutput * max_out)()
+        pops = np.zeros((max_out, self.prob.nstates))
+        atts = (MgAttempt * max_att)()
+        no, na = C.c_int64(0), C.c_int64(0)
+        rc = lib().orc_propagate_vmf(self.h, method, t0, tfinal, outs, pops.ctypes.data_as(f64p), max_out,
+                                     C.byref(no), atts, max_att, C.byref(na))
+        if rc != 0:
+            raise RuntimeError(lib().orc_error(self.h).decode())
+        return list(outs[: no.value]), pops[: no.value], list(atts[: na.value])
+
     def propagate(self, t0, tfinal, max_out=100000, max_att=200000):
         outs = (MgOutput * max_out)()
         pops = np.zeros((max_out, self.prob.nstates))
@@ -210,3 +237,10 @@ def tridiag_eig(alpha, beta):
     lib().orc_tridiag_eig(n, a.ctypes.data_as(f64p), b.ctypes.data_as(f64p),
                           w.ctypes.data_as(f64p), U.ctypes.data_as(f64p))
     return w, U.T
+
+
+def rk_scalar_step(method, lam, dt):
+    y5 = np.zeros(2)
+    y4 = np.zeros(2)
+    lib().orc_rk_scalar_step(method, lam.real, lam.imag, dt, y5.ctypes.data_as(f64p), y4.ctypes.data_as(f64p))
+    return complex(*y5), complex(*y4)

@@ -292,3 +292,65 @@ def test_rounding_alone_moves_free_running_trajectories(tmp_path):
     assert 0.0 < dc < 3e-8
     assert np.abs(p1 - p2).max() < 3e-8
     assert abs(len(a1) - len(a2)) <= 0.1 * len(a1) + 2
+
+
+@pytest.mark.parametrize("method,order", [(1, 5), (2, 5)])
+def test_rk_tableaux_orders(method, order):
+    """Cash-Karp and Dormand-Prince pairs (shared by the SPF and VMF
+    integrators) on y' = lam y: the propagated solution has local error
+    O(dt^6) and the embedded one O(dt^5), so halving dt divides them by ~64
+    and ~32; a wrong coefficient drops an order."""
+    import oracle_lib
+    lam = complex(-0.7, 1.3)
+    e5, e4 = [], []
+    for dt in (0.1, 0.05, 0.025):
+        y5, y4 = oracle_lib.rk_scalar_step(method, lam, dt)
+        ex = np.exp(lam * dt)
+        e5.append(abs(y5 - ex))
+        e4.append(abs(y4 - ex))
+    for a, b in zip(e5, e5[1:]):
+        assert 2 ** (order + 1) * 0.8 < a / b < 2 ** (order + 1) * 1.25
+    for a, b in zip(e4, e4[1:]):
+        assert 2 ** order * 0.8 < a / b < 2 ** order * 1.25
+
+
+def test_vmf_derivative_is_minus_i_h_a_and_projected_spf_equation(tmp_path):
+    """Calc_Derivs: dA = -i H A; dpsi is orthogonal to the SPFs of every
+    block (the MCTDH constraint g = 0 keeps the SPFs orthonormal)."""
+    prob = make_problem("t2", str(tmp_path))
+    A, psi = prob.A0 + 0.01, prob.psi0
+    o = Oracle(prob)
+    o.set_state(prob.A0, psi)
+    o.h1mode()
+    dA, dpsi = o.calc_derivs(A, psi)
+    HA = o.apply_h(A)
+    assert np.abs(dA - (-1j) * HA).max() < 1e-14 * max(1.0, np.abs(HA).max())
+    ip, _ = prob.ipsiSPF()
+    ngpm = prob.ngpm()
+    for m in range(prob.nmodes):
+        for s in range(prob.nstates if prob.MultiSet[m] else 1):
+            i0 = ip[m, s] - 1
+            nG, nS = ngpm[m], prob.nSPF[m, s]
+            phi = psi[i0:i0 + nG * nS].reshape(nS, nG).T
+            d = dpsi[i0:i0 + nG * nS].reshape(nS, nG).T
+            assert np.abs(phi.conj().T @ d).max() < 1e-12
+
+
+@pytest.mark.parametrize("method", [1, 2])
+def test_full_basis_vmf_equals_exact_propagation(method, tmp_path):
+    """VMF with n_k = G_k is exact MCTDH: C(t) follows exp(-iHt) on the
+    product grid to the integrator tolerance, norm conserved."""
+    spec = LVCSpec("fbv", 21, 2, 2, 6, 6, 4.0, tout=1.0, omega=(0.05, 0.30), kappa_sd=0.10,
+                   n_lambda=1, lam=(0.10, 0.10), gaps=(-0.2, 0.2), eps_int=1e-8, eps_psi=1e-9)
+    prob = make_problem(spec, str(tmp_path))
+    o = Oracle(prob)
+    outs, pops, atts = o.propagate_vmf(method, 0.0, prob.tfinal)
+    H = brute.full_hamiltonian(prob, prob.psi0)
+    w, V = np.linalg.eigh(H)
+    c0 = V.conj().T @ prob.A0
+    for out in outs:
+        c_ex = np.vdot(prob.A0, V @ (np.exp(-1j * w * out.time) * c0))
+        assert abs(complex(out.auto_re, out.auto_im) - c_ex) < 1e-6
+        assert out.norm == pytest.approx(1.0, abs=1e-6)
+    assert len(outs) == 5
+    assert any(not a.accepted for a in atts) or len(atts) > 5
