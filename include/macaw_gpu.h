@@ -118,6 +118,19 @@ int mg_observables(mg_ctx *ctx, double t, mg_output *out, double *pops);
 int mg_propagate(mg_ctx *ctx, double t0, double tfinal, mg_output *outs, double *pops,
                  int64_t max_out, int64_t *n_out, mg_attempt *atts, int64_t max_att,
                  int64_t *n_att);
+/* Calc_Derivs (derivatives.f90:39-243), DVR modes: the VMF time derivatives
+ * of the resident state, dA = -i H A (complex [nconf]) and the SPF equations
+ * of motion of every mode (complex [dimpsi]); host pointers (null skips). */
+int mg_calc_derivs(mg_ctx *ctx, double *dA_out, double *dpsi_out);
+/* propagate_wf, Runge-Kutta (VMF) branch (propagator.f90:108-140) with
+ * rkqs + rkck (method 1) or rkdp (method 2) (integrator.f90:36-221): the
+ * resident state from t0 to tfinal, outputs and attempts as in mg_propagate
+ * (an attempt's err_A holds its DeltaPsi error, err_psi is 0).  The
+ * Dormand-Prince derivatives of the first step are evaluated at t0 (the
+ * reference enters that step with arrays it never filled). */
+int mg_propagate_vmf(mg_ctx *ctx, int64_t method, double t0, double tfinal, mg_output *outs, double *pops,
+                     int64_t max_out, int64_t *n_out, mg_attempt *atts, int64_t max_att,
+                     int64_t *n_att);
 /* Algorithmic traffic (read B + write HB) and flops (8 per complex
  * multiply-add over all mode products) of one mg_apply_h. */
 int mg_apply_h_stats(mg_ctx *ctx, double *bytes_alg, double *flops_alg);

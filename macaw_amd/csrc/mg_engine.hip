@@ -151,6 +151,10 @@ struct mg_ctx {
     // Gram-slot partial sums of brick programs; per-slot mean-field Grams;
     // combination tables turning them (and ae * rho) into the mean fields
     DevBuf dgpart, dgslots, dmf_dst, dmf_rowptr, dmf_src, dmf_coef, dmf_trans, dident;
+    // VMF integration (allocated on first use): stage derivatives (H.A in the
+    // work type, dpsi in FP64), 5th / 4th order candidates
+    DevBuf dvA[7], dvP[7], dvAt, dvAe, dvPt, dvPe;
+    bool vmf_ready = false;
     bool mf_bricks = false;        // mean fields through brick programs (square chain ops, <= 8 SPFs)
     std::vector<const void *> mf_L, mf_R;   // per term: L tensor, R_g tensors (work type)
     int nblocks_red = 2048;      // blocks of the streaming reductions and Gram partial sums
@@ -850,7 +854,8 @@ void run_plan(mg_ctx *c, const Plan &p) {
         if (es * ((size_t)l.nmat * nmax * nmax + (size_t)l.nbuf * group_capacity(l.brick)) > 160 * 1024)
             throw HipError{"brick program exceeds the 160 KB of LDS"};
     (void)base;
-    if (!c->prof) {
+    static const bool verbose_all = std::getenv("MG_VERBOSE") != nullptr;
+    if (!c->prof && !verbose_all) {
         for (auto &l : p.launches) launch_plan_entry(c, p, l, nmax);
         if (p.nherm) launch_herm_batch(p.herm_ptrs.as<double2 *>(), p.herm_n.as<int>(), p.nherm, c->nspfmax, c->stream);
         return;
@@ -875,9 +880,9 @@ void run_plan(mg_ctx *c, const Plan &p) {
         c->prof_ms += ms;
         c->prof_launches++;
         if (verbose)
-            fprintf(stderr, "  launch %zu: tasks=%d blocks=%lld brick=%d nbuf=%d nmat=%d  %.3f ms\n", i,
+            fprintf(stderr, "  launch %zu: tasks=%d blocks=%lld brick=%d nbuf=%d nmat=%d ngram=%d  %.3f ms\n", i,
                     p.launches[i].n, (long long)p.launches[i].maxblocks, p.launches[i].brick, p.launches[i].nbuf,
-                    p.launches[i].nmat, ms);
+                    p.launches[i].nmat, p.launches[i].ngram, ms);
     }
 }
 
@@ -1910,6 +1915,131 @@ void observables(mg_ctx *c, double t, mg_output *out, double *pops) {
     out->energy = energy / tot;
 }
 
+// ----------------------------------------------------------------------------
+// Variable mean field (VMF) integration, DVR modes (derivatives.f90:39-243,
+// integrator.f90:36-221, propagator.f90:108-140).
+// Cash-Karp (1990) and Dormand-Prince (1980) pairs, as in integrator.f90:108-194.
+const double kCkB[6][5] = {{0, 0, 0, 0, 0}, {0.2, 0, 0, 0, 0}, {0.075, 0.225, 0, 0, 0}, {0.3, -0.9, 1.2, 0, 0},
+                           {-11.0 / 54.0, 2.5, -70.0 / 27.0, 35.0 / 27.0, 0},
+                           {1631.0 / 55296.0, 175.0 / 512.0, 575.0 / 13824.0, 44275.0 / 110592.0, 253.0 / 4096.0}};
+const double kCkC[7] = {37.0 / 378.0, 0, 250.0 / 621.0, 125.0 / 594.0, 0, 512.0 / 1771.0, 0};
+const double kCkD[7] = {2825.0 / 27648.0, 0, 18575.0 / 48384.0, 13525.0 / 55296.0, 277.0 / 14336.0, 0.25, 0};
+const double kDpB[6][5] = {{0, 0, 0, 0, 0}, {0.2, 0, 0, 0, 0}, {0.075, 0.225, 0, 0, 0},
+                           {44.0 / 45.0, -56.0 / 15.0, 32.0 / 9.0, 0, 0},
+                           {19372.0 / 6561.0, -25360.0 / 2187.0, 64448.0 / 6561.0, -212.0 / 729.0, 0},
+                           {9017.0 / 3168.0, -355.0 / 33.0, 46732.0 / 5247.0, 49.0 / 176.0, -5103.0 / 18656.0}};
+const double kDpC[7] = {35.0 / 384.0, 0, 500.0 / 1113.0, 125.0 / 192.0, -2187.0 / 6784.0, 11.0 / 84.0, 0};
+const double kDpD[7] = {5179.0 / 57600.0, 0, 7571.0 / 16695.0, 393.0 / 640.0, -92097.0 / 339200.0, 187.0 / 2100.0,
+                        0.025};
+
+void vmf_alloc(mg_ctx *c) {
+    if (c->vmf_ready) return;
+    const size_t N = (size_t)c->nconf, D = (size_t)c->dimpsi, z = sizeof(double2);
+    for (int q = 0; q < 7; q++) {
+        c->dvA[q].alloc(c->es * N);
+        c->dvP[q].alloc(z * D);
+    }
+    c->dvAt.alloc(z * N);
+    c->dvAe.alloc(z * N);
+    c->dvPt.alloc(z * D);
+    c->dvPe.alloc(z * D);
+    c->vmf_ready = true;
+}
+
+// Calc_Derivs at (A, psi): kA <- H A (work type; dA = der_factor kA, applied
+// in the stage combinations), kp <- dpsi of every mode.  Failures of the SPF
+// projector solve accumulate in dstatus.
+void calc_derivs(mg_ctx *c, const double2 *A, const double2 *psi, void *kA, double2 *kp) {
+    calc_rho(c, A);
+    calc_h1mode(c, psi);
+    calc_meanfield(c, A, true);
+    apply_h(c, c->prec ? c->dAf.p : (const void *)A, kA);
+    SpfParams P = spf_params(c, 0.0);
+    launch_der_spf(c->dspfmodes.as<SpfModeDesc>(), -c->nm, psi, kp, c->dspfscr.as<double2>(), P, dof_tables(c),
+                   c->dstatus.as<int>(), c->stream);
+    check_launch();
+}
+
+// y_out = y0 + sum_q scale * w[q] k[q] for the A part (scale = dt der_factor)
+// and the psi part (scale = dt).
+void vmf_combine(mg_ctx *c, const double *w, int nq, double dt, double2 *Aout, double2 *Pout) {
+    const void *kA[7], *kp[7];
+    double2 wA[7], wP[7];
+    for (int q = 0; q < nq; q++) {
+        kA[q] = c->dvA[q].p;
+        kp[q] = c->dvP[q].p;
+        wA[q] = make_double2(0.0, -dt * w[q]);   // dt w der_factor, der_factor = -i
+        wP[q] = make_double2(dt * w[q], 0.0);
+    }
+    launch_rk_combine(Aout, c->dA.as<double2>(), kA, wA, nq, c->nconf, c->prec, c->stream);
+    launch_rk_combine(Pout, c->dpsi.as<double2>(), kp, wP, nq, c->dimpsi, 0, c->stream);
+}
+
+// rkck / rkdp (integrator.f90:95-221) from the resident (A, psi); stage 0's
+// derivatives are in dvA[0], dvP[0]; Dormand-Prince leaves those of the
+// 5th-order point in dvA[6], dvP[6].
+void vmf_rk(mg_ctx *c, bool dp, double dt) {
+    double2 *At = c->dvAt.as<double2>(), *Pt = c->dvPt.as<double2>();
+    for (int st = 1; st < 6; st++) {
+        vmf_combine(c, dp ? kDpB[st] : kCkB[st], st, dt, At, Pt);
+        calc_derivs(c, At, Pt, c->dvA[st].p, c->dvP[st].as<double2>());
+    }
+    vmf_combine(c, dp ? kDpC : kCkC, 6, dt, At, Pt);
+    if (dp) {
+        calc_derivs(c, At, Pt, c->dvA[6].p, c->dvP[6].as<double2>());
+        vmf_combine(c, kDpD, 7, dt, c->dvAe.as<double2>(), c->dvPe.as<double2>());
+    } else {
+        vmf_combine(c, kCkD, 6, dt, c->dvAe.as<double2>(), c->dvPe.as<double2>());
+    }
+}
+
+void spf_status_check(mg_ctx *c) {
+    int st = 0;
+    HIP_CHECK(hipMemcpyAsync(&st, c->dstatus.p, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIP_CHECK(hipStreamSynchronize(c->stream));
+    if (st == 1) throw HipError{"SPF overlap matrix not positive definite"};
+}
+
+// rkqs (integrator.f90:36-91): error = DeltaPsi(5th, 4th order) / eps_int.
+void vmf_rkqs(mg_ctx *c, bool dp, double htry, double tt, double *hdid, double *hnext,
+              std::vector<mg_attempt> *log) {
+    const double safety = 0.9, pgrow = -0.2, pshrnk = -0.25, minscal = 0.1, maxscal = 5.0;
+    const double errcon = std::pow(5.0 / safety, 1.0 / pgrow);
+    const double minstep = c->min_stepsize;
+    double dt = htry, errmax = 0.0;
+    for (;;) {
+        int forced = -1;
+        if (c->replay_pos < c->replay.size()) {
+            const mg_attempt &r = c->replay[c->replay_pos++];
+            dt = r.h;
+            forced = (int)r.accepted;
+        }
+        launch_fill(reinterpret_cast<double2 *>(c->dstatus.p), 1, make_double2(0, 0), c->stream);
+        vmf_rk(c, dp, dt);
+        delta_psi_overlaps(c, c->dvPt.as<double2>(), c->dvPe.as<double2>());
+        errmax = delta_psi_contract(c, c->dvAt.as<double2>(), c->dvAe.as<double2>()) / c->eps_int;
+        spf_status_check(c);
+        const bool accept = (forced >= 0) ? (forced == 1) : (errmax < 1.0 || dt <= minstep);
+        if (log) log->push_back({tt, dt, errmax * c->eps_int, 0.0, (int64_t)accept, 0, 0});
+        if (accept) break;
+        dt = std::max(minscal * dt, safety * dt * std::pow(errmax, pshrnk));
+        dt = std::max(dt, minstep);
+    }
+    double hn = (errmax > errcon) ? safety * dt * std::pow(errmax, pgrow) : dt * maxscal;
+    hn = std::max(hn, minstep);
+    hn = std::min(hn, c->writetime);
+    *hnext = hn;
+    *hdid = dt;
+    copy_d2d(c, c->dA.as<double2>(), c->dvAt.as<double2>(), c->nconf);
+    copy_d2d(c, c->dpsi.as<double2>(), c->dvPt.as<double2>(), c->dimpsi);
+    if (dp) {   // first-same-as-last: the new point's derivatives become stage 0
+        std::swap(c->dvA[0].p, c->dvA[6].p);
+        std::swap(c->dvP[0].p, c->dvP[6].p);
+        // cached H.A plans are keyed by buffer address and stay valid for both
+    }
+    HIP_CHECK(hipStreamSynchronize(c->stream));
+}
+
 template <class F>
 int guarded(mg_ctx *c, F &&f) {
     if (!c) return -1;
@@ -2157,6 +2287,72 @@ int mg_der_spf(mg_ctx *c, int64_t im, const double *psi_in, double *dpsi_out) {
                        c->dspfscr.as<double2>(), P, dof_tables(c), c->dstatus.as<int>(), c->stream);
         HIP_CHECK(hipMemcpyAsync(dpsi_out, c->dpH2.p, sizeof(double2) * c->dimpsi, hipMemcpyDeviceToHost, c->stream));
         HIP_CHECK(hipStreamSynchronize(c->stream));
+    });
+}
+
+int mg_calc_derivs(mg_ctx *c, double *dA_out, double *dpsi_out) {
+    return guarded(c, [&] {
+        vmf_alloc(c);
+        launch_fill(reinterpret_cast<double2 *>(c->dstatus.p), 1, make_double2(0, 0), c->stream);
+        calc_derivs(c, c->dA.as<double2>(), c->dpsi.as<double2>(), c->dvA[0].p, c->dvP[0].as<double2>());
+        spf_status_check(c);
+        // dA = der_factor * H A, as complex FP64
+        launch_fill(c->dOut.as<double2>(), c->nconf, make_double2(0, 0), c->stream);
+        const void *k[1] = {c->dvA[0].p};
+        const double2 w[1] = {make_double2(0.0, -1.0)};
+        launch_rk_combine(c->dIn.as<double2>(), c->dOut.as<double2>(), k, w, 1, c->nconf, c->prec, c->stream);
+        if (dA_out)
+            HIP_CHECK(hipMemcpyAsync(dA_out, c->dIn.p, sizeof(double2) * c->nconf, hipMemcpyDeviceToHost, c->stream));
+        if (dpsi_out)
+            HIP_CHECK(hipMemcpyAsync(dpsi_out, c->dvP[0].p, sizeof(double2) * c->dimpsi, hipMemcpyDeviceToHost,
+                                     c->stream));
+        HIP_CHECK(hipStreamSynchronize(c->stream));
+    });
+}
+
+int mg_propagate_vmf(mg_ctx *c, int64_t method, double t0, double tfinal, mg_output *outs, double *pops,
+                     int64_t max_out, int64_t *n_out, mg_attempt *atts, int64_t max_att, int64_t *n_att) {
+    return guarded(c, [&] {
+        if (method != 1 && method != 2) throw HipError{"VMF method must be 1 (Cash-Karp) or 2 (Dormand-Prince)"};
+        if (!n_out) throw HipError{"n_out is required"};
+        const bool dp = (method == 2);
+        vmf_alloc(c);
+        const double wt = c->writetime, minstep = c->min_stepsize;
+        const double maxtime = tfinal + wt * 0.1;
+        double time = t0, stepsize = c->stepsize;
+        *n_out = 0;
+        if (n_att) *n_att = 0;
+        std::vector<mg_attempt> log;
+        if (dp) {   // the reference leaves these unfilled for its first step (propagator.f90:97-113)
+            launch_fill(reinterpret_cast<double2 *>(c->dstatus.p), 1, make_double2(0, 0), c->stream);
+            calc_derivs(c, c->dA.as<double2>(), c->dpsi.as<double2>(), c->dvA[0].p, c->dvP[0].as<double2>());
+            spf_status_check(c);
+        }
+        while (time < maxtime) {
+            if (*n_out < max_out) {
+                observables(c, time, &outs[*n_out], pops ? pops + c->ns * *n_out : nullptr);
+                (*n_out)++;
+            }
+            const double tnext = time + wt;
+            for (;;) {
+                double done = 0.0, next = 0.0;
+                if (!dp) {
+                    launch_fill(reinterpret_cast<double2 *>(c->dstatus.p), 1, make_double2(0, 0), c->stream);
+                    calc_derivs(c, c->dA.as<double2>(), c->dpsi.as<double2>(), c->dvA[0].p,
+                                c->dvP[0].as<double2>());
+                }
+                log.clear();
+                vmf_rkqs(c, dp, stepsize, time, &done, &next, atts ? &log : nullptr);
+                if (atts && n_att)
+                    for (auto &a : log)
+                        if (*n_att < max_att) atts[(*n_att)++] = a;
+                time = time + done;
+                if (time >= tnext - minstep) break;
+                const int64_t k = (int64_t)((tnext - time) / next);
+                stepsize = (k == 0) ? tnext - time : (tnext - time) / (double)(k + 1);
+            }
+        }
+        c->stepsize = stepsize;
     });
 }
 

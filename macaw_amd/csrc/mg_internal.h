@@ -205,6 +205,9 @@ void launch_norm2(const void *x, int64_t n, double *partials, int nblocks, doubl
                   hipStream_t s);
 void launch_convert(void *dst, int pdst, const void *src, int psrc, int64_t n, hipStream_t s);
 void launch_copy(void *dst, const void *src, int64_t n, int prec, hipStream_t s);
+// out = y0 + sum_q w[q] k[q]; y0/out complex FP64, k[q] in the storage type.
+void launch_rk_combine(double2 *out, const double2 *y0, const void *const *k, const double2 *w, int nq, int64_t n,
+                       int prec, hipStream_t s);
 void launch_hsum(double2 *hsum, const double2 *h1, const int *rowptr, const int *src,
                  const double2 *coef, int nmat, int nmax, hipStream_t s);
 int group_capacity(int brick);
@@ -212,6 +215,7 @@ void launch_rmf(const double2 *rho, const double2 *mf, double2 *rmf, const int *
                 int nspfmax, int nopmax, double eps, hipStream_t s);
 void launch_spf_propagate(const SpfModeDesc *d_modes, int nmodes, double2 *psi, double2 *scratch,
                           SpfParams p, DofTables dt, int *d_status, hipStream_t s);
+// mode < 0: every mode, -mode workgroups (pass -nmodes)
 void launch_der_spf(const SpfModeDesc *d_modes, int mode, const double2 *psi, double2 *dpsi,
                     double2 *scratch, SpfParams p, DofTables dt, int *d_status, hipStream_t s);
 void launch_mf_accum(double2 *dst1, double2 *dst2, const double2 *G, int n1, int n2, int ld,

@@ -16,6 +16,7 @@
 
 #include <algorithm>
 #include <cstdlib>
+#include <cstring>
 
 namespace mg {
 
@@ -1086,6 +1087,40 @@ void launch_convert(void *dst, int pdst, const void *src, int psrc, int64_t n, h
         hipLaunchKernelGGL((k_convert<float2, float2>), g, b, 0, s, (float2 *)dst, (const float2 *)src, n);
 }
 
+// Runge-Kutta stage combination: out = y0 + sum_q w[q] k[q] (complex
+// weights, zero weights skipped), y0 / out FP64, k in the storage type T.
+struct RkTerms {
+    const void *k[7];
+    double2 w[7];
+    int nq;
+};
+
+template <class T>
+__global__ void k_rk_combine(double2 *out, const double2 *y0, RkTerms t, int64_t n) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        double2 acc = y0[i];
+        for (int q = 0; q < t.nq; q++) {
+            if (t.w[q].x == 0.0 && t.w[q].y == 0.0) continue;
+            cfma(acc, t.w[q], to_d2(static_cast<const T *>(t.k[q])[i]));
+        }
+        out[i] = acc;
+    }
+}
+
+void launch_rk_combine(double2 *out, const double2 *y0, const void *const *k, const double2 *w, int nq, int64_t n,
+                       int prec, hipStream_t s) {
+    RkTerms t;
+    std::memset(&t, 0, sizeof t);
+    t.nq = nq;
+    for (int q = 0; q < nq; q++) {
+        t.k[q] = k[q];
+        t.w[q] = w[q];
+    }
+    const dim3 g((unsigned)grid_for(n)), b(256);
+    if (prec == 0) hipLaunchKernelGGL(k_rk_combine<double2>, g, b, 0, s, out, y0, t, n);
+    else hipLaunchKernelGGL(k_rk_combine<float2>, g, b, 0, s, out, y0, t, n);
+}
+
 // Plain copy (profiling reference for traffic counters).
 template <class T>
 __global__ void __launch_bounds__(256) k_copy(T *__restrict__ dst, const T *__restrict__ src, int64_t n) {
@@ -1526,11 +1561,12 @@ __global__ void __launch_bounds__(256) k_spf_propagate(const SpfModeDesc *__rest
     }
 }
 
-// Calc_Der_SPF for one mode on a full psi vector (dpsi receives that mode's blocks).
+// Calc_Der_SPF for one mode (mode >= 0) or for every mode (mode < 0, one
+// workgroup each) on a full psi vector; dpsi receives those modes' blocks.
 __global__ void __launch_bounds__(256) k_der_spf(const SpfModeDesc *__restrict__ modes, int mode, const double2 *psi,
                                                  double2 *dpsi, double2 *scratch, SpfParams P, DofTables dt,
                                                  int *status) {
-    const SpfModeDesc d = modes[mode];
+    const SpfModeDesc d = modes[mode >= 0 ? mode : blockIdx.x];
     const int tid = threadIdx.x, nth = blockDim.x;
     SpfScratch S = carve(scratch, d, P.nspfmax);
     __shared__ int fail;
@@ -1554,7 +1590,8 @@ __global__ void __launch_bounds__(256) k_der_spf(const SpfModeDesc *__restrict__
 
 void launch_der_spf(const SpfModeDesc *d_modes, int mode, const double2 *psi, double2 *dpsi, double2 *scratch,
                     SpfParams p, DofTables dt, int *d_status, hipStream_t s) {
-    hipLaunchKernelGGL(k_der_spf, dim3(1), dim3(256), 0, s, d_modes, mode, psi, dpsi, scratch, p, dt, d_status);
+    hipLaunchKernelGGL(k_der_spf, dim3(mode >= 0 ? 1 : -mode), dim3(256), 0, s, d_modes, mode, psi, dpsi, scratch, p,
+                       dt, d_status);
 }
 
 void launch_spf_propagate(const SpfModeDesc *d_modes, int nmodes, double2 *psi, double2 *scratch, SpfParams p,

@@ -42,6 +42,9 @@ def lib():
         L.mg_apply_h_profile.argtypes = [vp, vp, vp, f64p, i64p]
         L.mg_set_replay.argtypes = [vp, C.POINTER(MgAttempt), C.c_int64]
         L.mg_profile_copy.argtypes = [vp, C.c_int64, f64p]
+        L.mg_calc_derivs.argtypes = [vp, f64p, f64p]
+        L.mg_propagate_vmf.argtypes = [vp, C.c_int64, C.c_double, C.c_double, C.POINTER(MgOutput), f64p, C.c_int64,
+                                       i64p, C.POINTER(MgAttempt), C.c_int64, i64p]
         for f in ("mg_set_state", "mg_get_state", "mg_set_start"):
             getattr(L, f).argtypes = [vp, f64p, f64p]
         L.mg_calc_h1mode.argtypes = [vp, f64p]
@@ -183,6 +186,22 @@ class Engine:
         pops = np.zeros(self.prob.nstates)
         self._check(lib().mg_observables(self.h, t, C.byref(out), pops.ctypes.data_as(f64p)))
         return out, pops
+
+    def calc_derivs(self):
+        """VMF derivatives of the resident state: (-i H A, dpsi)."""
+        dA = np.zeros(self.nconf, np.complex128)
+        dpsi = np.zeros(self.dimpsi, np.complex128)
+        self._check(lib().mg_calc_derivs(self.h, cplx_ptr(dA), cplx_ptr(dpsi)))
+        return dA, dpsi
+
+    def propagate_vmf(self, method, t0, tfinal, max_out=100000, max_att=200000):
+        outs = (MgOutput * max_out)()
+        pops = np.zeros((max_out, self.prob.nstates))
+        atts = (MgAttempt * max_att)()
+        no, na = C.c_int64(0), C.c_int64(0)
+        self._check(lib().mg_propagate_vmf(self.h, method, t0, tfinal, outs, pops.ctypes.data_as(f64p), max_out,
+                                           C.byref(no), atts, max_att, C.byref(na)))
+        return list(outs[: no.value]), pops[: no.value], list(atts[: na.value])
 
     def propagate(self, t0, tfinal, max_out=100000, max_att=200000):
         outs = (MgOutput * max_out)()
