@@ -111,11 +111,16 @@ def measured_traffic(workload, precision, launches_per_ha):
     return t
 
 
-def run_engine(prob, precision, warmup, steps):
+def run_engine(prob, precision, warmup, steps, autotune=True):
     from macaw_amd.engine import Engine
     tout = prob.writetime
     eng = Engine(prob, precision=precision)
     tag = "mixed" if precision else "fp64"
+    brick = None
+    if autotune:
+        t_a = time.perf_counter()
+        brick, score = eng.autotune()
+        log(f"[{tag}] autotune: brick {brick} (score {score:.2f} ms) in {time.perf_counter() - t_a:.1f} s")
     log(f"[{tag}] warmup: {warmup} output intervals")
     t_w = time.perf_counter()
     outs_w, _, atts_w = eng.propagate(0.0, (warmup - 1) * tout)
@@ -129,7 +134,7 @@ def run_engine(prob, precision, warmup, steps):
     nha = sum(a.nkrylov1 + a.nkrylov2 for a in atts) + len(outs)
     log(f"[{tag}] timed done: {wall:.2f} s for {fs:.2f} fs ({fs / wall:.4f} fs/s), {len(atts)} attempts "
         f"({sum(1 for a in atts if not a.accepted)} rejected), {nha} H.A")
-    return eng, {"outs": outs_w + outs, "wall": wall, "fs": fs, "atts": atts, "nha": nha}
+    return eng, {"outs": outs_w + outs, "wall": wall, "fs": fs, "atts": atts, "nha": nha, "brick": brick}
 
 
 def main():
@@ -146,6 +151,7 @@ def main():
                     help="only the event-timed H.A applications and a plain copy of nconf elements "
                          "(the workload of the counter passes)")
     ap.add_argument("--precision", type=int, default=1, help="--ha-only: 1 mixed (default), 0 FP64")
+    ap.add_argument("--no-autotune", action="store_true", help="keep the default LDS brick size")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this engine runs on one GPU; multi-GPU figures are not produced "
@@ -167,7 +173,7 @@ def main():
         eng.close()
         print(json.dumps({"ha_only": True, "precision": args.precision, **rl}), flush=True)
         return
-    eng, mx = run_engine(prob, 1, args.warmup, args.steps)
+    eng, mx = run_engine(prob, 1, args.warmup, args.steps, not args.no_autotune)
     rl = roofline(eng, prob, args.roofline_reps, PEAK_FP32_TFLOPS)
     log(f"roofline (mixed): {rl['achieved']:.2f} TFLOP/s, H.A {rl['HA_ms']:.2f} ms")
     eng.close()
@@ -181,7 +187,8 @@ def main():
                    "precision": "mixed: complex FP32 H.A, Krylov v1.., mean-field chains; FP64 A, SPFs, "
                                 "rho, mean fields, reductions, DeltaPsi",
                    "step": "one 0.5 fs output interval", "attempts": len(mx["atts"]),
-                   "HA_per_fs": mx["nha"] / mx["fs"], "C_t_last": [last.auto_re, last.auto_im],
+                   "HA_per_fs": mx["nha"] / mx["fs"], "lds_brick": mx["brick"],
+                   "C_t_last": [last.auto_re, last.auto_im],
                    "norm_last": last.norm},
         "roofline": {k: rl[k] for k in ("bound", "achieved", "peak", "unit", "frac", "traffic")},
         "roofline_detail": {k: v for k, v in rl.items() if k not in ("bound", "achieved", "peak", "unit", "frac", "traffic")},
@@ -192,7 +199,7 @@ def main():
         result["roofline"]["traffic"] = tr["bytes_per_launch"]
         result["roofline_detail"]["traffic_source"] = tr["source"]
     if not args.no_fp64:
-        eng64, fx = run_engine(prob, 0, args.warmup, args.steps)
+        eng64, fx = run_engine(prob, 0, args.warmup, args.steps, not args.no_autotune)
         rl64 = roofline(eng64, prob, args.roofline_reps, PEAK_FP64_TFLOPS)
         eng64.close()
         n = min(len(fx["outs"]), len(mx["outs"]))

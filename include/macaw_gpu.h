@@ -118,6 +118,11 @@ int mg_observables(mg_ctx *ctx, double t, mg_output *out, double *pops);
 int mg_propagate(mg_ctx *ctx, double t0, double tfinal, mg_output *outs, double *pops,
                  int64_t max_out, int64_t *n_out, mg_attempt *atts, int64_t max_att,
                  int64_t *n_att);
+/* Tile autotuner: times H.A and the mean fields on the resident state for
+ * the LDS brick sizes (mode grouping) that fit, keeps the best (score =
+ * 8 x H.A + 2 x mean fields, their weight in a CMF2 attempt) and returns
+ * it.  reps <= 0: 2 timings per candidate.  No reference counterpart. */
+int mg_autotune(mg_ctx *ctx, int64_t reps, int64_t *best_brick, double *best_score_ms);
 /* Calc_Derivs (derivatives.f90:39-243), DVR modes: the VMF time derivatives
  * of the resident state, dA = -i H A (complex [nconf]) and the SPF equations
  * of motion of every mode (complex [dimpsi]); host pointers (null skips). */

@@ -213,6 +213,32 @@ int64_t largest_divisor_le(int64_t L, int64_t cap) {
     return 1;
 }
 
+// Mode groups for the current brick capacity (sizes: max over states): the
+// first group takes leading modes while their product fits the brick; later
+// groups at most brick/8, so that Lt >= 8 keeps the brick runs contiguous.
+void make_groups(mg_ctx *c) {
+    const int nm = c->nm, ns = c->ns;
+    c->groups.clear();
+    std::vector<int64_t> mx(nm, 0);
+    for (int m = 0; m < nm; m++)
+        for (int s = 0; s < ns; s++) mx[m] = std::max(mx[m], c->NS(m, s));
+    int m = 0;
+    while (m < nm) {
+        const int64_t limit = (m == 0) ? c->brick : std::max<int64_t>(c->brick / 8, mx[m]);
+        int64_t prod = mx[m];
+        int e = m + 1;
+        while (e < nm && e - m < kMaxGroupModes && prod * mx[e] <= limit) prod *= mx[e++];
+        c->groups.push_back({m, e});
+        m = e;
+    }
+}
+
+bool brick_fits_lds(const mg_ctx *c, int b) {
+    const int nmx = c->nspfmax <= 4 ? 4 : (c->nspfmax <= 8 ? 8 : 16);
+    const size_t lds = c->es * ((size_t)kMaxBrickMats * nmx * nmx + kMaxBrickBufs * (size_t)group_capacity(b));
+    return b >= 64 && lds <= 160 * 1024;
+}
+
 // ----------------------------------------------------------------------------
 // Setup: layout (init_calc.f90:25-34,160-181), operator basis
 // (hamilton.f90:91-147, 353-373) and the classification of Hamiltonian terms.
@@ -410,31 +436,31 @@ void setup(mg_ctx *c, const mg_problem *p) {
     c->brick = c->prec ? 2048 : 1024;
     if (const char *e = std::getenv("MG_BRICK")) {
         const int b = std::atoi(e);
-        const int nmx = c->nspfmax <= 4 ? 4 : (c->nspfmax <= 8 ? 8 : 16);
-        const size_t lds = c->es * ((size_t)kMaxBrickMats * nmx * nmx + kMaxBrickBufs * (size_t)group_capacity(b));
-        if (b >= 64 && lds <= 160 * 1024) c->brick = b;
+        if (brick_fits_lds(c, b)) c->brick = b;
     }
-    c->groups.clear();
-    {
-        std::vector<int64_t> mx(nm, 0);
-        for (int m = 0; m < nm; m++)
-            for (int s = 0; s < ns; s++) mx[m] = std::max(mx[m], c->NS(m, s));
-        int m = 0;
-        while (m < nm) {
-            const int64_t limit = (m == 0) ? c->brick : std::max<int64_t>(c->brick / 8, mx[m]);
-            int64_t prod = mx[m];
-            int e = m + 1;
-            while (e < nm && e - m < kMaxGroupModes && prod * mx[e] <= limit) prod *= mx[e++];
-            c->groups.push_back({m, e});
-            m = e;
-        }
-    }
+    make_groups(c);
     // mean fields through brick programs when every chain op is square and
     // the SPF counts fit one 8-wide Gram slot
     c->mf_bricks = c->nspfmax <= 8 && std::getenv("MG_MF_LEGACY") == nullptr;
     for (int t : c->mf_chain_terms)
         for (int m = 0; m < nm; m++)
             if (c->NS(m, (int)c->lhs[t] - 1) != c->NS(m, (int)c->rhs[t] - 1)) c->mf_bricks = false;
+}
+
+size_t mf_nterms(const mg_ctx *c) {
+    size_t n = 0;
+    for (size_t ti = 0; ti < c->mf_chain_terms.size(); ti++)
+        if (c->mf_partner[ti] != -2) n++;
+    return n;
+}
+
+// Mean-field work tensors: the legacy chains' AL/AR, or the brick path's
+// R_g / L tensors (groups x chain terms).  Grows only.
+void alloc_mf_tensors(mg_ctx *c) {
+    size_t mfb = 2 * (size_t)c->nm;
+    if (c->mf_bricks) mfb = std::max(mfb, c->groups.size() * std::max<size_t>(1, mf_nterms(c)));
+    const size_t need = c->es * mfb * (size_t)c->mixed;
+    if (c->dmfbuf.bytes < need) c->dmfbuf.alloc(need);
 }
 
 // ----------------------------------------------------------------------------
@@ -475,17 +501,10 @@ void allocate(mg_ctx *c) {
     c->dS3.alloc(z * nn * c->nm * c->ns);
     c->dGtmp.alloc(z * nn);
     c->dchain.alloc(c->es * 2 * std::max<size_t>(1, c->chains.size()) * (size_t)c->mixed);
-    {
-        size_t nterms = 0;
-        for (size_t ti = 0; ti < c->mf_chain_terms.size(); ti++)
-            if (c->mf_partner[ti] != -2) nterms++;
-        size_t mfb = 2 * (size_t)c->nm;
-        if (c->mf_bricks) mfb = std::max(mfb, c->groups.size() * std::max<size_t>(1, nterms));
-        c->dmfbuf.alloc(c->es * mfb * (size_t)c->mixed);
-        if (c->mf_bricks) {
-            c->dgslots.alloc(z * nn * c->nm * std::max<size_t>(1, nterms));
-            HIP_CHECK(hipMemset(c->dgslots.p, 0, c->dgslots.bytes));
-        }
+    alloc_mf_tensors(c);
+    if (c->mf_bricks) {
+        c->dgslots.alloc(z * nn * c->nm * std::max<size_t>(1, mf_nterms(c)));
+        HIP_CHECK(hipMemset(c->dgslots.p, 0, c->dgslots.bytes));
     }
     c->dkron.alloc(z * 2 * N);
     c->dpartials.alloc(sizeof(double2) * 64 * (size_t)c->nblocks_red);
@@ -2287,6 +2306,66 @@ int mg_der_spf(mg_ctx *c, int64_t im, const double *psi_in, double *dpsi_out) {
                        c->dspfscr.as<double2>(), P, dof_tables(c), c->dstatus.as<int>(), c->stream);
         HIP_CHECK(hipMemcpyAsync(dpsi_out, c->dpH2.p, sizeof(double2) * c->dimpsi, hipMemcpyDeviceToHost, c->stream));
         HIP_CHECK(hipStreamSynchronize(c->stream));
+    });
+}
+
+int mg_autotune(mg_ctx *c, int64_t reps, int64_t *best_brick, double *best_ms) {
+    return guarded(c, [&] {
+        // candidates around the default that fit the LDS; score = H.A time x 8
+        // + mean-field time x 2 (their weight in one CMF2 attempt)
+        const int def = c->prec ? 2048 : 1024;
+        std::vector<int> cand;
+        for (int b : {def / 4, def / 2, def, def * 2, def * 4})
+            if (brick_fits_lds(c, b)) cand.push_back(b);
+        const int R = reps > 0 ? (int)reps : 2;
+        // input: the resident A (its FP32 copy, written by calc_rho, in the mixed form)
+        const void *in = c->prec ? c->dAf.p : c->dA.p;
+        void *out = c->prec ? c->dYf.p : c->dOut.p;
+        double best = 1e300;
+        int bestb = c->brick;
+        for (int b : cand) {
+            HIP_CHECK(hipStreamSynchronize(c->stream));
+            c->plans.clear();
+            c->brick = b;
+            make_groups(c);
+            alloc_mf_tensors(c);
+            calc_rho(c, c->dA.as<double2>());
+            calc_h1mode(c, c->dpsi.as<double2>());
+            apply_h(c, in, out);                        // plan build and warm-up
+            calc_meanfield(c, c->dA.as<double2>(), true);
+            double tha = 1e300, tmf = 1e300;
+            for (int r = 0; r < R; r++) {
+                HIP_CHECK(hipEventRecord(c->ev0, c->stream));
+                apply_h(c, in, out);
+                HIP_CHECK(hipEventRecord(c->ev1, c->stream));
+                HIP_CHECK(hipEventSynchronize(c->ev1));
+                float ms = 0.f;
+                HIP_CHECK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+                tha = std::min(tha, (double)ms);
+                calc_rho(c, c->dA.as<double2>());
+                HIP_CHECK(hipEventRecord(c->ev0, c->stream));
+                calc_meanfield(c, c->dA.as<double2>(), true);
+                HIP_CHECK(hipEventRecord(c->ev1, c->stream));
+                HIP_CHECK(hipEventSynchronize(c->ev1));
+                HIP_CHECK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+                tmf = std::min(tmf, (double)ms);
+            }
+            const double score = 8.0 * tha + 2.0 * tmf;
+            if (std::getenv("MG_VERBOSE"))
+                fprintf(stderr, "autotune: brick %d  H.A %.3f ms  mean fields %.3f ms  score %.3f\n", b, tha, tmf,
+                        score);
+            if (score < best) {
+                best = score;
+                bestb = b;
+            }
+        }
+        HIP_CHECK(hipStreamSynchronize(c->stream));
+        c->plans.clear();
+        c->brick = bestb;
+        make_groups(c);
+        alloc_mf_tensors(c);
+        if (best_brick) *best_brick = bestb;
+        if (best_ms) *best_ms = best;
     });
 }
 

@@ -43,6 +43,7 @@ def lib():
         L.mg_set_replay.argtypes = [vp, C.POINTER(MgAttempt), C.c_int64]
         L.mg_profile_copy.argtypes = [vp, C.c_int64, f64p]
         L.mg_calc_derivs.argtypes = [vp, f64p, f64p]
+        L.mg_autotune.argtypes = [vp, C.c_int64, i64p, f64p]
         L.mg_propagate_vmf.argtypes = [vp, C.c_int64, C.c_double, C.c_double, C.POINTER(MgOutput), f64p, C.c_int64,
                                        i64p, C.POINTER(MgAttempt), C.c_int64, i64p]
         for f in ("mg_set_state", "mg_get_state", "mg_set_start"):
@@ -186,6 +187,12 @@ class Engine:
         pops = np.zeros(self.prob.nstates)
         self._check(lib().mg_observables(self.h, t, C.byref(out), pops.ctypes.data_as(f64p)))
         return out, pops
+
+    def autotune(self, reps=2):
+        """Pick the LDS brick size (mode grouping) by timing H.A and the mean fields."""
+        b, ms = C.c_int64(0), C.c_double(0)
+        self._check(lib().mg_autotune(self.h, reps, C.byref(b), C.byref(ms)))
+        return b.value, ms.value
 
     def calc_derivs(self):
         """VMF derivatives of the resident state: (-i H A, dpsi)."""

@@ -242,3 +242,31 @@ def test_apply_h_small_bricks_match_oracle(name, brick, tmp_path, monkeypatch):
     e2.set_replay(atts_o)
     outs_e, pops_e, _ = e2.propagate(0.0, 0.5 / 0.02418884326505)
     compare_outputs(outs_e, pops_e, outs_o, pops_o, 1e-10)
+
+
+@gpu
+@pytest.mark.parametrize("name", ["t5", "c1"])
+def test_autotune_keeps_parity(name, tmp_path):
+    """After the brick autotuner regroups the modes, every operation still
+    matches the oracle (and a replayed propagation does too)."""
+    prob = make_problem(name, str(tmp_path))
+    e = engine_for(prob)
+    brick, score = e.autotune()
+    assert brick >= 64 and score > 0.0
+    A, psi = perturbed_state(prob, 8)
+    o = Oracle(prob)
+    o.set_state(A, psi)
+    o.h1mode()
+    e.set_state(A, psi)
+    e.h1mode()
+    B = rand_c(np.random.default_rng(9), prob.nAConftot)
+    assert close(e.apply_h(B), o.apply_h(B), 1e-12)
+    assert close(e.rho(), o.rho(), 1e-12)
+    mf_o, rmf_o, _ = o.meanfield()
+    mf_e, rmf_e = e.meanfield()
+    assert close(mf_e, mf_o, 1e-12)
+    outs_o, pops_o, atts_o = Oracle(prob).propagate(0.0, 0.5 / 0.02418884326505)
+    e.set_state(prob.A0, prob.psi0)
+    e.set_replay(atts_o)
+    outs_e, pops_e, _ = e.propagate(0.0, 0.5 / 0.02418884326505)
+    compare_outputs(outs_e, pops_e, outs_o, pops_o, 1e-10)
