@@ -147,7 +147,7 @@ struct mg_ctx {
     DevBuf dgrid, dd1, dd2, dxd1, dngp, dmode_dofs, dmode_ndof, dspfmodes, dstatus;
     // mixed form only: FP64 first Krylov vector, FP32 copy of the A-vector,
     // FP32 in/out of the C-ABI H.A, FP32 copies of h1mode and the diagonal sums
-    DevBuf dV0, dAf, dXf, dYf, dh1f, dhsumf;
+    DevBuf dAf, dXf, dYf, dh1f, dhsumf;
     // Gram-slot partial sums of brick programs; per-slot mean-field Grams;
     // combination tables turning them (and ae * rho) into the mean fields
     DevBuf dgpart, dgslots, dmf_dst, dmf_rowptr, dmf_src, dmf_coef, dmf_trans, dident;
@@ -171,7 +171,8 @@ struct mg_ctx {
     // cached launch plans keyed by the buffers they read and write
     std::map<PlanKey, std::unique_ptr<Plan>> plans;
     std::map<PlanKey, std::unique_ptr<SpfPlan>> spf_plans;
-    std::vector<double> lan_alpha, lan_U;
+    std::vector<double> lan_alpha, lan_U, lan_n;   // Ritz values, vectors, Krylov norms
+    const double2 *lan_v0 = nullptr;               // u_0 of the last built space (FP64)
     int nkrylov = 0;
 
     int nsets(int m) const { return MultiSet[m] ? ns : 1; }
@@ -476,7 +477,6 @@ void allocate(mg_ctx *c) {
     c->dBK.alloc(c->es * N * kNKrylovMax);
     if (c->prec) {
         const size_t zf = sizeof(float2);
-        c->dV0.alloc(z * N);
         c->dAf.alloc(zf * N);
         c->dXf.alloc(zf * N);
         c->dYf.alloc(zf * N);
@@ -508,7 +508,7 @@ void allocate(mg_ctx *c) {
     }
     c->dkron.alloc(z * 2 * N);
     c->dpartials.alloc(sizeof(double2) * 64 * (size_t)c->nblocks_red);
-    c->dscal.alloc(sizeof(double) * 64);
+    c->dscal.alloc(sizeof(double) * 96);   // scalars; n_j^2 at 24..44; coefficients at 48..
     HIP_CHECK(hipMemsetAsync(c->dh1.p, 0, c->dh1.bytes, c->stream));
     HIP_CHECK(hipMemsetAsync(c->dmf.p, 0, c->dmf.bytes, c->stream));
     HIP_CHECK(hipMemsetAsync(c->drmf.p, 0, c->drmf.bytes, c->stream));
@@ -1765,41 +1765,44 @@ void tridiag_eig(int n, const double *alpha, const double *beta, double *w, doub
     for (int i = 0; i < n; i++) w[i] = A[i + n * i];
 }
 
-// B0 and BT are FP64.  In the mixed form the first Krylov vector is kept in
-// FP64 (dV0) beside its FP32 copy; H.A and the vectors v_1.. are FP32, so the
-// FP32 rounding enters the result only through the coefficients c_1..
+// B0 and BT are FP64.  The Krylov vectors are kept unnormalised, u_j = n_j v_j
+// with n_j^2 in device scalars (no normalisation pass; the 1/n factors enter
+// the recurrence kernel and the final coefficients).  u_0 is B0 itself (FP64,
+// remembered for the combination); H.A reads it directly in the FP64 form and
+// through an FP32 copy in the mixed form, where u_1.. are FP32 too, so FP32
+// rounding enters the propagated vector only through the coefficients c_1..
 // (|c_i| ~ (beta dt)^i), not through the O(1) coefficient of v_0.
 int lanczos(mg_ctx *c, const double2 *B0, double2 *BT, double dt, bool build) {
     const int64_t N = c->nconf;
     const int pr = c->prec;
     void *BK = c->dBK.p;
     double *dsc = c->dscal.as<double>();
+    double *nsq = dsc + 24;          // n_j^2, j < kNKrylovMax + 1
     double *part = c->dpartials.as<double>();
     const double eps_B = 1e-8;
     if (build) {
-        double alpha[kNKrylovMax], beta[kNKrylovMax];
-        double2 *v0 = pr ? c->dV0.as<double2>() : c->dBK.as<double2>();
-        HIP_CHECK(hipMemcpyAsync(v0, B0, sizeof(double2) * N, hipMemcpyDeviceToDevice, c->stream));
-        launch_norm2(v0, N, part, c->nblocks_red, dsc + 0, 0, c->stream);
-        launch_scale(v0, dsc + 0, N, 0, c->stream);
-        if (pr) launch_convert(BK, 1, v0, 0, N, c->stream);
+        double alpha[kNKrylovMax], beta[kNKrylovMax], nrm[kNKrylovMax + 1];
+        c->lan_v0 = B0;
+        launch_norm2(B0, N, part, c->nblocks_red, nsq + 0, 0, c->stream);
+        if (pr) launch_convert(BK, 1, B0, 0, N, c->stream);
+        const void *u0 = pr ? BK : (const void *)B0;
         int j = 1;      // number of basis vectors whose H-image is being formed
         double error = 0.0;
         for (;;) {
-            void *vj = c->at(BK, N * (j - 1)), *w = c->at(BK, N * j);
-            const void *vp = (j >= 2) ? c->at(BK, N * (j - 2)) : nullptr;
+            const void *uj = (j == 1) ? u0 : c->at(BK, N * (j - 1));
+            void *w = c->at(BK, N * j);
+            const void *up = (j >= 2) ? ((j == 2) ? u0 : c->at(BK, N * (j - 2))) : nullptr;
             if (j >= 2 && j == kNKrylovMax) throw HipError{"Maximum number of Krylov vectors reached"};
-            apply_h(c, vj, w);
-            double2 *alpha_d = reinterpret_cast<double2 *>(dsc + 2);
-            launch_dot(vj, w, N, part, c->nblocks_red, alpha_d, pr, c->stream);
-            launch_lanczos_update(w, vj, vp, dsc + 2, j >= 2 ? beta[j - 2] : 0.0, N, part, c->nblocks_red, dsc + 4,
-                                  pr, c->stream);
-            launch_scale(w, dsc + 4, N, pr, c->stream);
-            double h[3];
-            HIP_CHECK(hipMemcpyAsync(h, dsc + 2, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+            apply_h(c, uj, w);
+            double2 *dot_d = reinterpret_cast<double2 *>(dsc + 2);
+            launch_dot(uj, w, N, part, c->nblocks_red, dot_d, pr, c->stream);
+            launch_lanczos_step(w, uj, up, dsc + 2, nsq, j - 1, N, part, c->nblocks_red, nsq + j, pr, c->stream);
+            double h[2];
+            HIP_CHECK(hipMemcpyAsync(h, dsc + 2, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+            HIP_CHECK(hipMemcpyAsync(nrm + j - 1, nsq + j - 1, sizeof(double) * 2, hipMemcpyDeviceToHost, c->stream));
             HIP_CHECK(hipStreamSynchronize(c->stream));
-            alpha[j - 1] = h[0];
-            beta[j - 1] = std::sqrt(h[2]);
+            alpha[j - 1] = h[0] / nrm[j - 1];
+            beta[j - 1] = std::sqrt(nrm[j]);
             if (j == 1) {
                 error = beta[0] * std::fabs(dt);
                 j = 2;
@@ -1812,6 +1815,8 @@ int lanczos(mg_ctx *c, const double2 *B0, double2 *BT, double dt, bool build) {
         c->nkrylov = j;
         c->lan_alpha.assign(j, 0.0);
         c->lan_U.assign((size_t)j * j, 0.0);
+        c->lan_n.assign(j, 0.0);
+        for (int i = 0; i < j; i++) c->lan_n[i] = std::sqrt(nrm[i]);
         std::vector<double> w(j);
         tridiag_eig(j, alpha, beta, w.data(), c->lan_U.data());
         for (int r = 0; r < j; r++) c->lan_alpha[r] = w[r];
@@ -1822,12 +1827,11 @@ int lanczos(mg_ctx *c, const double2 *B0, double2 *BT, double dt, bool build) {
         cplx acc = 0;
         for (int r = 0; r < n; r++)
             acc += c->lan_U[i + n * r] * c->lan_U[0 + n * r] * std::exp(cplx(0.0, -1.0) * c->lan_alpha[r] * dt);
-        coef[i] = d2(acc);
+        coef[i] = d2(acc / c->lan_n[i]);     // v_i = u_i / n_i
     }
-    double2 *dcoef = reinterpret_cast<double2 *>(dsc + 16);
+    double2 *dcoef = reinterpret_cast<double2 *>(dsc + 48);
     HIP_CHECK(hipMemcpyAsync(dcoef, coef.data(), sizeof(double2) * n, hipMemcpyHostToDevice, c->stream));
-    if (pr) launch_combine_mixed(BT, c->dV0.as<double2>(), c->dBK.as<float2>(), N, dcoef, n, N, c->stream);
-    else launch_combine(BT, BK, N, dcoef, n, N, 0, c->stream);
+    launch_combine_v0(BT, c->lan_v0, BK, N, dcoef, n, N, pr, c->stream);
     launch_norm2(BT, N, part, c->nblocks_red, dsc + 6, 0, c->stream);
     launch_scale(BT, dsc + 6, N, 0, c->stream);
     HIP_CHECK(hipStreamSynchronize(c->stream));

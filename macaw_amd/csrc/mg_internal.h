@@ -193,14 +193,13 @@ void launch_fill(double2 *p, int64_t n, double2 v, hipStream_t s);
 void launch_axpby(void *y, const void *x, double a, double b, int64_t n, int prec, hipStream_t s);
 void launch_dot(const void *x, const void *y, int64_t n, double *partials, int nblocks,
                 double2 *out, int prec, hipStream_t s);
-void launch_lanczos_update(void *w, const void *vj, const void *vprev, const double *alpha,
-                           double beta_prev, int64_t n, double *partials, int nblocks, double *norm2,
-                           int prec, hipStream_t s);
 void launch_scale(void *x, const double *norm2, int64_t n, int prec, hipStream_t s);
 void launch_combine(void *out, const void *basis, int64_t stride, const double2 *coef, int k,
                     int64_t n, int prec, hipStream_t s);
-void launch_combine_mixed(double2 *out, const double2 *v0, const float2 *basis, int64_t stride,
-                          const double2 *coef, int k, int64_t n, hipStream_t s);
+void launch_lanczos_step(void *w, const void *uj, const void *uprev, const double *dot_re, const double *nsq, int j,
+                         int64_t n, double *partials, int nblocks, double *nsq_next, int prec, hipStream_t s);
+void launch_combine_v0(double2 *out, const double2 *v0, const void *basis, int64_t stride, const double2 *coef, int k,
+                       int64_t n, int prec, hipStream_t s);
 void launch_norm2(const void *x, int64_t n, double *partials, int nblocks, double *out, int prec,
                   hipStream_t s);
 void launch_convert(void *dst, int pdst, const void *src, int psrc, int64_t n, hipStream_t s);

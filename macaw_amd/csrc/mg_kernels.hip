@@ -968,51 +968,6 @@ void launch_norm2(const void *x, int64_t n, double *partials, int nblocks, doubl
     hipLaunchKernelGGL(k_norm2_final, dim3(1), dim3(64), 0, s, partials, nblocks, out);
 }
 
-// w <- (w - beta_prev * vprev) - alpha * vj, with alpha = Re(<vj|w>) read from
-// the device; accumulates |w|^2 partials.
-template <class T>
-__global__ void __launch_bounds__(256) k_lanczos_update(T *w, const T *vj, const T *vprev, const double *alpha_re,
-                                                        double beta_prev, int64_t n, double *partials) {
-    const double alpha = alpha_re[0];
-    double acc = 0.0;
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-        double2 a = to_d2(w[i]);
-        if (vprev) {
-            const double2 p = to_d2(vprev[i]);
-            a.x = a.x - beta_prev * p.x;
-            a.y = a.y - beta_prev * p.y;
-        }
-        const double2 v = to_d2(vj[i]);
-        a.x = a.x - alpha * v.x;
-        a.y = a.y - alpha * v.y;
-        const T st = from_d2<T>(a);
-        w[i] = st;
-        const double2 b = to_d2(st);
-        acc = fma(b.x, b.x, acc);
-        acc = fma(b.y, b.y, acc);
-    }
-    __shared__ double red[4];
-    acc = wave_sum(acc);
-    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        double a = 0.0;
-        for (int q = 0; q < (int)(blockDim.x >> 6); q++) a += red[q];
-        partials[blockIdx.x] = a;
-    }
-}
-
-void launch_lanczos_update(void *w, const void *vj, const void *vprev, const double *alpha, double beta_prev,
-                           int64_t n, double *partials, int nblocks, double *norm2, int prec, hipStream_t s) {
-    if (prec == 0)
-        hipLaunchKernelGGL(k_lanczos_update<double2>, dim3(nblocks), dim3(256), 0, s, (double2 *)w,
-                           (const double2 *)vj, (const double2 *)vprev, alpha, beta_prev, n, partials);
-    else
-        hipLaunchKernelGGL(k_lanczos_update<float2>, dim3(nblocks), dim3(256), 0, s, (float2 *)w,
-                           (const float2 *)vj, (const float2 *)vprev, alpha, beta_prev, n, partials);
-    hipLaunchKernelGGL(k_norm2_final, dim3(1), dim3(64), 0, s, partials, nblocks, norm2);
-}
-
 template <class T>
 __global__ void k_scale(T *x, const double *norm2, int64_t n) {
     const double d = sqrt(norm2[0]);
@@ -1049,9 +1004,59 @@ void launch_combine(void *out, const void *basis, int64_t stride, const double2 
         hipLaunchKernelGGL(k_combine<float2>, g, b, 0, s, (float2 *)out, (const float2 *)basis, stride, coef, k, n);
 }
 
-// Mixed form: out = coef[0] v0 (FP64) + sum_{j>=1} coef[j] basis_j (FP32).
-__global__ void k_combine_mixed(double2 *out, const double2 *v0, const float2 *basis, int64_t stride,
-                                const double2 *coef, int k, int64_t n) {
+// Lanczos step on unnormalised Krylov vectors u_j = n_j v_j (n_j^2 in
+// nsq[j] on the device): with h = H u_j and r = Re<u_j|h>,
+//   u_{j+1} = (h - (r / n_j^2) u_j) / n_j - (n_j / n_{j-1}) u_{j-1}
+// (= H v_j - alpha_j v_j - beta_{j-1} v_{j-1}, beta_{j-1} = n_j), written over
+// h; accumulates |u_{j+1}|^2 partials.  No normalisation pass is needed.
+template <class T>
+__global__ void __launch_bounds__(256) k_lanczos_step(T *w, const T *uj, const T *uprev, const double *dot_re,
+                                                      const double *nsq, int j, int64_t n, double *partials) {
+    const double nj = sqrt(nsq[j]);
+    const double a = 1.0 / nj, b = dot_re[0] / (nsq[j] * nj);
+    const double cp = (j >= 1 && uprev) ? sqrt(nsq[j] / nsq[j - 1]) : 0.0;
+    double acc = 0.0;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const double2 h = to_d2(w[i]), u = to_d2(uj[i]);
+        double2 r = make_double2(a * h.x - b * u.x, a * h.y - b * u.y);
+        if (cp != 0.0) {
+            const double2 p = to_d2(uprev[i]);
+            r.x -= cp * p.x;
+            r.y -= cp * p.y;
+        }
+        const T st = from_d2<T>(r);
+        w[i] = st;
+        const double2 q = to_d2(st);
+        acc = fma(q.x, q.x, acc);
+        acc = fma(q.y, q.y, acc);
+    }
+    __shared__ double red[4];
+    acc = wave_sum(acc);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0.0;
+        for (int q = 0; q < (int)(blockDim.x >> 6); q++) t += red[q];
+        partials[blockIdx.x] = t;
+    }
+}
+
+void launch_lanczos_step(void *w, const void *uj, const void *uprev, const double *dot_re, const double *nsq, int j,
+                         int64_t n, double *partials, int nblocks, double *nsq_next, int prec, hipStream_t s) {
+    if (prec == 0)
+        hipLaunchKernelGGL(k_lanczos_step<double2>, dim3(nblocks), dim3(256), 0, s, (double2 *)w,
+                           (const double2 *)uj, (const double2 *)uprev, dot_re, nsq, j, n, partials);
+    else
+        hipLaunchKernelGGL(k_lanczos_step<float2>, dim3(nblocks), dim3(256), 0, s, (float2 *)w, (const float2 *)uj,
+                           (const float2 *)uprev, dot_re, nsq, j, n, partials);
+    hipLaunchKernelGGL(k_norm2_final, dim3(1), dim3(64), 0, s, partials, nblocks, nsq_next);
+}
+
+// out = coef[0] v0 (FP64) + sum_{j>=1} coef[j] basis_j (storage type T, the
+// j-th vector at basis + j * stride).
+template <class T>
+__global__ void k_combine_v0(double2 *out, const double2 *v0, const T *basis, int64_t stride, const double2 *coef,
+                             int k, int64_t n) {
     __shared__ double2 c[kNKrylovMax];
     if (threadIdx.x < k) c[threadIdx.x] = coef[threadIdx.x];
     __syncthreads();
@@ -1062,10 +1067,13 @@ __global__ void k_combine_mixed(double2 *out, const double2 *v0, const float2 *b
     }
 }
 
-void launch_combine_mixed(double2 *out, const double2 *v0, const float2 *basis, int64_t stride,
-                          const double2 *coef, int k, int64_t n, hipStream_t s) {
-    hipLaunchKernelGGL(k_combine_mixed, dim3((unsigned)grid_for(n)), dim3(256), 0, s, out, v0, basis, stride, coef,
-                       k, n);
+void launch_combine_v0(double2 *out, const double2 *v0, const void *basis, int64_t stride, const double2 *coef, int k,
+                       int64_t n, int prec, hipStream_t s) {
+    const dim3 g((unsigned)grid_for(n)), b(256);
+    if (prec == 0)
+        hipLaunchKernelGGL(k_combine_v0<double2>, g, b, 0, s, out, v0, (const double2 *)basis, stride, coef, k, n);
+    else
+        hipLaunchKernelGGL(k_combine_v0<float2>, g, b, 0, s, out, v0, (const float2 *)basis, stride, coef, k, n);
 }
 
 // Precision conversion of complex vectors / matrices.
