@@ -23,9 +23,10 @@
 extern "C" {
 #endif
 
-/* Problem descriptor: Macaw's module data after read_input and the DVR part
- * of init_calc (psidef.f90:11-55, dvr.f90:14-27, hamilton.f90:24-36,
- * globals.f90:20-49).  Only DVR (grid) modes are described here. */
+/* Problem descriptor: Macaw's module data after read_input and init_calc
+ * (psidef.f90:11-55, dvr.f90:14-27, hamilton.f90:24-36, globals.f90:20-49).
+ * Grid (DVR) modes and Gaussian (G-MCTDH, "gwp") modes; the Gaussian fields
+ * sit at the end so a grid-only caller may leave them zero. */
 typedef struct mg_problem {
     int64_t nstates, nmodes, ndof;
     int64_t maxdof;               /* leading dimension of iDOF            */
@@ -48,6 +49,17 @@ typedef struct mg_problem {
     double writetime;             /* output interval, a.u.                 */
     double min_stepsize;          /* a.u.                                  */
     int64_t int_type_psi;         /* 1 = Cash-Karp, 2 = Dormand-Prince     */
+    /* Gaussian modes (psidef.f90:23-34,41; storage.f90; globals.f90:34-35).
+     * dof_gwp[ndof]: 1 for a DOF of grid type "gwp" (all DOFs of a mode share
+     * the type), or NULL when there is none.  gwp_a[maxdof, nspfmax, nmodes]:
+     * the fixed width parameter A < 0 of DOF k of Gaussian j of mode m
+     * (GWPa(k, j, i) of the i-th Gaussian mode; nspfmax = max nSPF); the
+     * SPF block of a Gaussian mode holds the xi(k, j) (complex, nmodeDOF per
+     * Gaussian) in psi.  eps_C, eps_ovl: Tikhonov parameters of the C-matrix
+     * solve and of the Gaussian overlap inverse. */
+    const int64_t *dof_gwp;
+    const double *gwp_a;
+    double eps_C, eps_ovl;
 } mg_problem;
 
 /* Observables written at every output time (output.f90:35-87,143-150). */

@@ -22,6 +22,7 @@ class MgProblem(C.Structure):
         ("eps_int", C.c_double), ("eps_psi", C.c_double), ("eps_rho", C.c_double),
         ("writetime", C.c_double), ("min_stepsize", C.c_double),
         ("int_type_psi", C.c_int64),
+        ("dof_gwp", i64p), ("gwp_a", f64p), ("eps_C", C.c_double), ("eps_ovl", C.c_double),
     ]
 
 
@@ -57,6 +58,13 @@ def make_mg_problem(prob):
         _i64(prob.lhs_el), _i64(prob.rhs_el), _i64(prob.iOpDof),
     ]
     (nmd, idof, ms, nspf, ngp, grid, d1, d2, xd1, ae, lhs, rhs, iop) = keep
+    gwp = getattr(prob, "dof_gwp", None)
+    if gwp is not None and np.any(gwp):
+        dg, ga = _i64(gwp), _f64(prob.gwp_a)
+        keep += [dg, ga]
+        dgp, gap = cptr(dg, i64p), cptr(ga)
+    else:
+        dgp, gap = None, None
     p = MgProblem(
         nstates=prob.nstates, nmodes=prob.nmodes, ndof=prob.ndof, maxdof=prob.maxDOF,
         nmodeDOF=cptr(nmd, i64p), iDOF=cptr(idof, i64p), MultiSet=cptr(ms, i64p),
@@ -66,7 +74,9 @@ def make_mg_problem(prob):
         iOpDof=cptr(iop, i64p),
         eps_int=prob.eps_int, eps_psi=prob.eps_psi, eps_rho=prob.eps_rho,
         writetime=prob.writetime, min_stepsize=prob.min_stepsize,
-        int_type_psi=prob.int_type_psi)
+        int_type_psi=prob.int_type_psi,
+        dof_gwp=dgp, gwp_a=gap,
+        eps_C=getattr(prob, "eps_C", 1e-8), eps_ovl=getattr(prob, "eps_ovl", 1e-8))
     return p, keep
 
 

@@ -2093,6 +2093,9 @@ int mg_create(const mg_problem *prob, int64_t precision, mg_ctx **out) {
     int rc = guarded(c, [&] {
         if (precision != 0 && precision != 1)
             throw HipError{"precision must be 0 (complex FP64) or 1 (mixed complex FP32/FP64)"};
+        if (prob->dof_gwp)
+            for (int64_t d = 0; d < prob->ndof; d++)
+                if (prob->dof_gwp[d]) throw HipError{"Gaussian (gwp) modes are not supported by this build"};
         int ndev = 0;
         HIP_CHECK(hipGetDeviceCount(&ndev));
         if (ndev < 1) throw HipError{"no HIP device"};

@@ -22,6 +22,8 @@ module macaw_gpu
       type(c_ptr) :: ae, lhs_el, rhs_el, iOpDof
       real(c_double) :: eps_int, eps_psi, eps_rho, writetime, min_stepsize
       integer(c_int64_t) :: int_type_psi
+      type(c_ptr) :: dof_gwp = c_null_ptr, gwp_a = c_null_ptr
+      real(c_double) :: eps_C = 1.d-8, eps_ovl = 1.d-8
    end type mg_problem
 
    type, bind(C) :: mg_output

@@ -40,7 +40,7 @@ class SPFDecl:
 
 @dataclass
 class GridDecl:
-    kind: str           # 'HO' (other grids are not part of this engine)
+    kind: str           # 'HO', or 'gwp' (Gaussian SPFs; npts/x0/mw = plot grid)
     npts: int
     x0: float
     mw: float
@@ -48,10 +48,13 @@ class GridDecl:
 
 @dataclass
 class InitSPF:
-    kind: str           # 'gauss'
+    kind: str           # 'gauss' or 'gwp'
     x0: float
     p0: float
     dx: float
+    dist: int = 0       # gwp: lattice type (0 = neighbour shells, 1 = gwpgrid_<mode> file)
+    ovl: float = 0.7    # gwp: overlap between neighbouring Gaussians
+    scw: float = 1.0    # gwp: width scaling per lattice step
 
 
 @dataclass
@@ -205,10 +208,17 @@ def _parse_spf(line: str, nstates: int) -> SPFDecl:
 
 
 def _parse_grid(line: str):
-    """One line of the GRID block (source/dvr.f90:32-103, HO branch)."""
+    """One line of the GRID block (source/dvr.f90:32-103, HO and gwp branches)."""
+    j = line.find("gwp")
+    if j >= 0:
+        k = int(line[:j])
+        toks = line[j + 3:].split()
+        if not toks:
+            return k, GridDecl("gwp", 101, -10.0, 10.0)
+        return k, GridDecl("gwp", int(toks[0]), _fortran_float(toks[1]), _fortran_float(toks[2]))
     j = line.find("HO")
     if j < 0:
-        raise InputError(f"only HO grids are supported by this engine: {line.strip()}")
+        raise InputError(f"only HO and gwp grids are supported by this engine: {line.strip()}")
     k = int(line[:j])
     toks = line[j + 2:].split()
     return k, GridDecl("HO", int(toks[0]), _fortran_float(toks[1]), _fortran_float(toks[2]))
@@ -225,11 +235,25 @@ def _parse_init_wf(deck: InputDeck, block: list) -> None:
     deck.init_state = int(m.group(1))
     for row in rows[2:]:
         j = row.find("gauss")
+        if j >= 0:
+            k = int(row[:j])
+            x0, p0, dx = (_fortran_float(t) for t in row[j + 5:].split()[:3])
+            deck.init_spf[k] = InitSPF("gauss", x0, p0, dx)
+            continue
+        j = row.find("gwp")
         if j < 0:
             raise InputError(f"The chosen initial SPF is unknown: {row.strip()}")
+        # k gwp <dist> x0 p0 dx [ovl = v] [scw = v] (source/init_calc.f90:676-714)
         k = int(row[:j])
-        x0, p0, dx = (_fortran_float(t) for t in row[j + 5:].split()[:3])
-        deck.init_spf[k] = InitSPF("gauss", x0, p0, dx)
+        toks = row[j + 3:].split()
+        dist = int(toks[0])
+        x0, p0, dx = (_fortran_float(t) for t in toks[1:4])
+        opts = {}
+        for key in ("ovl", "scw"):
+            m = re.search(key + r"\s*=\s*([-+0-9.eEdD]+)", row)
+            if m:
+                opts[key] = _fortran_float(m.group(1))
+        deck.init_spf[k] = InitSPF("gwp", x0, p0, dx, dist, opts.get("ovl", 0.7), opts.get("scw", 1.0))
 
 
 @dataclass

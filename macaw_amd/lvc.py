@@ -5,7 +5,10 @@ Writes Macaw-format input and operator files for the benchmark workloads:
 * ``c1``: 2 states, 5 modes, HO-DVR 24 points, 6 SPFs, 100 fs.
 * ``c2``: 4 states, 8 modes (+ diagonal quadratic terms), HO-DVR 32 points,
   8 SPFs (8^8 configurations per state), 200 fs.
-* ``tiny``: small variants used by the tests.
+* ``c3``: G-MCTDH, 3 states, 6 modes; modes 1-3 with 5 Gaussian SPFs (HO
+  ground-state widths on the reference's neighbour-shell lattice), modes 4-6
+  HO-DVR 28 points, 7 SPFs; C-matrix Tikhonov eps 1e-8, 150 fs.
+* ``t*``, ``g*``: small variants used by the tests (``g*`` with Gaussian modes).
 
 Energies are drawn in eV with ``numpy.random.default_rng(seed)`` and written
 in Hartree (divided by Macaw's ``au2eV = 27.2113961``).  Hamiltonian lines:
@@ -44,6 +47,14 @@ class LVCSpec:
     spf_integrator: str = "rk45"
     init_state: int = 0          # 0 -> top state
     multiset: bool = True
+    # G-MCTDH: the first gwp_modes modes carry ngauss Gaussian SPFs
+    gwp_modes: int = 0
+    ngauss: int = 5
+    gwp_ovl: float = 0.7
+    gwp_scw: float = 1.0
+    gwp_multiset: bool = True
+    eps_C: float = 1e-8
+    eps_S: float = 1e-8
 
 
 SPECS = {
@@ -65,6 +76,18 @@ SPECS = {
     # 8 SPFs everywhere (the 8x8 matrix-core path), two mode groups
     "t5": LVCSpec("t5", 15, 3, 4, 12, 8, 5.0, omega=(0.05, 0.30), kappa_sd=0.10,
                   gamma_sd=0.02, n_lambda=3, lam=(0.05, 0.15), gaps=(-0.2, 0.0, 0.2)),
+    # G-MCTDH: 3 states, modes 1-3 with 5 Gaussians, modes 4-6 HO-DVR 28 / 7 SPFs
+    "c3": LVCSpec("c3", 303, 3, 6, 28, 7, 150.0, omega=(0.05, 0.35), kappa_sd=0.10,
+                  n_lambda=3, lam=(0.05, 0.15), gaps=(-0.2, 0.0, 0.2), gwp_modes=3, ngauss=5,
+                  eps_C=1e-8),
+    # small G-MCTDH cases for the tests
+    "g1": LVCSpec("g1", 21, 2, 3, 8, 3, 5.0, omega=(0.05, 0.30), kappa_sd=0.10,
+                  n_lambda=1, lam=(0.10, 0.10), gaps=(-0.2, 0.2), gwp_modes=1, ngauss=3),
+    "g2": LVCSpec("g2", 22, 2, 3, 8, 3, 5.0, omega=(0.05, 0.30), kappa_sd=0.10,
+                  gamma_sd=0.02, n_lambda=2, lam=(0.05, 0.15), gaps=(-0.1, 0.1), gwp_modes=2,
+                  ngauss=3, gwp_multiset=False, spf_integrator="dopri"),
+    "g3": LVCSpec("g3", 23, 3, 4, 8, 3, 5.0, omega=(0.05, 0.30), kappa_sd=0.10,
+                  n_lambda=3, lam=(0.05, 0.15), gaps=(-0.2, 0.0, 0.2), gwp_modes=2, ngauss=4),
 }
 
 
@@ -119,21 +142,33 @@ def write_lvc(spec: LVCSpec, outdir: str) -> str:
     with open(os.path.join(outdir, opname), "w") as fh:
         fh.write("\n".join(op) + "\n")
     top = spec.init_state or S
-    set_tag = "M" if spec.multiset else "S"
     lines = [f"tfinal = {spec.tfinal}", f"tout = {spec.tout}", "",
              f"nstates = {S}", f"ndof = {f}", f"nmodes = {f}", "", "steps", "", "SPF"]
     for i in range(f):
-        counts = list(spec.nspf) if isinstance(spec.nspf, (list, tuple)) else [spec.nspf] * S
-        lines.append(f" {set_tag}, {i + 1} = " + ",".join(str(c) for c in counts))
+        gauss = i < spec.gwp_modes
+        multi = spec.gwp_multiset if gauss else spec.multiset
+        if gauss:
+            counts = [spec.ngauss] * S
+        else:
+            counts = list(spec.nspf) if isinstance(spec.nspf, (list, tuple)) else [spec.nspf] * S
+        lines.append(f" {'M' if multi else 'S'}, {i + 1} = " + ",".join(str(c) for c in counts))
     lines += ["", "GRID"]
     for i in range(f):
-        lines.append(f"  {i + 1}  HO  {spec.npts}  0.0  1.0")
+        if i < spec.gwp_modes:
+            lines.append(f"  {i + 1}  gwp  41  -8.0  8.0")
+        else:
+            lines.append(f"  {i + 1}  HO  {spec.npts}  0.0  1.0")
     lines += ["", f"Hamiltonian = {opname}", "", "INIT-WF", "   single-conf", f"   el  {top}"]
     for i in range(f):
-        lines.append(f"   {i + 1}   gauss  0.0  0.0  0.707107")
+        if i < spec.gwp_modes:
+            lines.append(f"   {i + 1}   gwp 0  0.0  0.0  0.707107    ovl = {spec.gwp_ovl}   scw = {spec.gwp_scw}")
+        else:
+            lines.append(f"   {i + 1}   gauss  0.0  0.0  0.707107")
     fd = lambda x: f"{x:.1e}".replace("e", "d")
-    lines += ["END-INIT-WF", "", f"eps_int = {fd(spec.eps_int)}",
-              "", "integration = CMF2", "Lanczos  1.d-7",
+    lines += ["END-INIT-WF", "", f"eps_int = {fd(spec.eps_int)}"]
+    if spec.gwp_modes:
+        lines += [f"eps_C = {fd(spec.eps_C)}", f"eps_S = {fd(spec.eps_S)}"]
+    lines += ["", "integration = CMF2", "Lanczos  1.d-7",
               f"{spec.spf_integrator}     {fd(spec.eps_psi)}", ""]
     path = os.path.join(outdir, f"{spec.name}.inp")
     with open(path, "w") as fh:

@@ -48,6 +48,10 @@ class Problem:
     A0: np.ndarray = None         # complex128[nAConftot]
     psi0: np.ndarray = None       # complex128[dimpsi]
     wgrid: np.ndarray = None
+    dof_gwp: np.ndarray = None    # int64[ndof], 1 = Gaussian DOF
+    gwp_a: np.ndarray = None      # float64[maxDOF, nSPFmax, nmodes] (Fortran order)
+    eps_C: float = 1e-8
+    eps_ovl: float = 1e-8
     extra: dict = field(default_factory=dict)
 
     # ---- derived layout (source/init_calc.f90:25-34, 160-181) ----
@@ -81,11 +85,21 @@ class Problem:
                 out[m] *= self.ngp[self.iDOF[k, m] - 1]
         return out
 
+    def gwp_mode(self, m: int) -> bool:
+        return self.dof_gwp is not None and bool(self.dof_gwp[self.iDOF[0, m] - 1])
+
+    def npar(self) -> np.ndarray:
+        """Parameters per SPF: grid points, or nmodeDOF for a Gaussian mode."""
+        ngpm = self.ngpm()
+        return np.array([int(self.nmodeDOF[m]) if self.gwp_mode(m) else int(ngpm[m])
+                         for m in range(self.nmodes)], dtype=np.int64)
+
     def ipsiSPF(self):
-        """1-based start of each (mode, state) SPF block and dimpsi."""
+        """1-based start of each (mode, state) SPF block and dimpsi
+        (source/init_calc.f90:160-181)."""
         ip = np.zeros((self.nmodes, self.nstates), dtype=np.int64)
         dim = 0
-        ngpm = self.ngpm()
+        ngpm = self.npar()
         for s in range(self.nstates):
             for m in range(self.nmodes):
                 if not self.MultiSet[m] and s > 0:
@@ -119,8 +133,14 @@ def build_problem(deck: InputDeck, operator_path: str | None = None) -> Problem:
     d1 = np.zeros((gmax, gmax, nd), order="F")
     d2 = np.zeros_like(d1)
     xd1 = np.zeros_like(d1)
+    dof_gwp = np.zeros(nd, dtype=np.int64)
     for k in range(nd):
         g = deck.grid[k + 1]
+        if g.kind == "gwp":
+            # Gaussian DOF: the declared grid only serves plotting in the reference
+            dof_gwp[k] = 1
+            grid[:g.npts, k] = np.linspace(g.x0, g.mw, g.npts)
+            continue
         x, w, a1, a2, ax, _ = harm_dvr(g.npts, g.x0, g.mw)
         n = g.npts
         grid[:n, k] = x
@@ -143,7 +163,14 @@ def build_problem(deck: InputDeck, operator_path: str | None = None) -> Problem:
                    eps_rho=deck.eps_rho, writetime=deck.tout,
                    min_stepsize=deck.min_stepsize, tinit=deck.tinit,
                    tfinal=deck.tfinal, int_type_psi=deck.int_type_psi or 1,
-                   wgrid=wgrid)
+                   wgrid=wgrid, eps_C=deck.eps_c, eps_ovl=deck.eps_ovl)
+    if dof_gwp.any():
+        for m in range(nm):
+            kinds = {int(dof_gwp[d - 1]) for d in iDOF[:nmodeDOF[m], m]}
+            if len(kinds) > 1:
+                raise ValueError("A mode cannot have simultaneously dvr and gwp representation")
+        prob.dof_gwp = dof_gwp
+        prob.gwp_a = np.zeros((maxdof, int(nSPF.max()), nm), order="F")
     prob.A0, prob.psi0 = initial_wavefunction(prob, deck)
     return prob
 
@@ -179,6 +206,9 @@ def initial_wavefunction(prob: Problem, deck: InputDeck):
     psi = np.zeros(dimpsi, dtype=np.complex128)
     ngpm = prob.ngpm()
     for m in range(nm):
+        if prob.gwp_mode(m):
+            gaussian_lattice(prob, deck, m, psi, ip)
+            continue
         nmd = int(prob.nmodeDOF[m])
         dofs = [int(d) for d in prob.iDOF[:nmd, m]]
         sizes = [int(prob.ngp[d - 1]) for d in dofs]
@@ -215,8 +245,88 @@ def initial_wavefunction(prob: Problem, deck: InputDeck):
     cnorm = 1.0 + 0.0j
     s = deck.init_state - 1
     for m in range(nm):
+        if prob.gwp_mode(m):
+            continue            # normalised Gaussians
         n = int(ngpm[m])
         v = psi[ip[m, s] - 1: ip[m, s] - 1 + n]
         cnorm = cnorm * np.vdot(v, v)
     A = A / np.sqrt(cnorm.real)
     return A, psi
+
+
+def gaussian_lattice(prob: Problem, deck: InputDeck, m: int, psi: np.ndarray, ip) -> None:
+    """Initial Gaussians of mode m (``SetSPF``, source/init_calc.f90:958-1081):
+    widths w_l = dx scw^|l| and centres xi_l along each DOF from the overlap
+    between neighbours, lattice indices by neighbour shells (distribution 0)
+    or from the file gwpgrid_<mode> (distribution 1); A = -(1/(2 w))^2 and
+    xi written for every electronic state."""
+    nmd = int(prob.nmodeDOF[m])
+    dofs = [int(d) for d in prob.iDOF[:nmd, m]]
+    nmax = int(prob.nSPF.max())
+    xi_v = np.zeros((2 * nmax + 1, nmd), dtype=np.complex128)
+    w_v = np.zeros((2 * nmax + 1, nmd))
+    dists = set()
+    for j, k in enumerate(dofs):
+        spec = deck.init_spf.get(k)
+        if spec is None or spec.kind != "gwp":
+            raise ValueError(f"Gaussian DOF {k} needs a gwp initial SPF")
+        dists.add(spec.dist)
+        dx, scw = spec.dx, spec.scw
+        for l in range(-nmax, nmax + 1):
+            w_v[l + nmax, j] = dx * scw ** abs(l)
+        xi_v[nmax, j] = 0.0
+        for l in range(1, nmax + 1):
+            d = -np.log(spec.ovl) + 0.5 * np.log(2 * scw / (1.0 + scw ** 2))
+            d = d * (1.0 + scw ** 2) / w_v[l - 1 + nmax, j] ** 2
+            d = (xi_v[l - 1 + nmax, j].real + np.sqrt(d)) / scw ** 2
+            xi_v[l + nmax, j] = d
+            xi_v[-l + nmax, j] = -d
+        eps = np.finfo(np.float64).eps
+        if max(abs(spec.x0), abs(spec.p0)) < eps:
+            phi = 0.0
+        else:
+            phi = complex(0.5 * spec.x0 / spec.dx ** 2, spec.p0)
+        if abs(phi) > eps:
+            xi_v[:, j] += phi
+    if len(dists) > 1:
+        raise ValueError(f"the same GWP distribution must be used for all DOFs of mode {m + 1}")
+    dist = dists.pop()
+    nS = int(prob.nSPF[m, :].max())
+    lat = np.zeros((nmd, nS), dtype=np.int64)
+    if nS > 1:
+        if dist == 0:
+            l = 1
+            i1, i2 = 0, 0
+            while l < nS:
+                for i3 in range(i1, i2 + 1):
+                    for j in range(nmd):
+                        for step in (-1, 1):
+                            cand = lat[:, i3].copy()
+                            cand[j] += step
+                            if any(np.array_equal(cand, lat[:, q]) for q in range(l)):
+                                continue
+                            lat[:, l] = cand
+                            l += 1
+                            if l >= nS:
+                                break
+                        if l >= nS:
+                            break
+                    if l >= nS:
+                        break
+                i1, i2 = i2 + 1, l - 1
+        elif dist == 1:
+            path = os.path.join(deck.basedir, f"gwpgrid_{m + 1}")
+            rows = np.loadtxt(path, dtype=np.int64, ndmin=2)
+            lat[:, :] = rows[:nS, :nmd].T
+        else:
+            raise ValueError(f"Unknown distribution type: {dist}")
+    xi = np.zeros((nmd, nS), dtype=np.complex128)
+    for l in range(nS):
+        for j in range(nmd):
+            xi[j, l] = xi_v[lat[j, l] + nmax, j]
+            prob.gwp_a[j, l, m] = -(0.5 / w_v[lat[j, l] + nmax, j]) ** 2
+    for s in range(prob.nstates):
+        n_s = int(prob.nSPF[m, s])
+        start = ip[m, s] - 1
+        psi[start:start + nmd * n_s] = xi[:, :n_s].reshape(-1, order="F")
+

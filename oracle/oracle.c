@@ -59,6 +59,16 @@ struct orc {
     double lan_alpha[NKRYLOV_MAX], lan_U[NKRYLOV_MAX * NKRYLOV_MAX];
     int nkrylov;
     cplx der_factor;
+    /* Gaussian (G-MCTDH) modes: psidef.f90:23-34,41, storage.f90:10-42,
+     * hamilton.f90:32 (Dh1modeD), cmf2_base.f90:28-29 (DD). */
+    int64_t ngwp;                 /* number of Gaussian modes */
+    int64_t *gwp;                 /* [nm] 1 = Gaussian mode */
+    int64_t *npar;                /* [nm] parameters per SPF: grid points or nmodeDOF */
+    double *gwpa;                 /* [maxdof * nspfmax * nm] widths A < 0 */
+    cplx *S00;                    /* [nspfmax^2 * nm * ns * ns] overlaps, S00(j1,j2,m,s1,s2) */
+    cplx *S00m1;                  /* [nspfmax^2 * nm * ns] regularised inverses */
+    cplx *DD;                     /* [nspfmax^2 * nm * ns] D matrices of the current state */
+    cplx *DHD;                    /* [nspfmax^2 * nopmax * nm] D^H h D */
     char err[256];
 };
 
@@ -68,6 +78,13 @@ struct orc {
 #define RHO(o, m, s) ((o)->rho + (size_t)(o)->nspfmax * (o)->nspfmax * ((m) + (o)->nm * (s)))
 #define RHOAVE(o, m) ((o)->rho_ave + (size_t)(o)->nspfmax * (o)->nspfmax * (m))
 #define NSPF(o, m, s) ((o)->nSPF[(m) + (o)->nm * (s)])
+#define NN(o) ((size_t)(o)->nspfmax * (o)->nspfmax)
+#define S00P(o, m, s1, s2) ((o)->S00 + NN(o) * ((m) + (o)->nm * ((s1) + (o)->ns * (s2))))
+#define S00M1(o, m, s) ((o)->S00m1 + NN(o) * ((m) + (o)->nm * (s)))
+#define DDP(o, dd, m, s) ((dd) + NN(o) * ((m) + (o)->nm * (s)))
+#define DHDP(o, m, op) ((o)->DHD + NN(o) * ((op) + (o)->nopmax * (m)))
+#define GWPA(o, k, j, m) ((o)->gwpa[(k) + (o)->p.maxdof * ((j) + (o)->nspfmax * (m))])
+#define NSETS(o, m) ((o)->MultiSet[m] ? (o)->ns : 1)
 #define IPSI(o, m, s) ((o)->ipsiSPF[(m) + (o)->nm * (s)])
 
 static void *xcalloc(size_t n, size_t sz) {
@@ -341,6 +358,113 @@ static void tridiag_eig(int64_t n, const double *alpha, const double *beta, doub
 }
 
 /* ---------------------------------------------------------------------- */
+/* Gaussian integrals.  g = exp(A x^2 + xi x), A < 0 real, xi complex; the
+ * normalised g has the real factor (-2A/pi)^(1/4) exp(Re(xi)^2 / (4A)). */
+
+/* <g1|g2> of normalised nmd-dimensional Gaussians (GauOvlMultiD,
+ * gmctdh_objects.f90:1497-1519, with its equal-width branch :1509-1513). */
+static cplx gau_ovl(const double *a1, const double *a2, const cplx *xi1, const cplx *xi2, int64_t nmd,
+                    int64_t sa1, int64_t sa2) {
+    int equal = 1;
+    for (int64_t k = 0; k < nmd; k++) if (!(fabs(a1[k * sa1] - a2[k * sa2]) < 2.220446049250313e-16)) equal = 0;
+    cplx c = 0;
+    if (equal) {
+        for (int64_t k = 0; k < nmd; k++) {
+            const cplx x = conj(xi1[k]) + xi2[k];
+            c += (creal(xi1[k]) * creal(xi1[k]) + creal(xi2[k]) * creal(xi2[k]) - 0.5 * x * x) / a1[k * sa1];
+        }
+        return cexp(0.25 * c);
+    }
+    double pr = 1.0;
+    for (int64_t k = 0; k < nmd; k++) {
+        const double A1 = a1[k * sa1], A2 = a2[k * sa2];
+        const cplx x = conj(xi1[k]) + xi2[k];
+        c += creal(xi1[k]) * creal(xi1[k]) / A1 + creal(xi2[k]) * creal(xi2[k]) / A2 - x * x / (A1 + A2);
+        pr *= 4.0 * A1 * A2 / ((A1 + A2) * (A1 + A2));
+    }
+    return cexp(0.25 * c) * sqrt(sqrt(pr));
+}
+
+/* Number of pairings: binomial(n, 2k) * (2k-1)!!, the moment coefficients
+ * of GauInt1D (hamilton.f90:761-785 tabulates them for n <= 20). */
+static double moment_coeff(int k, int n) {
+    double b = 1.0;
+    for (int i = 0; i < 2 * k; i++) b = b * (double)(n - i) / (double)(i + 1);
+    double df = 1.0;
+    for (int i = 2 * k - 1; i > 1; i -= 2) df *= (double)i;
+    return b * df;
+}
+
+static cplx cpow_int(cplx z, int n) {
+    cplx r = 1.0;
+    for (int i = 0; i < n; i++) r *= z;
+    return r;
+}
+
+/* <g1|O|g2> / <g1|g2> for the primitive operator code op (GauInt1D,
+ * hamilton.f90:745-849): 0 identity, -1 p, -2 d^2/dq^2, -3 qp + pq,
+ * 1..20 q^n.  Parametrised operators (codes > 200) are not supported:
+ * *bad is set. */
+static cplx gau_int(double A1, double A2, cplx xi1, cplx xi2, int64_t op, int *bad) {
+    const cplx x1c = conj(xi1);
+    if (op == 0) return 1.0;
+    if (op == -1) return -I * (A1 * xi2 - A2 * x1c) / (A1 + A2);
+    if (op == -2) {
+        const double s = A1 + A2;
+        const cplx t = (A2 * x1c - A1 * xi2) / s;
+        return t * t + 2.0 * A1 * A2 / s;
+    }
+    if (op == -3) {
+        const cplx g = A1 * A1 - A2 * A2 + (A2 * x1c - A1 * xi2) * (x1c + xi2);
+        return -I * g / ((A1 + A2) * (A1 + A2));
+    }
+    if (op == 1) return (x1c + xi2) / (-2.0 * (A1 + A2));
+    if (op == 2) {
+        const double s2 = 2.0 * (A1 + A2);
+        const cplx t = (x1c + xi2) / s2;
+        return t * t - 1.0 / s2;
+    }
+    if (op >= 3 && op <= 20) {
+        const int n = (int)op;
+        const cplx c = x1c + xi2;
+        const double a = -2.0 * (A1 + A2);
+        cplx g = 0;
+        if (n % 2) {
+            for (int k = 0; k <= n / 2; k++)
+                g += moment_coeff(k, n) / pow(a, (double)(n - k)) * cpow_int(c, n - 2 * k);
+        } else {
+            const cplx c2 = c * c / a;
+            cplx c1 = 1.0;
+            for (int k = n / 2; k >= 0; k--) {
+                g += moment_coeff(k, n) * c1;
+                c1 *= c2;
+            }
+            g /= pow(a, (double)(n / 2));
+        }
+        return g;
+    }
+    if (bad) *bad = 1;
+    return 0;
+}
+
+/* <dg1/dxi1|O|g2> / <g1|g2> (Gau10Int1D, hamilton.f90:853-902); the
+ * derivative is taken of the unnormalised g1 = exp(A1 x^2 + xi1 x). */
+static cplx gau10_int(double A1, double A2, cplx xi1, cplx xi2, int64_t op, int *bad) {
+    if (op == -1)
+        return -I * xi2 * gau_int(A1, A2, xi1, xi2, 1, bad) - I * (2.0 * A2) * gau_int(A1, A2, xi1, xi2, 2, bad);
+    if (op == -2)
+        return (A2 + A2 + xi2 * xi2) * gau_int(A1, A2, xi1, xi2, 1, bad) +
+               4.0 * A2 * xi2 * gau_int(A1, A2, xi1, xi2, 2, bad) +
+               4.0 * A2 * A2 * gau_int(A1, A2, xi1, xi2, 3, bad);
+    if (op == -3)
+        return -I * (gau_int(A1, A2, xi1, xi2, 1, bad) + 2.0 * xi2 * gau_int(A1, A2, xi1, xi2, 2, bad) +
+                     4.0 * A2 * gau_int(A1, A2, xi1, xi2, 3, bad));
+    if (op >= 0 && op <= 19) return gau_int(A1, A2, xi1, xi2, op + 1, bad);
+    if (bad) *bad = 1;
+    return 0;
+}
+
+/* ---------------------------------------------------------------------- */
 /* Setup: layout (init_calc.f90:25-34, 160-181) and operator basis
  * (alloc_h1mode hamilton.f90:91-147, alloc_IsIdentity :353-373). */
 
@@ -386,13 +510,23 @@ orc *orc_create(const mg_problem *p) {
         for (int64_t k = 0; k < o->nmodeDOF[m]; k++) g *= o->ngp[o->iDOF[k + p->maxdof * m] - 1];
         o->ngpm[m] = g;
     }
+    /* Gaussian modes: a mode is Gaussian when its first DOF is (init_calc.f90:56-75);
+     * its SPF block holds nmodeDOF complex xi per Gaussian (init_calc.f90:169-171) */
+    o->gwp = xcalloc(nm, sizeof(int64_t));
+    o->npar = xcalloc(nm, sizeof(int64_t));
+    for (int64_t m = 0; m < nm; m++) {
+        o->gwp[m] = p->dof_gwp ? (p->dof_gwp[o->iDOF[p->maxdof * m] - 1] != 0) : 0;
+        o->npar[m] = o->gwp[m] ? o->nmodeDOF[m] : o->ngpm[m];
+        o->ngwp += o->gwp[m];
+    }
+    if (o->ngwp && !p->gwp_a) snprintf(o->err, sizeof o->err, "Gaussian modes need gwp_a");
     o->ipsiSPF = xcalloc((size_t)nm * ns, sizeof(int64_t));
     int64_t dim = 0;
     for (int64_t s = 0; s < ns; s++)
         for (int64_t m = 0; m < nm; m++) {
             if (!o->MultiSet[m] && s > 0) { IPSI(o, m, s) = IPSI(o, m, 0); continue; }
             IPSI(o, m, s) = dim + 1;
-            dim += o->ngpm[m] * NSPF(o, m, s);
+            dim += o->npar[m] * NSPF(o, m, s);
         }
     o->dimpsi = dim;
     /* operator basis */
@@ -450,6 +584,17 @@ orc *orc_create(const mg_problem *p) {
     o->psi = xcalloc(o->dimpsi, sizeof(cplx));
     o->psistart = xcalloc(o->dimpsi, sizeof(cplx));
     o->BK = xcalloc((size_t)o->nconf * NKRYLOV_MAX, sizeof(cplx));
+    o->gwpa = xcalloc((size_t)p->maxdof * o->nspfmax * nm, sizeof(double));
+    if (o->ngwp && p->gwp_a) memcpy(o->gwpa, p->gwp_a, sizeof(double) * p->maxdof * o->nspfmax * nm);
+    o->S00 = xcalloc(nn * nm * ns * ns, sizeof(cplx));
+    o->S00m1 = xcalloc(nn * nm * ns, sizeof(cplx));
+    o->DD = xcalloc(nn * nm * ns, sizeof(cplx));
+    o->DHD = xcalloc(nn * o->nopmax * nm, sizeof(cplx));
+    /* diagonal of the intra-state overlaps stays 1 (storage.f90:55-63) */
+    for (int64_t m = 0; m < nm; m++)
+        if (o->gwp[m])
+            for (int64_t s = 0; s < ns; s++)
+                for (int64_t j = 0; j < NSPF(o, m, s); j++) S00P(o, m, s, s)[j + o->nspfmax * j] = 1.0;
     for (int64_t m = 0; m < nm; m++)
         if (o->nmodeDOF[m] > 8) snprintf(o->err, sizeof o->err, "more than 8 DOFs in a mode");
     return o;
@@ -461,7 +606,8 @@ void orc_destroy(orc *o) {
                     o->iOpDof, o->grid, o->d1, o->d2, o->xd1, o->ae, o->nAConf,
                     o->AStateOffset, o->ipsiSPF, o->ngpm, o->nOper, o->Oper, o->iOper,
                     o->IsIdentity, o->A, o->psi, o->Astart, o->psistart, o->h1, o->rho,
-                    o->rho_ave, o->mf, o->rmf, o->BK};
+                    o->rho_ave, o->mf, o->rmf, o->BK, o->gwp, o->npar, o->gwpa, o->S00,
+                    o->S00m1, o->DD, o->DHD};
     for (size_t i = 0; i < sizeof ptrs / sizeof ptrs[0]; i++) free(ptrs[i]);
     free(o);
 }
@@ -580,14 +726,283 @@ static void calc_h1mode_mode(orc *o, const cplx *psi0, int64_t m) {
     }
 }
 
+/* Xi of Gaussian j (0-based) of mode m in the SPF block of set s. */
+static const cplx *gxi(const orc *o, const cplx *psi, int64_t m, int64_t s, int64_t j) {
+    return psi + IPSI(o, m, s) - 1 + o->nmodeDOF[m] * j;
+}
+
+/* Calc_S00 (gmctdh_objects.f90:323-381) for Gaussian mode m: intra-state
+ * overlaps (diagonal kept at 1) and, for multi-set modes, the inter-state
+ * blocks and their Hermitian conjugates. */
+static void calc_s00(orc *o, const cplx *psi0, int64_t m) {
+    const int64_t nmax = o->nspfmax, nmd = o->nmodeDOF[m];
+    for (int64_t s = 0; s < NSETS(o, m); s++) {
+        const int64_t nS = NSPF(o, m, s);
+        cplx *S = S00P(o, m, s, s);
+        for (int64_t j2 = 0; j2 < nS; j2++)
+            for (int64_t j1 = 0; j1 < j2; j1++) {
+                S[j1 + nmax * j2] = gau_ovl(&GWPA(o, 0, j1, m), &GWPA(o, 0, j2, m), gxi(o, psi0, m, s, j1),
+                                            gxi(o, psi0, m, s, j2), nmd, 1, 1);
+                S[j2 + nmax * j1] = conj(S[j1 + nmax * j2]);
+            }
+    }
+    if (!o->MultiSet[m]) return;
+    for (int64_t s2 = 0; s2 < o->ns; s2++)
+        for (int64_t s1 = 0; s1 < s2; s1++) {
+            const int64_t n1 = NSPF(o, m, s1), n2 = NSPF(o, m, s2);
+            cplx *S = S00P(o, m, s1, s2), *T = S00P(o, m, s2, s1);
+            for (int64_t j2 = 0; j2 < n2; j2++)
+                for (int64_t j1 = 0; j1 < n1; j1++) {
+                    S[j1 + nmax * j2] = gau_ovl(&GWPA(o, 0, j1, m), &GWPA(o, 0, j2, m), gxi(o, psi0, m, s1, j1),
+                                                gxi(o, psi0, m, s2, j2), nmd, 1, 1);
+                    T[j2 + nmax * j1] = conj(S[j1 + nmax * j2]);
+                }
+        }
+}
+
+/* Calc_S00m1 (gmctdh_objects.f90:385-404): Tikhonov-regularised inverse
+ * of the intra-state overlap, eps_ovl. */
+static void calc_s00m1(orc *o, int64_t m) {
+    const int64_t nmax = o->nspfmax;
+    for (int64_t s = 0; s < NSETS(o, m); s++) {
+        const int64_t nS = NSPF(o, m, s);
+        cplx *a = xcalloc((size_t)nS * nS, sizeof(cplx)), *b = xcalloc((size_t)nS * nS, sizeof(cplx));
+        const cplx *S = S00P(o, m, s, s);
+        for (int64_t j2 = 0; j2 < nS; j2++)
+            for (int64_t j1 = 0; j1 < nS; j1++)
+                a[j1 + nS * j2] = (j1 <= j2) ? S[j1 + nmax * j2] : conj(S[j2 + nmax * j1]);
+        for (int64_t j = 0; j < nS; j++) a[j + nS * j] = creal(a[j + nS * j]);
+        std_reg_inv(nS, a, o->p.eps_ovl, 1, b);
+        cplx *R = S00M1(o, m, s);
+        for (int64_t j2 = 0; j2 < nS; j2++)
+            for (int64_t j1 = 0; j1 < nS; j1++) R[j1 + nmax * j2] = b[j1 + nS * j2];
+        free(a);
+        free(b);
+    }
+}
+
+/* Calc_h1mode, Gaussian branch (hamilton.f90:486-525): h = S00 prod_k
+ * GauInt1D; the identity operator copies S00.  Same-state matrices are
+ * Hermitian (the upper triangle is computed, the lower mirrored). */
+static int calc_h1mode_gwp(orc *o, const cplx *psi0, int64_t m) {
+    const int64_t nmax = o->nspfmax, nmd = o->nmodeDOF[m];
+    int bad = 0;
+    for (int64_t i = 0; i < o->nOper[m]; i++) {
+        const oper_t *op = &o->Oper[m * o->nopmax + i];
+        const int64_t s1 = op->lhs - 1, s2 = op->rhs - 1;
+        const int64_t n1 = NSPF(o, m, s1), n2 = NSPF(o, m, s2);
+        const cplx *S = S00P(o, m, s1, s2);
+        cplx *h = H1(o, m, i);
+        const int same = (s1 == s2);
+        for (int64_t j2 = 0; j2 < n2; j2++)
+            for (int64_t j1 = 0; j1 < (same ? j2 + 1 : n1); j1++) {
+                cplx v = S[j1 + nmax * j2];
+                if (!o->IsIdentity[m + o->nm * i]) {
+                    const cplx *x1 = gxi(o, psi0, m, s1, j1), *x2 = gxi(o, psi0, m, s2, j2);
+                    cplx pr = 1.0;
+                    for (int64_t k = 0; k < nmd; k++)
+                        pr *= gau_int(GWPA(o, k, j1, m), GWPA(o, k, j2, m), x1[k], x2[k], op->codes[k], &bad);
+                    v = v * pr;
+                }
+                h[j1 + nmax * j2] = v;
+                if (same) h[j2 + nmax * j1] = conj(v);
+            }
+    }
+    if (bad) snprintf(o->err, sizeof o->err, "operator not supported on a Gaussian mode");
+    return bad ? -1 : 0;
+}
+
+static void calc_h1mode_any(orc *o, const cplx *psi0, int64_t m) {
+    if (o->gwp[m]) calc_h1mode_gwp(o, psi0, m);
+    else calc_h1mode_mode(o, psi0, m);
+}
+
+/* Calc_S00 for every Gaussian mode, then Calc_h1mode for every mode (the
+ * order of cmf2_base.f90:83-90). */
+static void calc_operators(orc *o, const cplx *psi0) {
+    for (int64_t m = 0; m < o->nm; m++) if (o->gwp[m]) calc_s00(o, psi0, m);
+    for (int64_t m = 0; m < o->nm; m++) calc_h1mode_any(o, psi0, m);
+}
+
+/* Calc_DHD (hamilton.f90:535-574): D^H h D per Gaussian mode and operator,
+ * 1 for the identity (D^H S D = 1), entries with |x| <= eps_machine set to 0. */
+static void calc_dhd(orc *o, const cplx *DD) {
+    const int64_t nmax = o->nspfmax;
+    cplx *aux = xcalloc(NN(o), sizeof(cplx));
+    for (int64_t m = 0; m < o->nm; m++) {
+        if (!o->gwp[m]) continue;
+        for (int64_t i = 0; i < o->nOper[m]; i++) {
+            const oper_t *op = &o->Oper[m * o->nopmax + i];
+            const int64_t s1 = op->lhs - 1, s2 = op->rhs - 1;
+            const int64_t n1 = NSPF(o, m, s1), n2 = NSPF(o, m, s2);
+            cplx *R = DHDP(o, m, i);
+            memset(R, 0, sizeof(cplx) * NN(o));
+            if (o->IsIdentity[m + o->nm * i]) {
+                for (int64_t j = 0; j < (n1 < n2 ? n1 : n2); j++) R[j + nmax * j] = 1.0;
+                continue;
+            }
+            const cplx *h = H1(o, m, i), *D1 = DDP(o, DD, m, s1), *D2 = DDP(o, DD, m, s2);
+            if (s1 == s2) {
+                /* aux = h D; R = (D^H aux + aux^H D) / 2, upper triangle, real diagonal */
+                for (int64_t b = 0; b < n2; b++)
+                    for (int64_t a = 0; a < n1; a++) {
+                        cplx acc = 0;
+                        for (int64_t k = 0; k < n1; k++) acc += h[a + nmax * k] * D2[k + nmax * b];
+                        aux[a + nmax * b] = acc;
+                    }
+                for (int64_t b = 0; b < n1; b++)
+                    for (int64_t a = 0; a <= b; a++) {
+                        cplx x = 0, y = 0;
+                        for (int64_t k = 0; k < n1; k++) {
+                            x += conj(D1[k + nmax * a]) * aux[k + nmax * b];
+                            y += conj(aux[k + nmax * a]) * D1[k + nmax * b];
+                        }
+                        cplx v = 0.5 * x + 0.5 * y;
+                        if (a == b) v = creal(v);
+                        R[a + nmax * b] = v;
+                    }
+            } else {
+                /* aux = D1^H h; R = aux D2 */
+                for (int64_t b = 0; b < n2; b++)
+                    for (int64_t a = 0; a < n1; a++) {
+                        cplx acc = 0;
+                        for (int64_t k = 0; k < n1; k++) acc += conj(D1[k + nmax * a]) * h[k + nmax * b];
+                        aux[a + nmax * b] = acc;
+                    }
+                for (int64_t b = 0; b < n2; b++)
+                    for (int64_t a = 0; a < n1; a++) {
+                        cplx acc = 0;
+                        for (int64_t k = 0; k < n2; k++) acc += aux[a + nmax * k] * D2[k + nmax * b];
+                        R[a + nmax * b] = acc;
+                    }
+            }
+            const double eps = 2.220446049250313e-16;
+            for (int64_t b = 0; b < n2; b++)
+                for (int64_t a = 0; a < (s1 == s2 ? b + 1 : n1); a++)
+                    if (!(cabs(R[a + nmax * b]) > eps)) R[a + nmax * b] = 0;
+            if (s1 == s2)
+                for (int64_t b = 0; b < n1; b++)
+                    for (int64_t a = b + 1; a < n1; a++) R[a + nmax * b] = conj(R[b + nmax * a]);
+        }
+    }
+    free(aux);
+}
+
+/* InverseSquareRootU (maths.f90:621-644): D = U^-1 with S = U^H U
+ * (Cholesky), so D^H S D = 1 and D upper triangular. */
+static int inverse_sqrt_u(int64_t n, const cplx *S, int64_t lds, cplx *D, int64_t ldd) {
+    cplx *U = xcalloc((size_t)n * n, sizeof(cplx));
+    for (int64_t j = 0; j < n; j++) {
+        double d = creal(S[j + lds * j]);
+        for (int64_t k = 0; k < j; k++) d -= creal(conj(U[k + n * j]) * U[k + n * j]);
+        if (!(d > 0.0)) { free(U); return -1; }
+        const double ujj = sqrt(d);
+        U[j + n * j] = ujj;
+        for (int64_t i = j + 1; i < n; i++) {
+            cplx v = S[j + lds * i];
+            for (int64_t k = 0; k < j; k++) v -= conj(U[k + n * j]) * U[k + n * i];
+            U[j + n * i] = v / ujj;
+        }
+    }
+    /* upper-triangular inverse, column by column */
+    for (int64_t c = 0; c < n; c++) {
+        for (int64_t r = 0; r < n; r++) D[r + ldd * c] = 0;
+        D[c + ldd * c] = 1.0 / U[c + n * c];
+        for (int64_t r = c - 1; r >= 0; r--) {
+            cplx acc = 0;
+            for (int64_t k = r + 1; k <= c; k++) acc += U[r + n * k] * D[k + ldd * c];
+            D[r + ldd * c] = -acc / U[r + n * r];
+        }
+    }
+    free(U);
+    return 0;
+}
+
+/* D0 = InverseSquareRootU(S00(psi0)) for every Gaussian mode and set
+ * (cmf2_base.f90:72-80). */
+static int init_dd(orc *o, const cplx *psi0, cplx *DD) {
+    int rc = 0;
+    for (int64_t m = 0; m < o->nm; m++) {
+        if (!o->gwp[m]) continue;
+        calc_s00(o, psi0, m);
+        for (int64_t s = 0; s < NSETS(o, m); s++) {
+            cplx *S = S00P(o, m, s, s);
+            if (inverse_sqrt_u(NSPF(o, m, s), S, o->nspfmax, DDP(o, DD, m, s), o->nspfmax) != 0) {
+                snprintf(o->err, sizeof o->err, "Gaussian overlap not positive definite (mode %lld)",
+                         (long long)m + 1);
+                rc = -1;
+            }
+        }
+    }
+    return rc;
+}
+
+/* Mode products over the Gaussian modes only: Y_s = (x_m M_(m,set(s))) X_s,
+ * M from mat(o, m, set, scratch). */
+typedef const cplx *(*gmat_fn)(orc *o, int64_t m, int64_t set, const cplx *DD, cplx *scratch);
+
+static void gwp_chain(orc *o, const cplx *X, cplx *Y, gmat_fn mat, const cplx *DD) {
+    int64_t big = 0;
+    for (int64_t s = 0; s < o->ns; s++) if (o->nAConf[s] > big) big = o->nAConf[s];
+    cplx *p1 = xcalloc(big, sizeof(cplx)), *p2 = xcalloc(big, sizeof(cplx)), *sc = xcalloc(NN(o), sizeof(cplx));
+    int64_t sz[64];
+    for (int64_t s = 0; s < o->ns; s++) {
+        for (int64_t m = 0; m < o->nm; m++) sz[m] = NSPF(o, m, s);
+        memcpy(p1, X + o->AStateOffset[s], sizeof(cplx) * o->nAConf[s]);
+        for (int64_t m = 0; m < o->nm; m++) {
+            if (!o->gwp[m]) continue;
+            const cplx *M = mat(o, m, o->MultiSet[m] ? s : 0, DD, sc);
+            mode_product(o->nm, sz, m, M, o->nspfmax, sz[m], p1, p2);
+            cplx *sw = p1; p1 = p2; p2 = sw;
+        }
+        memcpy(Y + o->AStateOffset[s], p1, sizeof(cplx) * o->nAConf[s]);
+    }
+    free(p1); free(p2); free(sc);
+}
+
+/* (S D)^H = D^H S for A_to_B (cmf2_base.f90:264-305: B = D^H S A). */
+static const cplx *mat_dhs(orc *o, int64_t m, int64_t set, const cplx *DD, cplx *sc) {
+    const int64_t nmax = o->nspfmax, n = NSPF(o, m, set);
+    const cplx *S = S00P(o, m, set, set), *D = DDP(o, DD, m, set);
+    for (int64_t b = 0; b < n; b++)
+        for (int64_t a = 0; a < n; a++) {
+            cplx acc = 0;       /* (S D)[b, a] */
+            for (int64_t k = 0; k < n; k++) acc += S[b + nmax * k] * D[k + nmax * a];
+            sc[a + nmax * b] = conj(acc);
+        }
+    return sc;
+}
+
+static const cplx *mat_d(orc *o, int64_t m, int64_t set, const cplx *DD, cplx *sc) {
+    (void)sc;
+    return DDP(o, DD, m, set);
+}
+
+static void a_to_b(orc *o, const cplx *A, cplx *B, const cplx *DD) {
+    if (!o->ngwp) { if (A != B) memcpy(B, A, sizeof(cplx) * o->nconf); return; }
+    gwp_chain(o, A, B, mat_dhs, DD);
+}
+
+/* B_to_A (cmf2_base.f90:309-345): A = D B. */
+static void b_to_a(orc *o, const cplx *B, cplx *A, const cplx *DD) {
+    if (!o->ngwp) { if (A != B) memcpy(A, B, sizeof(cplx) * o->nconf); return; }
+    gwp_chain(o, B, A, mat_d, DD);
+}
+
 void orc_calc_h1mode(orc *o, double *h1_out) {
-    for (int64_t m = 0; m < o->nm; m++) calc_h1mode_mode(o, o->psi, m);
+    calc_operators(o, o->psi);
     if (h1_out) memcpy(h1_out, o->h1, sizeof(cplx) * o->nspfmax * o->nspfmax * o->nopmax * o->nm);
 }
 
-/* Calc_rho1mode (gmctdh_objects.f90:408-477) for DVR modes. */
+/* Calc_rho1mode (gmctdh_objects.f90:408-477).  With Gaussian modes the
+ * overlaps S00 of every other Gaussian mode are applied first (:439-447)
+ * and rho = (X^H A + A^H X) / 2 on the hole matrices (zher2k, :452-457);
+ * S00 must hold the overlaps of the current SPFs. */
 static void calc_rho(orc *o, const cplx *A0) {
     const int64_t nm = o->nm, nmax = o->nspfmax;
+    int64_t big = 0;
+    for (int64_t s = 0; s < o->ns; s++) if (o->nAConf[s] > big) big = o->nAConf[s];
+    cplx *X = o->ngwp ? xcalloc(big, sizeof(cplx)) : NULL, *T = o->ngwp ? xcalloc(big, sizeof(cplx)) : NULL;
     for (int64_t m = 0; m < nm; m++) {
         for (int64_t s = 0; s < o->ns; s++) {
             cplx *r = RHO(o, m, s);
@@ -596,7 +1011,23 @@ static void calc_rho(orc *o, const cplx *A0) {
             for (int64_t q = 0; q < m; q++) ll *= NSPF(o, q, s);
             for (int64_t q = m + 1; q < nm; q++) nl *= NSPF(o, q, s);
             const cplx *As = A0 + o->AStateOffset[s];
-            hole_gram(ll, nl, nS, nS, As, As, 1.0, r, nmax);
+            int other = 0;
+            for (int64_t q = 0; q < nm; q++) if (o->gwp[q] && q != m) other = 1;
+            if (!other) {
+                hole_gram(ll, nl, nS, nS, As, As, 1.0, r, nmax);
+            } else {
+                int64_t sz[64];
+                for (int64_t q = 0; q < nm; q++) sz[q] = NSPF(o, q, s);
+                memcpy(X, As, sizeof(cplx) * o->nAConf[s]);
+                for (int64_t q = 0; q < nm; q++) {
+                    if (!o->gwp[q] || q == m) continue;
+                    const int64_t jel = o->MultiSet[q] ? s : 0;
+                    mode_product(nm, sz, q, S00P(o, q, jel, jel), nmax, sz[q], X, T);
+                    cplx *sw = X; X = T; T = sw;
+                }
+                hole_gram(ll, nl, nS, nS, X, As, 0.5, r, nmax);
+                hole_gram(ll, nl, nS, nS, As, X, 0.5, r, nmax);
+            }
             for (int64_t j = 0; j < nS; j++) r[j + nmax * j] = creal(r[j + nmax * j]);
             for (int64_t b = 0; b < nS; b++)
                 for (int64_t a = b + 1; a < nS; a++) r[a + nmax * b] = conj(r[b + nmax * a]);
@@ -610,6 +1041,8 @@ static void calc_rho(orc *o, const cplx *A0) {
             }
         }
     }
+    free(X);
+    free(T);
 }
 
 void orc_calc_rho(orc *o, double *rho_out) {
@@ -650,7 +1083,7 @@ static int calc_meanfield(orc *o, const cplx *A0) {
             int64_t jm = nm - im;
             int64_t op = o->iOper[jm + nm * t] - 1;
             cplx *src = AR + (size_t)big * (im - 1), *dst = AR + (size_t)big * im;
-            if (o->IsIdentity[jm + nm * op]) { memcpy(dst, src, sizeof(cplx) * cur); continue; }
+            if (o->IsIdentity[jm + nm * op] && !o->gwp[jm]) { memcpy(dst, src, sizeof(cplx) * cur); continue; }
             int64_t nS1 = NSPF(o, jm, s1);
             mode_product(nm, sz, jm, H1(o, jm, op), nmax, nS1, src, dst);
             cur = cur / sz[jm] * nS1;
@@ -665,7 +1098,7 @@ static int calc_meanfield(orc *o, const cplx *A0) {
             int64_t m = im - 1;
             int64_t op = o->iOper[m + nm * t] - 1;
             cplx *src = AL + (size_t)big * (im - 1), *dst = AL + (size_t)big * im;
-            if (o->IsIdentity[m + nm * op]) { memcpy(dst, src, sizeof(cplx) * cur); continue; }
+            if (o->IsIdentity[m + nm * op] && !o->gwp[m]) { memcpy(dst, src, sizeof(cplx) * cur); continue; }
             int64_t nS1 = NSPF(o, m, s1), nS2 = NSPF(o, m, s2);
             for (int64_t a = 0; a < nS2; a++)
                 for (int64_t b = 0; b < nS1; b++) hh[a + nmax * b] = conj(H1(o, m, op)[b + nmax * a]);
@@ -698,9 +1131,10 @@ static int calc_meanfield(orc *o, const cplx *A0) {
     }
     free(AL);
     free(AR);
-    /* Calc_rhom1MF */
+    /* Calc_rhom1MF (grid modes only, gmctdh_objects.f90:1159) */
     cplx *rinv = xcalloc(nn, sizeof(cplx)), *rsq = xcalloc(nn, sizeof(cplx));
     for (int64_t m = 0; m < nm; m++) {
+        if (o->gwp[m]) continue;
         int64_t nsets = o->MultiSet[m] ? o->ns : 1;
         for (int64_t s = 0; s < nsets; s++) {
             int64_t nS = NSPF(o, m, s);
@@ -738,7 +1172,8 @@ int orc_calc_meanfield(orc *o, double *mf_out, double *rmf_out, double *energy_o
     return 0;
 }
 
-/* Calc_DHD_B, DVR modes (cmf2_derivatives.f90:1-80). */
+/* Calc_DHD_B (cmf2_derivatives.f90:1-80): h1mode on grid modes, D^H h D on
+ * Gaussian modes (DHD must be current), identities skipped on both. */
 static void apply_h(orc *o, const cplx *B, cplx *HB) {
     const int64_t nm = o->nm, nmax = o->nspfmax;
     memset(HB, 0, sizeof(cplx) * o->nconf);
@@ -753,7 +1188,7 @@ static void apply_h(orc *o, const cplx *B, cplx *HB) {
             int64_t op = o->iOper[m + nm * t] - 1;
             if (o->IsIdentity[m + nm * op]) continue;
             int64_t nS1 = NSPF(o, m, s1);
-            mode_product(nm, sz, m, H1(o, m, op), nmax, nS1, b2, b1);
+            mode_product(nm, sz, m, o->gwp[m] ? DHDP(o, m, op) : H1(o, m, op), nmax, nS1, b2, b1);
             cplx *sw = b1; b1 = b2; b2 = sw;
             sz[m] = nS1;
         }
@@ -831,8 +1266,214 @@ void orc_der_spf(orc *o, int64_t im, const double *psi_in, double *dpsi_out) {
     der_spf(o, im - 1, (const cplx *)psi_in, (cplx *)dpsi_out);
 }
 
+/* StdRegSol 'tikh' (maths.f90:87-162): x = U diag(w / (w^2 + eps^2)) U^H y
+ * for the Hermitian matrix whose upper triangle is in A (ld lda). */
+static void std_reg_sol_tikh(int64_t n, const cplx *A, int64_t lda, const cplx *y, double eps, cplx *x) {
+    cplx *F = xcalloc((size_t)n * n, sizeof(cplx)), *U = xcalloc((size_t)n * n, sizeof(cplx));
+    cplx *uy = xcalloc(n, sizeof(cplx));
+    double *w = xcalloc(n, sizeof(double));
+    for (int64_t j2 = 0; j2 < n; j2++)
+        for (int64_t j1 = 0; j1 < n; j1++)
+            F[j1 + n * j2] = (j1 < j2) ? A[j1 + lda * j2] : (j1 == j2 ? creal(A[j1 + lda * j1]) : conj(A[j2 + lda * j1]));
+    herm_eig(n, F, w, U);
+    for (int64_t k = 0; k < n; k++) {
+        cplx acc = 0;
+        for (int64_t i = 0; i < n; i++) acc += conj(U[i + n * k]) * y[i];
+        uy[k] = acc * (w[k] / (w[k] * w[k] + eps * eps));
+    }
+    for (int64_t i = 0; i < n; i++) {
+        cplx acc = 0;
+        for (int64_t k = 0; k < n; k++) acc += U[i + n * k] * uy[k];
+        x[i] = acc;
+    }
+    free(F); free(U); free(uy); free(w);
+}
+
+/* Calc_Der_SPF, Gaussian branch (cmf2_derivatives.f90:175-232) with
+ * Calc_CY (gmctdh_objects.f90:1198-1333, the form without DD0):
+ * S00, S00^-1 and h1mode of mode m are recomputed from psi0 (as the
+ * reference does, leaving them in the shared arrays); rho and the mean
+ * fields are those of the last calc_meanfield.  dpsi receives der_factor
+ * times the regularised solution of C dxi = Y; dDD = -tau D. */
+static int der_spf_gwp(orc *o, int64_t m, const cplx *psi0, const cplx *DD0, cplx *dpsi, cplx *dDD) {
+    const int64_t nmax = o->nspfmax, nmd = o->nmodeDOF[m], nm = o->nm;
+    int bad = 0;
+    calc_s00(o, psi0, m);
+    calc_s00m1(o, m);
+    if (calc_h1mode_gwp(o, psi0, m) != 0) return -1;
+    int64_t ncy = 0;
+    for (int64_t s = 0; s < o->ns; s++) if (NSPF(o, m, s) * nmd > ncy) ncy = NSPF(o, m, s) * nmd;
+    cplx *gx = xcalloc((size_t)nmd * nmax * nmax, sizeof(cplx));      /* gIntx(k, j1, j2) */
+    cplx *Sa0 = xcalloc((size_t)ncy * nmax, sizeof(cplx));
+    cplx *SaSm = xcalloc((size_t)ncy * nmax, sizeof(cplx));            /* Sa0 S00^-1 */
+    cplx *Sab = xcalloc((size_t)ncy * ncy, sizeof(cplx));
+    cplx *C = xcalloc((size_t)ncy * ncy, sizeof(cplx));
+    cplx *Y = xcalloc(ncy, sizeof(cplx));
+    cplx *Ha0 = xcalloc((size_t)ncy * nmax, sizeof(cplx));
+    cplx *gI = xcalloc(nmd, sizeof(cplx));
+    cplx *tau = xcalloc(NN(o), sizeof(cplx));
+#define GX(k, a, b) gx[(k) + nmd * ((a) + nmax * (b))]
+    for (int64_t s = 0; s < NSETS(o, m); s++) {
+        const int64_t nS = NSPF(o, m, s), nG = nS * nmd;
+        const cplx *S = S00P(o, m, s, s), *Sm = S00M1(o, m, s);
+        memset(Y, 0, sizeof(cplx) * ncy);
+        for (int64_t j2 = 0; j2 < nS; j2++)
+            for (int64_t j1 = 0; j1 <= j2; j1++) {
+                const cplx *x1 = gxi(o, psi0, m, s, j1), *x2 = gxi(o, psi0, m, s, j2);
+                for (int64_t k = 0; k < nmd; k++)
+                    GX(k, j1, j2) = gau_int(GWPA(o, k, j1, m), GWPA(o, k, j2, m), x1[k], x2[k], 1, &bad);
+            }
+        /* Sa0((j1 nmd + k), j2) = <x_k g_j1 | g_j2> */
+        for (int64_t j2 = 0; j2 < nS; j2++)
+            for (int64_t j1 = 0; j1 < nS; j1++)
+                for (int64_t k = 0; k < nmd; k++) {
+                    if (j1 <= j2) Sa0[(j1 * nmd + k) + ncy * j2] = S[j1 + nmax * j2] * GX(k, j1, j2);
+                }
+        for (int64_t j2 = 0; j2 < nS; j2++)
+            for (int64_t j1 = j2 + 1; j1 < nS; j1++)
+                for (int64_t k = 0; k < nmd; k++)
+                    Sa0[(j1 * nmd + k) + ncy * j2] = conj(Sa0[(j2 * nmd + k) + ncy * j1]);
+        /* Sa0 S00^-1 (zhemm 'R','U') */
+        for (int64_t c = 0; c < nS; c++)
+            for (int64_t r = 0; r < nG; r++) {
+                cplx acc = 0;
+                for (int64_t q = 0; q < nS; q++) acc += Sa0[r + ncy * q] * Sm[q + nmax * c];
+                SaSm[r + ncy * c] = acc;
+            }
+        /* Sab for the upper block triangle (j1 <= j2) */
+        for (int64_t i = 0; i < ncy * ncy; i++) Sab[i] = 0;
+        for (int64_t j2 = 0; j2 < nS; j2++)
+            for (int64_t k2 = 0; k2 < nmd; k2++)
+                for (int64_t j1 = 0; j1 <= j2; j1++)
+                    for (int64_t k1 = 0; k1 < nmd; k1++) {
+                        cplx v;
+                        if (k1 == k2)
+                            v = gau_int(GWPA(o, k1, j1, m), GWPA(o, k2, j2, m), gxi(o, psi0, m, s, j1)[k1],
+                                        gxi(o, psi0, m, s, j2)[k2], 2, &bad);
+                        else
+                            v = GX(k1, j1, j2) * GX(k2, j1, j2);
+                        Sab[(j1 * nmd + k1) + ncy * (j2 * nmd + k2)] = S[j1 + nmax * j2] * v;
+                    }
+        /* Sab -= (SaSm Sa0^H + Sa0 SaSm^H) / 2 on the upper triangle (zher2k) */
+        for (int64_t c = 0; c < nG; c++)
+            for (int64_t r = 0; r <= c; r++) {
+                cplx acc = 0;
+                for (int64_t q = 0; q < nS; q++)
+                    acc += SaSm[r + ncy * q] * conj(Sa0[c + ncy * q]) + Sa0[r + ncy * q] * conj(SaSm[c + ncy * q]);
+                cplx v = Sab[r + ncy * c] - 0.5 * acc;
+                if (r == c) v = creal(v);
+                Sab[r + ncy * c] = v;
+            }
+        /* C = rho o Sab (upper), rho of the set (multi-set) or rho_ave */
+        const cplx *rh = o->MultiSet[m] ? RHO(o, m, s) : RHOAVE(o, m);
+        for (int64_t j2 = 0; j2 < nS; j2++)
+            for (int64_t k2 = 0; k2 < nmd; k2++)
+                for (int64_t j1 = 0; j1 <= j2; j1++)
+                    for (int64_t k1 = 0; k1 < nmd; k1++) {
+                        const int64_t r = j1 * nmd + k1, c = j2 * nmd + k2;
+                        C[r + ncy * c] = rh[j1 + nmax * j2] * Sab[r + ncy * c];
+                    }
+        /* Y = sum_op MF(j1, j2) (H_a0 - Sa0 S^-1 h)(j1 k, j2) */
+        for (int64_t i = 0; i < o->nOper[m]; i++) {
+            if (o->IsIdentity[m + nm * i]) continue;
+            const oper_t *op = &o->Oper[m * o->nopmax + i];
+            if (op->lhs - 1 != s && o->MultiSet[m]) continue;
+            const int64_t s2 = op->rhs - 1, n2 = NSPF(o, m, s2);
+            const cplx *S12 = S00P(o, m, s, s2);
+            for (int64_t j2 = 0; j2 < n2; j2++)
+                for (int64_t j1 = 0; j1 < nS; j1++) {
+                    const cplx *x1 = gxi(o, psi0, m, s, j1), *x2 = gxi(o, psi0, m, s2, j2);
+                    for (int64_t k = 0; k < nmd; k++)
+                        gI[k] = gau_int(GWPA(o, k, j1, m), GWPA(o, k, j2, m), x1[k], x2[k], op->codes[k], &bad);
+                    for (int64_t k = 0; k < nmd; k++) {
+                        cplx pr = S12[j1 + nmax * j2];
+                        for (int64_t q = 0; q < nmd; q++)
+                            if (q != k) pr *= gI[q];
+                        Ha0[(j1 * nmd + k) + ncy * j2] =
+                            pr * gau10_int(GWPA(o, k, j1, m), GWPA(o, k, j2, m), x1[k], x2[k], op->codes[k], &bad);
+                    }
+                }
+            const cplx *h = H1(o, m, i);
+            for (int64_t j2 = 0; j2 < n2; j2++)
+                for (int64_t r = 0; r < nG; r++) {
+                    cplx acc = 0;
+                    for (int64_t q = 0; q < nS; q++) acc += SaSm[r + ncy * q] * h[q + nmax * j2];
+                    Ha0[r + ncy * j2] -= acc;
+                }
+            const cplx *mf = MF(o, m, i);
+            for (int64_t j2 = 0; j2 < n2; j2++)
+                for (int64_t j1 = 0; j1 < nS; j1++)
+                    for (int64_t k = 0; k < nmd; k++)
+                        Y[j1 * nmd + k] += mf[j1 + nmax * j2] * Ha0[(j1 * nmd + k) + ncy * j2];
+        }
+        /* dxi = der_factor * StdRegSol(C, Y, eps_C, 'tikh') */
+        cplx *dx = dpsi + IPSI(o, m, s) - 1;
+        std_reg_sol_tikh(nG, C, ncy, Y, o->p.eps_C, dx);
+        for (int64_t i = 0; i < nG; i++) dx[i] *= o->der_factor;
+        /* tau(j1, j2) = sum_k conj(SaSm(j2 nmd + k, j1)) dxi(j2, k);
+         * tau(j, j) += (1/2) sum_k Re(xi_jk) Re(dxi_jk) / A_jk */
+        for (int64_t j2 = 0; j2 < nS; j2++)
+            for (int64_t j1 = 0; j1 < nS; j1++) {
+                cplx acc = 0;
+                for (int64_t k = 0; k < nmd; k++) acc += conj(SaSm[(j2 * nmd + k) + ncy * j1]) * dx[j2 * nmd + k];
+                tau[j1 + nmax * j2] = acc;
+            }
+        for (int64_t j = 0; j < nS; j++) {
+            double d = 0.0;
+            const cplx *x = gxi(o, psi0, m, s, j);
+            for (int64_t k = 0; k < nmd; k++) d += creal(x[k]) * creal(dx[j * nmd + k]) / GWPA(o, k, j, m);
+            tau[j + nmax * j] += d * 0.5;
+        }
+        const cplx *D0 = DDP(o, DD0, m, s);
+        cplx *dD = DDP(o, dDD, m, s);
+        for (int64_t b = 0; b < nS; b++)
+            for (int64_t a = 0; a < nS; a++) {
+                cplx acc = 0;
+                for (int64_t q = 0; q < nS; q++) acc += tau[a + nmax * q] * D0[q + nmax * b];
+                dD[a + nmax * b] = -acc;
+            }
+    }
+#undef GX
+    free(gx); free(Sa0); free(SaSm); free(Sab); free(C); free(Y); free(Ha0); free(gI); free(tau);
+    if (bad) {
+        snprintf(o->err, sizeof o->err, "operator not supported on a Gaussian mode");
+        return -1;
+    }
+    return 0;
+}
+
+/* Calc_Err_SPF, Gaussian branch (cmf2_derivatives.f90:293-334):
+ * Tr(rho (S11 + S22 - S12 - S21)) from Gaussian overlaps. */
+static double err_spf_gwp(orc *o, int64_t m, const cplx *p1, const cplx *p2) {
+    const int64_t nmax = o->nspfmax, nmd = o->nmodeDOF[m];
+    double error = 0.0;
+    cplx *ov = xcalloc(NN(o), sizeof(cplx));
+    for (int64_t s = 0; s < NSETS(o, m); s++) {
+        const int64_t nS = NSPF(o, m, s);
+        for (int64_t j = 0; j < nS; j++)
+            ov[j + nmax * j] = 2.0 - 2.0 * creal(gau_ovl(&GWPA(o, 0, j, m), &GWPA(o, 0, j, m), gxi(o, p1, m, s, j),
+                                                         gxi(o, p2, m, s, j), nmd, 1, 1));
+        for (int64_t j2 = 0; j2 < nS; j2++)
+            for (int64_t j1 = 0; j1 < j2; j1++) {
+                const double *a1 = &GWPA(o, 0, j1, m), *a2 = &GWPA(o, 0, j2, m);
+                ov[j1 + nmax * j2] = gau_ovl(a1, a2, gxi(o, p1, m, s, j1), gxi(o, p1, m, s, j2), nmd, 1, 1) +
+                                     gau_ovl(a1, a2, gxi(o, p2, m, s, j1), gxi(o, p2, m, s, j2), nmd, 1, 1) -
+                                     gau_ovl(a1, a2, gxi(o, p1, m, s, j1), gxi(o, p2, m, s, j2), nmd, 1, 1) -
+                                     gau_ovl(a1, a2, gxi(o, p2, m, s, j1), gxi(o, p1, m, s, j2), nmd, 1, 1);
+            }
+        const cplx *r = o->MultiSet[m] ? RHO(o, m, s) : RHOAVE(o, m);
+        for (int64_t j2 = 0; j2 < nS; j2++) {
+            for (int64_t j1 = 0; j1 < j2; j1++) error += 2.0 * creal(r[j1 + nmax * j2] * conj(ov[j1 + nmax * j2]));
+            error += creal(r[j2 + nmax * j2] * ov[j2 + nmax * j2]);
+        }
+    }
+    free(ov);
+    return sqrt(fabs(error));
+}
+
 /* Calc_Err_SPF, DVR branch (cmf2_derivatives.f90:239-291). */
 static double err_spf(orc *o, int64_t m, const cplx *p1, const cplx *p2) {
+    if (o->gwp[m]) return err_spf_gwp(o, m, p1, p2);
     const int64_t nmax = o->nspfmax, nG = o->ngpm[m];
     int64_t nsets = o->MultiSet[m] ? o->ns : 1;
     double error = 0.0;
@@ -876,56 +1517,67 @@ static const double dpC[6] = {35.0 / 384.0, 0, 500.0 / 1113.0, 125.0 / 192.0,
 static const double dpD[7] = {5179.0 / 57600.0, 0, 7571.0 / 16695.0, 393.0 / 640.0,
                               -92097.0 / 339200.0, 187.0 / 2100.0, 0.025};
 
-/* rk_SPF_ck / rk_SPF_dp (cmf2_integration.f90:107-412) on the blocks of mode m. */
-static int rk_spf(orc *o, int64_t m, cplx *psi0, double htry, double *hdid, double *hnext) {
+/* Derivatives of the SPF block of mode m (and of its D matrices for a
+ * Gaussian mode). */
+static int der_any(orc *o, int64_t m, const cplx *psi, const cplx *DD, cplx *dpsi, cplx *dDD) {
+    if (o->gwp[m]) return der_spf_gwp(o, m, psi, DD, dpsi, dDD);
+    return der_spf(o, m, psi, dpsi);
+}
+
+/* out = y0 + dt * sum_q w[q] k[q] on the SPF block of mode m and, for a
+ * Gaussian mode, on its D blocks. */
+static void rk_stage(orc *o, int64_t m, const cplx *y0, const cplx *D0, double dt, const double *w, int nq,
+                     cplx *const *k, cplx *const *kD, cplx *out, cplx *outD, int first) {
+    const int64_t nmax = o->nspfmax;
+    for (int64_t s = 0; s < NSETS(o, m); s++) {
+        const int64_t i1 = IPSI(o, m, s) - 1, n = o->npar[m] * NSPF(o, m, s);
+        for (int64_t i = i1; i < i1 + n; i++) {
+            if (first) { out[i] = y0[i] + w[0] * dt * k[0][i]; continue; }
+            cplx acc = 0;
+            for (int q = 0; q < nq; q++) if (w[q] != 0.0) acc += w[q] * k[q][i];
+            out[i] = y0[i] + dt * acc;
+        }
+        if (!o->gwp[m]) continue;
+        const int64_t nS = NSPF(o, m, s);
+        const size_t off = NN(o) * (m + o->nm * s);
+        for (int64_t b = 0; b < nS; b++)
+            for (int64_t a = 0; a < nS; a++) {
+                const size_t i = off + a + nmax * b;
+                if (first) { outD[i] = D0[i] + w[0] * dt * kD[0][i]; continue; }
+                cplx acc = 0;
+                for (int q = 0; q < nq; q++) if (w[q] != 0.0) acc += w[q] * kD[q][i];
+                outD[i] = D0[i] + dt * acc;
+            }
+    }
+}
+
+/* rk_SPF_ck / rk_SPF_dp (cmf2_integration.f90:107-412) on the blocks of
+ * mode m; for a Gaussian mode the D matrices are integrated alongside. */
+static int rk_spf(orc *o, int64_t m, cplx *psi0, cplx *DD0, double htry, double *hdid, double *hnext) {
     const double safety = 0.9, pgrow = -0.2, pshrnk = -0.25, minscal = 0.1, maxscal = 5.0;
     const double errcon = pow(5.0 / safety, 1.0 / pgrow);
     const int dp = (o->p.int_type_psi == 2);
-    const int64_t D = o->dimpsi, nG = o->ngpm[m];
-    int64_t nsets = o->MultiSet[m] ? o->ns : 1;
-    cplx *k[7], *tmp = xcalloc(D, sizeof(cplx)), *perr = xcalloc(D, sizeof(cplx));
-    for (int i = 0; i < 7; i++) k[i] = xcalloc(D, sizeof(cplx));
+    const int64_t D = o->dimpsi;
+    const size_t nD = NN(o) * o->nm * o->ns;
+    cplx *k[7], *kD[7], *tmp = xcalloc(D, sizeof(cplx)), *perr = xcalloc(D, sizeof(cplx));
+    cplx *tmpD = xcalloc(nD, sizeof(cplx)), *errD = xcalloc(nD, sizeof(cplx));
+    for (int i = 0; i < 7; i++) { k[i] = xcalloc(D, sizeof(cplx)); kD[i] = xcalloc(nD, sizeof(cplx)); }
+    memcpy(tmpD, DD0, sizeof(cplx) * nD);
     double dt = htry, error = 0.0;
     int rc = 0;
     for (;;) {
-        rc |= der_spf(o, m, psi0, k[0]);
+        rc |= der_any(o, m, psi0, DD0, k[0], kD[0]);
         for (int st = 1; st < 6; st++) {
             const double *b = dp ? dpB[st] : ckB[st];
-            for (int64_t s = 0; s < nsets; s++) {
-                int64_t i1 = IPSI(o, m, s) - 1, n = nG * NSPF(o, m, s);
-                for (int64_t i = i1; i < i1 + n; i++) {
-                    if (st == 1) { tmp[i] = psi0[i] + b[0] * dt * k[0][i]; continue; }
-                    cplx acc = 0;
-                    for (int q = 0; q < st; q++) acc += b[q] * k[q][i];
-                    tmp[i] = psi0[i] + dt * acc;
-                }
-            }
-            rc |= der_spf(o, m, tmp, k[st]);
+            rk_stage(o, m, psi0, DD0, dt, b, st, k, kD, tmp, tmpD, st == 1);
+            rc |= der_any(o, m, tmp, tmpD, k[st], kD[st]);
         }
-        for (int64_t s = 0; s < nsets; s++) {
-            int64_t i1 = IPSI(o, m, s) - 1, n = nG * NSPF(o, m, s);
-            for (int64_t i = i1; i < i1 + n; i++) {
-                const double *c = dp ? dpC : ckC;
-                cplx acc = 0;
-                for (int q = 0; q < 6; q++) if (c[q] != 0.0) acc += c[q] * k[q][i];
-                tmp[i] = psi0[i] + dt * acc;
-                if (!dp) {
-                    cplx ae = 0;
-                    for (int q = 0; q < 6; q++) if (ckD[q] != 0.0) ae += ckD[q] * k[q][i];
-                    perr[i] = psi0[i] + dt * ae;
-                }
-            }
-        }
-        if (dp) {
-            rc |= der_spf(o, m, tmp, k[6]);
-            for (int64_t s = 0; s < nsets; s++) {
-                int64_t i1 = IPSI(o, m, s) - 1, n = nG * NSPF(o, m, s);
-                for (int64_t i = i1; i < i1 + n; i++) {
-                    cplx ae = 0;
-                    for (int q = 0; q < 7; q++) if (dpD[q] != 0.0) ae += dpD[q] * k[q][i];
-                    perr[i] = psi0[i] + dt * ae;
-                }
-            }
+        rk_stage(o, m, psi0, DD0, dt, dp ? dpC : ckC, 6, k, kD, tmp, tmpD, 0);
+        if (!dp) {
+            rk_stage(o, m, psi0, DD0, dt, ckD, 6, k, kD, perr, errD, 0);
+        } else {
+            rc |= der_any(o, m, tmp, tmpD, k[6], kD[6]);
+            rk_stage(o, m, psi0, DD0, dt, dpD, 7, k, kD, perr, errD, 0);
         }
         error = err_spf(o, m, tmp, perr) / o->p.eps_psi;
         if (error < 1.0) break;
@@ -935,23 +1587,26 @@ static int rk_spf(orc *o, int64_t m, cplx *psi0, double htry, double *hdid, doub
     }
     *hnext = (error > errcon) ? safety * dt * pow(error, pgrow) : dt * maxscal;
     *hdid = dt;
-    for (int64_t s = 0; s < nsets; s++) {
-        int64_t i1 = IPSI(o, m, s) - 1, n = nG * NSPF(o, m, s);
+    for (int64_t s = 0; s < NSETS(o, m); s++) {
+        int64_t i1 = IPSI(o, m, s) - 1, n = o->npar[m] * NSPF(o, m, s);
         memcpy(psi0 + i1, tmp + i1, sizeof(cplx) * n);
+        if (o->gwp[m]) {
+            const size_t off = NN(o) * (m + o->nm * s);
+            memcpy(DD0 + off, tmpD + off, sizeof(cplx) * NN(o));
+        }
     }
-    for (int i = 0; i < 7; i++) free(k[i]);
-    free(tmp);
-    free(perr);
+    for (int i = 0; i < 7; i++) { free(k[i]); free(kD[i]); }
+    free(tmp); free(perr); free(tmpD); free(errD);
     return rc;
 }
 
 /* propagate_SPF (cmf2_base.f90:210-260). */
-static int propagate_spf(orc *o, double dt, int64_t m, cplx *psi) {
+static int propagate_spf(orc *o, double dt, int64_t m, cplx *psi, cplx *DD) {
     double dt_done = 0.0, stepsize = dt, step_done, step_next;
     const double minstep = o->p.min_stepsize;
     int rc = 0;
     for (;;) {
-        rc |= rk_spf(o, m, psi, stepsize, &step_done, &step_next);
+        rc |= rk_spf(o, m, psi, DD, stepsize, &step_done, &step_next);
         dt_done = dt_done + step_done;
         if (dt_done >= dt || rc) break;
         if (step_next < minstep * 2) step_next = minstep * 2;
@@ -962,7 +1617,7 @@ static int propagate_spf(orc *o, double dt, int64_t m, cplx *psi) {
 }
 
 int orc_propagate_spf(orc *o, double dt, int64_t i, double *psi) {
-    return propagate_spf(o, dt, i - 1, (cplx *)psi);
+    return propagate_spf(o, dt, i - 1, (cplx *)psi, o->DD);
 }
 
 /* norm_wf and WF_Ovl, DVR modes (gmctdh_objects.f90:21-97, 101-180). */
@@ -977,14 +1632,23 @@ static double norm_wf(orc *o, const cplx *A0, const cplx *psi0) {
     for (int64_t s = 0; s < o->ns; s++) {
         for (int64_t m = 0; m < nm; m++) sz[m] = NSPF(o, m, s);
         memcpy(p1, A0 + o->AStateOffset[s], sizeof(cplx) * o->nAConf[s]);
+        /* grid modes first, then Gaussian modes (gmctdh_objects.f90:56-90) */
+        for (int64_t pass = 0; pass < 2; pass++)
         for (int64_t m = 0; m < nm; m++) {
-            int64_t nS = sz[m], nG = o->ngpm[m];
+            if (o->gwp[m] != pass) continue;
+            int64_t nS = sz[m], nG = o->ngpm[m], nmd = o->nmodeDOF[m];
             const cplx *phi = psi0 + IPSI(o, m, s) - 1;
             for (int64_t b = 0; b < nS; b++)
                 for (int64_t a = 0; a <= b; a++) {
                     cplx x = 0;
-                    for (int64_t g = 0; g < nG; g++) x += conj(phi[g + nG * a]) * phi[g + nG * b];
-                    if (a == b) x = creal(x);
+                    if (o->gwp[m]) {
+                        x = (a == b) ? 1.0
+                                     : gau_ovl(&GWPA(o, 0, a, m), &GWPA(o, 0, b, m), phi + nmd * a, phi + nmd * b,
+                                               nmd, 1, 1);
+                    } else {
+                        for (int64_t g = 0; g < nG; g++) x += conj(phi[g + nG * a]) * phi[g + nG * b];
+                        if (a == b) x = creal(x);
+                    }
                     S[a + nmax * b] = x;
                     S[b + nmax * a] = conj(x);
                 }
@@ -1011,13 +1675,19 @@ static cplx wf_ovl(orc *o, const cplx *A1, const cplx *psi1, const cplx *A2, con
     for (int64_t s = 0; s < o->ns; s++) {
         for (int64_t m = 0; m < nm; m++) sz[m] = NSPF(o, m, s);
         memcpy(p1, A2 + o->AStateOffset[s], sizeof(cplx) * o->nAConf[s]);
+        for (int64_t pass = 0; pass < 2; pass++)
         for (int64_t m = 0; m < nm; m++) {
-            int64_t nS = sz[m], nG = o->ngpm[m];
+            if (o->gwp[m] != pass) continue;
+            int64_t nS = sz[m], nG = o->ngpm[m], nmd = o->nmodeDOF[m];
             const cplx *f1 = psi1 + IPSI(o, m, s) - 1, *f2 = psi2 + IPSI(o, m, s) - 1;
             for (int64_t b = 0; b < nS; b++)
                 for (int64_t a = 0; a < nS; a++) {
                     cplx x = 0;
-                    for (int64_t g = 0; g < nG; g++) x += conj(f1[g + nG * a]) * f2[g + nG * b];
+                    if (o->gwp[m]) {
+                        x = gau_ovl(&GWPA(o, 0, a, m), &GWPA(o, 0, b, m), f1 + nmd * a, f2 + nmd * b, nmd, 1, 1);
+                    } else {
+                        for (int64_t g = 0; g < nG; g++) x += conj(f1[g + nG * a]) * f2[g + nG * b];
+                    }
                     S[a + nmax * b] = x;
                 }
             mode_product(nm, sz, m, S, nmax, nS, p1, p2);
@@ -1117,34 +1787,55 @@ int orc_lanczos(orc *o, const double *B0, double *BT, double dt, int build) {
     return lanczos(o, (const cplx *)B0, (cplx *)BT, dt, build);
 }
 
-/* cmf2 for DVR-only wavefunctions, where A_to_B / B_to_A are copies
- * (cmf2_base.f90:47-206). */
+/* cmf2 (cmf2_base.f90:47-206).  With Gaussian modes the coefficients are
+ * propagated as B = D^H S A in the basis orthonormalised by D (D0 from the
+ * Cholesky factor of S00 at the start of the step), the SPF propagation
+ * carries D along, and A = D B at the half step and at the end; without
+ * them A_to_B / B_to_A are copies. */
 static int cmf2(orc *o, double tt, double htry, double *hdid_out, double *hnext_out,
                 mg_attempt *atts, int64_t max_att, int64_t *n_att) {
     const int64_t N = o->nconf, D = o->dimpsi;
+    const size_t nD = NN(o) * o->nm * o->ns;
     const double safe = 0.95, maxscal = 1.5;
     cplx *B0 = xcalloc(N, sizeof(cplx)), *BH = xcalloc(N, sizeof(cplx)), *B02 = xcalloc(N, sizeof(cplx));
+    cplx *AH = o->ngwp ? xcalloc(N, sizeof(cplx)) : BH;
     cplx *pH = xcalloc(D, sizeof(cplx)), *pH2 = xcalloc(D, sizeof(cplx));
+    cplx *DH = xcalloc(nD, sizeof(cplx)), *DH2 = xcalloc(nD, sizeof(cplx));
     double hdid = htry;
     int rc = 0;
+    if (o->ngwp) rc |= init_dd(o, o->psi, o->DD);
     for (;;) {
+        if (rc) break;
+        for (int64_t m = 0; m < o->nm; m++) if (o->gwp[m]) calc_s00(o, o->psi, m);
         calc_rho(o, o->A);
-        for (int64_t m = 0; m < o->nm; m++) calc_h1mode_mode(o, o->psi, m);
+        for (int64_t m = 0; m < o->nm; m++) calc_h1mode_any(o, o->psi, m);
         calc_meanfield(o, o->A);
-        memcpy(B0, o->A, sizeof(cplx) * N);
+        if (o->ngwp) calc_dhd(o, o->DD);
+        a_to_b(o, o->A, B0, o->DD);
         int nk1 = lanczos(o, B0, BH, 0.5 * hdid, 1);
         if (nk1 < 0) { rc = -1; break; }
         memcpy(pH, o->psi, sizeof(cplx) * D);
-        for (int64_t m = 0; m < o->nm; m++) rc |= propagate_spf(o, 0.5 * hdid, m, pH);
-        /* AHalf = BHalf */
-        calc_rho(o, BH);
-        for (int64_t m = 0; m < o->nm; m++) calc_h1mode_mode(o, pH, m);
-        calc_meanfield(o, BH);
+        memcpy(DH, o->DD, sizeof(cplx) * nD);
+        for (int64_t pass = 0; pass < 2; pass++)
+            for (int64_t m = 0; m < o->nm; m++)
+                if (o->gwp[m] == pass) rc |= propagate_spf(o, 0.5 * hdid, m, pH, DH);
+        b_to_a(o, BH, AH, DH);
+        for (int64_t m = 0; m < o->nm; m++) if (o->gwp[m]) calc_s00(o, pH, m);
+        calc_rho(o, AH);
+        for (int64_t m = 0; m < o->nm; m++) calc_h1mode_any(o, pH, m);
+        calc_meanfield(o, AH);
         memcpy(pH2, o->psi, sizeof(cplx) * D);
-        for (int64_t m = 0; m < o->nm; m++) rc |= propagate_spf(o, 0.5 * hdid, m, pH2);
-        double err_psi = delta_psi(o, BH, pH, BH, pH2);
-        for (int64_t m = 0; m < o->nm; m++) rc |= propagate_spf(o, 0.5 * hdid, m, pH2);
-        for (int64_t m = 0; m < o->nm; m++) calc_h1mode_mode(o, pH2, m);
+        memcpy(DH2, o->DD, sizeof(cplx) * nD);
+        for (int64_t pass = 0; pass < 2; pass++)
+            for (int64_t m = 0; m < o->nm; m++)
+                if (o->gwp[m] == pass) rc |= propagate_spf(o, 0.5 * hdid, m, pH2, DH2);
+        double err_psi = delta_psi(o, AH, pH, AH, pH2);
+        for (int64_t pass = 0; pass < 2; pass++)
+            for (int64_t m = 0; m < o->nm; m++)
+                if (o->gwp[m] == pass) rc |= propagate_spf(o, 0.5 * hdid, m, pH2, DH2);
+        for (int64_t m = 0; m < o->nm; m++) if (o->gwp[m]) calc_s00(o, pH2, m);
+        for (int64_t m = 0; m < o->nm; m++) calc_h1mode_any(o, pH2, m);
+        if (o->ngwp) calc_dhd(o, DH2);
         int nk2 = lanczos(o, BH, B02, -0.5 * hdid, 1);
         if (nk2 < 0) { rc = -1; break; }
         for (int64_t i = 0; i < N; i++) B02[i] -= B0[i];
@@ -1169,8 +1860,9 @@ static int cmf2(orc *o, double tt, double htry, double *hdid_out, double *hnext_
             continue;
         }
         lanczos(o, BH, B0, 0.5 * hdid, 0);
-        memcpy(o->A, B0, sizeof(cplx) * N);
+        b_to_a(o, B0, o->A, DH2);
         memcpy(o->psi, pH2, sizeof(cplx) * D);
+        memcpy(o->DD, DH2, sizeof(cplx) * nD);
         double hn = hdid / sqrt(error), h2 = hdid * maxscal;
         hn = hn < h2 ? hn : h2;
         hn = hn < o->p.writetime ? hn : o->p.writetime;
@@ -1178,7 +1870,8 @@ static int cmf2(orc *o, double tt, double htry, double *hdid_out, double *hnext_
         break;
     }
     *hdid_out = hdid;
-    free(B0); free(BH); free(B02); free(pH); free(pH2);
+    free(B0); free(BH); free(B02); free(pH); free(pH2); free(DH); free(DH2);
+    if (o->ngwp) free(AH);
     return rc;
 }
 
@@ -1189,14 +1882,24 @@ int orc_cmf2_step(orc *o, double t, double htry, double *hdid, double *hnext,
 
 /* WriteOutput observables (output.f90:33-87, 143-150). */
 int orc_observables(orc *o, double t, mg_output *out, double *pops) {
+    for (int64_t m = 0; m < o->nm; m++) if (o->gwp[m]) calc_s00(o, o->psi, m);
     calc_rho(o, o->A);
-    for (int64_t m = 0; m < o->nm; m++) calc_h1mode_mode(o, o->psi, m);
+    for (int64_t m = 0; m < o->nm; m++) calc_h1mode_any(o, o->psi, m);
     calc_meanfield(o, o->A);
     double tot = 0.0;
+    const int64_t nmax = o->nspfmax;
     for (int64_t s = 0; s < o->ns; s++) {
         int64_t jel = o->MultiSet[0] ? s : 0;
         int64_t nS = NSPF(o, 0, jel);
         double pop = 0.0;
+        if (o->gwp[0]) {
+            /* Gaussian first mode: Re sum rho o S00 (output.f90:64-74) */
+            const cplx *r = RHO(o, 0, s), *S = S00P(o, 0, jel, jel);
+            for (int64_t j2 = 0; j2 < nS; j2++) {
+                for (int64_t j1 = 0; j1 < j2; j1++) pop += 2.0 * creal(r[j1 + nmax * j2] * S[j1 + nmax * j2]);
+                pop += creal(r[j2 + nmax * j2] * S[j2 + nmax * j2]);
+            }
+        } else
         for (int64_t i = 0; i < nS; i++) pop += creal(RHO(o, 0, s)[i + o->nspfmax * i]);
         if (pops) pops[s] = pop;
         tot += pop;
@@ -1245,6 +1948,10 @@ int orc_propagate(orc *o, double t0, double tfinal, mg_output *outs, double *pop
  * derivatives.f90:156, 237); dpsi from the DVR formula of every mode
  * (derivatives.f90:81-143, the same as Calc_Der_SPF's). */
 static int calc_derivs(orc *o, const cplx *A0, const cplx *psi0, cplx *dA, cplx *dpsi) {
+    if (o->ngwp) {
+        snprintf(o->err, sizeof o->err, "VMF with Gaussian modes is not supported");
+        return -1;
+    }
     calc_rho(o, A0);
     for (int64_t m = 0; m < o->nm; m++) calc_h1mode_mode(o, psi0, m);
     if (calc_meanfield(o, A0) != 0) return -1;
@@ -1408,3 +2115,53 @@ void orc_herm_eig(int64_t n, const double *A, double *w, double *U) {
 void orc_tridiag_eig(int64_t n, const double *alpha, const double *beta, double *w, double *U) {
     tridiag_eig(n, alpha, beta, w, U);
 }
+
+/* Gaussian-mode hooks for the pinning tests. */
+void orc_gau_ovl(int64_t nmd, const double *a1, const double *a2, const double *xi1, const double *xi2, double *out) {
+    cplx v = gau_ovl(a1, a2, (const cplx *)xi1, (const cplx *)xi2, nmd, 1, 1);
+    out[0] = creal(v);
+    out[1] = cimag(v);
+}
+int orc_gau_int(double A1, double A2, const double *xi1, const double *xi2, int64_t op, int64_t deriv, double *out) {
+    int bad = 0;
+    const cplx x1 = xi1[0] + I * xi1[1], x2 = xi2[0] + I * xi2[1];
+    cplx v = deriv ? gau10_int(A1, A2, x1, x2, op, &bad) : gau_int(A1, A2, x1, x2, op, &bad);
+    out[0] = creal(v);
+    out[1] = cimag(v);
+    return bad ? -1 : 0;
+}
+void orc_std_reg_sol_tikh(int64_t n, const double *A, const double *y, double eps, double *x) {
+    std_reg_sol_tikh(n, (const cplx *)A, n, (const cplx *)y, eps, (cplx *)x);
+}
+int orc_inverse_sqrt_u(int64_t n, const double *S, double *D) {
+    return inverse_sqrt_u(n, (const cplx *)S, n, (cplx *)D, n);
+}
+/* S00, then D0 = InverseSquareRootU(S00) of the current psi into dd_out
+ * ([nspfmax, nspfmax, nmodes, nstates]) and the overlaps into s00_out
+ * ([nspfmax, nspfmax, nmodes, nstates, nstates]); either may be NULL. */
+int orc_gwp_overlaps(orc *o, double *s00_out, double *dd_out) {
+    int rc = init_dd(o, o->psi, o->DD);
+    if (s00_out) memcpy(s00_out, o->S00, sizeof(cplx) * NN(o) * o->nm * o->ns * o->ns);
+    if (dd_out) memcpy(dd_out, o->DD, sizeof(cplx) * NN(o) * o->nm * o->ns);
+    return rc;
+}
+/* Calc_Der_SPF of Gaussian mode im (1-based) at psi_in with D = dd_in,
+ * using rho and the mean fields of the current state (A, psi). */
+int orc_der_spf_gwp(orc *o, int64_t im, const double *psi_in, const double *dd_in, double *dpsi_out,
+                    double *ddd_out) {
+    for (int64_t m = 0; m < o->nm; m++) if (o->gwp[m]) calc_s00(o, o->psi, m);
+    calc_rho(o, o->A);
+    for (int64_t m = 0; m < o->nm; m++) calc_h1mode_any(o, o->psi, m);
+    calc_meanfield(o, o->A);
+    return der_spf_gwp(o, im - 1, (const cplx *)psi_in, (const cplx *)dd_in, (cplx *)dpsi_out, (cplx *)ddd_out);
+}
+/* A_to_B / B_to_A with the D matrices dd ([nspfmax, nspfmax, nmodes,
+ * nstates]) and the overlaps of the current psi. */
+void orc_a_to_b(orc *o, const double *A, double *B, const double *dd) {
+    for (int64_t m = 0; m < o->nm; m++) if (o->gwp[m]) calc_s00(o, o->psi, m);
+    a_to_b(o, (const cplx *)A, (cplx *)B, (const cplx *)dd);
+}
+void orc_b_to_a(orc *o, const double *B, double *A, const double *dd) {
+    b_to_a(o, (const cplx *)B, (cplx *)A, (const cplx *)dd);
+}
+

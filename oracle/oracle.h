@@ -69,6 +69,19 @@ void orc_std_reg_inv(int64_t n, const double *A, double eps, int64_t type, doubl
 void orc_herm_eig(int64_t n, const double *A, double *w, double *U);
 void orc_tridiag_eig(int64_t n, const double *alpha, const double *beta, double *w, double *U);
 
+/* Gaussian (G-MCTDH) pieces: GauOvlMultiD, GauInt1D / Gau10Int1D (deriv = 1),
+ * StdRegSol 'tikh', InverseSquareRootU, S00 and D0 of the current psi,
+ * Calc_Der_SPF of a Gaussian mode, A_to_B / B_to_A. */
+void orc_gau_ovl(int64_t nmd, const double *a1, const double *a2, const double *xi1, const double *xi2, double *out);
+int orc_gau_int(double A1, double A2, const double *xi1, const double *xi2, int64_t op, int64_t deriv, double *out);
+void orc_std_reg_sol_tikh(int64_t n, const double *A, const double *y, double eps, double *x);
+int orc_inverse_sqrt_u(int64_t n, const double *S, double *D);
+int orc_gwp_overlaps(orc *o, double *s00_out, double *dd_out);
+int orc_der_spf_gwp(orc *o, int64_t im, const double *psi_in, const double *dd_in, double *dpsi_out,
+                    double *ddd_out);
+void orc_a_to_b(orc *o, const double *A, double *B, const double *dd);
+void orc_b_to_a(orc *o, const double *B, double *A, const double *dd);
+
 #ifdef __cplusplus
 }
 #endif

@@ -61,6 +61,18 @@ def lib():
         L.orc_propagate_vmf.argtypes = [C.c_void_p, C.c_int64, C.c_double, C.c_double, C.POINTER(MgOutput),
                                         f64p, C.c_int64, i64p, C.POINTER(MgAttempt), C.c_int64, i64p]
         L.orc_rk_scalar_step.argtypes = [C.c_int64, C.c_double, C.c_double, C.c_double, f64p, f64p]
+        L.orc_gau_ovl.argtypes = [C.c_int64, f64p, f64p, f64p, f64p, f64p]
+        L.orc_gau_int.restype = C.c_int
+        L.orc_gau_int.argtypes = [C.c_double, C.c_double, f64p, f64p, C.c_int64, C.c_int64, f64p]
+        L.orc_std_reg_sol_tikh.argtypes = [C.c_int64, f64p, f64p, C.c_double, f64p]
+        L.orc_inverse_sqrt_u.restype = C.c_int
+        L.orc_inverse_sqrt_u.argtypes = [C.c_int64, f64p, f64p]
+        L.orc_gwp_overlaps.restype = C.c_int
+        L.orc_gwp_overlaps.argtypes = [C.c_void_p, f64p, f64p]
+        L.orc_der_spf_gwp.restype = C.c_int
+        L.orc_der_spf_gwp.argtypes = [C.c_void_p, C.c_int64, f64p, f64p, f64p, f64p]
+        L.orc_a_to_b.argtypes = [C.c_void_p, f64p, f64p, f64p]
+        L.orc_b_to_a.argtypes = [C.c_void_p, f64p, f64p, f64p]
         _lib = L
     return _lib
 
@@ -207,8 +219,82 @@ class Oracle:
         return list(outs[: no.value]), pops[: no.value], list(atts[: na.value])
 
 
+    # ---- Gaussian (G-MCTDH) hooks ----
+    def gwp_overlaps(self):
+        """S00 [s2, s1, mode, j2, j1] and D0 [state, mode, j2, j1] of the current psi."""
+        ns, nm, n = self.prob.nstates, self.prob.nmodes, self.nspfmax
+        S = np.zeros((ns, ns, nm, n, n), np.complex128)
+        D = np.zeros((ns, nm, n, n), np.complex128)
+        rc = lib().orc_gwp_overlaps(self.h, cplx_ptr(S), cplx_ptr(D))
+        if rc != 0:
+            raise RuntimeError(lib().orc_error(self.h).decode())
+        return S, D
+
+    def der_spf_gwp(self, im, psi, D):
+        psi = np.ascontiguousarray(psi, np.complex128)
+        D = np.ascontiguousarray(D, np.complex128)
+        dpsi = np.zeros_like(psi)
+        dD = np.zeros_like(D)
+        rc = lib().orc_der_spf_gwp(self.h, im, cplx_ptr(psi), cplx_ptr(D), cplx_ptr(dpsi), cplx_ptr(dD))
+        if rc != 0:
+            raise RuntimeError(lib().orc_error(self.h).decode())
+        return dpsi, dD
+
+    def a_to_b(self, A, D):
+        A = np.ascontiguousarray(A, np.complex128)
+        B = np.zeros_like(A)
+        lib().orc_a_to_b(self.h, cplx_ptr(A), cplx_ptr(B), cplx_ptr(np.ascontiguousarray(D, np.complex128)))
+        return B
+
+    def b_to_a(self, B, D):
+        B = np.ascontiguousarray(B, np.complex128)
+        A = np.zeros_like(B)
+        lib().orc_b_to_a(self.h, cplx_ptr(B), cplx_ptr(A), cplx_ptr(np.ascontiguousarray(D, np.complex128)))
+        return A
+
+
 def cptr_i(a):
     return a.ctypes.data_as(i64p)
+
+
+def _c2(z):
+    return np.array([complex(z).real, complex(z).imag])
+
+
+def gau_ovl(a1, a2, xi1, xi2):
+    a1 = np.ascontiguousarray(a1, np.float64)
+    a2 = np.ascontiguousarray(a2, np.float64)
+    x1 = np.ascontiguousarray(np.asarray(xi1, np.complex128))
+    x2 = np.ascontiguousarray(np.asarray(xi2, np.complex128))
+    out = np.zeros(2)
+    lib().orc_gau_ovl(len(a1), a1.ctypes.data_as(f64p), a2.ctypes.data_as(f64p), cplx_ptr(x1), cplx_ptr(x2),
+                      out.ctypes.data_as(f64p))
+    return complex(out[0], out[1])
+
+
+def gau_int(A1, A2, xi1, xi2, op, deriv=False):
+    x1, x2, out = _c2(xi1), _c2(xi2), np.zeros(2)
+    rc = lib().orc_gau_int(A1, A2, x1.ctypes.data_as(f64p), x2.ctypes.data_as(f64p), op, 1 if deriv else 0,
+                           out.ctypes.data_as(f64p))
+    if rc != 0:
+        raise ValueError("unsupported operator")
+    return complex(out[0], out[1])
+
+
+def std_reg_sol_tikh(A, y, eps):
+    A = np.ascontiguousarray(np.asarray(A, np.complex128).T)
+    y = np.ascontiguousarray(y, np.complex128)
+    x = np.zeros_like(y)
+    lib().orc_std_reg_sol_tikh(len(y), cplx_ptr(A), cplx_ptr(y), eps, cplx_ptr(x))
+    return x
+
+
+def inverse_sqrt_u(S):
+    S = np.ascontiguousarray(np.asarray(S, np.complex128).T)
+    D = np.zeros_like(S)
+    if lib().orc_inverse_sqrt_u(S.shape[0], cplx_ptr(S), cplx_ptr(D)) != 0:
+        raise ValueError("not positive definite")
+    return D.T
 
 
 def std_reg_inv(A, eps, kind):

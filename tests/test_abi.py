@@ -43,8 +43,9 @@ def test_struct_layout_matches_header():
     from macaw_amd.abi import MgAttempt, MgOutput, MgProblem
     assert ctypes.sizeof(MgOutput) == 5 * 8
     assert ctypes.sizeof(MgAttempt) == 7 * 8
-    # 4 int64, 5 pointers, ngpmax, 4 pointers, nham, 4 pointers, 5 doubles, int64
-    assert ctypes.sizeof(MgProblem) == 8 * (4 + 5 + 1 + 4 + 1 + 4 + 5 + 1)
+    # 4 int64, 5 pointers, ngpmax, 4 pointers, nham, 4 pointers, 5 doubles, int64,
+    # then the Gaussian-mode fields: 2 pointers, 2 doubles
+    assert ctypes.sizeof(MgProblem) == 8 * (4 + 5 + 1 + 4 + 1 + 4 + 5 + 1 + 2 + 2)
 
 
 def test_input_deck_roundtrip(tmp_path):
