@@ -170,6 +170,23 @@ int mg_stepsize(const mg_ctx *ctx, double *h);
  * profiling (bytes moved = 2 n element size).  No reference counterpart. */
 int mg_profile_copy(mg_ctx *ctx, int64_t n, double *kernel_ms);
 
+/* Gaussian (G-MCTDH) modes.  mg_prepare_gwp: S00, h1mode, D0 =
+ * InverseSquareRootU(S00) (maths.f90:621-644) and D^H h D (Calc_DHD,
+ * hamilton.f90:535-574) of the current state, after which mg_apply_h is
+ * Calc_DHD_B in the orthonormalised basis (cmf2_derivatives.f90:1-80).
+ * mg_gwp_overlaps: Calc_S00 (gmctdh_objects.f90:323-381) as
+ * S00[nspfmax, nspfmax, nmodes, nstates, nstates] and D0[nspfmax, nspfmax,
+ * nmodes, nstates] of the current psi.  mg_der_spf_gwp: Calc_Der_SPF of
+ * Gaussian mode im (1-based) at (psi_in, D = dd_in) with rho and the mean
+ * fields of the current state (cmf2_derivatives.f90:175-232); dpsi_out gets
+ * that mode's blocks, ddd_out dD.  mg_a_to_b: A_to_B (cmf2_base.f90:264-305),
+ * B = D^H S A with the overlaps of the current psi. */
+int mg_prepare_gwp(mg_ctx *ctx);
+int mg_gwp_overlaps(mg_ctx *ctx, double *s00_out, double *dd_out);
+int mg_der_spf_gwp(mg_ctx *ctx, int64_t im, const double *psi_in, const double *dd_in, double *dpsi_out,
+                   double *ddd_out);
+int mg_a_to_b(mg_ctx *ctx, const double *A, const double *dd, double *B);
+
 #ifdef __cplusplus
 }
 #endif

@@ -127,6 +127,7 @@ struct mg_ctx {
     std::vector<int> hs_rowptr, hs_src;     // diagonal sums per (s, m)
     std::vector<cplx> hs_coef;
     std::vector<Chain> chains;              // H.A chain terms
+    std::vector<Chain> chains_g;            // with Gaussian modes: every term, S00 for identities on them
     std::vector<int> mf1_rowptr, mf1_src;   // one-mode same-state MF = ae * rho
     std::vector<cplx> mf1_coef;
     std::vector<int> mf_chain_terms;        // terms needing AL/AR chains
@@ -174,6 +175,23 @@ struct mg_ctx {
     std::vector<double> lan_alpha, lan_U, lan_n;   // Ritz values, vectors, Krylov norms
     const double2 *lan_v0 = nullptr;               // u_0 of the last built space (FP64)
     int nkrylov = 0;
+    // ---- Gaussian (G-MCTDH) modes ----
+    int ngwp = 0;
+    std::vector<int> gwp;                  // [nm] 1 = Gaussian mode
+    std::vector<int64_t> npar;             // [nm] SPF parameters per SPF (grid points or nmodeDOF)
+    std::vector<char> mfskip;              // [m + nm * i] mean-field chains skip this operator
+    std::vector<int> dvr_desc;             // [nm] index into dspfmodes, -1 for a Gaussian mode
+    int ndvr_desc = 0, ngwp_desc = 0;
+    unsigned gwp_mask = 0;                 // bit m set for a Gaussian mode
+    double eps_C = 1e-8, eps_ovl = 1e-8;
+    std::vector<double> gwpa;
+    // S00 [nn, nm, ns, ns]; intra-state S00 per state [nn, nm, ns]; D matrices
+    // of the state and of the two half-step SPF sets and of the observables;
+    // the H.A operator table (h1mode, D^H h D on Gaussian modes) and its FP32
+    // copy; A_to_B / B_to_A matrices; the A vector at the half step
+    DevBuf dgwpdesc, dgwpa, dS00, dSd, dDD, dDH, dDH2, dDobs, dhA, dhAf, dgwpscr, dABmats, dAH;
+    // VMF with Gaussian modes: identity D, Tikhonov S00^-1 and tau per (mode, state), -tau per set
+    DevBuf dDid, dSm1, dTauS, dTauSet;
 
     int nsets(int m) const { return MultiSet[m] ? ns : 1; }
     int64_t NS(int m, int s) const { return nSPF[m + (int64_t)nm * s]; }
@@ -186,6 +204,13 @@ struct mg_ctx {
     double2 *rho_ave(int m) const { return drho.as<double2>() + (size_t)nspfmax * nspfmax * (nm * ns + m); }
     double2 *mf(int m, int op) const { return dmf.as<double2>() + (size_t)nspfmax * nspfmax * (op + (size_t)nopmax * m); }
     double2 *S(int m, int s) const { return dS.as<double2>() + (size_t)nspfmax * nspfmax * (m + (size_t)nm * s); }
+    // H.A operator matrices: h1mode, or D^H h D on Gaussian modes
+    const void *haop(int m, int op) const {
+        if (!ngwp) return h1op(m, op);
+        const size_t o = (size_t)nspfmax * nspfmax * (op + (size_t)nopmax * m);
+        if (!prec) return dhA.as<double2>() + o;
+        return dhAf.as<float2>() + o;
+    }
     // operator matrices in the work-tensor storage type
     const void *h1op(int m, int op) const {
         if (!prec) return h1(m, op);
@@ -302,13 +327,39 @@ void setup(mg_ctx *c, const mg_problem *p) {
     c->ngpm.assign(nm, 1);
     for (int m = 0; m < nm; m++)
         for (int k = 0; k < c->nmodeDOF[m]; k++) c->ngpm[m] *= c->ngp[c->iDOF[k + (size_t)c->maxdof * m] - 1];
+    // Gaussian modes (init_calc.f90:56-75): the SPF block holds nmodeDOF xi per Gaussian
+    c->gwp.assign(nm, 0);
+    c->npar.assign(nm, 0);
+    c->ngwp = 0;
+    c->gwp_mask = 0;
+    for (int m = 0; m < nm; m++) {
+        if (p->dof_gwp) {
+            int kinds = 0;
+            for (int k = 0; k < c->nmodeDOF[m]; k++) kinds += p->dof_gwp[c->iDOF[k + (size_t)c->maxdof * m] - 1] ? 1 : 0;
+            if (kinds != 0 && kinds != c->nmodeDOF[m])
+                throw HipError{"A mode cannot have simultaneously dvr and gwp representation"};
+            c->gwp[m] = kinds ? 1 : 0;
+        }
+        c->npar[m] = c->gwp[m] ? c->nmodeDOF[m] : c->ngpm[m];
+        if (c->gwp[m]) {
+            c->ngwp++;
+            if (m >= 32) throw HipError{"Gaussian modes must be among the first 32 modes"};
+            c->gwp_mask |= 1u << m;
+        }
+    }
+    c->eps_C = p->eps_C > 0 ? p->eps_C : 1e-8;
+    c->eps_ovl = p->eps_ovl > 0 ? p->eps_ovl : 1e-8;
+    if (c->ngwp) {
+        if (!p->gwp_a) throw HipError{"Gaussian modes need gwp_a"};
+        c->gwpa.assign(p->gwp_a, p->gwp_a + (size_t)c->maxdof * c->nspfmax * nm);
+    }
     c->ipsi.assign((size_t)nm * ns, 0);
     int64_t dim = 0;
     for (int s = 0; s < ns; s++)
         for (int m = 0; m < nm; m++) {
             if (!c->MultiSet[m] && s > 0) { c->ipsi[m + (size_t)nm * s] = c->ipsi[m]; continue; }
             c->ipsi[m + (size_t)nm * s] = dim;
-            dim += c->ngpm[m] * c->NS(m, s);
+            dim += c->npar[m] * c->NS(m, s);
         }
     c->dimpsi = dim;
     // operator basis
@@ -344,6 +395,23 @@ void setup(mg_ctx *c, const mg_problem *p) {
             bool id = v.lhs == v.rhs;
             for (int k = 0; k < c->nmodeDOF[m] && id; k++) id = v.codes[k] == 0;
             c->isIdent[m + (size_t)nm * i] = id;
+        }
+    // the mean-field chains skip identities on grid modes only (SkipTerm,
+    // hamilton.f90:386-393); on a Gaussian mode the identity is S00
+    c->mfskip.assign((size_t)nm * c->nopmax, 0);
+    for (int m = 0; m < nm; m++)
+        for (int i = 0; i < c->nOper[m]; i++) c->mfskip[m + (size_t)nm * i] = c->isIdent[m + (size_t)nm * i] && !c->gwp[m];
+    // H in the Gaussian basis (VMF derivatives): every term as a chain, the
+    // identity on a Gaussian mode being S00
+    c->chains_g.clear();
+    if (c->ngwp)
+        for (int t = 0; t < c->nham; t++) {
+            std::vector<ChainOp> ops;
+            for (int m = 0; m < nm; m++) {
+                const int op = c->iOper[m + (size_t)nm * t];
+                if (!c->mfskip[m + (size_t)nm * op]) ops.push_back({m, op});
+            }
+            c->chains_g.push_back({(int)c->lhs[t] - 1, (int)c->rhs[t] - 1, c->ae[t], ops});
         }
     // term classification
     std::vector<std::vector<std::pair<int, cplx>>> hs((size_t)ns * nm), mf1((size_t)nm * c->nopmax);
@@ -411,6 +479,7 @@ void setup(mg_ctx *c, const mg_problem *p) {
     c->rmf_jobs.clear();
     c->n_rmf_jobs = 0;
     for (int m = 0; m < nm; m++) {
+        if (c->gwp[m]) continue;      // Calc_rhom1MF runs over the grid modes
         for (int s = 0; s < c->nsets(m); s++) {
             std::vector<int> job(36, 0);
             job[0] = m;
@@ -442,7 +511,7 @@ void setup(mg_ctx *c, const mg_problem *p) {
     make_groups(c);
     // mean fields through brick programs when every chain op is square and
     // the SPF counts fit one 8-wide Gram slot
-    c->mf_bricks = c->nspfmax <= 8 && std::getenv("MG_MF_LEGACY") == nullptr;
+    c->mf_bricks = c->nspfmax <= 8 && std::getenv("MG_MF_LEGACY") == nullptr && c->ngwp == 0;
     for (int t : c->mf_chain_terms)
         for (int m = 0; m < nm; m++)
             if (c->NS(m, (int)c->lhs[t] - 1) != c->NS(m, (int)c->rhs[t] - 1)) c->mf_bricks = false;
@@ -500,7 +569,7 @@ void allocate(mg_ctx *c) {
     c->dS2.alloc(z * nn * c->nm * c->ns);
     c->dS3.alloc(z * nn * c->nm * c->ns);
     c->dGtmp.alloc(z * nn);
-    c->dchain.alloc(c->es * 2 * std::max<size_t>(1, c->chains.size()) * (size_t)c->mixed);
+    c->dchain.alloc(c->es * 2 * std::max<size_t>(1, std::max(c->chains.size(), c->chains_g.size())) * (size_t)c->mixed);
     alloc_mf_tensors(c);
     if (c->mf_bricks) {
         c->dgslots.alloc(z * nn * c->nm * std::max<size_t>(1, mf_nterms(c)));
@@ -565,11 +634,15 @@ void allocate(mg_ctx *c) {
     upload_i(c->dngp, ngp);
     upload_i(c->dmode_dofs, mdofs);
     upload_i(c->dmode_ndof, mnd);
-    // SPF propagation descriptors and scratch
-    std::vector<SpfModeDesc> md(c->nm);
+    // SPF propagation descriptors and scratch (grid modes)
+    std::vector<SpfModeDesc> md;
+    c->dvr_desc.assign(c->nm, -1);
     size_t scr = 0;
     for (int m = 0; m < c->nm; m++) {
-        SpfModeDesc &d = md[m];
+        if (c->gwp[m]) continue;
+        c->dvr_desc[m] = (int)md.size();
+        md.emplace_back();
+        SpfModeDesc &d = md.back();
         std::memset(&d, 0, sizeof d);
         d.mode = m;
         d.nG = (int)c->ngpm[m];
@@ -597,13 +670,83 @@ void allocate(mg_ctx *c) {
         d.scr_off = (int)scr;
         scr += (size_t)10 * blk + (size_t)2 * d.nG * c->nspfmax + 2 * nn;
     }
-    c->dspfmodes.alloc(sizeof(SpfModeDesc) * md.size());
-    HIP_CHECK(hipMemcpy(c->dspfmodes.p, md.data(), c->dspfmodes.bytes, hipMemcpyHostToDevice));
+    c->ndvr_desc = (int)md.size();
+    c->dspfmodes.alloc(sizeof(SpfModeDesc) * std::max<size_t>(1, md.size()));
+    if (!md.empty()) HIP_CHECK(hipMemcpy(c->dspfmodes.p, md.data(), sizeof(SpfModeDesc) * md.size(), hipMemcpyHostToDevice));
     c->dspfscr.alloc(z * scr);
     c->dstatus.alloc(sizeof(double2) * 2);
+    // Gaussian modes: descriptors, widths, overlaps, D matrices, scratch
+    if (c->ngwp) {
+        std::vector<GwpDesc> gd;
+        size_t gscr = 0;
+        for (int m = 0; m < c->nm; m++) {
+            if (!c->gwp[m]) continue;
+            gd.emplace_back();
+            GwpDesc &d = gd.back();
+            std::memset(&d, 0, sizeof d);
+            d.mode = m;
+            d.nmd = (int)c->nmodeDOF[m];
+            d.nsets = c->nsets(m);
+            if (d.nsets > kMaxSets) throw HipError{"too many electronic states for a Gaussian mode"};
+            int blk = 0;
+            for (int st = 0; st < d.nsets; st++) {
+                d.nspf[st] = (int)c->NS(m, st);
+                d.off[st] = (int)c->IP(m, st);
+                d.rho_index[st] = c->MultiSet[m] ? (m + c->nm * st) : (c->nm * c->ns + m);
+                if (d.nspf[st] * d.nmd > kSmall) throw HipError{"Gaussians x DOFs of a mode exceed 16"};
+                blk += d.nmd * d.nspf[st];
+            }
+            d.blk = blk;
+            if (c->nOper[m] > kMaxGwpOps) throw HipError{"too many operators for a Gaussian mode"};
+            d.nop = c->nOper[m];
+            for (int i = 0; i < c->nOper[m]; i++) {
+                const Oper &v = c->opers[(size_t)m * c->nopmax + i];
+                d.op_lhs[i] = c->MultiSet[m] ? v.lhs - 1 : 0;
+                d.op_rhs[i] = c->MultiSet[m] ? v.rhs - 1 : 0;
+                d.op_ident[i] = c->isIdent[m + (size_t)c->nm * i] ? 1 : 0;
+                for (int k = 0; k < kMaxDofPerMode; k++) d.op_codes[i][k] = v.codes[k];
+                d.op_index[i] = i + c->nopmax * m;
+            }
+            d.scr_off = (int)gscr;
+            gscr += gwp_scratch_elems(d, c->nspfmax);
+        }
+        c->ngwp_desc = (int)gd.size();
+        c->dgwpdesc.alloc(sizeof(GwpDesc) * gd.size());
+        HIP_CHECK(hipMemcpy(c->dgwpdesc.p, gd.data(), sizeof(GwpDesc) * gd.size(), hipMemcpyHostToDevice));
+        c->dgwpscr.alloc(z * gscr);
+        c->dgwpa.alloc(sizeof(double) * c->gwpa.size());
+        HIP_CHECK(hipMemcpy(c->dgwpa.p, c->gwpa.data(), sizeof(double) * c->gwpa.size(), hipMemcpyHostToDevice));
+        const size_t nms = nn * c->nm * c->ns;
+        c->dS00.alloc(z * nms * c->ns);
+        c->dSd.alloc(z * nms);
+        c->dDD.alloc(z * nms);
+        c->dDH.alloc(z * nms);
+        c->dDH2.alloc(z * nms);
+        c->dDobs.alloc(z * nms);
+        c->dABmats.alloc(z * 2 * nms);
+        HIP_CHECK(hipMemset(c->dS00.p, 0, c->dS00.bytes));
+        HIP_CHECK(hipMemset(c->dSd.p, 0, c->dSd.bytes));
+        HIP_CHECK(hipMemset(c->dDD.p, 0, c->dDD.bytes));
+        c->dSm1.alloc(z * nms);
+        c->dTauS.alloc(z * nms);
+        c->dTauSet.alloc(z * nms);
+        {
+            std::vector<double2> I(nms, make_double2(0, 0));
+            for (int m = 0; m < c->nm; m++)
+                for (int st = 0; st < c->ns; st++)
+                    for (int j = 0; j < c->nspfmax; j++)
+                        I[nn * (m + (size_t)c->nm * st) + j + (size_t)c->nspfmax * j] = make_double2(1, 0);
+            c->dDid.alloc(z * nms);
+            HIP_CHECK(hipMemcpy(c->dDid.p, I.data(), z * nms, hipMemcpyHostToDevice));
+        }
+        c->dhA.alloc(z * nn * c->nopmax * c->nm);
+        if (c->prec) c->dhAf.alloc(sizeof(float2) * nn * c->nopmax * c->nm);
+        c->dAH.alloc(z * (size_t)c->nconf);
+    }
     // SPF product task scratch: 2 * nG * nspfmax per task
     int64_t maxng = 0;
-    for (int m = 0; m < c->nm; m++) maxng = std::max(maxng, c->ngpm[m]);
+    for (int m = 0; m < c->nm; m++)
+        if (!c->gwp[m]) maxng = std::max(maxng, c->ngpm[m]);
     HIP_CHECK(hipEventCreate(&c->ev0));
     HIP_CHECK(hipEventCreate(&c->ev1));
     // one scratch region per possible SPF task
@@ -630,6 +773,7 @@ size_t spf_scratch_base(const mg_ctx *c) {
     size_t scr = 0;
     const size_t nn = (size_t)c->nspfmax * c->nspfmax;
     for (int m = 0; m < c->nm; m++) {
+        if (c->gwp[m]) continue;
         size_t blk = 0;
         for (int s = 0; s < c->nsets(m); s++) blk += (size_t)c->ngpm[m] * c->NS(m, s);
         scr += 10 * blk + 2 * (size_t)c->ngpm[m] * c->nspfmax + 2 * nn;
@@ -924,12 +1068,16 @@ int64_t prod_range(const std::vector<int64_t> &sz, int a, int b) {
 //          sum of B_s + every chain into s through its last-group ops (ae
 //          folded into its last matrix), written once.
 // With one group only the final kind runs, reading the chains' inputs.
-std::unique_ptr<Plan> build_apply_h(mg_ctx *c, const void *B, void *HB) {
+// gbasis: H in the Gaussian (non-orthogonal) basis -- chains_g, h1mode on
+// every Gaussian mode, no diagonal sums (VMF with Gaussian modes).
+std::unique_ptr<Plan> build_apply_h(mg_ctx *c, const void *B, void *HB, bool gbasis = false) {
     PlanBuilder pb;
+    const std::vector<Chain> &chains = gbasis ? c->chains_g : c->chains;
+    auto opm = [&](int m, int op) { return gbasis ? c->h1op(m, op) : c->haop(m, op); };
     const int G = (int)c->groups.size();
     const int nm = c->nm;
     const size_t mixed = (size_t)c->mixed;
-    const size_t K = c->chains.size();
+    const size_t K = chains.size();
     auto Tbuf = [&](size_t k, int i) { return c->at(c->dchain.p, (int64_t)(mixed * (2 * k + i))); };
     auto state_sizes = [&](int s) {
         std::vector<int64_t> sz(nm);
@@ -940,7 +1088,7 @@ std::unique_ptr<Plan> build_apply_h(mg_ctx *c, const void *B, void *HB) {
         std::vector<int> ms;
         for (int m = c->groups[g].first; m < c->groups[g].second; m++) {
             const int row = s * nm + m;
-            if (c->hs_rowptr[row + 1] != c->hs_rowptr[row]) ms.push_back(m);
+            if (!gbasis && c->hs_rowptr[row + 1] != c->hs_rowptr[row]) ms.push_back(m);
         }
         return ms;
     };
@@ -954,15 +1102,15 @@ std::unique_ptr<Plan> build_apply_h(mg_ctx *c, const void *B, void *HB) {
     std::vector<int> curT(K, -1);
     std::vector<std::vector<int64_t>> csz(K);
     for (size_t k = 0; k < K; k++) {
-        cur[k] = c->at(B, c->off[c->chains[k].rhs]);
-        csz[k] = state_sizes(c->chains[k].rhs);
+        cur[k] = c->at(B, c->off[chains[k].rhs]);
+        csz[k] = state_sizes(chains[k].rhs);
     }
     // Apply chain k's ops of group g to the data in buffer `in`; the result is
     // left in the returned buffer (in place when square, else ping-pong with
     // `spare`).  last_to: if >= 0, the last op accumulates into that buffer
     // with `scale` folded into its matrix.
     auto chain_group_ops = [&](TaskBuilder &tb, size_t k, int g, int in, int spare, int last_to, cplx scale) {
-        const Chain &ch = c->chains[k];
+        const Chain &ch = chains[k];
         std::vector<ChainOp> ops = chain_ops(ch, g);
         int curb = in;
         for (size_t i = 0; i < ops.size(); i++) {
@@ -970,12 +1118,12 @@ std::unique_ptr<Plan> build_apply_h(mg_ctx *c, const void *B, void *HB) {
             const int nout = (int)c->NS(o.mode, ch.lhs);
             const bool last = (i + 1 == ops.size());
             if (last && last_to >= 0) {
-                tb.op(curb, last_to, o.mode, c->h1op(o.mode, o.op), 0, nout, 1, scale);
+                tb.op(curb, last_to, o.mode, opm(o.mode, o.op), 0, nout, 1, scale);
                 curb = last_to;
             } else {
                 const int ni = (int)tb.shape[curb][o.mode - tb.m0];
                 const int dst = (ni == nout) ? curb : (curb == in ? spare : in);
-                tb.op(curb, dst, o.mode, c->h1op(o.mode, o.op), 0, nout, 0);
+                tb.op(curb, dst, o.mode, opm(o.mode, o.op), 0, nout, 0);
                 curb = dst;
             }
             csz[k][o.mode] = nout;
@@ -999,19 +1147,19 @@ std::unique_ptr<Plan> build_apply_h(mg_ctx *c, const void *B, void *HB) {
             }
             tb.store(1, c->at(HB, c->off[b]), 0);
             for (size_t k = 0; k < K; k++) {
-                const Chain &ch = c->chains[k];
+                const Chain &ch = chains[k];
                 if (ch.rhs != b || chain_ops(ch, 0).empty()) continue;
                 // first op reads the untouched input brick (buffer 0)
                 const std::vector<ChainOp> ops = chain_ops(ch, 0);
                 const ChainOp &o = ops[0];
-                tb.op(0, 1, o.mode, c->h1op(o.mode, o.op), 0, (int)c->NS(o.mode, ch.lhs), 0);
+                tb.op(0, 1, o.mode, opm(o.mode, o.op), 0, (int)c->NS(o.mode, ch.lhs), 0);
                 csz[k][o.mode] = c->NS(o.mode, ch.lhs);
                 int curb = 1;
                 for (size_t i = 1; i < ops.size(); i++) {
                     const int nout = (int)c->NS(ops[i].mode, ch.lhs);
                     const int ni = (int)tb.shape[curb][ops[i].mode - tb.m0];
                     const int dst = (ni == nout) ? curb : (curb == 1 ? 2 : 1);
-                    tb.op(curb, dst, ops[i].mode, c->h1op(ops[i].mode, ops[i].op), 0, nout, 0);
+                    tb.op(curb, dst, ops[i].mode, opm(ops[i].mode, ops[i].op), 0, nout, 0);
                     csz[k][ops[i].mode] = nout;
                     curb = dst;
                 }
@@ -1037,7 +1185,7 @@ std::unique_ptr<Plan> build_apply_h(mg_ctx *c, const void *B, void *HB) {
                 diag.push_back(tb);
             }
             for (size_t k = 0; k < K; k++) {
-                const Chain &ch = c->chains[k];
+                const Chain &ch = chains[k];
                 if (chain_ops(ch, g).empty()) continue;
                 TaskBuilder tb(c, g, prod_range(csz[k], 0, c->groups[g].first),
                                prod_range(csz[k], c->groups[g].second, nm));
@@ -1059,31 +1207,65 @@ std::unique_ptr<Plan> build_apply_h(mg_ctx *c, const void *B, void *HB) {
     // final pass, per output state
     {
         const int g = G - 1;
-        std::vector<TaskBuilder> fin;
+        std::vector<std::vector<std::vector<size_t>>> fin_batches(c->ns);
         for (int s = 0; s < c->ns; s++) {
             const std::vector<int64_t> sz = state_sizes(s);
             const int64_t L = prod_range(sz, 0, c->groups[g].first);
-            TaskBuilder tb(c, g, L, 1);
-            if (G >= 2) tb.load(1, c->at(HB, c->off[s]), sz);
-            else tb.zero(1, sz);
             const std::vector<int> dm = diag_ops(s, g);
-            if (!dm.empty()) {
-                tb.load(0, c->at(B, c->off[s]), sz);
-                for (int m : dm) tb.op(0, 1, m, c->hsumop(s, m), 0, (int)sz[m], 1);
-            }
+            // chains into s in batches that fit one brick program's matrix
+            // and instruction tables; batch b > 0 continues from HB in a
+            // later phase
+            std::vector<std::vector<size_t>> batch(1);
+            int nmat = (int)dm.size(), ninstr = 3 + (int)dm.size();
             for (size_t k = 0; k < K; k++) {
-                const Chain &ch = c->chains[k];
+                const Chain &ch = chains[k];
                 if (ch.lhs != s) continue;
                 if (prod_range(csz[k], 0, c->groups[g].first) != L)
                     throw HipError{"chain temporary does not match its output state's layout"};
-                tb.load(0, cur[k], csz[k]);
-                if (chain_ops(ch, g).empty()) tb.axpy(0, 1, ch.ae);
-                else chain_group_ops(tb, k, g, 0, 2, 1, ch.ae);
+                const int km = (int)chain_ops(ch, g).size(), ki = 1 + std::max(1, km);
+                if (nmat + km > kMaxBrickMats - 2 || ninstr + ki > kMaxBrickInstr - 2) {
+                    batch.emplace_back();
+                    nmat = 0;
+                    ninstr = 2;
+                }
+                batch.back().push_back(k);
+                nmat += km;
+                ninstr += ki;
             }
-            tb.store(1, c->at(HB, c->off[s]), 0);
-            fin.push_back(tb);
+            fin_batches[s] = batch;
+            (void)L;
         }
-        pb.phase(fin);
+        size_t nb = 1;
+        for (int s = 0; s < c->ns; s++) nb = std::max(nb, fin_batches[s].size());
+        for (size_t b = 0; b < nb; b++) {
+            std::vector<TaskBuilder> fin;
+            for (int s = 0; s < c->ns; s++) {
+                if (b >= fin_batches[s].size()) continue;
+                const std::vector<int64_t> sz = state_sizes(s);
+                const int64_t L = prod_range(sz, 0, c->groups[g].first);
+                TaskBuilder tb(c, g, L, 1);
+                if (b == 0) {
+                    if (G >= 2) tb.load(1, c->at(HB, c->off[s]), sz);
+                    else tb.zero(1, sz);
+                    const std::vector<int> dm = diag_ops(s, g);
+                    if (!dm.empty()) {
+                        tb.load(0, c->at(B, c->off[s]), sz);
+                        for (int m : dm) tb.op(0, 1, m, c->hsumop(s, m), 0, (int)sz[m], 1);
+                    }
+                } else {
+                    tb.load(1, c->at(HB, c->off[s]), sz);
+                }
+                for (size_t k : fin_batches[s][b]) {
+                    const Chain &ch = chains[k];
+                    tb.load(0, cur[k], csz[k]);
+                    if (chain_ops(ch, g).empty()) tb.axpy(0, 1, ch.ae);
+                    else chain_group_ops(tb, k, g, 0, 2, 1, ch.ae);
+                }
+                tb.store(1, c->at(HB, c->off[s]), 0);
+                fin.push_back(tb);
+            }
+            pb.phase(fin);
+        }
     }
     return pb.finish(c->prec);
 }
@@ -1106,15 +1288,59 @@ void apply_h(mg_ctx *c, const void *B, void *HB) {
     check_launch();
 }
 
-// Y_s = (x_m M_(m,s)) X_s for every state (norm_wf / WF_Ovl chains), FP64.
-std::unique_ptr<Plan> build_apply_kron(mg_ctx *c, const double2 *X, double2 *Y, const double2 *mats) {
+// H A in the Gaussian basis (conj(Adot_tmp) / der_factor of Calc_MeanField,
+// gmctdh_objects.f90:550-578), work type.
+void apply_hg(mg_ctx *c, const void *B, void *HB) {
+    PlanKey key{(const void *)9, B, HB, nullptr};
+    const Plan &p = get_plan(c, key, [](mg_ctx *cc, const PlanKey &k) {
+        return build_apply_h(cc, k[1], (void *)k[2], true);
+    });
+    run_plan(c, p);
+    check_launch();
+}
+
+GwpParams gwp_params(mg_ctx *c, double dt = 0.0) {
+    GwpParams P;
+    P.gwpa = c->dgwpa.as<double>();
+    P.maxdof = c->maxdof;
+    P.rho = c->drho.as<double2>();
+    P.mf = c->dmf.as<double2>();
+    P.nspfmax = c->nspfmax;
+    P.nopmax = c->nopmax;
+    P.nm = c->nm;
+    P.ns = c->ns;
+    P.eps_psi = c->eps_psi;
+    P.eps_C = c->eps_C;
+    P.eps_ovl = c->eps_ovl;
+    P.min_stepsize = c->min_stepsize;
+    P.dt = dt;
+    P.dormand_prince = (c->int_type_psi == 2);
+    P.der_factor = make_double2(0.0, -1.0);
+    P.max_attempts = 200000;
+    return P;
+}
+
+// Y_s = (x_m M_(m,s)) X_s for every state over the modes in `mask` (bit m;
+// norm_wf / WF_Ovl chains over all modes, A_to_B / B_to_A and the overlap
+// weights of the density matrices over Gaussian modes), FP64.
+std::unique_ptr<Plan> build_apply_kron(mg_ctx *c, const double2 *X, double2 *Y, const double2 *mats,
+                                       unsigned mask) {
     PlanBuilder pb;
     const int G = (int)c->groups.size();
     const size_t nn = (size_t)c->nspfmax * c->nspfmax;
+    auto in_mask = [&](int m) { return m >= 32 || ((mask >> m) & 1u); };
+    int last = -1;
+    for (int g = 0; g < G; g++)
+        for (int m = c->groups[g].first; m < c->groups[g].second; m++)
+            if (in_mask(m)) last = g;
+    if (last < 0) throw HipError{"mode product chain without modes"};
     double2 *tmp = c->dkron.as<double2>();
     std::vector<const double2 *> cur(c->ns);
     for (int s = 0; s < c->ns; s++) cur[s] = X + c->off[s];
-    for (int g = 0; g < G; g++) {
+    for (int g = 0; g <= last; g++) {
+        bool any = false;
+        for (int m = c->groups[g].first; m < c->groups[g].second; m++) any = any || in_mask(m);
+        if (!any) continue;
         std::vector<TaskBuilder> tasks;
         for (int s = 0; s < c->ns; s++) {
             std::vector<int64_t> sz(c->nm);
@@ -1122,9 +1348,9 @@ std::unique_ptr<Plan> build_apply_kron(mg_ctx *c, const double2 *X, double2 *Y, 
             TaskBuilder tb(c, g, prod_range(sz, 0, c->groups[g].first), prod_range(sz, c->groups[g].second, c->nm));
             tb.load(0, cur[s], sz);
             for (int m = c->groups[g].first; m < c->groups[g].second; m++)
-                tb.op(0, 0, m, mats + nn * (m + (size_t)c->nm * s), 0, (int)sz[m], 0);
+                if (in_mask(m)) tb.op(0, 0, m, mats + nn * (m + (size_t)c->nm * s), 0, (int)sz[m], 0);
             // the temporary is updated in place from the second group on
-            double2 *out = (g == G - 1) ? Y + c->off[s] : tmp + c->off[s];
+            double2 *out = (g == last) ? Y + c->off[s] : tmp + c->off[s];
             tb.store(0, out, 0);
             cur[s] = out;
             tasks.push_back(tb);
@@ -1134,13 +1360,31 @@ std::unique_ptr<Plan> build_apply_kron(mg_ctx *c, const double2 *X, double2 *Y, 
     return pb.finish(0);
 }
 
-void apply_kron(mg_ctx *c, const double2 *X, double2 *Y, const double2 *mats) {
-    PlanKey key{(const void *)2, X, Y, mats};
+void apply_kron(mg_ctx *c, const double2 *X, double2 *Y, const double2 *mats, unsigned mask = ~0u) {
+    PlanKey key{(const void *)(uintptr_t)(mask == ~0u ? 2 : (0x100000000ull | mask)), X, Y, mats};
     const Plan &p = get_plan(c, key, [](mg_ctx *cc, const PlanKey &k) {
-        return build_apply_kron(cc, (const double2 *)k[1], (double2 *)k[2], (const double2 *)k[3]);
+        const uintptr_t t = (uintptr_t)k[0];
+        return build_apply_kron(cc, (const double2 *)k[1], (double2 *)k[2], (const double2 *)k[3],
+                                t == 2 ? ~0u : (unsigned)(t & 0xffffffffu));
     });
     run_plan(c, p);
     check_launch();
+}
+
+// A_to_B / B_to_A (cmf2_base.f90:264-345): B = D^H S A and A = D B over the
+// Gaussian modes, with the intra-state overlaps of the current SPFs (gwp_s00)
+// and the D matrices dd.
+void a_to_b(mg_ctx *c, const double2 *A, double2 *B, const double2 *dd) {
+    launch_gwp_ab_mats(c->dgwpdesc.as<GwpDesc>(), c->ngwp_desc, c->dSd.as<double2>(), dd, gwp_params(c),
+                       c->dABmats.as<double2>(), c->stream);
+    apply_kron(c, A, B, c->dABmats.as<double2>(), c->gwp_mask);
+}
+
+void b_to_a(mg_ctx *c, const double2 *B, double2 *A, const double2 *dd) {
+    launch_gwp_ab_mats(c->dgwpdesc.as<GwpDesc>(), c->ngwp_desc, c->dSd.as<double2>(), dd, gwp_params(c),
+                       c->dABmats.as<double2>(), c->stream);
+    const size_t half = (size_t)c->nspfmax * c->nspfmax * c->nm * c->ns;
+    apply_kron(c, B, A, c->dABmats.as<double2>() + half, c->gwp_mask);
 }
 
 double2 dot(mg_ctx *c, const void *x, const void *y, int64_t n, int prec = 0) {
@@ -1172,7 +1416,8 @@ void run_spf_tasks(mg_ctx *c, const PlanKey &key, std::vector<SpfTask> (*build)(
         std::vector<SpfTask> tasks = build(c, key);
         const size_t base = spf_scratch_base(c);
         int64_t maxng = 0;
-        for (int m = 0; m < c->nm; m++) maxng = std::max(maxng, c->ngpm[m]);
+        for (int m = 0; m < c->nm; m++)
+            if (!c->gwp[m]) maxng = std::max(maxng, c->ngpm[m]);
         for (size_t i = 0; i < tasks.size(); i++)
             tasks[i].scratch = c->dspfscr.as<double2>() + base + i * 2 * (size_t)maxng * c->nspfmax;
         auto p = std::make_unique<SpfPlan>();
@@ -1191,7 +1436,7 @@ std::vector<SpfTask> build_h1mode(mg_ctx *c, const PlanKey &key) {
     std::vector<SpfTask> tasks;
     for (int m = 0; m < c->nm; m++)
         for (int i = 0; i < c->nOper[m]; i++) {
-            if (c->isIdent[m + (size_t)c->nm * i]) continue;
+            if (c->gwp[m] || c->isIdent[m + (size_t)c->nm * i]) continue;
             const Oper &v = c->opers[(size_t)m * c->nopmax + i];
             const int s1 = v.lhs - 1, s2 = v.rhs - 1;
             SpfTask t;
@@ -1212,14 +1457,56 @@ std::vector<SpfTask> build_h1mode(mg_ctx *c, const PlanKey &key) {
     return tasks;
 }
 
+// Calc_S00 of every Gaussian mode (gmctdh_objects.f90:323-381).
+void gwp_s00(mg_ctx *c, const double2 *psi) {
+    launch_gwp_s00(c->dgwpdesc.as<GwpDesc>(), c->ngwp_desc, psi, gwp_params(c), c->dS00.as<double2>(),
+                   c->dSd.as<double2>(), c->stream);
+}
+
+// Calc_h1mode of every mode (hamilton.f90:416-531): grid modes through the
+// SPF block products, Gaussian modes from S00 and the Gaussian integrals.
+// Without Gaussian modes the H.A diagonal sums follow at once; with them
+// they wait for D^H h D (prepare_ha).
 void calc_h1mode(mg_ctx *c, const double2 *psi) {
     run_spf_tasks(c, PlanKey{(const void *)3, psi, nullptr, nullptr}, build_h1mode);
+    if (c->ngwp) {
+        gwp_s00(c, psi);
+        launch_gwp_h1(c->dgwpdesc.as<GwpDesc>(), c->ngwp_desc, psi, gwp_params(c), c->dS00.as<double2>(),
+                      c->dh1.as<double2>(), c->stream);
+        if (c->prec)
+            launch_convert(c->dh1f.p, 1, c->dh1.p, 0, (int64_t)(c->dh1.bytes / sizeof(double2)), c->stream);
+        check_launch();
+        return;
+    }
     launch_hsum(c->dhsum.as<double2>(), c->dh1.as<double2>(), c->dhs_rowptr.as<int>(), c->dhs_src.as<int>(),
                 c->dhs_coef.as<double2>(), c->ns * c->nm, c->nspfmax, c->stream);
     if (c->prec) {
         launch_convert(c->dh1f.p, 1, c->dh1.p, 0, (int64_t)(c->dh1.bytes / sizeof(double2)), c->stream);
         launch_convert(c->dhsumf.p, 1, c->dhsum.p, 0, (int64_t)(c->dhsum.bytes / sizeof(double2)), c->stream);
     }
+}
+
+// H.A operator table with Gaussian modes (Calc_DHD, hamilton.f90:535-574):
+// h1mode on grid modes, D^H h D on Gaussian modes, then the diagonal sums.
+void prepare_ha(mg_ctx *c, const double2 *dd) {
+    HIP_CHECK(hipMemcpyAsync(c->dhA.p, c->dh1.p, c->dhA.bytes, hipMemcpyDeviceToDevice, c->stream));
+    launch_gwp_dhd(c->dgwpdesc.as<GwpDesc>(), c->ngwp_desc, c->dh1.as<double2>(), dd, gwp_params(c),
+                   c->dhA.as<double2>(), c->stream);
+    launch_hsum(c->dhsum.as<double2>(), c->dhA.as<double2>(), c->dhs_rowptr.as<int>(), c->dhs_src.as<int>(),
+                c->dhs_coef.as<double2>(), c->ns * c->nm, c->nspfmax, c->stream);
+    if (c->prec) {
+        launch_convert(c->dhAf.p, 1, c->dhA.p, 0, (int64_t)(c->dhA.bytes / sizeof(double2)), c->stream);
+        launch_convert(c->dhsumf.p, 1, c->dhsum.p, 0, (int64_t)(c->dhsum.bytes / sizeof(double2)), c->stream);
+    }
+    check_launch();
+}
+
+// D0 = InverseSquareRootU(S00) per Gaussian mode and set (cmf2_base.f90:72-80);
+// needs gwp_s00 of the SPFs.
+void gwp_dd_init(mg_ctx *c, double2 *dd) {
+    launch_gwp_dd_init(c->dgwpdesc.as<GwpDesc>(), c->ngwp_desc, c->dSd.as<double2>(), gwp_params(c), dd,
+                       c->dstatus.as<int>(), c->stream);
+    check_launch();
 }
 
 // out(m, s) = <phi1_(m,s) | phi2_(m,s)>; sym for phi1 == phi2 (zherk).
@@ -1231,6 +1518,7 @@ std::vector<SpfTask> build_overlaps(mg_ctx *c, const PlanKey &key) {
     const size_t nn = (size_t)c->nspfmax * c->nspfmax;
     for (int s = 0; s < c->ns; s++)
         for (int m = 0; m < c->nm; m++) {
+            if (c->gwp[m]) continue;
             SpfTask t;
             std::memset(&t, 0, sizeof t);
             t.phiA = psi1 + c->IP(m, s);
@@ -1249,6 +1537,11 @@ std::vector<SpfTask> build_overlaps(mg_ctx *c, const PlanKey &key) {
 
 void calc_overlaps(mg_ctx *c, const double2 *psi1, const double2 *psi2, double2 *out) {
     run_spf_tasks(c, PlanKey{(const void *)4, psi1, psi2, out}, build_overlaps);
+    if (c->ngwp) {
+        launch_gwp_overlaps(c->dgwpdesc.as<GwpDesc>(), c->ngwp_desc, psi1, psi2, psi1 == psi2 ? 1 : 0, gwp_params(c),
+                            out, c->stream);
+        check_launch();
+    }
 }
 
 // norm_wf (gmctdh_objects.f90:21-97)
@@ -1298,7 +1591,41 @@ void calc_rho(mg_ctx *c, const double2 *A64, bool exact = false) {
         launch_convert(c->dAf.p, 1, A64, 0, c->nconf, c->stream);
         A = c->dAf.p;
     }
-    if (c->nspfmax <= 8) {
+    if (c->ngwp) {
+        // Overlap-weighted density matrices (gmctdh_objects.f90:439-457):
+        // rho(m) = Gram over the other modes of (A, X_m), X_m = (x_{q != m}
+        // S00_q) A over the Gaussian modes q; FP64.  X is shared by all grid
+        // modes (every Gaussian mode weighted).
+        unsigned have = 0xffffffffu;
+        double2 *X = c->dHtmp.as<double2>();
+        for (int m = 0; m < c->nm; m++) {
+            const unsigned mask = c->gwp_mask & ~(1u << m);
+            const double2 *Xm = A64;
+            if (mask) {
+                if (mask != have) apply_kron(c, A64, X, c->dSd.as<double2>(), mask);
+                have = mask;
+                Xm = X;
+            }
+            for (int st = 0; st < c->ns; st++) {
+                GramTask g;
+                std::memset(&g, 0, sizeof g);
+                int64_t ll = 1, nl = 1;
+                for (int q = 0; q < m; q++) ll *= c->NS(q, st);
+                for (int q = m + 1; q < c->nm; q++) nl *= c->NS(q, st);
+                g.X = A64 + c->off[st];
+                g.Y = Xm + c->off[st];
+                g.ll = ll;
+                g.nl = nl;
+                g.n1 = g.n2 = (int)c->NS(m, st);
+                g.G = c->rho(m, st);
+                g.ld = c->nspfmax;
+                g.alpha = make_double2(1, 0);
+                g.accumulate = 0;
+                g.hermitian = 1;
+                launch_gram(g, c->dpartials.as<double2>(), c->nblocks_red, c->nspfmax, 0, c->stream);
+            }
+        }
+    } else if (c->nspfmax <= 8) {
         // one brick pass per mode group and state: all the group's Grams at once
         const void *Ain = gp ? A : (const void *)A64;
         PlanKey key{(const void *)(gp ? 5 : 6), Ain, nullptr, nullptr};
@@ -1588,7 +1915,7 @@ void calc_meanfield(mg_ctx *c, const double2 *A64, bool af_ready = false) {
         for (int i = 1; i < c->nm; i++) {
             const int jm = c->nm - i;
             const int op = c->iOper[jm + (size_t)c->nm * t];
-            if (c->isIdent[jm + (size_t)c->nm * op]) { R[i] = R[i - 1]; continue; }
+            if (c->mfskip[jm + (size_t)c->nm * op]) { R[i] = R[i - 1]; continue; }
             ModeTask mt;
             int64_t ll = 1, nl = 1;
             for (int q = 0; q < jm; q++) ll *= sz[q];
@@ -1613,7 +1940,7 @@ void calc_meanfield(mg_ctx *c, const double2 *A64, bool af_ready = false) {
         for (int i = 1; i < c->nm; i++) {
             const int m = i - 1;
             const int op = c->iOper[m + (size_t)c->nm * t];
-            if (c->isIdent[m + (size_t)c->nm * op]) { Lc[i] = Lc[i - 1]; continue; }
+            if (c->mfskip[m + (size_t)c->nm * op]) { Lc[i] = Lc[i - 1]; continue; }
             ModeTask mt;
             int64_t ll = 1, nl = 1;
             for (int q = 0; q < m; q++) ll *= sz[q];
@@ -1686,11 +2013,19 @@ SpfParams spf_params(mg_ctx *c, double dt) {
 
 // Launch on stream `st`; the status word lands in pinned host memory and is
 // checked by spf_check once `st` has been synchronised.
-void propagate_spf_launch(mg_ctx *c, double2 *psi, double dt, hipStream_t st) {
+// Grid modes through k_spf_propagate; Gaussian modes (with their D matrices
+// dd) through k_gwp_propagate, cmf2_base.f90:117-124 (grid modes first).
+void propagate_spf_launch(mg_ctx *c, double2 *psi, double dt, hipStream_t st, double2 *dd = nullptr) {
     SpfParams P = spf_params(c, dt);
     launch_fill(reinterpret_cast<double2 *>(c->dstatus.p), 1, make_double2(0, 0), st);
-    launch_spf_propagate(c->dspfmodes.as<SpfModeDesc>(), c->nm, psi, c->dspfscr.as<double2>(), P, dof_tables(c),
-                         c->dstatus.as<int>(), st);
+    if (c->ndvr_desc > 0)
+        launch_spf_propagate(c->dspfmodes.as<SpfModeDesc>(), c->ndvr_desc, psi, c->dspfscr.as<double2>(), P,
+                             dof_tables(c), c->dstatus.as<int>(), st);
+    if (c->ngwp) {
+        if (!dd) throw HipError{"Gaussian-mode propagation without D matrices"};
+        launch_gwp_propagate(c->dgwpdesc.as<GwpDesc>(), c->ngwp_desc, psi, dd, c->dgwpscr.as<double2>(),
+                             gwp_params(c, dt), c->dstatus.as<int>(), st);
+    }
     check_launch();
     HIP_CHECK(hipMemcpyAsync(c->spf_status_host, c->dstatus.p, sizeof(int), hipMemcpyDeviceToHost, st));
 }
@@ -1699,19 +2034,21 @@ void spf_check(mg_ctx *c) {
     const int st = *c->spf_status_host;
     if (st == 1) throw HipError{"SPF overlap matrix not positive definite"};
     if (st == 2) throw HipError{"SPF Runge-Kutta did not converge"};
+    if (st == 3) throw HipError{"Gaussian overlap matrix not positive definite"};
+    if (st == 4) throw HipError{"operator not supported on a Gaussian mode"};
 }
 
-void propagate_spf(mg_ctx *c, double2 *psi, double dt) {
-    propagate_spf_launch(c, psi, dt, c->stream);
+void propagate_spf(mg_ctx *c, double2 *psi, double dt, double2 *dd = nullptr) {
+    propagate_spf_launch(c, psi, dt, c->stream, dd);
     HIP_CHECK(hipStreamSynchronize(c->stream));
     spf_check(c);
 }
 
 // Start an SPF propagation on stream2 after everything queued on stream so far.
-void propagate_spf_fork(mg_ctx *c, double2 *psi, double dt) {
+void propagate_spf_fork(mg_ctx *c, double2 *psi, double dt, double2 *dd = nullptr) {
     HIP_CHECK(hipEventRecord(c->ev_fork, c->stream));
     HIP_CHECK(hipStreamWaitEvent(c->stream2, c->ev_fork, 0));
-    propagate_spf_launch(c, psi, dt, c->stream2);
+    propagate_spf_launch(c, psi, dt, c->stream2, dd);
     HIP_CHECK(hipEventRecord(c->ev_join, c->stream2));
 }
 
@@ -1844,7 +2181,14 @@ void copy_d2d(mg_ctx *c, double2 *dst, const double2 *src, int64_t n) {
     HIP_CHECK(hipMemcpyAsync(dst, src, sizeof(double2) * n, hipMemcpyDeviceToDevice, c->stream));
 }
 
+void cmf2_step_gwp(mg_ctx *c, double tt, double htry, double *hdid_out, double *hnext_out,
+                   std::vector<mg_attempt> *log);
+
 void cmf2_step(mg_ctx *c, double tt, double htry, double *hdid_out, double *hnext_out, std::vector<mg_attempt> *log) {
+    if (c->ngwp) {
+        cmf2_step_gwp(c, tt, htry, hdid_out, hnext_out, log);
+        return;
+    }
     const int64_t N = c->nconf, D = c->dimpsi;
     double2 *A = c->dA.as<double2>(), *psi = c->dpsi.as<double2>();
     double2 *B0 = c->dB0.as<double2>(), *BH = c->dBH.as<double2>(), *B02 = c->dB02.as<double2>();
@@ -1903,13 +2247,105 @@ void cmf2_step(mg_ctx *c, double tt, double htry, double *hdid_out, double *hnex
     HIP_CHECK(hipStreamSynchronize(c->stream));
 }
 
+// cmf2 with Gaussian modes (cmf2_base.f90:47-206): the coefficients are
+// propagated as B = D^H S A in the basis orthonormalised by D, D0 from the
+// Cholesky factor of S00 at the start of the step; the SPF propagation
+// carries D along; A = D B at the half step and at the end.
+void cmf2_step_gwp(mg_ctx *c, double tt, double htry, double *hdid_out, double *hnext_out,
+                   std::vector<mg_attempt> *log) {
+    const int64_t N = c->nconf, D = c->dimpsi;
+    const size_t nD = (size_t)c->nspfmax * c->nspfmax * c->nm * c->ns;
+    double2 *A = c->dA.as<double2>(), *psi = c->dpsi.as<double2>(), *AH = c->dAH.as<double2>();
+    double2 *B0 = c->dB0.as<double2>(), *BH = c->dBH.as<double2>(), *B02 = c->dB02.as<double2>();
+    double2 *pH = c->dpH.as<double2>(), *pH2 = c->dpH2.as<double2>();
+    double2 *DD = c->dDD.as<double2>(), *DH = c->dDH.as<double2>(), *DH2 = c->dDH2.as<double2>();
+    const double safe = 0.95, maxscal = 1.5;
+    double hdid = htry;
+    launch_fill(reinterpret_cast<double2 *>(c->dstatus.p), 1, make_double2(0, 0), c->stream);
+    gwp_s00(c, psi);
+    gwp_dd_init(c, DD);
+    HIP_CHECK(hipMemcpyAsync(c->spf_status_host, c->dstatus.p, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIP_CHECK(hipStreamSynchronize(c->stream));
+    spf_check(c);
+    for (;;) {
+        int forced = -1;
+        if (c->replay_pos < c->replay.size()) {
+            const mg_attempt &r = c->replay[c->replay_pos++];
+            hdid = r.h;
+            forced = (int)r.accepted;
+        }
+        calc_h1mode(c, psi);                 // S00(psi) and h1mode
+        calc_rho(c, A);
+        calc_meanfield(c, A, true);
+        prepare_ha(c, DD);
+        a_to_b(c, A, B0, DD);
+        copy_d2d(c, pH, psi, D);
+        copy_d2d(c, DH, DD, (int64_t)nD);
+        propagate_spf_fork(c, pH, 0.5 * hdid, DH);
+        const int nk1 = lanczos(c, B0, BH, 0.5 * hdid, true);
+        propagate_spf_join(c);
+        b_to_a(c, BH, AH, DH);
+        calc_h1mode(c, pH);
+        calc_rho(c, AH);
+        calc_meanfield(c, AH, true);
+        copy_d2d(c, pH2, psi, D);
+        copy_d2d(c, DH2, DD, (int64_t)nD);
+        propagate_spf(c, pH2, 0.5 * hdid, DH2);
+        delta_psi_overlaps(c, pH, pH2);
+        const double err_psi = delta_psi_contract(c, AH, AH);
+        propagate_spf(c, pH2, 0.5 * hdid, DH2);
+        calc_h1mode(c, pH2);
+        prepare_ha(c, DH2);
+        const int nk2 = lanczos(c, BH, B02, -0.5 * hdid, true);
+        launch_axpby(B02, B0, -1.0, 1.0, N, 0, c->stream);
+        const double err_A = std::sqrt(norm2(c, B02, N)) * 0.5;
+        const double error = std::max(err_A, err_psi) / c->eps_int;
+        const bool reject = (forced >= 0) ? (forced == 0) : (error > 1.0);
+        if (log) log->push_back({tt, hdid, err_A, err_psi, (int64_t)(!reject), nk1, nk2});
+        if (reject) {
+            hdid = hdid / std::sqrt(error) * safe;
+            if (hdid < c->min_stepsize) throw HipError{"CMF2 step size fell below minstep"};
+            continue;
+        }
+        lanczos(c, BH, B0, 0.5 * hdid, false);
+        b_to_a(c, B0, A, DH2);
+        copy_d2d(c, psi, pH2, D);
+        copy_d2d(c, DD, DH2, (int64_t)nD);
+        double hn = std::min(hdid / std::sqrt(error), hdid * maxscal);
+        hn = std::min(hn, c->writetime);
+        *hnext_out = hn;
+        break;
+    }
+    *hdid_out = hdid;
+    HIP_CHECK(hipStreamSynchronize(c->stream));
+}
+
+// Energy with Gaussian modes: <A|H|A> = <B|D^H h D|B> with D0 from the
+// Cholesky factor of the current overlaps and B = D0^H S A (D0 D0^H = S^-1).
+double energy_gwp(mg_ctx *c, const double2 *A) {
+    double2 *Dobs = c->dDobs.as<double2>(), *Bo = c->dIn.as<double2>(), *HB = c->dOut.as<double2>();
+    gwp_dd_init(c, Dobs);
+    prepare_ha(c, Dobs);
+    a_to_b(c, A, Bo, Dobs);
+    if (c->prec) {
+        launch_convert(c->dXf.p, 1, Bo, 0, c->nconf, c->stream);
+        apply_h(c, c->dXf.p, c->dYf.p);
+        return dot(c, c->dXf.p, c->dYf.p, c->nconf, 1).x;
+    }
+    apply_h(c, Bo, HB);
+    return dot(c, Bo, HB, c->nconf).x;
+}
+
 // WriteOutput observables (output.f90:33-87, 143-150).
 void observables(mg_ctx *c, double t, mg_output *out, double *pops) {
     double2 *A = c->dA.as<double2>(), *psi = c->dpsi.as<double2>();
+    if (c->ngwp) gwp_s00(c, psi);
     calc_rho(c, A, true);
     calc_h1mode(c, psi);
     double energy;
-    if (c->prec) {
+    if (c->ngwp) {
+        energy = energy_gwp(c, A);
+    } else if (c->prec) {
         launch_convert(c->dAf.p, 1, A, 0, c->nconf, c->stream);
         apply_h(c, c->dAf.p, c->dYf.p);
         energy = dot(c, c->dAf.p, c->dYf.p, c->nconf, 1).x;
@@ -1921,11 +2357,26 @@ void observables(mg_ctx *c, double t, mg_output *out, double *pops) {
     std::vector<double2> r(nn * c->nm * c->ns);
     HIP_CHECK(hipMemcpyAsync(r.data(), c->drho.p, sizeof(double2) * r.size(), hipMemcpyDeviceToHost, c->stream));
     HIP_CHECK(hipStreamSynchronize(c->stream));
+    std::vector<double2> sd;
+    if (c->ngwp && c->gwp[0]) {
+        sd.resize(nn * c->nm * c->ns);
+        HIP_CHECK(hipMemcpyAsync(sd.data(), c->dSd.p, sizeof(double2) * sd.size(), hipMemcpyDeviceToHost, c->stream));
+        HIP_CHECK(hipStreamSynchronize(c->stream));
+    }
     double tot = 0.0;
+    const size_t ld = (size_t)c->nspfmax;
     for (int s = 0; s < c->ns; s++) {
         const int jel = c->MultiSet[0] ? s : 0;
         const int64_t n = c->NS(0, jel);
         double pop = 0.0;
+        if (!sd.empty()) {
+            // Gaussian first mode: Re sum rho o S00 (output.f90:64-74)
+            const double2 *rr = r.data() + nn * (0 + (size_t)c->nm * s), *S = sd.data() + nn * (0 + (size_t)c->nm * s);
+            for (int64_t j2 = 0; j2 < n; j2++) {
+                for (int64_t j1 = 0; j1 < j2; j1++) pop += 2.0 * cmul(rr[j1 + ld * j2], S[j1 + ld * j2]).x;
+                pop += cmul(rr[j2 + ld * j2], S[j2 + ld * j2]).x;
+            }
+        } else
         for (int64_t i = 0; i < n; i++) pop += r[nn * (0 + (size_t)c->nm * s) + i + (size_t)c->nspfmax * i].x;
         if (pops) pops[s] = pop;
         tot += pop;
@@ -1972,7 +2423,51 @@ void vmf_alloc(mg_ctx *c) {
 // Calc_Derivs at (A, psi): kA <- H A (work type; dA = der_factor kA, applied
 // in the stage combinations), kp <- dpsi of every mode.  Failures of the SPF
 // projector solve accumulate in dstatus.
+// Calc_Derivs with Gaussian modes (derivatives.f90:61-89, 155-238).  kA
+// receives dA / der_factor = S^-1 H_g A - i sum_m tau_m A (vmf_combine
+// multiplies by der_factor dt): H_g is H in the Gaussian basis (S00 for the
+// identity on Gaussian modes), S^-1 the Tikhonov S00^-1 on every Gaussian
+// mode, tau_m of Calc_Der_SPF; the Gaussian-parameter derivatives come from
+// k_gwp_der with D = 1 (its dD output is then -tau).
+void calc_derivs_gwp(mg_ctx *c, const double2 *A, const double2 *psi, void *kA, double2 *kp) {
+    const int64_t N = c->nconf;
+    calc_h1mode(c, psi);
+    calc_rho(c, A);
+    calc_meanfield(c, A, true);
+    launch_fill(kp, c->dimpsi, make_double2(0, 0), c->stream);
+    SpfParams P = spf_params(c, 0.0);
+    if (c->ndvr_desc > 0)
+        launch_der_spf(c->dspfmodes.as<SpfModeDesc>(), -c->ndvr_desc, psi, kp, c->dspfscr.as<double2>(), P,
+                       dof_tables(c), c->dstatus.as<int>(), c->stream);
+    const GwpParams G = gwp_params(c);
+    for (int i = 0; i < c->ngwp_desc; i++)
+        launch_gwp_der(c->dgwpdesc.as<GwpDesc>(), i, psi, c->dDid.as<double2>(), kp, c->dTauSet.as<double2>(),
+                       c->dgwpscr.as<double2>(), G, c->dstatus.as<int>(), c->stream);
+    launch_gwp_state_mats(c->dgwpdesc.as<GwpDesc>(), c->ngwp_desc, c->dTauSet.as<double2>(), -1.0, G,
+                          c->dTauS.as<double2>(), c->stream);
+    launch_gwp_s00m1(c->dgwpdesc.as<GwpDesc>(), c->ngwp_desc, c->dSd.as<double2>(), G, c->dSm1.as<double2>(),
+                     c->stream);
+    double2 *HgA = c->dB0.as<double2>(), *X = c->dBH.as<double2>(), *Z = c->dB02.as<double2>();
+    apply_hg(c, c->prec ? c->dAf.p : (const void *)A, kA);
+    if (c->prec) launch_convert(HgA, 0, kA, 1, N, c->stream);
+    else copy_d2d(c, HgA, static_cast<const double2 *>(kA), N);
+    apply_kron(c, HgA, X, c->dSm1.as<double2>(), c->gwp_mask);
+    for (int m = 0; m < c->nm; m++) {
+        if (!c->gwp[m]) continue;
+        apply_kron(c, A, Z, c->dTauS.as<double2>(), 1u << m);
+        launch_cscale(Z, Z, make_double2(0.0, -1.0), N, c->stream);
+        launch_axpby(X, Z, 1.0, 1.0, N, 0, c->stream);
+    }
+    if (c->prec) launch_convert(kA, 1, X, 0, N, c->stream);
+    else copy_d2d(c, static_cast<double2 *>(kA), X, N);
+    check_launch();
+}
+
 void calc_derivs(mg_ctx *c, const double2 *A, const double2 *psi, void *kA, double2 *kp) {
+    if (c->ngwp) {
+        calc_derivs_gwp(c, A, psi, kA, kp);
+        return;
+    }
     calc_rho(c, A);
     calc_h1mode(c, psi);
     calc_meanfield(c, A, true);
@@ -2093,9 +2588,6 @@ int mg_create(const mg_problem *prob, int64_t precision, mg_ctx **out) {
     int rc = guarded(c, [&] {
         if (precision != 0 && precision != 1)
             throw HipError{"precision must be 0 (complex FP64) or 1 (mixed complex FP32/FP64)"};
-        if (prob->dof_gwp)
-            for (int64_t d = 0; d < prob->ndof; d++)
-                if (prob->dof_gwp[d]) throw HipError{"Gaussian (gwp) modes are not supported by this build"};
         int ndev = 0;
         HIP_CHECK(hipGetDeviceCount(&ndev));
         if (ndev < 1) throw HipError{"no HIP device"};
@@ -2226,6 +2718,7 @@ int mg_calc_h1mode(mg_ctx *c, double *h1_out) {
 
 int mg_calc_rho(mg_ctx *c, double *rho_out) {
     return guarded(c, [&] {
+        if (c->ngwp) gwp_s00(c, c->dpsi.as<double2>());
         calc_rho(c, c->dA.as<double2>());
         if (rho_out) {
             const size_t nn = (size_t)c->nspfmax * c->nspfmax;
@@ -2256,6 +2749,74 @@ int mg_apply_h(mg_ctx *c, const double *B, double *HB) {
             apply_h(c, c->dIn.p, c->dOut.p);
         }
         HIP_CHECK(hipMemcpyAsync(HB, c->dOut.p, sizeof(double2) * c->nconf, hipMemcpyDeviceToHost, c->stream));
+        HIP_CHECK(hipStreamSynchronize(c->stream));
+    });
+}
+
+int mg_prepare_gwp(mg_ctx *c) {
+    return guarded(c, [&] {
+        if (!c->ngwp) return;
+        launch_fill(reinterpret_cast<double2 *>(c->dstatus.p), 1, make_double2(0, 0), c->stream);
+        calc_h1mode(c, c->dpsi.as<double2>());
+        gwp_dd_init(c, c->dDD.as<double2>());
+        prepare_ha(c, c->dDD.as<double2>());
+        HIP_CHECK(hipMemcpyAsync(c->spf_status_host, c->dstatus.p, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+        HIP_CHECK(hipStreamSynchronize(c->stream));
+        spf_check(c);
+    });
+}
+
+int mg_gwp_overlaps(mg_ctx *c, double *s00_out, double *dd_out) {
+    return guarded(c, [&] {
+        if (!c->ngwp) throw HipError{"no Gaussian modes"};
+        launch_fill(reinterpret_cast<double2 *>(c->dstatus.p), 1, make_double2(0, 0), c->stream);
+        gwp_s00(c, c->dpsi.as<double2>());
+        gwp_dd_init(c, c->dDD.as<double2>());
+        if (s00_out) HIP_CHECK(hipMemcpyAsync(s00_out, c->dS00.p, c->dS00.bytes, hipMemcpyDeviceToHost, c->stream));
+        if (dd_out) HIP_CHECK(hipMemcpyAsync(dd_out, c->dDD.p, c->dDD.bytes, hipMemcpyDeviceToHost, c->stream));
+        HIP_CHECK(hipMemcpyAsync(c->spf_status_host, c->dstatus.p, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+        HIP_CHECK(hipStreamSynchronize(c->stream));
+        spf_check(c);
+    });
+}
+
+int mg_der_spf_gwp(mg_ctx *c, int64_t im, const double *psi_in, const double *dd_in, double *dpsi_out,
+                   double *ddd_out) {
+    return guarded(c, [&] {
+        if (im < 1 || im > c->nm || !c->gwp[im - 1]) throw HipError{"mode is not a Gaussian mode"};
+        int idesc = 0;
+        for (int m = 0; m < im - 1; m++) idesc += c->gwp[m];
+        double2 *A = c->dA.as<double2>(), *psi = c->dpsi.as<double2>();
+        calc_h1mode(c, psi);
+        calc_rho(c, A, true);
+        calc_meanfield(c, A);
+        double2 *pin = c->dpH.as<double2>(), *pout = c->dpH2.as<double2>();
+        double2 *din = c->dDH.as<double2>(), *dout = c->dDH2.as<double2>();
+        HIP_CHECK(hipMemcpyAsync(pin, psi_in, sizeof(double2) * c->dimpsi, hipMemcpyHostToDevice, c->stream));
+        HIP_CHECK(hipMemcpyAsync(din, dd_in, c->dDH.bytes, hipMemcpyHostToDevice, c->stream));
+        HIP_CHECK(hipMemsetAsync(pout, 0, sizeof(double2) * c->dimpsi, c->stream));
+        HIP_CHECK(hipMemsetAsync(dout, 0, c->dDH2.bytes, c->stream));
+        launch_fill(reinterpret_cast<double2 *>(c->dstatus.p), 1, make_double2(0, 0), c->stream);
+        launch_gwp_der(c->dgwpdesc.as<GwpDesc>(), idesc, pin, din, pout, dout, c->dgwpscr.as<double2>(),
+                       gwp_params(c), c->dstatus.as<int>(), c->stream);
+        check_launch();
+        HIP_CHECK(hipMemcpyAsync(dpsi_out, pout, sizeof(double2) * c->dimpsi, hipMemcpyDeviceToHost, c->stream));
+        HIP_CHECK(hipMemcpyAsync(ddd_out, dout, c->dDH2.bytes, hipMemcpyDeviceToHost, c->stream));
+        HIP_CHECK(hipMemcpyAsync(c->spf_status_host, c->dstatus.p, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+        HIP_CHECK(hipStreamSynchronize(c->stream));
+        spf_check(c);
+    });
+}
+
+int mg_a_to_b(mg_ctx *c, const double *A, const double *dd, double *B) {
+    return guarded(c, [&] {
+        if (!c->ngwp) throw HipError{"no Gaussian modes"};
+        double2 *din = c->dDH.as<double2>();
+        HIP_CHECK(hipMemcpyAsync(c->dIn.p, A, sizeof(double2) * c->nconf, hipMemcpyHostToDevice, c->stream));
+        HIP_CHECK(hipMemcpyAsync(din, dd, c->dDH.bytes, hipMemcpyHostToDevice, c->stream));
+        gwp_s00(c, c->dpsi.as<double2>());
+        a_to_b(c, c->dIn.as<double2>(), c->dOut.as<double2>(), din);
+        HIP_CHECK(hipMemcpyAsync(B, c->dOut.p, sizeof(double2) * c->nconf, hipMemcpyDeviceToHost, c->stream));
         HIP_CHECK(hipStreamSynchronize(c->stream));
     });
 }
@@ -2305,11 +2866,13 @@ int mg_apply_h_stats(mg_ctx *c, double *bytes_alg, double *flops_alg) {
 int mg_der_spf(mg_ctx *c, int64_t im, const double *psi_in, double *dpsi_out) {
     return guarded(c, [&] {
         if (im < 1 || im > c->nm) throw HipError{"mode index out of range"};
+        const int desc = c->dvr_desc[im - 1];
+        if (desc < 0) throw HipError{"mode is a Gaussian mode (use mg_der_spf_gwp)"};
         HIP_CHECK(hipMemcpyAsync(c->dpH.p, psi_in, sizeof(double2) * c->dimpsi, hipMemcpyHostToDevice, c->stream));
         launch_fill(c->dpH2.as<double2>(), c->dimpsi, make_double2(0, 0), c->stream);
         launch_fill(reinterpret_cast<double2 *>(c->dstatus.p), 1, make_double2(0, 0), c->stream);
         SpfParams P = spf_params(c, 0.0);
-        launch_der_spf(c->dspfmodes.as<SpfModeDesc>(), (int)im - 1, c->dpH.as<double2>(), c->dpH2.as<double2>(),
+        launch_der_spf(c->dspfmodes.as<SpfModeDesc>(), desc, c->dpH.as<double2>(), c->dpH2.as<double2>(),
                        c->dspfscr.as<double2>(), P, dof_tables(c), c->dstatus.as<int>(), c->stream);
         HIP_CHECK(hipMemcpyAsync(dpsi_out, c->dpH2.p, sizeof(double2) * c->dimpsi, hipMemcpyDeviceToHost, c->stream));
         HIP_CHECK(hipStreamSynchronize(c->stream));

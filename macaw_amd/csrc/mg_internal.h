@@ -163,6 +163,38 @@ struct SpfModeDesc {
     int scr_off;                 // offset into the scratch pool (complex elements)
 };
 
+// Gaussian (G-MCTDH) mode: Gaussians exp(A x^2 + xi x) per DOF with fixed
+// widths A; the SPF block of a set holds xi(k, j), nmd per Gaussian.
+constexpr int kMaxSets = 8;
+constexpr int kMaxGwpOps = 32;
+struct GwpDesc {
+    int mode;                    // logical mode (0-based)
+    int nmd;                     // DOFs per Gaussian
+    int nsets;                   // states (multi-set) or 1
+    int nspf[kMaxSets];
+    int off[kMaxSets];           // psi offsets (complex elements) of the set blocks
+    int rho_index[kMaxSets];     // density matrix per set (into the rho array)
+    int blk;                     // complex elements of all set blocks
+    int nop;                     // operators of the mode (identity included)
+    int op_lhs[kMaxGwpOps], op_rhs[kMaxGwpOps];   // sets
+    int op_ident[kMaxGwpOps];
+    int op_codes[kMaxGwpOps][kMaxDofPerMode];
+    int op_index[kMaxGwpOps];    // operator index i of h1(m, i) / mf(m, i)
+    int scr_off;                 // scratch offset (complex elements)
+};
+
+struct GwpParams {
+    const double *gwpa;          // [maxdof, nspfmax, nmodes]
+    int maxdof;
+    const double2 *rho;          // density matrices, index via rho_index
+    const double2 *mf;           // [nspfmax^2 * nopmax * nmodes]
+    int nspfmax, nopmax, nm, ns;
+    double eps_psi, eps_C, eps_ovl, min_stepsize, dt;
+    int dormand_prince;
+    double2 der_factor;
+    int max_attempts;
+};
+
 struct SpfParams {
     const double2 *rmf;          // [nspfmax^2 * nopmax * nmodes]
     const double2 *rho;          // density matrices, index via rho_index
@@ -172,6 +204,86 @@ struct SpfParams {
     double2 der_factor;
     int max_attempts;
 };
+
+#ifdef __HIPCC__
+// ---------------------------------------------------------------------------
+// Small dense FP64 linear algebra in one thread (n <= 16).
+constexpr int kSmall = 16;
+
+// Cyclic complex Jacobi: A (n x n, ld kSmall, Hermitian) -> eigenvalues w,
+// eigenvectors U (columns).  Same rotation as the oracle's herm_eig.
+__device__ inline void dev_herm_eig(int n, double2 *A, double *w, double2 *U) {
+    for (int i = 0; i < n; i++)
+        for (int j = 0; j < n; j++) U[i + kSmall * j] = make_double2(i == j ? 1.0 : 0.0, 0.0);
+    for (int sweep = 0; sweep < 100; sweep++) {
+        double off = 0.0, diag = 0.0;
+        for (int j = 0; j < n; j++)
+            for (int i = 0; i < n; i++) {
+                double2 a = A[i + kSmall * j];
+                double a2 = a.x * a.x + a.y * a.y;
+                if (i != j) off += a2; else diag += a2;
+            }
+        if (off <= 1e-32 * (diag + 1e-300) || off == 0.0) break;
+        for (int p = 0; p < n - 1; p++)
+            for (int q = p + 1; q < n; q++) {
+                double2 apq = A[p + kSmall * q];
+                double aabs = hypot(apq.x, apq.y);
+                if (aabs == 0.0) continue;
+                double2 ph = make_double2(apq.x / aabs, apq.y / aabs);
+                double app = A[p + kSmall * p].x, aqq = A[q + kSmall * q].x;
+                double theta = (aqq - app) / (2.0 * aabs);
+                double tt = (theta >= 0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
+                double c = 1.0 / sqrt(tt * tt + 1.0), s = tt * c;
+                double2 jpp = make_double2(c, 0.0), jqq = cscale(c, cconj(ph)), jpq = make_double2(s, 0.0),
+                        jqp = cscale(-s, cconj(ph));
+                for (int k = 0; k < n; k++) {
+                    double2 akp = A[k + kSmall * p], akq = A[k + kSmall * q];
+                    A[k + kSmall * p] = cadd(cmul(akp, jpp), cmul(akq, jqp));
+                    A[k + kSmall * q] = cadd(cmul(akp, jpq), cmul(akq, jqq));
+                }
+                for (int k = 0; k < n; k++) {
+                    double2 apk = A[p + kSmall * k], aqk = A[q + kSmall * k];
+                    A[p + kSmall * k] = cadd(cmul(cconj(jpp), apk), cmul(cconj(jqp), aqk));
+                    A[q + kSmall * k] = cadd(cmul(cconj(jpq), apk), cmul(cconj(jqq), aqk));
+                }
+                A[p + kSmall * q] = make_double2(0.0, 0.0);
+                A[q + kSmall * p] = make_double2(0.0, 0.0);
+                A[p + kSmall * p].y = 0.0;
+                A[q + kSmall * q].y = 0.0;
+                for (int k = 0; k < n; k++) {
+                    double2 ukp = U[k + kSmall * p], ukq = U[k + kSmall * q];
+                    U[k + kSmall * p] = cadd(cmul(ukp, jpp), cmul(ukq, jqp));
+                    U[k + kSmall * q] = cadd(cmul(ukp, jpq), cmul(ukq, jqq));
+                }
+            }
+    }
+    for (int i = 0; i < n; i++) w[i] = A[i + kSmall * i].x;
+}
+#endif
+
+// Gaussian-mode launchers (mg_gwp.hip).  dd: D matrices [nspfmax^2, nmodes, nstates].
+size_t gwp_scratch_elems(const GwpDesc &d, int nspfmax);
+void launch_gwp_s00(const GwpDesc *d_descs, int ndesc, const double2 *psi, GwpParams p, double2 *s00,
+                    double2 *sd, hipStream_t s);
+void launch_gwp_h1(const GwpDesc *d_descs, int ndesc, const double2 *psi, GwpParams p, const double2 *s00,
+                   double2 *h1, hipStream_t s);
+void launch_gwp_overlaps(const GwpDesc *d_descs, int ndesc, const double2 *psi1, const double2 *psi2, int sym,
+                         GwpParams p, double2 *out, hipStream_t s);
+void launch_gwp_dd_init(const GwpDesc *d_descs, int ndesc, const double2 *sd, GwpParams p, double2 *dd,
+                        int *status, hipStream_t s);
+void launch_gwp_dhd(const GwpDesc *d_descs, int ndesc, const double2 *h1, const double2 *dd, GwpParams p,
+                    double2 *dhd, hipStream_t s);
+void launch_gwp_ab_mats(const GwpDesc *d_descs, int ndesc, const double2 *sd, const double2 *dd, GwpParams p,
+                        double2 *mats, hipStream_t s);
+void launch_gwp_propagate(const GwpDesc *d_descs, int ndesc, double2 *psi, double2 *dd, double2 *scratch,
+                          GwpParams p, int *status, hipStream_t s);
+void launch_gwp_der(const GwpDesc *d_descs, int idesc, const double2 *psi, const double2 *dd, double2 *dpsi,
+                    double2 *ddd, double2 *scratch, GwpParams p, int *status, hipStream_t s);
+void launch_gwp_s00m1(const GwpDesc *d_descs, int ndesc, const double2 *sd, GwpParams p, double2 *out,
+                      hipStream_t s);
+void launch_gwp_state_mats(const GwpDesc *d_descs, int ndesc, const double2 *src, double scale, GwpParams p,
+                           double2 *dst, hipStream_t s);
+void launch_cscale(double2 *out, const double2 *x, double2 alpha, int64_t n, hipStream_t s);
 
 // Host-side launchers (mg_kernels.hip).  prec: 0 = complex FP64 storage,
 // 1 = complex FP32 storage (reductions always FP64).

@@ -1197,60 +1197,6 @@ void launch_mat_axpy(double2 *dst, const double2 *src, double2 alpha, int n, int
     hipLaunchKernelGGL(k_mat_axpy, dim3(1), dim3(256), 0, s, dst, src, alpha, n, ld);
 }
 
-// ---------------------------------------------------------------------------
-// Small dense FP64 linear algebra in one thread (n <= 16).
-constexpr int kSmall = 16;
-
-// Cyclic complex Jacobi: A (n x n, ld kSmall, Hermitian) -> eigenvalues w,
-// eigenvectors U (columns).  Same rotation as the oracle's herm_eig.
-__device__ void dev_herm_eig(int n, double2 *A, double *w, double2 *U) {
-    for (int i = 0; i < n; i++)
-        for (int j = 0; j < n; j++) U[i + kSmall * j] = make_double2(i == j ? 1.0 : 0.0, 0.0);
-    for (int sweep = 0; sweep < 100; sweep++) {
-        double off = 0.0, diag = 0.0;
-        for (int j = 0; j < n; j++)
-            for (int i = 0; i < n; i++) {
-                double2 a = A[i + kSmall * j];
-                double a2 = a.x * a.x + a.y * a.y;
-                if (i != j) off += a2; else diag += a2;
-            }
-        if (off <= 1e-32 * (diag + 1e-300) || off == 0.0) break;
-        for (int p = 0; p < n - 1; p++)
-            for (int q = p + 1; q < n; q++) {
-                double2 apq = A[p + kSmall * q];
-                double aabs = hypot(apq.x, apq.y);
-                if (aabs == 0.0) continue;
-                double2 ph = make_double2(apq.x / aabs, apq.y / aabs);
-                double app = A[p + kSmall * p].x, aqq = A[q + kSmall * q].x;
-                double theta = (aqq - app) / (2.0 * aabs);
-                double tt = (theta >= 0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
-                double c = 1.0 / sqrt(tt * tt + 1.0), s = tt * c;
-                double2 jpp = make_double2(c, 0.0), jqq = cscale(c, cconj(ph)), jpq = make_double2(s, 0.0),
-                        jqp = cscale(-s, cconj(ph));
-                for (int k = 0; k < n; k++) {
-                    double2 akp = A[k + kSmall * p], akq = A[k + kSmall * q];
-                    A[k + kSmall * p] = cadd(cmul(akp, jpp), cmul(akq, jqp));
-                    A[k + kSmall * q] = cadd(cmul(akp, jpq), cmul(akq, jqq));
-                }
-                for (int k = 0; k < n; k++) {
-                    double2 apk = A[p + kSmall * k], aqk = A[q + kSmall * k];
-                    A[p + kSmall * k] = cadd(cmul(cconj(jpp), apk), cmul(cconj(jqp), aqk));
-                    A[q + kSmall * k] = cadd(cmul(cconj(jpq), apk), cmul(cconj(jqq), aqk));
-                }
-                A[p + kSmall * q] = zero2();
-                A[q + kSmall * p] = zero2();
-                A[p + kSmall * p].y = 0.0;
-                A[q + kSmall * q].y = 0.0;
-                for (int k = 0; k < n; k++) {
-                    double2 ukp = U[k + kSmall * p], ukq = U[k + kSmall * q];
-                    U[k + kSmall * p] = cadd(cmul(ukp, jpp), cmul(ukq, jqp));
-                    U[k + kSmall * q] = cadd(cmul(ukp, jpq), cmul(ukq, jqq));
-                }
-            }
-    }
-    for (int i = 0; i < n; i++) w[i] = A[i + kSmall * i].x;
-}
-
 // Calc_rhom1MF (gmctdh_objects.f90:1147-1194): per job (mode, set) the
 // 'poly'-regularised inverse of rho (maths.f90:8-83) times each mean field.
 // job layout: [mode, rho_index, n, nops, (op, n2) x nops], stride 36.

@@ -59,6 +59,10 @@ def lib():
         L.mg_observables.argtypes = [vp, C.c_double, C.POINTER(MgOutput), f64p]
         L.mg_propagate.argtypes = [vp, C.c_double, C.c_double, C.POINTER(MgOutput), f64p, C.c_int64,
                                    i64p, C.POINTER(MgAttempt), C.c_int64, i64p]
+        L.mg_prepare_gwp.argtypes = [vp]
+        L.mg_gwp_overlaps.argtypes = [vp, f64p, f64p]
+        L.mg_der_spf_gwp.argtypes = [vp, C.c_int64, f64p, f64p, f64p, f64p]
+        L.mg_a_to_b.argtypes = [vp, f64p, f64p, f64p]
         _lib = L
     return _lib
 
@@ -218,3 +222,32 @@ class Engine:
         self._check(lib().mg_propagate(self.h, t0, tfinal, outs, pops.ctypes.data_as(f64p), max_out,
                                        C.byref(no), atts, max_att, C.byref(na)))
         return list(outs[: no.value]), pops[: no.value], list(atts[: na.value])
+
+    # ---- Gaussian (G-MCTDH) modes ----
+    def prepare_gwp(self):
+        """S00, h1mode, D0 and D^H h D of the resident state (apply_h then
+        works in the orthonormalised basis)."""
+        self._check(lib().mg_prepare_gwp(self.h))
+
+    def gwp_overlaps(self):
+        ns, nm, n = self.prob.nstates, self.prob.nmodes, self.nspfmax
+        S = np.zeros((ns, ns, nm, n, n), np.complex128)
+        D = np.zeros((ns, nm, n, n), np.complex128)
+        self._check(lib().mg_gwp_overlaps(self.h, cplx_ptr(S), cplx_ptr(D)))
+        return S, D
+
+    def der_spf_gwp(self, im, psi, D):
+        psi = np.ascontiguousarray(psi, np.complex128)
+        D = np.ascontiguousarray(D, np.complex128)
+        dpsi = np.zeros_like(psi)
+        dD = np.zeros_like(D)
+        self._check(lib().mg_der_spf_gwp(self.h, im, cplx_ptr(psi), cplx_ptr(D), cplx_ptr(dpsi), cplx_ptr(dD)))
+        return dpsi, dD
+
+    def a_to_b(self, A, D):
+        A = np.ascontiguousarray(A, np.complex128)
+        D = np.ascontiguousarray(D, np.complex128)
+        B = np.zeros_like(A)
+        self._check(lib().mg_a_to_b(self.h, cplx_ptr(A), cplx_ptr(D), cplx_ptr(B)))
+        return B
+

@@ -7,7 +7,9 @@ Writes Macaw-format input and operator files for the benchmark workloads:
   8 SPFs (8^8 configurations per state), 200 fs.
 * ``c3``: G-MCTDH, 3 states, 6 modes; modes 1-3 with 5 Gaussian SPFs (HO
   ground-state widths on the reference's neighbour-shell lattice), modes 4-6
-  HO-DVR 28 points, 7 SPFs; C-matrix Tikhonov eps 1e-8, 150 fs.
+  HO-DVR 28 points, 7 SPFs; C-matrix Tikhonov eps 1e-8, CMF2 eps 1e-5 (the
+  DeltaPsi estimate of these wavefunctions has a rounding floor near 1e-7),
+  150 fs.
 * ``t*``, ``g*``: small variants used by the tests (``g*`` with Gaussian modes).
 
 Energies are drawn in eV with ``numpy.random.default_rng(seed)`` and written
@@ -79,15 +81,15 @@ SPECS = {
     # G-MCTDH: 3 states, modes 1-3 with 5 Gaussians, modes 4-6 HO-DVR 28 / 7 SPFs
     "c3": LVCSpec("c3", 303, 3, 6, 28, 7, 150.0, omega=(0.05, 0.35), kappa_sd=0.10,
                   n_lambda=3, lam=(0.05, 0.15), gaps=(-0.2, 0.0, 0.2), gwp_modes=3, ngauss=5,
-                  eps_C=1e-8),
+                  eps_C=1e-8, eps_int=1e-5),
     # small G-MCTDH cases for the tests
     "g1": LVCSpec("g1", 21, 2, 3, 8, 3, 5.0, omega=(0.05, 0.30), kappa_sd=0.10,
-                  n_lambda=1, lam=(0.10, 0.10), gaps=(-0.2, 0.2), gwp_modes=1, ngauss=3),
+                  n_lambda=1, lam=(0.10, 0.10), gaps=(-0.2, 0.2), gwp_modes=1, ngauss=3, eps_int=1e-6),
     "g2": LVCSpec("g2", 22, 2, 3, 8, 3, 5.0, omega=(0.05, 0.30), kappa_sd=0.10,
                   gamma_sd=0.02, n_lambda=2, lam=(0.05, 0.15), gaps=(-0.1, 0.1), gwp_modes=2,
-                  ngauss=3, gwp_multiset=False, spf_integrator="dopri"),
+                  ngauss=3, gwp_multiset=False, spf_integrator="dopri", eps_int=1e-6),
     "g3": LVCSpec("g3", 23, 3, 4, 8, 3, 5.0, omega=(0.05, 0.30), kappa_sd=0.10,
-                  n_lambda=3, lam=(0.05, 0.15), gaps=(-0.2, 0.0, 0.2), gwp_modes=2, ngauss=4),
+                  n_lambda=3, lam=(0.05, 0.15), gaps=(-0.2, 0.0, 0.2), gwp_modes=2, ngauss=4, eps_int=1e-6),
 }
 
 

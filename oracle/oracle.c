@@ -1203,6 +1203,36 @@ static void apply_h(orc *o, const cplx *B, cplx *HB) {
 
 void orc_apply_h(orc *o, const double *B, double *HB) { apply_h(o, (const cplx *)B, (cplx *)HB); }
 
+/* H A in the Gaussian basis: every term's h1mode on every Gaussian mode (the
+ * identity there is S00) and on the non-identity grid modes -- the vector
+ * conj(Adot_tmp) / der_factor of Calc_MeanField (gmctdh_objects.f90:550-578),
+ * used by the VMF derivatives (derivatives.f90:160-176). */
+static void apply_hg(orc *o, const cplx *B, cplx *HB) {
+    const int64_t nm = o->nm, nmax = o->nspfmax;
+    memset(HB, 0, sizeof(cplx) * o->nconf);
+    const int64_t big = mixed_size(o);
+    cplx *b1 = xcalloc(big, sizeof(cplx)), *b2 = xcalloc(big, sizeof(cplx));
+    int64_t sz[64];
+    for (int64_t t = 0; t < o->nham; t++) {
+        const int64_t s1 = o->lhs[t] - 1, s2 = o->rhs[t] - 1;
+        for (int64_t m = 0; m < nm; m++) sz[m] = NSPF(o, m, s2);
+        memcpy(b2, B + o->AStateOffset[s2], sizeof(cplx) * o->nAConf[s2]);
+        for (int64_t m = 0; m < nm; m++) {
+            int64_t op = o->iOper[m + nm * t] - 1;
+            if (o->IsIdentity[m + nm * op] && !o->gwp[m]) continue;
+            int64_t nS1 = NSPF(o, m, s1);
+            mode_product(nm, sz, m, H1(o, m, op), nmax, nS1, b2, b1);
+            cplx *sw = b1; b1 = b2; b2 = sw;
+            sz[m] = nS1;
+        }
+        cplx ae = o->ae[t];
+        cplx *out = HB + o->AStateOffset[s1];
+        for (int64_t i = 0; i < o->nAConf[s1]; i++) out[i] += ae * b2[i];
+    }
+    free(b1);
+    free(b2);
+}
+
 /* Calc_Der_SPF, DVR branch (cmf2_derivatives.f90:118-173); mode m is 0-based. */
 static int der_spf(orc *o, int64_t m, const cplx *psi0, cplx *dpsi) {
     const int64_t nmax = o->nspfmax, nG = o->ngpm[m], nm = o->nm;
@@ -1295,7 +1325,8 @@ static void std_reg_sol_tikh(int64_t n, const cplx *A, int64_t lda, const cplx *
  * reference does, leaving them in the shared arrays); rho and the mean
  * fields are those of the last calc_meanfield.  dpsi receives der_factor
  * times the regularised solution of C dxi = Y; dDD = -tau D. */
-static int der_spf_gwp(orc *o, int64_t m, const cplx *psi0, const cplx *DD0, cplx *dpsi, cplx *dDD) {
+static int der_spf_gwp(orc *o, int64_t m, const cplx *psi0, const cplx *DD0, cplx *dpsi, cplx *dDD,
+                       cplx *tau_out) {
     const int64_t nmax = o->nspfmax, nmd = o->nmodeDOF[m], nm = o->nm;
     int bad = 0;
     calc_s00(o, psi0, m);
@@ -1424,6 +1455,7 @@ static int der_spf_gwp(orc *o, int64_t m, const cplx *psi0, const cplx *DD0, cpl
             for (int64_t k = 0; k < nmd; k++) d += creal(x[k]) * creal(dx[j * nmd + k]) / GWPA(o, k, j, m);
             tau[j + nmax * j] += d * 0.5;
         }
+        if (tau_out) memcpy(DDP(o, tau_out, m, s), tau, sizeof(cplx) * NN(o));
         const cplx *D0 = DDP(o, DD0, m, s);
         cplx *dD = DDP(o, dDD, m, s);
         for (int64_t b = 0; b < nS; b++)
@@ -1520,7 +1552,7 @@ static const double dpD[7] = {5179.0 / 57600.0, 0, 7571.0 / 16695.0, 393.0 / 640
 /* Derivatives of the SPF block of mode m (and of its D matrices for a
  * Gaussian mode). */
 static int der_any(orc *o, int64_t m, const cplx *psi, const cplx *DD, cplx *dpsi, cplx *dDD) {
-    if (o->gwp[m]) return der_spf_gwp(o, m, psi, DD, dpsi, dDD);
+    if (o->gwp[m]) return der_spf_gwp(o, m, psi, DD, dpsi, dDD, NULL);
     return der_spf(o, m, psi, dpsi);
 }
 
@@ -1947,11 +1979,60 @@ int orc_propagate(orc *o, double t0, double tfinal, mg_output *outs, double *pop
  * dA = conj(Adot_tmp) = der_factor * H A (gmctdh_objects.f90:578,
  * derivatives.f90:156, 237); dpsi from the DVR formula of every mode
  * (derivatives.f90:81-143, the same as Calc_Der_SPF's). */
-static int calc_derivs(orc *o, const cplx *A0, const cplx *psi0, cplx *dA, cplx *dpsi) {
-    if (o->ngwp) {
-        snprintf(o->err, sizeof o->err, "VMF with Gaussian modes is not supported");
-        return -1;
+/* Calc_Derivs with Gaussian modes (derivatives.f90:61-89, 155-238):
+ * Gaussian-parameter derivatives from the regularised C/Y solve; dA =
+ * S^-1 (der_factor H A) - sum_m tau_m A with the Tikhonov S00^-1 on every
+ * Gaussian mode and tau_m of Calc_Der_SPF (the Gaussian-basis H includes
+ * S00 for the identity on Gaussian modes). */
+static int calc_derivs_gwp(orc *o, const cplx *A0, const cplx *psi0, cplx *dA, cplx *dpsi) {
+    const int64_t nm = o->nm, N = o->nconf;
+    const size_t nD = NN(o) * nm * o->ns;
+    for (int64_t m = 0; m < nm; m++)
+        if (o->gwp[m]) { calc_s00(o, psi0, m); calc_s00m1(o, m); }
+    calc_rho(o, A0);
+    for (int64_t m = 0; m < nm; m++) calc_h1mode_any(o, psi0, m);
+    if (calc_meanfield(o, A0) != 0) return -1;
+    memset(dpsi, 0, sizeof(cplx) * o->dimpsi);
+    int rc = 0;
+    for (int64_t m = 0; m < nm; m++) if (!o->gwp[m]) rc |= der_spf(o, m, psi0, dpsi);
+    cplx *Did = xcalloc(nD, sizeof(cplx)), *dD = xcalloc(nD, sizeof(cplx)), *tau = xcalloc(nD, sizeof(cplx));
+    for (int64_t m = 0; m < nm; m++)
+        for (int64_t s = 0; s < o->ns; s++)
+            for (int64_t j = 0; j < o->nspfmax; j++) DDP(o, Did, m, s)[j + o->nspfmax * j] = 1.0;
+    for (int64_t m = 0; m < nm; m++) {
+        if (!o->gwp[m]) continue;
+        /* der_spf_gwp recomputes S00, S00^-1 and h1mode of m from psi0: unchanged */
+        rc |= der_spf_gwp(o, m, psi0, Did, dpsi, dD, tau);
     }
+    /* dA = S^-1 (der_factor H A) over the Gaussian modes */
+    cplx *HA = xcalloc(N, sizeof(cplx)), *p1 = xcalloc(mixed_size(o), sizeof(cplx)), *p2 = xcalloc(mixed_size(o), sizeof(cplx));
+    apply_hg(o, A0, HA);
+    int64_t sz[64];
+    for (int64_t s = 0; s < o->ns; s++) {
+        for (int64_t m = 0; m < nm; m++) sz[m] = NSPF(o, m, s);
+        const int64_t n = o->nAConf[s];
+        for (int64_t i = 0; i < n; i++) p1[i] = o->der_factor * HA[o->AStateOffset[s] + i];
+        for (int64_t m = 0; m < nm; m++) {
+            if (!o->gwp[m]) continue;
+            const int64_t jel = o->MultiSet[m] ? s : 0;
+            mode_product(nm, sz, m, S00M1(o, m, jel), o->nspfmax, sz[m], p1, p2);
+            cplx *sw = p1; p1 = p2; p2 = sw;
+        }
+        memcpy(dA + o->AStateOffset[s], p1, sizeof(cplx) * n);
+        /* minus tau_m A for every Gaussian mode */
+        for (int64_t m = 0; m < nm; m++) {
+            if (!o->gwp[m]) continue;
+            const int64_t jel = o->MultiSet[m] ? s : 0;
+            mode_product(nm, sz, m, DDP(o, tau, m, jel), o->nspfmax, sz[m], A0 + o->AStateOffset[s], p2);
+            for (int64_t i = 0; i < n; i++) dA[o->AStateOffset[s] + i] -= p2[i];
+        }
+    }
+    free(Did); free(dD); free(tau); free(HA); free(p1); free(p2);
+    return rc;
+}
+
+static int calc_derivs(orc *o, const cplx *A0, const cplx *psi0, cplx *dA, cplx *dpsi) {
+    if (o->ngwp) return calc_derivs_gwp(o, A0, psi0, dA, dpsi);
     calc_rho(o, A0);
     for (int64_t m = 0; m < o->nm; m++) calc_h1mode_mode(o, psi0, m);
     if (calc_meanfield(o, A0) != 0) return -1;
@@ -2153,7 +2234,8 @@ int orc_der_spf_gwp(orc *o, int64_t im, const double *psi_in, const double *dd_i
     calc_rho(o, o->A);
     for (int64_t m = 0; m < o->nm; m++) calc_h1mode_any(o, o->psi, m);
     calc_meanfield(o, o->A);
-    return der_spf_gwp(o, im - 1, (const cplx *)psi_in, (const cplx *)dd_in, (cplx *)dpsi_out, (cplx *)ddd_out);
+    return der_spf_gwp(o, im - 1, (const cplx *)psi_in, (const cplx *)dd_in, (cplx *)dpsi_out, (cplx *)ddd_out,
+                       NULL);
 }
 /* A_to_B / B_to_A with the D matrices dd ([nspfmax, nspfmax, nmodes,
  * nstates]) and the overlaps of the current psi. */
@@ -2163,5 +2245,13 @@ void orc_a_to_b(orc *o, const double *A, double *B, const double *dd) {
 }
 void orc_b_to_a(orc *o, const double *B, double *A, const double *dd) {
     b_to_a(o, (const cplx *)B, (cplx *)A, (const cplx *)dd);
+}
+/* S00, h1mode, D0 and D^H h D of the current psi, so that orc_apply_h is
+ * Calc_DHD_B in the orthonormalised basis (cmf2_base.f90:72-96). */
+int orc_prepare_gwp(orc *o) {
+    int rc = init_dd(o, o->psi, o->DD);
+    calc_operators(o, o->psi);
+    calc_dhd(o, o->DD);
+    return rc;
 }
 

@@ -80,6 +80,7 @@ int orc_gwp_overlaps(orc *o, double *s00_out, double *dd_out);
 int orc_der_spf_gwp(orc *o, int64_t im, const double *psi_in, const double *dd_in, double *dpsi_out,
                     double *ddd_out);
 void orc_a_to_b(orc *o, const double *A, double *B, const double *dd);
+int orc_prepare_gwp(orc *o);
 void orc_b_to_a(orc *o, const double *B, double *A, const double *dd);
 
 #ifdef __cplusplus

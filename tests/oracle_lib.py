@@ -72,6 +72,8 @@ def lib():
         L.orc_der_spf_gwp.restype = C.c_int
         L.orc_der_spf_gwp.argtypes = [C.c_void_p, C.c_int64, f64p, f64p, f64p, f64p]
         L.orc_a_to_b.argtypes = [C.c_void_p, f64p, f64p, f64p]
+        L.orc_prepare_gwp.restype = C.c_int
+        L.orc_prepare_gwp.argtypes = [C.c_void_p]
         L.orc_b_to_a.argtypes = [C.c_void_p, f64p, f64p, f64p]
         _lib = L
     return _lib
@@ -239,6 +241,10 @@ class Oracle:
         if rc != 0:
             raise RuntimeError(lib().orc_error(self.h).decode())
         return dpsi, dD
+
+    def prepare_gwp(self):
+        if lib().orc_prepare_gwp(self.h) != 0:
+            raise RuntimeError(lib().orc_error(self.h).decode())
 
     def a_to_b(self, A, D):
         A = np.ascontiguousarray(A, np.complex128)

@@ -250,3 +250,33 @@ def test_gmctdh_conserves_norm_and_energy(name, tmp_path):
         assert abs(p.sum() - o.norm) < 1e-12
         assert abs(o.energy - outs[0].energy) < 1e-8
     assert pops[-1][0] > 1e-4            # population moves through the coupling
+
+
+@pytest.mark.parametrize("method", [1, 2])
+def test_vmf_coherent_state_is_exact(method, tmp_path):
+    """VMF G-MCTDH (derivatives.f90 Gaussian branch: S^-1 H A - tau A and the
+    C/Y parameter equations) on the displaced-oscillator coherent state."""
+    prob = _coherent_problem(str(tmp_path), 3)
+    w, k = 0.2 / AU2EV, 0.1 / AU2EV
+    outs, _, _ = Oracle(prob).propagate_vmf(method, 0.0, 3.0 / AU2FS)
+    b2 = (k / (w * np.sqrt(2))) ** 2
+    for x in outs:
+        t = x.time
+        c = np.exp(-b2 * (1 - np.exp(-1j * w * t))) * np.exp(-1j * (w / 2 - k * k / (2 * w)) * t)
+        assert abs(c - complex(x.auto_re, x.auto_im)) < 2e-7
+        assert abs(x.norm - 1.0) < 1e-8
+
+
+def test_vmf_matches_cmf2_on_coupled_case(tmp_path):
+    """VMF and CMF2 integrate the same equations of motion: on a coupled
+    two-state case with one Gaussian mode they agree to the integrators'
+    tolerances, and VMF conserves norm and energy."""
+    prob = make_problem("g1", str(tmp_path))
+    tf = 1.0 / AU2FS
+    oc, pc, _ = Oracle(prob).propagate(0.0, tf)
+    ov, pv, _ = Oracle(prob).propagate_vmf(2, 0.0, tf)
+    for a, b in zip(oc, ov):
+        assert abs(complex(a.auto_re, a.auto_im) - complex(b.auto_re, b.auto_im)) < 5e-7
+        assert abs(b.norm - 1.0) < 1e-7
+        assert abs(b.energy - ov[0].energy) < 1e-8
+    assert np.abs(pc - pv).max() < 5e-7

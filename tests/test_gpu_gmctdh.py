@@ -1,0 +1,207 @@
+"""G-MCTDH (Gaussian-SPF modes) on the HIP engine against the FP64 oracle,
+through the C-ABI.
+
+Bars: overlaps, D0, h1mode, density matrices, mean fields, A_to_B and
+D^H h D-based H.B are direct FP64 restatements: 1e-12 relative.  The
+Gaussian-parameter derivatives go through the Tikhonov-regularised solve of
+the nearly singular C matrix (eps_C = 1e-8 amplifies rounding of C by up to
+1 / (2 eps_C)): 1e-9 relative (measured 1e-14).  Trajectories: the oracle's
+attempt sequence replayed 1e-10 (measured 1e-14); free running 1e-7 (the
+accept/reject decisions sit on err_psi ~ eps_int = 1e-6, so a rounding-level
+flip changes the step sequence; measured 3e-10); the mixed form 1e-6.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from oracle_lib import Oracle
+from macaw_amd.lvc import make_problem
+
+gpu = pytest.mark.gpu
+AU2FS = 0.02418884326505
+CASES = ["g1", "g2", "g3"]
+
+
+def engine_for(prob, precision=0):
+    from macaw_amd.engine import Engine
+    return Engine(prob, precision=precision)
+
+
+def perturbed_A(prob, seed):
+    rng = np.random.default_rng(seed)
+    A = rng.normal(size=prob.nAConftot) + 1j * rng.normal(size=prob.nAConftot)
+    return A / np.linalg.norm(A)
+
+
+def rel(a, b):
+    return np.abs(a - b).max() / max(1e-300, np.abs(b).max())
+
+
+@gpu
+@pytest.mark.parametrize("name", CASES)
+def test_overlaps_h1mode_rho_meanfield(name, tmp_path):
+    prob = make_problem(name, str(tmp_path))
+    A = perturbed_A(prob, 1)
+    o, e = Oracle(prob), engine_for(prob)
+    o.set_state(A, prob.psi0)
+    e.set_state(A, prob.psi0)
+    So, Do = o.gwp_overlaps()
+    Se, De = e.gwp_overlaps()
+    # single-set modes carry one set: only their (0, 0) block is defined
+    for m in range(prob.nmodes):
+        if not prob.MultiSet[m]:
+            for S in (So, Se):
+                keep = S[0, 0, m].copy()
+                S[:, :, m] = 0
+                S[0, 0, m] = keep
+    assert rel(Se, So) < 1e-13 and rel(De, Do) < 1e-12
+    assert rel(e.h1mode(), o.h1mode()) < 1e-12
+    assert rel(e.rho(), o.rho()) < 1e-12
+    mfo, _, _ = o.meanfield()
+    mfe, _ = e.meanfield()
+    assert rel(mfe, mfo) < 1e-11
+
+
+@gpu
+@pytest.mark.parametrize("name", CASES)
+def test_a_to_b_and_apply_h(name, tmp_path):
+    prob = make_problem(name, str(tmp_path))
+    A = perturbed_A(prob, 2)
+    o, e = Oracle(prob), engine_for(prob)
+    _, D = o.gwp_overlaps()
+    assert rel(e.a_to_b(A, D), o.a_to_b(A, D)) < 1e-12
+    o.prepare_gwp()
+    e.prepare_gwp()
+    assert rel(e.apply_h(A), o.apply_h(A)) < 1e-12
+
+
+@gpu
+@pytest.mark.parametrize("name", CASES)
+def test_gaussian_derivatives(name, tmp_path):
+    prob = make_problem(name, str(tmp_path))
+    A = perturbed_A(prob, 3)
+    o, e = Oracle(prob), engine_for(prob)
+    o.set_state(A, prob.psi0)
+    e.set_state(A, prob.psi0)
+    _, D = o.gwp_overlaps()
+    for im in range(1, prob.nmodes + 1):
+        if not prob.gwp_mode(im - 1):
+            continue
+        dpo, dDo = o.der_spf_gwp(im, prob.psi0, D)
+        dpe, dDe = e.der_spf_gwp(im, prob.psi0, D)
+        ip, _ = prob.ipsiSPF()
+        sets = prob.nstates if prob.MultiSet[im - 1] else 1
+        for s in range(sets):
+            n = int(prob.nmodeDOF[im - 1] * prob.nSPF[im - 1, s])
+            blk = slice(ip[im - 1, s] - 1, ip[im - 1, s] - 1 + n)
+            print(name, im, s, "dxi rel", rel(dpe[blk], dpo[blk]))
+            assert rel(dpe[blk], dpo[blk]) < 1e-9
+            assert rel(dDe[s, im - 1], dDo[s, im - 1]) < 1e-9
+
+
+def max_errors(outs_e, pops_e, outs_o, pops_o):
+    assert len(outs_e) == len(outs_o)
+    dc = max(abs(complex(a.auto_re, a.auto_im) - complex(b.auto_re, b.auto_im)) for a, b in zip(outs_e, outs_o))
+    dn = max(abs(a.norm - b.norm) for a, b in zip(outs_e, outs_o))
+    dp = float(np.abs(pops_e - pops_o).max())
+    de = max(abs(a.energy - b.energy) for a, b in zip(outs_e, outs_o))
+    return dc, dn, dp, de
+
+
+@gpu
+@pytest.mark.parametrize("name,tf_fs", [("g1", 1.0), ("g2", 0.5), ("g3", 0.5)])
+def test_gmctdh_propagation_matches_oracle(name, tf_fs, tmp_path):
+    prob = make_problem(name, str(tmp_path))
+    tf = tf_fs / AU2FS
+    outs_o, pops_o, atts_o = Oracle(prob).propagate(0.0, tf)
+    e = engine_for(prob)
+    e.set_replay(atts_o)
+    outs_e, pops_e, atts_e = e.propagate(0.0, tf)
+    dc, dn, dp, de = max_errors(outs_e, pops_e, outs_o, pops_o)
+    print(f"{name}: replayed |dC|={dc:.2e} |dnorm|={dn:.2e} |dpop|={dp:.2e} |dE|={de:.2e}")
+    assert dc < 1e-10 and dn < 1e-10 and dp < 1e-10 and de < 1e-10
+    e2 = engine_for(prob)
+    outs_f, pops_f, atts_f = e2.propagate(0.0, tf)
+    dc, dn, dp, de = max_errors(outs_f, pops_f, outs_o, pops_o)
+    print(f"{name}: free-running |dC|={dc:.2e} attempts {len(atts_f)} vs {len(atts_o)}")
+    assert dc < 1e-7 and dn < 1e-7 and dp < 1e-7
+    e3 = engine_for(prob, precision=1)
+    outs_m, pops_m, _ = e3.propagate(0.0, tf)
+    dc, dn, dp, de = max_errors(outs_m, pops_m, outs_o, pops_o)
+    print(f"{name}: mixed |dC|={dc:.2e} |dpop|={dp:.2e}")
+    assert dc <= 1e-6 and dn <= 1e-6 and dp <= 1e-6
+
+
+@gpu
+def test_gmctdh_coherent_state_closed_form(tmp_path):
+    """The engine on the displaced-oscillator coherent state (closed form)."""
+    from test_gmctdh_pins import _coherent_problem
+    prob = _coherent_problem(str(tmp_path), 3)
+    w, k = 0.2 / 27.2113961, 0.1 / 27.2113961
+    outs, _, _ = engine_for(prob).propagate(0.0, 5.0 / AU2FS)
+    b2 = (k / (w * np.sqrt(2))) ** 2
+    for x in outs:
+        t = x.time
+        c = np.exp(-b2 * (1 - np.exp(-1j * w * t))) * np.exp(-1j * (w / 2 - k * k / (2 * w)) * t)
+        assert abs(c - complex(x.auto_re, x.auto_im)) < 2e-7
+
+
+@gpu
+def test_c3_short_window(tmp_path):
+    """The G-MCTDH workload (3 states, 3 Gaussian + 3 grid modes) over 0.2 fs:
+    FP64 and mixed engine against the oracle."""
+    prob = make_problem("c3", str(tmp_path))
+    prob.writetime = 0.1 / AU2FS
+    tf = 0.2 / AU2FS
+    outs_o, pops_o, atts_o = Oracle(prob).propagate(0.0, tf)
+    e = engine_for(prob)
+    e.set_replay(atts_o)
+    outs_e, pops_e, _ = e.propagate(0.0, tf)
+    dc, dn, dp, de = max_errors(outs_e, pops_e, outs_o, pops_o)
+    print(f"c3: replayed |dC|={dc:.2e} |dpop|={dp:.2e} |dE|={de:.2e}")
+    assert dc < 1e-10 and dp < 1e-10
+    outs_m, pops_m, _ = engine_for(prob, precision=1).propagate(0.0, tf)
+    dc, dn, dp, de = max_errors(outs_m, pops_m, outs_o, pops_o)
+    print(f"c3: mixed |dC|={dc:.2e} |dpop|={dp:.2e}")
+    assert dc <= 1e-6 and dp <= 1e-6
+
+
+@gpu
+@pytest.mark.parametrize("name", CASES)
+def test_vmf_derivatives_with_gaussian_modes(name, tmp_path):
+    """Calc_Derivs with Gaussian modes: dA = S^-1 (-i H_g A) - sum tau A and
+    the parameter derivatives (Tikhonov solves of C and S00): 1e-9."""
+    prob = make_problem(name, str(tmp_path))
+    A = perturbed_A(prob, 4)
+    o, e = Oracle(prob), engine_for(prob)
+    dA_o, dp_o = o.calc_derivs(A, prob.psi0)
+    e.set_state(A, prob.psi0)
+    dA_e, dp_e = e.calc_derivs()
+    print(name, "dA rel", rel(dA_e, dA_o), "dpsi rel", rel(dp_e, dp_o))
+    assert rel(dA_e, dA_o) < 1e-9
+    assert rel(dp_e, dp_o) < 1e-9
+
+
+@gpu
+@pytest.mark.parametrize("method", [1, 2])
+def test_vmf_propagation_with_gaussian_modes(method, tmp_path):
+    prob = make_problem("g3", str(tmp_path))
+    tf = 0.5 / AU2FS
+    outs_o, pops_o, atts_o = Oracle(prob).propagate_vmf(method, 0.0, tf)
+    e = engine_for(prob)
+    e.set_replay(atts_o)
+    outs_e, pops_e, _ = e.propagate_vmf(method, 0.0, tf)
+    dc, dn, dp, de = max_errors(outs_e, pops_e, outs_o, pops_o)
+    print(f"g3 vmf m{method}: replayed |dC|={dc:.2e} |dpop|={dp:.2e}")
+    assert dc < 1e-10 and dn < 1e-10 and dp < 1e-10
+    outs_f, pops_f, _ = engine_for(prob).propagate_vmf(method, 0.0, tf)
+    dc, dn, dp, de = max_errors(outs_f, pops_f, outs_o, pops_o)
+    print(f"g3 vmf m{method}: free-running |dC|={dc:.2e}")
+    # the VMF step controller compares DeltaPsi with eps_int = 1e-6 and a
+    # rounding-level flip of one decision moves C(t) by up to that much
+    assert dc < 1e-6 and dp < 1e-6
+    outs_m, pops_m, _ = engine_for(prob, precision=1).propagate_vmf(method, 0.0, tf)
+    dc, dn, dp, de = max_errors(outs_m, pops_m, outs_o, pops_o)
+    print(f"g3 vmf m{method}: mixed |dC|={dc:.2e}")
+    assert dc <= 1e-6 and dp <= 1e-6
