@@ -187,6 +187,31 @@ int mg_der_spf_gwp(mg_ctx *ctx, int64_t im, const double *psi_in, const double *
                    double *ddd_out);
 int mg_a_to_b(mg_ctx *ctx, const double *A, const double *dd, double *B);
 
+/* Learned CMF2 step controller (no counterpart in the reference; the
+ * reference's controller is cmf2_base.f90:178-199).  A 12-64-64-1 perceptron
+ * (SiLU, BF16 weights, FP32 accumulation) maps 12 step features to
+ * y = log(h_next / h) after each accepted step; the CMF2 error estimate still
+ * accepts or rejects every attempt and rejections use the classical rule.
+ * Parameters: MG_MLP_NPARAM floats (feature means and deviations, W1 [12x64],
+ * b1, W2 [64x64], b2, w3 [64], b3).  mg_mlp_train trains on n rows X[n x 12]
+ * -> t[n] (mean squared error, Adam, minibatches of `batch`, seeded
+ * shuffles) on the device; mg_mlp_forward / mg_mlp_grad evaluate the model
+ * and its loss gradient; mg_set_step_model(ctx, NULL) restores the classical
+ * controller.  mg_log_step_features records the features of every accepted
+ * step of the following propagations (mg_step_features reads them with the
+ * accepted step sizes). */
+#define MG_MLP_NFEAT 12
+#define MG_MLP_NPARAM 5081
+int mg_mlp_train(mg_ctx *ctx, const float *X, const float *t, int64_t n, int64_t epochs, int64_t batch, double lr,
+                 uint64_t seed, float *params, double *loss);
+int mg_mlp_forward(mg_ctx *ctx, const float *params, const float *X, int64_t n, float *y);
+int mg_mlp_grad(mg_ctx *ctx, const float *params, const float *X, const float *t, int64_t n, float *grad,
+                double *loss);
+int mg_set_step_model(mg_ctx *ctx, const float *params);
+int mg_log_step_features(mg_ctx *ctx, int64_t enable);
+int mg_step_features(mg_ctx *ctx, float *features, double *h, int64_t max, int64_t *n);
+int mg_controller_stats(mg_ctx *ctx, int64_t *proposals, int64_t *fallbacks);
+
 #ifdef __cplusplus
 }
 #endif

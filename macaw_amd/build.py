@@ -9,7 +9,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libmacaw_gpu.so")
-SOURCES = ["mg_kernels.hip", "mg_gwp.hip", "mg_engine.hip"]
+SOURCES = ["mg_kernels.hip", "mg_gwp.hip", "mg_mlp.hip", "mg_engine.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = ["--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",

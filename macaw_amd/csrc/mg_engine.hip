@@ -13,6 +13,7 @@
 #include <array>
 #include <complex>
 #include <map>
+#include <random>
 #include <memory>
 #include <cstdio>
 #include <cstdlib>
@@ -192,6 +193,16 @@ struct mg_ctx {
     DevBuf dgwpdesc, dgwpa, dS00, dSd, dDD, dDH, dDH2, dDobs, dhA, dhAf, dgwpscr, dABmats, dAH;
     // VMF with Gaussian modes: identity D, Tikhonov S00^-1 and tau per (mode, state), -tau per set
     DevBuf dDid, dSm1, dTauS, dTauSet;
+    // ---- learned step controller ----
+    DevBuf drmf_minw;                      // smallest rho eigenvalue per rho^-1 MF job
+    DevBuf dmf_prev, dmf_tmp;              // mean fields of the previous accepted step
+    bool mf_prev_valid = false;
+    DevBuf dmlp_p, dmlp_wb, dmlp_x, dmlp_y, dmlp_work;
+    bool mlp_on = false, log_features = false;
+    std::vector<double> hist_h, hist_e;    // last attempts, most recent first
+    std::vector<float> feat_log;           // 12 features per accepted step
+    std::vector<double> feat_h;            // the accepted step size of each entry
+    int64_t mlp_proposals = 0, mlp_fallbacks = 0;
 
     int nsets(int m) const { return MultiSet[m] ? ns : 1; }
     int64_t NS(int m, int s) const { return nSPF[m + (int64_t)nm * s]; }
@@ -743,6 +754,9 @@ void allocate(mg_ctx *c) {
         if (c->prec) c->dhAf.alloc(sizeof(float2) * nn * c->nopmax * c->nm);
         c->dAH.alloc(z * (size_t)c->nconf);
     }
+    c->drmf_minw.alloc(sizeof(double) * std::max(1, c->n_rmf_jobs));
+    c->dmf_prev.alloc(c->dmf.bytes);
+    c->dmf_tmp.alloc(c->dmf.bytes);
     // SPF product task scratch: 2 * nG * nspfmax per task
     int64_t maxng = 0;
     for (int m = 0; m < c->nm; m++)
@@ -1876,7 +1890,7 @@ void calc_meanfield_bricks(mg_ctx *c, const double2 *A64, bool af_ready) {
     launch_mat_combine(c->dmf_dst.as<double2 *>(), c->dmf_rowptr.as<int>(), c->dmf_src.as<const double2 *>(),
                        c->dmf_coef.as<double2>(), c->dmf_trans.as<int>(), c->nm * c->nopmax, c->nspfmax, c->stream);
     launch_rmf(c->drho.as<double2>(), c->dmf.as<double2>(), c->drmf.as<double2>(), c->drmf_jobs.as<int>(),
-               c->n_rmf_jobs, c->nspfmax, c->nopmax, c->eps_rho, c->stream);
+               c->n_rmf_jobs, c->nspfmax, c->nopmax, c->eps_rho, c->stream, c->drmf_minw.as<double>());
     check_launch();
 }
 
@@ -1990,7 +2004,7 @@ void calc_meanfield(mg_ctx *c, const double2 *A64, bool af_ready = false) {
         }
     }
     launch_rmf(c->drho.as<double2>(), c->dmf.as<double2>(), c->drmf.as<double2>(), c->drmf_jobs.as<int>(),
-               c->n_rmf_jobs, c->nspfmax, c->nopmax, c->eps_rho, c->stream);
+               c->n_rmf_jobs, c->nspfmax, c->nopmax, c->eps_rho, c->stream, c->drmf_minw.as<double>());
     check_launch();
 }
 
@@ -2181,6 +2195,105 @@ void copy_d2d(mg_ctx *c, double2 *dst, const double2 *src, int64_t n) {
     HIP_CHECK(hipMemcpyAsync(dst, src, sizeof(double2) * n, hipMemcpyDeviceToDevice, c->stream));
 }
 
+// ----------------------------------------------------------------------------
+// Learned step controller (no counterpart in the reference).  After an
+// accepted CMF2 step the 12 features below describe the step history and the
+// wavefunction; a 12-64-64-1 perceptron (mg_mlp.hip) maps them to
+// y = log(h_next / h).  The CMF2 error estimate still accepts or rejects
+// every attempt unchanged and rejections use the classical rule, so the
+// model can only change how often attempts are rejected, never the accuracy
+// criterion.
+void note_attempt(mg_ctx *c, double h, double error) {
+    c->hist_h.insert(c->hist_h.begin(), h);
+    c->hist_e.insert(c->hist_e.begin(), error);
+    if (c->hist_h.size() > 4) { c->hist_h.pop_back(); c->hist_e.pop_back(); }
+}
+
+// f0..f3 log h of the last 4 attempts (most recent first, padded with the
+// oldest), f4..f7 log10 of their error ratios err / eps_int, f8 log10 of the
+// relative change of the mean fields since the previous accepted step, f9
+// log10 of the smallest eigenvalue of the grid modes' density matrices, f10
+// log10 |tr rho(mode 1) - 1| (norm drift), f11 the most populated state
+// / (nstates - 1).  rho and the mean fields are those of the step's half
+// step, the last ones the step evaluated.
+void step_features(mg_ctx *c, float *f) {
+    const size_t nh = c->hist_h.size();
+    for (int i = 0; i < 4; i++) {
+        const size_t k = std::min<size_t>(i, nh - 1);
+        f[i] = (float)std::log(c->hist_h[k]);
+        f[4 + i] = (float)std::log10(std::max(c->hist_e[k], 1e-12));
+    }
+    const int64_t nmf = (int64_t)(c->dmf.bytes / sizeof(double2));
+    double rel = 0.0;
+    if (c->mf_prev_valid) {
+        HIP_CHECK(hipMemcpyAsync(c->dmf_tmp.p, c->dmf.p, c->dmf.bytes, hipMemcpyDeviceToDevice, c->stream));
+        launch_axpby(c->dmf_tmp.p, c->dmf_prev.p, -1.0, 1.0, nmf, 0, c->stream);
+        const double d = norm2(c, c->dmf_tmp.p, nmf), t = norm2(c, c->dmf.p, nmf);
+        rel = t > 0 ? std::sqrt(d / t) : 0.0;
+    }
+    HIP_CHECK(hipMemcpyAsync(c->dmf_prev.p, c->dmf.p, c->dmf.bytes, hipMemcpyDeviceToDevice, c->stream));
+    c->mf_prev_valid = true;
+    f[8] = (float)std::log10(rel + 1e-12);
+    double mw = 1.0;
+    if (c->n_rmf_jobs > 0) {
+        std::vector<double> w(c->n_rmf_jobs);
+        HIP_CHECK(hipMemcpyAsync(w.data(), c->drmf_minw.p, sizeof(double) * w.size(), hipMemcpyDeviceToHost, c->stream));
+        HIP_CHECK(hipStreamSynchronize(c->stream));
+        mw = *std::min_element(w.begin(), w.end());
+    }
+    f[9] = (float)std::log10(std::max(mw, 1e-30));
+    const size_t nn = (size_t)c->nspfmax * c->nspfmax;
+    std::vector<double2> r(nn * c->nm * c->ns);
+    HIP_CHECK(hipMemcpyAsync(r.data(), c->drho.p, sizeof(double2) * r.size(), hipMemcpyDeviceToHost, c->stream));
+    HIP_CHECK(hipStreamSynchronize(c->stream));
+    double tot = 0.0, best = -1.0;
+    int arg = 0;
+    for (int s = 0; s < c->ns; s++) {
+        double p = 0.0;
+        for (int64_t j = 0; j < c->NS(0, s); j++) p += r[nn * ((size_t)c->nm * s) + j + (size_t)c->nspfmax * j].x;
+        tot += p;
+        if (p > best) { best = p; arg = s; }
+    }
+    f[10] = (float)std::log10(std::fabs(tot - 1.0) + 1e-16);
+    f[11] = c->ns > 1 ? (float)arg / (float)(c->ns - 1) : 0.0f;
+}
+
+// Next step after an accepted attempt (hdid, error ratio): the classical
+// rule min(h / sqrt(err), 1.5 h) or the model's proposal h exp(y) clamped to
+// [0.2 h, 2 h]; both capped at the output interval.
+double propose_next(mg_ctx *c, double hdid, double error) {
+    const double classical = std::min(hdid / std::sqrt(error), hdid * 1.5);
+    double hn = classical;
+    if (c->mlp_on || c->log_features) {
+        float f[kMlpNFeat];
+        step_features(c, f);
+        if (c->log_features) {
+            c->feat_log.insert(c->feat_log.end(), f, f + kMlpNFeat);
+            c->feat_h.push_back(hdid);
+        }
+        if (c->mlp_on) {
+            HIP_CHECK(hipMemcpyAsync(c->dmlp_x.p, f, sizeof f, hipMemcpyHostToDevice, c->stream));
+            float *w = c->dmlp_work.as<float>();
+            MlpWork mw{w, w + kMlpHidden, w + 2 * kMlpHidden, w + 3 * kMlpHidden, w + 4 * kMlpHidden,
+                       w + 5 * kMlpHidden, w + 6 * kMlpHidden, w + 6 * kMlpHidden + 1};
+            launch_mlp_forward(c->dmlp_x.as<float>(), c->dmlp_p.as<float>(), c->dmlp_wb.as<uint16_t>(), mw,
+                               c->dmlp_y.as<float>(), nullptr, 1, c->stream);
+            check_launch();
+            float y = 0.f;
+            HIP_CHECK(hipMemcpyAsync(&y, c->dmlp_y.p, sizeof y, hipMemcpyDeviceToHost, c->stream));
+            HIP_CHECK(hipStreamSynchronize(c->stream));
+            const double cand = hdid * std::exp((double)y);
+            if (std::isfinite(cand) && cand > 0.0) {
+                hn = std::min(std::max(cand, 0.2 * hdid), 2.0 * hdid);
+                c->mlp_proposals++;
+            } else {
+                c->mlp_fallbacks++;
+            }
+        }
+    }
+    return std::min(hn, c->writetime);
+}
+
 void cmf2_step_gwp(mg_ctx *c, double tt, double htry, double *hdid_out, double *hnext_out,
                    std::vector<mg_attempt> *log);
 
@@ -2230,6 +2343,7 @@ void cmf2_step(mg_ctx *c, double tt, double htry, double *hdid_out, double *hnex
         const double error = std::max(err_A, err_psi) / c->eps_int;
         const bool reject = (forced >= 0) ? (forced == 0) : (error > 1.0);
         if (log) log->push_back({tt, hdid, err_A, err_psi, (int64_t)(!reject), nk1, nk2});
+        note_attempt(c, hdid, error);
         if (reject) {
             hdid = hdid / std::sqrt(error) * safe;
             if (hdid < c->min_stepsize) throw HipError{"CMF2 step size fell below minstep"};
@@ -2238,9 +2352,8 @@ void cmf2_step(mg_ctx *c, double tt, double htry, double *hdid_out, double *hnex
         lanczos(c, BH, B0, 0.5 * hdid, false);
         copy_d2d(c, A, B0, N);
         copy_d2d(c, psi, pH2, D);
-        double hn = std::min(hdid / std::sqrt(error), hdid * maxscal);
-        hn = std::min(hn, c->writetime);
-        *hnext_out = hn;
+        (void)maxscal;
+        *hnext_out = propose_next(c, hdid, error);
         break;
     }
     *hdid_out = hdid;
@@ -2302,6 +2415,7 @@ void cmf2_step_gwp(mg_ctx *c, double tt, double htry, double *hdid_out, double *
         const double error = std::max(err_A, err_psi) / c->eps_int;
         const bool reject = (forced >= 0) ? (forced == 0) : (error > 1.0);
         if (log) log->push_back({tt, hdid, err_A, err_psi, (int64_t)(!reject), nk1, nk2});
+        note_attempt(c, hdid, error);
         if (reject) {
             hdid = hdid / std::sqrt(error) * safe;
             if (hdid < c->min_stepsize) throw HipError{"CMF2 step size fell below minstep"};
@@ -2311,9 +2425,8 @@ void cmf2_step_gwp(mg_ctx *c, double tt, double htry, double *hdid_out, double *
         b_to_a(c, B0, A, DH2);
         copy_d2d(c, psi, pH2, D);
         copy_d2d(c, DD, DH2, (int64_t)nD);
-        double hn = std::min(hdid / std::sqrt(error), hdid * maxscal);
-        hn = std::min(hn, c->writetime);
-        *hnext_out = hn;
+        (void)maxscal;
+        *hnext_out = propose_next(c, hdid, error);
         break;
     }
     *hdid_out = hdid;
@@ -2818,6 +2931,205 @@ int mg_a_to_b(mg_ctx *c, const double *A, const double *dd, double *B) {
         a_to_b(c, c->dIn.as<double2>(), c->dOut.as<double2>(), din);
         HIP_CHECK(hipMemcpyAsync(B, c->dOut.p, sizeof(double2) * c->nconf, hipMemcpyDeviceToHost, c->stream));
         HIP_CHECK(hipStreamSynchronize(c->stream));
+    });
+}
+
+// ---- learned step controller ----
+static_assert(kMlpNParam == MG_MLP_NPARAM && kMlpNFeat == MG_MLP_NFEAT, "MLP parameter layout");
+namespace {
+struct MlpDev {
+    DevBuf p, wb, X, t, y, work, grad, m, v;
+    int cap = 0;
+    MlpWork w;
+    void reserve(int B) {
+        if (B <= cap) return;
+        cap = B;
+        X.alloc(sizeof(float) * (size_t)kMlpNFeat * B);
+        t.alloc(sizeof(float) * (size_t)B);
+        y.alloc(sizeof(float) * (size_t)B);
+        work.alloc(sizeof(float) * ((size_t)6 * kMlpHidden * B + 2 * (size_t)B));
+        float *f = work.as<float>();
+        const size_t hb = (size_t)kMlpHidden * B;
+        w = MlpWork{f, f + hb, f + 2 * hb, f + 3 * hb, f + 4 * hb, f + 5 * hb, f + 6 * hb, f + 6 * hb + B};
+    }
+    void init_params(const float *host) {
+        p.alloc(sizeof(float) * kMlpNParam);
+        wb.alloc(sizeof(uint16_t) * kMlpNWeights);
+        grad.alloc(sizeof(float) * kMlpNParam);
+        m.alloc(sizeof(float) * kMlpNParam);
+        v.alloc(sizeof(float) * kMlpNParam);
+        HIP_CHECK(hipMemcpy(p.p, host, sizeof(float) * kMlpNParam, hipMemcpyHostToDevice));
+        HIP_CHECK(hipMemset(m.p, 0, m.bytes));
+        HIP_CHECK(hipMemset(v.p, 0, v.bytes));
+    }
+};
+}  // namespace
+
+int mg_mlp_forward(mg_ctx *c, const float *params, const float *X, int64_t n, float *y) {
+    return guarded(c, [&] {
+        MlpDev d;
+        d.init_params(params);
+        const int B = (int)std::min<int64_t>(std::max<int64_t>(n, 1), 4096);
+        d.reserve(B);
+        launch_mlp_bf16(d.p.as<float>(), d.wb.as<uint16_t>(), c->stream);
+        for (int64_t o = 0; o < n; o += B) {
+            const int b = (int)std::min<int64_t>(B, n - o);
+            HIP_CHECK(hipMemcpyAsync(d.X.p, X + kMlpNFeat * o, sizeof(float) * kMlpNFeat * b, hipMemcpyHostToDevice,
+                                     c->stream));
+            launch_mlp_forward(d.X.as<float>(), d.p.as<float>(), d.wb.as<uint16_t>(), d.w, d.y.as<float>(), nullptr, b,
+                               c->stream);
+            HIP_CHECK(hipMemcpyAsync(y + o, d.y.p, sizeof(float) * b, hipMemcpyDeviceToHost, c->stream));
+        }
+        check_launch();
+        HIP_CHECK(hipStreamSynchronize(c->stream));
+    });
+}
+
+int mg_mlp_grad(mg_ctx *c, const float *params, const float *X, const float *t, int64_t n, float *grad,
+                double *loss) {
+    return guarded(c, [&] {
+        if (n < 1 || n > 65536) throw HipError{"mg_mlp_grad: 1 <= n <= 65536"};
+        MlpDev d;
+        d.init_params(params);
+        d.reserve((int)n);
+        HIP_CHECK(hipMemcpyAsync(d.X.p, X, sizeof(float) * kMlpNFeat * n, hipMemcpyHostToDevice, c->stream));
+        HIP_CHECK(hipMemcpyAsync(d.t.p, t, sizeof(float) * n, hipMemcpyHostToDevice, c->stream));
+        launch_mlp_bf16(d.p.as<float>(), d.wb.as<uint16_t>(), c->stream);
+        launch_mlp_forward(d.X.as<float>(), d.p.as<float>(), d.wb.as<uint16_t>(), d.w, d.y.as<float>(), d.t.as<float>(),
+                           (int)n, c->stream);
+        launch_mlp_backward_grad(d.X.as<float>(), d.p.as<float>(), d.wb.as<uint16_t>(), d.w, d.grad.as<float>(),
+                                 (int)n, c->stream);
+        check_launch();
+        std::vector<float> e(n);
+        const int off = 2 * kMlpNFeat;
+        for (int i = 0; i < off; i++) grad[i] = 0.0f;
+        HIP_CHECK(hipMemcpyAsync(grad + off, d.grad.p, sizeof(float) * (kMlpNParam - off), hipMemcpyDeviceToHost,
+                                 c->stream));
+        HIP_CHECK(hipMemcpyAsync(e.data(), d.w.err2, sizeof(float) * n, hipMemcpyDeviceToHost, c->stream));
+        HIP_CHECK(hipStreamSynchronize(c->stream));
+        double sum = 0.0;
+        for (float v : e) sum += v;
+        if (loss) *loss = sum / (double)n;
+    });
+}
+
+int mg_mlp_train(mg_ctx *c, const float *X, const float *t, int64_t n, int64_t epochs, int64_t batch, double lr,
+                 uint64_t seed, float *params, double *loss_out) {
+    return guarded(c, [&] {
+        if (n < 2 || epochs < 1 || batch < 1 || batch > 65536) throw HipError{"mg_mlp_train: bad sizes"};
+        const int F = kMlpNFeat, Hd = kMlpHidden;
+        std::vector<float> P(kMlpNParam, 0.0f);
+        // standardisation
+        for (int i = 0; i < F; i++) {
+            double mu = 0.0, sq = 0.0;
+            for (int64_t r = 0; r < n; r++) mu += X[F * r + i];
+            mu /= (double)n;
+            for (int64_t r = 0; r < n; r++) sq += (X[F * r + i] - mu) * (X[F * r + i] - mu);
+            P[i] = (float)mu;
+            P[F + i] = (float)std::max(std::sqrt(sq / (double)n), 1e-3);
+        }
+        // Glorot-uniform weights, zero biases, output bias at the target mean
+        std::mt19937_64 rng(seed);
+        auto uni = [&](double a) { return (float)(a * (2.0 * ((double)(rng() >> 11) / 9007199254740992.0) - 1.0)); };
+        int o = 2 * F;
+        for (int i = 0; i < F * Hd; i++) P[o + i] = uni(std::sqrt(6.0 / (F + Hd)));
+        o += F * Hd + Hd;
+        for (int i = 0; i < Hd * Hd; i++) P[o + i] = uni(std::sqrt(6.0 / (Hd + Hd)));
+        o += Hd * Hd + Hd;
+        for (int i = 0; i < Hd; i++) P[o + i] = uni(std::sqrt(6.0 / (Hd + 1)));
+        double tm = 0.0;
+        for (int64_t r = 0; r < n; r++) tm += t[r];
+        P[kMlpNParam - 1] = (float)(tm / (double)n);
+        MlpDev d;
+        d.init_params(P.data());
+        const int B = (int)std::min<int64_t>(batch, n);
+        d.reserve(B);
+        DevBuf dXall, dTall;
+        dXall.alloc(sizeof(float) * F * n);
+        dTall.alloc(sizeof(float) * n);
+        std::vector<int64_t> perm(n);
+        for (int64_t r = 0; r < n; r++) perm[r] = r;
+        std::vector<float> Xp((size_t)F * n), Tp(n);
+        int step = 0;
+        for (int64_t ep = 0; ep < epochs; ep++) {
+            for (int64_t r = n - 1; r > 0; r--) std::swap(perm[r], perm[(int64_t)(rng() % (uint64_t)(r + 1))]);
+            for (int64_t r = 0; r < n; r++) {
+                std::memcpy(&Xp[(size_t)F * r], X + (size_t)F * perm[r], sizeof(float) * F);
+                Tp[r] = t[perm[r]];
+            }
+            HIP_CHECK(hipMemcpyAsync(dXall.p, Xp.data(), sizeof(float) * F * n, hipMemcpyHostToDevice, c->stream));
+            HIP_CHECK(hipMemcpyAsync(dTall.p, Tp.data(), sizeof(float) * n, hipMemcpyHostToDevice, c->stream));
+            for (int64_t b0 = 0; b0 < n; b0 += B) {
+                const int b = (int)std::min<int64_t>(B, n - b0);
+                const float *Xb = dXall.as<float>() + (size_t)F * b0, *Tb = dTall.as<float>() + b0;
+                launch_mlp_bf16(d.p.as<float>(), d.wb.as<uint16_t>(), c->stream);
+                launch_mlp_forward(Xb, d.p.as<float>(), d.wb.as<uint16_t>(), d.w, d.y.as<float>(), Tb, b, c->stream);
+                launch_mlp_backward_grad(Xb, d.p.as<float>(), d.wb.as<uint16_t>(), d.w, d.grad.as<float>(), b,
+                                         c->stream);
+                launch_mlp_adam(d.p.as<float>(), d.grad.as<float>(), d.m.as<float>(), d.v.as<float>(), (float)lr,
+                                ++step, c->stream);
+            }
+            check_launch();
+            HIP_CHECK(hipStreamSynchronize(c->stream));
+        }
+        // final mean squared error over the training set
+        launch_mlp_bf16(d.p.as<float>(), d.wb.as<uint16_t>(), c->stream);
+        double sum = 0.0;
+        std::vector<float> e(B);
+        for (int64_t b0 = 0; b0 < n; b0 += B) {
+            const int b = (int)std::min<int64_t>(B, n - b0);
+            launch_mlp_forward(dXall.as<float>() + (size_t)F * b0, d.p.as<float>(), d.wb.as<uint16_t>(), d.w,
+                               d.y.as<float>(), dTall.as<float>() + b0, b, c->stream);
+            HIP_CHECK(hipMemcpyAsync(e.data(), d.w.err2, sizeof(float) * b, hipMemcpyDeviceToHost, c->stream));
+            HIP_CHECK(hipStreamSynchronize(c->stream));
+            for (int i = 0; i < b; i++) sum += e[i];
+        }
+        if (loss_out) *loss_out = sum / (double)n;
+        HIP_CHECK(hipMemcpy(params, d.p.p, sizeof(float) * kMlpNParam, hipMemcpyDeviceToHost));
+    });
+}
+
+int mg_set_step_model(mg_ctx *c, const float *params) {
+    return guarded(c, [&] {
+        if (!params) {
+            c->mlp_on = false;
+            return;
+        }
+        c->dmlp_p.alloc(sizeof(float) * kMlpNParam);
+        c->dmlp_wb.alloc(sizeof(uint16_t) * kMlpNWeights);
+        c->dmlp_x.alloc(sizeof(float) * kMlpNFeat);
+        c->dmlp_y.alloc(sizeof(float) * 4);
+        c->dmlp_work.alloc(sizeof(float) * (6 * kMlpHidden + 2));
+        HIP_CHECK(hipMemcpy(c->dmlp_p.p, params, sizeof(float) * kMlpNParam, hipMemcpyHostToDevice));
+        launch_mlp_bf16(c->dmlp_p.as<float>(), c->dmlp_wb.as<uint16_t>(), c->stream);
+        check_launch();
+        HIP_CHECK(hipStreamSynchronize(c->stream));
+        c->mlp_on = true;
+        c->mlp_proposals = c->mlp_fallbacks = 0;
+    });
+}
+
+int mg_log_step_features(mg_ctx *c, int64_t enable) {
+    return guarded(c, [&] {
+        c->log_features = enable != 0;
+        c->feat_log.clear();
+        c->feat_h.clear();
+    });
+}
+
+int mg_step_features(mg_ctx *c, float *feat_out, double *h_out, int64_t max, int64_t *n) {
+    return guarded(c, [&] {
+        const int64_t k = std::min<int64_t>(max, (int64_t)c->feat_h.size());
+        if (feat_out) std::memcpy(feat_out, c->feat_log.data(), sizeof(float) * kMlpNFeat * k);
+        if (h_out) std::memcpy(h_out, c->feat_h.data(), sizeof(double) * k);
+        *n = k;
+    });
+}
+
+int mg_controller_stats(mg_ctx *c, int64_t *proposals, int64_t *fallbacks) {
+    return guarded(c, [&] {
+        *proposals = c->mlp_proposals;
+        *fallbacks = c->mlp_fallbacks;
     });
 }
 

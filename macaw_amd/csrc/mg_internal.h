@@ -261,6 +261,22 @@ __device__ inline void dev_herm_eig(int n, double2 *A, double *w, double2 *U) {
 }
 #endif
 
+// Learned step controller (mg_mlp.hip): 12 -> 64 -> 64 -> 1 perceptron.
+constexpr int kMlpNFeat = 12;
+constexpr int kMlpHidden = 64;
+constexpr int kMlpNParam = 2 * kMlpNFeat + kMlpNFeat * kMlpHidden + kMlpHidden + kMlpHidden * kMlpHidden +
+                           kMlpHidden + kMlpHidden + 1;
+constexpr int kMlpNWeights = kMlpNFeat * kMlpHidden + kMlpHidden * kMlpHidden + kMlpHidden;
+struct MlpWork {                 // per-sample activations and gradients, batch-sized
+    float *z1, *h1, *z2, *h2, *dz1, *dz2, *dy, *err2;
+};
+void launch_mlp_bf16(const float *p, uint16_t *wb, hipStream_t s);
+void launch_mlp_forward(const float *X, const float *p, const uint16_t *wb, MlpWork w, float *y, const float *t,
+                        int B, hipStream_t s);
+void launch_mlp_backward_grad(const float *X, const float *p, const uint16_t *wb, MlpWork w, float *grad, int B,
+                              hipStream_t s);
+void launch_mlp_adam(float *p, const float *grad, float *m, float *v, float lr, int step, hipStream_t s);
+
 // Gaussian-mode launchers (mg_gwp.hip).  dd: D matrices [nspfmax^2, nmodes, nstates].
 size_t gwp_scratch_elems(const GwpDesc &d, int nspfmax);
 void launch_gwp_s00(const GwpDesc *d_descs, int ndesc, const double2 *psi, GwpParams p, double2 *s00,
@@ -322,8 +338,8 @@ void launch_rk_combine(double2 *out, const double2 *y0, const void *const *k, co
 void launch_hsum(double2 *hsum, const double2 *h1, const int *rowptr, const int *src,
                  const double2 *coef, int nmat, int nmax, hipStream_t s);
 int group_capacity(int brick);
-void launch_rmf(const double2 *rho, const double2 *mf, double2 *rmf, const int *jobs, int njobs,
-                int nspfmax, int nopmax, double eps, hipStream_t s);
+void launch_rmf(const double2 *rho, const double2 *mf, double2 *rmf, const int *jobs, int njobs, int nspfmax,
+                int nopmax, double eps, hipStream_t s, double *minw = nullptr);
 void launch_spf_propagate(const SpfModeDesc *d_modes, int nmodes, double2 *psi, double2 *scratch,
                           SpfParams p, DofTables dt, int *d_status, hipStream_t s);
 // mode < 0: every mode, -mode workgroups (pass -nmodes)

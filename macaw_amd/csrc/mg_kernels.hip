@@ -1200,8 +1200,9 @@ void launch_mat_axpy(double2 *dst, const double2 *src, double2 alpha, int n, int
 // Calc_rhom1MF (gmctdh_objects.f90:1147-1194): per job (mode, set) the
 // 'poly'-regularised inverse of rho (maths.f90:8-83) times each mean field.
 // job layout: [mode, rho_index, n, nops, (op, n2) x nops], stride 36.
+// minw (optional): the smallest eigenvalue of each job's rho.
 __global__ void k_rmf(const double2 *rho, const double2 *mf, double2 *rmf, const int *jobs, int njobs,
-                      int nspfmax, int nopmax, double eps) {
+                      int nspfmax, int nopmax, double eps, double *minw) {
     const int jb = blockIdx.x * blockDim.x + threadIdx.x;
     if (jb >= njobs) return;
     const int *J = jobs + 36 * jb;
@@ -1213,6 +1214,11 @@ __global__ void k_rmf(const double2 *rho, const double2 *mf, double2 *rmf, const
     for (int j = 0; j < n; j++)
         for (int i = 0; i < n; i++) A[i + kSmall * j] = r[i + nspfmax * j];
     dev_herm_eig(n, A, w, U);
+    if (minw) {
+        double mn = w[0];
+        for (int j = 1; j < n; j++) mn = fmin(mn, w[j]);
+        minw[jb] = mn;
+    }
     for (int j = 0; j < n; j++) {
         double v = w[j];
         if (v < eps) v = 0.5 * (eps + v * v / eps);
@@ -1241,10 +1247,10 @@ __global__ void k_rmf(const double2 *rho, const double2 *mf, double2 *rmf, const
 }
 
 void launch_rmf(const double2 *rho, const double2 *mf, double2 *rmf, const int *jobs, int njobs, int nspfmax,
-                int nopmax, double eps, hipStream_t s) {
+                int nopmax, double eps, hipStream_t s, double *minw) {
     if (njobs <= 0) return;
     hipLaunchKernelGGL(k_rmf, dim3((njobs + 63) / 64), dim3(64), 0, s, rho, mf, rmf, jobs, njobs, nspfmax, nopmax,
-                       eps);
+                       eps, minw);
 }
 
 // ---------------------------------------------------------------------------

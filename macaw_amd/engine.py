@@ -63,6 +63,15 @@ def lib():
         L.mg_gwp_overlaps.argtypes = [vp, f64p, f64p]
         L.mg_der_spf_gwp.argtypes = [vp, C.c_int64, f64p, f64p, f64p, f64p]
         L.mg_a_to_b.argtypes = [vp, f64p, f64p, f64p]
+        f32p = C.POINTER(C.c_float)
+        L.mg_mlp_train.argtypes = [vp, f32p, f32p, C.c_int64, C.c_int64, C.c_int64, C.c_double, C.c_uint64, f32p,
+                                   f64p]
+        L.mg_mlp_forward.argtypes = [vp, f32p, f32p, C.c_int64, f32p]
+        L.mg_mlp_grad.argtypes = [vp, f32p, f32p, f32p, C.c_int64, f32p, f64p]
+        L.mg_set_step_model.argtypes = [vp, f32p]
+        L.mg_log_step_features.argtypes = [vp, C.c_int64]
+        L.mg_step_features.argtypes = [vp, f32p, f64p, C.c_int64, i64p]
+        L.mg_controller_stats.argtypes = [vp, i64p, i64p]
         _lib = L
     return _lib
 
@@ -250,4 +259,60 @@ class Engine:
         B = np.zeros_like(A)
         self._check(lib().mg_a_to_b(self.h, cplx_ptr(A), cplx_ptr(D), cplx_ptr(B)))
         return B
+
+    # ---- learned step controller ----
+    def mlp_train(self, X, t, epochs=30, batch=256, lr=1e-3, seed=0):
+        X = np.ascontiguousarray(X, np.float32)
+        t = np.ascontiguousarray(t, np.float32)
+        params = np.zeros(MLP_NPARAM, np.float32)
+        loss = C.c_double(0)
+        self._check(lib().mg_mlp_train(self.h, _f32(X), _f32(t), len(t), epochs, batch, lr, seed, _f32(params),
+                                       C.byref(loss)))
+        return params, loss.value
+
+    def mlp_forward(self, params, X):
+        X = np.ascontiguousarray(X, np.float32)
+        params = np.ascontiguousarray(params, np.float32)
+        y = np.zeros(X.shape[0], np.float32)
+        self._check(lib().mg_mlp_forward(self.h, _f32(params), _f32(X), X.shape[0], _f32(y)))
+        return y
+
+    def mlp_grad(self, params, X, t):
+        X = np.ascontiguousarray(X, np.float32)
+        t = np.ascontiguousarray(t, np.float32)
+        params = np.ascontiguousarray(params, np.float32)
+        g = np.zeros(MLP_NPARAM, np.float32)
+        loss = C.c_double(0)
+        self._check(lib().mg_mlp_grad(self.h, _f32(params), _f32(X), _f32(t), len(t), _f32(g), C.byref(loss)))
+        return g, loss.value
+
+    def set_step_model(self, params):
+        if params is None:
+            self._check(lib().mg_set_step_model(self.h, None))
+            return
+        self._model = np.ascontiguousarray(params, np.float32)
+        self._check(lib().mg_set_step_model(self.h, _f32(self._model)))
+
+    def log_step_features(self, enable=True):
+        self._check(lib().mg_log_step_features(self.h, 1 if enable else 0))
+
+    def step_features(self, max_rows=1000000):
+        f = np.zeros((max_rows, MLP_NFEAT), np.float32)
+        h = np.zeros(max_rows)
+        n = C.c_int64(0)
+        self._check(lib().mg_step_features(self.h, _f32(f), h.ctypes.data_as(f64p), max_rows, C.byref(n)))
+        return f[: n.value].copy(), h[: n.value].copy()
+
+    def controller_stats(self):
+        a, b = C.c_int64(0), C.c_int64(0)
+        self._check(lib().mg_controller_stats(self.h, C.byref(a), C.byref(b)))
+        return a.value, b.value
+
+
+MLP_NFEAT = 12
+MLP_NPARAM = 5081
+
+
+def _f32(a):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
 
