@@ -9,7 +9,10 @@ intervals are timed.  value = femtoseconds propagated per wall second by the
 mixed-precision engine (complex FP32 H.A / Krylov / mean-field chains, FP64
 state, SPFs and reductions).  The same W+K intervals are then run by the FP64
 engine; max|dC(t)| over every output time is the accuracy gate (<= 1e-6) and
-the FP64 rate is reported beside it.
+the FP64 rate is reported beside it.  Both runs take their step proposals
+from the learned CMF2 controller (macaw_amd/step_controller.npy; the CMF2
+error test still accepts or rejects every attempt) unless --controller
+classical; a third run with the classical controller is reported beside them.
 
 Also reported: the roofline of the dominant kernel (the grouped mode-product
 pass of H.A) timed with a HIP event pair around every launch on the engine's
@@ -111,11 +114,21 @@ def measured_traffic(workload, precision, launches_per_ha):
     return t
 
 
-def run_engine(prob, precision, warmup, steps, autotune=True):
+def step_model(kind):
+    """Network of the learned CMF2 step controller, or None (classical)."""
+    if kind == "classical":
+        return None
+    import numpy as np
+    return np.load(os.path.join(ROOT, "macaw_amd", "step_controller.npy"))
+
+
+def run_engine(prob, precision, warmup, steps, autotune=True, model=None, tag=None):
     from macaw_amd.engine import Engine
     tout = prob.writetime
     eng = Engine(prob, precision=precision)
-    tag = "mixed" if precision else "fp64"
+    if model is not None:
+        eng.set_step_model(model)
+    tag = tag or ("mixed" if precision else "fp64")
     brick = None
     if autotune:
         t_a = time.perf_counter()
@@ -152,10 +165,14 @@ def main():
                          "(the workload of the counter passes)")
     ap.add_argument("--precision", type=int, default=1, help="--ha-only: 1 mixed (default), 0 FP64")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default LDS brick size")
+    ap.add_argument("--controller", choices=("learned", "classical"), default="learned",
+                    help="CMF2 step proposal of every run (the CMF2 error test accepts or rejects either way)")
+    ap.add_argument("--no-classical", action="store_true",
+                    help="skip the extra mixed run with the classical controller")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this engine runs on one GPU; multi-GPU figures are not produced "
-              "(see the sharding design in DESIGN.md)", file=sys.stderr)
+              "(a multi-GPU engine would shard the wavefunction by electronic state)", file=sys.stderr)
         sys.exit(2)
 
     log(f"building workload {args.workload}")
@@ -173,7 +190,8 @@ def main():
         eng.close()
         print(json.dumps({"ha_only": True, "precision": args.precision, **rl}), flush=True)
         return
-    eng, mx = run_engine(prob, 1, args.warmup, args.steps, not args.no_autotune)
+    model = step_model(args.controller)
+    eng, mx = run_engine(prob, 1, args.warmup, args.steps, not args.no_autotune, model)
     rl = roofline(eng, prob, args.roofline_reps, PEAK_FP32_TFLOPS)
     log(f"roofline (mixed): {rl['achieved']:.2f} TFLOP/s, H.A {rl['HA_ms']:.2f} ms")
     eng.close()
@@ -187,6 +205,8 @@ def main():
                    "precision": "mixed: complex FP32 H.A, Krylov v1.., mean-field chains; FP64 A, SPFs, "
                                 "rho, mean fields, reductions, DeltaPsi",
                    "step": "one 0.5 fs output interval", "attempts": len(mx["atts"]),
+                   "rejected": sum(1 for a in mx["atts"] if not a.accepted),
+                   "controller": args.controller,
                    "HA_per_fs": mx["nha"] / mx["fs"], "lds_brick": mx["brick"],
                    "C_t_last": [last.auto_re, last.auto_im],
                    "norm_last": last.norm},
@@ -199,7 +219,7 @@ def main():
         result["roofline"]["traffic"] = tr["bytes_per_launch"]
         result["roofline_detail"]["traffic_source"] = tr["source"]
     if not args.no_fp64:
-        eng64, fx = run_engine(prob, 0, args.warmup, args.steps, not args.no_autotune)
+        eng64, fx = run_engine(prob, 0, args.warmup, args.steps, not args.no_autotune, model)
         rl64 = roofline(eng64, prob, args.roofline_reps, PEAK_FP64_TFLOPS)
         eng64.close()
         n = min(len(fx["outs"]), len(mx["outs"]))
@@ -212,6 +232,14 @@ def main():
         result["fp64"] = {"value": v64, "unit": "fs/s", "speedup_mixed": value / v64,
                           "HA_ms": rl64["HA_ms"], "roofline_frac": rl64["frac"],
                           "achieved_TFLOPs": rl64["achieved"]}
+    if model is not None and not args.no_classical:
+        engc, cx = run_engine(prob, 1, args.warmup, args.steps, not args.no_autotune, None, "mixed-classical")
+        engc.close()
+        result["mixed_classical_controller"] = {
+            "value": cx["fs"] / cx["wall"], "unit": "fs/s", "attempts": len(cx["atts"]),
+            "rejected": sum(1 for a in cx["atts"] if not a.accepted),
+            "max_abs_dC_vs_learned": max(abs(complex(a.auto_re, a.auto_im) - complex(b.auto_re, b.auto_im))
+                                         for a, b in zip(cx["outs"], mx["outs"]))}
     if not args.no_cpu:
         log(f"cpu baseline on {args.cpu_threads} threads")
         result["cpu_baseline"] = cpu_baseline(prob, mx["nha"] / mx["fs"], args.cpu_threads)

@@ -200,6 +200,30 @@ int mg_a_to_b(mg_ctx *ctx, const double *A, const double *dd, double *B);
  * controller.  mg_log_step_features records the features of every accepted
  * step of the following propagations (mg_step_features reads them with the
  * accepted step sizes). */
+/* Ensemble of independent wavepackets (no counterpart in the reference):
+ * member i is the problem probs[i] with initial state (A0[i], psi0[i])
+ * (complex interleaved, as mg_set_state).  `nthreads` host threads each
+ * create a member's context, propagate it over [t0, tfinal] and destroy it;
+ * members start largest first and are admitted while their estimated device
+ * bytes fit in mem_fraction of the free device memory.  flags bit 0: run
+ * mg_autotune (1 repetition) on each member first.  Per member i:
+ * outs[i * max_out + k], pops[(i * max_out + k) * 8 + s] (states beyond 8
+ * not reported; pops may be NULL), n_out[i], status[i] (0 = success) and,
+ * if member_wall is not NULL, member_wall[i] = its wall seconds.  The first
+ * failure's message goes to errbuf.  Returns 0 if every member succeeded,
+ * 1 if any failed, -1 for bad arguments. */
+int mg_ensemble_propagate(const mg_problem *const *probs, const double *const *A0, const double *const *psi0,
+                          int64_t n, int64_t precision, int64_t nthreads, double mem_fraction, int64_t flags,
+                          double t0, double tfinal, mg_output *outs, double *pops, int64_t max_out, int64_t *n_out,
+                          int64_t *status, double *member_wall, char *errbuf, int64_t errlen);
+
+/* HIP graphs (no counterpart in the reference): every cached launch plan
+ * (H.A schedule, mean-field, density-matrix and Kronecker-chain brick
+ * programs) is captured into a HIP graph on its second use and replayed from
+ * then on; enable = 0 launches kernel by kernel (default 1; the environment
+ * variable MG_GRAPHS=0 sets the default off). */
+int mg_set_graphs(mg_ctx *ctx, int64_t enable);
+
 #define MG_MLP_NFEAT 12
 #define MG_MLP_NPARAM 5081
 int mg_mlp_train(mg_ctx *ctx, const float *X, const float *t, int64_t n, int64_t epochs, int64_t batch, double lr,

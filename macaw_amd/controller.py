@@ -116,10 +116,11 @@ def run(prob, params, t_fs, precision=0):
     return outs, pops, atts, wall, stats
 
 
-def evaluate(prob, params, t_fs, log=print):
+def evaluate(prob, params, t_fs, ref_factor=0.5, log=print):
     """Classical vs learned controller on one problem (FP64 engine), both
-    against a reference run at eps_int / 10."""
-    ref_prob = dataclasses.replace(prob, eps_int=prob.eps_int / 10)
+    against a reference run at eps_int * ref_factor (the DeltaPsi estimate
+    has a rounding floor near 2e-8, so much tighter references stall)."""
+    ref_prob = dataclasses.replace(prob, eps_int=prob.eps_int * ref_factor)
     ref, _, _, _, _ = run(ref_prob, None, t_fs)
     res = {}
     for tag, prm in (("classical", None), ("learned", params)):
@@ -127,10 +128,11 @@ def evaluate(prob, params, t_fs, log=print):
         dc = max(abs(complex(a.auto_re, a.auto_im) - complex(b.auto_re, b.auto_im)) for a, b in zip(outs, ref))
         rej = sum(1 for a in atts if not a.accepted)
         res[tag] = {"attempts": len(atts), "rejected": rej, "rejected_fraction": rej / max(1, len(atts)),
-                    "wall_s": wall, "max_dC_vs_tight_reference": dc, "model_proposals": stats[0],
+                    "wall_s": wall, "max_dC_vs_reference": dc, "reference_eps_int": ref_prob.eps_int,
+                    "model_proposals": stats[0],
                     "model_fallbacks": stats[1]}
         log(f"[evaluate] {tag}: {len(atts)} attempts, {rej} rejected ({rej / max(1, len(atts)):.1%}), "
-            f"{wall:.2f} s, max|dC| vs eps/10 run {dc:.2e}")
+            f"{wall:.2f} s, max|dC| vs the eps_int x {ref_factor} run {dc:.2e}")
     return res
 
 
@@ -155,6 +157,7 @@ def main(argv=None):
     v.add_argument("--spec", default="c1")
     v.add_argument("--seed", type=int, default=999)
     v.add_argument("--fs", type=float, default=20.0)
+    v.add_argument("--ref-factor", type=float, default=0.5)
     v.add_argument("--json")
     a = ap.parse_args(argv)
     log = lambda m: print(m, flush=True)
@@ -180,7 +183,7 @@ def main(argv=None):
         params = np.load(a.model)
         with tempfile.TemporaryDirectory() as d:
             prob = make_problem(seeded_spec(a.spec, a.seed), d)
-        res = evaluate(prob, params, a.fs, log=log)
+        res = evaluate(prob, params, a.fs, ref_factor=a.ref_factor, log=log)
         if a.json:
             with open(a.json, "w") as fh:
                 json.dump(res, fh, indent=1)

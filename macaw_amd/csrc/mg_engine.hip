@@ -14,6 +14,11 @@
 #include <complex>
 #include <map>
 #include <random>
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <mutex>
+#include <thread>
 #include <memory>
 #include <cstdio>
 #include <cstdlib>
@@ -84,6 +89,15 @@ struct Plan {
     DevBuf targets;     // double2 * per Gram slot
     DevBuf herm_ptrs, herm_n;   // matrices made exactly Hermitian after the plan
     int nherm = 0;
+    // HIP graph of the plan's launches, captured on the second run (the
+    // first one sizes the shared Gram partial buffer); `epoch` is the
+    // buffer generation it was captured against
+    mutable hipGraphExec_t gexec = nullptr;
+    mutable int runs = 0;
+    mutable int epoch = -1;
+    ~Plan() {
+        if (gexec) (void)hipGraphExecDestroy(gexec);
+    }
 };
 using PlanKey = std::array<const void *, 4>;
 struct SpfPlan {
@@ -193,6 +207,9 @@ struct mg_ctx {
     DevBuf dgwpdesc, dgwpa, dS00, dSd, dDD, dDH, dDH2, dDobs, dhA, dhAf, dgwpscr, dABmats, dAH;
     // VMF with Gaussian modes: identity D, Tikhonov S00^-1 and tau per (mode, state), -tau per set
     DevBuf dDid, dSm1, dTauS, dTauSet;
+    // ---- HIP graphs of the cached launch plans ----
+    bool use_graphs = true;
+    int graph_epoch = 0;
     // ---- learned step controller ----
     DevBuf drmf_minw;                      // smallest rho eigenvalue per rho^-1 MF job
     DevBuf dmf_prev, dmf_tmp;              // mean fields of the previous accepted step
@@ -239,6 +256,25 @@ struct mg_ctx {
 namespace {
 
 double2 d2(cplx z) { return make_double2(z.real(), z.imag()); }
+
+// Context of the C-ABI call running on this thread.  Host<->device copies
+// and fills are ordered on that context's stream and waited for, never put
+// on the legacy null stream: ensemble members run on several host threads,
+// and a null-stream operation would synchronise with (and disturb) the
+// graph capture of another member's stream.
+thread_local mg_ctx *tl_ctx = nullptr;
+
+void hcopy(void *dst, const void *src, size_t bytes, hipMemcpyKind kind) {
+    const hipStream_t s = tl_ctx ? tl_ctx->stream : nullptr;
+    HIP_CHECK(hipMemcpyAsync(dst, src, bytes, kind, s));
+    HIP_CHECK(hipStreamSynchronize(s));
+}
+
+void hzero(void *dst, size_t bytes) {
+    const hipStream_t s = tl_ctx ? tl_ctx->stream : nullptr;
+    HIP_CHECK(hipMemsetAsync(dst, 0, bytes, s));
+    HIP_CHECK(hipStreamSynchronize(s));
+}
 
 // Kernel launches report configuration errors asynchronously; surface them.
 void check_launch() { HIP_CHECK(hipGetLastError()); }
@@ -584,7 +620,7 @@ void allocate(mg_ctx *c) {
     alloc_mf_tensors(c);
     if (c->mf_bricks) {
         c->dgslots.alloc(z * nn * c->nm * std::max<size_t>(1, mf_nterms(c)));
-        HIP_CHECK(hipMemset(c->dgslots.p, 0, c->dgslots.bytes));
+        hzero(c->dgslots.p, c->dgslots.bytes);
     }
     c->dkron.alloc(z * 2 * N);
     c->dpartials.alloc(sizeof(double2) * 64 * (size_t)c->nblocks_red);
@@ -606,18 +642,18 @@ void allocate(mg_ctx *c) {
                     for (int j = 0; j < c->NS(m, s); j++)
                         h[nn * (i + (size_t)c->nopmax * m) + j + (size_t)c->nspfmax * j] = make_double2(1, 0);
                 }
-        HIP_CHECK(hipMemcpy(c->dh1.p, h.data(), z * h.size(), hipMemcpyHostToDevice));
+        hcopy(c->dh1.p, h.data(), z * h.size(), hipMemcpyHostToDevice);
         if (c->prec) launch_convert(c->dh1f.p, 1, c->dh1.p, 0, (int64_t)h.size(), c->stream);
     }
     auto upload_i = [&](DevBuf &b, const std::vector<int> &v) {
         b.alloc(sizeof(int) * std::max<size_t>(1, v.size()));
-        if (!v.empty()) HIP_CHECK(hipMemcpy(b.p, v.data(), sizeof(int) * v.size(), hipMemcpyHostToDevice));
+        if (!v.empty()) hcopy(b.p, v.data(), sizeof(int) * v.size(), hipMemcpyHostToDevice);
     };
     auto upload_c = [&](DevBuf &b, const std::vector<cplx> &v) {
         b.alloc(sizeof(double2) * std::max<size_t>(1, v.size()));
         std::vector<double2> t(v.size());
         for (size_t i = 0; i < v.size(); i++) t[i] = d2(v[i]);
-        if (!t.empty()) HIP_CHECK(hipMemcpy(b.p, t.data(), sizeof(double2) * t.size(), hipMemcpyHostToDevice));
+        if (!t.empty()) hcopy(b.p, t.data(), sizeof(double2) * t.size(), hipMemcpyHostToDevice);
     };
     upload_i(c->dhs_rowptr, c->hs_rowptr);
     upload_i(c->dhs_src, c->hs_src);
@@ -632,10 +668,10 @@ void allocate(mg_ctx *c) {
     c->dd1.alloc(sizeof(double) * gm * gm * c->ndof);
     c->dd2.alloc(sizeof(double) * gm * gm * c->ndof);
     c->dxd1.alloc(sizeof(double) * gm * gm * c->ndof);
-    HIP_CHECK(hipMemcpy(c->dgrid.p, c->grid.data(), c->dgrid.bytes, hipMemcpyHostToDevice));
-    HIP_CHECK(hipMemcpy(c->dd1.p, c->d1.data(), c->dd1.bytes, hipMemcpyHostToDevice));
-    HIP_CHECK(hipMemcpy(c->dd2.p, c->d2.data(), c->dd2.bytes, hipMemcpyHostToDevice));
-    HIP_CHECK(hipMemcpy(c->dxd1.p, c->xd1.data(), c->dxd1.bytes, hipMemcpyHostToDevice));
+    hcopy(c->dgrid.p, c->grid.data(), c->dgrid.bytes, hipMemcpyHostToDevice);
+    hcopy(c->dd1.p, c->d1.data(), c->dd1.bytes, hipMemcpyHostToDevice);
+    hcopy(c->dd2.p, c->d2.data(), c->dd2.bytes, hipMemcpyHostToDevice);
+    hcopy(c->dxd1.p, c->xd1.data(), c->dxd1.bytes, hipMemcpyHostToDevice);
     std::vector<int> ngp(c->ndof), mdofs((size_t)c->nm * kMaxDofPerMode, 0), mnd(c->nm);
     for (int k = 0; k < c->ndof; k++) ngp[k] = (int)c->ngp[k];
     for (int m = 0; m < c->nm; m++) {
@@ -683,7 +719,7 @@ void allocate(mg_ctx *c) {
     }
     c->ndvr_desc = (int)md.size();
     c->dspfmodes.alloc(sizeof(SpfModeDesc) * std::max<size_t>(1, md.size()));
-    if (!md.empty()) HIP_CHECK(hipMemcpy(c->dspfmodes.p, md.data(), sizeof(SpfModeDesc) * md.size(), hipMemcpyHostToDevice));
+    if (!md.empty()) hcopy(c->dspfmodes.p, md.data(), sizeof(SpfModeDesc) * md.size(), hipMemcpyHostToDevice);
     c->dspfscr.alloc(z * scr);
     c->dstatus.alloc(sizeof(double2) * 2);
     // Gaussian modes: descriptors, widths, overlaps, D matrices, scratch
@@ -723,10 +759,10 @@ void allocate(mg_ctx *c) {
         }
         c->ngwp_desc = (int)gd.size();
         c->dgwpdesc.alloc(sizeof(GwpDesc) * gd.size());
-        HIP_CHECK(hipMemcpy(c->dgwpdesc.p, gd.data(), sizeof(GwpDesc) * gd.size(), hipMemcpyHostToDevice));
+        hcopy(c->dgwpdesc.p, gd.data(), sizeof(GwpDesc) * gd.size(), hipMemcpyHostToDevice);
         c->dgwpscr.alloc(z * gscr);
         c->dgwpa.alloc(sizeof(double) * c->gwpa.size());
-        HIP_CHECK(hipMemcpy(c->dgwpa.p, c->gwpa.data(), sizeof(double) * c->gwpa.size(), hipMemcpyHostToDevice));
+        hcopy(c->dgwpa.p, c->gwpa.data(), sizeof(double) * c->gwpa.size(), hipMemcpyHostToDevice);
         const size_t nms = nn * c->nm * c->ns;
         c->dS00.alloc(z * nms * c->ns);
         c->dSd.alloc(z * nms);
@@ -735,9 +771,9 @@ void allocate(mg_ctx *c) {
         c->dDH2.alloc(z * nms);
         c->dDobs.alloc(z * nms);
         c->dABmats.alloc(z * 2 * nms);
-        HIP_CHECK(hipMemset(c->dS00.p, 0, c->dS00.bytes));
-        HIP_CHECK(hipMemset(c->dSd.p, 0, c->dSd.bytes));
-        HIP_CHECK(hipMemset(c->dDD.p, 0, c->dDD.bytes));
+        hzero(c->dS00.p, c->dS00.bytes);
+        hzero(c->dSd.p, c->dSd.bytes);
+        hzero(c->dDD.p, c->dDD.bytes);
         c->dSm1.alloc(z * nms);
         c->dTauS.alloc(z * nms);
         c->dTauSet.alloc(z * nms);
@@ -748,7 +784,7 @@ void allocate(mg_ctx *c) {
                     for (int j = 0; j < c->nspfmax; j++)
                         I[nn * (m + (size_t)c->nm * st) + j + (size_t)c->nspfmax * j] = make_double2(1, 0);
             c->dDid.alloc(z * nms);
-            HIP_CHECK(hipMemcpy(c->dDid.p, I.data(), z * nms, hipMemcpyHostToDevice));
+            hcopy(c->dDid.p, I.data(), z * nms, hipMemcpyHostToDevice);
         }
         c->dhA.alloc(z * nn * c->nopmax * c->nm);
         if (c->prec) c->dhAf.alloc(sizeof(float2) * nn * c->nopmax * c->nm);
@@ -879,7 +915,10 @@ struct TaskBuilder {
     // dimension nspfmax (adj: use its conjugate transpose), times scale
     void op(int src, int dst, int mode, const void *mat, int adj, int nout, int acc, cplx scale = 1.0) {
         if (mode < m0 || mode >= m1) throw HipError{"op outside the brick's mode group"};
-        if (t.nmat >= kMaxBrickMats) throw HipError{"too many op matrices in a brick program"};
+        if (t.nmat >= kMaxBrickMats)
+            throw HipError{"too many op matrices in a brick program (modes " + std::to_string(m0) + ".." +
+                           std::to_string(m1 - 1) + ", op on mode " + std::to_string(mode) + ", " +
+                           std::to_string(t.ninstr) + " instructions so far)"};
         const int p = mode - m0;
         std::vector<int64_t> sh = shape[src];
         const int ni = (int)sh[p];
@@ -989,18 +1028,18 @@ struct PlanBuilder {
         p->prec = prec;
         p->buf.alloc(sizeof(BrickTask) * std::max<size_t>(1, tasks.size()));
         if (!tasks.empty())
-            HIP_CHECK(hipMemcpy(p->buf.p, tasks.data(), sizeof(BrickTask) * tasks.size(), hipMemcpyHostToDevice));
+            hcopy(p->buf.p, tasks.data(), sizeof(BrickTask) * tasks.size(), hipMemcpyHostToDevice);
         p->launches = launches;
         p->targets.alloc(sizeof(double2 *) * std::max<size_t>(1, targets.size()));
         if (!targets.empty())
-            HIP_CHECK(hipMemcpy(p->targets.p, targets.data(), sizeof(double2 *) * targets.size(),
-                                hipMemcpyHostToDevice));
+            hcopy(p->targets.p, targets.data(), sizeof(double2 *) * targets.size(),
+                                hipMemcpyHostToDevice);
         p->nherm = (int)herm.size();
         p->herm_ptrs.alloc(sizeof(double2 *) * std::max<size_t>(1, herm.size()));
         p->herm_n.alloc(sizeof(int) * std::max<size_t>(1, herm.size()));
         if (!herm.empty()) {
-            HIP_CHECK(hipMemcpy(p->herm_ptrs.p, herm.data(), sizeof(double2 *) * herm.size(), hipMemcpyHostToDevice));
-            HIP_CHECK(hipMemcpy(p->herm_n.p, herm_n.data(), sizeof(int) * herm.size(), hipMemcpyHostToDevice));
+            hcopy(p->herm_ptrs.p, herm.data(), sizeof(double2 *) * herm.size(), hipMemcpyHostToDevice);
+            hcopy(p->herm_n.p, herm_n.data(), sizeof(int) * herm.size(), hipMemcpyHostToDevice);
         }
         return p;
     }
@@ -1013,6 +1052,7 @@ void launch_plan_entry(mg_ctx *c, const Plan &p, const Launch &l, int nmax) {
         if (c->dgpart.bytes < need) {
             HIP_CHECK(hipStreamSynchronize(c->stream));
             c->dgpart.alloc(need);
+            c->graph_epoch++;          // graphs captured against the old buffer are stale
         }
         launch_brick_tasks(base + l.off, l.n, l.maxblocks, nmax, l.brick, l.nbuf, l.nmat, p.prec, c->stream, l.ngram,
                            c->dgpart.as<double2>(), l.gram_blocks);
@@ -1033,6 +1073,26 @@ void run_plan(mg_ctx *c, const Plan &p) {
     (void)base;
     static const bool verbose_all = std::getenv("MG_VERBOSE") != nullptr;
     if (!c->prof && !verbose_all) {
+        if (c->use_graphs && p.runs >= 1) {
+            if (p.gexec && p.epoch != c->graph_epoch) {
+                HIP_CHECK(hipGraphExecDestroy(p.gexec));
+                p.gexec = nullptr;
+            }
+            if (!p.gexec) {
+                hipGraph_t g = nullptr;
+                HIP_CHECK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+                for (auto &l : p.launches) launch_plan_entry(c, p, l, nmax);
+                if (p.nherm)
+                    launch_herm_batch(p.herm_ptrs.as<double2 *>(), p.herm_n.as<int>(), p.nherm, c->nspfmax, c->stream);
+                HIP_CHECK(hipStreamEndCapture(c->stream, &g));
+                HIP_CHECK(hipGraphInstantiate(&p.gexec, g, nullptr, nullptr, 0));
+                HIP_CHECK(hipGraphDestroy(g));
+                p.epoch = c->graph_epoch;
+            }
+            HIP_CHECK(hipGraphLaunch(p.gexec, c->stream));
+            return;
+        }
+        p.runs++;
         for (auto &l : p.launches) launch_plan_entry(c, p, l, nmax);
         if (p.nherm) launch_herm_batch(p.herm_ptrs.as<double2 *>(), p.herm_n.as<int>(), p.nherm, c->nspfmax, c->stream);
         return;
@@ -1287,7 +1347,12 @@ std::unique_ptr<Plan> build_apply_h(mg_ctx *c, const void *B, void *HB, bool gba
 const Plan &get_plan(mg_ctx *c, const PlanKey &key, std::unique_ptr<Plan> (*build)(mg_ctx *, const PlanKey &)) {
     auto it = c->plans.find(key);
     if (it != c->plans.end()) return *it->second;
-    auto p = build(c, key);
+    std::unique_ptr<Plan> p;
+    try {
+        p = build(c, key);
+    } catch (const HipError &e) {
+        throw HipError{"building launch plan " + std::to_string((uintptr_t)key[0]) + ": " + e.msg};
+    }
     const Plan &ref = *p;
     c->plans[key] = std::move(p);
     return ref;
@@ -1438,7 +1503,7 @@ void run_spf_tasks(mg_ctx *c, const PlanKey &key, std::vector<SpfTask> (*build)(
         p->n = (int)tasks.size();
         p->buf.alloc(sizeof(SpfTask) * std::max<size_t>(1, tasks.size()));
         if (!tasks.empty())
-            HIP_CHECK(hipMemcpy(p->buf.p, tasks.data(), sizeof(SpfTask) * tasks.size(), hipMemcpyHostToDevice));
+            hcopy(p->buf.p, tasks.data(), sizeof(SpfTask) * tasks.size(), hipMemcpyHostToDevice);
         it = cache.emplace(key, std::move(p)).first;
     }
     launch_spf_tasks(it->second->buf.as<SpfTask>(), it->second->n, dof_tables(c), 0, c->stream);
@@ -1828,7 +1893,7 @@ void build_mf_combine(mg_ctx *c) {
         std::vector<double2> I(nn, make_double2(0, 0));
         for (int j = 0; j < c->nspfmax; j++) I[j + (size_t)c->nspfmax * j] = make_double2(1, 0);
         c->dident.alloc(sizeof(double2) * nn);
-        HIP_CHECK(hipMemcpy(c->dident.p, I.data(), sizeof(double2) * nn, hipMemcpyHostToDevice));
+        hcopy(c->dident.p, I.data(), sizeof(double2) * nn, hipMemcpyHostToDevice);
     }
     for (int r = 0; r < nrow; r++)
         for (int q = c->mf1_rowptr[r]; q < c->mf1_rowptr[r + 1]; q++) {
@@ -1865,7 +1930,7 @@ void build_mf_combine(mg_ctx *c) {
     }
     auto up = [&](DevBuf &b, const void *h, size_t bytes) {
         b.alloc(std::max<size_t>(bytes, 8));
-        if (bytes) HIP_CHECK(hipMemcpy(b.p, h, bytes, hipMemcpyHostToDevice));
+        if (bytes) hcopy(b.p, h, bytes, hipMemcpyHostToDevice);
     };
     up(c->dmf_dst, dst.data(), sizeof(double2 *) * dst.size());
     up(c->dmf_rowptr, rowptr.data(), sizeof(int) * rowptr.size());
@@ -2674,6 +2739,11 @@ void vmf_rkqs(mg_ctx *c, bool dp, double htry, double tt, double *hdid, double *
 template <class F>
 int guarded(mg_ctx *c, F &&f) {
     if (!c) return -1;
+    struct Scope {
+        mg_ctx *prev;
+        explicit Scope(mg_ctx *c) : prev(tl_ctx) { tl_ctx = c; }
+        ~Scope() { tl_ctx = prev; }
+    } scope(c);
     try {
         f();
         return 0;
@@ -2711,6 +2781,7 @@ int mg_create(const mg_problem *prob, int64_t precision, mg_ctx **out) {
         HIP_CHECK(hipHostMalloc((void **)&c->spf_status_host, sizeof(int), hipHostMallocDefault));
         *c->spf_status_host = 0;
         setup(c, prob);
+        if (const char *e = std::getenv("MG_GRAPHS")) c->use_graphs = std::atoi(e) != 0;
         allocate(c);
         c->stepsize = c->writetime * c->eps_int;  // propagator.f90:103
     });
@@ -2799,23 +2870,23 @@ int64_t mg_dimpsi(const mg_ctx *c) { return c ? c->dimpsi : -1; }
 
 int mg_set_state(mg_ctx *c, const double *A, const double *psi) {
     return guarded(c, [&] {
-        HIP_CHECK(hipMemcpy(c->dA.p, A, sizeof(double2) * c->nconf, hipMemcpyHostToDevice));
-        HIP_CHECK(hipMemcpy(c->dpsi.p, psi, sizeof(double2) * c->dimpsi, hipMemcpyHostToDevice));
+        hcopy(c->dA.p, A, sizeof(double2) * c->nconf, hipMemcpyHostToDevice);
+        hcopy(c->dpsi.p, psi, sizeof(double2) * c->dimpsi, hipMemcpyHostToDevice);
     });
 }
 
 int mg_get_state(mg_ctx *c, double *A, double *psi) {
     return guarded(c, [&] {
         HIP_CHECK(hipStreamSynchronize(c->stream));
-        if (A) HIP_CHECK(hipMemcpy(A, c->dA.p, sizeof(double2) * c->nconf, hipMemcpyDeviceToHost));
-        if (psi) HIP_CHECK(hipMemcpy(psi, c->dpsi.p, sizeof(double2) * c->dimpsi, hipMemcpyDeviceToHost));
+        if (A) hcopy(A, c->dA.p, sizeof(double2) * c->nconf, hipMemcpyDeviceToHost);
+        if (psi) hcopy(psi, c->dpsi.p, sizeof(double2) * c->dimpsi, hipMemcpyDeviceToHost);
     });
 }
 
 int mg_set_start(mg_ctx *c, const double *A, const double *psi) {
     return guarded(c, [&] {
-        HIP_CHECK(hipMemcpy(c->dAstart.p, A, sizeof(double2) * c->nconf, hipMemcpyHostToDevice));
-        HIP_CHECK(hipMemcpy(c->dpsistart.p, psi, sizeof(double2) * c->dimpsi, hipMemcpyHostToDevice));
+        hcopy(c->dAstart.p, A, sizeof(double2) * c->nconf, hipMemcpyHostToDevice);
+        hcopy(c->dpsistart.p, psi, sizeof(double2) * c->dimpsi, hipMemcpyHostToDevice);
     });
 }
 
@@ -2958,9 +3029,9 @@ struct MlpDev {
         grad.alloc(sizeof(float) * kMlpNParam);
         m.alloc(sizeof(float) * kMlpNParam);
         v.alloc(sizeof(float) * kMlpNParam);
-        HIP_CHECK(hipMemcpy(p.p, host, sizeof(float) * kMlpNParam, hipMemcpyHostToDevice));
-        HIP_CHECK(hipMemset(m.p, 0, m.bytes));
-        HIP_CHECK(hipMemset(v.p, 0, v.bytes));
+        hcopy(p.p, host, sizeof(float) * kMlpNParam, hipMemcpyHostToDevice);
+        hzero(m.p, m.bytes);
+        hzero(v.p, v.bytes);
     }
 };
 }  // namespace
@@ -3085,7 +3156,7 @@ int mg_mlp_train(mg_ctx *c, const float *X, const float *t, int64_t n, int64_t e
             for (int i = 0; i < b; i++) sum += e[i];
         }
         if (loss_out) *loss_out = sum / (double)n;
-        HIP_CHECK(hipMemcpy(params, d.p.p, sizeof(float) * kMlpNParam, hipMemcpyDeviceToHost));
+        hcopy(params, d.p.p, sizeof(float) * kMlpNParam, hipMemcpyDeviceToHost);
     });
 }
 
@@ -3100,12 +3171,128 @@ int mg_set_step_model(mg_ctx *c, const float *params) {
         c->dmlp_x.alloc(sizeof(float) * kMlpNFeat);
         c->dmlp_y.alloc(sizeof(float) * 4);
         c->dmlp_work.alloc(sizeof(float) * (6 * kMlpHidden + 2));
-        HIP_CHECK(hipMemcpy(c->dmlp_p.p, params, sizeof(float) * kMlpNParam, hipMemcpyHostToDevice));
+        hcopy(c->dmlp_p.p, params, sizeof(float) * kMlpNParam, hipMemcpyHostToDevice);
         launch_mlp_bf16(c->dmlp_p.as<float>(), c->dmlp_wb.as<uint16_t>(), c->stream);
         check_launch();
         HIP_CHECK(hipStreamSynchronize(c->stream));
         c->mlp_on = true;
         c->mlp_proposals = c->mlp_fallbacks = 0;
+    });
+}
+
+// Ensemble of independent wavepackets.  Every member is a context of its own
+// (buffers, two streams, launch plans and graphs), created, propagated and
+// destroyed by one of `nthreads` host threads, so the device interleaves the
+// kernels of up to nthreads members.  Members start largest first; a member
+// is admitted only while the estimated device bytes of the members in flight
+// stay within mem_fraction of the free device memory (one member is always
+// admitted, so an oversized member runs alone).
+namespace {
+double member_bytes(const mg_problem *p, int64_t precision) {
+    double nconf = 0.0;
+    for (int64_t s = 0; s < p->nstates; s++) {
+        double v = 1.0;
+        for (int64_t m = 0; m < p->nmodes; m++) {
+            const int64_t sm = p->MultiSet && p->MultiSet[m] ? s : 0;
+            v *= (double)p->nSPF[m + p->nmodes * sm];
+        }
+        nconf += v;
+    }
+    // A-vectors: state, start, Krylov space (up to 21), H.A, partial sums
+    // and mean-field scratch; 40 copies bounds what allocate() takes.
+    return nconf * (precision ? 8.0 : 16.0) * 40.0 + 64.0 * 1024 * 1024;
+}
+}  // namespace
+
+int mg_ensemble_propagate(const mg_problem *const *probs, const double *const *A0, const double *const *psi0,
+                          int64_t n, int64_t precision, int64_t nthreads, double mem_fraction, int64_t flags,
+                          double t0, double tfinal, mg_output *outs, double *pops, int64_t max_out, int64_t *n_out,
+                          int64_t *status, double *member_wall, char *errbuf, int64_t errlen) {
+    if (!probs || !A0 || !psi0 || n < 0 || !outs || !n_out || !status || max_out < 1) return -1;
+    if (errbuf && errlen > 0) errbuf[0] = 0;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return -1;
+    size_t free_b = 0, total_b = 0;
+    if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) return -1;
+    const double budget = std::max(0.05, std::min(0.95, mem_fraction)) * (double)free_b;
+    std::vector<int64_t> order((size_t)n);
+    std::vector<double> est((size_t)n);
+    for (int64_t i = 0; i < n; i++) {
+        order[(size_t)i] = i;
+        est[(size_t)i] = member_bytes(probs[i], precision);
+        status[i] = -1;
+        n_out[i] = 0;
+    }
+    std::stable_sort(order.begin(), order.end(), [&](int64_t a, int64_t b) { return est[a] > est[b]; });
+    std::mutex mu;
+    std::condition_variable cv;
+    double in_flight = 0.0;
+    int64_t active = 0, next = 0;
+    std::string first_err;
+    auto worker = [&]() {
+        if (hipSetDevice(dev) != hipSuccess) return;
+        for (;;) {
+            int64_t i;
+            {
+                std::unique_lock<std::mutex> lk(mu);
+                cv.wait(lk, [&] {
+                    return next >= n || active == 0 || in_flight + est[(size_t)order[(size_t)next]] <= budget;
+                });
+                if (next >= n) return;
+                i = order[(size_t)next];
+                next++;
+                active++;
+                in_flight += est[(size_t)i];
+            }
+            const auto w0 = std::chrono::steady_clock::now();
+            mg_ctx *c = nullptr;
+            int rc = mg_create(probs[i], precision, &c);
+            if (rc == 0) rc = mg_set_state(c, A0[i], psi0[i]);
+            if (rc == 0) rc = mg_set_start(c, A0[i], psi0[i]);
+            if (rc == 0 && (flags & 1)) {
+                int64_t b = 0;
+                double ms = 0.0;
+                rc = mg_autotune(c, 1, &b, &ms);
+            }
+            std::vector<double> pp((size_t)max_out * (size_t)std::max<int64_t>(1, probs[i]->nstates));
+            if (rc == 0)
+                rc = mg_propagate(c, t0, tfinal, outs + i * max_out, pp.data(), max_out, &n_out[i], nullptr, 0,
+                                  nullptr);
+            if (pops)
+                for (int64_t k = 0; k < n_out[i]; k++)
+                    for (int64_t st = 0; st < probs[i]->nstates && st < 8; st++)
+                        pops[((size_t)i * max_out + k) * 8 + st] = pp[(size_t)k * probs[i]->nstates + st];
+            std::string msg = (rc != 0 && c) ? c->err : std::string();
+            if (c) mg_destroy(c);
+            if (member_wall)
+                member_wall[i] = std::chrono::duration<double>(std::chrono::steady_clock::now() - w0).count();
+            std::lock_guard<std::mutex> lk(mu);
+            status[i] = rc;
+            if (rc != 0 && first_err.empty())
+                first_err = "member " + std::to_string(i) + ": " + (msg.empty() ? "creation failed" : msg);
+            active--;
+            in_flight -= est[(size_t)i];
+            cv.notify_all();
+        }
+    };
+    const int64_t nt = std::max<int64_t>(1, std::min<int64_t>(nthreads, std::max<int64_t>(n, 1)));
+    std::vector<std::thread> pool;
+    for (int64_t k = 1; k < nt; k++) pool.emplace_back(worker);
+    worker();
+    for (auto &t : pool) t.join();
+    if (!first_err.empty() && errbuf && errlen > 0) {
+        std::strncpy(errbuf, first_err.c_str(), (size_t)errlen - 1);
+        errbuf[errlen - 1] = 0;
+    }
+    for (int64_t i = 0; i < n; i++)
+        if (status[i] != 0) return 1;
+    return 0;
+}
+
+int mg_set_graphs(mg_ctx *c, int64_t enable) {
+    return guarded(c, [&] {
+        HIP_CHECK(hipStreamSynchronize(c->stream));
+        c->use_graphs = enable != 0;
     });
 }
 

@@ -429,7 +429,6 @@ __global__ void __launch_bounds__(GRAM ? 256 : 1024) k_brick(const BrickTask *__
 template <int NMAX, class T, bool GRAM>
 static void launch_brick_nmax(const BrickTask *d_tasks, int ntasks, int64_t max_blocks, int capacity, int nbuf,
                               int nmat, hipStream_t s, double2 *partials) {
-    static bool attr_set = false;
     static const int nthreads = [] {
         const char *e = std::getenv("MG_THREADS");
         const int v = e ? std::atoi(e) : 256;
@@ -437,11 +436,13 @@ static void launch_brick_nmax(const BrickTask *d_tasks, int ntasks, int64_t max_
     }();
     size_t shm = sizeof(T) * ((size_t)nmat * NMAX * NMAX + (size_t)nbuf * capacity);
     if (GRAM) shm = std::max(shm, sizeof(double) * 256 * 4);
-    if (!attr_set) {
+    // Thread-safe one-time initialisation (ensemble members launch from several host threads).
+    static const bool attr_set = [] {
         (void)hipFuncSetAttribute((const void *)k_brick<NMAX, T, GRAM>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                   160 * 1024);
-        attr_set = true;
-    }
+        return true;
+    }();
+    (void)attr_set;
     hipLaunchKernelGGL((k_brick<NMAX, T, GRAM>), dim3((unsigned)max_blocks, (unsigned)ntasks), dim3(GRAM ? 256 : nthreads), shm,
                        s, d_tasks, capacity, nmat, partials);
 }

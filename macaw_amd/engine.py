@@ -72,6 +72,10 @@ def lib():
         L.mg_log_step_features.argtypes = [vp, C.c_int64]
         L.mg_step_features.argtypes = [vp, f32p, f64p, C.c_int64, i64p]
         L.mg_controller_stats.argtypes = [vp, i64p, i64p]
+        L.mg_set_graphs.argtypes = [vp, C.c_int64]
+        L.mg_ensemble_propagate.argtypes = [vp, vp, vp, C.c_int64, C.c_int64, C.c_int64, C.c_double, C.c_int64,
+                                            C.c_double, C.c_double, C.POINTER(MgOutput), f64p, C.c_int64, i64p,
+                                            i64p, f64p, C.c_char_p, C.c_int64]
         _lib = L
     return _lib
 
@@ -259,6 +263,10 @@ class Engine:
         B = np.zeros_like(A)
         self._check(lib().mg_a_to_b(self.h, cplx_ptr(A), cplx_ptr(D), cplx_ptr(B)))
         return B
+
+    def set_graphs(self, enable=True):
+        """Replay the cached launch plans as HIP graphs (default) or launch kernel by kernel."""
+        self._check(lib().mg_set_graphs(self.h, 1 if enable else 0))
 
     # ---- learned step controller ----
     def mlp_train(self, X, t, epochs=30, batch=256, lr=1e-3, seed=0):

@@ -1,0 +1,75 @@
+"""Ensembles of independent wavepackets (mg_ensemble_propagate).
+
+Each member runs the same kernels with the same arguments as a context of
+its own, so the ensemble result of every member must be bit-identical to
+propagating that member alone, whatever the number of host threads and the
+order the members run in.  The member generator is checked on the CPU.
+"""
+import numpy as np
+import pytest
+
+from macaw_amd import ensemble
+
+gpu = pytest.mark.gpu
+
+
+def test_member_shapes_span_the_stated_ranges():
+    specs = [ensemble.member_spec(k) for k in range(256)]
+    assert {s.nstates for s in specs} == {2, 3, 4}
+    assert {s.nmodes for s in specs} == set(range(4, 9))
+    assert {s.nspf for s in specs} == set(range(4, 9))
+    assert min(s.npts for s in specs) >= 16 and max(s.npts for s in specs) <= 32
+    assert len({s.seed for s in specs}) == 256
+    # deterministic in (seed, k)
+    assert ensemble.member_spec(17) == ensemble.member_spec(17)
+    assert ensemble.member_spec(17, seed=1) != ensemble.member_spec(17)
+
+
+def test_members_build_problems_with_initial_states():
+    probs = ensemble.members(3, tfinal=0.5)
+    for k, p in enumerate(probs):
+        s = ensemble.member_spec(k, tfinal=0.5)
+        assert (p.nstates, p.nmodes) == (s.nstates, s.nmodes)
+        assert p.A0.shape == (p.nAConftot,)
+        assert np.isclose(np.vdot(p.A0, p.A0).real, 1.0)
+
+
+def small_members(n, limit=20000):
+    ks = [k for k in range(400) if ensemble.member_spec(k).nstates * ensemble.member_spec(k).nspf
+          ** ensemble.member_spec(k).nmodes <= limit][:n]
+    import tempfile
+    from macaw_amd.lvc import make_problem
+    out = []
+    for k in ks:
+        with tempfile.TemporaryDirectory() as d:
+            out.append(make_problem(ensemble.member_spec(k, tfinal=0.5), d))
+    return out
+
+
+@gpu
+@pytest.mark.parametrize("precision,threads", [(0, 4), (1, 3), (1, 1)])
+def test_ensemble_members_match_single_runs(precision, threads):
+    from macaw_amd.engine import Engine
+    probs = small_members(6)
+    t_fs = 0.5
+    outs, pops, wall, total = ensemble.propagate(probs, t_fs, precision=precision, threads=threads)
+    assert np.all(wall > 0) and total > 0
+    for i, p in enumerate(probs):
+        e = Engine(p, precision=precision)
+        o1, p1, _ = e.propagate(0.0, t_fs / ensemble.AU2FS)
+        e.close()
+        assert len(outs[i]) == len(o1), f"member {i}"
+        for a, b in zip(outs[i], o1):
+            assert (a.time, a.auto_re, a.auto_im, a.norm, a.energy) == (b.time, b.auto_re, b.auto_im, b.norm,
+                                                                        b.energy), f"member {i}"
+        assert np.array_equal(pops[i], p1), f"member {i}"
+
+
+@gpu
+def test_ensemble_reports_member_failure():
+    """Members the engine refuses (here: an unknown precision, refused on the
+    host before any kernel) are reported with their message."""
+    from macaw_amd.engine import EngineError
+    probs = small_members(2)
+    with pytest.raises(EngineError, match="member .*precision must be"):
+        ensemble.propagate(probs, 0.5, precision=2, threads=2)

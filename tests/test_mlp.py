@@ -127,3 +127,21 @@ def test_learned_controller_keeps_the_acceptance_criterion(tmp_path):
     print(f"classical {len(atts_c)} attempts ({sum(not a.accepted for a in atts_c)} rejected); learned "
           f"{len(atts_m)} ({sum(not a.accepted for a in atts_m)} rejected); |dC| {dc:.2e}")
     assert dc < 1e-6
+
+
+@gpu
+def test_shipped_controller_cuts_rejections_on_a_fresh_seed(tmp_path):
+    """The committed network (trained on logged classical runs of the 2- and
+    4-state grid workloads and the G-MCTDH workload, other seeds) against the
+    classical controller on a seed it never saw: fewer rejected attempts,
+    and C(t) as close to a run at half the tolerance as the classical run."""
+    import os
+    from macaw_amd.controller import evaluate, seeded_spec
+    from macaw_amd.lvc import make_problem
+    params = np.load(os.path.join(os.path.dirname(__file__), "..", "macaw_amd", "step_controller.npy"))
+    prob = make_problem(seeded_spec("c1", 4242), str(tmp_path))
+    res = evaluate(prob, params, 4.0)
+    c, m = res["classical"], res["learned"]
+    assert m["model_fallbacks"] == 0 and m["model_proposals"] > 0
+    assert m["rejected"] <= c["rejected"] // 2 + 1
+    assert m["max_dC_vs_reference"] <= max(2.0 * c["max_dC_vs_reference"], 1e-7)
