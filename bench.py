@@ -114,12 +114,12 @@ def measured_traffic(workload, precision, launches_per_ha):
     return t
 
 
-def step_model(kind):
+def step_model(kind, path=None):
     """Network of the learned CMF2 step controller, or None (classical)."""
     if kind == "classical":
         return None
     import numpy as np
-    return np.load(os.path.join(ROOT, "macaw_amd", "step_controller.npy"))
+    return np.load(path or os.path.join(ROOT, "macaw_amd", "step_controller.npy"))
 
 
 def run_engine(prob, precision, warmup, steps, autotune=True, model=None, tag=None):
@@ -167,6 +167,7 @@ def main():
     ap.add_argument("--no-autotune", action="store_true", help="keep the default LDS brick size")
     ap.add_argument("--controller", choices=("learned", "classical"), default="learned",
                     help="CMF2 step proposal of every run (the CMF2 error test accepts or rejects either way)")
+    ap.add_argument("--step-model", help="network file of the learned controller (default: the shipped one)")
     ap.add_argument("--no-classical", action="store_true",
                     help="skip the extra mixed run with the classical controller")
     args = ap.parse_args()
@@ -190,7 +191,7 @@ def main():
         eng.close()
         print(json.dumps({"ha_only": True, "precision": args.precision, **rl}), flush=True)
         return
-    model = step_model(args.controller)
+    model = step_model(args.controller, args.step_model)
     eng, mx = run_engine(prob, 1, args.warmup, args.steps, not args.no_autotune, model)
     rl = roofline(eng, prob, args.roofline_reps, PEAK_FP32_TFLOPS)
     log(f"roofline (mixed): {rl['achieved']:.2f} TFLOP/s, H.A {rl['HA_ms']:.2f} ms")

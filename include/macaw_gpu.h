@@ -206,7 +206,8 @@ int mg_a_to_b(mg_ctx *ctx, const double *A, const double *dd, double *B);
  * create a member's context, propagate it over [t0, tfinal] and destroy it;
  * members start largest first and are admitted while their estimated device
  * bytes fit in mem_fraction of the free device memory.  flags bit 0: run
- * mg_autotune (1 repetition) on each member first.  Per member i:
+ * mg_autotune (1 repetition) on each member first; bit 1: a progress line
+ * on stderr as each member finishes.  Per member i:
  * outs[i * max_out + k], pops[(i * max_out + k) * 8 + s] (states beyond 8
  * not reported; pops may be NULL), n_out[i], status[i] (0 = success) and,
  * if member_wall is not NULL, member_wall[i] = its wall seconds.  The first

@@ -19,7 +19,7 @@ target is y = log(h* / h_n), clipped to the controller's range
 [log 0.2, log 2].
 
 Command line:
-  python -m macaw_amd.controller collect --spec c1 --seeds 8 --fs 20 --out data.npz
+  python -m macaw_amd.controller collect --spec c1 --seeds 8 --fs 20 [--precision 1] --out data.npz
   python -m macaw_amd.controller train --data data.npz --out model.npy
   python -m macaw_amd.controller evaluate --model model.npy --spec c1 --seed 999 --fs 20
 """
@@ -95,7 +95,8 @@ def collect(spec_name, seeds, t_fs, precision=0, log=print):
         f, h = e.step_features()
         X, y, p = training_pairs(f, h, atts, prob.eps_int)
         rej = sum(1 for a in atts if not a.accepted)
-        log(f"[collect] {spec_name} seed {seed}: {len(atts)} attempts ({rej} rejected), {len(y)} pairs, "
+        log(f"[collect] {spec_name} (precision {precision}) seed {seed}: {len(atts)} attempts ({rej} rejected), "
+            f"{len(y)} pairs, "
             f"p = {p:.2f}, {time.perf_counter() - t0:.1f} s")
         Xs.append(X)
         ys.append(y)
@@ -116,15 +117,16 @@ def run(prob, params, t_fs, precision=0):
     return outs, pops, atts, wall, stats
 
 
-def evaluate(prob, params, t_fs, ref_factor=0.5, log=print):
-    """Classical vs learned controller on one problem (FP64 engine), both
-    against a reference run at eps_int * ref_factor (the DeltaPsi estimate
-    has a rounding floor near 2e-8, so much tighter references stall)."""
+def evaluate(prob, params, t_fs, ref_factor=0.5, precision=0, log=print):
+    """Classical vs learned controller on one problem (engine precision
+    `precision`), both against an FP64 reference run at eps_int * ref_factor
+    (the DeltaPsi estimate has a rounding floor near 2e-8, so much tighter
+    references stall)."""
     ref_prob = dataclasses.replace(prob, eps_int=prob.eps_int * ref_factor)
     ref, _, _, _, _ = run(ref_prob, None, t_fs)
     res = {}
     for tag, prm in (("classical", None), ("learned", params)):
-        outs, pops, atts, wall, stats = run(prob, prm, t_fs)
+        outs, pops, atts, wall, stats = run(prob, prm, t_fs, precision)
         dc = max(abs(complex(a.auto_re, a.auto_im) - complex(b.auto_re, b.auto_im)) for a, b in zip(outs, ref))
         rej = sum(1 for a in atts if not a.accepted)
         res[tag] = {"attempts": len(atts), "rejected": rej, "rejected_fraction": rej / max(1, len(atts)),
@@ -144,6 +146,7 @@ def main(argv=None):
     c.add_argument("--seeds", type=int, default=8)
     c.add_argument("--seed0", type=int, default=1000)
     c.add_argument("--fs", type=float, default=20.0)
+    c.add_argument("--precision", type=int, default=0, help="0 complex FP64, 1 mixed")
     c.add_argument("--out", required=True)
     t = sub.add_parser("train")
     t.add_argument("--data", nargs="+", required=True)
@@ -158,11 +161,12 @@ def main(argv=None):
     v.add_argument("--seed", type=int, default=999)
     v.add_argument("--fs", type=float, default=20.0)
     v.add_argument("--ref-factor", type=float, default=0.5)
+    v.add_argument("--precision", type=int, default=0, help="0 complex FP64, 1 mixed")
     v.add_argument("--json")
     a = ap.parse_args(argv)
     log = lambda m: print(m, flush=True)
     if a.cmd == "collect":
-        X, y = collect(a.spec, range(a.seed0, a.seed0 + a.seeds), a.fs, log=log)
+        X, y = collect(a.spec, range(a.seed0, a.seed0 + a.seeds), a.fs, precision=a.precision, log=log)
         np.savez(a.out, X=X, y=y)
         log(f"[collect] {len(y)} pairs -> {a.out}")
     elif a.cmd == "train":
@@ -183,7 +187,7 @@ def main(argv=None):
         params = np.load(a.model)
         with tempfile.TemporaryDirectory() as d:
             prob = make_problem(seeded_spec(a.spec, a.seed), d)
-        res = evaluate(prob, params, a.fs, ref_factor=a.ref_factor, log=log)
+        res = evaluate(prob, params, a.fs, ref_factor=a.ref_factor, precision=a.precision, log=log)
         if a.json:
             with open(a.json, "w") as fh:
                 json.dump(res, fh, indent=1)

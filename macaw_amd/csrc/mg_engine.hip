@@ -1206,11 +1206,15 @@ std::unique_ptr<Plan> build_apply_h(mg_ctx *c, const void *B, void *HB, bool gba
     };
 
     if (G >= 2) {
-        // first pass, per input state
+        // first pass, per input state; the chains out of a state continue in
+        // further programs of the same launch (each re-reading B_b) when
+        // they outgrow one program's matrix or instruction table (multi-set
+        // inter-state terms carry an overlap matrix on every mode)
         std::vector<TaskBuilder> first;
         for (int b = 0; b < c->ns; b++) {
             const std::vector<int64_t> sz = state_sizes(b);
-            TaskBuilder tb(c, 0, 1, prod_range(sz, c->groups[0].second, nm));
+            const int64_t Rb = prod_range(sz, c->groups[0].second, nm);
+            TaskBuilder tb(c, 0, 1, Rb);
             tb.load(0, c->at(B, c->off[b]), sz);
             const std::vector<int> dm = diag_ops(b, 0);
             if (dm.empty()) {
@@ -1225,6 +1229,11 @@ std::unique_ptr<Plan> build_apply_h(mg_ctx *c, const void *B, void *HB, bool gba
                 if (ch.rhs != b || chain_ops(ch, 0).empty()) continue;
                 // first op reads the untouched input brick (buffer 0)
                 const std::vector<ChainOp> ops = chain_ops(ch, 0);
+                if (tb.t.nmat + (int)ops.size() > kMaxBrickMats || tb.t.ninstr + (int)ops.size() + 1 > kMaxBrickInstr) {
+                    first.push_back(tb);
+                    tb = TaskBuilder(c, 0, 1, Rb);
+                    tb.load(0, c->at(B, c->off[b]), sz);
+                }
                 const ChainOp &o = ops[0];
                 tb.op(0, 1, o.mode, opm(o.mode, o.op), 0, (int)c->NS(o.mode, ch.lhs), 0);
                 csz[k][o.mode] = c->NS(o.mode, ch.lhs);
@@ -3227,7 +3236,7 @@ int mg_ensemble_propagate(const mg_problem *const *probs, const double *const *A
     std::mutex mu;
     std::condition_variable cv;
     double in_flight = 0.0;
-    int64_t active = 0, next = 0;
+    int64_t active = 0, next = 0, done = 0;
     std::string first_err;
     auto worker = [&]() {
         if (hipSetDevice(dev) != hipSuccess) return;
@@ -3270,6 +3279,13 @@ int mg_ensemble_propagate(const mg_problem *const *probs, const double *const *A
             status[i] = rc;
             if (rc != 0 && first_err.empty())
                 first_err = "member " + std::to_string(i) + ": " + (msg.empty() ? "creation failed" : msg);
+            done++;
+            if (flags & 2) {
+                std::fprintf(stderr, "[ensemble] member %lld done (%lld of %lld), status %d, %.2f s\n", (long long)i,
+                             (long long)done, (long long)n, rc,
+                             std::chrono::duration<double>(std::chrono::steady_clock::now() - w0).count());
+                std::fflush(stderr);
+            }
             active--;
             in_flight -= est[(size_t)i];
             cv.notify_all();

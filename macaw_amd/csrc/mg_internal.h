@@ -7,7 +7,11 @@
 
 namespace mg {
 
-constexpr int kMaxGroupModes = 8;   // modes fused into one LDS brick
+// Modes fused into one LDS brick.  The mean-field Gram program of a term
+// with an operator on every mode of a q-mode group holds q(q-1)/2 + q - 1 op
+// matrices (multi-set inter-state terms have an overlap on every mode); q = 6
+// keeps that within kMaxBrickMats.
+constexpr int kMaxGroupModes = 6;
 constexpr int kMaxDofPerMode = 4;
 constexpr int kNKrylovMax = 20;     // cmf2_integration.f90:12
 
@@ -89,6 +93,8 @@ struct GramTask {
 //   kBZero     a <- 0
 constexpr int kMaxBrickInstr = 40;
 constexpr int kMaxBrickMats = 24;
+static_assert(kMaxGroupModes * (kMaxGroupModes - 1) / 2 + kMaxGroupModes - 1 <= kMaxBrickMats,
+              "a mean-field Gram program over a full group must fit the matrix table");
 constexpr int kMaxBrickBufs = 3;
 //   kBGram     slot (mat) += sum over the brick's fibers of mode p of
 //              conj(a[f, i]) b[f, j], i < ni, j < no (ni, no <= 8): partial

@@ -56,7 +56,7 @@ def members(n: int, seed: int = 4000, tfinal: float = 1.0, eps_int: float = 1e-7
     return out
 
 
-def propagate(probs, t_fs, precision=1, threads=8, mem_fraction=0.7, autotune=False, strict=True):
+def propagate(probs, t_fs, precision=1, threads=8, mem_fraction=0.7, autotune=False, strict=True, progress=False):
     """Run mg_ensemble_propagate; returns (outs per member, pops per member,
     member wall seconds, total wall seconds).  strict=False: failed members
     are returned with no outputs, and the result gains (status, message)."""
@@ -85,7 +85,8 @@ def propagate(probs, t_fs, precision=1, threads=8, mem_fraction=0.7, autotune=Fa
     a_arr = (f64p * n)(*a_ptrs)
     p_arr = (f64p * n)(*p_ptrs)
     t0 = time.perf_counter()
-    rc = L.mg_ensemble_propagate(probs_arr, a_arr, p_arr, n, precision, threads, mem_fraction, 1 if autotune else 0,
+    flags = (1 if autotune else 0) | (2 if progress else 0)
+    rc = L.mg_ensemble_propagate(probs_arr, a_arr, p_arr, n, precision, threads, mem_fraction, flags,
                                  0.0, t_fs / AU2FS, outs, pops.ctypes.data_as(f64p), max_out,
                                  n_out.ctypes.data_as(i64p), status.ctypes.data_as(i64p), wall.ctypes.data_as(f64p),
                                  err, len(err))
@@ -123,8 +124,9 @@ def main(argv=None):
     res = {"members": a.members, "fs_per_member": a.fs, "threads": a.threads, "configurations": sizes}
     runs = {}
     for tag, prec in (("mixed", 1), ("fp64", 0)):
+        log(f"[ensemble] {tag}: propagating {a.members} members over {a.fs} fs")
         outs, pops, wall, total, status, msg = propagate(probs, a.fs, prec, a.threads, a.mem_fraction, a.autotune,
-                                                         strict=False)
+                                                         strict=False, progress=True)
         runs[tag] = (outs, pops, status)
         nok = int((status == 0).sum())
         res[tag] = {"wall_s": total, "ensemble_fs_per_s": nok * a.fs / total, "member_wall_s": wall.tolist(),

@@ -73,3 +73,36 @@ def test_ensemble_reports_member_failure():
     probs = small_members(2)
     with pytest.raises(EngineError, match="member .*precision must be"):
         ensemble.propagate(probs, 0.5, precision=2, threads=2)
+
+
+@gpu
+@pytest.mark.parametrize("precision,tol", [(0, 1e-12), (1, 1e-5)])
+def test_many_interstate_chains_per_state_match_oracle(precision, tol, tmp_path):
+    """Member 14 (4 states, 8 modes, 4 SPFs): four multi-set inter-state
+    chains leave state 2, each with an overlap on every mode of the 5-mode
+    first group, more op matrices than one brick program holds; the first
+    pass continues them in further programs.  H.A, rho and the mean fields
+    against the oracle."""
+    from oracle_lib import Oracle
+    from macaw_amd.engine import Engine
+    from macaw_amd.lvc import make_problem
+    prob = make_problem(ensemble.member_spec(14), str(tmp_path))
+    rng = np.random.default_rng(14)
+    A = rng.normal(size=prob.nAConftot) + 1j * rng.normal(size=prob.nAConftot)
+    A /= np.linalg.norm(A)
+    psi = prob.psi0 + 0.05 * (rng.normal(size=prob.dimpsi) + 1j * rng.normal(size=prob.dimpsi))
+    o = Oracle(prob)
+    o.set_state(A, psi)
+    o.h1mode()
+    e = Engine(prob, precision=precision)
+    e.set_state(A, psi)
+    e.h1mode(o.nopmax)
+    B = rng.normal(size=prob.nAConftot) + 1j * rng.normal(size=prob.nAConftot)
+    def rel(a, b):   # norm-wise relative error (FP32 rounding in the mixed form)
+        return np.linalg.norm(a - b) / np.linalg.norm(b)
+    assert rel(e.apply_h(B), o.apply_h(B)) <= tol
+    r_o = o.rho()
+    assert rel(e.rho(), r_o) <= tol
+    mf_o, _, _ = o.meanfield()
+    mf_e, _ = e.meanfield(o.nopmax)
+    assert rel(mf_e, mf_o) <= tol
