@@ -3263,15 +3263,21 @@ int mg_ensemble_propagate(const mg_problem *const *probs, const double *const *A
                 double ms = 0.0;
                 rc = mg_autotune(c, 1, &b, &ms);
             }
-            std::vector<double> pp((size_t)max_out * (size_t)std::max<int64_t>(1, probs[i]->nstates));
-            if (rc == 0)
-                rc = mg_propagate(c, t0, tfinal, outs + i * max_out, pp.data(), max_out, &n_out[i], nullptr, 0,
-                                  nullptr);
-            if (pops)
-                for (int64_t k = 0; k < n_out[i]; k++)
-                    for (int64_t st = 0; st < probs[i]->nstates && st < 8; st++)
-                        pops[((size_t)i * max_out + k) * 8 + st] = pp[(size_t)k * probs[i]->nstates + st];
-            std::string msg = (rc != 0 && c) ? c->err : std::string();
+            std::string msg;
+            try {
+                std::vector<double> pp((size_t)max_out * (size_t)std::max<int64_t>(1, probs[i]->nstates));
+                if (rc == 0)
+                    rc = mg_propagate(c, t0, tfinal, outs + i * max_out, pp.data(), max_out, &n_out[i], nullptr, 0,
+                                      nullptr);
+                if (pops)
+                    for (int64_t k = 0; k < n_out[i]; k++)
+                        for (int64_t st = 0; st < probs[i]->nstates && st < 8; st++)
+                            pops[((size_t)i * max_out + k) * 8 + st] = pp[(size_t)k * probs[i]->nstates + st];
+                if (rc != 0 && c) msg = c->err;
+            } catch (const std::exception &e) {   // host allocation failure
+                rc = 2;
+                msg = e.what();
+            }
             if (c) mg_destroy(c);
             if (member_wall)
                 member_wall[i] = std::chrono::duration<double>(std::chrono::steady_clock::now() - w0).count();

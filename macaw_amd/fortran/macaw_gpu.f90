@@ -132,6 +132,18 @@ module macaw_gpu
          integer(c_int64_t), value :: max_att
          integer(c_int64_t), intent(out) :: n_att
       end function
+      ! Learned CMF2 step proposal (no counterpart in the reference): params =
+      ! the network's MG_MLP_NPARAM reals; pass c_null_ptr for the classical rule.
+      integer(c_int) function mg_set_step_model(ctx, params) bind(C, name='mg_set_step_model')
+         import :: c_int, c_ptr
+         type(c_ptr), value :: ctx, params
+      end function
+      ! HIP-graph replay of the launch plans on (1) or off (0); on by default.
+      integer(c_int) function mg_set_graphs(ctx, enable) bind(C, name='mg_set_graphs')
+         import :: c_int, c_int64_t, c_ptr
+         type(c_ptr), value :: ctx
+         integer(c_int64_t), value :: enable
+      end function
    end interface
 
 contains
