@@ -78,6 +78,35 @@ def roofline(eng, prob, reps, peak):
     }
 
 
+def gram_profile(eng, prob, peak, reps=5):
+    """Density matrices and mean fields of the resident state: host-timed
+    calls (each ends in a stream synchronisation and a copy of the small
+    matrices), against the algorithmic flops of their all-but-one-mode
+    contractions: rho, every (state, mode); mean fields, the Grams of the
+    inter-state terms (mirror pairs counted once) on every mode."""
+    nconf = prob.nAConf()
+    n = prob.nSPF
+    rho_flops = sum(8.0 * nconf[s] * n[m, s] for s in range(prob.nstates) for m in range(prob.nmodes))
+    lhs, rhs = np.asarray(prob.lhs_el).ravel() - 1, np.asarray(prob.rhs_el).ravel() - 1
+    inter = [t for t in range(prob.nHam) if lhs[t] != rhs[t]]
+    mf_flops = 0.5 * sum(8.0 * nconf[rhs[t]] * n[m, rhs[t]] for t in inter for m in range(prob.nmodes))
+    eng.rho()
+    eng.meanfield()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        eng.rho()
+    t_rho = (time.perf_counter() - t0) / reps
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        eng.meanfield()
+    t_mf = (time.perf_counter() - t0) / reps
+    res = {"rho_ms": t_rho * 1e3, "rho_flops_alg": rho_flops, "rho_TFLOPs": rho_flops / t_rho / 1e12,
+           "rho_frac": rho_flops / t_rho / 1e12 / peak, "meanfield_ms": t_mf * 1e3,
+           "meanfield_gram_flops_alg": mf_flops, "meanfield_TFLOPs": mf_flops / t_mf / 1e12,
+           "meanfield_frac": mf_flops / t_mf / 1e12 / peak}
+    return {k: float(v) for k, v in res.items()}
+
+
 def cpu_baseline(prob, ha_per_fs, threads):
     """Oracle Calc_DHD_B at the full shape on `threads` host cores; the CPU
     fs/s is bounded above by 1 / (H.A applications per fs * t_HA)."""
@@ -195,6 +224,10 @@ def main():
     eng, mx = run_engine(prob, 1, args.warmup, args.steps, not args.no_autotune, model)
     rl = roofline(eng, prob, args.roofline_reps, PEAK_FP32_TFLOPS)
     log(f"roofline (mixed): {rl['achieved']:.2f} TFLOP/s, H.A {rl['HA_ms']:.2f} ms")
+    gp = gram_profile(eng, prob, PEAK_FP32_TFLOPS)
+    log(f"rho {gp['rho_ms']:.2f} ms ({gp['rho_TFLOPs']:.1f} TFLOP/s), mean fields {gp['meanfield_ms']:.2f} ms "
+        f"({gp['meanfield_TFLOPs']:.1f} TFLOP/s, Gram contractions)")
+    rl.update({"contractions_" + k: v for k, v in gp.items()})
     eng.close()
     value = mx["fs"] / mx["wall"]
     last = mx["outs"][-1]
