@@ -33,7 +33,7 @@ AU2FS = 0.02418884326505
 
 
 def member_spec(k: int, seed: int = 4000, tfinal: float = 1.0, eps_int: float = 1e-7) -> LVCSpec:
-    """Shape and energies of ensemble member k (deterministic in seed + k)."""
+    """Shape and energies of ensemble member k (deterministic in (seed, k))."""
     rng = np.random.default_rng([seed, k])
     ns = int(rng.integers(2, 5))
     nm = int(rng.integers(4, 9))
