@@ -113,6 +113,7 @@ def main(argv=None):
     ap.add_argument("--threads", type=int, default=8)
     ap.add_argument("--mem-fraction", type=float, default=0.7)
     ap.add_argument("--autotune", action="store_true")
+    ap.add_argument("--only", choices=("mixed", "fp64"), help="run one precision only (no comparison)")
     ap.add_argument("--json")
     a = ap.parse_args(argv)
     log = lambda m: print(m, flush=True)
@@ -124,6 +125,8 @@ def main(argv=None):
     res = {"members": a.members, "fs_per_member": a.fs, "threads": a.threads, "configurations": sizes}
     runs = {}
     for tag, prec in (("mixed", 1), ("fp64", 0)):
+        if a.only and tag != a.only:
+            continue
         log(f"[ensemble] {tag}: propagating {a.members} members over {a.fs} fs")
         outs, pops, wall, total, status, msg = propagate(probs, a.fs, prec, a.threads, a.mem_fraction, a.autotune,
                                                          strict=False, progress=True)
@@ -133,6 +136,11 @@ def main(argv=None):
                     "failed": [int(i) for i in np.nonzero(status)[0]], "first_error": msg}
         log(f"[ensemble] {tag}: {total:.1f} s, {nok * a.fs / total:.3f} member-fs/s "
             f"(slowest member {wall.max():.1f} s); {a.members - nok} failed{': ' + msg if msg else ''}")
+    if a.only:
+        if a.json:
+            with open(a.json, "w") as fh:
+                json.dump(res, fh)
+        return 0
     dC = []
     dP = []
     for i in range(a.members):
