@@ -82,6 +82,10 @@ class InputDeck:
     normalize: int = 0
     init_state: int = 1
     init_spf: dict = field(default_factory=dict)
+    # INIT-WF restart: file, chosen state (0 = all), file state per state
+    restart_file: str = ""
+    restart_state: int = 0
+    restart_assoc: list = field(default_factory=list)
     basedir: str = "."
 
 
@@ -225,10 +229,27 @@ def _parse_grid(line: str):
 
 
 def _parse_init_wf(deck: InputDeck, block: list) -> None:
-    """single-conf INIT-WF block (source/init_calc.f90:482-752, el/gauss branches)."""
+    """INIT-WF block: ``restart`` (source/init_calc.f90:290-333: file name,
+    chosen state, then one ``state <-- file state`` line per state) or
+    ``single-conf`` (source/init_calc.f90:482-752, el/gauss branches)."""
     rows = [b for b in block if b.strip()]
+    if rows and rows[0].strip() == "restart":
+        if len(rows) < 3 + deck.nstates:
+            raise InputError("INIT-WF restart: expected the file, the initial state and one association per state")
+        deck.restart_file = rows[1].strip()
+        deck.restart_state = int(rows[2].split()[0])
+        assoc = {}
+        for row in rows[3:3 + deck.nstates]:
+            j = row.find("<--")
+            if j < 0:
+                raise InputError("Error in INIT-WF: Electronic state associations must be specified")
+            assoc[int(row[:j])] = int(row[j + 3:].split()[0])
+        if sorted(assoc) != list(range(1, deck.nstates + 1)):
+            raise InputError("INIT-WF restart: every electronic state needs one association")
+        deck.restart_assoc = [assoc[s] for s in range(1, deck.nstates + 1)]
+        return
     if not rows or rows[0].strip() != "single-conf":
-        raise InputError("The INIT-WF block must start with the keyword single-conf")
+        raise InputError("The INIT-WF block must start with the keyword single-conf or restart")
     m = re.search(r"el\s+(\d+)", rows[1])
     if not m:
         raise InputError("single-conf: expected 'el <state>'")

@@ -171,7 +171,12 @@ def build_problem(deck: InputDeck, operator_path: str | None = None) -> Problem:
                 raise ValueError("A mode cannot have simultaneously dvr and gwp representation")
         prob.dof_gwp = dof_gwp
         prob.gwp_a = np.zeros((maxdof, int(nSPF.max()), nm), order="F")
-    prob.A0, prob.psi0 = initial_wavefunction(prob, deck)
+    if deck.restart_file:
+        from .restart import read_restart, restart_state
+        path = deck.restart_file if os.path.isabs(deck.restart_file) else os.path.join(deck.basedir, deck.restart_file)
+        prob.A0, prob.psi0 = restart_state(prob, read_restart(path), deck.restart_state, deck.restart_assoc)
+    else:
+        prob.A0, prob.psi0 = initial_wavefunction(prob, deck)
     return prob
 
 
