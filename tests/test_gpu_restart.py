@@ -24,7 +24,9 @@ def test_resumed_propagation_matches_uninterrupted(name, precision, tmp_path):
     _, pops_full, _ = e.propagate(0.0, t2)
     e.close()
     e = Engine(prob, precision=precision)
-    e.propagate(0.0, t1)
+    # a run over [t0, t] also propagates the interval after its last output:
+    # outputs up to t1 - tout leave the state at t1
+    e.propagate(0.0, t1 - prob.writetime)
     A, psi = e.get_state()
     e.close()
     path = str(tmp_path / "restart")
