@@ -11,7 +11,12 @@ propagate_wf (propagator.f90:257-306) and writes, in Macaw's formats:
 * ``spop`` (more than one state): header of propagator.f90:31, then
   ``(f10.4,2x,30(2x,f10.7))`` time [fs] and the state populations
   (output.f90:243-246);
-* ``restart``: the state at the last output time (output.f90:230-241).
+* ``restart``: the state at the last output time (output.f90:230-241);
+* ``output``: per output time the block of output.f90:152-224 (time, norm,
+  energy and its change from the start, electronic populations, natural
+  populations of every mode, and for Gaussian modes the gross Gaussian
+  populations and the population errors of dropping one Gaussian).  The
+  run's header lines (propagator.f90:71-85) are not written.
 
 The reference propagates one output interval past its last output; this
 driver stops at the last output time, whose state the restart file holds.
@@ -52,6 +57,73 @@ def write_spop(path: str, outs, pops) -> None:
             fh.write(spop_line(o.time, p) + "\n")
 
 
+def _mode_lines(label_values):
+    """'(a,i3,a,6(2x,f8.5))' and '(a,6(2x,f8.5))' continuation lines."""
+    lines = []
+    for im, vals in label_values:
+        vals = list(vals)
+        lines.append(f"    Mode {im:3d}:" + "".join(f"  {v:8.5f}" for v in vals[:6]))
+        for j in range(6, len(vals), 6):
+            lines.append("             " + "".join(f"  {v:8.5f}" for v in vals[j:j + 6]))
+    return lines
+
+
+def mode_quantities(prob, rho, psi):
+    """Natural populations of every (state, mode), and for Gaussian modes the
+    gross Gaussian populations and population errors (output.f90:94-135).
+    `rho[s, m]` as the engine returns it (the transpose of rho(j1, j2))."""
+    from .restart import gaussian_overlap
+    ns, nm = prob.nstates, prob.nmodes
+    ip, _ = prob.ipsiSPF()
+    npar = prob.npar()
+    nat, ggp, perr = {}, {}, {}
+    for s in range(ns):
+        for m in range(nm):
+            n = int(prob.nSPF[m, s])
+            r = rho[s, m][:n, :n].T                     # rho(j1, j2)
+            if not prob.gwp_mode(m):
+                nat[s, m] = np.sort(np.linalg.eigvalsh(r))[::-1]
+                continue
+            jel = s if prob.MultiSet[m] else 0
+            nj = int(prob.nSPF[m, jel])
+            start = int(ip[m, jel]) - 1
+            xi = psi[start:start + nj * int(npar[m])].reshape(int(npar[m]), nj, order="F")
+            S = gaussian_overlap(prob, m, xi)[:n, :n]   # S00(j1, j2) = <g_j1|g_j2>
+            w, V = np.linalg.eigh(S)
+            s12 = np.conj((V * np.sqrt(np.maximum(w, 0.0))) @ V.conj().T)   # conjugated square root
+            nat[s, m] = np.sort(np.linalg.eigvalsh(s12 @ r @ s12))[::-1]
+            g = np.array([sum((r[j2, j1] * S[j2, j1]).real for j2 in range(j1))
+                          + sum((r[j1, j2] * S[j1, j2]).real for j2 in range(j1, n)) for j1 in range(n)])
+            e = np.abs(2.0 * g - np.diag(r).real)
+            ggp[s, m] = g
+            perr[s, m] = np.sort(e)[::-1]
+    return nat, ggp, perr
+
+
+def output_block(prob, out, pops, energy_start, nat, ggp, perr):
+    """One output time of Macaw's `output` file (output.f90:152-224)."""
+    from .inputfile import AU2EV
+    lines = ["", f"Time = {out.time * AU2FS:9.3f} fs",
+             f" Norm = {float(np.sum(pops)):12.8f} ,    "
+             f" Energy = {out.energy * AU2EV:9.4f} eV,    "
+             f" Delta-E = {(out.energy - energy_start) * 1000.0 * AU2EV:9.4f} meV"]
+    if prob.nstates > 1:
+        lines.append(" Electronic populations:" + "".join(f"  {p:9.6f}" for p in pops[:30]))
+    lines.append("  Natural populations:")
+    for s in range(prob.nstates):
+        lines.append(f"   State {s + 1}")
+        lines += _mode_lines((m + 1, nat[s, m]) for m in range(prob.nmodes))
+    gm = [m for m in range(prob.nmodes) if prob.gwp_mode(m)]
+    if gm:
+        for title, table in (("  Gross Gaussian populations:", ggp),
+                             ("  Errors on the population by neglecting one Gaussian:", perr)):
+            lines.append(title)
+            for s in range(prob.nstates):
+                lines.append(f"   State {s + 1}")
+                lines += _mode_lines((m + 1, table[s, m]) for m in gm)
+    return lines
+
+
 def run(input_path: str, outdir: str = ".", precision: int = 0, controller: str = "classical", log=print):
     from .engine import Engine
     from .problem import load_problem
@@ -62,16 +134,36 @@ def run(input_path: str, outdir: str = ".", precision: int = 0, controller: str 
         e.set_step_model(np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "step_controller.npy")))
     wt = prob.writetime
     t0, tf = prob.tinit, prob.tfinal
-    outs, pops, atts = [], np.zeros((0, prob.nstates)), []
-    t_end = t0
-    if tf - t0 >= wt * 0.9:
-        # outputs at t0 .. tf - wt; the state then stands at tf
-        outs, pops, atts = e.propagate(t0, tf - wt)
-        t_end = outs[-1].time + wt
-    last, lp = e.observables(t_end)
-    outs = list(outs) + [last]
-    pops = np.vstack([pops, lp[None, :]])
     os.makedirs(outdir, exist_ok=True)
+    outs, pops, atts, block = [], [], [], []
+    energy_start = None
+    t = t0
+    # one output interval per call (the step size carries over between
+    # calls), so that rho of every output time is at hand for `output`
+    while True:
+        rho = e.rho()
+        _, psi = e.get_state()
+        last = t >= tf - wt * 0.1
+        if last:
+            o, p = e.observables(t)
+            a = []
+        else:
+            ol, pl, a = e.propagate(t, t)
+            o, p = ol[0], pl[0]
+        if energy_start is None:
+            energy_start = o.energy * o.norm      # energyMF at the start (propagator.f90:96)
+        block += output_block(prob, o, p, energy_start, *mode_quantities(prob, rho, psi))
+        outs.append(o)
+        pops.append(p)
+        atts += list(a)
+        if last:
+            break
+        acc = [x for x in a if x.accepted]
+        t = acc[-1].time + acc[-1].h            # the engine's own accumulated time
+    t_end = t
+    pops = np.asarray(pops)
+    with open(os.path.join(outdir, "output"), "w") as fh:
+        fh.write("\n".join(block) + "\n")
     write_auto(os.path.join(outdir, "auto"), outs)
     if prob.nstates > 1:
         write_spop(os.path.join(outdir, "spop"), outs, pops)

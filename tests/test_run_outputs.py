@@ -50,6 +50,52 @@ def test_lines_match_fortran_edit_descriptors(tmp_path):
         assert RUN.spop_line(t / AU2FS, p) == out[2 * k + 1], k
 
 
+FORTRAN_OUTPUT = """
+program outfmt
+  implicit none
+  integer :: j1, j2, im, nS
+  double precision :: t, norm, e, de, pop(3), natpop(9)
+  read(*,*) t, norm, e, de, pop, natpop
+  write(*,'(a,f9.3,a)') 'Time = ', t, ' fs'
+  write(*,'(a,f12.8,a,4x,a,f9.4,a,4x,a,f9.4,a)') ' Norm = ', norm, ' ,', ' Energy = ', e, ' eV,', ' Delta-E = ', de, ' meV'
+  write(*,'(a,30(2x,f9.6))') ' Electronic populations:', pop
+  im = 2
+  nS = 9
+  write(*,'(a,i3,a,6(2x,f8.5))') '    Mode ', im, ':', (natpop(j1), j1 = 1, min(nS,6))
+  j2 = 7
+  do
+     if (j2 .gt. nS) exit
+     write(*,'(a,6(2x,f8.5))') '             ', (natpop(j1), j1 = j2, min(nS,j2 + 5))
+     j2 = j2 + 6
+  end do
+end program outfmt
+"""
+
+
+@pytest.mark.skipif(not os.path.exists(FLANG), reason="flang not available")
+def test_output_block_matches_fortran_edit_descriptors(tmp_path):
+    import types
+    src = tmp_path / "outfmt.f90"
+    src.write_text(FORTRAN_OUTPUT)
+    exe = tmp_path / "outfmt"
+    subprocess.check_call([FLANG, "-O0", "-o", str(exe), str(src)], cwd=str(tmp_path))
+    AU2EV = 27.2113961
+    t, norm_pops = 37.25, (0.25, 0.5, 0.2499999)
+    energy, energy_start = 0.031415, 0.0314
+    nat = [0.9, 0.05, 0.03, 0.01, 0.005, 0.003, 0.001, 0.0005, 0.0005]
+    vals = [t, sum(norm_pops), energy * AU2EV, (energy - energy_start) * 1000 * AU2EV, *norm_pops, *nat]
+    out = subprocess.run([str(exe)], input=" ".join(repr(float(v)) for v in vals) + "\n", capture_output=True,
+                         text=True, check=True).stdout.splitlines()
+    prob = types.SimpleNamespace(nstates=3, nmodes=2, gwp_mode=lambda m: False)
+    o = types.SimpleNamespace(time=t / AU2FS, energy=energy)
+    lines = RUN.output_block(prob, o, np.array(norm_pops), energy_start,
+                             {(s, m): nat for s in range(3) for m in range(2)}, {}, {})
+    assert lines[0] == ""
+    assert lines[1:4] == out[:3]
+    mode2 = lines.index("   State 1") + 3             # state 1: mode 1 takes 2 lines (6 + 3 values)
+    assert lines[mode2: mode2 + 2] == out[3:5]
+
+
 @pytest.mark.gpu
 @pytest.mark.skipif(not os.path.exists(FLANG), reason="flang not available")
 def test_standalone_run_matches_fortran_driver(tmp_path):
@@ -67,3 +113,36 @@ def test_standalone_run_matches_fortran_driver(tmp_path):
     assert (tmp_path / "py" / "spop").read_text() == (tmp_path / "spop").read_text()
     d = read_restart(str(tmp_path / "py" / "restart"))
     assert abs(d["time"] - outs[-1].time) < 1e-9 and d["A"].size == prob.nAConftot
+    text = (tmp_path / "py" / "output").read_text()
+    assert text.count("Time = ") == len(outs)
+    # natural populations of a mode add up to the state population
+    blocks = text.split("Time = ")[1:]
+    for blk, p in zip(blocks, pops):
+        states = blk.split("   State ")[1:]
+        for s, st in enumerate(states):
+            for line in st.splitlines()[1:]:
+                if line.startswith("    Mode"):
+                    v = [float(x) for x in line.split(":")[1].split()]
+                    assert abs(sum(v) - p[s]) < 1e-4
+
+
+@pytest.mark.gpu
+def test_standalone_run_gaussian_output_sums(tmp_path):
+    """G-MCTDH input: per mode, the natural populations and (Gaussian modes)
+    the gross Gaussian populations each add up to the state population
+    (tr(S^1/2 rho S^1/2) = sum rho o S)."""
+    from macaw_amd.lvc import make_problem
+    make_problem("g1", str(tmp_path))
+    outs, pops = RUN.run(str(tmp_path / "g1.inp"), str(tmp_path / "py"))
+    text = (tmp_path / "py" / "output").read_text()
+    assert "Gross Gaussian populations" in text
+    blocks = text.split("Time = ")[1:]
+    assert len(blocks) == len(outs)
+    for blk, p in zip(blocks, pops):
+        body = blk.split("  Errors on the population")[0]
+        for part in body.split("  Natural populations:")[1].split("  Gross Gaussian populations:"):
+            for s, st in enumerate(part.split("   State ")[1:]):
+                for line in st.splitlines()[1:]:
+                    if line.startswith("    Mode"):
+                        v = [float(x) for x in line.split(":")[1].split()]
+                        assert abs(sum(v) - p[s]) < 1e-4, line
