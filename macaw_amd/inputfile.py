@@ -292,6 +292,9 @@ def parse_operator_file(path: str, nstates: int) -> list:
     with a != b gives both directions with the same coefficient, ``Za&b`` one
     direction.
     """
+    if not os.path.exists(path):
+        # source/input.f90:529-536
+        raise InputError(f"Input error: Operator file, {path} not found")
     terms = []
     with open(path) as fh:
         for raw in fh:

@@ -1,0 +1,54 @@
+"""Failure handling of the reference (SURVEY failure-detection row): the
+errors are raised on the host, so every GPU kernel still ends normally.
+
+* missing operator file: input error (input.f90:529-536);
+* CMF2 attempts rejected below the minimum step: error instead of the
+  reference's endless reject loop (cmf2_base.f90:181-184; the build's guard);
+* Krylov space exhausted (nKrylovMax = 20 reached): error
+  (cmf2_integration.f90:64-70).
+"""
+import re
+
+import pytest
+
+from macaw_amd.inputfile import InputError
+from macaw_amd.lvc import make_problem
+from macaw_amd.problem import load_problem
+
+gpu = pytest.mark.gpu
+
+
+def test_missing_operator_file(tmp_path):
+    make_problem("t1", str(tmp_path))
+    inp = (tmp_path / "t1.inp").read_text()
+    (tmp_path / "bad.inp").write_text(re.sub(r"Hamiltonian\s*=\s*\S+", "Hamiltonian = nowhere.op", inp))
+    with pytest.raises(InputError, match="nowhere.op not found"):
+        load_problem(str(tmp_path / "bad.inp"))
+
+
+@gpu
+def test_engine_min_step_guard(tmp_path):
+    """A rejected attempt whose retry step falls below min_stepsize ends the
+    step with an error (the reference's reject loop would shrink h forever).
+    The rejection is forced through the attempt replay and min_stepsize is
+    set far above any retry step, so the guard fires on the first retry."""
+    from macaw_amd.abi import MgAttempt
+    from macaw_amd.engine import Engine, EngineError
+    prob = make_problem("t1", str(tmp_path))
+    prob.min_stepsize = 1.0e6
+    e = Engine(prob)
+    e.set_replay([MgAttempt(time=0.0, h=1.0, err_A=0.0, err_psi=0.0, accepted=0, nkrylov1=0, nkrylov2=0)])
+    with pytest.raises(EngineError, match="minstep"):
+        e.cmf2_step(0.0, 1.0)
+    # the context stays usable after the error
+    out, pops = e.observables(0.0)
+    assert abs(sum(pops) - 1.0) < 1e-6
+
+
+@gpu
+def test_engine_krylov_overflow(tmp_path):
+    from macaw_amd.engine import Engine, EngineError
+    prob = make_problem("t1", str(tmp_path))
+    e = Engine(prob)
+    with pytest.raises(EngineError, match="Maximum number of Krylov vectors"):
+        e.cmf2_step(0.0, 1.0e3)     # ~24 fs in one step: the Lanczos error estimate never falls below 1e-8
