@@ -694,6 +694,7 @@ void allocate(mg_ctx *c) {
         d.mode = m;
         d.nG = (int)c->ngpm[m];
         d.nsets = c->nsets(m);
+        if (d.nsets > kMaxSets) throw HipError{"too many electronic states for a multi-set mode (at most 8)"};
         int blk = 0;
         for (int s = 0; s < d.nsets; s++) {
             d.off[s] = (int)c->IP(m, s);
@@ -715,7 +716,7 @@ void allocate(mg_ctx *c) {
         }
         d.nop = nop;
         d.scr_off = (int)scr;
-        scr += (size_t)10 * blk + (size_t)2 * d.nG * c->nspfmax + 2 * nn;
+        scr += (size_t)10 * blk + (size_t)2 * d.nG * c->nspfmax + (size_t)2 * kMaxSets * nn;
     }
     c->ndvr_desc = (int)md.size();
     c->dspfmodes.alloc(sizeof(SpfModeDesc) * std::max<size_t>(1, md.size()));

@@ -1281,8 +1281,8 @@ __device__ SpfScratch carve(double2 *base, const SpfModeDesc &d, int nspfmax) {
     s.perr = p; p += d.blk;
     s.sect = p; p += d.nG * nspfmax;
     s.sect2 = p; p += d.nG * nspfmax;
-    s.ovl = p; p += nspfmax * nspfmax;
-    s.od = p;
+    s.ovl = p; p += (size_t)kMaxSets * nspfmax * nspfmax;     // one n x n block per set
+    s.od = p;                                                 // kMaxSets blocks as well
     return s;
 }
 
@@ -1315,11 +1315,15 @@ __device__ void dev_der_spf(const SpfModeDesc &d, const SpfParams &P, const DofT
         }
         __syncthreads();
     }
-    // projector: d <- d - phi (phi^H phi)^-1 phi^H d
+    // projector: d <- d - phi (phi^H phi)^-1 phi^H d, every set at once:
+    // the sets' overlaps in parallel, each set's Cholesky solve on lane 0 of
+    // its own wave, then the updates (each set's arithmetic is unchanged)
+    const int nn = nmax * nmax;
     for (int s = 0; s < d.nsets; s++) {
         const int n = d.nspf[s];
         const double2 *phi = in + loff[s];
-        double2 *dd = out + loff[s];
+        const double2 *dd = out + loff[s];
+        double2 *ovl = S.ovl + nn * s, *od = S.od + nn * s;
         for (int e = tid; e < n * n; e += nth) {
             const int a = e % n, b = e / n;
             double2 x = zero2(), y = zero2();
@@ -1327,13 +1331,18 @@ __device__ void dev_der_spf(const SpfModeDesc &d, const SpfParams &P, const DofT
                 x = cadd(x, cconjmul(phi[g + nG * a], phi[g + nG * b]));
                 y = cadd(y, cconjmul(phi[g + nG * a], dd[g + nG * b]));
             }
-            S.ovl[a + n * b] = x;
-            S.od[a + n * b] = y;
+            ovl[a + n * b] = x;
+            od[a + n * b] = y;
         }
-        __syncthreads();
-        if (tid == 0) {
+    }
+    __syncthreads();
+    {
+        const int lane = tid & 63, wave = tid >> 6, nw = (nth + 63) >> 6;
+        for (int s = wave; s < d.nsets; s += nw) {
+            if (lane != 0) continue;
+            const int n = d.nspf[s];
             // Cholesky S = U^H U (upper), then solve for each column of od
-            double2 *U = S.ovl;
+            double2 *U = S.ovl + nn * s, *od = S.od + nn * s;
             for (int j = 0; j < n; j++) {
                 double dg = U[j + n * j].x;
                 for (int k = 0; k < j; k++) dg -= U[k + n * j].x * U[k + n * j].x + U[k + n * j].y * U[k + n * j].y;
@@ -1347,7 +1356,7 @@ __device__ void dev_der_spf(const SpfModeDesc &d, const SpfParams &P, const DofT
                 }
             }
             for (int c = 0; c < n; c++) {
-                double2 *y = S.od + n * c;
+                double2 *y = od + n * c;
                 for (int i = 0; i < n; i++) {
                     double2 v = y[i];
                     for (int k = 0; k < i; k++) v = csub(v, cconjmul(U[k + n * i], y[k]));
@@ -1362,15 +1371,21 @@ __device__ void dev_der_spf(const SpfModeDesc &d, const SpfParams &P, const DofT
                 }
             }
         }
-        __syncthreads();
+    }
+    __syncthreads();
+    for (int s = 0; s < d.nsets; s++) {
+        const int n = d.nspf[s];
+        const double2 *phi = in + loff[s];
+        double2 *dd = out + loff[s];
+        const double2 *od = S.od + nn * s;
         for (int e = tid; e < nG * n; e += nth) {
             const int g = e % nG, b = e / nG;
             double2 acc = zero2();
-            for (int k = 0; k < n; k++) cfma(acc, S.od[k + n * b], phi[g + nG * k]);
+            for (int k = 0; k < n; k++) cfma(acc, od[k + n * b], phi[g + nG * k]);
             dd[e] = csub(dd[e], acc);
         }
-        __syncthreads();
     }
+    __syncthreads();
 }
 
 // Calc_Err_SPF, DVR branch (cmf2_derivatives.f90:253-291).
