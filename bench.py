@@ -43,6 +43,19 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
+def describe(name, prob):
+    """Workload line of the JSON config, from the problem itself."""
+    if name == "c2":
+        return "LVC 4 states x 8 modes, HO-DVR 32, 8 SPFs (8^8 conf/state), CMF2 eps 1e-7"
+    gm = [m for m in range(prob.nmodes) if prob.gwp_mode(m)]
+    kinds = (f"{len(gm)} Gaussian modes ({int(prob.nSPF[gm[0], 0])} Gaussians), " if gm else "")
+    grid = [m for m in range(prob.nmodes) if not prob.gwp_mode(m)]
+    npts = int(prob.ngp[int(prob.iDOF[0, grid[0]]) - 1]) if grid else 0
+    return (f"LVC {prob.nstates} states x {prob.nmodes} modes, {kinds}{len(grid)} HO-DVR {npts} modes "
+            f"({int(prob.nSPF[grid[0], 0]) if grid else 0} SPFs), {prob.nAConftot} configurations, "
+            f"CMF2 eps {prob.eps_int:g}")
+
+
 def build_problem(name):
     from macaw_amd.lvc import make_problem
     with tempfile.TemporaryDirectory() as d:
@@ -235,7 +248,7 @@ def main():
         "metric": METRIC, "value": value, "unit": "fs/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": mx["wall"] / args.steps * 1e3, "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "c64/c128", "data": "synthetic",
-        "config": {"workload": "LVC 4 states x 8 modes, HO-DVR 32, 8 SPFs (8^8 conf/state), CMF2 eps 1e-7",
+        "config": {"workload": describe(args.workload, prob),
                    "precision": "mixed: complex FP32 H.A, Krylov v1.., mean-field chains; FP64 A, SPFs, "
                                 "rho, mean fields, reductions, DeltaPsi",
                    "step": "one 0.5 fs output interval", "attempts": len(mx["atts"]),
