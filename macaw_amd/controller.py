@@ -155,6 +155,8 @@ def main(argv=None):
     t.add_argument("--batch", type=int, default=256)
     t.add_argument("--lr", type=float, default=1e-3)
     t.add_argument("--seed", type=int, default=0)
+    t.add_argument("--repeat", type=int, nargs="*", default=[],
+                   help="how many times each --data file enters the training set (default 1 each)")
     v = sub.add_parser("evaluate")
     v.add_argument("--model", required=True)
     v.add_argument("--spec", default="c1")
@@ -173,8 +175,9 @@ def main(argv=None):
         from .engine import Engine
         from .lvc import make_problem
         parts = [np.load(p) for p in a.data]
-        X = np.concatenate([q["X"] for q in parts])
-        y = np.concatenate([q["y"] for q in parts])
+        rep = list(a.repeat) + [1] * (len(parts) - len(a.repeat))
+        X = np.concatenate([q["X"] for q, r in zip(parts, rep) for _ in range(r)])
+        y = np.concatenate([q["y"] for q, r in zip(parts, rep) for _ in range(r)])
         with tempfile.TemporaryDirectory() as d:
             e = Engine(make_problem("t1", d))
         t0 = time.perf_counter()
