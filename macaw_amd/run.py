@@ -12,6 +12,9 @@ propagate_wf (propagator.f90:257-306) and writes, in Macaw's formats:
   ``(f10.4,2x,30(2x,f10.7))`` time [fs] and the state populations
   (output.f90:243-246);
 * ``restart``: the state at the last output time (output.f90:230-241);
+* ``steps`` (``--steps``): header of propagator.f90:37-38, then per CMF2 step
+  ``(f12.6,3(2x,f12.6))`` time, attempted, done and suggested step [fs]
+  (propagator.f90:273-274);
 * ``output``: per output time the block of output.f90:152-224 (time, norm,
   energy and its change from the start, electronic populations, natural
   populations of every mode, and for Gaussian modes the gross Gaussian
@@ -124,7 +127,34 @@ def output_block(prob, out, pops, energy_start, nat, ggp, perr):
     return lines
 
 
-def run(input_path: str, outdir: str = ".", precision: int = 0, controller: str = "classical", log=print):
+STEPS_HEADER = ["#                Integration steps  [fs]", "#  time [fs]      attempted       done       suggested"]
+
+
+def steps_line(t, attempted, done, suggested) -> str:
+    """'(f12.6,3(2x,f12.6))' of propagator.f90:274, in fs."""
+    return f"{t * AU2FS:12.6f}" + "".join(f"  {v * AU2FS:12.6f}" for v in (attempted, done, suggested))
+
+
+def _interval_by_steps(e, prob, t, h, steps):
+    """One output interval as propagate_wf's CMF2 loop (propagator.f90:270-293)
+    through single CMF2 steps from step size h, recording (time, attempted,
+    done, suggested); the arithmetic is the engine's own interval loop, step
+    for step.  Returns the new time and step size."""
+    wt, minstep = prob.writetime, prob.min_stepsize
+    tnext = t + wt
+    while True:
+        done, nxt = e.cmf2_step(t, h)
+        steps.append((t, h, done, nxt))
+        t = t + done
+        if t >= tnext - minstep:
+            break
+        k = int((tnext - t) / nxt)
+        h = (tnext - t) if k == 0 else (tnext - t) / (k + 1)
+    return t, h
+
+
+def run(input_path: str, outdir: str = ".", precision: int = 0, controller: str = "classical", log=print,
+        steps: bool = False):
     from .engine import Engine
     from .problem import load_problem
     from .restart import write_restart
@@ -135,16 +165,17 @@ def run(input_path: str, outdir: str = ".", precision: int = 0, controller: str 
     wt = prob.writetime
     t0, tf = prob.tinit, prob.tfinal
     os.makedirs(outdir, exist_ok=True)
-    outs, pops, atts, block = [], [], [], []
+    outs, pops, atts, block, step_log = [], [], [], [], []
     energy_start = None
     t = t0
+    h = e.stepsize()                            # writetime * eps_int (propagator.f90:103)
     # one output interval per call (the step size carries over between
     # calls), so that rho of every output time is at hand for `output`
     while True:
         rho = e.rho()
         _, psi = e.get_state()
         last = t >= tf - wt * 0.1
-        if last:
+        if last or steps:
             o, p = e.observables(t)
             a = []
         else:
@@ -158,8 +189,11 @@ def run(input_path: str, outdir: str = ".", precision: int = 0, controller: str 
         atts += list(a)
         if last:
             break
-        acc = [x for x in a if x.accepted]
-        t = acc[-1].time + acc[-1].h            # the engine's own accumulated time
+        if steps:
+            t, h = _interval_by_steps(e, prob, t, h, step_log)
+        else:
+            acc = [x for x in a if x.accepted]
+            t = acc[-1].time + acc[-1].h        # the engine's own accumulated time
     t_end = t
     pops = np.asarray(pops)
     with open(os.path.join(outdir, "output"), "w") as fh:
@@ -167,6 +201,9 @@ def run(input_path: str, outdir: str = ".", precision: int = 0, controller: str 
     write_auto(os.path.join(outdir, "auto"), outs)
     if prob.nstates > 1:
         write_spop(os.path.join(outdir, "spop"), outs, pops)
+    if steps:
+        with open(os.path.join(outdir, "steps"), "w") as fh:
+            fh.write("\n".join(STEPS_HEADER + [steps_line(*r) for r in step_log]) + "\n")
     A, psi = e.get_state()
     write_restart(os.path.join(outdir, "restart"), prob, t_end, A, psi)
     e.close()
@@ -181,8 +218,9 @@ def main(argv=None):
     ap.add_argument("--outdir", default=".")
     ap.add_argument("--precision", type=int, default=0, choices=(0, 1))
     ap.add_argument("--controller", default="classical", choices=("classical", "learned"))
+    ap.add_argument("--steps", action="store_true", help="also write Macaw's `steps` file (one line per CMF2 step)")
     a = ap.parse_args(argv)
-    run(a.input, a.outdir, a.precision, a.controller, log=lambda m: print(m, flush=True))
+    run(a.input, a.outdir, a.precision, a.controller, log=lambda m: print(m, flush=True), steps=a.steps)
     return 0
 
 

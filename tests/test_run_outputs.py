@@ -146,3 +146,26 @@ def test_standalone_run_gaussian_output_sums(tmp_path):
                     if line.startswith("    Mode"):
                         v = [float(x) for x in line.split(":")[1].split()]
                         assert abs(sum(v) - p[s]) < 1e-4, line
+
+
+def test_steps_line_format():
+    # '(f12.6,3(2x,f12.6))' in fs
+    line = RUN.steps_line(1.0 / AU2FS, 0.01 / AU2FS, 0.01 / AU2FS, 0.015 / AU2FS)
+    assert line == "    1.000000      0.010000      0.010000      0.015000"
+
+
+@pytest.mark.gpu
+def test_steps_loop_reproduces_the_engine_loop(tmp_path):
+    """`--steps` drives the CMF2 loop one step at a time from the host; its
+    trajectory is the engine's own interval loop's (auto byte-equal), and the
+    steps file holds one line per accepted step, landing on the output times."""
+    from macaw_amd.lvc import make_problem
+    make_problem("t1", str(tmp_path))
+    RUN.run(str(tmp_path / "t1.inp"), str(tmp_path / "a"))
+    RUN.run(str(tmp_path / "t1.inp"), str(tmp_path / "b"), steps=True)
+    assert (tmp_path / "a" / "auto").read_text() == (tmp_path / "b" / "auto").read_text()
+    rows = np.loadtxt(str(tmp_path / "b" / "steps"), comments="#")
+    assert rows.shape[1] == 4 and len(rows) > 10
+    # times printed to 1e-6 fs: non-decreasing (the first steps are ~1e-7 fs)
+    assert np.all(np.diff(rows[:, 0]) >= 0) and np.all(rows[:, 2] <= rows[:, 1] + 1e-12)
+    assert abs(rows[-1, 0] + rows[-1, 2] - 5.0) < 2e-6          # the last step lands on tfinal
