@@ -2525,6 +2525,15 @@ double energy_gwp(mg_ctx *c, const double2 *A) {
 }
 
 // WriteOutput observables (output.f90:33-87, 143-150).
+// A wavefunction that is no longer finite ends the run with an error at the
+// next output time (the reference stops when LAPACK reports info != 0 on
+// such data, maths.f90:37-46).
+void check_finite(const mg_output &o, double t) {
+    if (!std::isfinite(o.norm) || !std::isfinite(o.energy) || !std::isfinite(o.auto_re) ||
+        !std::isfinite(o.auto_im))
+        throw HipError{"the wavefunction is not finite at t = " + std::to_string(t) + " au"};
+}
+
 void observables(mg_ctx *c, double t, mg_output *out, double *pops) {
     double2 *A = c->dA.as<double2>(), *psi = c->dpsi.as<double2>();
     if (c->ngwp) gwp_s00(c, psi);
@@ -3502,6 +3511,7 @@ int mg_propagate_vmf(mg_ctx *c, int64_t method, double t0, double tfinal, mg_out
         while (time < maxtime) {
             if (*n_out < max_out) {
                 observables(c, time, &outs[*n_out], pops ? pops + c->ns * *n_out : nullptr);
+                check_finite(outs[*n_out], time);
                 (*n_out)++;
             }
             const double tnext = time + wt;
@@ -3547,6 +3557,7 @@ int mg_propagate(mg_ctx *c, double t0, double tfinal, mg_output *outs, double *p
         while (time < maxtime) {
             if (*n_out < max_out) {
                 observables(c, time, &outs[*n_out], pops ? pops + c->ns * *n_out : nullptr);
+                check_finite(outs[*n_out], time);
                 (*n_out)++;
             }
             const double tnext = time + wt;

@@ -52,3 +52,19 @@ def test_engine_krylov_overflow(tmp_path):
     e = Engine(prob)
     with pytest.raises(EngineError, match="Maximum number of Krylov vectors"):
         e.cmf2_step(0.0, 1.0e3)     # ~24 fs in one step: the Lanczos error estimate never falls below 1e-8
+
+
+@gpu
+def test_non_finite_state_is_reported(tmp_path):
+    """A NaN in the coefficients is reported at the first output time, before
+    any CMF2 step runs on it (the kernels of the observables have no
+    data-dependent loops)."""
+    import numpy as np
+    from macaw_amd.engine import Engine, EngineError
+    prob = make_problem("t1", str(tmp_path))
+    e = Engine(prob)
+    A = prob.A0.copy()
+    A[3] = np.nan
+    e.set_state(A, prob.psi0)
+    with pytest.raises(EngineError, match="not finite"):
+        e.propagate(0.0, 1.0 / 0.02418884326505)
