@@ -98,6 +98,30 @@ def write_restart(path: str, prob, time: float, A: np.ndarray, psi: np.ndarray) 
         _write_record(fh, gwpa_array(prob).astype("<f8").tobytes(order="F"))
 
 
+def write_psi_record(fh, time: float, A: np.ndarray, psi: np.ndarray) -> None:
+    """One record of Macaw's `psi` trajectory file: time, AVector, psi in a
+    single unformatted record (propagator.f90:40-45, output.f90:248-252)."""
+    _write_record(fh, struct.pack("<d", float(time)) + np.ascontiguousarray(A, np.complex128).astype("<c16").tobytes()
+                  + np.ascontiguousarray(psi, np.complex128).astype("<c16").tobytes())
+
+
+def read_psi_file(path: str, nconf: int, dimpsi: int) -> list:
+    """Records (time, A, psi) of a `psi` trajectory file."""
+    import os
+    out = []
+    size = os.path.getsize(path)
+    with open(path, "rb") as fh:
+        while fh.tell() < size:
+            rec = _read_record(fh)
+            if len(rec) != 8 + 16 * (nconf + dimpsi):
+                raise RestartError("psi record does not match the layout")
+            (t,) = struct.unpack_from("<d", rec, 0)
+            A = np.frombuffer(rec, dtype="<c16", count=nconf, offset=8).astype(np.complex128)
+            psi = np.frombuffer(rec, dtype="<c16", count=dimpsi, offset=8 + 16 * nconf).astype(np.complex128)
+            out.append((t, A, psi))
+    return out
+
+
 def read_restart(path: str) -> dict:
     """Records of a restart file: nstates, nmodes, nSPF[nmodes, nstates],
     time, A, psi, GWPa (flat, Fortran order)."""

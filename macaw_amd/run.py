@@ -12,6 +12,9 @@ propagate_wf (propagator.f90:257-306) and writes, in Macaw's formats:
   ``(f10.4,2x,30(2x,f10.7))`` time [fs] and the state populations
   (output.f90:243-246);
 * ``restart``: the state at the last output time (output.f90:230-241);
+* ``psi`` (``--psi-every N``): the state (time, A, psi) as one unformatted
+  record every N output times from the first (propagator.f90:40-45,
+  294-298, output.f90:248-252);
 * ``steps`` (``--steps``): header of propagator.f90:37-38, then per CMF2 step
   ``(f12.6,3(2x,f12.6))`` time, attempted, done and suggested step [fs]
   (propagator.f90:273-274);
@@ -154,10 +157,10 @@ def _interval_by_steps(e, prob, t, h, steps):
 
 
 def run(input_path: str, outdir: str = ".", precision: int = 0, controller: str = "classical", log=print,
-        steps: bool = False):
+        steps: bool = False, psi_every: int = 0):
     from .engine import Engine
     from .problem import load_problem
-    from .restart import write_restart
+    from .restart import write_psi_record, write_restart
     prob = load_problem(input_path)
     e = Engine(prob, precision=precision)
     if controller == "learned":
@@ -171,9 +174,14 @@ def run(input_path: str, outdir: str = ".", precision: int = 0, controller: str 
     h = e.stepsize()                            # writetime * eps_int (propagator.f90:103)
     # one output interval per call (the step size carries over between
     # calls), so that rho of every output time is at hand for `output`
+    psi_fh = open(os.path.join(outdir, "psi"), "wb") if psi_every > 0 else None
+    k_out = 0
     while True:
         rho = e.rho()
-        _, psi = e.get_state()
+        A_t, psi = e.get_state()
+        if psi_fh is not None and k_out % psi_every == 0:
+            write_psi_record(psi_fh, t, A_t, psi)
+        k_out += 1
         last = t >= tf - wt * 0.1
         if last or steps:
             o, p = e.observables(t)
@@ -195,6 +203,8 @@ def run(input_path: str, outdir: str = ".", precision: int = 0, controller: str 
             acc = [x for x in a if x.accepted]
             t = acc[-1].time + acc[-1].h        # the engine's own accumulated time
     t_end = t
+    if psi_fh is not None:
+        psi_fh.close()
     pops = np.asarray(pops)
     with open(os.path.join(outdir, "output"), "w") as fh:
         fh.write("\n".join(block) + "\n")
@@ -219,8 +229,11 @@ def main(argv=None):
     ap.add_argument("--precision", type=int, default=0, choices=(0, 1))
     ap.add_argument("--controller", default="classical", choices=("classical", "learned"))
     ap.add_argument("--steps", action="store_true", help="also write Macaw's `steps` file (one line per CMF2 step)")
+    ap.add_argument("--psi-every", type=int, default=0,
+                    help="write the state to Macaw's `psi` file every N output times (0: never)")
     a = ap.parse_args(argv)
-    run(a.input, a.outdir, a.precision, a.controller, log=lambda m: print(m, flush=True), steps=a.steps)
+    run(a.input, a.outdir, a.precision, a.controller, log=lambda m: print(m, flush=True), steps=a.steps,
+        psi_every=a.psi_every)
     return 0
 
 

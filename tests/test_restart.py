@@ -179,3 +179,19 @@ def test_gaussian_problem_roundtrip(tmp_path):
     A, psi = R.restart_state(g, d, 0, list(range(1, g.nstates + 1)))
     assert np.array_equal(g.gwp_a, widths)
     assert np.array_equal(psi, g.psi0) and np.allclose(A, g.A0, atol=1e-14)
+
+
+def test_psi_trajectory_records(tmp_path):
+    prob = problem(tmp_path)
+    path = tmp_path / "psi"
+    states = [random_state(prob, 10 + k) for k in range(3)]
+    with open(path, "wb") as fh:
+        for k, (A, psi) in enumerate(states):
+            R.write_psi_record(fh, 20.0 * k, A, psi)
+    raw = path.read_bytes()
+    n = 8 + 16 * (prob.nAConftot + prob.dimpsi)
+    assert len(raw) == 3 * (n + 8) and struct.unpack_from("<i", raw, 0)[0] == n
+    recs = R.read_psi_file(str(path), prob.nAConftot, prob.dimpsi)
+    assert [r[0] for r in recs] == [0.0, 20.0, 40.0]
+    for (t, A, psi), (A0, psi0) in zip(recs, states):
+        assert np.array_equal(A, A0) and np.array_equal(psi, psi0)

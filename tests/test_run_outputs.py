@@ -162,10 +162,17 @@ def test_steps_loop_reproduces_the_engine_loop(tmp_path):
     from macaw_amd.lvc import make_problem
     make_problem("t1", str(tmp_path))
     RUN.run(str(tmp_path / "t1.inp"), str(tmp_path / "a"))
-    RUN.run(str(tmp_path / "t1.inp"), str(tmp_path / "b"), steps=True)
+    RUN.run(str(tmp_path / "t1.inp"), str(tmp_path / "b"), steps=True, psi_every=4)
     assert (tmp_path / "a" / "auto").read_text() == (tmp_path / "b" / "auto").read_text()
     rows = np.loadtxt(str(tmp_path / "b" / "steps"), comments="#")
     assert rows.shape[1] == 4 and len(rows) > 10
     # times printed to 1e-6 fs: non-decreasing (the first steps are ~1e-7 fs)
     assert np.all(np.diff(rows[:, 0]) >= 0) and np.all(rows[:, 2] <= rows[:, 1] + 1e-12)
     assert abs(rows[-1, 0] + rows[-1, 2] - 5.0) < 2e-6          # the last step lands on tfinal
+    # psi file: the state at outputs 0, 4 and 8 (0, 2 and 4 fs), the first the initial state
+    from macaw_amd.problem import load_problem
+    from macaw_amd.restart import read_psi_file
+    prob = load_problem(str(tmp_path / "t1.inp"))
+    recs = read_psi_file(str(tmp_path / "b" / "psi"), prob.nAConftot, prob.dimpsi)
+    assert [round(r[0] * AU2FS, 6) for r in recs] == [0.0, 2.0, 4.0]
+    assert np.array_equal(recs[0][1], prob.A0) and np.array_equal(recs[0][2], prob.psi0)
