@@ -11,7 +11,8 @@ first, under a device-memory budget.
 
 Command line:
   python -m macaw_amd.ensemble --members 256 --fs 1 --threads 8 --json out.json
-runs the ensemble in mixed precision and in complex FP64 and reports the
+runs the ensemble (each member's LDS brick size chosen by the autotuner
+unless --no-autotune) in mixed precision and in complex FP64 and reports the
 ensemble throughput (member-fs per wall second) and every member's
 max |C_mixed(t) - C_FP64(t)| over the output times.
 """
@@ -112,7 +113,8 @@ def main(argv=None):
     ap.add_argument("--fs", type=float, default=1.0)
     ap.add_argument("--threads", type=int, default=8)
     ap.add_argument("--mem-fraction", type=float, default=0.7)
-    ap.add_argument("--autotune", action="store_true")
+    ap.add_argument("--no-autotune", dest="autotune", action="store_false",
+                    help="keep the default LDS brick size instead of timing the candidates per member")
     ap.add_argument("--only", choices=("mixed", "fp64"), help="run one precision only (no comparison)")
     ap.add_argument("--json")
     a = ap.parse_args(argv)
