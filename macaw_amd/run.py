@@ -157,7 +157,7 @@ def _interval_by_steps(e, prob, t, h, steps):
 
 
 def run(input_path: str, outdir: str = ".", precision: int = 0, controller: str = "classical", log=print,
-        steps: bool = False, psi_every: int = 0):
+        steps: bool = False, psi_every: int = 0, autotune: bool = False):
     from .engine import Engine
     from .problem import load_problem
     from .restart import write_psi_record, write_restart
@@ -165,6 +165,9 @@ def run(input_path: str, outdir: str = ".", precision: int = 0, controller: str 
     e = Engine(prob, precision=precision)
     if controller == "learned":
         e.set_step_model(np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "step_controller.npy")))
+    if autotune:
+        brick, ms = e.autotune()
+        log(f"[run] autotune: LDS brick {brick} ({ms:.2f} ms per H.A + mean fields)")
     wt = prob.writetime
     t0, tf = prob.tinit, prob.tfinal
     os.makedirs(outdir, exist_ok=True)
@@ -229,11 +232,12 @@ def main(argv=None):
     ap.add_argument("--precision", type=int, default=0, choices=(0, 1))
     ap.add_argument("--controller", default="classical", choices=("classical", "learned"))
     ap.add_argument("--steps", action="store_true", help="also write Macaw's `steps` file (one line per CMF2 step)")
+    ap.add_argument("--autotune", action="store_true", help="choose the LDS brick size by timing the candidates")
     ap.add_argument("--psi-every", type=int, default=0,
                     help="write the state to Macaw's `psi` file every N output times (0: never)")
     a = ap.parse_args(argv)
     run(a.input, a.outdir, a.precision, a.controller, log=lambda m: print(m, flush=True), steps=a.steps,
-        psi_every=a.psi_every)
+        psi_every=a.psi_every, autotune=a.autotune)
     return 0
 
 
