@@ -45,6 +45,7 @@ class Problem:
     tinit: float = 0.0
     tfinal: float = 0.0
     int_type_psi: int = 1
+    int_type: int = 10            # 10 CMF2, 1 VMF Runge-Kutta (Cash-Karp), 2 VMF Dormand-Prince
     A0: np.ndarray = None         # complex128[nAConftot]
     psi0: np.ndarray = None       # complex128[dimpsi]
     wgrid: np.ndarray = None
@@ -162,7 +163,7 @@ def build_problem(deck: InputDeck, operator_path: str | None = None) -> Problem:
                    ae, lhs, rhs, iop, eps_int=deck.eps_int, eps_psi=deck.eps_psi,
                    eps_rho=deck.eps_rho, writetime=deck.tout,
                    min_stepsize=deck.min_stepsize, tinit=deck.tinit,
-                   tfinal=deck.tfinal, int_type_psi=deck.int_type_psi or 1,
+                   tfinal=deck.tfinal, int_type_psi=deck.int_type_psi or 1, int_type=deck.int_type,
                    wgrid=wgrid, eps_C=deck.eps_c, eps_ovl=deck.eps_ovl)
     if dof_gwp.any():
         for m in range(nm):

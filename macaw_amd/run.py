@@ -4,7 +4,9 @@
 
 Reads the input and operator files (the keyword subset the engine runs),
 propagates from tinit to tfinal with the output cadence of Macaw's
-propagate_wf (propagator.f90:257-306) and writes, in Macaw's formats:
+propagate_wf (CMF2, propagator.f90:257-306, or the variable-mean-field
+Runge-Kutta branch, :108-140, for ``integration = rk45 / dopri``) and writes,
+in Macaw's formats:
 
 * ``auto``: header of propagator.f90:26, then per output time
   ``(2x,f10.4,5x,3(1x,f18.14))`` time [fs], Re C, Im C, |C| (output.f90:228-229);
@@ -170,6 +172,8 @@ def run(input_path: str, outdir: str = ".", precision: int = 0, controller: str 
         log(f"[run] autotune: LDS brick {brick} ({ms:.2f} ms per H.A + mean fields)")
     wt = prob.writetime
     t0, tf = prob.tinit, prob.tfinal
+    # integration = rk45 / dopri: variable mean field (propagator.f90:108-140)
+    vmf = prob.int_type in (1, 2)
     os.makedirs(outdir, exist_ok=True)
     outs, pops, atts, block, step_log = [], [], [], [], []
     energy_start = None
@@ -186,9 +190,12 @@ def run(input_path: str, outdir: str = ".", precision: int = 0, controller: str 
             write_psi_record(psi_fh, t, A_t, psi)
         k_out += 1
         last = t >= tf - wt * 0.1
-        if last or steps:
+        if last or (steps and not vmf):
             o, p = e.observables(t)
             a = []
+        elif vmf:
+            ol, pl, a = e.propagate_vmf(prob.int_type, t, t)
+            o, p = ol[0], pl[0]
         else:
             ol, pl, a = e.propagate(t, t)
             o, p = ol[0], pl[0]
@@ -200,7 +207,7 @@ def run(input_path: str, outdir: str = ".", precision: int = 0, controller: str 
         atts += list(a)
         if last:
             break
-        if steps:
+        if steps and not vmf:
             t, h = _interval_by_steps(e, prob, t, h, step_log)
         else:
             acc = [x for x in a if x.accepted]

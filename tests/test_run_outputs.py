@@ -176,3 +176,28 @@ def test_steps_loop_reproduces_the_engine_loop(tmp_path):
     recs = read_psi_file(str(tmp_path / "b" / "psi"), prob.nAConftot, prob.dimpsi)
     assert [round(r[0] * AU2FS, 6) for r in recs] == [0.0, 2.0, 4.0]
     assert np.array_equal(recs[0][1], prob.A0) and np.array_equal(recs[0][2], prob.psi0)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("method,name", [(2, "dopri"), (1, "rk45")])
+def test_standalone_vmf_input_matches_single_call(method, name, tmp_path):
+    """An input with `integration = dopri / rk45` runs the variable-mean-field
+    propagator; one output interval per call reproduces a single call."""
+    import re
+    from macaw_amd.engine import Engine
+    from macaw_amd.lvc import make_problem
+    from macaw_amd.problem import load_problem
+    make_problem("t1", str(tmp_path))
+    inp = (tmp_path / "t1.inp").read_text()
+    inp = re.sub(r"integration = CMF2\n[^\n]*\n[^\n]*\n", f"integration = {name}\n", inp)
+    (tmp_path / "v.inp").write_text(inp)
+    prob = load_problem(str(tmp_path / "v.inp"))
+    assert prob.int_type == method
+    outs, pops = RUN.run(str(tmp_path / "v.inp"), str(tmp_path / "o"))
+    e = Engine(prob)
+    ref, rp, _ = e.propagate_vmf(method, prob.tinit, prob.tfinal)
+    e.close()
+    assert len(ref) >= len(outs)
+    for a, b in zip(outs, ref):
+        assert abs(a.time - b.time) < 1e-9
+        assert abs(complex(a.auto_re, a.auto_im) - complex(b.auto_re, b.auto_im)) < 1e-12
