@@ -1,7 +1,8 @@
 /*
  * oracle.c — CPU FP64 restatement of Macaw's CMF2 propagation path
- * (DavidPicconi/Macaw, source/ *.f90), DVR (grid) modes, multi-set and
- * single-set.
+ * (DavidPicconi/Macaw, source/ *.f90): DVR (grid) modes and frozen-width
+ * Gaussian (G-MCTDH) modes, multi-set and single-set; CMF2 and the
+ * variable-mean-field Runge-Kutta branch.
  *
  * TEST INFRASTRUCTURE ONLY: the checker for the HIP engine.  Nothing in the
  * engine links or calls it.
@@ -12,9 +13,12 @@
  * file does (conjugation is exact in IEEE arithmetic).  Every mode product
  * below is  out[l, i, r] = sum_j M[i, j] in[l, j, r]  over the mode index.
  *
- * Parity pins: see tests/test_oracle_pins.py (brute-force Kronecker H,
- * einsum mean fields, eigh-based regularised inverses, expm Lanczos,
- * full-basis MCTDH == exact propagation, Hartree/Rabi closed forms).
+ * Parity pins: tests/test_oracle_pins.py (brute-force Kronecker H, einsum
+ * mean fields, eigh-based regularised inverses, expm Lanczos, DeltaPsi and
+ * the SPF error norm against grid-space vectors, Runge-Kutta orders,
+ * full-basis CMF2 and VMF == exact propagation) and tests/test_gmctdh_pins.py
+ * (Gaussian integrals and overlaps against quadrature, the Tikhonov solve and
+ * Cholesky inverse against numpy, coherent-state and separable closed forms).
  */
 #include <complex.h>
 #include <math.h>
@@ -1503,7 +1507,11 @@ static double err_spf_gwp(orc *o, int64_t m, const cplx *p1, const cplx *p2) {
     return sqrt(fabs(error));
 }
 
-/* Calc_Err_SPF, DVR branch (cmf2_derivatives.f90:239-291). */
+/* Calc_Err_SPF, DVR branch (cmf2_derivatives.f90:239-291).  As in the
+ * reference, rho(j1, j2) = sum conj(A_j1) A_j2 is paired with conj(ov(j1, j2))
+ * = <d_j2|d_j1>, so the result is the norm of the change of the wavefunction
+ * with conjugated coefficients (the norm of the change itself when rho is
+ * real); kept, since it decides the step sequence. */
 static double err_spf(orc *o, int64_t m, const cplx *p1, const cplx *p2) {
     if (o->gwp[m]) return err_spf_gwp(o, m, p1, p2);
     const int64_t nmax = o->nspfmax, nG = o->ngpm[m];
@@ -1747,6 +1755,13 @@ static double delta_psi(orc *o, const cplx *A1, const cplx *psi1, const cplx *A2
     double o11 = norm_wf(o, A1, psi1), o22 = norm_wf(o, A2, psi2);
     cplx o12 = wf_ovl(o, A1, psi1, A2, psi2);
     return sqrt(fabs(2.0 * (1.0 - creal(o12) / sqrt(o11 * o22))));
+}
+
+double orc_delta_psi(orc *o, const double *A1, const double *psi1, const double *A2, const double *psi2) {
+    return delta_psi(o, (const cplx *)A1, (const cplx *)psi1, (const cplx *)A2, (const cplx *)psi2);
+}
+double orc_err_spf(orc *o, int64_t im, const double *psi1, const double *psi2) {
+    return err_spf(o, im, (const cplx *)psi1, (const cplx *)psi2);
 }
 
 static double vnorm(const cplx *x, int64_t n) {

@@ -53,6 +53,11 @@ void orc_der_spf(orc *o, int64_t im, const double *psi_in, double *dpsi_out);
 double orc_norm_wf(orc *o, const double *A, const double *psi);
 void orc_wf_ovl(orc *o, const double *A1, const double *psi1, const double *A2,
                 const double *psi2, double *ovl);
+/* Test hooks: DeltaPsi (gmctdh_objects.f90:206-227), and Calc_Err_SPF
+ * (cmf2_derivatives.f90:239-334) of mode im (0-based) with the density
+ * matrices of the last orc_calc_rho. */
+double orc_delta_psi(orc *o, const double *A1, const double *psi1, const double *A2, const double *psi2);
+double orc_err_spf(orc *o, int64_t im, const double *psi1, const double *psi2);
 /* Lanczos (cmf2_integration.f90:30-103); returns nKrylov or -1 on overflow. */
 int orc_lanczos(orc *o, const double *B0, double *BT, double dt, int build);
 /* propagate_SPF (cmf2_base.f90:210-260) for DVR mode index i (1-based) on psi. */

@@ -40,6 +40,10 @@ def lib():
         L.orc_norm_wf.restype = C.c_double
         L.orc_norm_wf.argtypes = [C.c_void_p, f64p, f64p]
         L.orc_wf_ovl.argtypes = [C.c_void_p, f64p, f64p, f64p, f64p, f64p]
+        L.orc_delta_psi.restype = C.c_double
+        L.orc_delta_psi.argtypes = [C.c_void_p, f64p, f64p, f64p, f64p]
+        L.orc_err_spf.restype = C.c_double
+        L.orc_err_spf.argtypes = [C.c_void_p, C.c_int64, f64p, f64p]
         L.orc_lanczos.restype = C.c_int
         L.orc_lanczos.argtypes = [C.c_void_p, f64p, f64p, C.c_double, C.c_int]
         L.orc_propagate_spf.restype = C.c_int
@@ -157,6 +161,15 @@ class Oracle:
 
     def norm_wf(self, A, psi):
         return lib().orc_norm_wf(self.h, cplx_ptr(np.ascontiguousarray(A)), cplx_ptr(np.ascontiguousarray(psi)))
+
+    def delta_psi(self, A1, psi1, A2, psi2):
+        return lib().orc_delta_psi(self.h, *(cplx_ptr(np.ascontiguousarray(x, np.complex128))
+                                             for x in (A1, psi1, A2, psi2)))
+
+    def err_spf(self, im, psi1, psi2):
+        """Calc_Err_SPF of mode im (0-based) with the density matrices of the last rho()."""
+        return lib().orc_err_spf(self.h, im, *(cplx_ptr(np.ascontiguousarray(x, np.complex128))
+                                               for x in (psi1, psi2)))
 
     def wf_ovl(self, A1, psi1, A2, psi2):
         out = np.zeros(2)

@@ -141,6 +141,57 @@ def test_overlaps_and_a_b_transforms(tmp_path):
     assert abs(o.norm_wf(A, prob.psi0) - np.vdot(B, B).real) < 1e-10 * np.vdot(B, B).real
 
 
+def test_err_spf_of_a_gaussian_mode_is_the_norm_of_the_change(tmp_path):
+    """Calc_Err_SPF of the Gaussian mode of g1 (the other modes are grid
+    modes with orthonormal SPFs): || Psi(conj A, xi1) - Psi(conj A, xi2) ||
+    (the reference's trace pairs rho with the transposed overlaps, as on grid
+    modes), with the normalised Gaussians exp(a x^2 + xi x) sampled on a fine
+    grid."""
+    import brute
+    prob = make_problem("g1", str(tmp_path))
+    assert prob.gwp_mode(0) and int(prob.nmodeDOF[0]) == 1
+    rng = np.random.default_rng(8)
+    A = rng.normal(size=prob.nAConftot) + 1j * rng.normal(size=prob.nAConftot)
+    A /= np.linalg.norm(A)
+    ip, _ = prob.ipsiSPF()
+    nsets = prob.nstates if prob.MultiSet[0] else 1
+    psi1, psi2 = prob.psi0, prob.psi0.copy()
+    for s in range(nsets):
+        n = int(prob.nSPF[0, s])
+        psi2[ip[0, s] - 1: ip[0, s] - 1 + n] += 0.05 * (rng.normal(size=n) + 1j * rng.normal(size=n))
+    o = Oracle(prob)
+    o.set_state(A, psi1)
+    o.prepare_gwp()
+    o.rho()
+    err = o.err_spf(0, psi1, psi2)
+
+    x = np.linspace(-15.0, 15.0, 30001)
+    h = x[1] - x[0]
+
+    def gaussians(psi, s):
+        jel = s if prob.MultiSet[0] else 0
+        n = int(prob.nSPF[0, jel])
+        a = prob.gwp_a[0, :n, 0]
+        xi = psi[ip[0, jel] - 1: ip[0, jel] - 1 + n]
+        assert np.abs(-xi.real / (2 * a)).max() < 5.0          # centres well inside the grid
+        G = np.exp(np.outer(x * x, a) + np.outer(x, xi))
+        return G / np.sqrt(np.sum(np.abs(G) ** 2, axis=0) * h)
+
+    def change_norm(Av):
+        total = 0.0
+        for s in range(prob.nstates):
+            T = brute.state_tensor(prob, Av, s)
+            T = brute.apply_mode(T, gaussians(psi1, s) - gaussians(psi2, s), 0)
+            for m in range(1, prob.nmodes):
+                T = brute.apply_mode(T, brute.spf_block(prob, psi1, m, s), m)
+            total += np.sum(np.abs(T) ** 2) * h
+        return np.sqrt(total)
+
+    assert err > 1e-2
+    assert err == pytest.approx(change_norm(A.conj()), rel=1e-9)
+    assert abs(err - change_norm(A)) > 1e-3 * err
+
+
 def _coherent_problem(d, ng, w_ev=0.2, k_ev=0.1):
     with open(os.path.join(d, "h.op"), "w") as fh:
         fh.write(f"{-w_ev / 2 / AU2EV:.15e} | 1 |1 dq^2\n{w_ev / 2 / AU2EV:.15e} | 1 |1 q^2\n"

@@ -208,23 +208,65 @@ def test_der_spf_matches_dense_projection(case):
             np.testing.assert_allclose(got, ref, atol=1e-9 * max(1, np.abs(ref).max()))
 
 
+def grid_vector(prob, A, psi):
+    """The wavefunction on the full grid product space, states concatenated."""
+    vec = []
+    for s in range(prob.nstates):
+        T = brute.state_tensor(prob, A, s)
+        for m in range(prob.nmodes):
+            T = brute.apply_mode(T, brute.spf_block(prob, psi, m, s), m)
+        vec.append(T.reshape(-1, order="F"))
+    return np.concatenate(vec)
+
+
 def test_norm_and_overlap(case):
     A, psi = perturbed_state(case, 12)
     o = Oracle(case)
     ov = o.wf_ovl(case.A0, case.psi0, A, psi)
     # <Psi0|Psi> on the full grid product space
-    def full(Av, pv):
-        vec = 0
-        for s in range(case.nstates):
-            T = brute.state_tensor(case, Av, s)
-            for m in range(case.nmodes):
-                T = brute.apply_mode(T, brute.spf_block(case, pv, m, s), m)
-            vec = vec if s else []
-            vec.append(T.reshape(-1, order="F"))
-        return np.concatenate(vec)
-    v0, v1 = full(case.A0, case.psi0), full(A, psi)
+    v0, v1 = grid_vector(case, case.A0, case.psi0), grid_vector(case, A, psi)
     assert abs(ov - np.vdot(v0, v1)) < 1e-12
     assert o.norm_wf(A, psi) == pytest.approx(np.vdot(v1, v1).real, rel=1e-12)
+
+
+def test_delta_psi_is_the_distance_of_the_normalised_states(case):
+    """DeltaPsi = || Psi1/|Psi1| - Psi2/|Psi2| || on the grid product space
+    (non-orthonormal SPFs, unnormalised coefficients)."""
+    rng = np.random.default_rng(13)
+    A1, psi1 = perturbed_state(case, 14)
+    A2 = 1.3 * (A1 + 0.05 * rand_c(rng, case.nAConftot))
+    psi2 = psi1 + 0.02 * rand_c(rng, case.dimpsi)
+    v1, v2 = grid_vector(case, A1, psi1), grid_vector(case, A2, psi2)
+    ref = np.linalg.norm(v1 / np.linalg.norm(v1) - v2 / np.linalg.norm(v2))
+    assert ref > 1e-2
+    assert Oracle(case).delta_psi(A1, psi1, A2, psi2) == pytest.approx(ref, rel=1e-10)
+
+
+def test_err_spf_is_the_norm_of_the_change(case):
+    """Calc_Err_SPF of mode m, psi1 and psi2 differing only in mode m (every
+    set of a multi-set mode, the shared block of a single-set one), the other
+    modes' SPFs orthonormal.  The reference's trace (cmf2_derivatives.f90:
+    277-291) pairs rho(j1, j2) = sum conj(A_j1) A_j2 with <dphi_j2|dphi_j1>,
+    i.e. it is || Psi(conj A, psi1) - Psi(conj A, psi2) ||: the norm of the
+    change for conjugated coefficients, equal to || Psi(A, psi1) - Psi(A, psi2) ||
+    only when rho is real.  The oracle keeps the reference's form (it decides
+    the step sequence), and the test checks that the two differ here."""
+    rng = np.random.default_rng(15)
+    A, _ = perturbed_state(case, 16)
+    o = Oracle(case)
+    o.set_state(A, case.psi0)
+    o.rho()
+    ip, _ = case.ipsiSPF()
+    ngpm = case.ngpm()
+    v1, v1c = grid_vector(case, A, case.psi0), grid_vector(case, A.conj(), case.psi0)
+    for m in range(case.nmodes):
+        psi2 = case.psi0.copy()
+        for s in range(case.nstates if case.MultiSet[m] else 1):
+            n = int(case.nSPF[m, s]) * int(ngpm[m])
+            psi2[ip[m, s] - 1: ip[m, s] - 1 + n] += 0.05 * rand_c(rng, n)
+        err = o.err_spf(m, case.psi0, psi2)
+        assert err == pytest.approx(np.linalg.norm(v1c - grid_vector(case, A.conj(), psi2)), rel=1e-11)
+        assert abs(err - np.linalg.norm(v1 - grid_vector(case, A, psi2))) > 1e-3 * err
 
 
 def test_full_basis_cmf2_equals_exact_propagation(tmp_path):
