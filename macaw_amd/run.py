@@ -20,11 +20,13 @@ in Macaw's formats:
 * ``steps`` (``--steps``): header of propagator.f90:37-38, then per CMF2 step
   ``(f12.6,3(2x,f12.6))`` time, attempted, done and suggested step [fs]
   (propagator.f90:273-274);
-* ``output``: per output time the block of output.f90:152-224 (time, norm,
-  energy and its change from the start, electronic populations, natural
-  populations of every mode, and for Gaussian modes the gross Gaussian
-  populations and the population errors of dropping one Gaussian).  The
-  run's header lines (propagator.f90:71-85) are not written.
+* ``output``: the propagation interval and the ``psi`` period
+  (propagator.f90:71-84), then per output time the block of
+  output.f90:152-224 (time, norm, energy and its change from the start,
+  electronic populations, natural populations of every mode, and for
+  Gaussian modes the gross Gaussian populations and the population errors of
+  dropping one Gaussian).  The echo of the input that precedes them in
+  Macaw's file is not written.
 
 The reference propagates one output interval past its last output; this
 driver stops at the last output time, whose state the restart file holds.
@@ -108,6 +110,16 @@ def mode_quantities(prob, rho, psi):
     return nat, ggp, perr
 
 
+def output_header(t0_au: float, tf_au: float, psi_period_au: float = 0.0):
+    """The lines propagate_wf writes to `output` before the first output time
+    (propagator.f90:71-84): the propagation interval and, when the `psi` file
+    is written, its period."""
+    lines = ["", f"Propagation from {t0_au * AU2FS:9.3f} fs to {tf_au * AU2FS:9.3f} fs"]
+    if psi_period_au > 0.0:
+        lines.append(f"The wavefunction is stored every {psi_period_au * AU2FS:7.3f} fs")
+    return lines + [""]
+
+
 def output_block(prob, out, pops, energy_start, nat, ggp, perr):
     """One output time of Macaw's `output` file (output.f90:152-224)."""
     from .inputfile import AU2EV
@@ -175,7 +187,8 @@ def run(input_path: str, outdir: str = ".", precision: int = 0, controller: str 
     # integration = rk45 / dopri: variable mean field (propagator.f90:108-140)
     vmf = prob.int_type in (1, 2)
     os.makedirs(outdir, exist_ok=True)
-    outs, pops, atts, block, step_log = [], [], [], [], []
+    outs, pops, atts, step_log = [], [], [], []
+    block = output_header(t0, tf, psi_every * wt)
     energy_start = None
     t = t0
     h = e.stepsize()                            # writetime * eps_int (propagator.f90:103)

@@ -96,6 +96,32 @@ def test_output_block_matches_fortran_edit_descriptors(tmp_path):
     assert lines[mode2: mode2 + 2] == out[3:5]
 
 
+FORTRAN_HEADER = """
+program hdr
+  implicit none
+  double precision :: t0, tf, tpsi
+  read(*,*) t0, tf, tpsi
+  write(*,*)
+  write(*,'(a,f9.3,a,f9.3,a)') 'Propagation from ', t0, ' fs to ', tf, ' fs'
+  write(*,'(a,f7.3,a)') 'The wavefunction is stored every ', tpsi, ' fs'
+  write(*,*)
+end program hdr
+"""
+
+
+@pytest.mark.skipif(not os.path.exists(FLANG), reason="flang not available")
+def test_output_header_matches_fortran_edit_descriptors(tmp_path):
+    src = tmp_path / "hdr.f90"
+    src.write_text(FORTRAN_HEADER)
+    exe = tmp_path / "hdr"
+    subprocess.check_call([FLANG, "-O0", "-o", str(exe), str(src)], cwd=str(tmp_path))
+    t0, tf, tpsi = 0.0, 123.45678, 2.5
+    out = subprocess.run([str(exe)], input=f"{t0!r} {tf!r} {tpsi!r}\n", capture_output=True, text=True,
+                         check=True).stdout.split("\n")[:-1]
+    assert RUN.output_header(t0 / AU2FS, tf / AU2FS, tpsi / AU2FS) == out
+    assert RUN.output_header(t0 / AU2FS, tf / AU2FS) == out[:2] + out[3:]
+
+
 @pytest.mark.gpu
 @pytest.mark.skipif(not os.path.exists(FLANG), reason="flang not available")
 def test_standalone_run_matches_fortran_driver(tmp_path):
