@@ -2790,6 +2790,9 @@ int mg_create(const mg_problem *prob, int64_t precision, mg_ctx **out) {
     int rc = guarded(c, [&] {
         if (precision != 0 && precision != 1)
             throw HipError{"precision must be 0 (complex FP64) or 1 (mixed complex FP32/FP64)"};
+        // host tables and the descriptor's limits first (setup makes no HIP
+        // call): an unsupported problem is rejected before the device is used
+        setup(c, prob);
         int ndev = 0;
         HIP_CHECK(hipGetDeviceCount(&ndev));
         if (ndev < 1) throw HipError{"no HIP device"};
@@ -2799,7 +2802,6 @@ int mg_create(const mg_problem *prob, int64_t precision, mg_ctx **out) {
         HIP_CHECK(hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming));
         HIP_CHECK(hipHostMalloc((void **)&c->spf_status_host, sizeof(int), hipHostMallocDefault));
         *c->spf_status_host = 0;
-        setup(c, prob);
         if (const char *e = std::getenv("MG_GRAPHS")) c->use_graphs = std::atoi(e) != 0;
         allocate(c);
         c->stepsize = c->writetime * c->eps_int;  // propagator.f90:103

@@ -5,7 +5,10 @@ errors are raised on the host, so every GPU kernel still ends normally.
 * CMF2 attempts rejected below the minimum step: error instead of the
   reference's endless reject loop (cmf2_base.f90:181-184; the build's guard);
 * Krylov space exhausted (nKrylovMax = 20 reached): error
-  (cmf2_integration.f90:64-70).
+  (cmf2_integration.f90:64-70);
+* a wavefunction that is no longer finite: error at the next output time;
+* problems beyond the engine's limits (more than 16 SPFs per mode, more than
+  8 electronic states): rejected by mg_create before any device call.
 """
 import re
 
@@ -24,6 +27,22 @@ def test_missing_operator_file(tmp_path):
     (tmp_path / "bad.inp").write_text(re.sub(r"Hamiltonian\s*=\s*\S+", "Hamiltonian = nowhere.op", inp))
     with pytest.raises(InputError, match="nowhere.op not found"):
         load_problem(str(tmp_path / "bad.inp"))
+
+
+@pytest.mark.parametrize("change, message", [
+    (dict(nspf=17, npts=20, nmodes=2), "more than 16 SPFs per mode"),
+    (dict(nstates=9, gaps=tuple(0.05 * k for k in range(9)), n_lambda=2), "at most 8 electronic states"),
+])
+def test_unsupported_problems_are_rejected_before_any_device_call(change, message, tmp_path):
+    """The engine's size limits are checked on the host before the first HIP
+    call, so an unsupported problem gets its own message (here on a machine
+    without a GPU, where any device call would fail with a different one)."""
+    import dataclasses
+    from macaw_amd.engine import Engine, EngineError
+    from macaw_amd.lvc import SPECS
+    prob = make_problem(dataclasses.replace(SPECS["t1"], **change), str(tmp_path))
+    with pytest.raises(EngineError, match=message):
+        Engine(prob)
 
 
 @gpu
