@@ -1,4 +1,4 @@
-"""Failure handling of the reference (SURVEY failure-detection row): the
+"""Failure handling of the reference and the engine: the
 errors are raised on the host, so every GPU kernel still ends normally.
 
 * missing operator file: input error (input.f90:529-536);
