@@ -3,7 +3,7 @@ errors are raised on the host, so every GPU kernel still ends normally.
 
 * missing operator file: input error (input.f90:529-536);
 * CMF2 attempts rejected below the minimum step: error instead of the
-  reference's endless reject loop (cmf2_base.f90:181-184; the build's guard);
+  reference's endless reject loop (cmf2_base.f90:181-184; the engine's guard);
 * Krylov space exhausted (nKrylovMax = 20 reached): error
   (cmf2_integration.f90:64-70);
 * a wavefunction that is no longer finite: error at the next output time;
