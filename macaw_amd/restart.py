@@ -125,15 +125,25 @@ def read_psi_file(path: str, nconf: int, dimpsi: int) -> list:
 def read_restart(path: str) -> dict:
     """Records of a restart file: nstates, nmodes, nSPF[nmodes, nstates],
     time, A, psi, GWPa (flat, Fortran order)."""
+    def unpack(fmt, rec, what):
+        if len(rec) != struct.calcsize(fmt):
+            raise RestartError(f"{what} record has the wrong length")
+        return struct.unpack(fmt, rec)
+
     with open(path, "rb") as fh:
-        ns, nm = struct.unpack("<2q", _read_record(fh))
-        nspf = np.zeros((nm, ns), dtype=np.int64)
+        ns, nm = unpack("<2q", _read_record(fh), "nstates/nmodes")
+        if ns < 1 or nm < 1:
+            raise RestartError(f"invalid header: {ns} states, {nm} modes")
+        rows = []                                   # sized by the file, not by the header
         for s in range(ns):
             v = np.frombuffer(_read_record(fh), dtype="<i8")
             if v.size != nm:
                 raise RestartError("nSPF record has the wrong length")
-            nspf[:, s] = v
-        (time,) = struct.unpack("<d", _read_record(fh))
+            rows.append(v.astype(np.int64))
+        nspf = np.stack(rows, axis=1)
+        if nspf.min() < 1:
+            raise RestartError("nSPF record holds a count below 1")
+        (time,) = unpack("<d", _read_record(fh), "time")
         A = np.frombuffer(_read_record(fh), dtype="<c16").astype(np.complex128)
         psi = np.frombuffer(_read_record(fh), dtype="<c16").astype(np.complex128)
         gwpa = np.frombuffer(_read_record(fh), dtype="<f8").astype(np.float64)

@@ -195,3 +195,18 @@ def test_psi_trajectory_records(tmp_path):
     assert [r[0] for r in recs] == [0.0, 20.0, 40.0]
     for (t, A, psi), (A0, psi0) in zip(recs, states):
         assert np.array_equal(A, A0) and np.array_equal(psi, psi0)
+
+
+def test_damaged_files_are_rejected(tmp_path):
+    prob = problem(tmp_path)
+    path = tmp_path / "restart"
+    R.write_restart(str(path), prob, 1.0, prob.A0, prob.psi0)
+    raw = path.read_bytes()
+    for k, bad in enumerate([raw[:-3],                                  # truncated last marker
+                             raw[:len(raw) // 2],                        # truncated inside a record
+                             raw[:4] + b"\x00" * 4 + raw[8:],            # damaged first record (nstates = 0)
+                             raw[:-4] + struct.pack("<i", 7)]):          # trailing marker disagrees
+        p = tmp_path / f"bad{k}"
+        p.write_bytes(bad)
+        with pytest.raises(R.RestartError):
+            R.read_restart(str(p))
