@@ -115,10 +115,12 @@ struct mg_ctx {
     hipStream_t stream2 = nullptr;
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     int *spf_status_host = nullptr;   // pinned: status of the last SPF propagation
-    // 0: complex FP64 throughout.  1: mixed form -- the A-vector, SPFs, rho,
-    // mean-field matrices and every reduction stay FP64; the H.A chains, the
-    // Krylov vectors beyond the first and the mean-field mode-product chains
-    // are stored and multiplied in complex FP32.
+    // 0: complex FP64 throughout.  1: mixed form -- the A-vector, SPFs, rho
+    // (Grams of the FP64 A), mean-field matrices, the first Krylov product
+    // and every reduction stay FP64; the other H.A chains, the Krylov vectors
+    // beyond the first and the mean-field mode-product chains are stored and
+    // multiplied in complex FP32.  MG_LAN64=0 / MG_RHO64=0 / MG_MF64=1 move
+    // the first Krylov product / rho / the mean fields to the other side.
     int prec = 0;
     size_t es = sizeof(double2);   // bytes per complex element of the FP32/FP64 work tensors
     // ---- problem ----
@@ -156,6 +158,25 @@ struct mg_ctx {
     DevBuf dA, dpsi, dAstart, dpsistart, dBK, dB0, dBH, dB02, dpH, dpH2, dHtmp, dIn, dOut;
     DevBuf dh1, dhsum, drho, dmf, drmf, dS, dS2, dS3, dGtmp;
     DevBuf dchain;                 // 2 * nchains * mixed
+    // mixed form: the first Krylov product H u_0 runs on the FP64 start
+    // vector with FP64 matrices (chain temporaries and result in FP64), so
+    // that u_1 = H u_0 - alpha u_0 carries no FP32 cancellation error
+    DevBuf dchain64, dW64;
+    bool lan64 = true;
+    // mixed form: mean fields from the FP64 A-vector with FP64 chains
+    // (MG_MF64=1; off by default: FP32 chains do not move the step sequence)
+    DevBuf dmfbuf64;
+    bool mf64 = false;
+    // mixed form: density matrices from the FP64 A-vector.  Their small
+    // eigenvalues enter rho^-1 MF; Grams of an FP32 copy of A put enough
+    // noise into the SPF error estimate (DeltaPsi) to flip one CMF2
+    // accept/reject decision in six against the FP64 run
+    bool rho64 = true;
+    // MG_PHASES=1: host wall time per CMF2 phase (stream synchronised at
+    // every phase boundary), printed to stderr by mg_destroy
+    bool phases = false;
+    double phase_ms[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    std::chrono::steady_clock::time_point phase_t0;
     DevBuf dmfbuf;                 // 2 * nm * mixed (AL / AR)
     DevBuf dkron;                  // 2 * nconf-sized temps for overlap chains
     DevBuf dspfscr, dpartials, dscal;
@@ -592,6 +613,10 @@ void allocate(mg_ctx *c) {
     c->dpsistart.alloc(z * D);
     c->dBK.alloc(c->es * N * kNKrylovMax);
     if (c->prec) {
+        if (const char *e = std::getenv("MG_LAN64")) c->lan64 = std::atoi(e) != 0;
+        if (c->lan64) c->dW64.alloc(z * N);
+        if (const char *e = std::getenv("MG_MF64")) c->mf64 = std::atoi(e) != 0;
+        if (const char *e = std::getenv("MG_RHO64")) c->rho64 = std::atoi(e) != 0;
         const size_t zf = sizeof(float2);
         c->dAf.alloc(zf * N);
         c->dXf.alloc(zf * N);
@@ -1145,15 +1170,31 @@ int64_t prod_range(const std::vector<int64_t> &sz, int a, int b) {
 // With one group only the final kind runs, reading the chains' inputs.
 // gbasis: H in the Gaussian (non-orthogonal) basis -- chains_g, h1mode on
 // every Gaussian mode, no diagonal sums (VMF with Gaussian modes).
-std::unique_ptr<Plan> build_apply_h(mg_ctx *c, const void *B, void *HB, bool gbasis = false) {
+// f64: complex FP64 storage and matrices in a mixed-form context (the first
+// Krylov product of the mixed Lanczos, whose input is the FP64 start vector).
+std::unique_ptr<Plan> build_apply_h(mg_ctx *c, const void *B, void *HB, bool gbasis = false, bool f64 = false) {
     PlanBuilder pb;
     const std::vector<Chain> &chains = gbasis ? c->chains_g : c->chains;
-    auto opm = [&](int m, int op) { return gbasis ? c->h1op(m, op) : c->haop(m, op); };
+    const bool wide = f64 && c->prec;
+    const int sprec = wide ? 0 : c->prec;
+    const size_t esz = sprec ? sizeof(float2) : sizeof(double2);
+    auto opm = [&](int m, int op) -> const void * {
+        if (wide) {
+            if (gbasis || !c->ngwp) return c->h1(m, op);
+            return c->dhA.as<double2>() + (size_t)c->nspfmax * c->nspfmax * (op + (size_t)c->nopmax * m);
+        }
+        return gbasis ? c->h1op(m, op) : c->haop(m, op);
+    };
+    auto hsumop = [&](int s, int m) -> const void * { return wide ? (const void *)c->hsum(s, m) : c->hsumop(s, m); };
+    auto at = [&](const void *p, int64_t n) { return static_cast<void *>(static_cast<char *>(const_cast<void *>(p)) + (size_t)n * esz); };
     const int G = (int)c->groups.size();
     const int nm = c->nm;
     const size_t mixed = (size_t)c->mixed;
     const size_t K = chains.size();
-    auto Tbuf = [&](size_t k, int i) { return c->at(c->dchain.p, (int64_t)(mixed * (2 * k + i))); };
+    if (wide && c->dchain64.bytes < sizeof(double2) * 2 * std::max<size_t>(1, K) * mixed)
+        c->dchain64.alloc(sizeof(double2) * 2 * std::max<size_t>(1, K) * mixed);
+    void *chainbuf = wide ? c->dchain64.p : c->dchain.p;
+    auto Tbuf = [&](size_t k, int i) { return at(chainbuf, (int64_t)(mixed * (2 * k + i))); };
     auto state_sizes = [&](int s) {
         std::vector<int64_t> sz(nm);
         for (int m = 0; m < nm; m++) sz[m] = c->NS(m, s);
@@ -1177,7 +1218,7 @@ std::unique_ptr<Plan> build_apply_h(mg_ctx *c, const void *B, void *HB, bool gba
     std::vector<int> curT(K, -1);
     std::vector<std::vector<int64_t>> csz(K);
     for (size_t k = 0; k < K; k++) {
-        cur[k] = c->at(B, c->off[chains[k].rhs]);
+        cur[k] = at(B, c->off[chains[k].rhs]);
         csz[k] = state_sizes(chains[k].rhs);
     }
     // Apply chain k's ops of group g to the data in buffer `in`; the result is
@@ -1216,15 +1257,15 @@ std::unique_ptr<Plan> build_apply_h(mg_ctx *c, const void *B, void *HB, bool gba
             const std::vector<int64_t> sz = state_sizes(b);
             const int64_t Rb = prod_range(sz, c->groups[0].second, nm);
             TaskBuilder tb(c, 0, 1, Rb);
-            tb.load(0, c->at(B, c->off[b]), sz);
+            tb.load(0, at(B, c->off[b]), sz);
             const std::vector<int> dm = diag_ops(b, 0);
             if (dm.empty()) {
                 tb.zero(1, sz);
             } else {
                 for (size_t i = 0; i < dm.size(); i++)
-                    tb.op(0, 1, dm[i], c->hsumop(b, dm[i]), 0, (int)sz[dm[i]], i > 0 ? 1 : 0);
+                    tb.op(0, 1, dm[i], hsumop(b, dm[i]), 0, (int)sz[dm[i]], i > 0 ? 1 : 0);
             }
-            tb.store(1, c->at(HB, c->off[b]), 0);
+            tb.store(1, at(HB, c->off[b]), 0);
             for (size_t k = 0; k < K; k++) {
                 const Chain &ch = chains[k];
                 if (ch.rhs != b || chain_ops(ch, 0).empty()) continue;
@@ -1233,7 +1274,7 @@ std::unique_ptr<Plan> build_apply_h(mg_ctx *c, const void *B, void *HB, bool gba
                 if (tb.t.nmat + (int)ops.size() > kMaxBrickMats || tb.t.ninstr + (int)ops.size() + 1 > kMaxBrickInstr) {
                     first.push_back(tb);
                     tb = TaskBuilder(c, 0, 1, Rb);
-                    tb.load(0, c->at(B, c->off[b]), sz);
+                    tb.load(0, at(B, c->off[b]), sz);
                 }
                 const ChainOp &o = ops[0];
                 tb.op(0, 1, o.mode, opm(o.mode, o.op), 0, (int)c->NS(o.mode, ch.lhs), 0);
@@ -1262,10 +1303,10 @@ std::unique_ptr<Plan> build_apply_h(mg_ctx *c, const void *B, void *HB, bool gba
                 if (dm.empty()) continue;
                 const std::vector<int64_t> sz = state_sizes(s);
                 TaskBuilder tb(c, g, prod_range(sz, 0, c->groups[g].first), prod_range(sz, c->groups[g].second, nm));
-                tb.load(0, c->at(B, c->off[s]), sz);
+                tb.load(0, at(B, c->off[s]), sz);
                 for (size_t i = 0; i < dm.size(); i++)
-                    tb.op(0, 1, dm[i], c->hsumop(s, dm[i]), 0, (int)sz[dm[i]], i > 0 ? 1 : 0);
-                tb.store(1, c->at(HB, c->off[s]), 1);
+                    tb.op(0, 1, dm[i], hsumop(s, dm[i]), 0, (int)sz[dm[i]], i > 0 ? 1 : 0);
+                tb.store(1, at(HB, c->off[s]), 1);
                 diag.push_back(tb);
             }
             for (size_t k = 0; k < K; k++) {
@@ -1329,15 +1370,15 @@ std::unique_ptr<Plan> build_apply_h(mg_ctx *c, const void *B, void *HB, bool gba
                 const int64_t L = prod_range(sz, 0, c->groups[g].first);
                 TaskBuilder tb(c, g, L, 1);
                 if (b == 0) {
-                    if (G >= 2) tb.load(1, c->at(HB, c->off[s]), sz);
+                    if (G >= 2) tb.load(1, at(HB, c->off[s]), sz);
                     else tb.zero(1, sz);
                     const std::vector<int> dm = diag_ops(s, g);
                     if (!dm.empty()) {
-                        tb.load(0, c->at(B, c->off[s]), sz);
-                        for (int m : dm) tb.op(0, 1, m, c->hsumop(s, m), 0, (int)sz[m], 1);
+                        tb.load(0, at(B, c->off[s]), sz);
+                        for (int m : dm) tb.op(0, 1, m, hsumop(s, m), 0, (int)sz[m], 1);
                     }
                 } else {
-                    tb.load(1, c->at(HB, c->off[s]), sz);
+                    tb.load(1, at(HB, c->off[s]), sz);
                 }
                 for (size_t k : fin_batches[s][b]) {
                     const Chain &ch = chains[k];
@@ -1345,13 +1386,13 @@ std::unique_ptr<Plan> build_apply_h(mg_ctx *c, const void *B, void *HB, bool gba
                     if (chain_ops(ch, g).empty()) tb.axpy(0, 1, ch.ae);
                     else chain_group_ops(tb, k, g, 0, 2, 1, ch.ae);
                 }
-                tb.store(1, c->at(HB, c->off[s]), 0);
+                tb.store(1, at(HB, c->off[s]), 0);
                 fin.push_back(tb);
             }
             pb.phase(fin);
         }
     }
-    return pb.finish(c->prec);
+    return pb.finish(sprec);
 }
 
 const Plan &get_plan(mg_ctx *c, const PlanKey &key, std::unique_ptr<Plan> (*build)(mg_ctx *, const PlanKey &)) {
@@ -1372,6 +1413,16 @@ void apply_h(mg_ctx *c, const void *B, void *HB) {
     PlanKey key{(const void *)1, B, HB, nullptr};
     const Plan &p = get_plan(c, key, [](mg_ctx *cc, const PlanKey &k) {
         return build_apply_h(cc, k[1], (void *)k[2]);
+    });
+    run_plan(c, p);
+    check_launch();
+}
+
+// H B with complex FP64 storage and matrices in a mixed-form context.
+void apply_h64(mg_ctx *c, const double2 *B, double2 *HB) {
+    PlanKey key{(const void *)10, B, HB, nullptr};
+    const Plan &p = get_plan(c, key, [](mg_ctx *cc, const PlanKey &k) {
+        return build_apply_h(cc, k[1], (void *)k[2], false, true);
     });
     run_plan(c, p);
     check_launch();
@@ -1670,12 +1721,13 @@ double delta_psi_contract(mg_ctx *c, const double2 *A1, const double2 *A2) {
 
 // ----------------------------------------------------------------------------
 // Density matrices and mean fields.
-// In the mixed form the Gram contractions read an FP32 copy of A (left in
-// dAf for the mean-field chains that follow) and accumulate in FP64;
-// exact = true reads the FP64 A (the reported populations and norm).
+// The Gram contractions read the FP64 A (mixed form included, unless rho64
+// is switched off: then an FP32 copy of A, left in dAf for the mean-field
+// chains that follow) and accumulate in FP64; exact = true always reads the
+// FP64 A (the reported populations and norm).
 void calc_rho(mg_ctx *c, const double2 *A64, bool exact = false) {
     const void *A = A64;
-    const int gp = exact ? 0 : c->prec;
+    const int gp = (exact || c->rho64) ? 0 : c->prec;
     if (gp) {
         launch_convert(c->dAf.p, 1, A64, 0, c->nconf, c->stream);
         A = c->dAf.p;
@@ -1791,17 +1843,26 @@ void calc_rho(mg_ctx *c, const double2 *A64, bool exact = false) {
 // Gram, then moves L on by O_m^H; L_{g+1} is stored in place for pass g+1.
 // The per-(term, mode) Grams and ae * rho for one-mode terms are then summed
 // into the mean-field matrices (mirror terms as ae* G^H) by k_mat_combine.
-std::unique_ptr<Plan> build_meanfield(mg_ctx *c, const void *A) {
+// f64: complex FP64 storage and matrices in a mixed-form context.
+std::unique_ptr<Plan> build_meanfield(mg_ctx *c, const void *A, bool f64 = false) {
     PlanBuilder pb;
     const int G = (int)c->groups.size(), nm = c->nm;
     const size_t mixed = (size_t)c->mixed, nn = (size_t)c->nspfmax * c->nspfmax;
+    const bool wide = f64 && c->prec;
+    const int sprec = wide ? 0 : c->prec;
+    const size_t esz = sprec ? sizeof(float2) : sizeof(double2);
+    auto at = [&](const void *p, int64_t n) { return static_cast<void *>(static_cast<char *>(const_cast<void *>(p)) + (size_t)n * esz); };
+    auto h1op = [&](int m, int op) -> const void * { return wide ? (const void *)c->h1(m, op) : c->h1op(m, op); };
     std::vector<int> terms;
     for (size_t ti = 0; ti < c->mf_chain_terms.size(); ti++)
         if (c->mf_partner[ti] != -2) terms.push_back((int)ti);
-    if (c->dmfbuf.bytes < c->es * mixed * (size_t)G * terms.size() ||
+    if (wide && c->dmfbuf64.bytes < esz * mixed * (size_t)G * terms.size())
+        c->dmfbuf64.alloc(esz * mixed * (size_t)G * std::max<size_t>(1, terms.size()));
+    DevBuf &mfbuf = wide ? c->dmfbuf64 : c->dmfbuf;
+    if (mfbuf.bytes < esz * mixed * (size_t)G * terms.size() ||
         c->dgslots.bytes < sizeof(double2) * nn * nm * terms.size())
         throw HipError{"mean-field work buffers too small"};
-    auto tensor = [&](size_t k, int g) { return c->at(c->dmfbuf.p, (int64_t)(mixed * (k * G + g))); };
+    auto tensor = [&](size_t k, int g) { return at(mfbuf.p, (int64_t)(mixed * (k * G + g))); };
     auto is_op = [&](int t, int m) {
         const int op = c->iOper[m + (size_t)nm * t];
         return !c->isIdent[m + (size_t)nm * op];
@@ -1811,7 +1872,7 @@ std::unique_ptr<Plan> build_meanfield(mg_ctx *c, const void *A) {
     std::vector<std::vector<const void *>> R(terms.size(), std::vector<const void *>(G));
     // R passes
     for (size_t k = 0; k < terms.size(); k++)
-        R[k][G - 1] = c->at(A, c->off[(int)c->rhs[c->mf_chain_terms[terms[k]]] - 1]);
+        R[k][G - 1] = at(A, c->off[(int)c->rhs[c->mf_chain_terms[terms[k]]] - 1]);
     for (int g = G - 2; g >= 0; g--) {
         std::vector<TaskBuilder> tasks;
         for (size_t k = 0; k < terms.size(); k++) {
@@ -1831,7 +1892,7 @@ std::unique_ptr<Plan> build_meanfield(mg_ctx *c, const void *A) {
             tb.load(0, R[k][g1], sz);
             for (int m : ms) {
                 const int op = c->iOper[m + (size_t)nm * t];
-                tb.op(0, 0, m, c->h1op(m, op), 0, (int)c->NS(m, a), 0);
+                tb.op(0, 0, m, h1op(m, op), 0, (int)c->NS(m, a), 0);
             }
             tb.store(0, tensor(k, g), 0);
             R[k][g] = tensor(k, g);
@@ -1842,7 +1903,7 @@ std::unique_ptr<Plan> build_meanfield(mg_ctx *c, const void *A) {
     // Gram passes
     std::vector<const void *> Lcur(terms.size());
     for (size_t k = 0; k < terms.size(); k++)
-        Lcur[k] = c->at(A, c->off[(int)c->lhs[c->mf_chain_terms[terms[k]]] - 1]);
+        Lcur[k] = at(A, c->off[(int)c->lhs[c->mf_chain_terms[terms[k]]] - 1]);
     int last_mode_with_op = -1;
     for (int g = 0; g < G; g++) {
         std::vector<TaskBuilder> tasks;
@@ -1867,13 +1928,13 @@ std::unique_ptr<Plan> build_meanfield(mg_ctx *c, const void *A) {
                 int rsrc = 1;
                 for (size_t j = i + 1; j < ms.size(); j++) {
                     const int op = c->iOper[ms[j] + (size_t)nm * t];
-                    tb.op(rsrc, 2, ms[j], c->h1op(ms[j], op), 0, (int)sz[ms[j]], 0);
+                    tb.op(rsrc, 2, ms[j], h1op(ms[j], op), 0, (int)sz[ms[j]], 0);
                     rsrc = 2;
                 }
                 tb.gram(0, rsrc, m, slot(k, m));
                 if (m != last_mode_with_op) {
                     const int op = c->iOper[m + (size_t)nm * t];
-                    tb.op(0, 0, m, c->h1op(m, op), 1, (int)sz[m], 0);
+                    tb.op(0, 0, m, h1op(m, op), 1, (int)sz[m], 0);
                 }
             }
             if (g + 1 < G) {
@@ -1886,7 +1947,7 @@ std::unique_ptr<Plan> build_meanfield(mg_ctx *c, const void *A) {
         }
         pb.phase(tasks);
     }
-    return pb.finish(c->prec);
+    return pb.finish(sprec);
 }
 
 // Combination table: MF(m, op) = sum ae * rho (one-mode terms) + sum ae G_t(m)
@@ -1951,14 +2012,14 @@ void build_mf_combine(mg_ctx *c) {
 
 void calc_meanfield_bricks(mg_ctx *c, const double2 *A64, bool af_ready) {
     const void *A = A64;
-    if (c->prec) {
-        if (!af_ready) launch_convert(c->dAf.p, 1, A64, 0, c->nconf, c->stream);
+    if (c->prec && !c->mf64) {
+        if (!af_ready || c->rho64) launch_convert(c->dAf.p, 1, A64, 0, c->nconf, c->stream);
         A = c->dAf.p;
     }
-    PlanKey key{(const void *)7, A, nullptr, nullptr};
+    PlanKey key{(const void *)(c->mf64 ? 11 : 7), A, nullptr, nullptr};
     auto it = c->plans.find(key);
     if (it == c->plans.end()) {
-        it = c->plans.emplace(key, build_meanfield(c, A)).first;
+        it = c->plans.emplace(key, build_meanfield(c, A, c->mf64)).first;
         if (!c->dmf_rowptr.p) build_mf_combine(c);
     }
     if (!c->mf_chain_terms.empty()) run_plan(c, *it->second);
@@ -1985,7 +2046,7 @@ void calc_meanfield(mg_ctx *c, const double2 *A64, bool af_ready = false) {
     // FP32 copy of A; the Gram contractions accumulate in FP64 either way
     const void *A = A64;
     if (c->prec && !c->mf_chain_terms.empty()) {
-        if (!af_ready) launch_convert(c->dAf.p, 1, A64, 0, c->nconf, c->stream);
+        if (!af_ready || c->rho64) launch_convert(c->dAf.p, 1, A64, 0, c->nconf, c->stream);
         A = c->dAf.p;
     }
     void *AL = c->dmfbuf.p;
@@ -2219,10 +2280,22 @@ int lanczos(mg_ctx *c, const double2 *B0, double2 *BT, double dt, bool build) {
             void *w = c->at(BK, N * j);
             const void *up = (j >= 2) ? ((j == 2) ? u0 : c->at(BK, N * (j - 2))) : nullptr;
             if (j >= 2 && j == kNKrylovMax) throw HipError{"Maximum number of Krylov vectors reached"};
-            apply_h(c, uj, w);
             double2 *dot_d = reinterpret_cast<double2 *>(dsc + 2);
-            launch_dot(uj, w, N, part, c->nblocks_red, dot_d, pr, c->stream);
-            launch_lanczos_step(w, uj, up, dsc + 2, nsq, j - 1, N, part, c->nblocks_red, nsq + j, pr, c->stream);
+            if (j == 1 && pr && c->lan64) {
+                // u_1 = H u_0 - alpha u_0 in FP64 from the FP64 start vector,
+                // then stored as the FP32 Krylov vector
+                double2 *w64 = c->dW64.as<double2>();
+                apply_h64(c, B0, w64);
+                launch_dot(B0, w64, N, part, c->nblocks_red, dot_d, 0, c->stream);
+                launch_lanczos_step(w64, B0, nullptr, dsc + 2, nsq, 0, N, part, c->nblocks_red, nsq + 1, 0,
+                                    c->stream);
+                launch_convert(w, 1, w64, 0, N, c->stream);
+            } else {
+                apply_h(c, uj, w);
+                launch_dot(uj, w, N, part, c->nblocks_red, dot_d, pr, c->stream);
+                launch_lanczos_step(w, uj, up, dsc + 2, nsq, j - 1, N, part, c->nblocks_red, nsq + j, pr,
+                                    c->stream);
+            }
             double h[2];
             HIP_CHECK(hipMemcpyAsync(h, dsc + 2, sizeof(double), hipMemcpyDeviceToHost, c->stream));
             HIP_CHECK(hipMemcpyAsync(nrm + j - 1, nsq + j - 1, sizeof(double) * 2, hipMemcpyDeviceToHost, c->stream));
@@ -2372,6 +2445,18 @@ double propose_next(mg_ctx *c, double hdid, double error) {
 void cmf2_step_gwp(mg_ctx *c, double tt, double htry, double *hdid_out, double *hnext_out,
                    std::vector<mg_attempt> *log);
 
+// Phase buckets of MG_PHASES: 0 rho + h1mode, 1 mean fields, 2 SPF
+// propagation (the synchronous one), 3 Lanczos, 4 DeltaPsi, 5 copies,
+// errors and the controller.
+const char *const kPhaseNames[6] = {"rho+h1mode", "meanfield", "spf(sync)", "lanczos", "deltapsi", "other"};
+void phase(mg_ctx *c, int bucket) {
+    if (!c->phases) return;
+    HIP_CHECK(hipStreamSynchronize(c->stream));
+    const auto t = std::chrono::steady_clock::now();
+    c->phase_ms[bucket] += std::chrono::duration<double, std::milli>(t - c->phase_t0).count();
+    c->phase_t0 = t;
+}
+
 void cmf2_step(mg_ctx *c, double tt, double htry, double *hdid_out, double *hnext_out, std::vector<mg_attempt> *log) {
     if (c->ngwp) {
         cmf2_step_gwp(c, tt, htry, hdid_out, hnext_out, log);
@@ -2390,29 +2475,41 @@ void cmf2_step(mg_ctx *c, double tt, double htry, double *hdid_out, double *hnex
             hdid = r.h;
             forced = (int)r.accepted;
         }
+        if (c->phases) phase(c, 5);
         calc_rho(c, A);
         calc_h1mode(c, psi);
+        phase(c, 0);
         calc_meanfield(c, A, true);
+        phase(c, 1);
         copy_d2d(c, B0, A, N);
         copy_d2d(c, pH, psi, D);
         // the SPF half step needs only rho(A) and the mean fields: it runs on
         // stream2 while the Lanczos propagation of A proceeds
         propagate_spf_fork(c, pH, 0.5 * hdid);
         const int nk1 = lanczos(c, B0, BH, 0.5 * hdid, true);
+        phase(c, 3);
         propagate_spf_join(c);
+        phase(c, 2);
         calc_rho(c, BH);
         calc_h1mode(c, pH);
+        phase(c, 0);
         calc_meanfield(c, BH, true);
+        phase(c, 1);
         copy_d2d(c, pH2, psi, D);
         propagate_spf(c, pH2, 0.5 * hdid);
+        phase(c, 2);
         // DeltaPsi reads pH2 only through its SPF overlaps: once those are
         // queued, the second SPF half step overlaps the A-vector contractions
         delta_psi_overlaps(c, pH, pH2);
         propagate_spf_fork(c, pH2, 0.5 * hdid);
         const double err_psi = delta_psi_contract(c, BH, BH);
+        phase(c, 4);
         propagate_spf_join(c);
+        phase(c, 2);
         calc_h1mode(c, pH2);
+        phase(c, 0);
         const int nk2 = lanczos(c, BH, B02, -0.5 * hdid, true);
+        phase(c, 3);
         launch_axpby(B02, B0, -1.0, 1.0, N, 0, c->stream);
         const double err_A = std::sqrt(norm2(c, B02, N)) * 0.5;
         const double error = std::max(err_A, err_psi) / c->eps_int;
@@ -2424,11 +2521,14 @@ void cmf2_step(mg_ctx *c, double tt, double htry, double *hdid_out, double *hnex
             if (hdid < c->min_stepsize) throw HipError{"CMF2 step size fell below minstep"};
             continue;
         }
+        phase(c, 5);
         lanczos(c, BH, B0, 0.5 * hdid, false);
+        phase(c, 3);
         copy_d2d(c, A, B0, N);
         copy_d2d(c, psi, pH2, D);
         (void)maxscal;
         *hnext_out = propose_next(c, hdid, error);
+        phase(c, 5);
         break;
     }
     *hdid_out = hdid;
@@ -2803,6 +2903,8 @@ int mg_create(const mg_problem *prob, int64_t precision, mg_ctx **out) {
         HIP_CHECK(hipHostMalloc((void **)&c->spf_status_host, sizeof(int), hipHostMallocDefault));
         *c->spf_status_host = 0;
         if (const char *e = std::getenv("MG_GRAPHS")) c->use_graphs = std::atoi(e) != 0;
+        if (const char *e = std::getenv("MG_PHASES")) c->phases = std::atoi(e) != 0;
+        c->phase_t0 = std::chrono::steady_clock::now();
         allocate(c);
         c->stepsize = c->writetime * c->eps_int;  // propagator.f90:103
     });
@@ -2817,6 +2919,11 @@ int mg_create(const mg_problem *prob, int64_t precision, mg_ctx **out) {
 
 int mg_destroy(mg_ctx *c) {
     if (!c) return 0;
+    if (c->phases) {
+        fprintf(stderr, "[mg] CMF2 phases (ms):");
+        for (int i = 0; i < 6; i++) fprintf(stderr, " %s %.1f", kPhaseNames[i], c->phase_ms[i]);
+        fprintf(stderr, "\n");
+    }
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
