@@ -130,6 +130,13 @@ int mg_observables(mg_ctx *ctx, double t, mg_output *out, double *pops);
 int mg_propagate(mg_ctx *ctx, double t0, double tfinal, mg_output *outs, double *pops,
                  int64_t max_out, int64_t *n_out, mg_attempt *atts, int64_t max_att,
                  int64_t *n_att);
+/* inorm_psi of propagate_wf (globals.f90:23-26, propagator.f90:129-144,
+ * 288-305; input keyword `normalize`): 0 never, 1 at the end of the first
+ * output interval, 2 at the end of every output interval, 3 after every
+ * integration step as well.  Renormalisation divides A by the square root of
+ * the wavefunction norm (Trace_wf).  Applies to mg_propagate and
+ * mg_propagate_vmf; mode 1 resets to 0 after its renormalisation. */
+int mg_set_normalize(mg_ctx *ctx, int64_t mode);
 /* Tile autotuner: times H.A and the mean fields on the resident state for
  * the LDS brick sizes (mode grouping) that fit, keeps the best (score =
  * 8 x H.A + 2 x mean fields, their weight in a CMF2 attempt) and returns

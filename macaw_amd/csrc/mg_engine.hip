@@ -199,6 +199,7 @@ struct mg_ctx {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     double last_ms = 0.0, bytes_alg = 0.0, flops_alg = 0.0;
     double stepsize = 0.0;         // carried between mg_propagate calls
+    int inorm_psi = 0;             // propagate_wf's renormalisation mode (globals.f90:23-26)
     std::vector<mg_attempt> replay;
     size_t replay_pos = 0;
     bool prof = false;             // event pair around every group launch
@@ -336,7 +337,51 @@ bool brick_fits_lds(const mg_ctx *c, int b) {
 // ----------------------------------------------------------------------------
 // Setup: layout (init_calc.f90:25-34,160-181), operator basis
 // (hamilton.f90:91-147, 353-373) and the classification of Hamiltonian terms.
+// Every size and index table of the descriptor is range-checked here, on
+// the host, before anything is copied or addressed: the tables below become
+// kernel addressing (state offsets, operator tables, DOF ids), so a malformed
+// descriptor must end in an error message, never in a device access.
+void check_descriptor(const mg_problem *p) {
+    auto need = [](bool ok, const std::string &what) {
+        if (!ok) throw HipError{"invalid problem descriptor: " + what};
+    };
+    need(p->nstates >= 1 && p->nmodes >= 1 && p->ndof >= 1 && p->nham >= 1,
+         "nstates, nmodes, ndof and nham must be >= 1");
+    need(p->nmodes <= 64 && p->ndof <= 256 && p->nham <= 100000, "nmodes <= 64, ndof <= 256, nham <= 100000");
+    need(p->maxdof >= 1 && p->ngpmax >= 1, "maxdof and ngpmax must be >= 1");
+    need(p->nmodeDOF && p->iDOF && p->MultiSet && p->nSPF && p->ngp && p->grid && p->d1 && p->d2 && p->xd1 &&
+             p->ae && p->lhs_el && p->rhs_el && p->iOpDof,
+         "a table pointer is null");
+    need(p->int_type_psi == 1 || p->int_type_psi == 2,
+         "int_type_psi must be 1 (Cash-Karp) or 2 (Dormand-Prince); integrator not yet implemented for the SPFs");
+    need(p->eps_int > 0 && p->eps_psi > 0 && p->eps_rho > 0 && p->writetime > 0 && p->min_stepsize >= 0,
+         "eps_int, eps_psi, eps_rho and writetime must be > 0, min_stepsize >= 0");
+    for (int64_t k = 0; k < p->ndof; k++)
+        need(p->ngp[k] >= 1 && p->ngp[k] <= p->ngpmax, "ngp(" + std::to_string(k + 1) + ") outside 1..ngpmax");
+    std::vector<int> used(p->ndof, 0);
+    for (int64_t m = 0; m < p->nmodes; m++) {
+        const int64_t nd = p->nmodeDOF[m];
+        need(nd >= 1 && nd <= p->maxdof, "nmodeDOF(" + std::to_string(m + 1) + ") outside 1..maxdof");
+        for (int64_t k = 0; k < nd; k++) {
+            const int64_t d = p->iDOF[k + p->maxdof * m];
+            need(d >= 1 && d <= p->ndof, "iDOF(" + std::to_string(k + 1) + "," + std::to_string(m + 1) +
+                                             ") outside 1..ndof");
+            need(!used[d - 1]++, "DOF " + std::to_string(d) + " belongs to two modes");
+        }
+        for (int64_t s = 0; s < p->nstates; s++)
+            need(p->nSPF[m + p->nmodes * s] >= 1,
+                 "nSPF(" + std::to_string(m + 1) + "," + std::to_string(s + 1) + ") must be >= 1");
+    }
+    for (int64_t t = 0; t < p->nham; t++) {
+        need(p->lhs_el[t] >= 1 && p->lhs_el[t] <= p->nstates && p->rhs_el[t] >= 1 && p->rhs_el[t] <= p->nstates,
+             "lhs_el / rhs_el of term " + std::to_string(t + 1) + " outside 1..nstates");
+        need(std::isfinite(p->ae[2 * t]) && std::isfinite(p->ae[2 * t + 1]),
+             "coefficient of term " + std::to_string(t + 1) + " is not finite");
+    }
+}
+
 void setup(mg_ctx *c, const mg_problem *p) {
+    check_descriptor(p);
     c->ns = (int)p->nstates;
     c->nm = (int)p->nmodes;
     c->ndof = (int)p->ndof;
@@ -2855,6 +2900,23 @@ void vmf_rkqs(mg_ctx *c, bool dp, double htry, double tt, double *hdid, double *
     HIP_CHECK(hipStreamSynchronize(c->stream));
 }
 
+// Renormalisation of propagate_wf (propagator.f90:129-144, 288-305):
+// A /= sqrt(Trace_wf(A, psi)), the norm with the SPF overlaps
+// (gmctdh_objects.f90:184-202, one packet).  Mode 3 after every step that
+// does not end an output interval; every nonzero mode at the end of an
+// interval, mode 1 only once.
+void renormalise(mg_ctx *c) {
+    const double n = norm_wf(c, c->dA.as<double2>(), c->dpsi.as<double2>());
+    if (!(n > 0.0) || !std::isfinite(n)) throw HipError{"cannot renormalise: the wavefunction norm is not positive"};
+    launch_axpby(c->dA.p, c->dA.p, 1.0 / std::sqrt(n), 0.0, c->nconf, 0, c->stream);
+    check_launch();
+}
+void renormalise_interval(mg_ctx *c) {
+    if (c->inorm_psi == 0) return;
+    renormalise(c);
+    if (c->inorm_psi == 1) c->inorm_psi = 0;
+}
+
 template <class F>
 int guarded(mg_ctx *c, F &&f) {
     if (!c) return -1;
@@ -2863,14 +2925,24 @@ int guarded(mg_ctx *c, F &&f) {
         explicit Scope(mg_ctx *c) : prev(tl_ctx) { tl_ctx = c; }
         ~Scope() { tl_ctx = prev; }
     } scope(c);
+    // On an error, wait for both streams before returning: a step can throw
+    // (Krylov overflow, minimum step) while the SPF propagation it forked is
+    // still running on stream2, and a retry on the same context must not
+    // overlap it.
+    auto drain = [c] {
+        if (c->stream2) (void)hipStreamSynchronize(c->stream2);
+        if (c->stream) (void)hipStreamSynchronize(c->stream);
+    };
     try {
         f();
         return 0;
     } catch (const HipError &e) {
         c->err = e.msg;
+        drain();
         return 1;
     } catch (const std::exception &e) {
         c->err = e.what();
+        drain();
         return 2;
     }
 }
@@ -3640,7 +3712,9 @@ int mg_propagate_vmf(mg_ctx *c, int64_t method, double t0, double tfinal, mg_out
                 if (time >= tnext - minstep) break;
                 const int64_t k = (int64_t)((tnext - time) / next);
                 stepsize = (k == 0) ? tnext - time : (tnext - time) / (double)(k + 1);
+                if (c->inorm_psi == 3) renormalise(c);
             }
+            renormalise_interval(c);
         }
         c->stepsize = stepsize;
     });
@@ -3652,6 +3726,13 @@ int mg_cmf2_step(mg_ctx *c, double t, double htry, double *hdid, double *hnext) 
 
 int mg_observables(mg_ctx *c, double t, mg_output *out, double *pops) {
     return guarded(c, [&] { observables(c, t, out, pops); });
+}
+
+int mg_set_normalize(mg_ctx *c, int64_t mode) {
+    return guarded(c, [&] {
+        if (mode < 0 || mode > 3) throw HipError{"normalize must be 0, 1, 2 or 3"};
+        c->inorm_psi = (int)mode;
+    });
 }
 
 int mg_propagate(mg_ctx *c, double t0, double tfinal, mg_output *outs, double *pops, int64_t max_out, int64_t *n_out,
@@ -3682,7 +3763,9 @@ int mg_propagate(mg_ctx *c, double t0, double tfinal, mg_output *outs, double *p
                 const int64_t k = (int64_t)((tnext - time) / next);
                 stepsize = (k == 0) ? tnext - time : (tnext - time) / (double)(k + 1);
                 c->stepsize = stepsize;
+                if (c->inorm_psi == 3) renormalise(c);
             }
+            renormalise_interval(c);
         }
         c->stepsize = stepsize;
     });

@@ -73,6 +73,7 @@ def lib():
         L.mg_step_features.argtypes = [vp, f32p, f64p, C.c_int64, i64p]
         L.mg_controller_stats.argtypes = [vp, i64p, i64p]
         L.mg_set_graphs.argtypes = [vp, C.c_int64]
+        L.mg_set_normalize.argtypes = [vp, C.c_int64]
         L.mg_ensemble_propagate.argtypes = [vp, vp, vp, C.c_int64, C.c_int64, C.c_int64, C.c_double, C.c_int64,
                                             C.c_double, C.c_double, C.POINTER(MgOutput), f64p, C.c_int64, i64p,
                                             i64p, f64p, C.c_char_p, C.c_int64]
@@ -263,6 +264,11 @@ class Engine:
         B = np.zeros_like(A)
         self._check(lib().mg_a_to_b(self.h, cplx_ptr(A), cplx_ptr(D), cplx_ptr(B)))
         return B
+
+    def set_normalize(self, mode: int):
+        """inorm_psi of propagate_wf: 0 never, 1 first interval, 2 every
+        output interval, 3 every step (renormalise A by Trace_wf)."""
+        self._check(lib().mg_set_normalize(self.h, int(mode)))
 
     def set_graphs(self, enable=True):
         """Replay the cached launch plans as HIP graphs (default) or launch kernel by kernel."""

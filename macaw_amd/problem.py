@@ -16,7 +16,7 @@ from dataclasses import dataclass, field
 import numpy as np
 
 from .dvr import harm_dvr
-from .inputfile import InputDeck, parse_input, parse_operator_file
+from .inputfile import InputDeck, InputError, parse_input, parse_operator_file
 
 
 @dataclass
@@ -46,6 +46,7 @@ class Problem:
     tfinal: float = 0.0
     int_type_psi: int = 1
     int_type: int = 10            # 10 CMF2, 1 VMF Runge-Kutta (Cash-Karp), 2 VMF Dormand-Prince
+    normalize: int = 0            # inorm_psi (globals.f90:23-26)
     A0: np.ndarray = None         # complex128[nAConftot]
     psi0: np.ndarray = None       # complex128[dimpsi]
     wgrid: np.ndarray = None
@@ -117,6 +118,16 @@ class Problem:
 
 def build_problem(deck: InputDeck, operator_path: str | None = None) -> Problem:
     ns, nm, nd = deck.nstates, deck.nmodes, deck.ndof
+    if deck.int_type == 10:
+        # the CMF2 step integrates A only by Lanczos and the SPFs only by
+        # Cash-Karp / Dormand-Prince; the reference stops on the others when
+        # the step runs (cmf2_base.f90:97-100, 252-255)
+        if deck.int_type_a != 5:
+            raise InputError("Error: integrator not yet implemented for the A coefficients")
+        if deck.int_type_psi not in (1, 2):
+            raise InputError("Error: integrator not yet implemented for the SPFs")
+    if deck.normalize not in (0, 1, 2, 3):
+        raise InputError(f"normalize = {deck.normalize}: expected 0, 1, 2 or 3")
     maxdof = max(len(s.dofs) for s in deck.spf)
     iDOF = np.zeros((maxdof, nm), dtype=np.int64, order="F")
     nmodeDOF = np.zeros(nm, dtype=np.int64)
@@ -164,7 +175,7 @@ def build_problem(deck: InputDeck, operator_path: str | None = None) -> Problem:
                    eps_rho=deck.eps_rho, writetime=deck.tout,
                    min_stepsize=deck.min_stepsize, tinit=deck.tinit,
                    tfinal=deck.tfinal, int_type_psi=deck.int_type_psi or 1, int_type=deck.int_type,
-                   wgrid=wgrid, eps_C=deck.eps_c, eps_ovl=deck.eps_ovl)
+                   wgrid=wgrid, eps_C=deck.eps_c, eps_ovl=deck.eps_ovl, normalize=deck.normalize)
     if dof_gwp.any():
         for m in range(nm):
             kinds = {int(dof_gwp[d - 1]) for d in iDOF[:nmodeDOF[m], m]}

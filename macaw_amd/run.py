@@ -176,7 +176,11 @@ def run(input_path: str, outdir: str = ".", precision: int = 0, controller: str 
     from .problem import load_problem
     from .restart import write_psi_record, write_restart
     prob = load_problem(input_path)
+    if steps and prob.normalize:
+        raise ValueError("--steps drives single CMF2 steps from the host and does not renormalise; "
+                         "run without --steps when the input sets `normalize`")
     e = Engine(prob, precision=precision)
+    e.set_normalize(prob.normalize)
     if controller == "learned":
         e.set_step_model(np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "step_controller.npy")))
     if autotune:

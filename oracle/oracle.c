@@ -58,6 +58,7 @@ struct orc {
     cplx *rho_ave;                /* [nspfmax^2 * nm] */
     cplx *mf, *rmf;               /* [nspfmax^2 * nopmax * nm] */
     double energyMF;
+    int64_t inorm_psi;            /* propagate_wf renormalisation (globals.f90:23-26) */
     /* Lanczos storage */
     cplx *BK;                     /* [nconf * NKRYLOV_MAX] */
     double lan_alpha[NKRYLOV_MAX], lan_U[NKRYLOV_MAX * NKRYLOV_MAX];
@@ -1961,6 +1962,22 @@ int orc_observables(orc *o, double t, mg_output *out, double *pops) {
 }
 
 /* propagate_wf, CMF2 branch (propagator.f90:87-106, 257-306). */
+/* Renormalisation of propagate_wf (propagator.f90:129-144, 288-305):
+ * A /= sqrt(Trace_wf(A, psi)) (gmctdh_objects.f90:184-202, one packet).
+ * Mode 3 after every step that does not end an output interval; every
+ * nonzero mode at the end of an interval, mode 1 only once. */
+static void renormalise(orc *o) {
+    const double n = norm_wf(o, o->A, o->psi);
+    const double f = 1.0 / sqrt(n);
+    for (int64_t i = 0; i < o->nconf; i++) o->A[i] *= f;
+}
+static void renormalise_interval(orc *o) {
+    if (o->inorm_psi == 0) return;
+    renormalise(o);
+    if (o->inorm_psi == 1) o->inorm_psi = 0;
+}
+void orc_set_normalize(orc *o, int64_t mode) { o->inorm_psi = mode; }
+
 int orc_propagate(orc *o, double t0, double tfinal, mg_output *outs, double *pops,
                   int64_t max_out, int64_t *n_out, mg_attempt *atts, int64_t max_att,
                   int64_t *n_att) {
@@ -1982,7 +1999,9 @@ int orc_propagate(orc *o, double t0, double tfinal, mg_output *outs, double *pop
             if (time >= tnext - minstep) break;
             int64_t k = (int64_t)((tnext - time) / next);
             stepsize = (k == 0) ? tnext - time : (tnext - time) / (double)(k + 1);
+            if (o->inorm_psi == 3) renormalise(o);
         }
+        renormalise_interval(o);
     }
     return 0;
 }
@@ -2167,7 +2186,9 @@ int orc_propagate_vmf(orc *o, int64_t method, double t0, double tfinal, mg_outpu
             if (time >= tnext - minstep) break;
             int64_t k = (int64_t)((tnext - time) / next);
             stepsize = (k == 0) ? tnext - time : (tnext - time) / (double)(k + 1);
+            if (o->inorm_psi == 3) renormalise(o);
         }
+        if (!rc) renormalise_interval(o);
     }
     for (int q = 0; q < 7; q++) { free(kA[q]); free(kp[q]); }
     return rc ? -1 : 0;

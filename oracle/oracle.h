@@ -68,6 +68,10 @@ int orc_observables(orc *o, double t, mg_output *out, double *pops);
 int orc_propagate(orc *o, double t0, double tfinal, mg_output *outs, double *pops,
                   int64_t max_out, int64_t *n_out, mg_attempt *atts, int64_t max_att,
                   int64_t *n_att);
+/* inorm_psi of propagate_wf (globals.f90:23-26): 0 never, 1 first output
+ * interval, 2 every interval, 3 every step; used by orc_propagate and
+ * orc_propagate_vmf. */
+void orc_set_normalize(orc *o, int64_t mode);
 
 /* Small dense kernels exposed for pinning tests. */
 void orc_std_reg_inv(int64_t n, const double *A, double eps, int64_t type, double *B);

@@ -45,6 +45,60 @@ def test_unsupported_problems_are_rejected_before_any_device_call(change, messag
         Engine(prob)
 
 
+def _bad_lhs(p):
+    p.lhs_el = p.lhs_el.copy()
+    p.lhs_el[0] = 0
+
+
+def _bad_rhs(p):
+    p.rhs_el = p.rhs_el.copy()
+    p.rhs_el[-1] = p.nstates + 1
+
+
+def _bad_idof(p):
+    p.iDOF = p.iDOF.copy()
+    p.iDOF[0, 0] = p.ndof + 3
+
+
+def _zero_spf(p):
+    p.nSPF = p.nSPF.copy()
+    p.nSPF[1, 0] = 0
+
+
+def _small_maxdof(p):
+    p.nmodeDOF = p.nmodeDOF.copy()
+    p.nmodeDOF[0] = p.maxDOF + 1
+
+
+def _bad_int_type(p):
+    p.int_type_psi = 3
+
+
+def _nan_coefficient(p):
+    p.ae = p.ae.copy()
+    p.ae[2] = complex("nan")
+
+
+@pytest.mark.parametrize("mutate, message", [
+    (_bad_lhs, "lhs_el / rhs_el of term 1 outside"),
+    (_bad_rhs, "lhs_el / rhs_el of term"),
+    (_bad_idof, r"iDOF\(1,1\) outside 1..ndof"),
+    (_zero_spf, r"nSPF\(2,1\) must be >= 1"),
+    (_small_maxdof, r"nmodeDOF\(1\) outside 1..maxdof"),
+    (_bad_int_type, r"int_type_psi must be 1 \(Cash-Karp\) or 2 .*integrator not yet implemented for the SPFs"),
+    (_nan_coefficient, "coefficient of term 3 is not finite"),
+])
+def test_malformed_descriptors_are_rejected_before_any_device_call(mutate, message, tmp_path):
+    """Out-of-range index tables (states, DOF ids, SPF counts, DOFs per mode)
+    and an unknown SPF integrator are refused by mg_create on the host, with
+    the offending field named; none of them reaches a kernel's addressing."""
+    from macaw_amd.engine import Engine, EngineError
+    prob = make_problem("t1", str(tmp_path))
+    mutate(prob)
+    with pytest.raises(EngineError, match="invalid problem descriptor: " + message):
+        Engine(prob)
+
+
 @gpu
 def test_engine_min_step_guard(tmp_path):
     """A rejected attempt whose retry step falls below min_stepsize ends the
