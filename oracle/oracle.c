@@ -59,6 +59,7 @@ struct orc {
     cplx *mf, *rmf;               /* [nspfmax^2 * nopmax * nm] */
     double energyMF;
     int64_t inorm_psi;            /* propagate_wf renormalisation (globals.f90:23-26) */
+    int64_t cmf2_fault;           /* test hook: a planted mistake in cmf2 (orc_set_cmf2_fault) */
     /* Lanczos storage */
     cplx *BK;                     /* [nconf * NKRYLOV_MAX] */
     double lan_alpha[NKRYLOV_MAX], lan_U[NKRYLOV_MAX * NKRYLOV_MAX];
@@ -1868,10 +1869,12 @@ static int cmf2(orc *o, double tt, double htry, double *hdid_out, double *hnext_
             for (int64_t m = 0; m < o->nm; m++)
                 if (o->gwp[m] == pass) rc |= propagate_spf(o, 0.5 * hdid, m, pH, DH);
         b_to_a(o, BH, AH, DH);
-        for (int64_t m = 0; m < o->nm; m++) if (o->gwp[m]) calc_s00(o, pH, m);
-        calc_rho(o, AH);
-        for (int64_t m = 0; m < o->nm; m++) calc_h1mode_any(o, pH, m);
-        calc_meanfield(o, AH);
+        if (o->cmf2_fault != 1) {   /* fault 1: keep the t0 mean fields */
+            for (int64_t m = 0; m < o->nm; m++) if (o->gwp[m]) calc_s00(o, pH, m);
+            calc_rho(o, AH);
+            for (int64_t m = 0; m < o->nm; m++) calc_h1mode_any(o, pH, m);
+            calc_meanfield(o, AH);
+        }
         memcpy(pH2, o->psi, sizeof(cplx) * D);
         memcpy(DH2, o->DD, sizeof(cplx) * nD);
         for (int64_t pass = 0; pass < 2; pass++)
@@ -1907,7 +1910,16 @@ static int cmf2(orc *o, double tt, double htry, double *hdid_out, double *hnext_
             }
             continue;
         }
-        lanczos(o, BH, B0, 0.5 * hdid, 0);
+        if (o->cmf2_fault == 2) {
+            /* fault 2: A over the whole step with the t0 Hamiltonian */
+            for (int64_t m = 0; m < o->nm; m++) calc_h1mode_any(o, o->psi, m);
+            if (o->ngwp) calc_dhd(o, o->DD);
+            a_to_b(o, o->A, B0, o->DD);
+            memcpy(BH, B0, sizeof(cplx) * N);
+            if (lanczos(o, BH, B0, hdid, 1) < 0) { rc = -1; break; }
+        } else {
+            lanczos(o, BH, B0, 0.5 * hdid, 0);
+        }
         b_to_a(o, B0, o->A, DH2);
         memcpy(o->psi, pH2, sizeof(cplx) * D);
         memcpy(o->DD, DH2, sizeof(cplx) * nD);
@@ -1922,6 +1934,12 @@ static int cmf2(orc *o, double tt, double htry, double *hdid_out, double *hnext_
     if (o->ngwp) free(AH);
     return rc;
 }
+
+/* Test hook: plant a mistake in the CMF2 step, so that the order test can
+ * show it fails.  1: the SPFs' second half step uses the mean fields of t0
+ * (no re-evaluation at the half step); 2: the accepted A is propagated over
+ * the whole step with the t0 Hamiltonian.  0 restores the scheme. */
+void orc_set_cmf2_fault(orc *o, int64_t fault) { o->cmf2_fault = fault; }
 
 int orc_cmf2_step(orc *o, double t, double htry, double *hdid, double *hnext,
                   mg_attempt *atts, int64_t max_att, int64_t *n_att) {

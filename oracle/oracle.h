@@ -72,6 +72,9 @@ int orc_propagate(orc *o, double t0, double tfinal, mg_output *outs, double *pop
  * interval, 2 every interval, 3 every step; used by orc_propagate and
  * orc_propagate_vmf. */
 void orc_set_normalize(orc *o, int64_t mode);
+/* Test hook: a planted mistake in cmf2 (0 none, 1 mean fields of t0 for the
+ * second SPF half step, 2 A propagated with the t0 Hamiltonian). */
+void orc_set_cmf2_fault(orc *o, int64_t fault);
 
 /* Small dense kernels exposed for pinning tests. */
 void orc_std_reg_inv(int64_t n, const double *A, double eps, int64_t type, double *B);

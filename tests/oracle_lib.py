@@ -56,6 +56,7 @@ def lib():
         L.orc_propagate.argtypes = [C.c_void_p, C.c_double, C.c_double, C.POINTER(MgOutput),
                                     f64p, C.c_int64, i64p, C.POINTER(MgAttempt), C.c_int64, i64p]
         L.orc_set_normalize.argtypes = [C.c_void_p, C.c_int64]
+        L.orc_set_cmf2_fault.argtypes = [C.c_void_p, C.c_int64]
         L.orc_std_reg_inv.argtypes = [C.c_int64, f64p, C.c_double, C.c_int64, f64p]
         L.orc_herm_eig.argtypes = [C.c_int64, f64p, f64p, f64p]
         L.orc_tridiag_eig.argtypes = [C.c_int64, f64p, f64p, f64p, f64p]
@@ -222,6 +223,9 @@ class Oracle:
         if rc != 0:
             raise RuntimeError(lib().orc_error(self.h).decode())
         return list(outs[: no.value]), pops[: no.value], list(atts[: na.value])
+
+    def set_cmf2_fault(self, fault):
+        lib().orc_set_cmf2_fault(self.h, int(fault))
 
     def set_normalize(self, mode):
         lib().orc_set_normalize(self.h, int(mode))

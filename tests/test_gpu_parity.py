@@ -270,3 +270,37 @@ def test_autotune_keeps_parity(name, tmp_path):
     e.set_replay(atts_o)
     outs_e, pops_e, _ = e.propagate(0.0, 0.5 / 0.02418884326505)
     compare_outputs(outs_e, pops_e, outs_o, pops_o, 1e-10)
+
+
+@gpu
+def test_cmf2_fixed_steps_match_oracle_and_are_second_order(tmp_path):
+    """The CMF2 order pin of the oracle, on the engine: N fixed steps forced
+    through the attempt replay (every attempt accepted), the same states as
+    the oracle's fixed-step runs, and the self-convergence ratio ~4 of a
+    second-order scheme."""
+    from macaw_amd.abi import MgAttempt
+    from macaw_amd.engine import Engine
+    prob = make_problem("t1", str(tmp_path))
+    prob.eps_int = 1e6
+    prob.eps_psi = 1e-12
+    states = {}
+    for n in (4, 8, 16):
+        h = 2.0 / 0.02418884326505 / n
+        e = Engine(prob)
+        e.set_replay([MgAttempt(time=0.0, h=h, err_A=0.0, err_psi=0.0, accepted=1, nkrylov1=0, nkrylov2=0)] * n)
+        o = Oracle(prob)
+        t = 0.0
+        for _ in range(n):
+            hd, _ = e.cmf2_step(t, h)
+            rc, hd_o, _, _ = o.cmf2_step(t, h)
+            assert rc == 0 and hd == h and hd_o == h
+            t += hd
+        Ae, pe = e.get_state()
+        Ao, po = o.get_state()
+        assert o.delta_psi(Ae, pe, Ao, po) < 1e-10
+        states[n] = (Ae, pe)
+        e.close()
+    o = Oracle(prob)
+    d1 = o.delta_psi(*states[4], *states[8])
+    d2 = o.delta_psi(*states[8], *states[16])
+    assert 3.6 < d1 / d2 < 4.4, (d1, d2)

@@ -396,3 +396,75 @@ def test_full_basis_vmf_equals_exact_propagation(method, tmp_path):
         assert out.norm == pytest.approx(1.0, abs=1e-6)
     assert len(outs) == 5
     assert any(not a.accepted for a in atts) or len(atts) > 5
+
+
+# ---------------------------------------------------------------------------
+# The CMF2 scheme itself (cmf2_base.f90:47-206): fixed steps (every attempt
+# accepted: eps_int far above any error), SPF Runge-Kutta at 1e-12 so that
+# its error stays below the scheme's.  CMF2 is second order (Beck & Meyer,
+# Z. Phys. D 42, 113 (1997)), so the distance of the wavefunctions reached
+# with N and 2N steps falls by ~4 per halving; each planted mistake below
+# (the oracle's test hook) drops the scheme to first order (~2).
+
+def _fixed_step_state(prob, n, fault=0, T_fs=2.0):
+    o = Oracle(prob)
+    o.set_cmf2_fault(fault)
+    h = T_fs / 0.02418884326505 / n
+    t = 0.0
+    for _ in range(n):
+        rc, hd, hn, atts = o.cmf2_step(t, h)
+        assert rc == 0 and len(atts) == 1 and atts[0].accepted and hd == h
+        t += hd
+    return o.get_state()
+
+
+def _self_convergence(prob, fault=0, ns=(4, 8, 16)):
+    st = {n: _fixed_step_state(prob, n, fault) for n in ns}
+    o = Oracle(prob)
+    d = [o.delta_psi(*st[a], *st[b]) for a, b in zip(ns[:-1], ns[1:])]
+    return [d[i] / d[i + 1] for i in range(len(d) - 1)], d
+
+
+@pytest.fixture(scope="module")
+def fixed_step_case(tmp_path_factory):
+    prob = make_problem("t1", str(tmp_path_factory.mktemp("cmf2order")))
+    prob.eps_int = 1e6
+    prob.eps_psi = 1e-12
+    return prob
+
+
+def test_cmf2_is_second_order(fixed_step_case):
+    ratios, d = _self_convergence(fixed_step_case, ns=(4, 8, 16, 32))
+    assert d[0] > 1e-5                    # the scheme's error dominates the tolerances
+    assert 3.6 < ratios[0] < 4.4 and 3.6 < ratios[1] < 4.4, ratios
+
+
+@pytest.mark.parametrize("fault", [1, 2])
+def test_cmf2_order_test_catches_planted_mistakes(fixed_step_case, fault):
+    """1: no mean-field re-evaluation at the half step; 2: the accepted A
+    propagated with the t0 Hamiltonian.  Both are first order."""
+    ratios, _ = _self_convergence(fixed_step_case, fault, ns=(4, 8, 16, 32))
+    assert max(ratios) < 3.0, ratios
+
+
+def test_cmf2_interval_landing(tmp_path):
+    """propagate_wf's landing rule (propagator.f90:276-280): the outputs fall
+    on t0 + k tout, the accepted steps of an interval add up to tout exactly
+    (to rounding) and none crosses an output time."""
+    prob = make_problem("t1", str(tmp_path))
+    o = Oracle(prob)
+    outs, pops, atts = o.propagate(0.0, 2.0 / 0.02418884326505)
+    tout = prob.writetime
+    assert len(outs) == 5
+    for k, out in enumerate(outs):
+        assert abs(out.time - k * tout) < 1e-9 * tout
+    acc = [a for a in atts if a.accepted]
+    for k in range(4):
+        lo, hi = k * tout, (k + 1) * tout
+        steps = [a for a in acc if lo - 1e-9 * tout <= a.time < hi - 1e-9 * tout]
+        assert abs(sum(a.h for a in steps) - tout) < 1e-9 * tout
+        assert all(a.time + a.h <= hi * (1 + 1e-12) for a in steps)
+    # rejected attempts retry from the same time with a smaller step
+    for a, b in zip(atts[:-1], atts[1:]):
+        if not a.accepted:
+            assert b.time == a.time and b.h < a.h
