@@ -9,8 +9,14 @@ module macaw_gpu
    public :: mg_problem, mg_output, mg_attempt
    public :: mg_create, mg_destroy, mg_last_error, mg_nconf, mg_dimpsi, mg_dims
    public :: mg_set_state, mg_get_state, mg_set_start, mg_calc_h1mode, mg_calc_rho
-   public :: mg_calc_meanfield, mg_apply_h, mg_der_spf, mg_cmf2_step, mg_observables
-   public :: mg_propagate, mg_check
+   public :: mg_calc_meanfield, mg_apply_h, mg_apply_h_device, mg_der_spf, mg_cmf2_step, mg_observables
+   public :: mg_propagate, mg_set_normalize, mg_stepsize, mg_set_replay, mg_autotune
+   public :: mg_calc_derivs, mg_propagate_vmf
+   public :: mg_prepare_gwp, mg_gwp_overlaps, mg_der_spf_gwp, mg_a_to_b
+   public :: mg_apply_h_stats, mg_apply_h_profile, mg_profile_copy
+   public :: mg_set_step_model, mg_set_graphs, mg_log_step_features, mg_step_features, mg_controller_stats
+   public :: mg_mlp_train, mg_mlp_forward, mg_mlp_grad, mg_ensemble_propagate
+   public :: mg_check
 
    ! Problem descriptor (psidef.f90:11-55, dvr.f90:14-27, hamilton.f90:24-36)
    type, bind(C) :: mg_problem
@@ -143,6 +149,162 @@ module macaw_gpu
          import :: c_int, c_int64_t, c_ptr
          type(c_ptr), value :: ctx
          integer(c_int64_t), value :: enable
+      end function
+      ! inorm_psi of propagate_wf (globals.f90:23-26): 0 never, 1 first
+      ! interval, 2 every output interval, 3 every step.
+      integer(c_int) function mg_set_normalize(ctx, mode) bind(C, name='mg_set_normalize')
+         import :: c_int, c_int64_t, c_ptr
+         type(c_ptr), value :: ctx
+         integer(c_int64_t), value :: mode
+      end function
+      integer(c_int) function mg_stepsize(ctx, h) bind(C, name='mg_stepsize')
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: ctx
+         real(c_double), intent(out) :: h
+      end function
+      integer(c_int) function mg_set_replay(ctx, log, n) bind(C, name='mg_set_replay')
+         import :: c_int, c_int64_t, c_ptr, mg_attempt
+         type(c_ptr), value :: ctx
+         type(mg_attempt), intent(in) :: log(*)
+         integer(c_int64_t), value :: n
+      end function
+      integer(c_int) function mg_autotune(ctx, reps, best_brick, best_score_ms) bind(C, name='mg_autotune')
+         import :: c_int, c_int64_t, c_ptr, c_double
+         type(c_ptr), value :: ctx
+         integer(c_int64_t), value :: reps
+         integer(c_int64_t), intent(out) :: best_brick
+         real(c_double), intent(out) :: best_score_ms
+      end function
+      integer(c_int) function mg_apply_h_device(ctx, dB, dHB) bind(C, name='mg_apply_h_device')
+         import :: c_int, c_ptr
+         type(c_ptr), value :: ctx, dB, dHB
+      end function
+      integer(c_int) function mg_apply_h_stats(ctx, bytes_alg, flops_alg) bind(C, name='mg_apply_h_stats')
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: ctx
+         real(c_double), intent(out) :: bytes_alg, flops_alg
+      end function
+      integer(c_int) function mg_apply_h_profile(ctx, dB, dHB, kernel_ms, nlaunch) bind(C, name='mg_apply_h_profile')
+         import :: c_int, c_int64_t, c_ptr, c_double
+         type(c_ptr), value :: ctx, dB, dHB
+         real(c_double), intent(out) :: kernel_ms
+         integer(c_int64_t), intent(out) :: nlaunch
+      end function
+      integer(c_int) function mg_profile_copy(ctx, n, kernel_ms) bind(C, name='mg_profile_copy')
+         import :: c_int, c_int64_t, c_ptr, c_double
+         type(c_ptr), value :: ctx
+         integer(c_int64_t), value :: n
+         real(c_double), intent(out) :: kernel_ms
+      end function
+      ! Calc_Derivs (derivatives.f90:39-243) and propagate_wf's Runge-Kutta
+      ! branch (propagator.f90:108-140); method 1 Cash-Karp, 2 Dormand-Prince.
+      integer(c_int) function mg_calc_derivs(ctx, dA, dpsi) bind(C, name='mg_calc_derivs')
+         import :: c_int, c_ptr, c_double_complex
+         type(c_ptr), value :: ctx
+         complex(c_double_complex), intent(out) :: dA(*), dpsi(*)
+      end function
+      integer(c_int) function mg_propagate_vmf(ctx, method, t0, tfinal, outs, pops, max_out, n_out, &
+                                               atts, max_att, n_att) bind(C, name='mg_propagate_vmf')
+         import :: c_int, c_int64_t, c_ptr, c_double, mg_output, mg_attempt
+         type(c_ptr), value :: ctx
+         integer(c_int64_t), value :: method
+         real(c_double), value :: t0, tfinal
+         type(mg_output), intent(out) :: outs(*)
+         real(c_double), intent(out) :: pops(*)
+         integer(c_int64_t), value :: max_out
+         integer(c_int64_t), intent(out) :: n_out
+         type(mg_attempt), intent(out) :: atts(*)
+         integer(c_int64_t), value :: max_att
+         integer(c_int64_t), intent(out) :: n_att
+      end function
+      ! Gaussian (G-MCTDH) modes: Calc_S00 / InverseSquareRootU / Calc_DHD
+      ! (mg_prepare_gwp), the overlaps and D0, Calc_Der_SPF of a Gaussian
+      ! mode, A_to_B (cmf2_base.f90:264-305).
+      integer(c_int) function mg_prepare_gwp(ctx) bind(C, name='mg_prepare_gwp')
+         import :: c_int, c_ptr
+         type(c_ptr), value :: ctx
+      end function
+      integer(c_int) function mg_gwp_overlaps(ctx, s00, dd) bind(C, name='mg_gwp_overlaps')
+         import :: c_int, c_ptr, c_double_complex
+         type(c_ptr), value :: ctx
+         complex(c_double_complex), intent(out) :: s00(*), dd(*)
+      end function
+      integer(c_int) function mg_der_spf_gwp(ctx, im, psi, dd, dpsi, ddd) bind(C, name='mg_der_spf_gwp')
+         import :: c_int, c_int64_t, c_ptr, c_double_complex
+         type(c_ptr), value :: ctx
+         integer(c_int64_t), value :: im
+         complex(c_double_complex), intent(in) :: psi(*), dd(*)
+         complex(c_double_complex), intent(out) :: dpsi(*), ddd(*)
+      end function
+      integer(c_int) function mg_a_to_b(ctx, A, dd, B) bind(C, name='mg_a_to_b')
+         import :: c_int, c_ptr, c_double_complex
+         type(c_ptr), value :: ctx
+         complex(c_double_complex), intent(in) :: A(*), dd(*)
+         complex(c_double_complex), intent(out) :: B(*)
+      end function
+      ! Learned step controller: training, evaluation and the step logs.
+      integer(c_int) function mg_log_step_features(ctx, enable) bind(C, name='mg_log_step_features')
+         import :: c_int, c_int64_t, c_ptr
+         type(c_ptr), value :: ctx
+         integer(c_int64_t), value :: enable
+      end function
+      integer(c_int) function mg_step_features(ctx, features, h, max, n) bind(C, name='mg_step_features')
+         import :: c_int, c_int64_t, c_ptr, c_float, c_double
+         type(c_ptr), value :: ctx
+         real(c_float), intent(out) :: features(*)
+         real(c_double), intent(out) :: h(*)
+         integer(c_int64_t), value :: max
+         integer(c_int64_t), intent(out) :: n
+      end function
+      integer(c_int) function mg_controller_stats(ctx, proposals, fallbacks) bind(C, name='mg_controller_stats')
+         import :: c_int, c_int64_t, c_ptr
+         type(c_ptr), value :: ctx
+         integer(c_int64_t), intent(out) :: proposals, fallbacks
+      end function
+      integer(c_int) function mg_mlp_train(ctx, X, t, n, epochs, batch, lr, seed, params, loss) &
+            bind(C, name='mg_mlp_train')
+         import :: c_int, c_int64_t, c_ptr, c_float, c_double
+         type(c_ptr), value :: ctx
+         real(c_float), intent(in) :: X(*), t(*)
+         integer(c_int64_t), value :: n, epochs, batch
+         real(c_double), value :: lr
+         integer(c_int64_t), value :: seed
+         real(c_float), intent(inout) :: params(*)
+         real(c_double), intent(out) :: loss
+      end function
+      integer(c_int) function mg_mlp_forward(ctx, params, X, n, y) bind(C, name='mg_mlp_forward')
+         import :: c_int, c_int64_t, c_ptr, c_float
+         type(c_ptr), value :: ctx
+         real(c_float), intent(in) :: params(*), X(*)
+         integer(c_int64_t), value :: n
+         real(c_float), intent(out) :: y(*)
+      end function
+      integer(c_int) function mg_mlp_grad(ctx, params, X, t, n, grad, loss) bind(C, name='mg_mlp_grad')
+         import :: c_int, c_int64_t, c_ptr, c_float, c_double
+         type(c_ptr), value :: ctx
+         real(c_float), intent(in) :: params(*), X(*), t(*)
+         integer(c_int64_t), value :: n
+         real(c_float), intent(out) :: grad(*)
+         real(c_double), intent(out) :: loss
+      end function
+      ! Ensemble of independent wavepackets: probs, A0, psi0 are arrays of
+      ! n C pointers (c_loc of each member's descriptor and state).
+      integer(c_int) function mg_ensemble_propagate(probs, A0, psi0, n, precision, nthreads, mem_fraction, &
+            flags, t0, tfinal, outs, pops, max_out, n_out, status, member_wall, errbuf, errlen) &
+            bind(C, name='mg_ensemble_propagate')
+         import :: c_int, c_int64_t, c_ptr, c_double, c_char, mg_output
+         type(c_ptr), intent(in) :: probs(*), A0(*), psi0(*)
+         integer(c_int64_t), value :: n, precision, nthreads
+         real(c_double), value :: mem_fraction
+         integer(c_int64_t), value :: flags
+         real(c_double), value :: t0, tfinal
+         type(mg_output), intent(out) :: outs(*)
+         real(c_double), intent(out) :: pops(*)
+         integer(c_int64_t), value :: max_out
+         integer(c_int64_t), intent(out) :: n_out(*), status(*)
+         real(c_double), intent(out) :: member_wall(*)
+         character(kind=c_char), intent(out) :: errbuf(*)
+         integer(c_int64_t), value :: errlen
       end function
    end interface
 

@@ -62,3 +62,54 @@ def test_fortran_driver_matches_oracle(tmp_path):
         assert abs(row[1] - o.auto_re) < 1e-8 and abs(row[2] - o.auto_im) < 1e-8
     spop = np.loadtxt(str(tmp_path / "spop"), comments="#")
     assert np.abs(spop[:, 1:] - pops).max() < 2e-7   # f10.7 output
+
+
+REF_SRC = "/root/reference/source"
+FFLAGS = ["-fdefault-integer-8", "-fdefault-real-8", "-fdefault-double-8"]
+
+
+@pytest.mark.skipif(not os.path.exists(FLANG), reason="flang not available")
+@pytest.mark.skipif(not os.path.isdir(REF_SRC), reason="Macaw sources not present")
+def test_cmf2_base_dropin_compiles_against_macaw_modules(tmp_path):
+    """The drop-in module of INTEGRATION.md, committed as
+    macaw_amd/fortran/cmf2_base_dropin.f90, compiled against Macaw's own
+    sysparam / globals / psidef (built here from the reference sources, as
+    they are) and declaration mirrors of dvr / hamilton with the reference's
+    attributes, then linked into a propagate_wf-shaped program with the
+    engine library.  Module arrays without TARGET would fail c_loc here."""
+    if not os.path.exists(os.path.join(ROOT, "macaw_amd", "libmacaw_gpu.so")):
+        from macaw_amd.build import build
+        build()
+    d = str(tmp_path)
+    srcs = [os.path.join(REF_SRC, f) for f in ("sysparam.f90", "globals.f90", "psidef.f90")]
+    srcs += [os.path.join(ROOT, "tests", "fortran", "dvr_hamilton_decl.f90"),
+             os.path.join(FDIR, "macaw_gpu.f90"), os.path.join(FDIR, "cmf2_base_dropin.f90"),
+             os.path.join(ROOT, "tests", "fortran", "use_dropin.f90")]
+    objs = []
+    for src in srcs:
+        obj = os.path.join(d, os.path.basename(src) + ".o")
+        r = subprocess.run([FLANG, *FFLAGS, "-J", d, "-c", src, "-o", obj], capture_output=True, text=True)
+        assert r.returncode == 0, (src, r.stderr)
+        objs.append(obj)
+    exe = os.path.join(d, "use_dropin")
+    r = subprocess.run([FLANG, "-o", exe, *objs, "-L" + os.path.join(ROOT, "macaw_amd"), "-lmacaw_gpu",
+                        "-Wl,-rpath," + os.path.join(ROOT, "macaw_amd")], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    syms = subprocess.run(["nm", objs[-2]], capture_output=True, text=True).stdout
+    for name in ("mg_create", "mg_set_start", "mg_set_state", "mg_cmf2_step", "mg_get_state"):
+        assert " U " + name in syms
+
+
+def test_every_exported_entry_point_has_a_fortran_binding():
+    """include/macaw_gpu.h and macaw_amd/fortran/macaw_gpu.f90 declare the
+    same entry points, and every bound name is public."""
+    import re
+    hdr = open(os.path.join(ROOT, "include", "macaw_gpu.h")).read()
+    exported = set(re.findall(r"^(?:int|int64_t|const char \*)\s*(mg_\w+)\(", hdr, re.M))
+    f90 = open(os.path.join(FDIR, "macaw_gpu.f90")).read()
+    bound = set(re.findall(r"bind\(C, name='(mg_\w+)'\)", f90))
+    public = set()
+    for line in re.findall(r"^\s*public ::(.*)$", f90, re.M):
+        public |= {x.strip() for x in line.split(",")}
+    assert exported and exported == bound, (exported - bound, bound - exported)
+    assert bound <= public, bound - public
