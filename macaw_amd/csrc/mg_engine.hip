@@ -153,6 +153,7 @@ struct mg_ctx {
     int n_rmf_jobs = 0;
     // ---- grouping ----
     int brick = 1024;
+    int run_bytes = 32;            // shortest row of a strided brick (make_groups)
     std::vector<std::pair<int, int>> groups;
     // ---- device buffers ----
     DevBuf dA, dpsi, dAstart, dpsistart, dBK, dB0, dBH, dB02, dpH, dpH2, dHtmp, dIn, dOut;
@@ -309,17 +310,20 @@ int64_t largest_divisor_le(int64_t L, int64_t cap) {
 }
 
 // Mode groups for the current brick capacity (sizes: max over states): the
-// first group takes leading modes while their product fits the brick; later
-// groups at most brick/8, so that Lt >= 8 keeps the brick runs contiguous.
+// first group takes leading modes while their product fits the brick; a
+// later group is strided (its brick rows are Lt consecutive elements), so it
+// takes at most brick / Lt_min modes' worth, Lt_min = run_bytes / element
+// size (rows of at least run_bytes).
 void make_groups(mg_ctx *c) {
     const int nm = c->nm, ns = c->ns;
     c->groups.clear();
     std::vector<int64_t> mx(nm, 0);
     for (int m = 0; m < nm; m++)
         for (int s = 0; s < ns; s++) mx[m] = std::max(mx[m], c->NS(m, s));
+    const int64_t lt_min = std::max<int64_t>(1, c->run_bytes / (int64_t)c->es);
     int m = 0;
     while (m < nm) {
-        const int64_t limit = (m == 0) ? c->brick : std::max<int64_t>(c->brick / 8, mx[m]);
+        const int64_t limit = (m == 0) ? c->brick : std::max<int64_t>(c->brick / lt_min, mx[m]);
         int64_t prod = mx[m];
         int e = m + 1;
         while (e < nm && e - m < kMaxGroupModes && prod * mx[e] <= limit) prod *= mx[e++];
@@ -617,6 +621,10 @@ void setup(mg_ctx *c, const mg_problem *p) {
     // capacity in complex elements (tuning).  Two padded bricks plus the op
     // matrices must fit the 160 KB of LDS a workgroup may address.
     c->brick = c->prec ? 2048 : 1024;
+    if (const char *e = std::getenv("MG_RUN_BYTES")) {
+        const int r = std::atoi(e);
+        if (r >= 8 && r <= 512) c->run_bytes = r;
+    }
     if (const char *e = std::getenv("MG_BRICK")) {
         const int b = std::atoi(e);
         if (brick_fits_lds(c, b)) c->brick = b;
@@ -3596,19 +3604,42 @@ int mg_autotune(mg_ctx *c, int64_t reps, int64_t *best_brick, double *best_ms) {
         // candidates around the default that fit the LDS; score = H.A time x 8
         // + mean-field time x 2 (their weight in one CMF2 attempt)
         const int def = c->prec ? 2048 : 1024;
-        std::vector<int> cand;
-        for (int b : {def / 4, def / 2, def, def * 2, def * 4})
-            if (brick_fits_lds(c, b)) cand.push_back(b);
+        // candidates: LDS brick sizes around the default that fit, each with
+        // strided-group rows of 32 and 64 bytes when they group the modes
+        // differently
+        std::vector<std::pair<int, int>> cand;
+        for (int b : {def / 4, def / 2, def, def * 2, def * 4}) {
+            if (!brick_fits_lds(c, b)) continue;
+            std::vector<std::pair<int, int>> seen;
+            for (int rb : {32, 64}) {
+                c->brick = b;
+                c->run_bytes = rb;
+                make_groups(c);
+                bool dup = false;
+                for (auto &q : cand)
+                    if (q.first == b) {
+                        c->brick = b;
+                        c->run_bytes = q.second;
+                        const auto g0 = c->groups;
+                        c->run_bytes = rb;
+                        make_groups(c);
+                        dup = dup || (g0 == c->groups);
+                    }
+                if (!dup) cand.push_back({b, rb});
+            }
+        }
         const int R = reps > 0 ? (int)reps : 2;
-        // input: the resident A (its FP32 copy, written by calc_rho, in the mixed form)
+        // input: the resident A (an FP32 copy in the mixed form)
+        if (c->prec) launch_convert(c->dAf.p, 1, c->dA.p, 0, c->nconf, c->stream);
         const void *in = c->prec ? c->dAf.p : c->dA.p;
         void *out = c->prec ? c->dYf.p : c->dOut.p;
         double best = 1e300;
-        int bestb = c->brick;
-        for (int b : cand) {
+        int bestb = c->brick, bestr = c->run_bytes;
+        for (auto [b, rb] : cand) {
             HIP_CHECK(hipStreamSynchronize(c->stream));
             c->plans.clear();
             c->brick = b;
+            c->run_bytes = rb;
             make_groups(c);
             alloc_mf_tensors(c);
             calc_rho(c, c->dA.as<double2>());
@@ -3634,16 +3665,18 @@ int mg_autotune(mg_ctx *c, int64_t reps, int64_t *best_brick, double *best_ms) {
             }
             const double score = 8.0 * tha + 2.0 * tmf;
             if (std::getenv("MG_VERBOSE"))
-                fprintf(stderr, "autotune: brick %d  H.A %.3f ms  mean fields %.3f ms  score %.3f\n", b, tha, tmf,
-                        score);
+                fprintf(stderr, "autotune: brick %d rows %d B (%zu groups)  H.A %.3f ms  mean fields %.3f ms  score %.3f\n",
+                        b, rb, c->groups.size(), tha, tmf, score);
             if (score < best) {
                 best = score;
                 bestb = b;
+                bestr = rb;
             }
         }
         HIP_CHECK(hipStreamSynchronize(c->stream));
         c->plans.clear();
         c->brick = bestb;
+        c->run_bytes = bestr;
         make_groups(c);
         alloc_mf_tensors(c);
         if (best_brick) *best_brick = bestb;

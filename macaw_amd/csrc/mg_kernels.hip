@@ -323,6 +323,18 @@ template <> struct GramMfma<double2> {
     static __device__ __forceinline__ int row(int q, int v) { return 4 * v + q; }
 };
 
+// Workgroups are dispatched round-robin over the 8 XCDs (linear workgroup id
+// modulo 8), and each XCD has its own L2.  With a grid width that is a
+// multiple of 8, block x runs on XCD x % 8 whatever its task row; the remap
+// gives the blocks of one XCD consecutive logical indices, so the bricks a
+// XCD works on at the same time are neighbours in memory and short runs
+// (32-byte rows of a strided brick) complete their 128-byte lines in that
+// XCD's L2 instead of in several.  A bijection on [0, G); identity otherwise.
+__device__ __forceinline__ int64_t xcd_block(int b, int G) {
+    if (G & 7) return b;
+    return (int64_t)(b & 7) * (G >> 3) + (b >> 3);
+}
+
 // GRAM: the launch carries kBGram instructions; the grid is persistent over
 // the bricks and every workgroup ends by writing its Gram partials.
 template <int NMAX, class T, bool GRAM>
@@ -333,7 +345,7 @@ __global__ void __launch_bounds__(GRAM ? 256 : 1024) k_brick(const BrickTask *__
     const int Lt = (int)t.Lt, Rt = (int)t.Rt;
     const int64_t nlb = L / Lt, nrb = R / Rt;
     const int64_t nbricks = nlb * nrb;
-    if (!GRAM && (int64_t)blockIdx.x >= nbricks) return;
+    if (!GRAM && xcd_block(blockIdx.x, gridDim.x) >= nbricks) return;
     const FastDiv fd_Lt = t.fd_Lt;
     extern __shared__ char lds_raw[];
     T *mats = reinterpret_cast<T *>(lds_raw);
@@ -360,7 +372,7 @@ __global__ void __launch_bounds__(GRAM ? 256 : 1024) k_brick(const BrickTask *__
     const int nins = t.ninstr;
     // a workgroup walks its task's bricks with the grid stride, so the op
     // matrices staged in LDS above serve every brick it takes
-    for (int64_t bi = blockIdx.x; bi < nbricks; bi += gridDim.x) {
+    for (int64_t bi = xcd_block(blockIdx.x, gridDim.x); bi < nbricks; bi += gridDim.x) {
         const int64_t l0 = (bi % nlb) * Lt, r0 = (bi / nlb) * Rt;
         for (int k = 0; k < nins; k++) {
             const BrickInstr &I = t.ins[k];
