@@ -180,6 +180,99 @@ def write_lvc(spec: LVCSpec, outdir: str) -> str:
 
 def make_problem(spec: LVCSpec | str, outdir: str):
     from .problem import load_problem
+    if spec == "c3j":
+        return make_c3_jittered(outdir)
     if isinstance(spec, str):
         spec = SPECS[spec]
     return load_problem(write_lvc(spec, outdir))
+
+
+def make_c3_jittered(outdir: str, eps_int: float = 1e-7, sd: float = 0.2):
+    """C3 as the workload states it: the Gaussian SPFs of modes 1-3 keep the
+    HO ground-state width (A = -1/2) and get centres q0 ~ N(0, sd) (for
+    A = -1/2, xi = -2 A q0 = q0), drawn from the generator seeded with
+    (303, 1) so the Hamiltonian draws of seed 303 are unchanged; one draw
+    per (state, mode, Gaussian).  The initial state (HO ground state on the
+    top state) is the least-squares projection of the ground-state Gaussian
+    on each mode's jittered Gaussians, A = product of the per-mode
+    coefficients on the top state, normalised.  Macaw's input has no way to
+    place Gaussians at random, so the state goes through a Macaw `restart`
+    file (output.f90:230-241) read by an INIT-WF `restart` block
+    (init_calc.f90:290-481), the route the reference itself offers."""
+    import dataclasses
+    from .problem import load_problem
+    from .restart import gaussian_overlap, trace_norm, write_restart
+    spec = dataclasses.replace(SPECS["c3"], name="c3j", eps_int=eps_int)
+    base_path = write_lvc(spec, outdir)
+    prob = load_problem(base_path)
+    rng = np.random.default_rng([spec.seed, 1])
+    ip, _ = prob.ipsiSPF()
+    npar = prob.npar()
+    psi = prob.psi0.copy()
+    top = spec.nstates - 1 if spec.init_state == 0 else spec.init_state - 1
+    off = prob.AStateOffset()
+    coef = []
+    for m in range(prob.nmodes):
+        if not prob.gwp_mode(m):
+            continue
+        for st in range(prob.nstates):
+            if st > 0 and not prob.MultiSet[m]:
+                continue
+            n = int(prob.nSPF[m, st])
+            nmd = int(npar[m])
+            start = int(ip[m, st]) - 1
+            xi = (rng.normal(0.0, sd, size=(nmd, n)) + 0j) * (-2.0 * prob.gwp_a[:nmd, :n, m])
+            psi[start:start + nmd * n] = xi.reshape(-1, order="F")
+            if st == top or (st == 0 and not prob.MultiSet[m]):
+                # projection of the ground-state Gaussian (xi = 0) on the set
+                xa = np.concatenate([xi, np.zeros((nmd, 1), complex)], axis=1)
+                S6 = _overlap_with_ground(prob, m, xa)
+                c = np.linalg.solve(S6[:n, :n], S6[:n, n])
+                coef.append((m, c))
+    A = np.zeros(prob.nAConftot, np.complex128)
+    dims = [int(prob.nSPF[m, top]) for m in range(prob.nmodes)]
+    a = np.ones(1, complex)
+    for m in range(prob.nmodes):
+        v = np.zeros(dims[m], complex)
+        cm = [c for mm, c in coef if mm == m]
+        if cm:
+            v[:] = cm[0]
+        else:
+            v[0] = 1.0     # grid modes: the ground state is the first SPF
+        a = np.kron(v, a)  # mode 1 fastest
+    A[off[top]: off[top] + a.size] = a
+    A /= np.sqrt(trace_norm(prob, A, psi))
+    rpath = os.path.join(outdir, "c3j.restart")
+    write_restart(rpath, prob, 0.0, A, psi)
+    text = open(base_path).read()
+    i, j = text.index("INIT-WF"), text.index("END-INIT-WF")
+    assoc = "\n".join(f"   {s + 1} <-- {s + 1}" for s in range(prob.nstates))
+    text = text[:i] + f"INIT-WF\n   restart\n   {os.path.basename(rpath)}\n   0\n{assoc}\n" + text[j:]
+    path = os.path.join(outdir, "c3j.inp")
+    with open(path, "w") as fh:
+        fh.write(text)
+    return load_problem(path)
+
+
+def _overlap_with_ground(prob, m, xi):
+    """Normalised overlaps of the Gaussians xi[:, :n] of mode m and one more
+    Gaussian (the last column) of the same width (the mode's first width)."""
+    from .restart import gaussian_overlap
+    nmd, n1 = xi.shape
+    a = prob.gwp_a
+    saved = a[:nmd, :n1, m].copy() if a.shape[1] >= n1 else None
+    if a.shape[1] < n1:
+        grown = np.zeros((a.shape[0], n1, a.shape[2]), order="F")
+        grown[:, :a.shape[1], :] = a
+        grown[:nmd, a.shape[1]:, m] = a[:nmd, :1, m]
+        tmp = prob.gwp_a
+        prob.gwp_a = grown
+        try:
+            return gaussian_overlap(prob, m, xi)
+        finally:
+            prob.gwp_a = tmp
+    prob.gwp_a[:nmd, n1 - 1, m] = prob.gwp_a[:nmd, 0, m]
+    try:
+        return gaussian_overlap(prob, m, xi)
+    finally:
+        prob.gwp_a[:nmd, :n1, m] = saved

@@ -5,6 +5,7 @@ normalisation, and the input-file route.  The resumed GPU propagation is in
 tests/test_gpu_restart.py.
 """
 import dataclasses
+import os
 import re
 import struct
 
@@ -210,3 +211,64 @@ def test_damaged_files_are_rejected(tmp_path):
         p.write_bytes(bad)
         with pytest.raises(R.RestartError):
             R.read_restart(str(p))
+
+
+def test_c3_jittered_workload(tmp_path):
+    """C3 as the workload states it: the generator's jittered Gaussians go
+    through a Macaw restart file and come back through INIT-WF restart
+    unchanged; the widths are the HO ground state's (A = -1/2), the centres
+    q0 = Re xi (for A = -1/2) scatter with the requested deviation 0.2, the
+    state is normalised and represents the HO ground state on the top state
+    to the accuracy of the jittered basis."""
+    from macaw_amd.lvc import make_problem, SPECS
+    from macaw_amd.restart import gaussian_overlap, read_restart, trace_norm
+    prob = make_problem("c3j", str(tmp_path))
+    d = read_restart(str(tmp_path / "c3j.restart"))
+    assert np.allclose(d["psi"], prob.psi0) and np.allclose(d["A"], prob.A0, atol=1e-15)
+    assert abs(trace_norm(prob, prob.A0, prob.psi0) - 1.0) < 1e-12
+    ip, _ = prob.ipsiSPF()
+    q0 = []
+    for m in range(3):
+        assert np.allclose(prob.gwp_a[0, :5, m], -0.5, atol=1e-6)   # width 0.707107 of the input
+        for s in range(prob.nstates):
+            xi = prob.psi0[int(ip[m, s]) - 1: int(ip[m, s]) - 1 + 5]
+            assert np.all(xi.imag == 0.0)
+            q0 += list(xi.real / (-2.0 * prob.gwp_a[0, :5, m]))
+    q0 = np.array(q0)
+    assert len(q0) == 45 and 0.13 < q0.std() < 0.27 and abs(q0.mean()) < 0.1
+    # only the top state is populated, as a product of per-mode expansions
+    off = prob.AStateOffset()
+    top = prob.nstates - 1
+    assert np.abs(prob.A0[: off[top]]).max() == 0.0
+    # fidelity with the HO ground state: per Gaussian mode |<g0|P g0>| with P
+    # the projector on the jittered Gaussians
+    for m in range(3):
+        xi = prob.psi0[int(ip[m, top]) - 1: int(ip[m, top]) - 1 + 5].reshape(1, 5)
+        xa = np.concatenate([xi, np.zeros((1, 1), complex)], axis=1)
+        from macaw_amd.lvc import _overlap_with_ground
+        S6 = _overlap_with_ground(prob, m, xa)
+        c = np.linalg.solve(S6[:5, :5], S6[:5, 5])
+        fid = (S6[5, :5] @ c).real
+        assert 1.0 - 1e-6 < fid <= 1.0 + 1e-9
+    assert SPECS["c3"].eps_C == 1e-8
+
+
+def test_c3_jittered_spf_error_floor(tmp_path):
+    """Why the jittered C3 cannot run at eps_int = 1e-7: the SPF error
+    estimate (DeltaPsi of the two half-step SPF sets, cmf2_base.f90:140)
+    stops falling with the step at its rounding floor of a few 1e-8, while
+    the A error falls as h^2; at h = 1e-3 fs it is already above 1e-7."""
+    import sys
+    sys.path.insert(0, os.path.join(os.path.dirname(os.path.abspath(__file__))))
+    from oracle_lib import Oracle
+    from macaw_amd.lvc import make_problem
+    prob = make_problem("c3j", str(tmp_path))
+    prob.eps_int = 1e6
+    errs = {}
+    for h_fs in (1e-3, 1e-5, 1e-6):
+        rc, hd, hn, atts = Oracle(prob).cmf2_step(0.0, h_fs / 0.02418884326505)
+        assert rc == 0
+        errs[h_fs] = (atts[0].err_A, atts[0].err_psi)
+    assert errs[1e-3][1] > 1e-7
+    assert 5e-9 < errs[1e-6][1] < 1e-7 and 5e-9 < errs[1e-5][1] < 1e-7
+    assert errs[1e-6][0] < 1e-12
