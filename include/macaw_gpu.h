@@ -234,12 +234,23 @@ int mg_set_graphs(mg_ctx *ctx, int64_t enable);
 
 #define MG_MLP_NFEAT 12
 #define MG_MLP_NPARAM 5081
+/* Largest batch of mg_mlp_forward_noalloc. */
+#define MG_MLP_NOALLOC_MAX_ROWS 4096
 int mg_mlp_train(mg_ctx *ctx, const float *X, const float *t, int64_t n, int64_t epochs, int64_t batch, double lr,
                  uint64_t seed, float *params, double *loss);
 int mg_mlp_forward(mg_ctx *ctx, const float *params, const float *X, int64_t n, float *y);
 int mg_mlp_grad(mg_ctx *ctx, const float *params, const float *X, const float *t, int64_t n, float *grad,
                 double *loss);
 int mg_set_step_model(mg_ctx *ctx, const float *params);
+/* mg_mlp_forward without any device allocation (no counterpart in the
+ * reference): for callers that cannot allocate device memory -- HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc.  Same
+ * arguments and results as mg_mlp_forward for 1 <= n <=
+ * MG_MLP_NOALLOC_MAX_ROWS (larger n is refused with an error); its device
+ * memory is a fixed 6.6 MB array held by every GPU the library runs on until
+ * the process ends.  All calls on one GPU share that array: calls that
+ * overlap on one GPU (several threads or contexts at once) race. */
+int mg_mlp_forward_noalloc(mg_ctx *ctx, const float *params, const float *X, int64_t n, float *y);
 int mg_log_step_features(mg_ctx *ctx, int64_t enable);
 int mg_step_features(mg_ctx *ctx, float *features, double *h, int64_t max, int64_t *n);
 int mg_controller_stats(mg_ctx *ctx, int64_t *proposals, int64_t *fallbacks);

@@ -1822,7 +1822,7 @@ void calc_rho(mg_ctx *c, const double2 *A64, bool exact = false) {
     } else if (c->nspfmax <= 8) {
         // one brick pass per mode group and state: all the group's Grams at once
         const void *Ain = gp ? A : (const void *)A64;
-        PlanKey key{(const void *)(gp ? 5 : 6), Ain, nullptr, nullptr};
+        PlanKey key{(const void *)(uintptr_t)(gp ? 5 : 6), Ain, nullptr, nullptr};
         auto it = c->plans.find(key);
         if (it == c->plans.end()) {
             const size_t esz = gp ? sizeof(float2) : sizeof(double2);
@@ -2069,7 +2069,7 @@ void calc_meanfield_bricks(mg_ctx *c, const double2 *A64, bool af_ready) {
         if (!af_ready || c->rho64) launch_convert(c->dAf.p, 1, A64, 0, c->nconf, c->stream);
         A = c->dAf.p;
     }
-    PlanKey key{(const void *)(c->mf64 ? 11 : 7), A, nullptr, nullptr};
+    PlanKey key{(const void *)(uintptr_t)(c->mf64 ? 11 : 7), A, nullptr, nullptr};
     auto it = c->plans.find(key);
     if (it == c->plans.end()) {
         it = c->plans.emplace(key, build_meanfield(c, A, c->mf64)).first;
@@ -2925,6 +2925,8 @@ void renormalise_interval(mg_ctx *c) {
     if (c->inorm_psi == 1) c->inorm_psi = 0;
 }
 
+float *noalloc_scratch();   // the no-allocation entry point's device array (below)
+
 template <class F>
 int guarded(mg_ctx *c, F &&f) {
     if (!c) return -1;
@@ -2978,6 +2980,7 @@ int mg_create(const mg_problem *prob, int64_t precision, mg_ctx **out) {
         if (ndev < 1) throw HipError{"no HIP device"};
         HIP_CHECK(hipStreamCreate(&c->stream));
         HIP_CHECK(hipStreamCreate(&c->stream2));
+        (void)noalloc_scratch();   // the device address of the no-allocation scratch, once per device
         HIP_CHECK(hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming));
         HIP_CHECK(hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming));
         HIP_CHECK(hipHostMalloc((void **)&c->spf_status_host, sizeof(int), hipHostMallocDefault));
@@ -3260,6 +3263,73 @@ int mg_mlp_forward(mg_ctx *c, const float *params, const float *X, int64_t n, fl
             HIP_CHECK(hipMemcpyAsync(y + o, d.y.p, sizeof(float) * b, hipMemcpyDeviceToHost, c->stream));
         }
         check_launch();
+        HIP_CHECK(hipStreamSynchronize(c->stream));
+    });
+}
+
+// ---------------------------------------------------------------------------
+// mg_mlp_forward_noalloc: mg_mlp_forward without any device allocation.  Its
+// device memory (parameters, BF16 weights, one batch of inputs, outputs and
+// activations) is the file-scope array `scratch`, MG_MLP_NOALLOC_MAX_ROWS
+// rows at most.  Host code never passes `scratch` itself to a kernel or a
+// copy (in host code the name is a stand-in, not a device address): the
+// device address comes from hipGetSymbolAddress, once per device, at the
+// first mg_create on that device, and is kept per device.  Every call on a
+// device shares that device's array, so calls that overlap on one GPU race.
+namespace {
+constexpr size_t kNaRows = MG_MLP_NOALLOC_MAX_ROWS;
+constexpr size_t kNaAlign = 64;   // floats (256 bytes)
+constexpr size_t na_round(size_t n) { return (n + kNaAlign - 1) / kNaAlign * kNaAlign; }
+constexpr size_t kNaParams = 0;
+constexpr size_t kNaWb = kNaParams + na_round(kMlpNParam);
+constexpr size_t kNaX = kNaWb + na_round((kMlpNWeights + 1) / 2);
+constexpr size_t kNaY = kNaX + na_round((size_t)kMlpNFeat * kNaRows);
+constexpr size_t kNaWork = kNaY + na_round(kNaRows);
+constexpr size_t kNaTotal = kNaWork + na_round((size_t)6 * kMlpHidden * kNaRows + 2 * kNaRows);
+}  // namespace
+
+static __device__ __align__(256) float scratch[kNaTotal];
+
+namespace {
+constexpr int kNaMaxDevices = 64;
+std::mutex na_mutex;
+float *na_ptr[kNaMaxDevices] = {nullptr};
+
+// Device address of `scratch` on the current device (looked up once per
+// device; the lookup is made by mg_create, so a context's device has it).
+float *noalloc_scratch() {
+    int dev = 0;
+    HIP_CHECK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kNaMaxDevices) throw HipError{"device index out of range for the no-allocation scratch"};
+    std::lock_guard<std::mutex> lock(na_mutex);
+    if (!na_ptr[dev]) {
+        void *p = nullptr;
+        HIP_CHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) throw HipError{"hipGetSymbolAddress returned no address for the no-allocation scratch"};
+        na_ptr[dev] = static_cast<float *>(p);
+    }
+    return na_ptr[dev];
+}
+}  // namespace
+
+int mg_mlp_forward_noalloc(mg_ctx *c, const float *params, const float *X, int64_t n, float *y) {
+    return guarded(c, [&] {
+        if (n < 1 || n > (int64_t)kNaRows)
+            throw HipError{"mg_mlp_forward_noalloc: n = " + std::to_string(n) + " rows; between 1 and " +
+                           std::to_string(kNaRows) + " rows (MG_MLP_NOALLOC_MAX_ROWS)"};
+        if (!params || !X || !y) throw HipError{"mg_mlp_forward_noalloc: null pointer"};
+        float *base = noalloc_scratch();
+        float *dp = base + kNaParams, *dx = base + kNaX, *dy = base + kNaY, *f = base + kNaWork;
+        uint16_t *dwb = reinterpret_cast<uint16_t *>(base + kNaWb);
+        const int B = (int)n;
+        const size_t hb = (size_t)kMlpHidden * B;
+        const MlpWork w{f, f + hb, f + 2 * hb, f + 3 * hb, f + 4 * hb, f + 5 * hb, f + 6 * hb, f + 6 * hb + B};
+        HIP_CHECK(hipMemcpyAsync(dp, params, sizeof(float) * kMlpNParam, hipMemcpyHostToDevice, c->stream));
+        launch_mlp_bf16(dp, dwb, c->stream);
+        HIP_CHECK(hipMemcpyAsync(dx, X, sizeof(float) * kMlpNFeat * B, hipMemcpyHostToDevice, c->stream));
+        launch_mlp_forward(dx, dp, dwb, w, dy, nullptr, B, c->stream);
+        check_launch();
+        HIP_CHECK(hipMemcpyAsync(y, dy, sizeof(float) * B, hipMemcpyDeviceToHost, c->stream));
         HIP_CHECK(hipStreamSynchronize(c->stream));
     });
 }

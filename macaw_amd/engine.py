@@ -67,6 +67,7 @@ def lib():
         L.mg_mlp_train.argtypes = [vp, f32p, f32p, C.c_int64, C.c_int64, C.c_int64, C.c_double, C.c_uint64, f32p,
                                    f64p]
         L.mg_mlp_forward.argtypes = [vp, f32p, f32p, C.c_int64, f32p]
+        L.mg_mlp_forward_noalloc.argtypes = [vp, f32p, f32p, C.c_int64, f32p]
         L.mg_mlp_grad.argtypes = [vp, f32p, f32p, f32p, C.c_int64, f32p, f64p]
         L.mg_set_step_model.argtypes = [vp, f32p]
         L.mg_log_step_features.argtypes = [vp, C.c_int64]
@@ -289,6 +290,15 @@ class Engine:
         params = np.ascontiguousarray(params, np.float32)
         y = np.zeros(X.shape[0], np.float32)
         self._check(lib().mg_mlp_forward(self.h, _f32(params), _f32(X), X.shape[0], _f32(y)))
+        return y
+
+    def mlp_forward_noalloc(self, params, X):
+        """mg_mlp_forward without device allocation (at most
+        MG_MLP_NOALLOC_MAX_ROWS = 4096 rows)."""
+        X = np.ascontiguousarray(X, np.float32)
+        params = np.ascontiguousarray(params, np.float32)
+        y = np.zeros(X.shape[0], np.float32)
+        self._check(lib().mg_mlp_forward_noalloc(self.h, _f32(params), _f32(X), X.shape[0], _f32(y)))
         return y
 
     def mlp_grad(self, params, X, t):

@@ -15,7 +15,7 @@ module macaw_gpu
    public :: mg_prepare_gwp, mg_gwp_overlaps, mg_der_spf_gwp, mg_a_to_b
    public :: mg_apply_h_stats, mg_apply_h_profile, mg_profile_copy
    public :: mg_set_step_model, mg_set_graphs, mg_log_step_features, mg_step_features, mg_controller_stats
-   public :: mg_mlp_train, mg_mlp_forward, mg_mlp_grad, mg_ensemble_propagate
+   public :: mg_mlp_train, mg_mlp_forward, mg_mlp_forward_noalloc, mg_mlp_grad, mg_ensemble_propagate
    public :: mg_check
 
    ! Problem descriptor (psidef.f90:11-55, dvr.f90:14-27, hamilton.f90:24-36)
@@ -273,6 +273,15 @@ module macaw_gpu
          real(c_double), intent(out) :: loss
       end function
       integer(c_int) function mg_mlp_forward(ctx, params, X, n, y) bind(C, name='mg_mlp_forward')
+         import :: c_int, c_int64_t, c_ptr, c_float
+         type(c_ptr), value :: ctx
+         real(c_float), intent(in) :: params(*), X(*)
+         integer(c_int64_t), value :: n
+         real(c_float), intent(out) :: y(*)
+      end function
+      ! mg_mlp_forward without device allocation, n <= 4096 rows (for callers
+      ! that cannot allocate device memory; calls overlapping on one GPU race)
+      integer(c_int) function mg_mlp_forward_noalloc(ctx, params, X, n, y) bind(C, name='mg_mlp_forward_noalloc')
          import :: c_int, c_int64_t, c_ptr, c_float
          type(c_ptr), value :: ctx
          real(c_float), intent(in) :: params(*), X(*)
