@@ -80,6 +80,7 @@ class InputDeck:
     int_type_psi: int = 0
     steps: bool = False
     normalize: int = 0
+    expect: list = field(default_factory=list)   # operator files of `expect = file`
     init_state: int = 1
     init_spf: dict = field(default_factory=dict)
     # INIT-WF restart: file, chosen state (0 = all), file state per state
@@ -147,6 +148,10 @@ def parse_input(path: str) -> InputDeck:
                 i += 1
         elif name == "hamiltonian":
             deck.hamiltonian = value
+        elif name == "expect":
+            # an operator whose expectation value is written at every output
+            # time (input.f90:196-215; output.f90:449-498)
+            deck.expect.append(value)
         elif name == "tinit":
             deck.tinit = _fortran_float(value) / AU2FS
         elif name == "tfinal":

@@ -47,6 +47,7 @@ class Problem:
     int_type_psi: int = 1
     int_type: int = 10            # 10 CMF2, 1 VMF Runge-Kutta (Cash-Karp), 2 VMF Dormand-Prince
     normalize: int = 0            # inorm_psi (globals.f90:23-26)
+    expect: list = field(default_factory=list)   # [(file, terms)] of `expect = file`
     A0: np.ndarray = None         # complex128[nAConftot]
     psi0: np.ndarray = None       # complex128[dimpsi]
     wgrid: np.ndarray = None
@@ -176,6 +177,9 @@ def build_problem(deck: InputDeck, operator_path: str | None = None) -> Problem:
                    min_stepsize=deck.min_stepsize, tinit=deck.tinit,
                    tfinal=deck.tfinal, int_type_psi=deck.int_type_psi or 1, int_type=deck.int_type,
                    wgrid=wgrid, eps_C=deck.eps_c, eps_ovl=deck.eps_ovl, normalize=deck.normalize)
+    for f in deck.expect:
+        epath = f if os.path.isabs(f) else os.path.join(deck.basedir, f)
+        prob.expect.append((f, parse_operator_file(epath, ns)))
     if dof_gwp.any():
         for m in range(nm):
             kinds = {int(dof_gwp[d - 1]) for d in iDOF[:nmodeDOF[m], m]}
@@ -190,6 +194,22 @@ def build_problem(deck: InputDeck, operator_path: str | None = None) -> Problem:
     else:
         prob.A0, prob.psi0 = initial_wavefunction(prob, deck)
     return prob
+
+
+def with_operator(prob: Problem, terms) -> Problem:
+    """The problem with its Hamiltonian replaced by the operator `terms` (the
+    same layout, state and tables): an expectation operator run through the
+    engine's H.A machinery."""
+    import dataclasses
+    nd = prob.ndof
+    ae = np.array([t.coeff for t in terms], dtype=np.complex128)
+    lhs = np.array([t.lhs for t in terms], dtype=np.int64)
+    rhs = np.array([t.rhs for t in terms], dtype=np.int64)
+    iop = np.zeros((nd, len(terms)), dtype=np.int64, order="F")
+    for j, t in enumerate(terms):
+        for kd, code in t.ops.items():
+            iop[kd - 1, j] = code
+    return dataclasses.replace(prob, ae=ae, lhs_el=lhs, rhs_el=rhs, iOpDof=iop, expect=[])
 
 
 def load_problem(input_path: str) -> Problem:

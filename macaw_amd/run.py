@@ -20,6 +20,9 @@ in Macaw's formats:
 * ``steps`` (``--steps``): header of propagator.f90:37-38, then per CMF2 step
   ``(f12.6,3(2x,f12.6))`` time, attempted, done and suggested step [fs]
   (propagator.f90:273-274);
+* ``expectation`` (``expect = file`` in the input): per output time
+  ``(1x,f8.2,2x,800(3x,f14.9,1x,f14.9))`` time [fs] and <Psi|O|Psi> of every
+  expectation operator (output.f90:449-498);
 * ``output``: the propagation interval and the ``psi`` period
   (propagator.f90:71-84), then per output time the block of
   output.f90:152-224 (time, norm, energy and its change from the start,
@@ -144,6 +147,45 @@ def output_block(prob, out, pops, energy_start, nat, ggp, perr):
     return lines
 
 
+def expectation_line(t_au: float, values) -> str:
+    """'(1x,f8.2,2x,800(3x, f14.9,1x,f14.9))' of output.f90:497."""
+    return f" {t_au * AU2FS:8.2f}  " + "".join(f"   {v.real:14.9f} {v.imag:14.9f}" for v in values)
+
+
+class Expectations:
+    """<Psi|O|Psi> of the input's expectation operators (output.f90:449-498):
+    one engine context per operator, holding the operator as its Hamiltonian
+    (same layout and tables).  O's matrix elements come from the current SPFs
+    (Calc_h1mode); with Gaussian modes the coefficients are taken to the
+    basis orthonormalised by D0 first (B = D^H S A, cmf2_base.f90:264-305),
+    where <Psi|O|Psi> = <B|D^H O D|B>."""
+
+    def __init__(self, prob):
+        from .engine import Engine
+        from .problem import with_operator
+        self.prob = prob
+        self.engines = [Engine(with_operator(prob, terms), precision=0) for _, terms in prob.expect]
+        self.gwp = any(prob.gwp_mode(m) for m in range(prob.nmodes))
+
+    def __call__(self, A, psi):
+        vals = []
+        for e in self.engines:
+            e.set_state(A, psi)
+            if self.gwp:
+                e.prepare_gwp()
+                _, D = e.gwp_overlaps()
+                B = e.a_to_b(A, D)
+            else:
+                e.h1mode()
+                B = A
+            vals.append(complex(np.vdot(B, e.apply_h(B))))
+        return vals
+
+    def close(self):
+        for e in self.engines:
+            e.close()
+
+
 STEPS_HEADER = ["#                Integration steps  [fs]", "#  time [fs]      attempted       done       suggested"]
 
 
@@ -199,10 +241,14 @@ def run(input_path: str, outdir: str = ".", precision: int = 0, controller: str 
     # one output interval per call (the step size carries over between
     # calls), so that rho of every output time is at hand for `output`
     psi_fh = open(os.path.join(outdir, "psi"), "wb") if psi_every > 0 else None
+    expect = Expectations(prob) if prob.expect else None
+    expect_lines = []
     k_out = 0
     while True:
         rho = e.rho()
         A_t, psi = e.get_state()
+        if expect is not None:
+            expect_lines.append(expectation_line(t, expect(A_t, psi)))
         if psi_fh is not None and k_out % psi_every == 0:
             write_psi_record(psi_fh, t, A_t, psi)
         k_out += 1
@@ -236,6 +282,10 @@ def run(input_path: str, outdir: str = ".", precision: int = 0, controller: str 
     with open(os.path.join(outdir, "output"), "w") as fh:
         fh.write("\n".join(block) + "\n")
     write_auto(os.path.join(outdir, "auto"), outs)
+    if expect is not None:
+        expect.close()
+        with open(os.path.join(outdir, "expectation"), "w") as fh:
+            fh.write("\n".join(expect_lines) + "\n")
     if prob.nstates > 1:
         write_spop(os.path.join(outdir, "spop"), outs, pops)
     if steps:

@@ -227,3 +227,93 @@ def test_standalone_vmf_input_matches_single_call(method, name, tmp_path):
     for a, b in zip(outs, ref):
         assert abs(a.time - b.time) < 1e-9
         assert abs(complex(a.auto_re, a.auto_im) - complex(b.auto_re, b.auto_im)) < 1e-12
+
+
+FORTRAN_EXPECT = """
+program expfmt
+  implicit none
+  integer :: i, n, k
+  double precision :: t
+  double complex :: e(3)
+  read(*,*) n
+  do i = 1, n
+     read(*,*) t, (e(k), k = 1, 3)
+     write(*,'(1x,f8.2,2x,800(3x, f14.9,1x,f14.9))') t, e
+  end do
+end program expfmt
+"""
+
+
+@pytest.mark.skipif(not os.path.exists(FLANG), reason="flang not available")
+def test_expectation_line_matches_fortran_edit_descriptor(tmp_path):
+    """The `expectation` file line of output.f90:497 (three operators)."""
+    src = tmp_path / "expfmt.f90"
+    src.write_text(FORTRAN_EXPECT)
+    exe = tmp_path / "expfmt"
+    subprocess.check_call([FLANG, "-O0", "-o", str(exe), str(src)], cwd=str(tmp_path))
+    rng = np.random.default_rng(11)
+    rows = [(0.0, [1.0 + 0j, -0.5 + 1e-12j, 0j])]
+    for _ in range(15):
+        rows.append((float(rng.uniform(0, 300)), [complex(*rng.normal(scale=3.0, size=2)) for _ in range(3)]))
+    text = f"{len(rows)}\n" + "".join(
+        f"{t!r} " + " ".join(f"({v.real!r},{v.imag!r})" for v in vals) + "\n" for t, vals in rows)
+    out = subprocess.run([str(exe)], input=text, capture_output=True, text=True, check=True).stdout.splitlines()
+    for k, (t, vals) in enumerate(rows):
+        assert RUN.expectation_line(t / AU2FS, vals) == out[k], (k, RUN.expectation_line(t / AU2FS, vals), out[k])
+
+
+def test_expect_keyword_reads_the_operator_files(tmp_path):
+    from macaw_amd.lvc import make_problem
+    from macaw_amd.problem import load_problem
+    make_problem("t1", str(tmp_path))
+    (tmp_path / "q1.op").write_text("# <q1^2>\n 1.0  |  1  |1  q^2\n")
+    inp = (tmp_path / "t1.inp").read_text()
+    inp = inp.replace("Hamiltonian = ", "expect = q1.op\nexpect = t1.op\nHamiltonian = ")
+    (tmp_path / "e.inp").write_text(inp)
+    prob = load_problem(str(tmp_path / "e.inp"))
+    assert [f for f, _ in prob.expect] == ["q1.op", "t1.op"]
+    q1 = prob.expect[0][1]
+    assert len(q1) == 2 and all(t.ops == {1: 2} for t in q1)   # "1" = every diagonal state
+    assert len(prob.expect[1][1]) == prob.nHam
+
+
+@pytest.mark.gpu
+def test_expectation_values_match_oracle(tmp_path):
+    """<Psi|O|Psi> of the standalone run's expectation operators against the
+    oracle's H.A with the operator (grid modes, and Gaussian modes through
+    B = D^H S A); the Hamiltonian as an expectation operator gives the
+    energy of the output record (energyMF / norm)."""
+    import sys
+    sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+    from oracle_lib import Oracle
+    from macaw_amd.engine import Engine
+    from macaw_amd.lvc import make_problem
+    from macaw_amd.problem import load_problem, with_operator
+    for name in ("t2", "g1"):
+        d = tmp_path / name
+        d.mkdir()
+        make_problem(name, str(d))
+        (d / "q.op").write_text("# <q1 q2> + <q1^2>\n 1.0 | 1 |1 q |2 q\n 0.5 | 1 |1 q^2\n")
+        inp = (d / f"{name}.inp").read_text().replace("Hamiltonian = ", f"expect = q.op\nexpect = {name}.op\nHamiltonian = ")
+        (d / "e.inp").write_text(inp)
+        prob = load_problem(str(d / "e.inp"))
+        e = Engine(prob)
+        e.propagate(0.0, 1.0 / AU2FS)
+        A, psi = e.get_state()
+        out, _ = e.observables(1.0 / AU2FS)
+        vals = RUN.Expectations(prob)(A, psi)
+        for (f, terms), v in zip(prob.expect, vals):
+            o = Oracle(with_operator(prob, terms))
+            o.set_state(A, psi)
+            if name.startswith("g"):
+                o.prepare_gwp()
+                _, D = o.gwp_overlaps()
+                B = o.a_to_b(A, D)
+            else:
+                o.h1mode()
+                B = A
+            ref = complex(np.vdot(B, o.apply_h(B)))
+            assert abs(v - ref) < 1e-10 * max(1.0, abs(ref)), (name, f, v, ref)
+            assert abs(v.imag) < 1e-10 * max(1.0, abs(v))
+        assert abs(vals[1].real / out.norm - out.energy) < 1e-9 * abs(out.energy), (vals[1], out.energy, out.norm)
+        e.close()
