@@ -60,6 +60,16 @@ typedef struct mg_problem {
     const int64_t *dof_gwp;
     const double *gwp_a;
     double eps_C, eps_ovl;
+    /* Parametrised one-DOF operators (hamilton.f90:212-300; input.f90:950-
+     * 1090): operator code 200 + p multiplies by gridoper(:, p, k) on the
+     * grid of DVR DOF k, code -200 - p applies the projector phi phi^T with
+     * phi = gridoper(:, p, k) (MultTensDiag / MultTensProj,
+     * hamilton.f90:598-661).  gridoper[ngpmax, nopparam, ndof] is DOF-indexed
+     * (Macaw's gridOper is indexed by DVR-dof: pass gridOper(:, p, indDVR(k))
+     * for DOF k); nopparam = 0 and NULL when there is none.  Not supported on
+     * Gaussian DOFs. */
+    int64_t nopparam;
+    const double *gridoper;
 } mg_problem;
 
 /* Observables written at every output time (output.f90:35-87,143-150). */

@@ -23,6 +23,7 @@ class MgProblem(C.Structure):
         ("writetime", C.c_double), ("min_stepsize", C.c_double),
         ("int_type_psi", C.c_int64),
         ("dof_gwp", i64p), ("gwp_a", f64p), ("eps_C", C.c_double), ("eps_ovl", C.c_double),
+        ("nopparam", C.c_int64), ("gridoper", f64p),
     ]
 
 
@@ -65,6 +66,13 @@ def make_mg_problem(prob):
         dgp, gap = cptr(dg, i64p), cptr(ga)
     else:
         dgp, gap = None, None
+    gop = getattr(prob, "gridoper", None)
+    if gop is not None and gop.size:
+        go = _f64(gop)
+        keep += [go]
+        nop, gop_p = int(gop.shape[1]), cptr(go)
+    else:
+        nop, gop_p = 0, None
     p = MgProblem(
         nstates=prob.nstates, nmodes=prob.nmodes, ndof=prob.ndof, maxdof=prob.maxDOF,
         nmodeDOF=cptr(nmd, i64p), iDOF=cptr(idof, i64p), MultiSet=cptr(ms, i64p),
@@ -76,7 +84,8 @@ def make_mg_problem(prob):
         writetime=prob.writetime, min_stepsize=prob.min_stepsize,
         int_type_psi=prob.int_type_psi,
         dof_gwp=dgp, gwp_a=gap,
-        eps_C=getattr(prob, "eps_C", 1e-8), eps_ovl=getattr(prob, "eps_ovl", 1e-8))
+        eps_C=getattr(prob, "eps_C", 1e-8), eps_ovl=getattr(prob, "eps_ovl", 1e-8),
+        nopparam=nop, gridoper=gop_p)
     return p, keep
 
 

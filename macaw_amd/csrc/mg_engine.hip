@@ -127,6 +127,8 @@ struct mg_ctx {
     int ns = 0, nm = 0, ndof = 0, nham = 0, maxdof = 0, ngpmax = 0;
     std::vector<int64_t> nmodeDOF, iDOF, MultiSet, nSPF, ngp, lhs, rhs, iOpDof;
     std::vector<double> grid, d1, d2, xd1;
+    int nopparam = 0;
+    std::vector<double> gridoper;  // [ngpmax, nopparam, ndof] parametrised operators on the grids
     std::vector<cplx> ae;
     double eps_int = 1e-8, eps_psi = 1e-8, eps_rho = 1e-8, writetime = 1.0, min_stepsize = 0.0;
     int int_type_psi = 1;
@@ -182,7 +184,7 @@ struct mg_ctx {
     DevBuf dkron;                  // 2 * nconf-sized temps for overlap chains
     DevBuf dspfscr, dpartials, dscal;
     DevBuf dhs_rowptr, dhs_src, dhs_coef, dmf1_rowptr, dmf1_src, dmf1_coef, drmf_jobs;
-    DevBuf dgrid, dd1, dd2, dxd1, dngp, dmode_dofs, dmode_ndof, dspfmodes, dstatus;
+    DevBuf dgrid, dd1, dd2, dxd1, dngp, dmode_dofs, dmode_ndof, dspfmodes, dstatus, dgridoper;
     // mixed form only: FP64 first Krylov vector, FP32 copy of the A-vector,
     // FP32 in/out of the C-ABI H.A, FP32 copies of h1mode and the diagonal sums
     DevBuf dAf, dXf, dYf, dh1f, dhsumf;
@@ -356,6 +358,8 @@ void check_descriptor(const mg_problem *p) {
     need(p->nmodeDOF && p->iDOF && p->MultiSet && p->nSPF && p->ngp && p->grid && p->d1 && p->d2 && p->xd1 &&
              p->ae && p->lhs_el && p->rhs_el && p->iOpDof,
          "a table pointer is null");
+    need(p->nopparam >= 0 && p->nopparam <= 10000 && (p->nopparam == 0 || p->gridoper),
+         "nopparam must be 0..10000, with gridoper when it is not 0");
     need(p->int_type_psi == 1 || p->int_type_psi == 2,
          "int_type_psi must be 1 (Cash-Karp) or 2 (Dormand-Prince); integrator not yet implemented for the SPFs");
     need(p->eps_int > 0 && p->eps_psi > 0 && p->eps_rho > 0 && p->writetime > 0 && p->min_stepsize >= 0,
@@ -418,8 +422,14 @@ void setup(mg_ctx *c, const mg_problem *p) {
         if (c->nmodeDOF[m] > kMaxDofPerMode) throw HipError{"too many DOFs in a logical mode"};
     }
     if (ns > 8) throw HipError{"at most 8 electronic states are supported"};
+    // primitive operator codes: -3..20, and the parametrised operators 200 + p
+    // (diagonal) and -200 - p (projector) of the descriptor's table
+    c->nopparam = (int)p->nopparam;
+    if (c->nopparam > 0) c->gridoper.assign(p->gridoper, p->gridoper + gm * (size_t)c->nopparam * nd);
     for (int64_t v : c->iOpDof)
-        if (v < -3 || v > 20) throw HipError{"primitive operator code not supported by the GPU engine"};
+        if (!((v >= -3 && v <= 20) || (v >= 201 && v <= 200 + c->nopparam) ||
+              (v <= -201 && v >= -200 - c->nopparam)))
+            throw HipError{"primitive operator code " + std::to_string(v) + " not supported by the GPU engine"};
     // layout
     c->nspfmax = 0;
     for (int64_t v : c->nSPF) c->nspfmax = std::max<int>(c->nspfmax, (int)v);
@@ -464,6 +474,13 @@ void setup(mg_ctx *c, const mg_problem *p) {
             c->gwp_mask |= 1u << m;
         }
     }
+    for (int m = 0; m < nm; m++)
+        for (int k = 0; k < c->nmodeDOF[m] && c->gwp[m]; k++)
+            for (int t = 0; t < c->nham; t++) {
+                const int64_t v = c->iOpDof[(c->iDOF[k + (size_t)c->maxdof * m] - 1) + (size_t)nd * t];
+                if (v > 200 || v <= -200)
+                    throw HipError{"parametrised operator on a Gaussian DOF: not supported by the GPU engine"};
+            }
     c->eps_C = p->eps_C > 0 ? p->eps_C : 1e-8;
     c->eps_ovl = p->eps_ovl > 0 ? p->eps_ovl : 1e-8;
     if (c->ngwp) {
@@ -750,6 +767,9 @@ void allocate(mg_ctx *c) {
     hcopy(c->dd1.p, c->d1.data(), c->dd1.bytes, hipMemcpyHostToDevice);
     hcopy(c->dd2.p, c->d2.data(), c->dd2.bytes, hipMemcpyHostToDevice);
     hcopy(c->dxd1.p, c->xd1.data(), c->dxd1.bytes, hipMemcpyHostToDevice);
+    c->dgridoper.alloc(sizeof(double) * std::max<size_t>(1, c->gridoper.size()));
+    if (!c->gridoper.empty())
+        hcopy(c->dgridoper.p, c->gridoper.data(), sizeof(double) * c->gridoper.size(), hipMemcpyHostToDevice);
     std::vector<int> ngp(c->ndof), mdofs((size_t)c->nm * kMaxDofPerMode, 0), mnd(c->nm);
     for (int k = 0; k < c->ndof; k++) ngp[k] = (int)c->ngp[k];
     for (int m = 0; m < c->nm; m++) {
@@ -894,6 +914,8 @@ DofTables dof_tables(const mg_ctx *c) {
     t.ngp = c->dngp.as<int>();
     t.mode_dofs = c->dmode_dofs.as<int>();
     t.mode_ndof = c->dmode_ndof.as<int>();
+    t.oper = c->dgridoper.as<double>();
+    t.nparam = c->nopparam;
     return t;
 }
 

@@ -151,6 +151,8 @@ struct DofTables {
     const int *ngp;              // [ndof]
     const int *mode_dofs;        // [nmodes, kMaxDofPerMode] 0-based DOF ids
     const int *mode_ndof;        // [nmodes]
+    const double *oper;          // [ngpmax, nparam, ndof] parametrised operators on the grids
+    int nparam;
 };
 
 // SPF propagation of one logical mode over dt (propagate_SPF + rk_SPF_ck/dp).

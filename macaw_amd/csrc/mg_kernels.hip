@@ -810,6 +810,34 @@ __device__ void apply_prim_block(const DofTables &dt, int mode, int k, int code,
     const int ng = dt.ngp[dof];
     const int total = nG * nS;
     const int tid = threadIdx.x, nth = blockDim.x;
+    if (code > 200) {
+        // parametrised diagonal operator: its values on this DOF's grid
+        // (MultTensDiag with gridOper, hamilton.f90:467-468)
+        const double *f = dt.oper + (int64_t)dt.ngpmax * ((code - 201) + (int64_t)dt.nparam * dof);
+        for (int e = tid; e < total; e += nth) blk[e] = cscale(f[(e / ll) % ng], blk[e]);
+        __syncthreads();
+        return;
+    }
+    if (code <= -200) {
+        // projector phi phi^T along this DOF (MultTensProj, hamilton.f90:632-661)
+        const double *phi = dt.oper + (int64_t)dt.ngpmax * ((-code - 201) + (int64_t)dt.nparam * dof);
+        for (int e = tid; e < total; e += nth) {
+            const int l = e % ll;
+            const int g2 = (e / ll) % ng;
+            const int rest = e / (ll * ng);
+            const double2 *c = blk + (int64_t)rest * ng * ll + l;
+            double2 acc = zero2();
+            for (int g = 0; g < ng; g++) {
+                acc.x = fma(phi[g], c[g * ll].x, acc.x);
+                acc.y = fma(phi[g], c[g * ll].y, acc.y);
+            }
+            tmp[e] = cscale(phi[g2], acc);
+        }
+        __syncthreads();
+        for (int e = tid; e < total; e += nth) blk[e] = tmp[e];
+        __syncthreads();
+        return;
+    }
     if (code >= 1) {
         const double *x = dt.grid + (int64_t)dt.ngpmax * dof;
         for (int e = tid; e < total; e += nth) {

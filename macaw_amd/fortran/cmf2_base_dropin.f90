@@ -26,8 +26,8 @@ module cmf2_base
    use iso_c_binding
    use macaw_gpu
    use psidef
-   use dvr, only : dvrType, ngp, ngpMax, grid, d1DVR, d2DVR, xd1DVR
-   use hamilton, only : nHam, ae, lhs_el, rhs_el, iOpDof
+   use dvr, only : dvrType, ngp, ngpMax, grid, d1DVR, d2DVR, xd1DVR, gridOper
+   use hamilton, only : nHam, ae, lhs_el, rhs_el, iOpDof, nOpParam
    use globals
    implicit none
    private
@@ -36,7 +36,8 @@ module cmf2_base
    type(mg_problem), save :: prob
    integer(c_int64_t), allocatable, target, save :: nmd_t(:), idof_t(:,:), ms_t(:), nspf_t(:,:), ngp_t(:), &
                                                     lhs_t(:), rhs_t(:), iop_t(:,:), gwp_t(:)
-   real(c_double), allocatable, target, save :: grid_t(:,:), d1_t(:,:,:), d2_t(:,:,:), xd1_t(:,:,:), gwpa_t(:,:,:)
+   real(c_double), allocatable, target, save :: grid_t(:,:), d1_t(:,:,:), d2_t(:,:,:), xd1_t(:,:,:), gwpa_t(:,:,:), &
+                                                gop_t(:,:,:)
    complex(c_double_complex), allocatable, target, save :: ae_t(:)
    integer(c_int64_t), save :: ngmax_t = 0
 
@@ -79,6 +80,12 @@ contains
       prob%eps_ovl = eps_ovl
       prob%dof_gwp = c_null_ptr
       prob%gwp_a = c_null_ptr
+      prob%nopparam = 0
+      prob%gridoper = c_null_ptr
+      if (nOpParam > 0) then
+         prob%nopparam = nOpParam
+         prob%gridoper = c_loc(gop_t)
+      end if
       if (nmodGWP > 0) then
          allocate(gwp_t(ndof), gwpa_t(size(iDOF, 1), nSPFmax, nmodes))
          gwp_t = merge(1_c_int64_t, 0_c_int64_t, dvrType == 'gwp')
@@ -108,6 +115,7 @@ contains
                                        grid_t, d1_t, d2_t, xd1_t, ae_t)
       if (allocated(gwp_t)) deallocate(gwp_t)
       if (allocated(gwpa_t)) deallocate(gwpa_t)
+      if (allocated(gop_t)) deallocate(gop_t)
       allocate(nmd_t(size(nmodeDOF)), idof_t(size(iDOF, 1), size(iDOF, 2)), ms_t(size(MultiSet)), &
                nspf_t(size(nSPF, 1), size(nSPF, 2)), ngp_t(size(ngp)))
       nmd_t = nmodeDOF
@@ -137,6 +145,17 @@ contains
          d2_t(1:n, 1:n, k) = d2DVR(1:n, 1:n, indDVR(k))
          xd1_t(1:n, 1:n, k) = xd1DVR(1:n, 1:n, indDVR(k))
       end do
+      ! parametrised operators' grid values, DVR-dof indexed in Macaw
+      ! (hamilton.f90:212), DOF-indexed in the descriptor
+      if (nOpParam > 0) then
+         allocate(gop_t(ngmax_t, nOpParam, ndof))
+         gop_t = 0.0_c_double
+         do k = 1, ndof
+            if (dvrType(k) == 'gwp') cycle
+            n = ngp(k)
+            gop_t(1:n, 1:nOpParam, k) = gridOper(1:n, 1:nOpParam, indDVR(k))
+         end do
+      end if
    end subroutine copy_tables
 
    ! Same signature as cmf2_base.f90:47.  The state is uploaded, stepped on

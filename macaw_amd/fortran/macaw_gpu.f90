@@ -30,6 +30,9 @@ module macaw_gpu
       integer(c_int64_t) :: int_type_psi
       type(c_ptr) :: dof_gwp = c_null_ptr, gwp_a = c_null_ptr
       real(c_double) :: eps_C = 1.d-8, eps_ovl = 1.d-8
+      ! parametrised operators: gridoper(ngpmax, nopparam, ndof), DOF-indexed
+      integer(c_int64_t) :: nopparam = 0
+      type(c_ptr) :: gridoper = c_null_ptr
    end type mg_problem
 
    type, bind(C) :: mg_output

@@ -31,6 +31,84 @@ class InputError(ValueError):
     """Raised for input the reference rejects with ``stop``."""
 
 
+# Parametrised one-DOF operators (input.f90:950-1090; OpParamFunc of
+# hamilton.f90:212-300): 0 Gaussian exp(p1 x + p2 x^2), 1 interpolation of a
+# (x, y) table, 2 indicator of [p1 - p2/2, p1 + p2/2], 3 Hermite function
+# x^r exp(p1 x + p2 x^2), 4 sin(p1 x)^p2, 5 cos(p1 x)^p2, 6 projector on
+# eigenstate r of -p2 d^2/dx^2 + V(x) with V interpolated from a file.
+# Operator codes: 200 + p (diagonal functions) and -200 - p (projector),
+# p = 1-based index in the table of the problem.
+@dataclass
+class OpParam:
+    func: int
+    p1: float = 0.0
+    p2: float = 0.0
+    p3: float = 0.0
+    file: str = ""
+
+
+class OpParams:
+    """The parametrised operators of a problem's operator files, in reading
+    order (the Hamiltonian's, then the expectation operators'); an
+    interpolation file read twice is one entry (input.f90:969-979)."""
+
+    def __init__(self):
+        self.items = []
+
+    def add(self, op: OpParam) -> int:
+        if op.func == 1:
+            for i, o in enumerate(self.items):
+                if o.func == 1 and o.file == op.file:
+                    return i + 1
+        self.items.append(op)
+        return len(self.items)
+
+
+def _param_operator(name: str, registry: OpParams, basedir: str):
+    """Code of a parametrised operator name, or None (input.f90:950-1090)."""
+    def after(key):
+        return name[name.index(key) + len(key):].strip()
+
+    def nums(text, n, default=None):
+        toks = text.replace(",", " ").split()
+        if not toks and default is not None:
+            return default
+        if len(toks) < n:
+            raise InputError(f"Operator {name}: expected {n} parameters")
+        return [_fortran_float(t) for t in toks[:n]]
+
+    def fname(text):
+        t = text.strip().split()[0] if text.strip() else ""
+        t = t.strip("'\"")
+        return t if os.path.isabs(t) else os.path.join(basedir, t)
+
+    if "gau" in name:
+        p1, p2 = nums(after("gau"), 2)
+        return 200 + registry.add(OpParam(0, p1, p2))
+    if "interp" in name:
+        return 200 + registry.add(OpParam(1, file=fname(after("interp"))))
+    if "square" in name:
+        p1, p2 = nums(after("square"), 2)
+        return 200 + registry.add(OpParam(2, p1, p2))
+    if "hermite" in name:
+        r, p1, p2 = nums(after("hermite"), 3)
+        return 200 + registry.add(OpParam(3, p1, p2, float(int(r))))
+    if "sin^-2" in name:
+        raise InputError("ERROR: sin^-2 operator allowed only for Leg-DVR")
+    if "sin" in name:
+        p1, p2 = nums(after("sin"), 2, default=[1.0, 1.0])
+        return 200 + registry.add(OpParam(4, p1, p2))
+    if "cos" in name:
+        p1, p2 = nums(after("cos"), 2, default=[1.0, 1.0])
+        return 200 + registry.add(OpParam(5, p1, p2))
+    if "eig_proj" in name:
+        rest = after("eig_proj").split()
+        if len(rest) < 3:
+            raise InputError(f"Operator {name}: expected the potential file, the kinetic prefactor and the level")
+        return -200 - registry.add(OpParam(6, 0.0, _fortran_float(rest[1]), float(int(rest[2])), fname(rest[0])))
+    return None
+
+
 @dataclass
 class SPFDecl:
     multiset: bool
@@ -40,7 +118,7 @@ class SPFDecl:
 
 @dataclass
 class GridDecl:
-    kind: str           # 'HO', or 'gwp' (Gaussian SPFs; npts/x0/mw = plot grid)
+    kind: str           # 'HO' (x0, m w), 'sin' (x0 = first, mw = last point), or 'gwp' (plot grid)
     npts: int
     x0: float
     mw: float
@@ -217,7 +295,12 @@ def _parse_spf(line: str, nstates: int) -> SPFDecl:
 
 
 def _parse_grid(line: str):
-    """One line of the GRID block (source/dvr.f90:32-103, HO and gwp branches)."""
+    """One line of the GRID block (source/dvr.f90:32-103, sin, HO and gwp branches)."""
+    j = line.find("sin")
+    if j >= 0:
+        k = int(line[:j])
+        toks = line[j + 3:].split()
+        return k, GridDecl("sin", int(toks[0]), _fortran_float(toks[1]), _fortran_float(toks[2]))
     j = line.find("gwp")
     if j >= 0:
         k = int(line[:j])
@@ -227,7 +310,7 @@ def _parse_grid(line: str):
         return k, GridDecl("gwp", int(toks[0]), _fortran_float(toks[1]), _fortran_float(toks[2]))
     j = line.find("HO")
     if j < 0:
-        raise InputError(f"only HO and gwp grids are supported by this engine: {line.strip()}")
+        raise InputError(f"only sin, HO and gwp grids are supported by this engine: {line.strip()}")
     k = int(line[:j])
     toks = line[j + 2:].split()
     return k, GridDecl("HO", int(toks[0]), _fortran_float(toks[1]), _fortran_float(toks[2]))
@@ -290,13 +373,16 @@ class HamTerm:
     ops: dict           # dof -> primitive operator code (identity omitted)
 
 
-def parse_operator_file(path: str, nstates: int) -> list:
+def parse_operator_file(path: str, nstates: int, registry: OpParams | None = None) -> list:
     """Hamiltonian terms of an operator file (source/input.f90:752-1144).
 
     ``1`` and a bare coefficient give one term per electronic state, ``Sa&b``
     with a != b gives both directions with the same coefficient, ``Za&b`` one
-    direction.
+    direction.  Parametrised operators are entered in `registry` (a new one
+    when None) and get the codes 200 + p / -200 - p.
     """
+    registry = registry if registry is not None else OpParams()
+    basedir = os.path.dirname(os.path.abspath(path))
     if not os.path.exists(path):
         # source/input.f90:529-536
         raise InputError(f"Input error: Operator file, {path} not found")
@@ -333,11 +419,17 @@ def parse_operator_file(path: str, nstates: int) -> list:
                 f = f.rstrip()
                 kdof = int(f[:2]) if f[:2].strip() else int(f.split()[0])
                 name = f[2:].strip()
+                if kdof == 0:
+                    # time-dependent coefficient (input.f90:924-947)
+                    raise InputError("time-dependent coefficients (dof 0 'gau') are not supported by this engine")
                 if name == "1":
                     continue
-                if name not in OP_CODES:
-                    raise InputError(f"Operator {name} not implemented")
-                ops[kdof] = OP_CODES[name]
+                code = _param_operator(name, registry, basedir)
+                if code is None:
+                    if name not in OP_CODES:
+                        raise InputError(f"Operator {name} not implemented")
+                    code = OP_CODES[name]
+                ops[kdof] = code
             for a, b in pairs:
                 terms.append(HamTerm(complex(coeff), a, b, dict(ops)))
     return terms

@@ -15,8 +15,8 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
-from .dvr import harm_dvr
-from .inputfile import InputDeck, InputError, parse_input, parse_operator_file
+from .dvr import harm_dvr, param_operator_on_grid, sin_dvr
+from .inputfile import InputDeck, InputError, OpParams, parse_input, parse_operator_file
 
 
 @dataclass
@@ -47,6 +47,10 @@ class Problem:
     int_type_psi: int = 1
     int_type: int = 10            # 10 CMF2, 1 VMF Runge-Kutta (Cash-Karp), 2 VMF Dormand-Prince
     normalize: int = 0            # inorm_psi (globals.f90:23-26)
+    # parametrised operators (codes 200 + p, -200 - p): the table of their
+    # definitions and their values on every DVR DOF's grid
+    op_params: list = field(default_factory=list)
+    gridoper: np.ndarray = None   # float64[ngpMax, nOpParam, ndof] (Fortran order)
     expect: list = field(default_factory=list)   # [(file, terms)] of `expect = file`
     A0: np.ndarray = None         # complex128[nAConftot]
     psi0: np.ndarray = None       # complex128[dimpsi]
@@ -154,7 +158,10 @@ def build_problem(deck: InputDeck, operator_path: str | None = None) -> Problem:
             dof_gwp[k] = 1
             grid[:g.npts, k] = np.linspace(g.x0, g.mw, g.npts)
             continue
-        x, w, a1, a2, ax, _ = harm_dvr(g.npts, g.x0, g.mw)
+        if g.kind == "sin":
+            x, w, a1, a2, ax, _ = sin_dvr(g.npts, g.x0, g.mw)
+        else:
+            x, w, a1, a2, ax, _ = harm_dvr(g.npts, g.x0, g.mw)
         n = g.npts
         grid[:n, k] = x
         wgrid[:n, k] = w
@@ -162,7 +169,8 @@ def build_problem(deck: InputDeck, operator_path: str | None = None) -> Problem:
         d2[:n, :n, k] = a2
         xd1[:n, :n, k] = ax
     path = operator_path or os.path.join(deck.basedir, deck.hamiltonian)
-    terms = parse_operator_file(path, ns)
+    registry = OpParams()
+    terms = parse_operator_file(_find_file(path), ns, registry)
     nham = len(terms)
     ae = np.array([t.coeff for t in terms], dtype=np.complex128)
     lhs = np.array([t.lhs for t in terms], dtype=np.int64)
@@ -179,7 +187,24 @@ def build_problem(deck: InputDeck, operator_path: str | None = None) -> Problem:
                    wgrid=wgrid, eps_C=deck.eps_c, eps_ovl=deck.eps_ovl, normalize=deck.normalize)
     for f in deck.expect:
         epath = f if os.path.isabs(f) else os.path.join(deck.basedir, f)
-        prob.expect.append((f, parse_operator_file(epath, ns)))
+        prob.expect.append((f, parse_operator_file(_find_file(epath), ns, registry)))
+    prob.op_params = list(registry.items)
+    if prob.op_params:
+        # values on every DVR DOF's grid (hamilton.f90:212-300); a
+        # parametrised operator on a Gaussian DOF is not supported here
+        npar = len(prob.op_params)
+        prob.gridoper = np.zeros((gmax, npar, nd), order="F")
+        for k in range(nd):
+            if dof_gwp[k]:
+                continue
+            n = int(ngp[k])
+            for p_, op in enumerate(prob.op_params):
+                prob.gridoper[:n, p_, k] = param_operator_on_grid(op, grid[:n, k], d2[:n, :n, k])
+        for _, terms_ in [("", terms)] + prob.expect:
+            for t in terms_:
+                for kd, c in t.ops.items():
+                    if (c > 200 or c <= -200) and dof_gwp[kd - 1]:
+                        raise InputError(f"parametrised operator on the Gaussian DOF {kd}: not supported by this engine")
     if dof_gwp.any():
         for m in range(nm):
             kinds = {int(dof_gwp[d - 1]) for d in iDOF[:nmodeDOF[m], m]}
@@ -194,6 +219,21 @@ def build_problem(deck: InputDeck, operator_path: str | None = None) -> Problem:
     else:
         prob.A0, prob.psi0 = initial_wavefunction(prob, deck)
     return prob
+
+
+def _find_file(path: str) -> str:
+    """The file, or when it does not exist the single file of its directory
+    whose name matches it without regard to case (Macaw's own example names
+    `Ham_interp.op` for `ham_interp.op`, written on a case-insensitive file
+    system); the missing-file error otherwise."""
+    if os.path.exists(path):
+        return path
+    d, b = os.path.split(path)
+    try:
+        hits = [f for f in os.listdir(d or ".") if f.lower() == b.lower()]
+    except OSError:
+        hits = []
+    return os.path.join(d, hits[0]) if len(hits) == 1 else path
 
 
 def with_operator(prob: Problem, terms) -> Problem:

@@ -43,6 +43,8 @@ struct orc {
     /* owned copies of the descriptor arrays */
     int64_t *nmodeDOF, *iDOF, *MultiSet, *nSPF, *ngp, *lhs, *rhs, *iOpDof;
     double *grid, *d1, *d2, *xd1;
+    int64_t nopparam;
+    double *gridoper;             /* [ngpmax, nopparam, ndof] parametrised operators on the grids */
     cplx *ae;
     int64_t ns, nm, nham, nspfmax, nopmax, ngpmax;
     int64_t *nAConf, *AStateOffset, nconf, dimpsi;
@@ -494,6 +496,8 @@ orc *orc_create(const mg_problem *p) {
     o->d1 = dup_f64(p->d1, (size_t)p->ngpmax * p->ngpmax * nd);
     o->d2 = dup_f64(p->d2, (size_t)p->ngpmax * p->ngpmax * nd);
     o->xd1 = dup_f64(p->xd1, (size_t)p->ngpmax * p->ngpmax * nd);
+    o->nopparam = p->nopparam > 0 ? p->nopparam : 0;
+    o->gridoper = o->nopparam ? dup_f64(p->gridoper, (size_t)p->ngpmax * o->nopparam * nd) : NULL;
     o->ae = xcalloc(p->nham, sizeof(cplx));
     for (int64_t t = 0; t < p->nham; t++) o->ae[t] = p->ae[2 * t] + I * p->ae[2 * t + 1];
     o->der_factor = -I;
@@ -613,7 +617,7 @@ void orc_destroy(orc *o) {
                     o->AStateOffset, o->ipsiSPF, o->ngpm, o->nOper, o->Oper, o->iOper,
                     o->IsIdentity, o->A, o->psi, o->Astart, o->psistart, o->h1, o->rho,
                     o->rho_ave, o->mf, o->rmf, o->BK, o->gwp, o->npar, o->gwpa, o->S00,
-                    o->S00m1, o->DD, o->DHD};
+                    o->S00m1, o->DD, o->DHD, o->gridoper};
     for (size_t i = 0; i < sizeof ptrs / sizeof ptrs[0]; i++) free(ptrs[i]);
     free(o);
 }
@@ -655,6 +659,31 @@ static void apply_prim(orc *o, int64_t m, int64_t k, int64_t code, int64_t nS, c
     for (int64_t q = k + 1; q < o->nmodeDOF[m]; q++) nl *= o->ngp[o->iDOF[q + maxdof * m] - 1];
     const int64_t ng = o->ngp[dof];
     const int64_t nblk = nS * nl;
+    if (code > 200) {   /* MultTensDiag with gridOper (hamilton.f90:467-468) */
+        const double *f = o->gridoper + gmax * ((code - 201) + o->nopparam * dof);
+        for (int64_t b = 0; b < nblk; b++)
+            for (int64_t g = 0; g < ng; g++) {
+                cplx *c = blk + (b * ng + g) * ll;
+                for (int64_t l = 0; l < ll; l++) c[l] *= f[g];
+            }
+        return;
+    }
+    if (code <= -200) { /* MultTensProj: x_k -> phi phi^T (hamilton.f90:632-661) */
+        const double *phi = o->gridoper + gmax * ((-code - 201) + o->nopparam * dof);
+        cplx *tmp = xcalloc(ll, sizeof(cplx));
+        for (int64_t b = 0; b < nblk; b++) {
+            cplx *c = blk + b * ng * ll;
+            for (int64_t l = 0; l < ll; l++) {
+                cplx acc = 0;
+                for (int64_t g = 0; g < ng; g++) acc += c[l + ll * g] * phi[g];
+                tmp[l] = acc;
+            }
+            for (int64_t g = 0; g < ng; g++)
+                for (int64_t l = 0; l < ll; l++) c[l + ll * g] = tmp[l] * phi[g];
+        }
+        free(tmp);
+        return;
+    }
     if (code >= 1 && code <= 20) {
         const double *x = o->grid + gmax * dof;
         for (int64_t b = 0; b < nblk; b++)

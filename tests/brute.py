@@ -19,6 +19,11 @@ def prim_matrix(prob, dof: int, code: int) -> np.ndarray:
     n = int(prob.ngp[dof - 1])
     if code == 0:
         return np.eye(n, dtype=complex)
+    if code > 200:
+        return np.diag(prob.gridoper[:n, code - 201, dof - 1]).astype(complex)
+    if code <= -200:
+        phi = prob.gridoper[:n, -code - 201, dof - 1]
+        return np.outer(phi, phi).astype(complex)
     if code >= 1:
         return np.diag(prob.grid[:n, dof - 1] ** code).astype(complex)
     if code == -1:

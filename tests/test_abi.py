@@ -44,8 +44,10 @@ def test_struct_layout_matches_header():
     assert ctypes.sizeof(MgOutput) == 5 * 8
     assert ctypes.sizeof(MgAttempt) == 7 * 8
     # 4 int64, 5 pointers, ngpmax, 4 pointers, nham, 4 pointers, 5 doubles, int64,
-    # then the Gaussian-mode fields: 2 pointers, 2 doubles
-    assert ctypes.sizeof(MgProblem) == 8 * (4 + 5 + 1 + 4 + 1 + 4 + 5 + 1 + 2 + 2)
+    # then the Gaussian-mode fields: 2 pointers, 2 doubles; then the
+    # parametrised operators: nopparam and the gridoper pointer
+    assert ctypes.sizeof(MgProblem) == 8 * (4 + 5 + 1 + 4 + 1 + 4 + 5 + 1 + 2 + 2 + 2)
+    assert MgProblem.nopparam.offset == 8 * 29 and MgProblem.gridoper.offset == 8 * 30
 
 
 def test_input_deck_roundtrip(tmp_path):
