@@ -181,6 +181,12 @@ int mg_apply_h_profile(mg_ctx *ctx, const void *dB, void *dHB, double *kernel_ms
 int mg_set_replay(mg_ctx *ctx, const mg_attempt *log, int64_t n);
 /* Current CMF2 step size carried between mg_propagate calls (a.u.). */
 int mg_stepsize(const mg_ctx *ctx, double *h);
+/* Sets that step size: the first attempt of the next mg_propagate /
+ * mg_propagate_vmf call (propagate_wf's `stepsize`, propagator.f90:103,
+ * which Macaw starts at tout * eps_int and carries through the run).  Lets a
+ * run resumed from a saved state continue with the step it had reached.
+ * h must be finite and > 0. */
+int mg_set_stepsize(mg_ctx *ctx, double h);
 /* Plain device copy of n complex elements of the context's work type
  * (complex FP32 for precision 1, FP64 otherwise) between two internal
  * buffers, event-timed; n <= nconf.  A traffic/bandwidth reference for

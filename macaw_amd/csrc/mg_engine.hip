@@ -635,7 +635,7 @@ void setup(mg_ctx *c, const mg_problem *p) {
         }
     }
     // mode groups (sizes: max over states); MG_BRICK overrides the LDS brick
-    // capacity in complex elements (tuning).  Two padded bricks plus the op
+    // capacity in complex elements (tuning).  Three bricks plus the op
     // matrices must fit the 160 KB of LDS a workgroup may address.
     c->brick = c->prec ? 2048 : 1024;
     if (const char *e = std::getenv("MG_RUN_BYTES")) {
@@ -643,8 +643,12 @@ void setup(mg_ctx *c, const mg_problem *p) {
         if (r >= 8 && r <= 512) c->run_bytes = r;
     }
     if (const char *e = std::getenv("MG_BRICK")) {
+        // tuning only: a brick that leaves room for two buffers is accepted;
+        // a program that needs three is then refused by run_plan's LDS check
         const int b = std::atoi(e);
-        if (brick_fits_lds(c, b)) c->brick = b;
+        const int nmx = c->nspfmax <= 4 ? 4 : (c->nspfmax <= 8 ? 8 : 16);
+        if (b >= 64 && c->es * ((size_t)kMaxBrickMats * nmx * nmx + 2 * (size_t)group_capacity(b)) <= 160 * 1024)
+            c->brick = b;
     }
     make_groups(c);
     // mean fields through brick programs when every chain op is square and
@@ -3064,6 +3068,13 @@ int mg_stepsize(const mg_ctx *c, double *h) {
     if (!c || !h) return -1;
     *h = c->stepsize;
     return 0;
+}
+
+int mg_set_stepsize(mg_ctx *c, double h) {
+    return guarded(c, [&] {
+        if (!(std::isfinite(h) && h > 0.0)) throw HipError{"step size must be finite and > 0"};
+        c->stepsize = h;
+    });
 }
 
 int mg_apply_h_profile(mg_ctx *c, const void *dB, void *dHB, double *kernel_ms, int64_t *nlaunch) {

@@ -55,7 +55,7 @@ __host__ __device__ inline uint32_t fdiv(uint32_t n, const FastDiv &f) {
 // group multi-index (first mode fastest, size M), r over the modes after the
 // group (size R).  A workgroup holds an Lt x M x Rt brick in LDS (Rt > 1 only
 // when Lt == L, so that the brick is contiguous), stored as
-// lt + Lt * (g + M * rt) with one padding slot every 8 elements.
+// lt + Lt * (g + M * rt).
 struct ModeTask {
     const void *X;               // storage type
     void *Y;

@@ -62,10 +62,6 @@ __device__ __forceinline__ double wave_sum(double v) {
     return v;
 }
 
-// LDS index padding: one slot every 8 complex values breaks the bank
-// conflicts of contiguous fibers (stride-1 mode inside a brick).
-__device__ __forceinline__ int pad(int i) { return i + (i >> 3); }
-
 // Global-memory loads/stores through address space 1: the task tables hand
 // the kernels generic pointers, which would otherwise become flat accesses
 // that also count against the LDS wait counter.
@@ -161,12 +157,12 @@ __device__ __forceinline__ void mfma_op8(const T *src, T *dst, const T *M, int n
         const int ff = valid ? f : 0;
         const int outer = (int)fdiv((uint32_t)ff, fds), inner = ff - outer * stride;
         const int base = inner + outer * stride * 8;
-        T x0 = src[pad(base + q * stride)];
-        T x1 = src[pad(base + (q + 4) * stride)];
+        T x0 = src[base + q * stride];
+        T x1 = src[base + (q + 4) * stride];
         if (!valid) { x0 = zeroT<T>(); x1 = zeroT<T>(); }
         V d;
         if (acc) {
-            const T c0 = dst[pad(base + ya * stride)], c1 = dst[pad(base + yb * stride)];
+            const T c0 = dst[base + ya * stride], c1 = dst[base + yb * stride];
             d[0] = c0.x; d[1] = c0.y; d[2] = c1.x; d[3] = c1.y;
         } else {
             d[0] = 0; d[1] = 0; d[2] = 0; d[3] = 0;
@@ -178,13 +174,18 @@ __device__ __forceinline__ void mfma_op8(const T *src, T *dst, const T *M, int n
         if (valid) {
             T o0, o1;
             o0.x = d[0]; o0.y = d[1]; o1.x = d[2]; o1.y = d[3];
-            dst[pad(base + ya * stride)] = o0;
-            dst[pad(base + yb * stride)] = o1;
+            dst[base + ya * stride] = o0;
+            dst[base + yb * stride] = o1;
         }
     }
 }
 
-int group_capacity(int brick) { return brick + (brick >> 3) + 1; }
+// Bricks are stored in LDS unpadded, element i in slot i.  Measured at
+// 4 x 8^8 against one padding slot every 8 values: mean fields 10.2 -> 7.2 ms
+// (mixed) and 19.6 -> 13.4 ms (FP64), H.A 4.46 -> 4.07 ms; a multi-level
+// padding and an XOR swizzle that a 32-bank model makes conflict-free for
+// every access pattern of these kernels were both slower than no padding.
+int group_capacity(int brick) { return brick; }
 
 // ---------------------------------------------------------------------------
 // Brick programs (see BrickTask).  LDS: [nmat op matrices][nbuf buffers].
@@ -208,8 +209,8 @@ __device__ __forceinline__ void brick_load(T *buf, const T *X, int64_t l0, int64
         for (int u = 0; u < kLoadUnroll; u++) {
             const int idx = base + u * nth + tid;
             if (full || idx < tot) {
-                if (add) buf[pad(idx)] = caddT(buf[pad(idx)], cmulT(alpha, v[u]));
-                else buf[pad(idx)] = v[u];
+                if (add) buf[idx] = caddT(buf[idx], cmulT(alpha, v[u]));
+                else buf[idx] = v[u];
             }
         }
     }
@@ -237,7 +238,7 @@ __device__ __forceinline__ void brick_store(const T *buf, T *Y, int64_t l0, int6
         for (int u = 0; u < kLoadUnroll; u++) {
             const int idx = base + u * nth + tid;
             if (full || idx < tot) {
-                T v = cmulT(alpha, buf[pad(idx)]);
+                T v = cmulT(alpha, buf[idx]);
                 if (acc) v = caddT(yv[u], v);
                 gstore(Y + a[u], v);
             }
@@ -256,15 +257,15 @@ __device__ __forceinline__ void brick_op_generic(const T *src, T *dst, const T *
         const int bo = inner + outer * stride * no;
         T x[NMAX];
 #pragma unroll
-        for (int j = 0; j < NMAX; j++) x[j] = (j < ni) ? src[pad(bi + j * stride)] : zeroT<T>();
+        for (int j = 0; j < NMAX; j++) x[j] = (j < ni) ? src[bi + j * stride] : zeroT<T>();
 #pragma unroll
         for (int i = 0; i < NMAX; i++) {
             if (i < no) {
-                T r = acc ? dst[pad(bo + i * stride)] : zeroT<T>();
+                T r = acc ? dst[bo + i * stride] : zeroT<T>();
 #pragma unroll
                 for (int j = 0; j < NMAX; j++)
                     if (j < ni) cfma(r, Mq[i + NMAX * j], x[j]);
-                dst[pad(bo + i * stride)] = r;
+                dst[bo + i * stride] = r;
             }
         }
     }
@@ -288,8 +289,8 @@ template <> struct GramMfma<float2> {
             const int f = 4 * st + q;
             const int ff = f < nfib ? f : 0;
             const int outer = (int)fdiv((uint32_t)ff, fds), inner = ff - outer * stride;
-            const float2 xv = X[pad(inner + outer * stride * n1 + (e < n1 ? e : 0) * stride)];
-            const float2 yv = Y[pad(inner + outer * stride * n2 + (e < n2 ? e : 0) * stride)];
+            const float2 xv = X[inner + outer * stride * n1 + (e < n1 ? e : 0) * stride];
+            const float2 yv = Y[inner + outer * stride * n2 + (e < n2 ? e : 0) * stride];
             const float av = (f < nfib && e < n1) ? (im ? xv.y : xv.x) : 0.f;
             const float bv = (f < nfib && e < n2) ? (im ? yv.y : yv.x) : 0.f;
             d = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bv, d, 0, 0, 0);
@@ -311,8 +312,8 @@ template <> struct GramMfma<double2> {
             const int f = 4 * st + q;
             const int ff = f < nfib ? f : 0;
             const int outer = (int)fdiv((uint32_t)ff, fds), inner = ff - outer * stride;
-            const double2 xv = X[pad(inner + outer * stride * n1 + (e < n1 ? e : 0) * stride)];
-            const double2 yv = Y[pad(inner + outer * stride * n2 + (e < n2 ? e : 0) * stride)];
+            const double2 xv = X[inner + outer * stride * n1 + (e < n1 ? e : 0) * stride];
+            const double2 yv = Y[inner + outer * stride * n2 + (e < n2 ? e : 0) * stride];
             const double av = (f < nfib && e < n1) ? (im ? xv.y : xv.x) : 0.0;
             const double bv = (f < nfib && e < n2) ? (im ? yv.y : yv.x) : 0.0;
             d = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, d, 0, 0, 0);
@@ -392,13 +393,13 @@ __global__ void __launch_bounds__(GRAM ? 256 : 1024) k_brick(const BrickTask *__
                 const int tot = Lt * I.M * Rt;
                 const T al = from_d2<T>(I.alpha);
                 for (int idx = tid; idx < tot; idx += nth)
-                    B[pad(idx)] = caddT(B[pad(idx)], cmulT(al, A[pad(idx)]));
+                    B[idx] = caddT(B[idx], cmulT(al, A[idx]));
             } else if (code == kBStore) {
                 brick_store(A, static_cast<T *>(const_cast<void *>(I.ptr)), l0, r0, L, Lt, Rt, fd_Lt, I.M, I.fd_M,
                             I.acc, from_d2<T>(I.alpha));
             } else if (code == kBZero) {
                 const int tot = Lt * I.M * Rt;
-                for (int idx = tid; idx < tot; idx += nth) A[pad(idx)] = zeroT<T>();
+                for (int idx = tid; idx < tot; idx += nth) A[idx] = zeroT<T>();
             } else if (GRAM && code == kBGram) {
                 double part[4] = {0.0, 0.0, 0.0, 0.0};
                 GramMfma<T>::run(A, B, I.ni, I.no, I.nfib, I.stride, I.fd_stride, part);

@@ -39,6 +39,7 @@ def lib():
         L.mg_dimpsi.argtypes = [vp]
         L.mg_dims.argtypes = [vp, i64p, i64p]
         L.mg_stepsize.argtypes = [vp, f64p]
+        L.mg_set_stepsize.argtypes = [vp, C.c_double]
         L.mg_apply_h_profile.argtypes = [vp, vp, vp, f64p, i64p]
         L.mg_set_replay.argtypes = [vp, C.POINTER(MgAttempt), C.c_int64]
         L.mg_profile_copy.argtypes = [vp, C.c_int64, f64p]
@@ -184,6 +185,10 @@ class Engine:
         h = C.c_double(0)
         self._check(lib().mg_stepsize(self.h, C.byref(h)))
         return h.value
+
+    def set_stepsize(self, h):
+        """First attempted step of the next propagate call (a.u.)."""
+        self._check(lib().mg_set_stepsize(self.h, float(h)))
 
     def apply_h_stats(self):
         b, f = C.c_double(0), C.c_double(0)

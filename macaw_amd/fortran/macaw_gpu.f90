@@ -10,7 +10,7 @@ module macaw_gpu
    public :: mg_create, mg_destroy, mg_last_error, mg_nconf, mg_dimpsi, mg_dims
    public :: mg_set_state, mg_get_state, mg_set_start, mg_calc_h1mode, mg_calc_rho
    public :: mg_calc_meanfield, mg_apply_h, mg_apply_h_device, mg_der_spf, mg_cmf2_step, mg_observables
-   public :: mg_propagate, mg_set_normalize, mg_stepsize, mg_set_replay, mg_autotune
+   public :: mg_propagate, mg_set_normalize, mg_stepsize, mg_set_stepsize, mg_set_replay, mg_autotune
    public :: mg_calc_derivs, mg_propagate_vmf
    public :: mg_prepare_gwp, mg_gwp_overlaps, mg_der_spf_gwp, mg_a_to_b
    public :: mg_apply_h_stats, mg_apply_h_profile, mg_profile_copy
@@ -164,6 +164,11 @@ module macaw_gpu
          import :: c_int, c_ptr, c_double
          type(c_ptr), value :: ctx
          real(c_double), intent(out) :: h
+      end function
+      integer(c_int) function mg_set_stepsize(ctx, h) bind(C, name='mg_set_stepsize')
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: ctx
+         real(c_double), value :: h
       end function
       integer(c_int) function mg_set_replay(ctx, log, n) bind(C, name='mg_set_replay')
          import :: c_int, c_int64_t, c_ptr, mg_attempt

@@ -2,6 +2,7 @@
 disk, and the precision comparison of such runs.
 
     python -m macaw_amd.trajectory run --workload c2 --precision 0 --out DIR [--t-end FS] [--budget S]
+    python -m macaw_amd.trajectory run --workload c2 --precision 0 --out DIR --resume [--budget S]
     python -m macaw_amd.trajectory run --workload c2 --precision 1 --replay DIR64/attempts.txt --out DIR
     python -m macaw_amd.trajectory compare --ref DIR64 --runs DIR_A DIR_B ... [--md FILE]
 
@@ -14,7 +15,11 @@ every finished interval.  It stops at --t-end or when the next interval would
 pass the wall budget (the mean interval time so far).  With --replay the
 engine takes the step sizes and accept/reject decisions of that attempt log
 (mg_set_replay), which removes the controller from a precision comparison;
-the run stops where the log ends.
+the run stops where the log ends.  When a run stops on its budget it saves
+its state (time, A, SPFs, the step size reached) to DIR/state.npz; --resume
+continues from there, appending to the same files, so a long run can be
+split over several processes.  A resumed segment starts the learned
+controller's step history afresh (the step size itself carries over).
 
 `compare` reads runs and reports, against the reference run, the largest
 differences in C(t), populations, norm and energy over the common output
@@ -72,6 +77,18 @@ def run(args):
     prob = build(args.workload)
     os.makedirs(args.out, exist_ok=True)
     eng = Engine(prob, precision=args.precision)
+    state_path = os.path.join(args.out, "state.npz")
+    resume = None
+    if args.resume:
+        if args.replay:
+            raise SystemExit("--resume and --replay cannot be combined")
+        with np.load(state_path) as z:
+            resume = {k: z[k] for k in z.files}
+        with open(os.path.join(args.out, "meta.json")) as fh:
+            prev = json.load(fh)
+        if (prev["workload"], prev["precision"], prev["controller"]) != (args.workload, args.precision,
+                                                                        args.controller):
+            raise SystemExit(f"--resume: {args.out} holds a run of another workload, precision or controller")
     if args.controller == "learned":
         eng.set_step_model(np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "step_controller.npy")))
     t_log_end = None
@@ -90,14 +107,28 @@ def run(args):
         t_end = min(t_end, t_log_end)
     meta = {"workload": args.workload, "precision": args.precision, "controller": args.controller,
             "replay": args.replay or None, "budget_s": args.budget, "t_end_fs": args.t_end}
+    t, k, natt, wall_prev, segments = 0.0, 0, 0, 0.0, 1
+    if resume is not None:
+        meta = prev
+        meta["t_end_fs"] = args.t_end
+        t = float(resume["t"])
+        eng.set_state(resume["A"], resume["psi"])
+        eng.set_stepsize(float(resume["stepsize"]))
+        natt, wall_prev = int(prev.get("attempts", 0)), float(prev.get("wall_s", 0.0))
+        segments = int(prev.get("segments", 1)) + 1
+        _log(f"resuming at {t * AU2FS:.3f} fs (segment {segments}), step {float(resume['stepsize']) * AU2FS:.3e} fs")
     with open(os.path.join(args.out, "meta.json"), "w") as fh:
         json.dump(meta, fh)
-    fo = open(os.path.join(args.out, "outputs.txt"), "w")
-    fa = open(os.path.join(args.out, "attempts.txt"), "w")
-    fo.write("# time_fs  Re_C  Im_C  norm  energy_au  " + "  ".join(f"pop{s + 1}" for s in range(prob.nstates)) + "\n")
-    fa.write("# " + "  ".join(ATT_FIELDS) + "\n")
-    t, k, wall0 = 0.0, 0, time.perf_counter()
-    natt = 0
+    mode = "a" if resume is not None else "w"
+    fo = open(os.path.join(args.out, "outputs.txt"), mode)
+    fa = open(os.path.join(args.out, "attempts.txt"), mode)
+    if resume is None:
+        fo.write("# time_fs  Re_C  Im_C  norm  energy_au  " + "  ".join(f"pop{s + 1}" for s in range(prob.nstates))
+                 + "\n")
+        fa.write("# " + "  ".join(ATT_FIELDS) + "\n")
+    wall0 = time.perf_counter()
+    # the record at the resume time was written when the previous segment stopped
+    skip_record = resume is not None
     while True:
         last = t >= t_end - 0.1 * tout
         t_int = time.perf_counter()
@@ -107,9 +138,11 @@ def run(args):
         else:
             ol, pl, atts = eng.propagate(t, t)
             o, p = ol[0], pl[0]
-        fo.write(f"{o.time * AU2FS:.10f} {o.auto_re:.17e} {o.auto_im:.17e} {o.norm:.17e} {o.energy:.17e} "
-                 + " ".join(f"{x:.17e}" for x in p) + "\n")
-        fo.flush()
+        if not skip_record:
+            fo.write(f"{o.time * AU2FS:.10f} {o.auto_re:.17e} {o.auto_im:.17e} {o.norm:.17e} {o.energy:.17e} "
+                     + " ".join(f"{x:.17e}" for x in p) + "\n")
+            fo.flush()
+        skip_record = False
         for a in atts:
             fa.write(f"{a.time!r} {a.h!r} {a.err_A!r} {a.err_psi!r} {a.accepted} {a.nkrylov1} {a.nkrylov2}\n")
         fa.flush()
@@ -122,15 +155,22 @@ def run(args):
         el = time.perf_counter() - wall0
         _log(f"{t * AU2FS:.2f} fs  {el:.0f} s  {natt} attempts  interval {time.perf_counter() - t_int:.1f} s  "
              f"C = {o.auto_re:+.8f}{o.auto_im:+.8f}i")
-        if el + el / k > args.budget:
-            # one more output record at the time reached, then stop
+        if el + el / k > args.budget or (args.max_intervals and k >= args.max_intervals):
+            # one more output record at the time reached, and the state to
+            # resume from, then stop
             o, p = eng.observables(t)
             fo.write(f"{o.time * AU2FS:.10f} {o.auto_re:.17e} {o.auto_im:.17e} {o.norm:.17e} {o.energy:.17e} "
                      + " ".join(f"{x:.17e}" for x in p) + "\n")
             fo.flush()
+            if not args.replay:
+                A, psi = eng.get_state()
+                tmp = state_path + ".tmp.npz"
+                np.savez(tmp, t=t, A=A, psi=psi, stepsize=eng.stepsize())
+                os.replace(tmp, state_path)
             break
-    wall = time.perf_counter() - wall0
-    meta.update({"t_reached_fs": t * AU2FS, "wall_s": wall, "attempts": natt, "fs_per_s": t * AU2FS / wall})
+    wall = wall_prev + time.perf_counter() - wall0
+    meta.update({"t_reached_fs": t * AU2FS, "wall_s": wall, "attempts": natt, "fs_per_s": t * AU2FS / wall,
+                 "segments": segments})
     with open(os.path.join(args.out, "meta.json"), "w") as fh:
         json.dump(meta, fh)
     fo.close()
@@ -190,6 +230,9 @@ def main(argv=None):
     r.add_argument("--budget", type=float, default=1e9, help="wall seconds")
     r.add_argument("--replay", help="attempts.txt of another run to replay")
     r.add_argument("--autotune", action="store_true")
+    r.add_argument("--resume", action="store_true", help="continue the run saved in --out")
+    r.add_argument("--max-intervals", type=int, default=0,
+                   help="stop (and save the state) after this many output intervals of this segment")
     r.add_argument("--out", required=True)
     c = sub.add_parser("compare")
     c.add_argument("--ref", required=True)
