@@ -158,6 +158,9 @@ class InputDeck:
     int_type_psi: int = 0
     steps: bool = False
     normalize: int = 0
+    densmat: list = field(default_factory=list)   # modes (1-based) of `densmat = m`
+    density: bool = False
+    wp_ovl: bool = False
     expect: list = field(default_factory=list)   # operator files of `expect = file`
     init_state: int = 1
     init_spf: dict = field(default_factory=dict)
@@ -248,6 +251,14 @@ def parse_input(path: str) -> InputDeck:
             deck.min_stepsize = _fortran_float(value) / AU2FS
         elif name == "steps":
             deck.steps = True
+        elif name == "densmat":
+            # input.f90:277-282: one mode per keyword, its density matrices
+            # and SPFs written at every output time
+            deck.densmat.append(int(value))
+        elif name == "density":
+            deck.density = True
+        elif name == "wp_ovl":
+            deck.wp_ovl = True
         elif name == "normalize":
             deck.normalize = int(value)
         elif name == "integration":

@@ -28,8 +28,12 @@ in Macaw's formats:
   output.f90:152-224 (time, norm, energy and its change from the start,
   electronic populations, natural populations of every mode, and for
   Gaussian modes the gross Gaussian populations and the population errors of
-  dropping one Gaussian).  The echo of the input that precedes them in
-  Macaw's file is not written.
+  dropping one Gaussian), followed with ``density`` by the populations of
+  the DVR basis functions (output.f90:277-307).  The echo of the input that
+  precedes them in Macaw's file is not written;
+* ``densmat`` (``densmat = m``), ``density`` (``density``) and ``wpOvl``
+  (``wp_ovl``): per output time, as macaw_amd/densities.py describes;
+* ``log``: the first tables of Macaw's log (input.f90:391-413).
 
 The reference propagates one output interval past its last output; this
 driver stops at the last output time, whose state the restart file holds.
@@ -216,7 +220,8 @@ def run(input_path: str, outdir: str = ".", precision: int = 0, controller: str 
         steps: bool = False, psi_every: int = 0, autotune: bool = False):
     from .engine import Engine
     from .problem import load_problem
-    from .restart import write_psi_record, write_restart
+    from . import densities as DENS
+    from .restart import _write_record as write_record, write_psi_record, write_restart
     prob = load_problem(input_path)
     if steps and prob.normalize:
         raise ValueError("--steps drives single CMF2 steps from the host and does not renormalise; "
@@ -241,6 +246,12 @@ def run(input_path: str, outdir: str = ".", precision: int = 0, controller: str 
     # one output interval per call (the step size carries over between
     # calls), so that rho of every output time is at hand for `output`
     psi_fh = open(os.path.join(outdir, "psi"), "wb") if psi_every > 0 else None
+    ex = prob.extra
+    with open(os.path.join(outdir, "log"), "w") as fh:
+        fh.write("\n".join(DENS.log_tables(prob, ex["grids"], ex["hamiltonian_file"])) + "\n")
+    dm_fh = open(os.path.join(outdir, "densmat"), "wb") if ex.get("densmat") else None
+    dens_fh = open(os.path.join(outdir, "density"), "wb") if ex.get("density") else None
+    wpovl = [] if ex.get("wp_ovl") else None
     expect = Expectations(prob) if prob.expect else None
     expect_lines = []
     k_out = 0
@@ -265,6 +276,16 @@ def run(input_path: str, outdir: str = ".", precision: int = 0, controller: str 
         if energy_start is None:
             energy_start = o.energy * o.norm      # energyMF at the start (propagator.f90:96)
         block += output_block(prob, o, p, energy_start, *mode_quantities(prob, rho, psi))
+        if dm_fh is not None:
+            DENS.write_densmat(dm_fh, prob, t, rho, psi, ex["densmat"])
+        if dens_fh is not None:
+            recs, lines = DENS.density_records(prob, t, rho, psi, ex["trafo"])
+            for rec in recs:
+                write_record(dens_fh, rec)
+            block += lines
+        if wpovl is not None:
+            # one wave packet: the 1 x 1 overlap matrix <Psi|Psi> (Calc_SWP)
+            wpovl += DENS.wpovl_lines(t, complex(float(np.sum(p)), 0.0), header=not wpovl)
         outs.append(o)
         pops.append(p)
         atts += list(a)
@@ -276,8 +297,12 @@ def run(input_path: str, outdir: str = ".", precision: int = 0, controller: str 
             acc = [x for x in a if x.accepted]
             t = acc[-1].time + acc[-1].h        # the engine's own accumulated time
     t_end = t
-    if psi_fh is not None:
-        psi_fh.close()
+    for fh in (psi_fh, dm_fh, dens_fh):
+        if fh is not None:
+            fh.close()
+    if wpovl is not None:
+        with open(os.path.join(outdir, "wpOvl"), "w") as fh:
+            fh.write("\n".join(wpovl) + "\n")
     pops = np.asarray(pops)
     with open(os.path.join(outdir, "output"), "w") as fh:
         fh.write("\n".join(block) + "\n")

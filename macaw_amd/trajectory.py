@@ -77,7 +77,7 @@ def run(args):
     prob = build(args.workload)
     os.makedirs(args.out, exist_ok=True)
     eng = Engine(prob, precision=args.precision)
-    state_path = os.path.join(args.out, "state.npz")
+    state_path = args.state or os.path.join(args.out, "state.npz")
     resume = None
     if args.resume:
         if args.replay:
@@ -162,7 +162,7 @@ def run(args):
             fo.write(f"{o.time * AU2FS:.10f} {o.auto_re:.17e} {o.auto_im:.17e} {o.norm:.17e} {o.energy:.17e} "
                      + " ".join(f"{x:.17e}" for x in p) + "\n")
             fo.flush()
-            if not args.replay:
+            if not args.replay and state_path != "none":
                 A, psi = eng.get_state()
                 tmp = state_path + ".tmp.npz"
                 np.savez(tmp, t=t, A=A, psi=psi, stepsize=eng.stepsize())
@@ -231,6 +231,7 @@ def main(argv=None):
     r.add_argument("--replay", help="attempts.txt of another run to replay")
     r.add_argument("--autotune", action="store_true")
     r.add_argument("--resume", action="store_true", help="continue the run saved in --out")
+    r.add_argument("--state", help="file of the saved state (default OUT/state.npz; 'none': not saved)")
     r.add_argument("--max-intervals", type=int, default=0,
                    help="stop (and save the state) after this many output intervals of this segment")
     r.add_argument("--out", required=True)
