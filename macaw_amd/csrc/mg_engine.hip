@@ -4,7 +4,9 @@
 //
 // The state (A-vector, SPFs) and every work array live in HBM for the whole
 // propagation; the host only issues launches and reads back the few scalars
-// the control flow needs (Lanczos alpha/beta, step errors).
+// the control flow needs (step errors and decisions; with Gaussian modes also
+// the Lanczos alpha/beta).  A grid-mode CMF2 attempt takes every decision on
+// the device and runs as one HIP graph.
 #include "macaw_gpu.h"
 #include "mg_internal.h"
 
@@ -235,6 +237,16 @@ struct mg_ctx {
     // ---- HIP graphs of the cached launch plans ----
     bool use_graphs = true;
     int graph_epoch = 0;
+    // ---- device-controlled CMF2 attempt (grid modes) and its HIP graph ----
+    DevBuf datt;                       // AttemptDev, then two LanczosDev
+    AttemptDev *att_host = nullptr;    // pinned: inputs written before, results read after an attempt
+    hipGraphExec_t att_exec = nullptr; // one whole attempt, captured on the second attempt
+    int att_epoch = -1, att_kmax = 0, att_runs = 0;
+    int nk_seen = 2;                   // largest Krylov size of the graph-run attempts so far
+    bool att_graphs = true;            // MG_ATTEMPT_GRAPH=0: the attempt runs eagerly
+    int att_kmax_cap = kNKrylovMax;    // MG_ATTEMPT_KMAX (test aid): fewer iterations per graph-run build
+    bool capturing = false;            // an attempt is being captured: plans launch their kernels
+    const int *plan_skip = nullptr;    // skip flag handed to the brick launches being enqueued
     // ---- learned step controller ----
     DevBuf drmf_minw;                      // smallest rho eigenvalue per rho^-1 MF job
     DevBuf dmf_prev, dmf_tmp;              // mean fields of the previous accepted step
@@ -1164,7 +1176,8 @@ void launch_plan_entry(mg_ctx *c, const Plan &p, const Launch &l, int nmax) {
         launch_gram_slots_final(c->dgpart.as<double2>(), l.ngram, l.gram_blocks,
                                 p.targets.as<double2 *>() + l.slot_off, c->nspfmax, c->stream);
     } else {
-        launch_brick_tasks(base + l.off, l.n, l.maxblocks, nmax, l.brick, l.nbuf, l.nmat, p.prec, c->stream);
+        launch_brick_tasks(base + l.off, l.n, l.maxblocks, nmax, l.brick, l.nbuf, l.nmat, p.prec, c->stream, 0,
+                           nullptr, 0, c->plan_skip);
     }
 }
 
@@ -1178,7 +1191,9 @@ void run_plan(mg_ctx *c, const Plan &p) {
     (void)base;
     static const bool verbose_all = std::getenv("MG_VERBOSE") != nullptr;
     if (!c->prof && !verbose_all) {
-        if (c->use_graphs && p.runs >= 1) {
+        // a plan graph holds its launches' arguments: plans enqueued with a
+        // skip flag or into a captured attempt launch their kernels directly
+        if (c->use_graphs && p.runs >= 1 && !c->capturing && !c->plan_skip) {
             if (p.gexec && p.epoch != c->graph_epoch) {
                 HIP_CHECK(hipGraphExecDestroy(p.gexec));
                 p.gexec = nullptr;
@@ -2237,6 +2252,8 @@ SpfParams spf_params(mg_ctx *c, double dt) {
     P.dormand_prince = (c->int_type_psi == 2);
     P.der_factor = make_double2(0.0, -1.0);
     P.max_attempts = 200000;
+    P.dt_src = nullptr;
+    P.dt_scale = 1.0;
     return P;
 }
 
@@ -2536,78 +2553,256 @@ void phase(mg_ctx *c, int bucket) {
     c->phase_t0 = t;
 }
 
+// ----------------------------------------------------------------------------
+// Device-controlled CMF2 attempt for grid-mode problems (cmf2_base.f90:76-203).
+// Every decision of an attempt is taken on the device: the Lanczos stopping
+// test and propagator coefficients (k_lan_ctrl, k_lan_coef), DeltaPsi and
+// err_A, the accept/reject decision and the copies that commit an accepted
+// step (k_attempt_decide, k_copy_if); the attempted step h comes from device
+// memory.  The host writes h (and a replayed decision) before, and reads the
+// errors, Krylov sizes and the decision after an attempt, so the whole
+// attempt is captured once into a HIP graph and replayed for every later
+// attempt.  A Krylov build enqueues kmax iterations; those past convergence
+// return at once on a device flag.
+
+AttemptDev *att_dev(mg_ctx *c) { return c->datt.as<AttemptDev>(); }
+LanczosDev *lan_dev(mg_ctx *c, int i) {
+    return reinterpret_cast<LanczosDev *>(c->datt.as<char>() + sizeof(AttemptDev) + 64 + (size_t)i * sizeof(LanczosDev));
+}
+
+// Lanczos (cmf2_integration.f90:30-103) from B0 (FP64) over dtscale * h; BT
+// FP64 and normalised.  build = false reuses L's Krylov space.
+void lanczos_dev(mg_ctx *c, const double2 *B0, double2 *BT, LanczosDev *L, double dtscale, bool build, int kmax) {
+    const int64_t N = c->nconf;
+    const int pr = c->prec;
+    void *BK = c->dBK.p;
+    double *dsc = c->dscal.as<double>();
+    double *nsq = dsc + 24;          // n_j^2, j <= kNKrylovMax
+    double *part = c->dpartials.as<double>();
+    double2 *dot_d = reinterpret_cast<double2 *>(dsc + 2);
+    const AttemptDev *at = att_dev(c);
+    if (build) {
+        launch_lan_begin(L, c->stream);
+        launch_norm2(B0, N, part, c->nblocks_red, nsq + 0, 0, c->stream);
+        if (pr) launch_convert(BK, 1, B0, 0, N, c->stream);
+        const void *u0 = pr ? BK : (const void *)B0;
+        for (int j = 1; j <= kmax; j++) {
+            const void *uj = (j == 1) ? u0 : c->at(BK, N * (j - 1));
+            void *w = c->at(BK, N * j);
+            const void *up = (j >= 2) ? ((j == 2) ? u0 : c->at(BK, N * (j - 2))) : nullptr;
+            const int *skip = (j >= 2) ? &L->done : nullptr;
+            if (j == 1 && pr && c->lan64) {
+                double2 *w64 = c->dW64.as<double2>();
+                apply_h64(c, B0, w64);
+                launch_dot(B0, w64, N, part, c->nblocks_red, dot_d, 0, c->stream);
+                launch_lanczos_step(w64, B0, nullptr, dsc + 2, nsq, 0, N, part, c->nblocks_red, nsq + 1, 0, c->stream);
+                launch_convert(w, 1, w64, 0, N, c->stream);
+            } else {
+                struct SkipScope {
+                    mg_ctx *c;
+                    ~SkipScope() { c->plan_skip = nullptr; }
+                } scope{c};
+                c->plan_skip = skip;
+                apply_h(c, uj, w);
+                launch_dot(uj, w, N, part, c->nblocks_red, dot_d, pr, c->stream, skip);
+                launch_lanczos_step(w, uj, up, dsc + 2, nsq, j - 1, N, part, c->nblocks_red, nsq + j, pr, c->stream,
+                                    skip);
+            }
+            launch_lan_ctrl(L, at, dot_d, nsq, j, dtscale, kmax, c->stream);
+        }
+    }
+    double2 *dcoef = reinterpret_cast<double2 *>(dsc + 48);
+    launch_lan_coef(L, at, dtscale, build ? 1 : 0, dcoef, c->stream);
+    // B0 is u_0 of the space (also when it is reused)
+    launch_combine_v0(BT, B0, BK, N, dcoef, 0, N, pr, c->stream, &L->nk);
+    launch_norm2(BT, N, part, c->nblocks_red, dsc + 6, 0, c->stream);
+    launch_scale(BT, dsc + 6, N, 0, c->stream);
+    check_launch();
+}
+
+// DeltaPsi (gmctdh_objects.f90:204-227) of (A1, psi1) and (A1, psi2) from
+// the SPF overlaps queued by delta_psi_overlaps; the result in at->err_psi.
+void delta_psi_contract_dev(mg_ctx *c, const double2 *A1) {
+    double *part = c->dpartials.as<double>();
+    double2 *o = reinterpret_cast<double2 *>(c->dscal.as<double>() + 88);   // o11, o22, o12
+    double2 *T = c->dHtmp.as<double2>();
+    apply_kron(c, A1, T, c->dS.as<double2>());
+    launch_dot(T, A1, c->nconf, part, c->nblocks_red, o + 0, 0, c->stream);
+    apply_kron(c, A1, T, c->dS3.as<double2>());
+    launch_dot(T, A1, c->nconf, part, c->nblocks_red, o + 1, 0, c->stream);
+    apply_kron(c, A1, T, c->dS2.as<double2>());
+    launch_dot(A1, T, c->nconf, part, c->nblocks_red, o + 2, 0, c->stream);
+    launch_attempt_psi(att_dev(c), o + 0, o + 1, o + 2, c->stream);
+}
+
+// SPF half step over dtscale * h (h on the device) on stream st.
+void propagate_spf_dev(mg_ctx *c, double2 *psi, double dtscale, hipStream_t st) {
+    SpfParams P = spf_params(c, 0.0);
+    P.dt_src = &att_dev(c)->h;
+    P.dt_scale = dtscale;
+    launch_spf_propagate(c->dspfmodes.as<SpfModeDesc>(), c->ndvr_desc, psi, c->dspfscr.as<double2>(), P,
+                         dof_tables(c), c->dstatus.as<int>(), st);
+}
+
+// Everything of one attempt, in stream order (graph-safe: no host sync, no
+// allocation, no host-side decision).
+void cmf2_attempt_enqueue(mg_ctx *c, int kmax) {
+    const int64_t N = c->nconf, D = c->dimpsi;
+    double2 *A = c->dA.as<double2>(), *psi = c->dpsi.as<double2>();
+    double2 *B0 = c->dB0.as<double2>(), *BH = c->dBH.as<double2>(), *B02 = c->dB02.as<double2>();
+    double2 *pH = c->dpH.as<double2>(), *pH2 = c->dpH2.as<double2>();
+    AttemptDev *at = att_dev(c);
+    LanczosDev *L1 = lan_dev(c, 0), *L2 = lan_dev(c, 1);
+    launch_fill(reinterpret_cast<double2 *>(c->dstatus.p), 1, make_double2(0, 0), c->stream);
+    calc_rho(c, A);
+    calc_h1mode(c, psi);
+    phase(c, 0);
+    calc_meanfield(c, A, true);
+    phase(c, 1);
+    copy_d2d(c, B0, A, N);
+    copy_d2d(c, pH, psi, D);
+    // the SPF half step needs only rho(A) and the mean fields: stream2, beside
+    // the Lanczos propagation of A
+    HIP_CHECK(hipEventRecord(c->ev_fork, c->stream));
+    HIP_CHECK(hipStreamWaitEvent(c->stream2, c->ev_fork, 0));
+    propagate_spf_dev(c, pH, 0.5, c->stream2);
+    HIP_CHECK(hipEventRecord(c->ev_join, c->stream2));
+    lanczos_dev(c, B0, BH, L1, 0.5, true, kmax);
+    phase(c, 3);
+    HIP_CHECK(hipStreamWaitEvent(c->stream, c->ev_join, 0));
+    phase(c, 2);
+    calc_rho(c, BH);
+    calc_h1mode(c, pH);
+    phase(c, 0);
+    calc_meanfield(c, BH, true);
+    phase(c, 1);
+    copy_d2d(c, pH2, psi, D);
+    propagate_spf_dev(c, pH2, 0.5, c->stream);
+    phase(c, 2);
+    // DeltaPsi reads pH2 only through its SPF overlaps: once those are
+    // queued, the second SPF half step overlaps the A-vector contractions
+    delta_psi_overlaps(c, pH, pH2);
+    HIP_CHECK(hipEventRecord(c->ev_fork, c->stream));
+    HIP_CHECK(hipStreamWaitEvent(c->stream2, c->ev_fork, 0));
+    propagate_spf_dev(c, pH2, 0.5, c->stream2);
+    HIP_CHECK(hipEventRecord(c->ev_join, c->stream2));
+    delta_psi_contract_dev(c, BH);
+    phase(c, 4);
+    HIP_CHECK(hipStreamWaitEvent(c->stream, c->ev_join, 0));
+    phase(c, 2);
+    calc_h1mode(c, pH2);
+    phase(c, 0);
+    lanczos_dev(c, BH, B02, L2, -0.5, true, kmax);
+    phase(c, 3);
+    launch_axpby(B02, B0, -1.0, 1.0, N, 0, c->stream);
+    double *n2 = c->dscal.as<double>() + 12;
+    launch_norm2(B02, N, c->dpartials.as<double>(), c->nblocks_red, n2, 0, c->stream);
+    launch_attempt_decide(at, L1, L2, n2, c->eps_int, c->dstatus.as<int>(), c->stream);
+    // the accepted step: A from the half step with the same Krylov space
+    // (computed either way; committed only when accepted)
+    lanczos_dev(c, BH, B0, L2, 0.5, false, kmax);
+    phase(c, 3);
+    launch_copy_if(A, B0, N, &at->accept, c->stream);
+    launch_copy_if(psi, pH2, D, &at->accept, c->stream);
+    HIP_CHECK(hipMemcpyAsync(c->att_host, at, sizeof(AttemptDev), hipMemcpyDeviceToHost, c->stream));
+    check_launch();
+    phase(c, 5);
+}
+
+// One attempt: eagerly on the first attempts (they build every launch plan
+// and size the Gram partial buffer), then through its captured graph.
+void cmf2_attempt_run(mg_ctx *c, int kmax) {
+    const bool graphs = c->use_graphs && c->att_graphs && !c->phases && !c->prof;
+    if (c->att_exec && (c->att_epoch != c->graph_epoch || c->att_kmax != kmax || !graphs)) {
+        HIP_CHECK(hipGraphExecDestroy(c->att_exec));
+        c->att_exec = nullptr;
+    }
+    if (!graphs || c->att_runs < 1) {
+        cmf2_attempt_enqueue(c, kmax);
+        c->att_runs++;
+        return;
+    }
+    if (!c->att_exec) {
+        const int epoch = c->graph_epoch;
+        hipGraph_t g = nullptr;
+        c->capturing = true;
+        HIP_CHECK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        try {
+            cmf2_attempt_enqueue(c, kmax);
+        } catch (...) {
+            hipGraph_t junk = nullptr;
+            (void)hipStreamEndCapture(c->stream, &junk);
+            if (junk) (void)hipGraphDestroy(junk);
+            c->capturing = false;
+            throw;
+        }
+        c->capturing = false;
+        HIP_CHECK(hipStreamEndCapture(c->stream, &g));
+        if (c->graph_epoch != epoch) {   // a buffer was reallocated while enqueuing
+            (void)hipGraphDestroy(g);
+            throw HipError{"attempt graph: launch plans changed during capture"};
+        }
+        HIP_CHECK(hipGraphInstantiate(&c->att_exec, g, nullptr, nullptr, 0));
+        HIP_CHECK(hipGraphDestroy(g));
+        c->att_epoch = epoch;
+        c->att_kmax = kmax;
+    }
+    HIP_CHECK(hipGraphLaunch(c->att_exec, c->stream));
+}
+
 void cmf2_step(mg_ctx *c, double tt, double htry, double *hdid_out, double *hnext_out, std::vector<mg_attempt> *log) {
     if (c->ngwp) {
         cmf2_step_gwp(c, tt, htry, hdid_out, hnext_out, log);
         return;
     }
-    const int64_t N = c->nconf, D = c->dimpsi;
-    double2 *A = c->dA.as<double2>(), *psi = c->dpsi.as<double2>();
-    double2 *B0 = c->dB0.as<double2>(), *BH = c->dBH.as<double2>(), *B02 = c->dB02.as<double2>();
-    double2 *pH = c->dpH.as<double2>(), *pH2 = c->dpH2.as<double2>();
-    const double safe = 0.95, maxscal = 1.5;
+    if (!c->datt.p) c->datt.alloc(sizeof(AttemptDev) + 64 + 2 * sizeof(LanczosDev));
+    const double safe = 0.95;
     double hdid = htry;
+    bool full = false;
     for (;;) {
         int forced = -1;
+        const size_t rpos = c->replay_pos;
         if (c->replay_pos < c->replay.size()) {
             const mg_attempt &r = c->replay[c->replay_pos++];
             hdid = r.h;
             forced = (int)r.accepted;
         }
+        // iterations enqueued per Krylov build: the reference's limit on the
+        // first (eager) attempt and after a build that needed more, else two
+        // beyond the largest Krylov size seen (cmf2_integration.f90:57-64)
+        const int kmax = (c->att_runs == 0 || full)
+                             ? kNKrylovMax - 1
+                             : std::min({kNKrylovMax - 1, c->nk_seen + 2, c->att_kmax_cap});
+        full = false;
         if (c->phases) phase(c, 5);
-        calc_rho(c, A);
-        calc_h1mode(c, psi);
-        phase(c, 0);
-        calc_meanfield(c, A, true);
-        phase(c, 1);
-        copy_d2d(c, B0, A, N);
-        copy_d2d(c, pH, psi, D);
-        // the SPF half step needs only rho(A) and the mean fields: it runs on
-        // stream2 while the Lanczos propagation of A proceeds
-        propagate_spf_fork(c, pH, 0.5 * hdid);
-        const int nk1 = lanczos(c, B0, BH, 0.5 * hdid, true);
-        phase(c, 3);
-        propagate_spf_join(c);
-        phase(c, 2);
-        calc_rho(c, BH);
-        calc_h1mode(c, pH);
-        phase(c, 0);
-        calc_meanfield(c, BH, true);
-        phase(c, 1);
-        copy_d2d(c, pH2, psi, D);
-        propagate_spf(c, pH2, 0.5 * hdid);
-        phase(c, 2);
-        // DeltaPsi reads pH2 only through its SPF overlaps: once those are
-        // queued, the second SPF half step overlaps the A-vector contractions
-        delta_psi_overlaps(c, pH, pH2);
-        propagate_spf_fork(c, pH2, 0.5 * hdid);
-        const double err_psi = delta_psi_contract(c, BH, BH);
-        phase(c, 4);
-        propagate_spf_join(c);
-        phase(c, 2);
-        calc_h1mode(c, pH2);
-        phase(c, 0);
-        const int nk2 = lanczos(c, BH, B02, -0.5 * hdid, true);
-        phase(c, 3);
-        launch_axpby(B02, B0, -1.0, 1.0, N, 0, c->stream);
-        const double err_A = std::sqrt(norm2(c, B02, N)) * 0.5;
-        const double error = std::max(err_A, err_psi) / c->eps_int;
-        const bool reject = (forced >= 0) ? (forced == 0) : (error > 1.0);
-        if (log) log->push_back({tt, hdid, err_A, err_psi, (int64_t)(!reject), nk1, nk2});
-        note_attempt(c, hdid, error);
+        c->att_host->h = hdid;
+        c->att_host->forced = forced;
+        HIP_CHECK(hipMemcpyAsync(att_dev(c), c->att_host, 2 * sizeof(double), hipMemcpyHostToDevice, c->stream));
+        cmf2_attempt_run(c, kmax);
+        HIP_CHECK(hipStreamSynchronize(c->stream));
+        HIP_CHECK(hipStreamSynchronize(c->stream2));
+        const AttemptDev r = *c->att_host;
+        if (r.spf_status == 1) throw HipError{"SPF overlap matrix not positive definite"};
+        if (r.spf_status == 2) throw HipError{"SPF Runge-Kutta did not converge"};
+        if (r.overflow == 1) throw HipError{"Maximum number of Krylov vectors reached"};
+        if (r.overflow == 2) {
+            // the Krylov space outgrew the enqueued iterations: the attempt
+            // committed nothing; repeat it with the reference's limit
+            full = true;
+            c->replay_pos = rpos;
+            continue;
+        }
+        c->nk_seen = std::max(c->nk_seen, std::max(r.nk1, r.nk2));
+        const bool reject = !r.accept;
+        if (log) log->push_back({tt, hdid, r.err_A, r.err_psi, (int64_t)(!reject), r.nk1, r.nk2});
+        note_attempt(c, hdid, r.error);
         if (reject) {
-            hdid = hdid / std::sqrt(error) * safe;
+            hdid = hdid / std::sqrt(r.error) * safe;
             if (hdid < c->min_stepsize) throw HipError{"CMF2 step size fell below minstep"};
             continue;
         }
-        phase(c, 5);
-        lanczos(c, BH, B0, 0.5 * hdid, false);
-        phase(c, 3);
-        copy_d2d(c, A, B0, N);
-        copy_d2d(c, psi, pH2, D);
-        (void)maxscal;
-        *hnext_out = propose_next(c, hdid, error);
-        phase(c, 5);
+        *hnext_out = propose_next(c, hdid, r.error);
+        if (c->phases) phase(c, 5);
         break;
     }
     *hdid_out = hdid;
@@ -3013,6 +3208,10 @@ int mg_create(const mg_problem *prob, int64_t precision, mg_ctx **out) {
         *c->spf_status_host = 0;
         if (const char *e = std::getenv("MG_GRAPHS")) c->use_graphs = std::atoi(e) != 0;
         if (const char *e = std::getenv("MG_PHASES")) c->phases = std::atoi(e) != 0;
+        if (const char *e = std::getenv("MG_ATTEMPT_GRAPH")) c->att_graphs = std::atoi(e) != 0;
+        if (const char *e = std::getenv("MG_ATTEMPT_KMAX")) c->att_kmax_cap = std::max(2, std::atoi(e));
+        HIP_CHECK(hipHostMalloc((void **)&c->att_host, sizeof(AttemptDev), hipHostMallocDefault));
+        std::memset(c->att_host, 0, sizeof(AttemptDev));
         c->phase_t0 = std::chrono::steady_clock::now();
         allocate(c);
         c->stepsize = c->writetime * c->eps_int;  // propagator.f90:103
@@ -3041,6 +3240,8 @@ int mg_destroy(mg_ctx *c) {
     if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
     if (c->ev_join) (void)hipEventDestroy(c->ev_join);
     if (c->spf_status_host) (void)hipHostFree(c->spf_status_host);
+    if (c->att_exec) (void)hipGraphExecDestroy(c->att_exec);
+    if (c->att_host) (void)hipHostFree(c->att_host);
     hipStream_t s = c->stream, s2 = c->stream2;
     delete c;
     if (s) (void)hipStreamDestroy(s);
@@ -3741,6 +3942,7 @@ int mg_autotune(mg_ctx *c, int64_t reps, int64_t *best_brick, double *best_ms) {
         for (auto [b, rb] : cand) {
             HIP_CHECK(hipStreamSynchronize(c->stream));
             c->plans.clear();
+            c->graph_epoch++;          // an attempt graph holds the cleared plans' task tables
             c->brick = b;
             c->run_bytes = rb;
             make_groups(c);
@@ -3778,6 +3980,7 @@ int mg_autotune(mg_ctx *c, int64_t reps, int64_t *best_brick, double *best_ms) {
         }
         HIP_CHECK(hipStreamSynchronize(c->stream));
         c->plans.clear();
+        c->graph_epoch++;
         c->brick = bestb;
         c->run_bytes = bestr;
         make_groups(c);

@@ -211,6 +211,10 @@ struct SpfParams {
     int dormand_prince;
     double2 der_factor;
     int max_attempts;
+    // when set, the interval is dt_src[0] * dt_scale, read on the device (a
+    // CMF2 attempt replayed from a graph takes its step size from memory)
+    const double *dt_src;
+    double dt_scale;
 };
 
 #ifdef __HIPCC__
@@ -314,9 +318,12 @@ void launch_cscale(double2 *out, const double2 *x, double2 alpha, int64_t n, hip
 // Gram launches (ngram_total > 0) run a persistent grid of `gram_blocks`
 // workgroups over the bricks and write partials[(slot * 64 + e) * gram_blocks
 // + block] (e = i + 8 j); launch_gram_slots_final sums them into G[slot].
+// skip: when non-null and *skip is set on the device, every workgroup
+// returns at once (an iteration of a device-controlled Krylov build past its
+// convergence)
 void launch_brick_tasks(const BrickTask *d_tasks, int ntasks, int64_t max_blocks_per_task, int nmax,
                         int brick, int nbuf, int nmat, int prec, hipStream_t s, int ngram_total = 0,
-                        double2 *partials = nullptr, int gram_blocks = 0);
+                        double2 *partials = nullptr, int gram_blocks = 0, const int *skip = nullptr);
 void launch_gram_slots_final(const double2 *partials, int nslots, int nblocks, double2 *const *targets,
                              int ld, hipStream_t s);
 void launch_herm_batch(double2 *const *targets, const int *n, int count, int ld, hipStream_t s);
@@ -328,14 +335,16 @@ void launch_spf_tasks(const SpfTask *d_tasks, int ntasks, DofTables dt, int max_
 void launch_fill(double2 *p, int64_t n, double2 v, hipStream_t s);
 void launch_axpby(void *y, const void *x, double a, double b, int64_t n, int prec, hipStream_t s);
 void launch_dot(const void *x, const void *y, int64_t n, double *partials, int nblocks,
-                double2 *out, int prec, hipStream_t s);
+                double2 *out, int prec, hipStream_t s, const int *skip = nullptr);
 void launch_scale(void *x, const double *norm2, int64_t n, int prec, hipStream_t s);
 void launch_combine(void *out, const void *basis, int64_t stride, const double2 *coef, int k,
                     int64_t n, int prec, hipStream_t s);
 void launch_lanczos_step(void *w, const void *uj, const void *uprev, const double *dot_re, const double *nsq, int j,
-                         int64_t n, double *partials, int nblocks, double *nsq_next, int prec, hipStream_t s);
+                         int64_t n, double *partials, int nblocks, double *nsq_next, int prec, hipStream_t s,
+                         const int *skip = nullptr);
+// kdev: when non-null, the number of basis vectors is read on the device
 void launch_combine_v0(double2 *out, const double2 *v0, const void *basis, int64_t stride, const double2 *coef, int k,
-                       int64_t n, int prec, hipStream_t s);
+                       int64_t n, int prec, hipStream_t s, const int *kdev = nullptr);
 void launch_norm2(const void *x, int64_t n, double *partials, int nblocks, double *out, int prec,
                   hipStream_t s);
 void launch_convert(void *dst, int pdst, const void *src, int psrc, int64_t n, hipStream_t s);
@@ -356,5 +365,33 @@ void launch_der_spf(const SpfModeDesc *d_modes, int mode, const double2 *psi, do
 void launch_mf_accum(double2 *dst1, double2 *dst2, const double2 *G, int n1, int n2, int ld,
                      hipStream_t s);
 void launch_mat_axpy(double2 *dst, const double2 *src, double2 alpha, int n, int ld, hipStream_t s);
+
+// ---------------------------------------------------------------------------
+// Device-controlled CMF2 attempt (every decision of cmf2_base.f90:76-203 on
+// the device, so that an attempt is one HIP graph).
+// One Krylov build of the Lanczos integrator (cmf2_integration.f90:30-103).
+struct LanczosDev {
+    double alpha[kNKrylovMax], beta[kNKrylovMax], nrm[kNKrylovMax + 1];
+    double error;
+    int done;       // set once the error test passed (later iterations skip)
+    int nk;         // Krylov size
+    int overflow;   // 1: kNKrylovMax reached; 2: the enqueued iterations ran out
+    double w[kNKrylovMax], U[kNKrylovMax * kNKrylovMax], ln[kNKrylovMax];
+};
+// Inputs (h, forced decision) and results of one attempt.
+struct AttemptDev {
+    double h;       // attempted step (written by the host before the launch)
+    int forced;     // -1: the error test decides; 0 / 1: replayed reject / accept
+    double err_A, err_psi, error;
+    int accept, nk1, nk2, overflow, spf_status;
+};
+void launch_lan_begin(LanczosDev *L, hipStream_t s);
+void launch_lan_ctrl(LanczosDev *L, const AttemptDev *at, const double2 *dot, const double *nsq, int j,
+                     double dtscale, int kmax, hipStream_t s);
+void launch_lan_coef(LanczosDev *L, const AttemptDev *at, double dtscale, int build, double2 *coef, hipStream_t s);
+void launch_attempt_psi(AttemptDev *at, const double2 *o11, const double2 *o22, const double2 *o12, hipStream_t s);
+void launch_attempt_decide(AttemptDev *at, const LanczosDev *L1, const LanczosDev *L2, const double *n2,
+                           double eps_int, const int *spf_status, hipStream_t s);
+void launch_copy_if(double2 *dst, const double2 *src, int64_t n, const int *flag, hipStream_t s);
 
 }  // namespace mg

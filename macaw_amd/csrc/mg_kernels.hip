@@ -340,7 +340,8 @@ __device__ __forceinline__ int64_t xcd_block(int b, int G) {
 // the bricks and every workgroup ends by writing its Gram partials.
 template <int NMAX, class T, bool GRAM>
 __global__ void __launch_bounds__(GRAM ? 256 : 1024) k_brick(const BrickTask *__restrict__ tasks, int capacity, int nmat_lds,
-                                                double2 *__restrict__ partials) {
+                                                double2 *__restrict__ partials, const int *skip) {
+    if (!GRAM && skip && *skip) return;
     const BrickTask &t = tasks[blockIdx.y];
     const int64_t L = t.L, R = t.R;
     const int Lt = (int)t.Lt, Rt = (int)t.Rt;
@@ -443,7 +444,7 @@ __global__ void __launch_bounds__(GRAM ? 256 : 1024) k_brick(const BrickTask *__
 
 template <int NMAX, class T, bool GRAM>
 static void launch_brick_nmax(const BrickTask *d_tasks, int ntasks, int64_t max_blocks, int capacity, int nbuf,
-                              int nmat, hipStream_t s, double2 *partials) {
+                              int nmat, hipStream_t s, double2 *partials, const int *skip) {
     static const int nthreads = [] {
         const char *e = std::getenv("MG_THREADS");
         const int v = e ? std::atoi(e) : 256;
@@ -459,27 +460,30 @@ static void launch_brick_nmax(const BrickTask *d_tasks, int ntasks, int64_t max_
     }();
     (void)attr_set;
     hipLaunchKernelGGL((k_brick<NMAX, T, GRAM>), dim3((unsigned)max_blocks, (unsigned)ntasks), dim3(GRAM ? 256 : nthreads), shm,
-                       s, d_tasks, capacity, nmat, partials);
+                       s, d_tasks, capacity, nmat, partials, skip);
 }
 
 template <class T, bool GRAM>
 static void launch_brick_T(const BrickTask *d_tasks, int ntasks, int64_t mb, int nmax, int capacity, int nbuf,
-                           int nmat, hipStream_t s, double2 *partials) {
-    if (nmax <= 4) launch_brick_nmax<4, T, GRAM>(d_tasks, ntasks, mb, capacity, nbuf, nmat, s, partials);
-    else if (nmax <= 8) launch_brick_nmax<8, T, GRAM>(d_tasks, ntasks, mb, capacity, nbuf, nmat, s, partials);
-    else launch_brick_nmax<16, T, GRAM>(d_tasks, ntasks, mb, capacity, nbuf, nmat, s, partials);
+                           int nmat, hipStream_t s, double2 *partials, const int *skip) {
+    if (nmax <= 4) launch_brick_nmax<4, T, GRAM>(d_tasks, ntasks, mb, capacity, nbuf, nmat, s, partials, skip);
+    else if (nmax <= 8) launch_brick_nmax<8, T, GRAM>(d_tasks, ntasks, mb, capacity, nbuf, nmat, s, partials, skip);
+    else launch_brick_nmax<16, T, GRAM>(d_tasks, ntasks, mb, capacity, nbuf, nmat, s, partials, skip);
 }
 
 void launch_brick_tasks(const BrickTask *d_tasks, int ntasks, int64_t max_blocks_per_task, int nmax, int brick,
                         int nbuf, int nmat, int prec, hipStream_t s, int ngram_total, double2 *partials,
-                        int gram_blocks) {
+                        int gram_blocks, const int *skip) {
     if (ntasks <= 0 || max_blocks_per_task <= 0) return;
     const int capacity = group_capacity(brick);
     if (ngram_total > 0) {
+        // Gram launches always run (their partials feed a final reduction)
         if (prec == 0)
-            launch_brick_T<double2, true>(d_tasks, ntasks, gram_blocks, nmax, capacity, nbuf, nmat, s, partials);
+            launch_brick_T<double2, true>(d_tasks, ntasks, gram_blocks, nmax, capacity, nbuf, nmat, s, partials,
+                                          nullptr);
         else
-            launch_brick_T<float2, true>(d_tasks, ntasks, gram_blocks, nmax, capacity, nbuf, nmat, s, partials);
+            launch_brick_T<float2, true>(d_tasks, ntasks, gram_blocks, nmax, capacity, nbuf, nmat, s, partials,
+                                         nullptr);
         return;
     }
     // workgroups per task: at most 4096 (complex FP32) / 1024 (complex FP64),
@@ -493,9 +497,9 @@ void launch_brick_tasks(const BrickTask *d_tasks, int ntasks, int64_t max_blocks
     const int64_t grid_cap = grid_env >= 0 ? grid_env : (prec ? 4096 : 1024);
     const int64_t gx = grid_cap > 0 ? std::min(max_blocks_per_task, grid_cap) : max_blocks_per_task;
     if (prec == 0)
-        launch_brick_T<double2, false>(d_tasks, ntasks, gx, nmax, capacity, nbuf, nmat, s, nullptr);
+        launch_brick_T<double2, false>(d_tasks, ntasks, gx, nmax, capacity, nbuf, nmat, s, nullptr, skip);
     else
-        launch_brick_T<float2, false>(d_tasks, ntasks, gx, nmax, capacity, nbuf, nmat, s, nullptr);
+        launch_brick_T<float2, false>(d_tasks, ntasks, gx, nmax, capacity, nbuf, nmat, s, nullptr, skip);
 }
 
 // One wave per (slot, entry): the slot's block partials summed in a fixed
@@ -943,7 +947,9 @@ void launch_axpby(void *y, const void *x, double a, double b, int64_t n, int pre
 }
 
 template <class T>
-__global__ void __launch_bounds__(256) k_dot_partial(const T *x, const T *y, int64_t n, double *partials) {
+__global__ void __launch_bounds__(256) k_dot_partial(const T *x, const T *y, int64_t n, double *partials,
+                                                     const int *skip) {
+    if (skip && *skip) return;
     double sx = 0.0, sy = 0.0;
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
         const double2 a = to_d2(x[i]), b = to_d2(y[i]);
@@ -967,7 +973,9 @@ __global__ void __launch_bounds__(256) k_dot_partial(const T *x, const T *y, int
 }
 
 // One wave sums the block partials in a fixed order (deterministic).
-__global__ void __launch_bounds__(64) k_dot_final(const double *partials, int nblocks, double2 *out) {
+__global__ void __launch_bounds__(64) k_dot_final(const double *partials, int nblocks, double2 *out,
+                                                  const int *skip) {
+    if (skip && *skip) return;
     double ax = 0.0, ay = 0.0;
     for (int k = threadIdx.x; k < nblocks; k += 64) { ax += partials[2 * k]; ay += partials[2 * k + 1]; }
     ax = wave_sum(ax);
@@ -976,14 +984,14 @@ __global__ void __launch_bounds__(64) k_dot_final(const double *partials, int nb
 }
 
 void launch_dot(const void *x, const void *y, int64_t n, double *partials, int nblocks, double2 *out, int prec,
-                hipStream_t s) {
+                hipStream_t s, const int *skip) {
     if (prec == 0)
         hipLaunchKernelGGL(k_dot_partial<double2>, dim3(nblocks), dim3(256), 0, s, (const double2 *)x,
-                           (const double2 *)y, n, partials);
+                           (const double2 *)y, n, partials, skip);
     else
         hipLaunchKernelGGL(k_dot_partial<float2>, dim3(nblocks), dim3(256), 0, s, (const float2 *)x,
-                           (const float2 *)y, n, partials);
-    hipLaunchKernelGGL(k_dot_final, dim3(1), dim3(64), 0, s, partials, nblocks, out);
+                           (const float2 *)y, n, partials, skip);
+    hipLaunchKernelGGL(k_dot_final, dim3(1), dim3(64), 0, s, partials, nblocks, out, skip);
 }
 
 template <class T>
@@ -1005,7 +1013,9 @@ __global__ void __launch_bounds__(256) k_norm2_partial(const T *x, int64_t n, do
     }
 }
 
-__global__ void __launch_bounds__(64) k_norm2_final(const double *partials, int nblocks, double *out) {
+__global__ void __launch_bounds__(64) k_norm2_final(const double *partials, int nblocks, double *out,
+                                                    const int *skip = nullptr) {
+    if (skip && *skip) return;
     double a = 0.0;
     for (int k = threadIdx.x; k < nblocks; k += 64) a += partials[k];
     a = wave_sum(a);
@@ -1019,7 +1029,7 @@ void launch_norm2(const void *x, int64_t n, double *partials, int nblocks, doubl
     else
         hipLaunchKernelGGL(k_norm2_partial<float2>, dim3(nblocks), dim3(256), 0, s, (const float2 *)x, n,
                            partials);
-    hipLaunchKernelGGL(k_norm2_final, dim3(1), dim3(64), 0, s, partials, nblocks, out);
+    hipLaunchKernelGGL(k_norm2_final, dim3(1), dim3(64), 0, s, partials, nblocks, out, (const int *)nullptr);
 }
 
 template <class T>
@@ -1065,7 +1075,9 @@ void launch_combine(void *out, const void *basis, int64_t stride, const double2 
 // h; accumulates |u_{j+1}|^2 partials.  No normalisation pass is needed.
 template <class T>
 __global__ void __launch_bounds__(256) k_lanczos_step(T *w, const T *uj, const T *uprev, const double *dot_re,
-                                                      const double *nsq, int j, int64_t n, double *partials) {
+                                                      const double *nsq, int j, int64_t n, double *partials,
+                                                      const int *skip) {
+    if (skip && *skip) return;
     const double nj = sqrt(nsq[j]);
     const double a = 1.0 / nj, b = dot_re[0] / (nsq[j] * nj);
     const double cp = (j >= 1 && uprev) ? sqrt(nsq[j] / nsq[j - 1]) : 0.0;
@@ -1096,21 +1108,23 @@ __global__ void __launch_bounds__(256) k_lanczos_step(T *w, const T *uj, const T
 }
 
 void launch_lanczos_step(void *w, const void *uj, const void *uprev, const double *dot_re, const double *nsq, int j,
-                         int64_t n, double *partials, int nblocks, double *nsq_next, int prec, hipStream_t s) {
+                         int64_t n, double *partials, int nblocks, double *nsq_next, int prec, hipStream_t s,
+                         const int *skip) {
     if (prec == 0)
         hipLaunchKernelGGL(k_lanczos_step<double2>, dim3(nblocks), dim3(256), 0, s, (double2 *)w,
-                           (const double2 *)uj, (const double2 *)uprev, dot_re, nsq, j, n, partials);
+                           (const double2 *)uj, (const double2 *)uprev, dot_re, nsq, j, n, partials, skip);
     else
         hipLaunchKernelGGL(k_lanczos_step<float2>, dim3(nblocks), dim3(256), 0, s, (float2 *)w, (const float2 *)uj,
-                           (const float2 *)uprev, dot_re, nsq, j, n, partials);
-    hipLaunchKernelGGL(k_norm2_final, dim3(1), dim3(64), 0, s, partials, nblocks, nsq_next);
+                           (const float2 *)uprev, dot_re, nsq, j, n, partials, skip);
+    hipLaunchKernelGGL(k_norm2_final, dim3(1), dim3(64), 0, s, partials, nblocks, nsq_next, skip);
 }
 
 // out = coef[0] v0 (FP64) + sum_{j>=1} coef[j] basis_j (storage type T, the
 // j-th vector at basis + j * stride).
 template <class T>
 __global__ void k_combine_v0(double2 *out, const double2 *v0, const T *basis, int64_t stride, const double2 *coef,
-                             int k, int64_t n) {
+                             int k, int64_t n, const int *kdev) {
+    if (kdev) k = *kdev;
     __shared__ double2 c[kNKrylovMax];
     if (threadIdx.x < k) c[threadIdx.x] = coef[threadIdx.x];
     __syncthreads();
@@ -1122,12 +1136,14 @@ __global__ void k_combine_v0(double2 *out, const double2 *v0, const T *basis, in
 }
 
 void launch_combine_v0(double2 *out, const double2 *v0, const void *basis, int64_t stride, const double2 *coef, int k,
-                       int64_t n, int prec, hipStream_t s) {
+                       int64_t n, int prec, hipStream_t s, const int *kdev) {
     const dim3 g((unsigned)grid_for(n)), b(256);
     if (prec == 0)
-        hipLaunchKernelGGL(k_combine_v0<double2>, g, b, 0, s, out, v0, (const double2 *)basis, stride, coef, k, n);
+        hipLaunchKernelGGL(k_combine_v0<double2>, g, b, 0, s, out, v0, (const double2 *)basis, stride, coef, k, n,
+                           kdev);
     else
-        hipLaunchKernelGGL(k_combine_v0<float2>, g, b, 0, s, out, v0, (const float2 *)basis, stride, coef, k, n);
+        hipLaunchKernelGGL(k_combine_v0<float2>, g, b, 0, s, out, v0, (const float2 *)basis, stride, coef, k, n,
+                           kdev);
 }
 
 // Precision conversion of complex vectors / matrices.
@@ -1509,7 +1525,8 @@ __global__ void __launch_bounds__(256) k_spf_propagate(const SpfModeDesc *__rest
     const double safety = 0.9, pgrow = -0.2, pshrnk = -0.25, minscal = 0.1, maxscal = 5.0;
     const double errcon = pow(5.0 / safety, 1.0 / pgrow);
     const bool dp = P.dormand_prince != 0;
-    double dt_done = 0.0, stepsize = P.dt;
+    const double DT = P.dt_src ? P.dt_src[0] * P.dt_scale : P.dt;
+    double dt_done = 0.0, stepsize = DT;
     int attempts = 0;
     for (;;) {
         // ---- one adaptive RK step (rk_SPF_ck / rk_SPF_dp) ----
@@ -1573,10 +1590,10 @@ __global__ void __launch_bounds__(256) k_spf_propagate(const SpfModeDesc *__rest
         __syncthreads();
         // ---- step landing (propagate_SPF) ----
         dt_done = dt_done + h;
-        if (dt_done >= P.dt) break;
+        if (dt_done >= DT) break;
         if (hnext < P.min_stepsize * 2) hnext = P.min_stepsize * 2;
-        stepsize = (P.dt - dt_done) / (double)((long long)((P.dt - dt_done) / hnext) + 1);
-        if (stepsize > P.dt - dt_done) stepsize = P.dt - dt_done;
+        stepsize = (DT - dt_done) / (double)((long long)((DT - dt_done) / hnext) + 1);
+        if (stepsize > DT - dt_done) stepsize = DT - dt_done;
     }
     __syncthreads();
     if (tid == 0 && (fail || attempts > P.max_attempts)) atomicMax(status, fail ? 1 : 2);
@@ -1626,6 +1643,166 @@ void launch_der_spf(const SpfModeDesc *d_modes, int mode, const double2 *psi, do
 void launch_spf_propagate(const SpfModeDesc *d_modes, int nmodes, double2 *psi, double2 *scratch, SpfParams p,
                           DofTables dt, int *d_status, hipStream_t s) {
     hipLaunchKernelGGL(k_spf_propagate, dim3(nmodes), dim3(256), 0, s, d_modes, psi, scratch, p, dt, d_status);
+}
+
+
+// ---------------------------------------------------------------------------
+// Device-controlled CMF2 attempt: the Lanczos stopping test, the propagator
+// coefficients and the accept/reject decision on the device (one thread
+// each), so that a whole attempt is one HIP graph.
+__global__ void k_lan_begin(LanczosDev *L) {
+    L->done = 0;
+    L->nk = 0;
+    L->overflow = 0;
+    L->error = 0.0;
+}
+
+void launch_lan_begin(LanczosDev *L, hipStream_t s) { hipLaunchKernelGGL(k_lan_begin, dim3(1), dim3(1), 0, s, L); }
+
+// After the H-image of basis vector j - 1 (cmf2_integration.f90:56-84):
+// alpha_{j-1} = <u|Hu> / n_{j-1}^2, beta_{j-1} = n_j (the vectors are kept
+// unnormalised, u_j = n_j v_j), error *= beta_{j-1} / j |dt| from j = 2 on;
+// the build stops at the first j >= 2 with error < eps_B = 1e-8.  kmax is
+// the last iteration enqueued: reaching it unconverged sets overflow (1 if
+// it is the reference's own limit kNKrylovMax - 1, else 2).
+__global__ void k_lan_ctrl(LanczosDev *L, const AttemptDev *at, const double2 *dot, const double *nsq, int j,
+                           double dtscale, int kmax) {
+    if (L->done) return;
+    const double dt = fabs(at->h * dtscale);
+    L->nrm[j - 1] = nsq[j - 1];
+    L->nrm[j] = nsq[j];
+    L->alpha[j - 1] = dot->x / nsq[j - 1];
+    L->beta[j - 1] = sqrt(nsq[j]);
+    if (j == 1) {
+        L->error = L->beta[0] * dt;
+        return;
+    }
+    L->error = L->error * L->beta[j - 1] / (double)j * dt;
+    if (L->error < 1e-8) {
+        L->done = 1;
+        L->nk = j;
+    } else if (j >= kmax) {
+        L->done = 1;
+        L->nk = j;
+        L->overflow = (kmax >= kNKrylovMax - 1) ? 1 : 2;
+    }
+}
+
+void launch_lan_ctrl(LanczosDev *L, const AttemptDev *at, const double2 *dot, const double *nsq, int j,
+                     double dtscale, int kmax, hipStream_t s) {
+    hipLaunchKernelGGL(k_lan_ctrl, dim3(1), dim3(1), 0, s, L, at, dot, nsq, j, dtscale, kmax);
+}
+
+// Eigen-decomposition of the Krylov tridiagonal (cyclic Jacobi, the host
+// routine's rotations) when build is set, then the coefficients of
+// B(t + dt) = sum_i coef_i u_i: coef_i = sum_r U_ir U_0r exp(-i w_r dt) / n_i
+// (cmf2_integration.f90:86-97).
+__global__ void k_lan_coef(LanczosDev *L, const AttemptDev *at, double dtscale, int build, double2 *coef) {
+    const int n = L->nk;
+    if (build) {
+        double A[kNKrylovMax * kNKrylovMax];
+        double *U = L->U;
+        for (int i = 0; i < n * n; i++) A[i] = 0.0;
+        for (int i = 0; i < n; i++) {
+            A[i + n * i] = L->alpha[i];
+            if (i + 1 < n) { A[i + n * (i + 1)] = L->beta[i]; A[(i + 1) + n * i] = L->beta[i]; }
+        }
+        for (int i = 0; i < n; i++)
+            for (int jj = 0; jj < n; jj++) U[i + n * jj] = (i == jj) ? 1.0 : 0.0;
+        for (int sweep = 0; sweep < 100; sweep++) {
+            double off = 0.0, dg = 0.0;
+            for (int jj = 0; jj < n; jj++)
+                for (int i = 0; i < n; i++) {
+                    const double v = A[i + n * jj] * A[i + n * jj];
+                    if (i != jj) off += v; else dg += v;
+                }
+            if (off <= 1e-32 * (dg + 1e-300) || off == 0.0) break;
+            for (int p = 0; p < n - 1; p++)
+                for (int q = p + 1; q < n; q++) {
+                    const double apq = A[p + n * q];
+                    if (apq == 0.0) continue;
+                    const double theta = (A[q + n * q] - A[p + n * p]) / (2.0 * apq);
+                    const double t = (theta >= 0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
+                    const double cc = 1.0 / sqrt(t * t + 1.0), ss = t * cc;
+                    for (int k = 0; k < n; k++) {
+                        const double akp = A[k + n * p], akq = A[k + n * q];
+                        A[k + n * p] = cc * akp - ss * akq;
+                        A[k + n * q] = ss * akp + cc * akq;
+                    }
+                    for (int k = 0; k < n; k++) {
+                        const double apk = A[p + n * k], aqk = A[q + n * k];
+                        A[p + n * k] = cc * apk - ss * aqk;
+                        A[q + n * k] = ss * apk + cc * aqk;
+                    }
+                    A[p + n * q] = A[q + n * p] = 0.0;
+                    for (int k = 0; k < n; k++) {
+                        const double ukp = U[k + n * p], ukq = U[k + n * q];
+                        U[k + n * p] = cc * ukp - ss * ukq;
+                        U[k + n * q] = ss * ukp + cc * ukq;
+                    }
+                }
+        }
+        for (int i = 0; i < n; i++) {
+            L->w[i] = A[i + n * i];
+            L->ln[i] = sqrt(L->nrm[i]);
+        }
+    }
+    const double dt = at->h * dtscale;
+    for (int i = 0; i < n; i++) {
+        double re = 0.0, im = 0.0;
+        for (int r = 0; r < n; r++) {
+            double sn, cs;
+            sincos(-L->w[r] * dt, &sn, &cs);
+            const double u = L->U[i + n * r] * L->U[0 + n * r];
+            re += u * cs;
+            im += u * sn;
+        }
+        coef[i] = make_double2(re / L->ln[i], im / L->ln[i]);
+    }
+}
+
+void launch_lan_coef(LanczosDev *L, const AttemptDev *at, double dtscale, int build, double2 *coef, hipStream_t s) {
+    hipLaunchKernelGGL(k_lan_coef, dim3(1), dim3(1), 0, s, L, at, dtscale, build, coef);
+}
+
+// DeltaPsi (gmctdh_objects.f90:204-227) from the three overlaps.
+__global__ void k_attempt_psi(AttemptDev *at, const double2 *o11, const double2 *o22, const double2 *o12) {
+    at->err_psi = sqrt(fabs(2.0 * (1.0 - o12->x / sqrt(o11->x * o22->x))));
+}
+
+void launch_attempt_psi(AttemptDev *at, const double2 *o11, const double2 *o22, const double2 *o12, hipStream_t s) {
+    hipLaunchKernelGGL(k_attempt_psi, dim3(1), dim3(1), 0, s, at, o11, o22, o12);
+}
+
+// err_A = |B02 - B0| / 2, error = max(err_A, err_psi) / eps_int, rejected
+// when error > 1 (cmf2_base.f90:177-184) or as replayed; an unfinished
+// Krylov build always rejects (the host then reports or repeats it).
+__global__ void k_attempt_decide(AttemptDev *at, const LanczosDev *L1, const LanczosDev *L2, const double *n2,
+                                 double eps_int, const int *spf_status) {
+    at->err_A = sqrt(n2[0]) * 0.5;
+    at->error = fmax(at->err_A, at->err_psi) / eps_int;
+    bool reject = (at->forced >= 0) ? (at->forced == 0) : (at->error > 1.0);
+    at->overflow = max(L1->overflow, L2->overflow);
+    at->spf_status = spf_status[0];
+    if (at->overflow || at->spf_status) reject = true;
+    at->accept = reject ? 0 : 1;
+    at->nk1 = L1->nk;
+    at->nk2 = L2->nk;
+}
+
+void launch_attempt_decide(AttemptDev *at, const LanczosDev *L1, const LanczosDev *L2, const double *n2,
+                           double eps_int, const int *spf_status, hipStream_t s) {
+    hipLaunchKernelGGL(k_attempt_decide, dim3(1), dim3(1), 0, s, at, L1, L2, n2, eps_int, spf_status);
+}
+
+__global__ void k_copy_if(double2 *dst, const double2 *src, int64_t n, const int *flag) {
+    if (!*flag) return;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
+void launch_copy_if(double2 *dst, const double2 *src, int64_t n, const int *flag, hipStream_t s) {
+    hipLaunchKernelGGL(k_copy_if, dim3((unsigned)grid_for(n)), dim3(256), 0, s, dst, src, n, flag);
 }
 
 }  // namespace mg

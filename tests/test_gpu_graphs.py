@@ -49,3 +49,22 @@ def test_propagation_with_graphs_is_bit_identical(name, precision, tf_fs, tmp_pa
     for a, b in zip(og, oe):
         assert (a.auto_re, a.auto_im, a.norm, a.energy) == (b.auto_re, b.auto_im, b.norm, b.energy)
     assert np.array_equal(pg, pe)
+
+
+@gpu
+@pytest.mark.parametrize("precision", [0, 1])
+def test_krylov_build_outgrowing_the_graph_is_repeated(precision, tmp_path, monkeypatch):
+    """A CMF2 attempt runs as one graph with a fixed number of Lanczos
+    iterations per Krylov build; a build that needs more is repeated with the
+    reference's limit.  Capping the graph at 2 iterations forces that on
+    every attempt: the trajectory equals the uncapped one bit for bit."""
+    prob = make_problem("t5", str(tmp_path))
+    tf = 1.0 / AU2FS
+    oa, pa, aa = engine(prob, precision, True).propagate(0.0, tf)
+    monkeypatch.setenv("MG_ATTEMPT_KMAX", "2")
+    ob, pb, ab = engine(prob, precision, True).propagate(0.0, tf)
+    assert [(a.h, a.accepted, a.nkrylov1, a.nkrylov2) for a in aa] == \
+        [(a.h, a.accepted, a.nkrylov1, a.nkrylov2) for a in ab]
+    for a, b in zip(oa, ob):
+        assert (a.auto_re, a.auto_im, a.norm, a.energy) == (b.auto_re, b.auto_im, b.norm, b.energy)
+    assert np.array_equal(pa, pb)
