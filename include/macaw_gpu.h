@@ -243,9 +243,14 @@ int mg_ensemble_propagate(const mg_problem *const *probs, const double *const *A
 
 /* HIP graphs (no counterpart in the reference): every cached launch plan
  * (H.A schedule, mean-field, density-matrix and Kronecker-chain brick
- * programs) is captured into a HIP graph on its second use and replayed from
- * then on; enable = 0 launches kernel by kernel (default 1; the environment
- * variable MG_GRAPHS=0 sets the default off). */
+ * programs) is captured into its own graph on its second use and replayed
+ * from then on; enable = 0 launches kernel by kernel (default 1; the
+ * environment variable MG_GRAPHS=0 sets the default off).  With
+ * MG_ATTEMPT_GRAPH=1 a whole CMF2 attempt (every decision on the device) is
+ * captured into one graph instead; off by default because its replay
+ * serialises the SPF half steps that otherwise overlap the Krylov builds on
+ * a second stream, which made it 11% slower on a small problem and no faster
+ * on the others measured.  Results are bit-identical either way. */
 int mg_set_graphs(mg_ctx *ctx, int64_t enable);
 
 #define MG_MLP_NFEAT 12

@@ -214,9 +214,6 @@ struct mg_ctx {
     // cached launch plans keyed by the buffers they read and write
     std::map<PlanKey, std::unique_ptr<Plan>> plans;
     std::map<PlanKey, std::unique_ptr<SpfPlan>> spf_plans;
-    std::vector<double> lan_alpha, lan_U, lan_n;   // Ritz values, vectors, Krylov norms
-    const double2 *lan_v0 = nullptr;               // u_0 of the last built space (FP64)
-    int nkrylov = 0;
     // ---- Gaussian (G-MCTDH) modes ----
     int ngwp = 0;
     std::vector<int> gwp;                  // [nm] 1 = Gaussian mode
@@ -243,8 +240,9 @@ struct mg_ctx {
     hipGraphExec_t att_exec = nullptr; // one whole attempt, captured on the second attempt
     int att_epoch = -1, att_kmax = 0, att_runs = 0;
     int nk_seen = 2;                   // largest Krylov size of the graph-run attempts so far
-    bool att_graphs = true;            // MG_ATTEMPT_GRAPH=0: the attempt runs eagerly
+    bool att_graphs = false;           // MG_ATTEMPT_GRAPH=1: the attempt as one graph (slower, see mg_set_graphs)
     int att_kmax_cap = kNKrylovMax;    // MG_ATTEMPT_KMAX (test aid): fewer iterations per graph-run build
+    bool att_fork = true;              // MG_ATTEMPT_FORK=0: the SPF half steps on the engine stream
     bool capturing = false;            // an attempt is being captured: plans launch their kernels
     const int *plan_skip = nullptr;    // skip flag handed to the brick launches being enqueued
     // ---- learned step controller ----
@@ -1551,6 +1549,8 @@ GwpParams gwp_params(mg_ctx *c, double dt = 0.0) {
     P.dormand_prince = (c->int_type_psi == 2);
     P.der_factor = make_double2(0.0, -1.0);
     P.max_attempts = 200000;
+    P.dt_src = nullptr;
+    P.dt_scale = 1.0;
     return P;
 }
 
@@ -2257,25 +2257,6 @@ SpfParams spf_params(mg_ctx *c, double dt) {
     return P;
 }
 
-// Launch on stream `st`; the status word lands in pinned host memory and is
-// checked by spf_check once `st` has been synchronised.
-// Grid modes through k_spf_propagate; Gaussian modes (with their D matrices
-// dd) through k_gwp_propagate, cmf2_base.f90:117-124 (grid modes first).
-void propagate_spf_launch(mg_ctx *c, double2 *psi, double dt, hipStream_t st, double2 *dd = nullptr) {
-    SpfParams P = spf_params(c, dt);
-    launch_fill(reinterpret_cast<double2 *>(c->dstatus.p), 1, make_double2(0, 0), st);
-    if (c->ndvr_desc > 0)
-        launch_spf_propagate(c->dspfmodes.as<SpfModeDesc>(), c->ndvr_desc, psi, c->dspfscr.as<double2>(), P,
-                             dof_tables(c), c->dstatus.as<int>(), st);
-    if (c->ngwp) {
-        if (!dd) throw HipError{"Gaussian-mode propagation without D matrices"};
-        launch_gwp_propagate(c->dgwpdesc.as<GwpDesc>(), c->ngwp_desc, psi, dd, c->dgwpscr.as<double2>(),
-                             gwp_params(c, dt), c->dstatus.as<int>(), st);
-    }
-    check_launch();
-    HIP_CHECK(hipMemcpyAsync(c->spf_status_host, c->dstatus.p, sizeof(int), hipMemcpyDeviceToHost, st));
-}
-
 void spf_check(mg_ctx *c) {
     const int st = *c->spf_status_host;
     if (st == 1) throw HipError{"SPF overlap matrix not positive definite"};
@@ -2284,157 +2265,7 @@ void spf_check(mg_ctx *c) {
     if (st == 4) throw HipError{"operator not supported on a Gaussian mode"};
 }
 
-void propagate_spf(mg_ctx *c, double2 *psi, double dt, double2 *dd = nullptr) {
-    propagate_spf_launch(c, psi, dt, c->stream, dd);
-    HIP_CHECK(hipStreamSynchronize(c->stream));
-    spf_check(c);
-}
-
-// Start an SPF propagation on stream2 after everything queued on stream so far.
-void propagate_spf_fork(mg_ctx *c, double2 *psi, double dt, double2 *dd = nullptr) {
-    HIP_CHECK(hipEventRecord(c->ev_fork, c->stream));
-    HIP_CHECK(hipStreamWaitEvent(c->stream2, c->ev_fork, 0));
-    propagate_spf_launch(c, psi, dt, c->stream2, dd);
-    HIP_CHECK(hipEventRecord(c->ev_join, c->stream2));
-}
-
-// Make stream wait for the forked propagation and check its status.
-void propagate_spf_join(mg_ctx *c) {
-    HIP_CHECK(hipStreamWaitEvent(c->stream, c->ev_join, 0));
-    HIP_CHECK(hipEventSynchronize(c->ev_join));
-    spf_check(c);
-}
-
-// ----------------------------------------------------------------------------
-// Lanczos (cmf2_integration.f90:30-103).  Returns nKrylov.
-void tridiag_eig(int n, const double *alpha, const double *beta, double *w, double *U) {
-    std::vector<double> A((size_t)n * n, 0.0);
-    for (int i = 0; i < n; i++) {
-        A[i + n * i] = alpha[i];
-        if (i + 1 < n) { A[i + n * (i + 1)] = beta[i]; A[(i + 1) + n * i] = beta[i]; }
-    }
-    for (int i = 0; i < n; i++)
-        for (int j = 0; j < n; j++) U[i + n * j] = (i == j);
-    for (int sweep = 0; sweep < 100; sweep++) {
-        double off = 0.0, dg = 0.0;
-        for (int j = 0; j < n; j++)
-            for (int i = 0; i < n; i++) (i != j ? off : dg) += A[i + n * j] * A[i + n * j];
-        if (off <= 1e-32 * (dg + 1e-300) || off == 0.0) break;
-        for (int p = 0; p < n - 1; p++)
-            for (int q = p + 1; q < n; q++) {
-                const double apq = A[p + n * q];
-                if (apq == 0.0) continue;
-                const double theta = (A[q + n * q] - A[p + n * p]) / (2.0 * apq);
-                const double t = (theta >= 0 ? 1.0 : -1.0) / (std::fabs(theta) + std::sqrt(theta * theta + 1.0));
-                const double cc = 1.0 / std::sqrt(t * t + 1.0), ss = t * cc;
-                for (int k = 0; k < n; k++) {
-                    const double akp = A[k + n * p], akq = A[k + n * q];
-                    A[k + n * p] = cc * akp - ss * akq;
-                    A[k + n * q] = ss * akp + cc * akq;
-                }
-                for (int k = 0; k < n; k++) {
-                    const double apk = A[p + n * k], aqk = A[q + n * k];
-                    A[p + n * k] = cc * apk - ss * aqk;
-                    A[q + n * k] = ss * apk + cc * aqk;
-                }
-                A[p + n * q] = A[q + n * p] = 0.0;
-                for (int k = 0; k < n; k++) {
-                    const double ukp = U[k + n * p], ukq = U[k + n * q];
-                    U[k + n * p] = cc * ukp - ss * ukq;
-                    U[k + n * q] = ss * ukp + cc * ukq;
-                }
-            }
-    }
-    for (int i = 0; i < n; i++) w[i] = A[i + n * i];
-}
-
-// B0 and BT are FP64.  The Krylov vectors are kept unnormalised, u_j = n_j v_j
-// with n_j^2 in device scalars (no normalisation pass; the 1/n factors enter
-// the recurrence kernel and the final coefficients).  u_0 is B0 itself (FP64,
-// remembered for the combination); H.A reads it directly in the FP64 form and
-// through an FP32 copy in the mixed form, where u_1.. are FP32 too, so FP32
-// rounding enters the propagated vector only through the coefficients c_1..
-// (|c_i| ~ (beta dt)^i), not through the O(1) coefficient of v_0.
-int lanczos(mg_ctx *c, const double2 *B0, double2 *BT, double dt, bool build) {
-    const int64_t N = c->nconf;
-    const int pr = c->prec;
-    void *BK = c->dBK.p;
-    double *dsc = c->dscal.as<double>();
-    double *nsq = dsc + 24;          // n_j^2, j < kNKrylovMax + 1
-    double *part = c->dpartials.as<double>();
-    const double eps_B = 1e-8;
-    if (build) {
-        double alpha[kNKrylovMax], beta[kNKrylovMax], nrm[kNKrylovMax + 1];
-        c->lan_v0 = B0;
-        launch_norm2(B0, N, part, c->nblocks_red, nsq + 0, 0, c->stream);
-        if (pr) launch_convert(BK, 1, B0, 0, N, c->stream);
-        const void *u0 = pr ? BK : (const void *)B0;
-        int j = 1;      // number of basis vectors whose H-image is being formed
-        double error = 0.0;
-        for (;;) {
-            const void *uj = (j == 1) ? u0 : c->at(BK, N * (j - 1));
-            void *w = c->at(BK, N * j);
-            const void *up = (j >= 2) ? ((j == 2) ? u0 : c->at(BK, N * (j - 2))) : nullptr;
-            if (j >= 2 && j == kNKrylovMax) throw HipError{"Maximum number of Krylov vectors reached"};
-            double2 *dot_d = reinterpret_cast<double2 *>(dsc + 2);
-            if (j == 1 && pr && c->lan64) {
-                // u_1 = H u_0 - alpha u_0 in FP64 from the FP64 start vector,
-                // then stored as the FP32 Krylov vector
-                double2 *w64 = c->dW64.as<double2>();
-                apply_h64(c, B0, w64);
-                launch_dot(B0, w64, N, part, c->nblocks_red, dot_d, 0, c->stream);
-                launch_lanczos_step(w64, B0, nullptr, dsc + 2, nsq, 0, N, part, c->nblocks_red, nsq + 1, 0,
-                                    c->stream);
-                launch_convert(w, 1, w64, 0, N, c->stream);
-            } else {
-                apply_h(c, uj, w);
-                launch_dot(uj, w, N, part, c->nblocks_red, dot_d, pr, c->stream);
-                launch_lanczos_step(w, uj, up, dsc + 2, nsq, j - 1, N, part, c->nblocks_red, nsq + j, pr,
-                                    c->stream);
-            }
-            double h[2];
-            HIP_CHECK(hipMemcpyAsync(h, dsc + 2, sizeof(double), hipMemcpyDeviceToHost, c->stream));
-            HIP_CHECK(hipMemcpyAsync(nrm + j - 1, nsq + j - 1, sizeof(double) * 2, hipMemcpyDeviceToHost, c->stream));
-            HIP_CHECK(hipStreamSynchronize(c->stream));
-            alpha[j - 1] = h[0] / nrm[j - 1];
-            beta[j - 1] = std::sqrt(nrm[j]);
-            if (j == 1) {
-                error = beta[0] * std::fabs(dt);
-                j = 2;
-                continue;
-            }
-            error = error * beta[j - 1] / (double)j * std::fabs(dt);
-            if (error < eps_B) break;
-            j++;
-        }
-        c->nkrylov = j;
-        c->lan_alpha.assign(j, 0.0);
-        c->lan_U.assign((size_t)j * j, 0.0);
-        c->lan_n.assign(j, 0.0);
-        for (int i = 0; i < j; i++) c->lan_n[i] = std::sqrt(nrm[i]);
-        std::vector<double> w(j);
-        tridiag_eig(j, alpha, beta, w.data(), c->lan_U.data());
-        for (int r = 0; r < j; r++) c->lan_alpha[r] = w[r];
-    }
-    const int n = c->nkrylov;
-    std::vector<double2> coef(n);
-    for (int i = 0; i < n; i++) {
-        cplx acc = 0;
-        for (int r = 0; r < n; r++)
-            acc += c->lan_U[i + n * r] * c->lan_U[0 + n * r] * std::exp(cplx(0.0, -1.0) * c->lan_alpha[r] * dt);
-        coef[i] = d2(acc / c->lan_n[i]);     // v_i = u_i / n_i
-    }
-    double2 *dcoef = reinterpret_cast<double2 *>(dsc + 48);
-    HIP_CHECK(hipMemcpyAsync(dcoef, coef.data(), sizeof(double2) * n, hipMemcpyHostToDevice, c->stream));
-    launch_combine_v0(BT, c->lan_v0, BK, N, dcoef, n, N, pr, c->stream);
-    launch_norm2(BT, N, part, c->nblocks_red, dsc + 6, 0, c->stream);
-    launch_scale(BT, dsc + 6, N, 0, c->stream);
-    HIP_CHECK(hipStreamSynchronize(c->stream));
-    return n;
-}
-
-// ----------------------------------------------------------------------------
-// cmf2 for DVR-only wavefunctions (cmf2_base.f90:47-206).
+// Device-to-device copy of n complex FP64 elements on the engine stream.
 void copy_d2d(mg_ctx *c, double2 *dst, const double2 *src, int64_t n) {
     HIP_CHECK(hipMemcpyAsync(dst, src, sizeof(double2) * n, hipMemcpyDeviceToDevice, c->stream));
 }
@@ -2538,8 +2369,6 @@ double propose_next(mg_ctx *c, double hdid, double error) {
     return std::min(hn, c->writetime);
 }
 
-void cmf2_step_gwp(mg_ctx *c, double tt, double htry, double *hdid_out, double *hnext_out,
-                   std::vector<mg_attempt> *log);
 
 // Phase buckets of MG_PHASES: 0 rho + h1mode, 1 mean fields, 2 SPF
 // propagation (the synchronous one), 3 Lanczos, 4 DeltaPsi, 5 copies,
@@ -2635,13 +2464,39 @@ void delta_psi_contract_dev(mg_ctx *c, const double2 *A1) {
     launch_attempt_psi(att_dev(c), o + 0, o + 1, o + 2, c->stream);
 }
 
-// SPF half step over dtscale * h (h on the device) on stream st.
-void propagate_spf_dev(mg_ctx *c, double2 *psi, double dtscale, hipStream_t st) {
-    SpfParams P = spf_params(c, 0.0);
-    P.dt_src = &att_dev(c)->h;
-    P.dt_scale = dtscale;
-    launch_spf_propagate(c->dspfmodes.as<SpfModeDesc>(), c->ndvr_desc, psi, c->dspfscr.as<double2>(), P,
-                         dof_tables(c), c->dstatus.as<int>(), st);
+// SPF half step over dtscale * h (h on the device) on stream st; Gaussian
+// modes carry their D matrices dd (cmf2_base.f90:110-124).
+void propagate_spf_dev(mg_ctx *c, double2 *psi, double dtscale, hipStream_t st, double2 *dd = nullptr) {
+    if (c->ndvr_desc > 0) {
+        SpfParams P = spf_params(c, 0.0);
+        P.dt_src = &att_dev(c)->h;
+        P.dt_scale = dtscale;
+        launch_spf_propagate(c->dspfmodes.as<SpfModeDesc>(), c->ndvr_desc, psi, c->dspfscr.as<double2>(), P,
+                             dof_tables(c), c->dstatus.as<int>(), st);
+    }
+    if (c->ngwp) {
+        if (!dd) throw HipError{"Gaussian-mode propagation without D matrices"};
+        GwpParams G = gwp_params(c, 0.0);
+        G.dt_src = &att_dev(c)->h;
+        G.dt_scale = dtscale;
+        launch_gwp_propagate(c->dgwpdesc.as<GwpDesc>(), c->ngwp_desc, psi, dd, c->dgwpscr.as<double2>(), G,
+                             c->dstatus.as<int>(), st);
+    }
+}
+
+// Second stream for an SPF half step that overlaps the A-vector work after
+// everything queued so far (the stream itself when MG_ATTEMPT_FORK=0), and
+// the join back.
+hipStream_t attempt_fork(mg_ctx *c) {
+    if (!c->att_fork) return c->stream;
+    HIP_CHECK(hipEventRecord(c->ev_fork, c->stream));
+    HIP_CHECK(hipStreamWaitEvent(c->stream2, c->ev_fork, 0));
+    return c->stream2;
+}
+void attempt_join(mg_ctx *c) {
+    if (!c->att_fork) return;
+    HIP_CHECK(hipEventRecord(c->ev_join, c->stream2));
+    HIP_CHECK(hipStreamWaitEvent(c->stream, c->ev_join, 0));
 }
 
 // Everything of one attempt, in stream order (graph-safe: no host sync, no
@@ -2663,13 +2518,10 @@ void cmf2_attempt_enqueue(mg_ctx *c, int kmax) {
     copy_d2d(c, pH, psi, D);
     // the SPF half step needs only rho(A) and the mean fields: stream2, beside
     // the Lanczos propagation of A
-    HIP_CHECK(hipEventRecord(c->ev_fork, c->stream));
-    HIP_CHECK(hipStreamWaitEvent(c->stream2, c->ev_fork, 0));
-    propagate_spf_dev(c, pH, 0.5, c->stream2);
-    HIP_CHECK(hipEventRecord(c->ev_join, c->stream2));
+    propagate_spf_dev(c, pH, 0.5, attempt_fork(c));
     lanczos_dev(c, B0, BH, L1, 0.5, true, kmax);
     phase(c, 3);
-    HIP_CHECK(hipStreamWaitEvent(c->stream, c->ev_join, 0));
+    attempt_join(c);
     phase(c, 2);
     calc_rho(c, BH);
     calc_h1mode(c, pH);
@@ -2682,21 +2534,17 @@ void cmf2_attempt_enqueue(mg_ctx *c, int kmax) {
     // DeltaPsi reads pH2 only through its SPF overlaps: once those are
     // queued, the second SPF half step overlaps the A-vector contractions
     delta_psi_overlaps(c, pH, pH2);
-    HIP_CHECK(hipEventRecord(c->ev_fork, c->stream));
-    HIP_CHECK(hipStreamWaitEvent(c->stream2, c->ev_fork, 0));
-    propagate_spf_dev(c, pH2, 0.5, c->stream2);
-    HIP_CHECK(hipEventRecord(c->ev_join, c->stream2));
+    propagate_spf_dev(c, pH2, 0.5, attempt_fork(c));
     delta_psi_contract_dev(c, BH);
     phase(c, 4);
-    HIP_CHECK(hipStreamWaitEvent(c->stream, c->ev_join, 0));
+    attempt_join(c);
     phase(c, 2);
     calc_h1mode(c, pH2);
     phase(c, 0);
     lanczos_dev(c, BH, B02, L2, -0.5, true, kmax);
     phase(c, 3);
-    launch_axpby(B02, B0, -1.0, 1.0, N, 0, c->stream);
     double *n2 = c->dscal.as<double>() + 12;
-    launch_norm2(B02, N, c->dpartials.as<double>(), c->nblocks_red, n2, 0, c->stream);
+    launch_diff_norm2(B02, B0, N, c->dpartials.as<double>(), c->nblocks_red, n2, c->stream);
     launch_attempt_decide(at, L1, L2, n2, c->eps_int, c->dstatus.as<int>(), c->stream);
     // the accepted step: A from the half step with the same Krylov space
     // (computed either way; committed only when accepted)
@@ -2709,6 +2557,56 @@ void cmf2_attempt_enqueue(mg_ctx *c, int kmax) {
     phase(c, 5);
 }
 
+// The attempt with Gaussian modes (cmf2_base.f90:76-203): the coefficients
+// are propagated as B = D^H S A in the basis orthonormalised by D (D0 from
+// the step's start, carried by the SPF propagation), A = D B at the half
+// step and, committed only when accepted, at the end.  Same device-side
+// decisions as the grid attempt.
+void cmf2_attempt_enqueue_gwp(mg_ctx *c, int kmax) {
+    const int64_t N = c->nconf, D = c->dimpsi;
+    const int64_t nD = (int64_t)c->nspfmax * c->nspfmax * c->nm * c->ns;
+    double2 *A = c->dA.as<double2>(), *psi = c->dpsi.as<double2>(), *AH = c->dAH.as<double2>();
+    double2 *B0 = c->dB0.as<double2>(), *BH = c->dBH.as<double2>(), *B02 = c->dB02.as<double2>();
+    double2 *pH = c->dpH.as<double2>(), *pH2 = c->dpH2.as<double2>();
+    double2 *DD = c->dDD.as<double2>(), *DH = c->dDH.as<double2>(), *DH2 = c->dDH2.as<double2>();
+    AttemptDev *at = att_dev(c);
+    LanczosDev *L1 = lan_dev(c, 0), *L2 = lan_dev(c, 1);
+    launch_fill(reinterpret_cast<double2 *>(c->dstatus.p), 1, make_double2(0, 0), c->stream);
+    calc_h1mode(c, psi);                 // S00(psi) and h1mode
+    calc_rho(c, A);
+    calc_meanfield(c, A, true);
+    prepare_ha(c, DD);
+    a_to_b(c, A, B0, DD);
+    copy_d2d(c, pH, psi, D);
+    copy_d2d(c, DH, DD, nD);
+    propagate_spf_dev(c, pH, 0.5, attempt_fork(c), DH);
+    lanczos_dev(c, B0, BH, L1, 0.5, true, kmax);
+    attempt_join(c);
+    b_to_a(c, BH, AH, DH);
+    calc_h1mode(c, pH);
+    calc_rho(c, AH);
+    calc_meanfield(c, AH, true);
+    copy_d2d(c, pH2, psi, D);
+    copy_d2d(c, DH2, DD, nD);
+    propagate_spf_dev(c, pH2, 0.5, c->stream, DH2);
+    delta_psi_overlaps(c, pH, pH2);
+    delta_psi_contract_dev(c, AH);
+    propagate_spf_dev(c, pH2, 0.5, c->stream, DH2);
+    calc_h1mode(c, pH2);
+    prepare_ha(c, DH2);
+    lanczos_dev(c, BH, B02, L2, -0.5, true, kmax);
+    double *n2 = c->dscal.as<double>() + 12;
+    launch_diff_norm2(B02, B0, N, c->dpartials.as<double>(), c->nblocks_red, n2, c->stream);
+    launch_attempt_decide(at, L1, L2, n2, c->eps_int, c->dstatus.as<int>(), c->stream);
+    lanczos_dev(c, BH, B0, L2, 0.5, false, kmax);
+    b_to_a(c, B0, AH, DH2);              // A of the accepted step, committed below
+    launch_copy_if(A, AH, N, &at->accept, c->stream);
+    launch_copy_if(psi, pH2, D, &at->accept, c->stream);
+    launch_copy_if(DD, DH2, nD, &at->accept, c->stream);
+    HIP_CHECK(hipMemcpyAsync(c->att_host, at, sizeof(AttemptDev), hipMemcpyDeviceToHost, c->stream));
+    check_launch();
+}
+
 // One attempt: eagerly on the first attempts (they build every launch plan
 // and size the Gram partial buffer), then through its captured graph.
 void cmf2_attempt_run(mg_ctx *c, int kmax) {
@@ -2717,8 +2615,12 @@ void cmf2_attempt_run(mg_ctx *c, int kmax) {
         HIP_CHECK(hipGraphExecDestroy(c->att_exec));
         c->att_exec = nullptr;
     }
+    auto enqueue = [&] {
+        if (c->ngwp) cmf2_attempt_enqueue_gwp(c, kmax);
+        else cmf2_attempt_enqueue(c, kmax);
+    };
     if (!graphs || c->att_runs < 1) {
-        cmf2_attempt_enqueue(c, kmax);
+        enqueue();
         c->att_runs++;
         return;
     }
@@ -2728,7 +2630,7 @@ void cmf2_attempt_run(mg_ctx *c, int kmax) {
         c->capturing = true;
         HIP_CHECK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
         try {
-            cmf2_attempt_enqueue(c, kmax);
+            enqueue();
         } catch (...) {
             hipGraph_t junk = nullptr;
             (void)hipStreamEndCapture(c->stream, &junk);
@@ -2752,8 +2654,13 @@ void cmf2_attempt_run(mg_ctx *c, int kmax) {
 
 void cmf2_step(mg_ctx *c, double tt, double htry, double *hdid_out, double *hnext_out, std::vector<mg_attempt> *log) {
     if (c->ngwp) {
-        cmf2_step_gwp(c, tt, htry, hdid_out, hnext_out, log);
-        return;
+        // the step's D0 from the Cholesky factor of S00 (cmf2_base.f90:61-66)
+        launch_fill(reinterpret_cast<double2 *>(c->dstatus.p), 1, make_double2(0, 0), c->stream);
+        gwp_s00(c, c->dpsi.as<double2>());
+        gwp_dd_init(c, c->dDD.as<double2>());
+        HIP_CHECK(hipMemcpyAsync(c->spf_status_host, c->dstatus.p, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+        HIP_CHECK(hipStreamSynchronize(c->stream));
+        spf_check(c);
     }
     if (!c->datt.p) c->datt.alloc(sizeof(AttemptDev) + 64 + 2 * sizeof(LanczosDev));
     const double safe = 0.95;
@@ -2782,8 +2689,10 @@ void cmf2_step(mg_ctx *c, double tt, double htry, double *hdid_out, double *hnex
         HIP_CHECK(hipStreamSynchronize(c->stream));
         HIP_CHECK(hipStreamSynchronize(c->stream2));
         const AttemptDev r = *c->att_host;
-        if (r.spf_status == 1) throw HipError{"SPF overlap matrix not positive definite"};
-        if (r.spf_status == 2) throw HipError{"SPF Runge-Kutta did not converge"};
+        if (r.spf_status) {
+            *c->spf_status_host = r.spf_status;
+            spf_check(c);
+        }
         if (r.overflow == 1) throw HipError{"Maximum number of Krylov vectors reached"};
         if (r.overflow == 2) {
             // the Krylov space outgrew the enqueued iterations: the attempt
@@ -2803,79 +2712,6 @@ void cmf2_step(mg_ctx *c, double tt, double htry, double *hdid_out, double *hnex
         }
         *hnext_out = propose_next(c, hdid, r.error);
         if (c->phases) phase(c, 5);
-        break;
-    }
-    *hdid_out = hdid;
-    HIP_CHECK(hipStreamSynchronize(c->stream));
-}
-
-// cmf2 with Gaussian modes (cmf2_base.f90:47-206): the coefficients are
-// propagated as B = D^H S A in the basis orthonormalised by D, D0 from the
-// Cholesky factor of S00 at the start of the step; the SPF propagation
-// carries D along; A = D B at the half step and at the end.
-void cmf2_step_gwp(mg_ctx *c, double tt, double htry, double *hdid_out, double *hnext_out,
-                   std::vector<mg_attempt> *log) {
-    const int64_t N = c->nconf, D = c->dimpsi;
-    const size_t nD = (size_t)c->nspfmax * c->nspfmax * c->nm * c->ns;
-    double2 *A = c->dA.as<double2>(), *psi = c->dpsi.as<double2>(), *AH = c->dAH.as<double2>();
-    double2 *B0 = c->dB0.as<double2>(), *BH = c->dBH.as<double2>(), *B02 = c->dB02.as<double2>();
-    double2 *pH = c->dpH.as<double2>(), *pH2 = c->dpH2.as<double2>();
-    double2 *DD = c->dDD.as<double2>(), *DH = c->dDH.as<double2>(), *DH2 = c->dDH2.as<double2>();
-    const double safe = 0.95, maxscal = 1.5;
-    double hdid = htry;
-    launch_fill(reinterpret_cast<double2 *>(c->dstatus.p), 1, make_double2(0, 0), c->stream);
-    gwp_s00(c, psi);
-    gwp_dd_init(c, DD);
-    HIP_CHECK(hipMemcpyAsync(c->spf_status_host, c->dstatus.p, sizeof(int), hipMemcpyDeviceToHost, c->stream));
-    HIP_CHECK(hipStreamSynchronize(c->stream));
-    spf_check(c);
-    for (;;) {
-        int forced = -1;
-        if (c->replay_pos < c->replay.size()) {
-            const mg_attempt &r = c->replay[c->replay_pos++];
-            hdid = r.h;
-            forced = (int)r.accepted;
-        }
-        calc_h1mode(c, psi);                 // S00(psi) and h1mode
-        calc_rho(c, A);
-        calc_meanfield(c, A, true);
-        prepare_ha(c, DD);
-        a_to_b(c, A, B0, DD);
-        copy_d2d(c, pH, psi, D);
-        copy_d2d(c, DH, DD, (int64_t)nD);
-        propagate_spf_fork(c, pH, 0.5 * hdid, DH);
-        const int nk1 = lanczos(c, B0, BH, 0.5 * hdid, true);
-        propagate_spf_join(c);
-        b_to_a(c, BH, AH, DH);
-        calc_h1mode(c, pH);
-        calc_rho(c, AH);
-        calc_meanfield(c, AH, true);
-        copy_d2d(c, pH2, psi, D);
-        copy_d2d(c, DH2, DD, (int64_t)nD);
-        propagate_spf(c, pH2, 0.5 * hdid, DH2);
-        delta_psi_overlaps(c, pH, pH2);
-        const double err_psi = delta_psi_contract(c, AH, AH);
-        propagate_spf(c, pH2, 0.5 * hdid, DH2);
-        calc_h1mode(c, pH2);
-        prepare_ha(c, DH2);
-        const int nk2 = lanczos(c, BH, B02, -0.5 * hdid, true);
-        launch_axpby(B02, B0, -1.0, 1.0, N, 0, c->stream);
-        const double err_A = std::sqrt(norm2(c, B02, N)) * 0.5;
-        const double error = std::max(err_A, err_psi) / c->eps_int;
-        const bool reject = (forced >= 0) ? (forced == 0) : (error > 1.0);
-        if (log) log->push_back({tt, hdid, err_A, err_psi, (int64_t)(!reject), nk1, nk2});
-        note_attempt(c, hdid, error);
-        if (reject) {
-            hdid = hdid / std::sqrt(error) * safe;
-            if (hdid < c->min_stepsize) throw HipError{"CMF2 step size fell below minstep"};
-            continue;
-        }
-        lanczos(c, BH, B0, 0.5 * hdid, false);
-        b_to_a(c, B0, A, DH2);
-        copy_d2d(c, psi, pH2, D);
-        copy_d2d(c, DD, DH2, (int64_t)nD);
-        (void)maxscal;
-        *hnext_out = propose_next(c, hdid, error);
         break;
     }
     *hdid_out = hdid;
@@ -3210,6 +3046,7 @@ int mg_create(const mg_problem *prob, int64_t precision, mg_ctx **out) {
         if (const char *e = std::getenv("MG_PHASES")) c->phases = std::atoi(e) != 0;
         if (const char *e = std::getenv("MG_ATTEMPT_GRAPH")) c->att_graphs = std::atoi(e) != 0;
         if (const char *e = std::getenv("MG_ATTEMPT_KMAX")) c->att_kmax_cap = std::max(2, std::atoi(e));
+        if (const char *e = std::getenv("MG_ATTEMPT_FORK")) c->att_fork = std::atoi(e) != 0;
         HIP_CHECK(hipHostMalloc((void **)&c->att_host, sizeof(AttemptDev), hipHostMallocDefault));
         std::memset(c->att_host, 0, sizeof(AttemptDev));
         c->phase_t0 = std::chrono::steady_clock::now();

@@ -708,7 +708,8 @@ __global__ void __launch_bounds__(64) k_gwp_propagate(const GwpDesc *__restrict_
     const double safety = 0.9, pgrow = -0.2, pshrnk = -0.25, minscal = 0.1, maxscal = 5.0;
     const double errcon = pow(5.0 / safety, 1.0 / pgrow);
     const bool dp = P.dormand_prince != 0;
-    double dt_done = 0.0, stepsize = P.dt;
+    const double DT = P.dt_src ? P.dt_src[0] * P.dt_scale : P.dt;
+    double dt_done = 0.0, stepsize = DT;
     int attempts = 0;
     for (;;) {
         double h = stepsize, error = 0.0;
@@ -746,11 +747,11 @@ __global__ void __launch_bounds__(64) k_gwp_propagate(const GwpDesc *__restrict_
         for (int e = tid; e < nD; e += nth) S.D0[e] = S.tmpD[e];
         __syncthreads();
         dt_done = dt_done + h;
-        if (dt_done >= P.dt) break;
+        if (dt_done >= DT) break;
         double hn = hnext;
         if (hn < P.min_stepsize * 2) hn = P.min_stepsize * 2;
-        stepsize = (P.dt - dt_done) / (double)((long long)((P.dt - dt_done) / hn) + 1);
-        if (stepsize > P.dt - dt_done) stepsize = P.dt - dt_done;
+        stepsize = (DT - dt_done) / (double)((long long)((DT - dt_done) / hn) + 1);
+        if (stepsize > DT - dt_done) stepsize = DT - dt_done;
     }
     __syncthreads();
     if (tid == 0) {

@@ -201,6 +201,8 @@ struct GwpParams {
     int dormand_prince;
     double2 der_factor;
     int max_attempts;
+    const double *dt_src;        // as in SpfParams
+    double dt_scale;
 };
 
 struct SpfParams {
@@ -347,6 +349,9 @@ void launch_combine_v0(double2 *out, const double2 *v0, const void *basis, int64
                        int64_t n, int prec, hipStream_t s, const int *kdev = nullptr);
 void launch_norm2(const void *x, int64_t n, double *partials, int nblocks, double *out, int prec,
                   hipStream_t s);
+// out = |y - x|^2 (complex FP64 vectors)
+void launch_diff_norm2(const double2 *y, const double2 *x, int64_t n, double *partials, int nblocks, double *out,
+                       hipStream_t s);
 void launch_convert(void *dst, int pdst, const void *src, int psrc, int64_t n, hipStream_t s);
 void launch_copy(void *dst, const void *src, int64_t n, int prec, hipStream_t s);
 // out = y0 + sum_q w[q] k[q]; y0/out complex FP64, k[q] in the storage type.

@@ -285,15 +285,25 @@ template <> struct GramMfma<float2> {
         const int i = lane & 15, q = lane >> 4, e = i & 7, im = i >> 3;
         mg_floatx4 d = {0.f, 0.f, 0.f, 0.f};
         const int nsteps = (nfib + 3) >> 2;
-        for (int st = wave; st < nsteps; st += nw) {
-            const int f = 4 * st + q;
-            const int ff = f < nfib ? f : 0;
-            const int outer = (int)fdiv((uint32_t)ff, fds), inner = ff - outer * stride;
-            const float2 xv = X[inner + outer * stride * n1 + (e < n1 ? e : 0) * stride];
-            const float2 yv = Y[inner + outer * stride * n2 + (e < n2 ? e : 0) * stride];
-            const float av = (f < nfib && e < n1) ? (im ? xv.y : xv.x) : 0.f;
-            const float bv = (f < nfib && e < n2) ? (im ? yv.y : yv.x) : 0.f;
-            d = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bv, d, 0, 0, 0);
+        // four steps' loads in flight before their MFMAs (same order of
+        // accumulation; steps past the end add exact zeros)
+        constexpr int U = 4;
+        for (int st0 = wave; st0 < nsteps; st0 += U * nw) {
+            float av[U], bv[U];
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                const int st = st0 + u * nw;
+                const int f = 4 * st + q;
+                const bool ok = st < nsteps && f < nfib;
+                const int ff = ok ? f : 0;
+                const int outer = (int)fdiv((uint32_t)ff, fds), inner = ff - outer * stride;
+                const float2 xv = X[inner + outer * stride * n1 + (e < n1 ? e : 0) * stride];
+                const float2 yv = Y[inner + outer * stride * n2 + (e < n2 ? e : 0) * stride];
+                av[u] = (ok && e < n1) ? (im ? xv.y : xv.x) : 0.f;
+                bv[u] = (ok && e < n2) ? (im ? yv.y : yv.x) : 0.f;
+            }
+#pragma unroll
+            for (int u = 0; u < U; u++) d = __builtin_amdgcn_mfma_f32_16x16x4f32(av[u], bv[u], d, 0, 0, 0);
         }
 #pragma unroll
         for (int v = 0; v < 4; v++) acc[v] += (double)d[v];
@@ -308,15 +318,25 @@ template <> struct GramMfma<double2> {
         const int i = lane & 15, q = lane >> 4, e = i & 7, im = i >> 3;
         mg_doublex4 d = {0.0, 0.0, 0.0, 0.0};
         const int nsteps = (nfib + 3) >> 2;
-        for (int st = wave; st < nsteps; st += nw) {
-            const int f = 4 * st + q;
-            const int ff = f < nfib ? f : 0;
-            const int outer = (int)fdiv((uint32_t)ff, fds), inner = ff - outer * stride;
-            const double2 xv = X[inner + outer * stride * n1 + (e < n1 ? e : 0) * stride];
-            const double2 yv = Y[inner + outer * stride * n2 + (e < n2 ? e : 0) * stride];
-            const double av = (f < nfib && e < n1) ? (im ? xv.y : xv.x) : 0.0;
-            const double bv = (f < nfib && e < n2) ? (im ? yv.y : yv.x) : 0.0;
-            d = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, d, 0, 0, 0);
+        // four steps' loads in flight before their MFMAs (same order of
+        // accumulation; steps past the end add exact zeros)
+        constexpr int U = 4;
+        for (int st0 = wave; st0 < nsteps; st0 += U * nw) {
+            double av[U], bv[U];
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                const int st = st0 + u * nw;
+                const int f = 4 * st + q;
+                const bool ok = st < nsteps && f < nfib;
+                const int ff = ok ? f : 0;
+                const int outer = (int)fdiv((uint32_t)ff, fds), inner = ff - outer * stride;
+                const double2 xv = X[inner + outer * stride * n1 + (e < n1 ? e : 0) * stride];
+                const double2 yv = Y[inner + outer * stride * n2 + (e < n2 ? e : 0) * stride];
+                av[u] = (ok && e < n1) ? (im ? xv.y : xv.x) : 0.0;
+                bv[u] = (ok && e < n2) ? (im ? yv.y : yv.x) : 0.0;
+            }
+#pragma unroll
+            for (int u = 0; u < U; u++) d = __builtin_amdgcn_mfma_f64_16x16x4f64(av[u], bv[u], d, 0, 0, 0);
         }
 #pragma unroll
         for (int v = 0; v < 4; v++) acc[v] += d[v];
@@ -1013,6 +1033,27 @@ __global__ void __launch_bounds__(256) k_norm2_partial(const T *x, int64_t n, do
     }
 }
 
+// |y - x|^2 partials (complex FP64), the same accumulation as k_norm2_partial
+// of the stored difference: err_A of a CMF2 attempt without writing B02 - B0.
+__global__ void __launch_bounds__(256) k_diff_norm2_partial(const double2 *y, const double2 *x, int64_t n,
+                                                            double *partials) {
+    double acc = 0.0;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const double2 a = make_double2(y[i].x - x[i].x, y[i].y - x[i].y);
+        acc = fma(a.x, a.x, acc);
+        acc = fma(a.y, a.y, acc);
+    }
+    __shared__ double red[4];
+    acc = wave_sum(acc);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double a = 0.0;
+        for (int w = 0; w < (int)(blockDim.x >> 6); w++) a += red[w];
+        partials[blockIdx.x] = a;
+    }
+}
+
 __global__ void __launch_bounds__(64) k_norm2_final(const double *partials, int nblocks, double *out,
                                                     const int *skip = nullptr) {
     if (skip && *skip) return;
@@ -1029,6 +1070,12 @@ void launch_norm2(const void *x, int64_t n, double *partials, int nblocks, doubl
     else
         hipLaunchKernelGGL(k_norm2_partial<float2>, dim3(nblocks), dim3(256), 0, s, (const float2 *)x, n,
                            partials);
+    hipLaunchKernelGGL(k_norm2_final, dim3(1), dim3(64), 0, s, partials, nblocks, out, (const int *)nullptr);
+}
+
+void launch_diff_norm2(const double2 *y, const double2 *x, int64_t n, double *partials, int nblocks, double *out,
+                       hipStream_t s) {
+    hipLaunchKernelGGL(k_diff_norm2_partial, dim3(nblocks), dim3(256), 0, s, y, x, n, partials);
     hipLaunchKernelGGL(k_norm2_final, dim3(1), dim3(64), 0, s, partials, nblocks, out, (const int *)nullptr);
 }
 

@@ -1,4 +1,4 @@
-"""HIP-graph replay of the cached launch plans.
+"""HIP-graph replay of the cached launch plans and of whole CMF2 attempts.
 
 Graphs hold the same kernels with the same arguments as the eager launches,
 so every replay must give bit-identical results: H.A replayed several times
@@ -39,10 +39,17 @@ def test_apply_h_replays_match_eager(precision, tmp_path):
 
 
 @gpu
-@pytest.mark.parametrize("name,precision,tf_fs", [("t5", 0, 1.0), ("t5", 1, 1.0), ("t2", 0, 2.0), ("g3", 0, 0.3)])
-def test_propagation_with_graphs_is_bit_identical(name, precision, tf_fs, tmp_path):
+@pytest.mark.parametrize("name,precision,tf_fs,attempt_graph,fork",
+                         [("t5", 0, 1.0, 1, 1), ("t5", 1, 1.0, 1, 1), ("t2", 0, 2.0, 1, 1), ("g3", 0, 0.3, 1, 1),
+                          ("t5", 1, 1.0, 1, 0), ("t5", 1, 1.0, 0, 1), ("g3", 0, 0.3, 0, 1)])
+def test_propagation_with_graphs_is_bit_identical(name, precision, tf_fs, attempt_graph, fork, tmp_path,
+                                                  monkeypatch):
+    """Whole attempts as one graph (MG_ATTEMPT_GRAPH=1), with and without the
+    second stream for the SPF half steps, and the per-plan graphs alone."""
     prob = make_problem(name, str(tmp_path))
     tf = tf_fs / AU2FS
+    monkeypatch.setenv("MG_ATTEMPT_GRAPH", str(attempt_graph))
+    monkeypatch.setenv("MG_ATTEMPT_FORK", str(fork))
     og, pg, ag = engine(prob, precision, True).propagate(0.0, tf)
     oe, pe, ae = engine(prob, precision, False).propagate(0.0, tf)
     assert len(ag) == len(ae)
@@ -60,6 +67,7 @@ def test_krylov_build_outgrowing_the_graph_is_repeated(precision, tmp_path, monk
     every attempt: the trajectory equals the uncapped one bit for bit."""
     prob = make_problem("t5", str(tmp_path))
     tf = 1.0 / AU2FS
+    monkeypatch.setenv("MG_ATTEMPT_GRAPH", "1")
     oa, pa, aa = engine(prob, precision, True).propagate(0.0, tf)
     monkeypatch.setenv("MG_ATTEMPT_KMAX", "2")
     ob, pb, ab = engine(prob, precision, True).propagate(0.0, tf)
