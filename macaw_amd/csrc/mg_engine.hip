@@ -118,10 +118,10 @@ struct mg_ctx {
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     int *spf_status_host = nullptr;   // pinned: status of the last SPF propagation
     // 0: complex FP64 throughout.  1: mixed form -- the A-vector, SPFs, rho
-    // (Grams of the FP64 A), mean-field matrices, the first Krylov product
-    // and every reduction stay FP64; the other H.A chains, the Krylov vectors
-    // beyond the first and the mean-field mode-product chains are stored and
-    // multiplied in complex FP32.  MG_LAN64=0 / MG_RHO64=0 / MG_MF64=1 move
+    // (Grams of the FP64 A), mean-field matrices and every reduction stay
+    // FP64; the H.A chains of every Krylov product, the Krylov vectors beyond
+    // the start vector and the mean-field mode-product chains are stored and
+    // multiplied in complex FP32.  MG_LAN64=1 / MG_RHO64=0 / MG_MF64=1 move
     // the first Krylov product / rho / the mean fields to the other side.
     int prec = 0;
     size_t es = sizeof(double2);   // bytes per complex element of the FP32/FP64 work tensors
@@ -163,11 +163,13 @@ struct mg_ctx {
     DevBuf dA, dpsi, dAstart, dpsistart, dBK, dB0, dBH, dB02, dpH, dpH2, dHtmp, dIn, dOut;
     DevBuf dh1, dhsum, drho, dmf, drmf, dS, dS2, dS3, dGtmp;
     DevBuf dchain;                 // 2 * nchains * mixed
-    // mixed form: the first Krylov product H u_0 runs on the FP64 start
-    // vector with FP64 matrices (chain temporaries and result in FP64), so
-    // that u_1 = H u_0 - alpha u_0 carries no FP32 cancellation error
+    // mixed form, MG_LAN64=1: the first Krylov product H u_0 runs on the
+    // FP64 start vector with FP64 matrices (chain temporaries and result in
+    // FP64), so that u_1 = H u_0 - alpha u_0 carries no FP32 rounding.  Off
+    // by default: over 0-71.5 fs of the 4 x 8^8 workload it moves C(t) by
+    // 5.7e-8 instead of 1.9e-8 from the FP64 engine and costs 9% of the rate
     DevBuf dchain64, dW64;
-    bool lan64 = true;
+    bool lan64 = false;
     // mixed form: mean fields from the FP64 A-vector with FP64 chains
     // (MG_MF64=1; off by default: FP32 chains do not move the step sequence)
     DevBuf dmfbuf64;
@@ -2400,8 +2402,11 @@ LanczosDev *lan_dev(mg_ctx *c, int i) {
 }
 
 // Lanczos (cmf2_integration.f90:30-103) from B0 (FP64) over dtscale * h; BT
-// FP64 and normalised.  build = false reuses L's Krylov space.
-void lanczos_dev(mg_ctx *c, const double2 *B0, double2 *BT, LanczosDev *L, double dtscale, bool build, int kmax) {
+// FP64 and normalised.  build = false reuses L's Krylov space.  b0f: an FP32
+// copy of B0 already in memory (the mixed form's mean-field input), used as
+// the work-type u_0 instead of converting B0 again.
+void lanczos_dev(mg_ctx *c, const double2 *B0, double2 *BT, LanczosDev *L, double dtscale, bool build, int kmax,
+                 const void *b0f = nullptr) {
     const int64_t N = c->nconf;
     const int pr = c->prec;
     void *BK = c->dBK.p;
@@ -2413,8 +2418,8 @@ void lanczos_dev(mg_ctx *c, const double2 *B0, double2 *BT, LanczosDev *L, doubl
     if (build) {
         launch_lan_begin(L, c->stream);
         launch_norm2(B0, N, part, c->nblocks_red, nsq + 0, 0, c->stream);
-        if (pr) launch_convert(BK, 1, B0, 0, N, c->stream);
-        const void *u0 = pr ? BK : (const void *)B0;
+        if (pr && !b0f) launch_convert(BK, 1, B0, 0, N, c->stream);
+        const void *u0 = pr ? (b0f ? b0f : BK) : (const void *)B0;
         for (int j = 1; j <= kmax; j++) {
             const void *uj = (j == 1) ? u0 : c->at(BK, N * (j - 1));
             void *w = c->at(BK, N * j);
@@ -2518,8 +2523,10 @@ void cmf2_attempt_enqueue(mg_ctx *c, int kmax) {
     copy_d2d(c, pH, psi, D);
     // the SPF half step needs only rho(A) and the mean fields: stream2, beside
     // the Lanczos propagation of A
+    // the mixed form's mean fields left an FP32 copy of A (= B0) in dAf
+    const void *a_f32 = (c->prec && c->mf_bricks && !c->mf64) ? c->dAf.p : nullptr;
     propagate_spf_dev(c, pH, 0.5, attempt_fork(c));
-    lanczos_dev(c, B0, BH, L1, 0.5, true, kmax);
+    lanczos_dev(c, B0, BH, L1, 0.5, true, kmax, a_f32);
     phase(c, 3);
     attempt_join(c);
     phase(c, 2);
@@ -2541,7 +2548,8 @@ void cmf2_attempt_enqueue(mg_ctx *c, int kmax) {
     phase(c, 2);
     calc_h1mode(c, pH2);
     phase(c, 0);
-    lanczos_dev(c, BH, B02, L2, -0.5, true, kmax);
+    // dAf still holds the FP32 copy of BH from the second mean fields
+    lanczos_dev(c, BH, B02, L2, -0.5, true, kmax, a_f32);
     phase(c, 3);
     double *n2 = c->dscal.as<double>() + 12;
     launch_diff_norm2(B02, B0, N, c->dpartials.as<double>(), c->nblocks_red, n2, c->stream);

@@ -1318,56 +1318,126 @@ void launch_mat_axpy(double2 *dst, const double2 *src, double2 alpha, int n, int
 // 'poly'-regularised inverse of rho (maths.f90:8-83) times each mean field.
 // job layout: [mode, rho_index, n, nops, (op, n2) x nops], stride 36.
 // minw (optional): the smallest eigenvalue of each job's rho.
-__global__ void k_rmf(const double2 *rho, const double2 *mf, double2 *rmf, const int *jobs, int njobs,
-                      int nspfmax, int nopmax, double eps, double *minw) {
-    const int jb = blockIdx.x * blockDim.x + threadIdx.x;
+// rho^-1 MF per (mode, state) job, one wavefront each: the cyclic Jacobi of
+// dev_herm_eig with every rotation's column, row and eigenvector updates
+// spread over the lanes (lane k updates element k; each element's arithmetic
+// and the sweep test are those of the one-thread routine), then the
+// regularised inverse and the products with the mean-field matrices, one
+// element per lane.
+__global__ void __launch_bounds__(64) k_rmf(const double2 *rho, const double2 *mf, double2 *rmf, const int *jobs,
+                                            int njobs, int nspfmax, int nopmax, double eps, double *minw) {
+    const int jb = blockIdx.x;
     if (jb >= njobs) return;
+    const int lane = threadIdx.x;
+    __shared__ double2 A[kSmall * kSmall], U[kSmall * kSmall];
+    __shared__ double w[kSmall];
+    __shared__ int converged;
     const int *J = jobs + 36 * jb;
     const int m = J[0], ri = J[1], n = J[2], nops = J[3];
-    double2 A[kSmall * kSmall], U[kSmall * kSmall];
-    double w[kSmall];
     const int nn = nspfmax * nspfmax;
     const double2 *r = rho + (int64_t)ri * nn;
-    for (int j = 0; j < n; j++)
-        for (int i = 0; i < n; i++) A[i + kSmall * j] = r[i + nspfmax * j];
-    dev_herm_eig(n, A, w, U);
-    if (minw) {
-        double mn = w[0];
-        for (int j = 1; j < n; j++) mn = fmin(mn, w[j]);
-        minw[jb] = mn;
+    for (int e = lane; e < n * n; e += 64) {
+        const int i = e % n, j = e / n;
+        A[i + kSmall * j] = r[i + nspfmax * j];
+        U[i + kSmall * j] = make_double2(i == j ? 1.0 : 0.0, 0.0);
     }
-    for (int j = 0; j < n; j++) {
-        double v = w[j];
-        if (v < eps) v = 0.5 * (eps + v * v / eps);
-        w[j] = 1.0 / v;
-    }
-    // rinv (Hermitian, upper then mirrored) into A
-    for (int j2 = 0; j2 < n; j2++)
-        for (int j1 = 0; j1 <= j2; j1++) {
-            double2 acc = zero2();
-            for (int k = 0; k < n; k++) acc = cadd(acc, cscale(w[k], cmul(U[j1 + kSmall * k], cconj(U[j2 + kSmall * k]))));
-            if (j1 == j2) acc.y = 0.0;
-            A[j1 + kSmall * j2] = acc;
-            A[j2 + kSmall * j1] = cconj(acc);
+    __syncthreads();
+    for (int sweep = 0; sweep < 100; sweep++) {
+        if (lane == 0) {
+            double off = 0.0, diag = 0.0;
+            for (int j = 0; j < n; j++)
+                for (int i = 0; i < n; i++) {
+                    const double2 a = A[i + kSmall * j];
+                    const double a2 = a.x * a.x + a.y * a.y;
+                    if (i != j) off += a2; else diag += a2;
+                }
+            converged = (off <= 1e-32 * (diag + 1e-300) || off == 0.0) ? 1 : 0;
         }
+        __syncthreads();
+        if (converged) break;
+        for (int p = 0; p < n - 1; p++)
+            for (int q = p + 1; q < n; q++) {
+                const double2 apq = A[p + kSmall * q];
+                const double aabs = hypot(apq.x, apq.y);
+                if (aabs == 0.0) continue;
+                const double2 ph = make_double2(apq.x / aabs, apq.y / aabs);
+                const double app = A[p + kSmall * p].x, aqq = A[q + kSmall * q].x;
+                const double theta = (aqq - app) / (2.0 * aabs);
+                const double tt = (theta >= 0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
+                const double c = 1.0 / sqrt(tt * tt + 1.0), sn = tt * c;
+                const double2 jpp = make_double2(c, 0.0), jqq = cscale(c, cconj(ph)), jpq = make_double2(sn, 0.0),
+                              jqp = cscale(-sn, cconj(ph));
+                // every lane has read the rotation's inputs before any update
+                __syncthreads();
+                if (lane < n) {
+                    const int k = lane;
+                    const double2 akp = A[k + kSmall * p], akq = A[k + kSmall * q];
+                    A[k + kSmall * p] = cadd(cmul(akp, jpp), cmul(akq, jqp));
+                    A[k + kSmall * q] = cadd(cmul(akp, jpq), cmul(akq, jqq));
+                    const double2 ukp = U[k + kSmall * p], ukq = U[k + kSmall * q];
+                    U[k + kSmall * p] = cadd(cmul(ukp, jpp), cmul(ukq, jqp));
+                    U[k + kSmall * q] = cadd(cmul(ukp, jpq), cmul(ukq, jqq));
+                }
+                __syncthreads();
+                if (lane < n) {
+                    const int k = lane;
+                    const double2 apk = A[p + kSmall * k], aqk = A[q + kSmall * k];
+                    A[p + kSmall * k] = cadd(cmul(cconj(jpp), apk), cmul(cconj(jqp), aqk));
+                    A[q + kSmall * k] = cadd(cmul(cconj(jpq), apk), cmul(cconj(jqq), aqk));
+                }
+                __syncthreads();
+                if (lane == 0) {
+                    A[p + kSmall * q] = make_double2(0.0, 0.0);
+                    A[q + kSmall * p] = make_double2(0.0, 0.0);
+                    A[p + kSmall * p].y = 0.0;
+                    A[q + kSmall * q].y = 0.0;
+                }
+                __syncthreads();
+            }
+    }
+    if (lane == 0) {
+        for (int i = 0; i < n; i++) w[i] = A[i + kSmall * i].x;
+        if (minw) {
+            double mn = w[0];
+            for (int j = 1; j < n; j++) mn = fmin(mn, w[j]);
+            minw[jb] = mn;
+        }
+        for (int j = 0; j < n; j++) {
+            double v = w[j];
+            if (v < eps) v = 0.5 * (eps + v * v / eps);
+            w[j] = 1.0 / v;
+        }
+    }
+    __syncthreads();
+    // rinv (Hermitian: the upper triangle, mirrored) into A
+    for (int e = lane; e < n * n; e += 64) {
+        const int j1 = e % n, j2 = e / n;
+        if (j1 > j2) continue;
+        double2 acc = zero2();
+        for (int k = 0; k < n; k++) acc = cadd(acc, cscale(w[k], cmul(U[j1 + kSmall * k], cconj(U[j2 + kSmall * k]))));
+        if (j1 == j2) acc.y = 0.0;
+        A[j1 + kSmall * j2] = acc;
+        A[j2 + kSmall * j1] = cconj(acc);
+    }
+    __syncthreads();
     for (int q = 0; q < nops; q++) {
         const int op = J[4 + 2 * q], n2 = J[5 + 2 * q];
         const double2 *src = mf + (int64_t)nn * (op + (int64_t)nopmax * m);
         double2 *dst = rmf + (int64_t)nn * (op + (int64_t)nopmax * m);
-        for (int b = 0; b < n2; b++)
-            for (int a = 0; a < n; a++) {
-                double2 acc = zero2();
-                for (int k = 0; k < n; k++) acc = cadd(acc, cmul(A[a + kSmall * k], src[k + nspfmax * b]));
-                dst[a + nspfmax * b] = acc;
-            }
+        for (int e = lane; e < n * n2; e += 64) {
+            const int a = e % n, b = e / n;
+            double2 acc = zero2();
+            for (int k = 0; k < n; k++) acc = cadd(acc, cmul(A[a + kSmall * k], src[k + nspfmax * b]));
+            dst[a + nspfmax * b] = acc;
+        }
     }
 }
 
 void launch_rmf(const double2 *rho, const double2 *mf, double2 *rmf, const int *jobs, int njobs, int nspfmax,
                 int nopmax, double eps, hipStream_t s, double *minw) {
+    // nspfmax <= kSmall: the engine refuses more SPFs per mode at set-up
     if (njobs <= 0) return;
-    hipLaunchKernelGGL(k_rmf, dim3((njobs + 63) / 64), dim3(64), 0, s, rho, mf, rmf, jobs, njobs, nspfmax, nopmax,
-                       eps, minw);
+    hipLaunchKernelGGL(k_rmf, dim3(njobs), dim3(64), 0, s, rho, mf, rmf, jobs, njobs, nspfmax, nopmax, eps, minw);
 }
 
 // ---------------------------------------------------------------------------
