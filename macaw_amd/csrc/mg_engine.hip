@@ -162,6 +162,7 @@ struct mg_ctx {
     // ---- device buffers ----
     DevBuf dA, dpsi, dAstart, dpsistart, dBK, dB0, dBH, dB02, dpH, dpH2, dHtmp, dIn, dOut;
     DevBuf dh1, dhsum, drho, dmf, drmf, dS, dS2, dS3, dGtmp;
+    DevBuf ddpsi;                  // DeltaPsi overlap Grams: 3 sets x nstates x nspfmax^2
     DevBuf dchain;                 // 2 * nchains * mixed
     // mixed form, MG_LAN64=1: the first Krylov product H u_0 runs on the
     // FP64 start vector with FP64 matrices (chain temporaries and result in
@@ -245,6 +246,7 @@ struct mg_ctx {
     bool att_graphs = false;           // MG_ATTEMPT_GRAPH=1: the attempt as one graph (slower, see mg_set_graphs)
     int att_kmax_cap = kNKrylovMax;    // MG_ATTEMPT_KMAX (test aid): fewer iterations per graph-run build
     bool att_fork = true;              // MG_ATTEMPT_FORK=0: the SPF half steps on the engine stream
+    bool dps_fused = true;             // MG_DPSI_FUSED=0: DeltaPsi as three chains and dots
     bool capturing = false;            // an attempt is being captured: plans launch their kernels
     const int *plan_skip = nullptr;    // skip flag handed to the brick launches being enqueued
     // ---- learned step controller ----
@@ -1607,6 +1609,54 @@ void apply_kron(mg_ctx *c, const double2 *X, double2 *Y, const double2 *mats, un
     check_launch();
 }
 
+// DeltaPsi's three overlaps <A1|(x_m S_m) A1> (S, S3, S2: the SPF overlap
+// sets of delta_psi_overlaps) in one plan.  Groups before the last carry the
+// three chains in temporaries (dkron twice, dHtmp); the last group's program
+// loads A1 once, finishes each chain in a second buffer and accumulates the
+// Gram A1^H T over the group's last mode, whose trace is <A1|T>: no chain is
+// written out whole and no separate dot pass reads it back.
+std::unique_ptr<Plan> build_delta_psi(mg_ctx *c, const double2 *A1) {
+    PlanBuilder pb;
+    const int G = (int)c->groups.size(), nm = c->nm, ns = c->ns;
+    const size_t nn = (size_t)c->nspfmax * c->nspfmax;
+    const double2 *mats[3] = {c->dS.as<double2>(), c->dS3.as<double2>(), c->dS2.as<double2>()};
+    double2 *tmp[3] = {c->dkron.as<double2>(), c->dkron.as<double2>() + c->nconf, c->dHtmp.as<double2>()};
+    if (c->ddpsi.bytes < sizeof(double2) * 3 * ns * nn) c->ddpsi.alloc(sizeof(double2) * 3 * ns * nn);
+    double2 *slots = c->ddpsi.as<double2>();
+    for (int g = 0; g < G; g++) {
+        const int m0 = c->groups[g].first, m1 = c->groups[g].second;
+        std::vector<TaskBuilder> tasks;
+        for (int s = 0; s < ns; s++) {
+            std::vector<int64_t> sz(nm);
+            for (int m = 0; m < nm; m++) sz[m] = c->NS(m, s);
+            const int64_t L = prod_range(sz, 0, m0), R = prod_range(sz, m1, nm);
+            auto ops = [&](TaskBuilder &tb, int buf, int k) {
+                for (int m = m0; m < m1; m++) tb.op(buf, buf, m, mats[k] + nn * (m + (size_t)nm * s), 0, (int)sz[m], 0);
+            };
+            if (g < G - 1) {
+                for (int k = 0; k < 3; k++) {
+                    TaskBuilder tb(c, g, L, R);
+                    tb.load(0, (g == 0 ? A1 : tmp[k]) + c->off[s], sz);
+                    ops(tb, 0, k);
+                    tb.store(0, tmp[k] + c->off[s], 0);
+                    tasks.push_back(tb);
+                }
+            } else {
+                TaskBuilder tb(c, g, L, R);
+                tb.load(0, A1 + c->off[s], sz);
+                for (int k = 0; k < 3; k++) {
+                    tb.load(1, (G == 1 ? A1 : tmp[k]) + c->off[s], sz);
+                    ops(tb, 1, k);
+                    tb.gram(0, 1, m1 - 1, slots + nn * ((size_t)k * ns + s));
+                }
+                tasks.push_back(tb);
+            }
+        }
+        pb.phase(tasks);
+    }
+    return pb.finish(0);
+}
+
 // A_to_B / B_to_A (cmf2_base.f90:264-345): B = D^H S A and A = D B over the
 // Gaussian modes, with the intra-state overlaps of the current SPFs (gwp_s00)
 // and the D matrices dd.
@@ -2404,9 +2454,11 @@ LanczosDev *lan_dev(mg_ctx *c, int i) {
 // Lanczos (cmf2_integration.f90:30-103) from B0 (FP64) over dtscale * h; BT
 // FP64 and normalised.  build = false reuses L's Krylov space.  b0f: an FP32
 // copy of B0 already in memory (the mixed form's mean-field input), used as
-// the work-type u_0 instead of converting B0 again.
+// the work-type u_0 instead of converting B0 again.  normalise = false
+// leaves BT unnormalised with its squared norm in dscal[6], for a consumer
+// that normalises as it reads.
 void lanczos_dev(mg_ctx *c, const double2 *B0, double2 *BT, LanczosDev *L, double dtscale, bool build, int kmax,
-                 const void *b0f = nullptr) {
+                 const void *b0f = nullptr, bool normalise = true) {
     const int64_t N = c->nconf;
     const int pr = c->prec;
     void *BK = c->dBK.p;
@@ -2450,7 +2502,7 @@ void lanczos_dev(mg_ctx *c, const double2 *B0, double2 *BT, LanczosDev *L, doubl
     // B0 is u_0 of the space (also when it is reused)
     launch_combine_v0(BT, B0, BK, N, dcoef, 0, N, pr, c->stream, &L->nk);
     launch_norm2(BT, N, part, c->nblocks_red, dsc + 6, 0, c->stream);
-    launch_scale(BT, dsc + 6, N, 0, c->stream);
+    if (normalise) launch_scale(BT, dsc + 6, N, 0, c->stream);
     check_launch();
 }
 
@@ -2459,6 +2511,18 @@ void lanczos_dev(mg_ctx *c, const double2 *B0, double2 *BT, LanczosDev *L, doubl
 void delta_psi_contract_dev(mg_ctx *c, const double2 *A1) {
     double *part = c->dpartials.as<double>();
     double2 *o = reinterpret_cast<double2 *>(c->dscal.as<double>() + 88);   // o11, o22, o12
+    if (c->nspfmax <= 8 && c->ngwp == 0 && c->dps_fused) {
+        // real parts only enter err_psi, and Re <T|A1> = Re <A1|T>
+        PlanKey key{(const void *)12, A1, nullptr, nullptr};
+        const Plan &p = get_plan(c, key, [](mg_ctx *cc, const PlanKey &k) {
+            return build_delta_psi(cc, (const double2 *)k[1]);
+        });
+        run_plan(c, p);
+        launch_dpsi_traces(c->ddpsi.as<double2>(), c->ns, c->nspfmax, o, c->stream);
+        launch_attempt_psi(att_dev(c), o + 0, o + 1, o + 2, c->stream);
+        check_launch();
+        return;
+    }
     double2 *T = c->dHtmp.as<double2>();
     apply_kron(c, A1, T, c->dS.as<double2>());
     launch_dot(T, A1, c->nconf, part, c->nblocks_red, o + 0, 0, c->stream);
@@ -2548,17 +2612,20 @@ void cmf2_attempt_enqueue(mg_ctx *c, int kmax) {
     phase(c, 2);
     calc_h1mode(c, pH2);
     phase(c, 0);
-    // dAf still holds the FP32 copy of BH from the second mean fields
-    lanczos_dev(c, BH, B02, L2, -0.5, true, kmax, a_f32);
+    // dAf still holds the FP32 copy of BH from the second mean fields; B02
+    // and the accepted B0 stay unnormalised (squared norm in dscal[6]) and
+    // are normalised by their one reader, as the scaling pass would
+    double *bt_n2 = c->dscal.as<double>() + 6;
+    lanczos_dev(c, BH, B02, L2, -0.5, true, kmax, a_f32, false);
     phase(c, 3);
     double *n2 = c->dscal.as<double>() + 12;
-    launch_diff_norm2(B02, B0, N, c->dpartials.as<double>(), c->nblocks_red, n2, c->stream);
+    launch_diff_norm2(B02, B0, N, c->dpartials.as<double>(), c->nblocks_red, n2, c->stream, bt_n2);
     launch_attempt_decide(at, L1, L2, n2, c->eps_int, c->dstatus.as<int>(), c->stream);
     // the accepted step: A from the half step with the same Krylov space
     // (computed either way; committed only when accepted)
-    lanczos_dev(c, BH, B0, L2, 0.5, false, kmax);
+    lanczos_dev(c, BH, B0, L2, 0.5, false, kmax, nullptr, false);
     phase(c, 3);
-    launch_copy_if(A, B0, N, &at->accept, c->stream);
+    launch_copy_if(A, B0, N, &at->accept, c->stream, bt_n2);
     launch_copy_if(psi, pH2, D, &at->accept, c->stream);
     HIP_CHECK(hipMemcpyAsync(c->att_host, at, sizeof(AttemptDev), hipMemcpyDeviceToHost, c->stream));
     check_launch();
@@ -3055,6 +3122,7 @@ int mg_create(const mg_problem *prob, int64_t precision, mg_ctx **out) {
         if (const char *e = std::getenv("MG_ATTEMPT_GRAPH")) c->att_graphs = std::atoi(e) != 0;
         if (const char *e = std::getenv("MG_ATTEMPT_KMAX")) c->att_kmax_cap = std::max(2, std::atoi(e));
         if (const char *e = std::getenv("MG_ATTEMPT_FORK")) c->att_fork = std::atoi(e) != 0;
+        if (const char *e = std::getenv("MG_DPSI_FUSED")) c->dps_fused = std::atoi(e) != 0;
         HIP_CHECK(hipHostMalloc((void **)&c->att_host, sizeof(AttemptDev), hipHostMallocDefault));
         std::memset(c->att_host, 0, sizeof(AttemptDev));
         c->phase_t0 = std::chrono::steady_clock::now();

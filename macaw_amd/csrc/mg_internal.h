@@ -351,7 +351,7 @@ void launch_norm2(const void *x, int64_t n, double *partials, int nblocks, doubl
                   hipStream_t s);
 // out = |y - x|^2 (complex FP64 vectors)
 void launch_diff_norm2(const double2 *y, const double2 *x, int64_t n, double *partials, int nblocks, double *out,
-                       hipStream_t s);
+                       hipStream_t s, const double *ynorm2 = nullptr);
 void launch_convert(void *dst, int pdst, const void *src, int psrc, int64_t n, hipStream_t s);
 void launch_copy(void *dst, const void *src, int64_t n, int prec, hipStream_t s);
 // out = y0 + sum_q w[q] k[q]; y0/out complex FP64, k[q] in the storage type.
@@ -394,9 +394,11 @@ void launch_lan_begin(LanczosDev *L, hipStream_t s);
 void launch_lan_ctrl(LanczosDev *L, const AttemptDev *at, const double2 *dot, const double *nsq, int j,
                      double dtscale, int kmax, hipStream_t s);
 void launch_lan_coef(LanczosDev *L, const AttemptDev *at, double dtscale, int build, double2 *coef, hipStream_t s);
+void launch_dpsi_traces(const double2 *G, int ns, int ld, double2 *o, hipStream_t s);
 void launch_attempt_psi(AttemptDev *at, const double2 *o11, const double2 *o22, const double2 *o12, hipStream_t s);
 void launch_attempt_decide(AttemptDev *at, const LanczosDev *L1, const LanczosDev *L2, const double *n2,
                            double eps_int, const int *spf_status, hipStream_t s);
-void launch_copy_if(double2 *dst, const double2 *src, int64_t n, const int *flag, hipStream_t s);
+void launch_copy_if(double2 *dst, const double2 *src, int64_t n, const int *flag, hipStream_t s,
+                    const double *srcnorm2 = nullptr);
 
 }  // namespace mg

@@ -279,8 +279,15 @@ __device__ __forceinline__ void brick_op_generic(const T *src, T *dst, const T *
 // FP64 across bricks.
 template <class T> struct GramMfma;
 template <> struct GramMfma<float2> {
+    // X == Y (density matrices): one LDS load per step instead of two
     static __device__ __forceinline__ void run(const float2 *X, const float2 *Y, int n1, int n2, int nfib,
                                                int stride, FastDiv fds, double acc[4]) {
+        if (X == Y) run_t<true>(X, Y, n1, n2, nfib, stride, fds, acc);
+        else run_t<false>(X, Y, n1, n2, nfib, stride, fds, acc);
+    }
+    template <bool SAME>
+    static __device__ __forceinline__ void run_t(const float2 *X, const float2 *Y, int n1, int n2, int nfib,
+                                                 int stride, FastDiv fds, double acc[4]) {
         const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
         const int i = lane & 15, q = lane >> 4, e = i & 7, im = i >> 3;
         mg_floatx4 d = {0.f, 0.f, 0.f, 0.f};
@@ -298,7 +305,7 @@ template <> struct GramMfma<float2> {
                 const int ff = ok ? f : 0;
                 const int outer = (int)fdiv((uint32_t)ff, fds), inner = ff - outer * stride;
                 const float2 xv = X[inner + outer * stride * n1 + (e < n1 ? e : 0) * stride];
-                const float2 yv = Y[inner + outer * stride * n2 + (e < n2 ? e : 0) * stride];
+                const float2 yv = SAME ? xv : Y[inner + outer * stride * n2 + (e < n2 ? e : 0) * stride];
                 av[u] = (ok && e < n1) ? (im ? xv.y : xv.x) : 0.f;
                 bv[u] = (ok && e < n2) ? (im ? yv.y : yv.x) : 0.f;
             }
@@ -312,8 +319,15 @@ template <> struct GramMfma<float2> {
     static __device__ __forceinline__ int row(int q, int v) { return 4 * q + v; }
 };
 template <> struct GramMfma<double2> {
+    // X == Y (density matrices): one LDS load per step instead of two
     static __device__ __forceinline__ void run(const double2 *X, const double2 *Y, int n1, int n2, int nfib,
                                                int stride, FastDiv fds, double acc[4]) {
+        if (X == Y) run_t<true>(X, Y, n1, n2, nfib, stride, fds, acc);
+        else run_t<false>(X, Y, n1, n2, nfib, stride, fds, acc);
+    }
+    template <bool SAME>
+    static __device__ __forceinline__ void run_t(const double2 *X, const double2 *Y, int n1, int n2, int nfib,
+                                                 int stride, FastDiv fds, double acc[4]) {
         const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
         const int i = lane & 15, q = lane >> 4, e = i & 7, im = i >> 3;
         mg_doublex4 d = {0.0, 0.0, 0.0, 0.0};
@@ -331,7 +345,7 @@ template <> struct GramMfma<double2> {
                 const int ff = ok ? f : 0;
                 const int outer = (int)fdiv((uint32_t)ff, fds), inner = ff - outer * stride;
                 const double2 xv = X[inner + outer * stride * n1 + (e < n1 ? e : 0) * stride];
-                const double2 yv = Y[inner + outer * stride * n2 + (e < n2 ? e : 0) * stride];
+                const double2 yv = SAME ? xv : Y[inner + outer * stride * n2 + (e < n2 ? e : 0) * stride];
                 av[u] = (ok && e < n1) ? (im ? xv.y : xv.x) : 0.0;
                 bv[u] = (ok && e < n2) ? (im ? yv.y : yv.x) : 0.0;
             }
@@ -1036,10 +1050,15 @@ __global__ void __launch_bounds__(256) k_norm2_partial(const T *x, int64_t n, do
 // |y - x|^2 partials (complex FP64), the same accumulation as k_norm2_partial
 // of the stored difference: err_A of a CMF2 attempt without writing B02 - B0.
 __global__ void __launch_bounds__(256) k_diff_norm2_partial(const double2 *y, const double2 *x, int64_t n,
-                                                            double *partials) {
+                                                            double *partials, const double *ynorm2) {
+    // ynorm2: y is the unnormalised vector of squared norm ynorm2[0]; its
+    // normalised values are formed as k_scale forms them
+    const double d = ynorm2 ? sqrt(ynorm2[0]) : 1.0;
     double acc = 0.0;
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-        const double2 a = make_double2(y[i].x - x[i].x, y[i].y - x[i].y);
+        double2 yi = y[i];
+        if (ynorm2) yi = make_double2(yi.x / d, yi.y / d);
+        const double2 a = make_double2(yi.x - x[i].x, yi.y - x[i].y);
         acc = fma(a.x, a.x, acc);
         acc = fma(a.y, a.y, acc);
     }
@@ -1074,8 +1093,8 @@ void launch_norm2(const void *x, int64_t n, double *partials, int nblocks, doubl
 }
 
 void launch_diff_norm2(const double2 *y, const double2 *x, int64_t n, double *partials, int nblocks, double *out,
-                       hipStream_t s) {
-    hipLaunchKernelGGL(k_diff_norm2_partial, dim3(nblocks), dim3(256), 0, s, y, x, n, partials);
+                       hipStream_t s, const double *ynorm2) {
+    hipLaunchKernelGGL(k_diff_norm2_partial, dim3(nblocks), dim3(256), 0, s, y, x, n, partials, ynorm2);
     hipLaunchKernelGGL(k_norm2_final, dim3(1), dim3(64), 0, s, partials, nblocks, out, (const int *)nullptr);
 }
 
@@ -1883,6 +1902,23 @@ void launch_lan_coef(LanczosDev *L, const AttemptDev *at, double dtscale, int bu
 }
 
 // DeltaPsi (gmctdh_objects.f90:204-227) from the three overlaps.
+// o[k] = sum over states of the trace of DeltaPsi's overlap Gram (set k,
+// state s), in a fixed order
+__global__ void k_dpsi_traces(const double2 *G, int ns, int ld, double2 *o) {
+    const int k = threadIdx.x;
+    if (k >= 3) return;
+    double2 acc = make_double2(0.0, 0.0);
+    for (int s = 0; s < ns; s++) {
+        const double2 *g = G + (size_t)ld * ld * ((size_t)k * ns + s);
+        for (int i = 0; i < ld; i++) acc = make_double2(acc.x + g[i + ld * i].x, acc.y + g[i + ld * i].y);
+    }
+    o[k] = acc;
+}
+
+void launch_dpsi_traces(const double2 *G, int ns, int ld, double2 *o, hipStream_t s) {
+    hipLaunchKernelGGL(k_dpsi_traces, dim3(1), dim3(64), 0, s, G, ns, ld, o);
+}
+
 __global__ void k_attempt_psi(AttemptDev *at, const double2 *o11, const double2 *o22, const double2 *o12) {
     at->err_psi = sqrt(fabs(2.0 * (1.0 - o12->x / sqrt(o11->x * o22->x))));
 }
@@ -1912,14 +1948,20 @@ void launch_attempt_decide(AttemptDev *at, const LanczosDev *L1, const LanczosDe
     hipLaunchKernelGGL(k_attempt_decide, dim3(1), dim3(1), 0, s, at, L1, L2, n2, eps_int, spf_status);
 }
 
-__global__ void k_copy_if(double2 *dst, const double2 *src, int64_t n, const int *flag) {
+// srcnorm2: src is unnormalised with squared norm srcnorm2[0]; dst receives
+// it normalised as k_scale would
+__global__ void k_copy_if(double2 *dst, const double2 *src, int64_t n, const int *flag, const double *srcnorm2) {
     if (!*flag) return;
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-        dst[i] = src[i];
+    const double d = srcnorm2 ? sqrt(srcnorm2[0]) : 1.0;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const double2 a = src[i];
+        dst[i] = srcnorm2 ? make_double2(a.x / d, a.y / d) : a;
+    }
 }
 
-void launch_copy_if(double2 *dst, const double2 *src, int64_t n, const int *flag, hipStream_t s) {
-    hipLaunchKernelGGL(k_copy_if, dim3((unsigned)grid_for(n)), dim3(256), 0, s, dst, src, n, flag);
+void launch_copy_if(double2 *dst, const double2 *src, int64_t n, const int *flag, hipStream_t s,
+                    const double *srcnorm2) {
+    hipLaunchKernelGGL(k_copy_if, dim3((unsigned)grid_for(n)), dim3(256), 0, s, dst, src, n, flag, srcnorm2);
 }
 
 }  // namespace mg
