@@ -155,6 +155,7 @@ struct mg_ctx {
     std::vector<int> mf_partner;            // per chain term: mirror term index, -1 none, -2 derived
     std::vector<int> rmf_jobs;
     int n_rmf_jobs = 0;
+    size_t spf_scr_elems = 0;      // complex elements of the SPF propagation scratch (all grid modes)
     // ---- grouping ----
     int brick = 1024;
     int run_bytes = 32;            // shortest row of a strided brick (make_groups)
@@ -831,9 +832,30 @@ void allocate(mg_ctx *c) {
             nop++;
         }
         d.nop = nop;
+        d.nuq = 0;
+        for (int q = 0; q < nop; q++) {
+            int u = 0;
+            for (; u < d.nuq; u++) {
+                bool same = d.uq_set[u] == d.op_rhs_set[q];
+                for (int k = 0; k < kMaxDofPerMode; k++) same = same && d.uq_codes[u][k] == d.op_codes[q][k];
+                if (same) break;
+            }
+            if (u == d.nuq) {
+                d.uq_set[u] = d.op_rhs_set[q];
+                d.uq_ident[u] = 1;
+                for (int k = 0; k < kMaxDofPerMode; k++) {
+                    d.uq_codes[u][k] = d.op_codes[q][k];
+                    if (d.op_codes[q][k] != 0) d.uq_ident[u] = 0;
+                }
+                d.nuq++;
+            }
+            d.op_uq[q] = u;
+        }
         d.scr_off = (int)scr;
-        scr += (size_t)10 * blk + (size_t)2 * d.nG * c->nspfmax + (size_t)2 * kMaxSets * nn;
+        scr += (size_t)10 * blk + (size_t)2 * d.nG * c->nspfmax + (size_t)2 * kMaxSets * nn +
+               (size_t)d.nuq * d.nG * c->nspfmax;
     }
+    c->spf_scr_elems = scr;
     c->ndvr_desc = (int)md.size();
     c->dspfmodes.alloc(sizeof(SpfModeDesc) * std::max<size_t>(1, md.size()));
     if (!md.empty()) hcopy(c->dspfmodes.p, md.data(), sizeof(SpfModeDesc) * md.size(), hipMemcpyHostToDevice);
@@ -938,16 +960,9 @@ DofTables dof_tables(const mg_ctx *c) {
 }
 
 size_t spf_scratch_base(const mg_ctx *c) {
-    // SPF propagation scratch comes first; SPF-product task scratch after it
-    size_t scr = 0;
-    const size_t nn = (size_t)c->nspfmax * c->nspfmax;
-    for (int m = 0; m < c->nm; m++) {
-        if (c->gwp[m]) continue;
-        size_t blk = 0;
-        for (int s = 0; s < c->nsets(m); s++) blk += (size_t)c->ngpm[m] * c->NS(m, s);
-        scr += 10 * blk + 2 * (size_t)c->ngpm[m] * c->nspfmax + 2 * nn;
-    }
-    return scr;
+    // SPF propagation scratch comes first (every grid mode's region, as the
+    // descriptors lay it out); SPF-product task scratch after it
+    return c->spf_scr_elems;
 }
 
 // ----------------------------------------------------------------------------

@@ -504,7 +504,7 @@ __device__ void store_mode(const GwpDesc &d, int nmax, int nm, const double2 *y,
 
 __global__ void __launch_bounds__(64) k_gwp_s00(const GwpDesc *__restrict__ descs, const double2 *psi, GwpParams P,
                                                 double2 *s00, double2 *sd) {
-    const GwpDesc d = descs[blockIdx.x];
+    const GwpDesc &d = descs[blockIdx.x];
     const int nmax = P.nspfmax, nn = nmax * nmax, m = d.mode;
     // set pairs (intra- and, for multi-set modes, inter-state)
     for (int e = threadIdx.x; e < d.nsets * d.nsets * nn; e += blockDim.x) {
@@ -532,7 +532,7 @@ __global__ void __launch_bounds__(64) k_gwp_s00(const GwpDesc *__restrict__ desc
 
 __global__ void __launch_bounds__(64) k_gwp_h1(const GwpDesc *__restrict__ descs, const double2 *psi, GwpParams P,
                                                const double2 *s00, double2 *h1) {
-    const GwpDesc d = descs[blockIdx.x];
+    const GwpDesc &d = descs[blockIdx.x];
     const int nmax = P.nspfmax, nn = nmax * nmax, m = d.mode;
     int bad = 0;
     for (int e = threadIdx.x; e < d.nop * nn; e += blockDim.x) {
@@ -560,7 +560,7 @@ __global__ void __launch_bounds__(64) k_gwp_h1(const GwpDesc *__restrict__ descs
 
 __global__ void __launch_bounds__(64) k_gwp_overlaps(const GwpDesc *__restrict__ descs, const double2 *psi1,
                                                      const double2 *psi2, int sym, GwpParams P, double2 *out) {
-    const GwpDesc d = descs[blockIdx.x];
+    const GwpDesc &d = descs[blockIdx.x];
     const int nmax = P.nspfmax, nn = nmax * nmax, m = d.mode;
     for (int e = threadIdx.x; e < P.ns * nn; e += blockDim.x) {
         const int st = e / nn, r = e % nn, j1 = r % nmax, j2 = r / nmax;
@@ -575,7 +575,7 @@ __global__ void __launch_bounds__(64) k_gwp_overlaps(const GwpDesc *__restrict__
 // D = U^-1 with S = U^H U, one thread per (mode, set).
 __global__ void k_gwp_dd_init(const GwpDesc *__restrict__ descs, const double2 *sd, GwpParams P, double2 *dd,
                               int *status) {
-    const GwpDesc d = descs[blockIdx.x];
+    const GwpDesc &d = descs[blockIdx.x];
     const int s = threadIdx.x;
     if (s >= d.nsets) return;
     const int nmax = P.nspfmax, nn = nmax * nmax, n = d.nspf[s];
@@ -611,7 +611,7 @@ __global__ void k_gwp_dd_init(const GwpDesc *__restrict__ descs, const double2 *
 // D^H h D per operator, one thread per operator of the mode.
 __global__ void k_gwp_dhd(const GwpDesc *__restrict__ descs, const double2 *h1, const double2 *dd, GwpParams P,
                           double2 *dhd) {
-    const GwpDesc d = descs[blockIdx.x];
+    const GwpDesc &d = descs[blockIdx.x];
     const int q = threadIdx.x;
     if (q >= d.nop) return;
     const int nmax = P.nspfmax, nn = nmax * nmax;
@@ -671,7 +671,7 @@ __global__ void k_gwp_dhd(const GwpDesc *__restrict__ descs, const double2 *h1, 
 // and mats[nn * nm * ns ..].
 __global__ void k_gwp_ab_mats(const GwpDesc *__restrict__ descs, const double2 *sd, const double2 *dd, GwpParams P,
                               double2 *mats) {
-    const GwpDesc d = descs[blockIdx.x];
+    const GwpDesc &d = descs[blockIdx.x];
     const int nmax = P.nspfmax, nn = nmax * nmax;
     const size_t half = (size_t)nn * P.nm * P.ns;
     for (int e = threadIdx.x; e < P.ns * nn; e += blockDim.x) {
@@ -696,7 +696,7 @@ __global__ void k_gwp_ab_mats(const GwpDesc *__restrict__ descs, const double2 *
 // 4 unsupported operator.
 __global__ void __launch_bounds__(64) k_gwp_propagate(const GwpDesc *__restrict__ descs, double2 *psi, double2 *dd,
                                                       double2 *scratch, GwpParams P, int *status) {
-    const GwpDesc d = descs[blockIdx.x];
+    const GwpDesc &d = descs[blockIdx.x];
     const int tid = threadIdx.x, nth = blockDim.x;
     const int nmax = P.nspfmax, nn = nmax * nmax, nD = nn * d.nsets;
     GwpScratch S = gwp_carve(scratch, d, nmax);
@@ -768,7 +768,7 @@ __global__ void __launch_bounds__(64) k_gwp_propagate(const GwpDesc *__restrict_
 __global__ void __launch_bounds__(64) k_gwp_der(const GwpDesc *__restrict__ descs, int idesc, const double2 *psi,
                                                 const double2 *dd, double2 *dpsi, double2 *ddd, double2 *scratch,
                                                 GwpParams P, int *status) {
-    const GwpDesc d = descs[idesc];
+    const GwpDesc &d = descs[idesc];
     const int nmax = P.nspfmax;
     GwpScratch S = gwp_carve(scratch, d, nmax);
     __shared__ int fail;
@@ -783,7 +783,7 @@ __global__ void __launch_bounds__(64) k_gwp_der(const GwpDesc *__restrict__ desc
 // Tikhonov inverse of the intra-state overlap per (mode, state) (Calc_S00m1,
 // gmctdh_objects.f90:385-404), one thread per state.
 __global__ void k_gwp_s00m1(const GwpDesc *__restrict__ descs, const double2 *sd, GwpParams P, double2 *out) {
-    const GwpDesc d = descs[blockIdx.x];
+    const GwpDesc &d = descs[blockIdx.x];
     const int st = threadIdx.x;
     if (st >= P.ns) return;
     const int nmax = P.nspfmax, nn = nmax * nmax, set = d.nsets == 1 ? 0 : st, n = d.nspf[set];
@@ -797,7 +797,7 @@ __global__ void k_gwp_s00m1(const GwpDesc *__restrict__ descs, const double2 *sd
 // dst(m, state) = scale * src(m, set of the state), per Gaussian mode.
 __global__ void k_gwp_state_mats(const GwpDesc *__restrict__ descs, const double2 *src, double scale, GwpParams P,
                                  double2 *dst) {
-    const GwpDesc d = descs[blockIdx.x];
+    const GwpDesc &d = descs[blockIdx.x];
     const int nn = P.nspfmax * P.nspfmax;
     for (int e = threadIdx.x; e < P.ns * nn; e += blockDim.x) {
         const int st = e / nn, r = e % nn, set = d.nsets == 1 ? 0 : st;

@@ -169,6 +169,13 @@ struct SpfModeDesc {
     int rho_index[8];            // index of the density matrix per set (into rho array)
     int blk;                     // complex elements of all set blocks of the mode
     int scr_off;                 // offset into the scratch pool (complex elements)
+    // the distinct (primitive codes, rhs set) products of the operators,
+    // each formed once per derivative; op_uq: an operator's product
+    int nuq;
+    int uq_set[32];
+    int uq_ident[32];            // all codes zero: the SPFs themselves
+    int uq_codes[32][kMaxDofPerMode];
+    int op_uq[32];
 };
 
 // Gaussian (G-MCTDH) mode: Gaussians exp(A x^2 + xi x) per DOF with fixed

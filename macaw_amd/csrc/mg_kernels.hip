@@ -1462,7 +1462,7 @@ void launch_rmf(const double2 *rho, const double2 *mf, double2 *rmf, const int *
 // ---------------------------------------------------------------------------
 // SPF propagation: one workgroup per logical mode.
 struct SpfScratch {
-    double2 *y0, *k[7], *tmp, *perr, *sect, *sect2, *ovl, *od;
+    double2 *y0, *k[7], *tmp, *perr, *sect, *sect2, *ovl, *od, *prod;
 };
 
 __device__ SpfScratch carve(double2 *base, const SpfModeDesc &d, int nspfmax) {
@@ -1475,7 +1475,8 @@ __device__ SpfScratch carve(double2 *base, const SpfModeDesc &d, int nspfmax) {
     s.sect = p; p += d.nG * nspfmax;
     s.sect2 = p; p += d.nG * nspfmax;
     s.ovl = p; p += (size_t)kMaxSets * nspfmax * nspfmax;     // one n x n block per set
-    s.od = p;                                                 // kMaxSets blocks as well
+    s.od = p; p += (size_t)kMaxSets * nspfmax * nspfmax;      // kMaxSets blocks as well
+    s.prod = p;                                               // d.nuq blocks of nG x nspfmax
     return s;
 }
 
@@ -1489,25 +1490,38 @@ __device__ void dev_der_spf(const SpfModeDesc &d, const SpfParams &P, const DofT
         int o = 0;
         for (int s = 0; s < d.nsets; s++) { loff[s] = o; o += nG * d.nspf[s]; }
     }
-    for (int e = tid; e < d.blk; e += nth) out[e] = zero2();
-    __syncthreads();
-    for (int q = 0; q < d.nop; q++) {
-        const int s1 = d.op_lhs_set[q], s2 = d.op_rhs_set[q];
-        const int n1 = d.nspf[s1], n2 = d.nspf[s2];
-        for (int e = tid; e < nG * n2; e += nth) S.sect[e] = in[loff[s2] + e];
+    // the primitive products h phi of the distinct (codes, rhs set) pairs,
+    // each formed once (several operators share one: the same one-mode
+    // operator in several mean-field terms)
+    for (int u = 0; u < d.nuq; u++) {
+        if (d.uq_ident[u]) continue;
+        const int s2 = d.uq_set[u], n2 = d.nspf[s2];
+        double2 *Pu = S.prod + (size_t)u * nG * nmax;
+        for (int e = tid; e < nG * n2; e += nth) Pu[e] = in[loff[s2] + e];
         __syncthreads();
         for (int k = 0; k < dt.mode_ndof[d.mode]; k++)
-            if (d.op_codes[q][k] != 0) apply_prim_block(dt, d.mode, k, d.op_codes[q][k], n2, nG, S.sect, S.sect2);
-        const double2 *r = P.rmf + (int64_t)nmax * nmax * d.op_index[q];
-        double2 *o = out + loff[s1];
-        for (int e = tid; e < nG * n1; e += nth) {
-            const int g = e % nG, j1 = e / nG;
-            double2 acc = zero2();
-            for (int j2 = 0; j2 < n2; j2++) cfma(acc, r[j1 + nmax * j2], S.sect[g + nG * j2]);
-            o[e] = cadd(o[e], cmul(P.der_factor, acc));
-        }
-        __syncthreads();
+            if (d.uq_codes[u][k] != 0) apply_prim_block(dt, d.mode, k, d.uq_codes[u][k], n2, nG, Pu, S.sect2);
     }
+    __syncthreads();
+    // out(g, j1) of set s1 = sum over the operators q into s1, in operator
+    // order, of der_factor * (rmf_q h_q phi_s2)(g, j1)
+    for (int e = tid; e < d.blk; e += nth) {
+        int s1 = 0;
+        while (s1 + 1 < d.nsets && e >= loff[s1 + 1]) s1++;
+        const int loc = e - loff[s1], g = loc % nG, j1 = loc / nG;
+        double2 o = zero2();
+        for (int q = 0; q < d.nop; q++) {
+            if (d.op_lhs_set[q] != s1) continue;
+            const int s2 = d.op_rhs_set[q], n2 = d.nspf[s2], u = d.op_uq[q];
+            const double2 *src = d.uq_ident[u] ? in + loff[s2] : S.prod + (size_t)u * nG * nmax;
+            const double2 *r = P.rmf + (int64_t)nmax * nmax * d.op_index[q];
+            double2 acc = zero2();
+            for (int j2 = 0; j2 < n2; j2++) cfma(acc, r[j1 + nmax * j2], src[g + nG * j2]);
+            o = cadd(o, cmul(P.der_factor, acc));
+        }
+        out[e] = o;
+    }
+    __syncthreads();
     // projector: d <- d - phi (phi^H phi)^-1 phi^H d, every set at once:
     // the sets' overlaps in parallel, each set's Cholesky solve on lane 0 of
     // its own wave, then the updates (each set's arithmetic is unchanged)
@@ -1643,7 +1657,7 @@ __constant__ double c_dpD[7] = {5179.0 / 57600.0, 0, 7571.0 / 16695.0, 393.0 / 6
 // attempts.
 __global__ void __launch_bounds__(256) k_spf_propagate(const SpfModeDesc *__restrict__ modes, double2 *psi,
                                                        double2 *scratch, SpfParams P, DofTables dt, int *status) {
-    const SpfModeDesc d = modes[blockIdx.x];
+    const SpfModeDesc &d = modes[blockIdx.x];
     const int tid = threadIdx.x, nth = blockDim.x;
     SpfScratch S = carve(scratch, d, P.nspfmax);
     __shared__ int fail;
@@ -1748,7 +1762,7 @@ __global__ void __launch_bounds__(256) k_spf_propagate(const SpfModeDesc *__rest
 __global__ void __launch_bounds__(256) k_der_spf(const SpfModeDesc *__restrict__ modes, int mode, const double2 *psi,
                                                  double2 *dpsi, double2 *scratch, SpfParams P, DofTables dt,
                                                  int *status) {
-    const SpfModeDesc d = modes[mode >= 0 ? mode : blockIdx.x];
+    const SpfModeDesc &d = modes[mode >= 0 ? mode : blockIdx.x];
     const int tid = threadIdx.x, nth = blockDim.x;
     SpfScratch S = carve(scratch, d, P.nspfmax);
     __shared__ int fail;
