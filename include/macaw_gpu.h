@@ -91,7 +91,10 @@ typedef struct mg_ctx mg_ctx;
 
 /* ~ init_calc.f90:1-226 (tables) + alloc_h1mode/alloc_IsIdentity
  * (hamilton.f90:91-373) + SetupCMF2 (cmf2_base.f90:15-43).  precision:
- * 0 = complex FP64 everywhere. */
+ * 0 = complex FP64 everywhere; 1 = mixed: the H.A chains of every Krylov
+ * product, the Krylov vectors after the start vector and the mean-field
+ * chains in complex FP32, the A-vector, SPFs, density matrices, mean-field
+ * matrices, reductions and DeltaPsi in FP64. */
 int mg_create(const mg_problem *prob, int64_t precision, mg_ctx **ctx);
 int mg_destroy(mg_ctx *ctx);
 const char *mg_last_error(const mg_ctx *ctx);

@@ -815,6 +815,7 @@ void allocate(mg_ctx *c) {
         int blk = 0;
         for (int s = 0; s < d.nsets; s++) {
             d.off[s] = (int)c->IP(m, s);
+            d.loc_off[s] = blk;
             d.nspf[s] = (int)c->NS(m, s);
             d.rho_index[s] = c->MultiSet[m] ? (m + c->nm * s) : (c->nm * c->ns + m);
             blk += d.nG * d.nspf[s];
@@ -850,6 +851,17 @@ void allocate(mg_ctx *c) {
                 d.nuq++;
             }
             d.op_uq[q] = u;
+        }
+        d.prod_direct = c->nmodeDOF[m] == 1 ? 1 : 0;
+        d.nprod = 0;
+        d.prod_off[0] = 0;
+        for (int u = 0; u < d.nuq; u++) {
+            if (d.uq_ident[u]) continue;
+            const int code = d.uq_codes[u][0];
+            if (!((code >= 1 && code <= 200) || code == -1 || code == -2 || code == -3)) d.prod_direct = 0;
+            d.prod_uq[d.nprod] = u;
+            d.prod_off[d.nprod + 1] = d.prod_off[d.nprod] + d.nG * d.nspf[d.uq_set[u]];
+            d.nprod++;
         }
         d.scr_off = (int)scr;
         scr += (size_t)10 * blk + (size_t)2 * d.nG * c->nspfmax + (size_t)2 * kMaxSets * nn +

@@ -161,6 +161,7 @@ struct SpfModeDesc {
     int nG;
     int nsets;
     int off[8];                  // offsets (complex elements) of the set blocks in psi
+    int loc_off[8];              // offsets of the set blocks in the mode-local block
     int nspf[8];                 // SPFs per set
     int nop;                     // operators of the mode (non-identity)
     int op_lhs_set[32], op_rhs_set[32];
@@ -176,6 +177,13 @@ struct SpfModeDesc {
     int uq_ident[32];            // all codes zero: the SPFs themselves
     int uq_codes[32][kMaxDofPerMode];
     int op_uq[32];
+    // one-DOF modes whose products are powers of x or DVR matrices: formed in
+    // one pass over the nprod non-identity products (product k: uq index
+    // prod_uq[k], elements [prod_off[k], prod_off[k + 1]) of the pass)
+    int prod_direct;
+    int nprod;
+    int prod_uq[32];
+    int prod_off[33];
 };
 
 // Gaussian (G-MCTDH) mode: Gaussians exp(A x^2 + xi x) per DOF with fixed

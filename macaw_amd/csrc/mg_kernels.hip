@@ -17,6 +17,13 @@
 #include <algorithm>
 #include <cstdlib>
 #include <cstring>
+#include <type_traits>
+
+// 1: FP32 ops along a unit-stride mode use mfma_op8_unit (build with
+// -DMG_UNIT_STRIDE_OP=0 for the general lane layout, for comparisons)
+#ifndef MG_UNIT_STRIDE_OP
+#define MG_UNIT_STRIDE_OP 1
+#endif
 
 namespace mg {
 
@@ -130,9 +137,58 @@ template <> struct MfmaTraits<double2> {
 // takes part (MFMA needs full EXEC); lanes past nfib load zeros and skip the
 // store.  In place (src == dst) is safe: a wave reads its 16 fibers before
 // it writes them, and no other wave touches them.
-template <class T>
+// FP32 op along a unit-stride mode (the first mode of the leading group):
+// lane group q takes components 2q and 2q+1 of its fiber in one 16-byte LDS
+// access (a wave's 64 lanes read 1 KiB contiguously, no bank conflicts), the
+// real embedding of the op matrix is ordered to match, and the outputs
+// y_2q, y_2q+1 of the FP32 layout are contiguous too.  Same products as
+// mfma_op8, summed in another order.
+__device__ __forceinline__ void mfma_op8_unit(const float2 *src, float2 *dst, const float2 *M, int nfib, int acc) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    const int j = lane & 15, q = lane >> 4;
+    float a[4];
+    {
+        const int y = j >> 1, po = j & 1;
+#pragma unroll
+        for (int c = 0; c < 4; c++) {
+            const int x = 2 * q + (c >> 1), pi = c & 1;
+            const float2 m = M[y + 8 * x];
+            a[c] = (po == 0) ? (pi == 0 ? m.x : -m.y) : (pi == 0 ? m.y : m.x);
+        }
+    }
+    for (int g = wave; g * 16 < nfib; g += nw) {
+        const int f = g * 16 + j;
+        const bool valid = f < nfib;
+        const int base = (valid ? f : 0) * 8 + 2 * q;
+        float4 xv = *reinterpret_cast<const float4 *>(src + base);
+        if (!valid) xv = make_float4(0.f, 0.f, 0.f, 0.f);
+        mg_floatx4 d;
+        if (acc) {
+            const float4 cv = *reinterpret_cast<const float4 *>(dst + base);
+            d[0] = cv.x; d[1] = cv.y; d[2] = cv.z; d[3] = cv.w;
+        } else {
+            d[0] = 0.f; d[1] = 0.f; d[2] = 0.f; d[3] = 0.f;
+        }
+        d = __builtin_amdgcn_mfma_f32_16x16x4f32(a[0], xv.x, d, 0, 0, 0);
+        d = __builtin_amdgcn_mfma_f32_16x16x4f32(a[1], xv.y, d, 0, 0, 0);
+        d = __builtin_amdgcn_mfma_f32_16x16x4f32(a[2], xv.z, d, 0, 0, 0);
+        d = __builtin_amdgcn_mfma_f32_16x16x4f32(a[3], xv.w, d, 0, 0, 0);
+        if (valid) *reinterpret_cast<float4 *>(dst + base) = make_float4(d[0], d[1], d[2], d[3]);
+    }
+}
+
+// UNIT: take the unit-stride path where it applies (the streaming brick
+// programs; the Gram programs keep one code path: the second one costs them
+// registers and occupancy, measured 18% slower mean fields)
+template <class T, bool UNIT>
 __device__ __forceinline__ void mfma_op8(const T *src, T *dst, const T *M, int nfib, int stride, FastDiv fds,
                                          int acc) {
+    if constexpr (UNIT && std::is_same<T, float2>::value) {
+        if (MG_UNIT_STRIDE_OP && stride == 1) {
+            mfma_op8_unit(src, dst, M, nfib, acc);
+            return;
+        }
+    }
     typedef MfmaTraits<T> Tr;
     typedef typename Tr::S S;
     typedef typename Tr::V V;
@@ -421,7 +477,7 @@ __global__ void __launch_bounds__(GRAM ? 256 : 1024) k_brick(const BrickTask *__
             } else if (code == kBOp) {
                 const T *Mq = mats + I.mat * NMAX * NMAX;
                 if (NMAX == 8 && I.ni == 8 && I.no == 8)
-                    mfma_op8<T>(A, B, Mq, I.nfib, I.stride, I.fd_stride, I.acc);
+                    mfma_op8<T, !GRAM>(A, B, Mq, I.nfib, I.stride, I.fd_stride, I.acc);
                 else
                     brick_op_generic<NMAX, T>(A, B, Mq, I.ni, I.no, I.nfib, I.stride, I.fd_stride, I.acc);
             } else if (code == kBAxpy) {
@@ -1461,15 +1517,19 @@ void launch_rmf(const double2 *rho, const double2 *mf, double2 *rmf, const int *
 
 // ---------------------------------------------------------------------------
 // SPF propagation: one workgroup per logical mode.
+// k: the seven Runge-Kutta stage derivatives, stage q at k + q * blk
 struct SpfScratch {
-    double2 *y0, *k[7], *tmp, *perr, *sect, *sect2, *ovl, *od, *prod;
+    double2 *y0, *k, *tmp, *perr, *sect, *sect2, *ovl, *od, *prod;
+    int blk;
+    __device__ double2 *stage(int q) const { return k + (size_t)q * blk; }
 };
 
 __device__ SpfScratch carve(double2 *base, const SpfModeDesc &d, int nspfmax) {
     SpfScratch s;
     double2 *p = base + d.scr_off;
     s.y0 = p; p += d.blk;
-    for (int i = 0; i < 7; i++) { s.k[i] = p; p += d.blk; }
+    s.k = p; p += (size_t)7 * d.blk;
+    s.blk = d.blk;
     s.tmp = p; p += d.blk;
     s.perr = p; p += d.blk;
     s.sect = p; p += d.nG * nspfmax;
@@ -1480,27 +1540,71 @@ __device__ SpfScratch carve(double2 *base, const SpfModeDesc &d, int nspfmax) {
     return s;
 }
 
+// The projector's and the error norm's small matrices (read serially by one
+// lane per set) in LDS rather than in the global scratch pool; the launch
+// provides spf_lds_bytes(nspfmax) of dynamic LDS.
+__device__ __forceinline__ void spf_lds_overlaps(SpfScratch &S, int nspfmax) {
+    extern __shared__ double2 spf_lds[];
+    S.ovl = spf_lds;
+    S.od = spf_lds + (size_t)kMaxSets * nspfmax * nspfmax;
+}
+static size_t spf_lds_bytes(int nspfmax) { return sizeof(double2) * 2 * kMaxSets * (size_t)nspfmax * nspfmax; }
+// 16 SPFs and 8 sets take 64 KiB plus the kernels' static LDS: allow more
+// than the default dynamic limit, once per kernel (thread-safe statics)
+template <class K> static void spf_allow_lds(K kernel) {
+    (void)hipFuncSetAttribute((const void *)kernel, hipFuncAttributeMaxDynamicSharedMemorySize, 96 * 1024);
+}
+
 // Calc_Der_SPF, DVR branch, on mode-local blocks (cmf2_derivatives.f90:118-173).
 __device__ void dev_der_spf(const SpfModeDesc &d, const SpfParams &P, const DofTables &dt, const double2 *in,
                             double2 *out, SpfScratch &S, int *fail) {
     const int tid = threadIdx.x, nth = blockDim.x;
     const int nG = d.nG, nmax = P.nspfmax;
-    int loff[8];
-    {
-        int o = 0;
-        for (int s = 0; s < d.nsets; s++) { loff[s] = o; o += nG * d.nspf[s]; }
-    }
+    const int *loff = d.loc_off;
     // the primitive products h phi of the distinct (codes, rhs set) pairs,
     // each formed once (several operators share one: the same one-mode
-    // operator in several mean-field terms)
-    for (int u = 0; u < d.nuq; u++) {
-        if (d.uq_ident[u]) continue;
-        const int s2 = d.uq_set[u], n2 = d.nspf[s2];
-        double2 *Pu = S.prod + (size_t)u * nG * nmax;
-        for (int e = tid; e < nG * n2; e += nth) Pu[e] = in[loff[s2] + e];
-        __syncthreads();
-        for (int k = 0; k < dt.mode_ndof[d.mode]; k++)
-            if (d.uq_codes[u][k] != 0) apply_prim_block(dt, d.mode, k, d.uq_codes[u][k], n2, nG, Pu, S.sect2);
+    // operator in several mean-field terms).  One-DOF modes with powers of x
+    // or DVR matrices form them all in one pass, element by element, with
+    // apply_prim_block's arithmetic; other modes apply the DOFs in turn.
+    if (d.prod_direct) {
+        const int dof = dt.mode_dofs[d.mode * kMaxDofPerMode];
+        const double *x = dt.grid + (int64_t)dt.ngpmax * dof;
+        for (int i = tid; i < d.prod_off[d.nprod]; i += nth) {
+            int k = 0;
+            while (k + 1 < d.nprod && i >= d.prod_off[k + 1]) k++;
+            const int u = d.prod_uq[k], e = i - d.prod_off[k];
+            const int s2 = d.uq_set[u], code = d.uq_codes[u][0];
+            const double2 *c0 = in + loff[s2];
+            const int g2 = e % nG, j = e / nG;
+            double2 v;
+            if (code >= 1) {
+                double f = x[g2];
+                for (int q = 1; q < code; q++) f *= x[g2];
+                v = cscale(f, c0[e]);
+            } else {
+                const double *Mat = (code == -1 ? dt.d1 : (code == -2 ? dt.d2 : dt.xd1)) +
+                                    (int64_t)dt.ngpmax * dt.ngpmax * dof;
+                const double2 *c = c0 + (int64_t)j * nG;
+                double2 acc = zero2();
+                for (int g = 0; g < nG; g++) {
+                    const double m = Mat[g + (int64_t)dt.ngpmax * g2];
+                    acc.x = fma(m, c[g].x, acc.x);
+                    acc.y = fma(m, c[g].y, acc.y);
+                }
+                v = (code == -1 || code == -3) ? make_double2(-acc.y, acc.x) : acc;
+            }
+            S.prod[(size_t)u * nG * nmax + e] = v;
+        }
+    } else {
+        for (int u = 0; u < d.nuq; u++) {
+            if (d.uq_ident[u]) continue;
+            const int s2 = d.uq_set[u], n2 = d.nspf[s2];
+            double2 *Pu = S.prod + (size_t)u * nG * nmax;
+            for (int e = tid; e < nG * n2; e += nth) Pu[e] = in[loff[s2] + e];
+            __syncthreads();
+            for (int k = 0; k < dt.mode_ndof[d.mode]; k++)
+                if (d.uq_codes[u][k] != 0) apply_prim_block(dt, d.mode, k, d.uq_codes[u][k], n2, nG, Pu, S.sect2);
+        }
     }
     __syncthreads();
     // out(g, j1) of set s1 = sum over the operators q into s1, in operator
@@ -1660,6 +1764,7 @@ __global__ void __launch_bounds__(256) k_spf_propagate(const SpfModeDesc *__rest
     const SpfModeDesc &d = modes[blockIdx.x];
     const int tid = threadIdx.x, nth = blockDim.x;
     SpfScratch S = carve(scratch, d, P.nspfmax);
+    spf_lds_overlaps(S, P.nspfmax);
     __shared__ int fail;
     if (tid == 0) fail = 0;
     // gather the mode's set blocks
@@ -1682,47 +1787,47 @@ __global__ void __launch_bounds__(256) k_spf_propagate(const SpfModeDesc *__rest
         // ---- one adaptive RK step (rk_SPF_ck / rk_SPF_dp) ----
         double h = stepsize, error = 0.0;
         for (;;) {
-            dev_der_spf(d, P, dt, S.y0, S.k[0], S, &fail);
+            dev_der_spf(d, P, dt, S.y0, S.stage(0), S, &fail);
             for (int st = 1; st < 6; st++) {
                 for (int e = tid; e < d.blk; e += nth) {
                     double2 v;
                     if (st == 1) {
                         const double c = (dp ? c_dpB[1][0] : c_ckB[1][0]) * h;
-                        v = cadd(S.y0[e], cscale(c, S.k[0][e]));
+                        v = cadd(S.y0[e], cscale(c, S.stage(0)[e]));
                     } else {
                         double2 acc = zero2();
                         for (int q = 0; q < st; q++) {
                             const double b = dp ? c_dpB[st][q] : c_ckB[st][q];
-                            acc = cadd(acc, cscale(b, S.k[q][e]));
+                            acc = cadd(acc, cscale(b, S.stage(q)[e]));
                         }
                         v = cadd(S.y0[e], cscale(h, acc));
                     }
                     S.tmp[e] = v;
                 }
                 __syncthreads();
-                dev_der_spf(d, P, dt, S.tmp, S.k[st], S, &fail);
+                dev_der_spf(d, P, dt, S.tmp, S.stage(st), S, &fail);
             }
             for (int e = tid; e < d.blk; e += nth) {
                 double2 acc = zero2();
                 for (int q = 0; q < 6; q++) {
                     const double c = dp ? c_dpC[q] : c_ckC[q];
-                    if (c != 0.0) acc = cadd(acc, cscale(c, S.k[q][e]));
+                    if (c != 0.0) acc = cadd(acc, cscale(c, S.stage(q)[e]));
                 }
                 S.tmp[e] = cadd(S.y0[e], cscale(h, acc));
                 if (!dp) {
                     double2 ae = zero2();
                     for (int q = 0; q < 6; q++)
-                        if (c_ckD[q] != 0.0) ae = cadd(ae, cscale(c_ckD[q], S.k[q][e]));
+                        if (c_ckD[q] != 0.0) ae = cadd(ae, cscale(c_ckD[q], S.stage(q)[e]));
                     S.perr[e] = cadd(S.y0[e], cscale(h, ae));
                 }
             }
             __syncthreads();
             if (dp) {
-                dev_der_spf(d, P, dt, S.tmp, S.k[6], S, &fail);
+                dev_der_spf(d, P, dt, S.tmp, S.stage(6), S, &fail);
                 for (int e = tid; e < d.blk; e += nth) {
                     double2 ae = zero2();
                     for (int q = 0; q < 7; q++)
-                        if (c_dpD[q] != 0.0) ae = cadd(ae, cscale(c_dpD[q], S.k[q][e]));
+                        if (c_dpD[q] != 0.0) ae = cadd(ae, cscale(c_dpD[q], S.stage(q)[e]));
                     S.perr[e] = cadd(S.y0[e], cscale(h, ae));
                 }
                 __syncthreads();
@@ -1765,6 +1870,7 @@ __global__ void __launch_bounds__(256) k_der_spf(const SpfModeDesc *__restrict__
     const SpfModeDesc &d = modes[mode >= 0 ? mode : blockIdx.x];
     const int tid = threadIdx.x, nth = blockDim.x;
     SpfScratch S = carve(scratch, d, P.nspfmax);
+    spf_lds_overlaps(S, P.nspfmax);
     __shared__ int fail;
     if (tid == 0) fail = 0;
     int o = 0;
@@ -1774,11 +1880,11 @@ __global__ void __launch_bounds__(256) k_der_spf(const SpfModeDesc *__restrict__
         o += n;
     }
     __syncthreads();
-    dev_der_spf(d, P, dt, S.y0, S.k[0], S, &fail);
+    dev_der_spf(d, P, dt, S.y0, S.stage(0), S, &fail);
     o = 0;
     for (int s = 0; s < d.nsets; s++) {
         const int n = d.nG * d.nspf[s];
-        for (int e = tid; e < n; e += nth) dpsi[d.off[s] + e] = S.k[0][o + e];
+        for (int e = tid; e < n; e += nth) dpsi[d.off[s] + e] = S.stage(0)[o + e];
         o += n;
     }
     if (tid == 0 && fail) atomicMax(status, 1);
@@ -1786,13 +1892,19 @@ __global__ void __launch_bounds__(256) k_der_spf(const SpfModeDesc *__restrict__
 
 void launch_der_spf(const SpfModeDesc *d_modes, int mode, const double2 *psi, double2 *dpsi, double2 *scratch,
                     SpfParams p, DofTables dt, int *d_status, hipStream_t s) {
-    hipLaunchKernelGGL(k_der_spf, dim3(mode >= 0 ? 1 : -mode), dim3(256), 0, s, d_modes, mode, psi, dpsi, scratch, p,
+    static const bool lds_set = (spf_allow_lds(k_der_spf), true);
+    (void)lds_set;
+    hipLaunchKernelGGL(k_der_spf, dim3(mode >= 0 ? 1 : -mode), dim3(256), spf_lds_bytes(p.nspfmax), s, d_modes, mode,
+                       psi, dpsi, scratch, p,
                        dt, d_status);
 }
 
 void launch_spf_propagate(const SpfModeDesc *d_modes, int nmodes, double2 *psi, double2 *scratch, SpfParams p,
                           DofTables dt, int *d_status, hipStream_t s) {
-    hipLaunchKernelGGL(k_spf_propagate, dim3(nmodes), dim3(256), 0, s, d_modes, psi, scratch, p, dt, d_status);
+    static const bool lds_set = (spf_allow_lds(k_spf_propagate), true);
+    (void)lds_set;
+    hipLaunchKernelGGL(k_spf_propagate, dim3(nmodes), dim3(256), spf_lds_bytes(p.nspfmax), s, d_modes, psi, scratch,
+                       p, dt, d_status);
 }
 
 
@@ -1850,7 +1962,8 @@ void launch_lan_ctrl(LanczosDev *L, const AttemptDev *at, const double2 *dot, co
 __global__ void k_lan_coef(LanczosDev *L, const AttemptDev *at, double dtscale, int build, double2 *coef) {
     const int n = L->nk;
     if (build) {
-        double A[kNKrylovMax * kNKrylovMax];
+        // one thread: the tridiagonal in LDS rather than in private memory
+        __shared__ double A[kNKrylovMax * kNKrylovMax];
         double *U = L->U;
         for (int i = 0; i < n * n; i++) A[i] = 0.0;
         for (int i = 0; i < n; i++) {
