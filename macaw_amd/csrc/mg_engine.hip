@@ -83,6 +83,7 @@ struct Launch {
     int ngram = 0;      // Gram slots (persistent grid, partials, final reduction)
     int slot_off = 0;   // first slot's entry in Plan::targets
     int gram_blocks = 0;
+    int gram_task_max = 0;  // most Gram slots of one task (kernel accumulator count)
 };
 struct Plan {
     DevBuf buf;
@@ -1147,7 +1148,7 @@ struct PlanBuilder {
     void phase(std::vector<TaskBuilder> ts) {
         if (ts.empty()) return;
         int64_t mx = 0;
-        int br = 1, nbuf = 1, nmat = 0, ngram = 0;
+        int br = 1, nbuf = 1, nmat = 0, ngram = 0, gtask = 0;
         const int slot_off = (int)targets.size();
         for (auto &tb : ts) {
             mx = std::max(mx, (tb.t.L / tb.t.Lt) * (tb.t.R / tb.t.Rt));
@@ -1156,12 +1157,14 @@ struct PlanBuilder {
             nmat = std::max(nmat, tb.t.nmat);
             tb.t.gram_base = ngram;
             ngram += tb.t.ngram;
+            gtask = std::max(gtask, tb.t.ngram);
             targets.insert(targets.end(), tb.targets.begin(), tb.targets.end());
         }
         Launch l{tasks.size(), (int)ts.size(), mx, br, nbuf, nmat};
         l.ngram = ngram;
         l.slot_off = slot_off;
         l.gram_blocks = (int)std::min<int64_t>(512, mx);
+        l.gram_task_max = gtask;
         launches.push_back(l);
         for (auto &tb : ts) tasks.push_back(tb.t);
     }
@@ -1201,7 +1204,7 @@ void launch_plan_entry(mg_ctx *c, const Plan &p, const Launch &l, int nmax) {
             c->graph_epoch++;          // graphs captured against the old buffer are stale
         }
         launch_brick_tasks(base + l.off, l.n, l.maxblocks, nmax, l.brick, l.nbuf, l.nmat, p.prec, c->stream, l.ngram,
-                           c->dgpart.as<double2>(), l.gram_blocks);
+                           c->dgpart.as<double2>(), l.gram_blocks, nullptr, l.gram_task_max);
         launch_gram_slots_final(c->dgpart.as<double2>(), l.ngram, l.gram_blocks,
                                 p.targets.as<double2 *>() + l.slot_off, c->nspfmax, c->stream);
     } else {

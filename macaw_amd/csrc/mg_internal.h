@@ -340,7 +340,8 @@ void launch_cscale(double2 *out, const double2 *x, double2 alpha, int64_t n, hip
 // convergence)
 void launch_brick_tasks(const BrickTask *d_tasks, int ntasks, int64_t max_blocks_per_task, int nmax,
                         int brick, int nbuf, int nmat, int prec, hipStream_t s, int ngram_total = 0,
-                        double2 *partials = nullptr, int gram_blocks = 0, const int *skip = nullptr);
+                        double2 *partials = nullptr, int gram_blocks = 0, const int *skip = nullptr,
+                        int gram_task_max = kMaxGrams);
 void launch_gram_slots_final(const double2 *partials, int nslots, int nblocks, double2 *const *targets,
                              int ld, hipStream_t s);
 void launch_herm_batch(double2 *const *targets, const int *n, int count, int ld, hipStream_t s);

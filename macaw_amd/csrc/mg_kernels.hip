@@ -428,7 +428,9 @@ __device__ __forceinline__ int64_t xcd_block(int b, int G) {
 
 // GRAM: the launch carries kBGram instructions; the grid is persistent over
 // the bricks and every workgroup ends by writing its Gram partials.
-template <int NMAX, class T, bool GRAM>
+// NG: Gram slots the program's accumulators hold (kMaxGrams, or 4 when no
+// task of the launch has more: fewer registers, more waves per SIMD)
+template <int NMAX, class T, bool GRAM, int NG = kMaxGrams>
 __global__ void __launch_bounds__(GRAM ? 256 : 1024) k_brick(const BrickTask *__restrict__ tasks, int capacity, int nmat_lds,
                                                 double2 *__restrict__ partials, const int *skip) {
     if (!GRAM && skip && *skip) return;
@@ -456,9 +458,9 @@ __global__ void __launch_bounds__(GRAM ? 256 : 1024) k_brick(const BrickTask *__
         mats[e] = v;
     }
     __syncthreads();
-    double gacc[GRAM ? kMaxGrams : 1][4];
+    double gacc[GRAM ? NG : 1][4];
 #pragma unroll
-    for (int g = 0; g < (GRAM ? kMaxGrams : 1); g++)
+    for (int g = 0; g < (GRAM ? NG : 1); g++)
 #pragma unroll
         for (int v = 0; v < 4; v++) gacc[g][v] = 0.0;
     const int nins = t.ninstr;
@@ -495,7 +497,7 @@ __global__ void __launch_bounds__(GRAM ? 256 : 1024) k_brick(const BrickTask *__
                 double part[4] = {0.0, 0.0, 0.0, 0.0};
                 GramMfma<T>::run(A, B, I.ni, I.no, I.nfib, I.stride, I.fd_stride, part);
 #pragma unroll
-                for (int g = 0; g < (GRAM ? kMaxGrams : 1); g++)
+                for (int g = 0; g < (GRAM ? NG : 1); g++)
                     if (g == I.mat)
 #pragma unroll
                         for (int v = 0; v < 4; v++) gacc[g][v] += part[v];
@@ -509,7 +511,7 @@ __global__ void __launch_bounds__(GRAM ? 256 : 1024) k_brick(const BrickTask *__
         const int lane = tid & 63, wave = tid >> 6, nw = nth >> 6;
         const int q = lane >> 4, col = lane & 15;
 #pragma unroll
-        for (int g = 0; g < (GRAM ? kMaxGrams : 1); g++) {
+        for (int g = 0; g < (GRAM ? NG : 1); g++) {
             if (g >= t.ngram) break;
 #pragma unroll
             for (int v = 0; v < 4; v++) red[wave * 256 + GramMfma<T>::row(q, v) * 16 + col] = gacc[g][v];
@@ -532,7 +534,7 @@ __global__ void __launch_bounds__(GRAM ? 256 : 1024) k_brick(const BrickTask *__
     }
 }
 
-template <int NMAX, class T, bool GRAM>
+template <int NMAX, class T, bool GRAM, int NG>
 static void launch_brick_nmax(const BrickTask *d_tasks, int ntasks, int64_t max_blocks, int capacity, int nbuf,
                               int nmat, hipStream_t s, double2 *partials, const int *skip) {
     static const int nthreads = [] {
@@ -544,36 +546,46 @@ static void launch_brick_nmax(const BrickTask *d_tasks, int ntasks, int64_t max_
     if (GRAM) shm = std::max(shm, sizeof(double) * 256 * 4);
     // Thread-safe one-time initialisation (ensemble members launch from several host threads).
     static const bool attr_set = [] {
-        (void)hipFuncSetAttribute((const void *)k_brick<NMAX, T, GRAM>, hipFuncAttributeMaxDynamicSharedMemorySize,
+        (void)hipFuncSetAttribute((const void *)k_brick<NMAX, T, GRAM, NG>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                   160 * 1024);
         return true;
     }();
     (void)attr_set;
-    hipLaunchKernelGGL((k_brick<NMAX, T, GRAM>), dim3((unsigned)max_blocks, (unsigned)ntasks), dim3(GRAM ? 256 : nthreads), shm,
+    hipLaunchKernelGGL((k_brick<NMAX, T, GRAM, NG>), dim3((unsigned)max_blocks, (unsigned)ntasks), dim3(GRAM ? 256 : nthreads), shm,
                        s, d_tasks, capacity, nmat, partials, skip);
 }
 
-template <class T, bool GRAM>
+template <class T, bool GRAM, int NG = kMaxGrams>
 static void launch_brick_T(const BrickTask *d_tasks, int ntasks, int64_t mb, int nmax, int capacity, int nbuf,
                            int nmat, hipStream_t s, double2 *partials, const int *skip) {
-    if (nmax <= 4) launch_brick_nmax<4, T, GRAM>(d_tasks, ntasks, mb, capacity, nbuf, nmat, s, partials, skip);
-    else if (nmax <= 8) launch_brick_nmax<8, T, GRAM>(d_tasks, ntasks, mb, capacity, nbuf, nmat, s, partials, skip);
-    else launch_brick_nmax<16, T, GRAM>(d_tasks, ntasks, mb, capacity, nbuf, nmat, s, partials, skip);
+    if (nmax <= 4) launch_brick_nmax<4, T, GRAM, NG>(d_tasks, ntasks, mb, capacity, nbuf, nmat, s, partials, skip);
+    else if (nmax <= 8) launch_brick_nmax<8, T, GRAM, NG>(d_tasks, ntasks, mb, capacity, nbuf, nmat, s, partials, skip);
+    else launch_brick_nmax<16, T, GRAM, NG>(d_tasks, ntasks, mb, capacity, nbuf, nmat, s, partials, skip);
 }
 
 void launch_brick_tasks(const BrickTask *d_tasks, int ntasks, int64_t max_blocks_per_task, int nmax, int brick,
                         int nbuf, int nmat, int prec, hipStream_t s, int ngram_total, double2 *partials,
-                        int gram_blocks, const int *skip) {
+                        int gram_blocks, const int *skip, int gram_task_max) {
     if (ntasks <= 0 || max_blocks_per_task <= 0) return;
     const int capacity = group_capacity(brick);
     if (ngram_total > 0) {
         // Gram launches always run (their partials feed a final reduction)
-        if (prec == 0)
-            launch_brick_T<double2, true>(d_tasks, ntasks, gram_blocks, nmax, capacity, nbuf, nmat, s, partials,
-                                          nullptr);
-        else
-            launch_brick_T<float2, true>(d_tasks, ntasks, gram_blocks, nmax, capacity, nbuf, nmat, s, partials,
-                                         nullptr);
+        const bool ng4 = gram_task_max <= 4;
+        if (prec == 0) {
+            if (ng4)
+                launch_brick_T<double2, true, 4>(d_tasks, ntasks, gram_blocks, nmax, capacity, nbuf, nmat, s,
+                                                 partials, nullptr);
+            else
+                launch_brick_T<double2, true>(d_tasks, ntasks, gram_blocks, nmax, capacity, nbuf, nmat, s, partials,
+                                              nullptr);
+        } else {
+            if (ng4)
+                launch_brick_T<float2, true, 4>(d_tasks, ntasks, gram_blocks, nmax, capacity, nbuf, nmat, s,
+                                                partials, nullptr);
+            else
+                launch_brick_T<float2, true>(d_tasks, ntasks, gram_blocks, nmax, capacity, nbuf, nmat, s, partials,
+                                             nullptr);
+        }
         return;
     }
     // workgroups per task: at most 4096 (complex FP32) / 1024 (complex FP64),
