@@ -2465,16 +2465,17 @@ void phase(mg_ctx *c, int bucket) {
 }
 
 // ----------------------------------------------------------------------------
-// Device-controlled CMF2 attempt for grid-mode problems (cmf2_base.f90:76-203).
-// Every decision of an attempt is taken on the device: the Lanczos stopping
-// test and propagator coefficients (k_lan_ctrl, k_lan_coef), DeltaPsi and
-// err_A, the accept/reject decision and the copies that commit an accepted
-// step (k_attempt_decide, k_copy_if); the attempted step h comes from device
-// memory.  The host writes h (and a replayed decision) before, and reads the
-// errors, Krylov sizes and the decision after an attempt, so the whole
-// attempt is captured once into a HIP graph and replayed for every later
-// attempt.  A Krylov build enqueues kmax iterations; those past convergence
-// return at once on a device flag.
+// Device-controlled CMF2 attempt (cmf2_base.f90:76-203), grid and Gaussian
+// modes.  Every decision of an attempt is taken on the device: the Lanczos
+// stopping test and propagator coefficients (k_lan_ctrl, k_lan_coef),
+// DeltaPsi and err_A, the accept/reject decision and the copies that commit
+// an accepted step (k_attempt_decide, k_copy_if); the attempted step h comes
+// from device memory.  The host writes h (and a replayed decision) before,
+// and reads the errors, Krylov sizes and the decision after an attempt: the
+// attempt is one stream-ordered sequence with no host synchronisation inside,
+// and with MG_ATTEMPT_GRAPH=1 it is captured once into a HIP graph and
+// replayed for every later attempt.  A Krylov build enqueues kmax
+// iterations; those past convergence return at once on a device flag.
 
 AttemptDev *att_dev(mg_ctx *c) { return c->datt.as<AttemptDev>(); }
 LanczosDev *lan_dev(mg_ctx *c, int i) {
