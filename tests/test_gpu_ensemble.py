@@ -66,6 +66,31 @@ def test_ensemble_members_match_single_runs(precision, threads):
 
 
 @gpu
+def test_ensemble_members_match_oracle_replayed():
+    """The ensemble is bit-identical to single runs (above); a single run of
+    each member, driven by the oracle's attempt sequence (replayed), matches
+    the oracle's C(t), populations and norm to arithmetic rounding, so the
+    members are checked against the oracle and not only against the engine
+    itself."""
+    from oracle_lib import Oracle
+    from macaw_amd.engine import Engine
+    probs = small_members(3)
+    tf = 0.5 / ensemble.AU2FS
+    for i, p in enumerate(probs):
+        outs_o, pops_o, atts_o = Oracle(p).propagate(0.0, tf)
+        e = Engine(p)
+        e.set_replay(atts_o)
+        outs_e, pops_e, atts_e = e.propagate(0.0, tf)
+        e.close()
+        assert len(outs_e) == len(outs_o), f"member {i}"
+        for a, b in zip(outs_e, outs_o):
+            assert abs(complex(a.auto_re, a.auto_im) - complex(b.auto_re, b.auto_im)) < 1e-10, f"member {i}"
+            assert abs(a.norm - b.norm) < 1e-10, f"member {i}"
+        assert np.abs(pops_e - pops_o).max() < 1e-10, f"member {i}"
+        assert [a.accepted for a in atts_e] == [a.accepted for a in atts_o], f"member {i}"
+
+
+@gpu
 def test_ensemble_reports_member_failure():
     """Members the engine refuses (here: an unknown precision, refused on the
     host before any kernel) are reported with their message."""
